@@ -1,0 +1,232 @@
+/*
+ * rt.h — C ABI of the MI355X (gfx950) software-renderer hot path.
+ *
+ * The reference renderer (vikkre/master_thesis, software_renderer/) has no
+ * C ABI: its hot loop is reached through C++ classes.  Each entry point below
+ * replaces one of those interfaces; the comment above it names the reference
+ * file:line it stands in for.  Everything here is plain C: pointers, sizes,
+ * status codes.  No torch or HIP types cross this boundary (a HIP stream is
+ * passed as an opaque void*).
+ *
+ * Conventions
+ *  - Every function returns RT_OK (0) or a negative RT_E_* code.  The message
+ *    for the last failure on a context is returned by rt_last_error(ctx);
+ *    rt_last_error(NULL) returns the last failure of rt_create on this thread.
+ *    Nothing throws across the boundary.
+ *  - ptr_kind says where caller buffers live.  RT_PTR_HOST: the call copies
+ *    through device staging and returns when the result is in host memory.
+ *    RT_PTR_DEVICE: the pointers are device pointers (e.g. torch tensors on
+ *    the same GPU); the work is enqueued on the context stream and the call
+ *    returns without waiting.
+ *  - One context is used by one host thread at a time.  Contexts are
+ *    independent of each other.
+ *  - Matrices are row-major float[16], m[i][j] at [4*i+j], and act on ROW
+ *    vectors (v' = v * M), matching software_renderer/math/matrix.h:245-253.
+ */
+#ifndef MT_RT_H
+#define MT_RT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------- */
+#define RT_OK             0
+#define RT_E_INVALID     (-1) /* bad argument (NULL, size 0 where not allowed, out of range) */
+#define RT_E_DEVICE      (-2) /* HIP runtime error or no usable gfx950 device              */
+#define RT_E_NOMEM       (-3) /* device allocation failed                                  */
+#define RT_E_STATE       (-4) /* call order: e.g. trace before rt_build                    */
+#define RT_E_UNSUPPORTED (-5) /* feature or size beyond what this build supports           */
+
+#define RT_PTR_HOST   0
+#define RT_PTR_DEVICE 1
+
+#define RT_MISS 0xFFFFFFFFu   /* rt_hit.tri of a ray that hit nothing */
+
+typedef struct rt_ctx rt_ctx;  /* opaque context */
+
+/* ---- rays and hits ------------------------------------------------------- */
+/* One ray, 32 bytes.  Replaces software_renderer/math/ray.h:6-16 (Ray: origin,
+ * direction, cached 1/direction).  tmin/tmax are extensions: a hit is
+ * accepted when t > 1e-7 (triangle.cpp:22,38), t > tmin and t < tmax.
+ * tmin = 0, tmax = +inf gives the reference's behaviour exactly. */
+typedef struct rt_ray {
+    float ox, oy, oz, tmin;
+    float dx, dy, dz, tmax;
+} rt_ray;
+
+/* Closest hit, 16 bytes.  tri is the caller's triangle index (RT_MISS on a
+ * miss).  t, u, v are the Moller-Trumbore values of the winning triangle as
+ * triangle.cpp:21-45 computes them.  The winner is the triangle whose hit
+ * point o + t*d has the smallest squared distance to the ray origin
+ * (graphics_object.cpp:75-83); exact ties go to the lowest triangle index. */
+typedef struct rt_hit {
+    uint32_t tri;
+    float t, u, v;
+} rt_hit;
+
+/* ---- scene --------------------------------------------------------------- */
+/* Per-material record.  Mirrors the GraphicsObject fields the path tracer
+ * reads (graphic/graphics_object.h:28-52, thresholds from
+ * graphics_object.cpp:48-51). */
+typedef struct rt_material {
+    float color[3];              /* GraphicsObject::color (albedo; a light's emitted colour) */
+    float diffuse_threshold;     /* wd / (wd + wr + wt)                                     */
+    float reflect_threshold;     /* diffuse_threshold + wr / (wd + wr + wt)                 */
+    float transparent_threshold; /* reflect_threshold + wt / (wd + wr + wt)                 */
+    float refraction_index;      /* GraphicsObject::refractionIndex                         */
+    uint32_t light_source;       /* GraphicsObject::lightSource                             */
+} rt_material;                   /* 32 bytes */
+
+/* World-space triangle mesh, the flattening of every GraphicsObject after
+ * GraphicsObject::init (graphics_object.cpp:12-41): positions and normals
+ * already multiplied by the object matrix.  Replaces the Scene / GraphicsObject
+ * / Mesh hierarchy (graphic/scene.h:9-23) as the geometry the tracer sees. */
+typedef struct rt_scene_desc {
+    const float*       pos;      /* 3 * num_vertices                          */
+    const float*       nrm;      /* 3 * num_vertices (not renormalised)       */
+    const uint32_t*    idx;      /* 3 * num_triangles vertex indices          */
+    const uint32_t*    tri_mat;  /* num_triangles material indices            */
+    const rt_material* mat;      /* num_materials                             */
+    uint32_t num_vertices, num_triangles, num_materials;
+} rt_scene_desc;
+
+/* ---- acceleration structure --------------------------------------------- */
+/* 4-wide BVH node, 128 bytes (one HBM/L2 line), structure-of-arrays over the
+ * four children.  child[k]: RT_CHILD_EMPTY, an inner node index, or a leaf
+ * RT_LEAF_BIT | (count-1) << RT_LEAF_COUNT_SHIFT | first (index into the
+ * leaf-ordered triangle records).  Node 0 is the root. */
+#define RT_CHILD_EMPTY      0xFFFFFFFFu
+#define RT_LEAF_BIT         0x80000000u
+#define RT_LEAF_COUNT_SHIFT 27
+#define RT_LEAF_FIRST_MASK  0x07FFFFFFu
+#define RT_MAX_LEAF_SIZE    16u
+typedef struct rt_bvh4_node {
+    float lo_x[4], hi_x[4];
+    float lo_y[4], hi_y[4];
+    float lo_z[4], hi_z[4];
+    uint32_t child[4];
+    uint32_t meta[4];            /* [0] = parent node index, rest reserved */
+} rt_bvh4_node;
+
+/* Leaf-ordered triangle record, 48 bytes: v0, edge0 = v1 - v0,
+ * edge1 = v2 - v0 (triangle.cpp:4-17) and the caller's triangle index. */
+typedef struct rt_tri_rec {
+    float v0[3]; uint32_t id;
+    float e0[3]; uint32_t pad0;
+    float e1[3]; uint32_t pad1;
+} rt_tri_rec;
+
+typedef struct rt_build_opts {
+    uint32_t leaf_size;   /* max triangles per leaf, 1..16 (0 = default 4) */
+    uint32_t flags;       /* reserved, 0 */
+} rt_build_opts;
+
+typedef struct rt_bvh_info {
+    uint32_t num_nodes;       /* BVH4 nodes                                  */
+    uint32_t num_leaves;      /* leaf children                               */
+    uint32_t depth;           /* BVH4 levels (root = 1)                      */
+    uint32_t num_triangles;
+    float    build_ms;        /* device time of the last build (HIP events)  */
+    uint32_t leaf_size;       /* leaf size the build used                    */
+} rt_bvh_info;
+
+/* Per-launch traversal counters (filled when requested; costs one extra
+ * atomic per wave). */
+typedef struct rt_trace_counters {
+    uint64_t rays;
+    uint64_t node_visits;     /* BVH4 nodes fetched (128 B each)             */
+    uint64_t tri_tests;       /* triangle records fetched (48 B each)        */
+    uint64_t hits;
+} rt_trace_counters;
+
+/* ---- camera and render settings ------------------------------------------ */
+/* Inverse view and projection matrices and the eye point, as
+ * GraphicsEngine::render computes them (graphics_engine.cpp:42-49). */
+typedef struct rt_camera {
+    float view_inv[16];
+    float proj_inv[16];
+    float origin[3];
+    uint32_t width, height;
+} rt_camera;
+
+#define RT_RENDER_DETERMINISTIC 0x1u  /* one path slot per pixel, samples summed in order (bit-reproducible) */
+#define RT_RENDER_SKY           0x2u  /* extension: a miss returns throughput * sky (reference: black)     */
+
+typedef struct rt_settings {
+    uint32_t max_depth;       /* PathTracer visionJumpCount (path_tracer.cpp:14)  */
+    uint32_t spp;             /* PathTracer raysPerPixel (path_tracer.cpp:15)     */
+    uint64_t seed;            /* Philox key                                       */
+    uint32_t flags;           /* RT_RENDER_*                                      */
+    uint32_t sample_offset;   /* index of the first sample (progressive renders)  */
+    float    sky[3];          /* RT_RENDER_SKY radiance                           */
+    uint32_t num_views;       /* reserved for batched views; 0 or 1              */
+} rt_settings;
+
+typedef struct rt_timing {
+    float last_kernel_ms;     /* device time of the dominant kernel of the last call   */
+    float last_call_ms;       /* device time of the whole last call (first to last op) */
+    uint32_t launches;        /* kernel launches the last call made                    */
+} rt_timing;
+
+/* ---- context ------------------------------------------------------------- */
+/* Creates a context on HIP device `device`.  Replaces GraphicsEngine
+ * construction (graphic/graphics_engine.cpp:7-10). */
+int  rt_create(rt_ctx** out, int device);
+/* Frees every device buffer the context owns (graphics_engine.cpp:12-14). */
+void rt_destroy(rt_ctx* ctx);
+const char* rt_last_error(const rt_ctx* ctx);
+/* Enqueue subsequent work on this hipStream_t (NULL = the context's own). */
+int  rt_set_stream(rt_ctx* ctx, void* hip_stream);
+int  rt_get_timing(const rt_ctx* ctx, rt_timing* out);
+
+/* Copies a world-space scene to the device.  Replaces GraphicsEngine::init's
+ * per-object init + Scene::addObject/init (graphics_engine.cpp:20-30,
+ * graphics_object.cpp:12-52, scene.cpp:9-20). */
+int  rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* scene);
+
+/* Replaces the colours of all materials (albedo / emission) without a
+ * re-upload; used by the differentiable renderer.  3 * num_materials floats. */
+int  rt_set_material_colors(rt_ctx* ctx, const float* rgb, int ptr_kind);
+
+/* Builds the 4-wide BVH on the device (Morton codes, radix sort, Karras
+ * hierarchy, bottom-up bounds, SAH-guided collapse).  Replaces
+ * BVH::init / BVH::rebuild (math/bounding_volume_hierachy.cpp:47-79) for
+ * both levels (mesh.cpp:24-37, scene.cpp:13-20). */
+int  rt_build(rt_ctx* ctx, const rt_build_opts* opts, rt_bvh_info* info);
+
+/* Copies the built BVH out (for checking).  Either pointer may be NULL to
+ * query sizes; *num_nodes / *num_tris are in-out capacities. */
+int  rt_bvh_export(rt_ctx* ctx, rt_bvh4_node* nodes, uint32_t* num_nodes,
+                   rt_tri_rec* tris, uint32_t* num_tris);
+
+/* Closest hit for n rays.  Replaces Scene::traceRay's geometric part
+ * (graphic/scene.cpp:22-41 -> graphics_object.cpp:64-87 ->
+ * triangle.cpp:21-45).  counters may be NULL. */
+int  rt_trace_closest(rt_ctx* ctx, const rt_ray* rays, uint32_t n,
+                      rt_hit* hits, int ptr_kind, rt_trace_counters* counters);
+
+/* Renders one frame with the diffuse path tracer.  Replaces the frame loop
+ * GraphicsEngine::render (graphics_engine.cpp:42-97) calling
+ * PathTracer::renderPixel (graphic/path_tracer.cpp:18-97) per pixel.
+ * rgb: width*height*3 floats (the unclamped pixel values), may be NULL.
+ * rgb8: width*height*3 bytes, clamp to [0,1] then (char)(v*255)
+ * (graphics_engine.cpp:90-95), may be NULL. */
+int  rt_render(rt_ctx* ctx, const rt_camera* cam, const rt_settings* s,
+               float* rgb, uint8_t* rgb8, int ptr_kind);
+
+/* Gradient of a scalar image loss with respect to every material colour,
+ * by path replay.  d_image: dL/d(rgb) (width*height*3), d_color: output
+ * 3*num_materials (overwritten).  Same camera/settings/seed as the forward
+ * render give the gradient of that exact render.  No reference counterpart:
+ * the reference has no differentiable renderer. */
+int  rt_render_adjoint(rt_ctx* ctx, const rt_camera* cam, const rt_settings* s,
+                       const float* d_image, float* d_color, int ptr_kind);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MT_RT_H */

@@ -1,0 +1,462 @@
+// lbvh.hip — 4-wide BVH built on the GPU.
+//
+// Replaces the reference's agglomerative build (software_renderer/math/
+// bounding_volume_hierachy.cpp:9-27,47-75: repeatedly merge the closest pair of
+// centroids, O(n^3)) with a linear BVH:
+//   1. 30-bit Morton code of each triangle's box centre (k_morton)
+//   2. LSD radix sort, 4 passes of 8 bits, stable (k_rs_hist/k_rs_scan/k_rs_scatter)
+//   3. binary hierarchy from the sorted codes (Karras, "Maximizing parallelism
+//      in the construction of BVHs, octrees, and k-d trees", HPG 2012) (k_karras)
+//   4. bounds bottom-up, second arrival at each node unions its children (k_bounds)
+//   5. top-down collapse to 4-wide nodes: a node opens its largest-area inner
+//      child until it has four children; subtrees of <= leaf_size triangles
+//      become leaves over a contiguous range of the sorted order (k_collapse)
+//   6. leaf-ordered triangle records v0 / edge0 / edge1 / id (k_tri_records)
+// The topology does not affect results: the traversal's answer is defined by
+// the closest-hit rule alone (see trace.hip).  Boxes are padded outward so the
+// box test never rejects a triangle the intersection test would accept.
+#include "rt_internal.hpp"
+#include "rt_core.hpp"
+
+namespace rt {
+namespace {
+
+constexpr int RS_BLOCK = 256;
+constexpr int RS_ITEMS = 8;
+constexpr int RS_TILE = RS_BLOCK * RS_ITEMS;
+
+// counters[] slots
+constexpr int CTR_NODES = 0;
+constexpr int CTR_QOUT = 1;
+constexpr int CTR_LEAVES = 2;
+constexpr int CTR_ERROR = 3;   // set when a structural bound is exceeded (never for a valid tree)
+
+__device__ __forceinline__ uint32_t expand10(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+__device__ __forceinline__ float quant(float x, float lo, float inv_ext) {
+    float q = (x - lo) * inv_ext * 1024.0f;
+    return fminf(fmaxf(q, 0.0f), 1023.0f);
+}
+
+__global__ void k_morton(uint32_t n, const float* __restrict__ pos, const uint32_t* __restrict__ idx,
+                         float lx, float ly, float lz, float ix, float iy, float iz,
+                         uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, float* __restrict__ leafbox) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t a = idx[3 * i], b = idx[3 * i + 1], c = idx[3 * i + 2];
+    float lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) {
+        float pa = pos[3 * a + k], pb = pos[3 * b + k], pc = pos[3 * c + k];
+        lo[k] = fminf(pa, fminf(pb, pc));
+        hi[k] = fmaxf(pa, fmaxf(pb, pc));
+    }
+    for (int k = 0; k < 3; ++k) {
+        leafbox[6 * i + k] = lo[k];
+        leafbox[6 * i + 3 + k] = hi[k];
+    }
+    uint32_t qx = (uint32_t)quant(0.5f * (lo[0] + hi[0]), lx, ix);
+    uint32_t qy = (uint32_t)quant(0.5f * (lo[1] + hi[1]), ly, iy);
+    uint32_t qz = (uint32_t)quant(0.5f * (lo[2] + hi[2]), lz, iz);
+    keys[i] = (expand10(qx) << 2) | (expand10(qy) << 1) | expand10(qz);
+    vals[i] = i;
+}
+
+__global__ void __launch_bounds__(RS_BLOCK) k_rs_hist(const uint32_t* __restrict__ keys, uint32_t n, int shift,
+                                                      uint32_t nblocks, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t cnt[256];
+    const int t = threadIdx.x;
+    cnt[t] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * RS_TILE;
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        uint32_t i = base + j * RS_BLOCK + t;
+        if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(uint32_t)t * nblocks + blockIdx.x] = cnt[t];
+}
+
+// Exclusive scan of `total` counters in place, one block of 1024 threads.
+__global__ void __launch_bounds__(1024) k_rs_scan(uint32_t* __restrict__ h, uint32_t total) {
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t chunk = (total + 1023u) / 1024u;
+    const uint32_t b = t * chunk, e = min(total, b + chunk);
+    uint32_t s = 0;
+    for (uint32_t i = b; i < e; ++i) s += h[i];
+    part[t] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        uint32_t v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - s;  // exclusive prefix of this chunk
+    for (uint32_t i = b; i < e; ++i) {
+        uint32_t x = h[i];
+        h[i] = run;
+        run += x;
+    }
+}
+
+// Stable scatter: each of the RS_ITEMS rounds ranks 256 consecutive keys.
+// Within a wave, keys with equal digits are found with 8 ballots; across the
+// 4 waves of the block the per-wave counts are prefix-summed in LDS.
+__global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                         uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                         uint32_t n, int shift, uint32_t nblocks,
+                                                         const uint32_t* __restrict__ hist) {
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[RS_BLOCK / 64][256];
+    const int t = threadIdx.x;
+    const int w = t >> 6, lane = t & 63;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    run[t] = hist[(uint32_t)t * nblocks + blockIdx.x];
+    const uint32_t base = blockIdx.x * RS_TILE;
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        for (int k = 0; k < RS_BLOCK / 64; ++k) wcnt[k][t] = 0;
+        __syncthreads();
+        uint32_t i = base + j * RS_BLOCK + t;
+        bool valid = i < n;
+        uint32_t key = valid ? kin[i] : 0u;
+        uint32_t val = valid ? vin[i] : 0u;
+        uint32_t dig = (key >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+        for (int bit = 0; bit < 8; ++bit) {
+            int on = (dig >> bit) & 1;
+            uint64_t bal = __ballot(on);
+            peers &= on ? bal : ~bal;
+        }
+        uint32_t rank = (uint32_t)__popcll(peers & lt);
+        if (valid && rank == 0) wcnt[w][dig] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t off = run[dig] + rank;
+            for (int k = 0; k < w; ++k) off += wcnt[k][dig];
+            kout[off] = key;
+            vout[off] = val;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+        for (int k = 0; k < RS_BLOCK / 64; ++k) add += wcnt[k][t];
+        run[t] += add;
+        __syncthreads();
+    }
+}
+
+// inner layout (uint32 arrays of length n): left, right, first, last, iparent, lparent
+struct Inner {
+    uint32_t *left, *right, *first, *last, *iparent, *lparent;
+};
+
+__device__ __forceinline__ int kdelta(const uint32_t* __restrict__ k, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    uint32_t a = k[i], b = k[j];
+    if (a != b) return __clz(a ^ b);
+    return 32 + __clz((uint32_t)i ^ (uint32_t)j);
+}
+
+__global__ void k_karras(const uint32_t* __restrict__ keys, int n, Inner in) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    int d = (kdelta(keys, n, i, i + 1) - kdelta(keys, n, i, i - 1)) > 0 ? 1 : -1;
+    int dmin = kdelta(keys, n, i, i - d);
+    int lmax = 2;
+    while (kdelta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (kdelta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    int j = i + l * d;
+    int dnode = kdelta(keys, n, i, j);
+    int s = 0, t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (kdelta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    int gamma = i + s * d + min(d, 0);
+    int lo = min(i, j), hi = max(i, j);
+    uint32_t left = (lo == gamma) ? (RT_LEAF_BIT | (uint32_t)gamma) : (uint32_t)gamma;
+    uint32_t right = (hi == gamma + 1) ? (RT_LEAF_BIT | (uint32_t)(gamma + 1)) : (uint32_t)(gamma + 1);
+    in.left[i] = left;
+    in.right[i] = right;
+    in.first[i] = (uint32_t)lo;
+    in.last[i] = (uint32_t)hi;
+    if (left & RT_LEAF_BIT) in.lparent[left & ~RT_LEAF_BIT] = (uint32_t)i; else in.iparent[left] = (uint32_t)i;
+    if (right & RT_LEAF_BIT) in.lparent[right & ~RT_LEAF_BIT] = (uint32_t)i; else in.iparent[right] = (uint32_t)i;
+    if (i == 0) in.iparent[0] = 0xFFFFFFFFu;
+}
+
+__device__ __forceinline__ void load_box(uint32_t ref, const uint32_t* __restrict__ vals, const float* leafbox,
+                                         const float* innerbox, float b[6]) {
+    const float* src = (ref & RT_LEAF_BIT) ? leafbox + 6u * vals[ref & ~RT_LEAF_BIT] : innerbox + 6u * ref;
+    for (int k = 0; k < 6; ++k) b[k] = __builtin_nontemporal_load(src + k);
+}
+
+__global__ void k_bounds(int n, Inner in, const uint32_t* __restrict__ vals, const float* leafbox,
+                         float* innerbox, uint32_t* visit) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    uint32_t p = in.lparent[j];
+    // a binary tree over n <= 2^27 leaves built from 30-bit codes + index
+    // bits is at most 64 levels deep; the bound only guards against a bug
+    for (int level = 0; level < 96; ++level) {
+        uint32_t old = __hip_atomic_fetch_add(&visit[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 0) return;  // the sibling subtree is not finished; its thread continues
+        float a[6], b[6];
+        load_box(in.left[p], vals, leafbox, innerbox, a);
+        load_box(in.right[p], vals, leafbox, innerbox, b);
+        for (int k = 0; k < 3; ++k) {
+            innerbox[6u * p + k] = fminf(a[k], b[k]);
+            innerbox[6u * p + 3 + k] = fmaxf(a[3 + k], b[3 + k]);
+        }
+        if (p == 0) return;
+        p = in.iparent[p];
+    }
+}
+
+struct Entry {
+    uint32_t ref, first, count;
+    float b[6];
+};
+
+__device__ __forceinline__ void make_entry(uint32_t ref, const Inner& in, const uint32_t* vals, const float* leafbox,
+                                           const float* innerbox, Entry& e) {
+    e.ref = ref;
+    if (ref & RT_LEAF_BIT) {
+        e.first = ref & ~RT_LEAF_BIT;
+        e.count = 1;
+    } else {
+        e.first = in.first[ref];
+        e.count = in.last[ref] - e.first + 1;
+    }
+    load_box(ref, vals, leafbox, innerbox, e.b);
+}
+
+__device__ __forceinline__ float half_area(const float b[6]) {
+    float x = b[3] - b[0], y = b[4] - b[1], z = b[5] - b[2];
+    return x * y + y * z + z * x;
+}
+
+// One item per thread: (binary inner node, 4-wide slot, parent slot).
+__global__ void k_collapse(const uint4* __restrict__ qin, uint32_t nq, uint4* __restrict__ qout, Inner in,
+                           const uint32_t* __restrict__ vals, const float* leafbox, const float* innerbox,
+                           uint32_t leaf_size, float pad, rt_bvh4_node* __restrict__ nodes, uint32_t max_nodes,
+                           uint32_t max_queue, uint32_t* counters) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nq) return;
+    uint4 it = qin[k];
+    Entry e[4];
+    int cnt = 2;
+    make_entry(in.left[it.x], in, vals, leafbox, innerbox, e[0]);
+    make_entry(in.right[it.x], in, vals, leafbox, innerbox, e[1]);
+    while (cnt < 4) {
+        int best = -1;
+        float best_a = -1.0f;
+        for (int c = 0; c < cnt; ++c) {
+            if (!(e[c].ref & RT_LEAF_BIT) && e[c].count > leaf_size) {
+                float a = half_area(e[c].b);
+                if (a > best_a) { best_a = a; best = c; }
+            }
+        }
+        if (best < 0) break;
+        uint32_t r = e[best].ref;
+        make_entry(in.left[r], in, vals, leafbox, innerbox, e[best]);
+        make_entry(in.right[r], in, vals, leafbox, innerbox, e[cnt]);
+        ++cnt;
+    }
+    rt_bvh4_node nd;
+    uint32_t leaves = 0;
+    for (int c = 0; c < 4; ++c) {
+        if (c < cnt) {
+            nd.lo_x[c] = e[c].b[0] - pad; nd.hi_x[c] = e[c].b[3] + pad;
+            nd.lo_y[c] = e[c].b[1] - pad; nd.hi_y[c] = e[c].b[4] + pad;
+            nd.lo_z[c] = e[c].b[2] - pad; nd.hi_z[c] = e[c].b[5] + pad;
+            if (e[c].count <= leaf_size) {
+                nd.child[c] = RT_LEAF_BIT | ((e[c].count - 1u) << RT_LEAF_COUNT_SHIFT) | e[c].first;
+                ++leaves;
+            } else {
+                uint32_t slot = atomicAdd(&counters[CTR_NODES], 1u);
+                uint32_t q = atomicAdd(&counters[CTR_QOUT], 1u);
+                if (slot < max_nodes && q < max_queue) {
+                    qout[q] = make_uint4(e[c].ref, slot, it.y, 0u);
+                    nd.child[c] = slot;
+                } else {
+                    counters[CTR_ERROR] = 1u;
+                    nd.child[c] = RT_CHILD_EMPTY;
+                }
+            }
+        } else {
+            nd.lo_x[c] = nd.lo_y[c] = nd.lo_z[c] = INFINITY;
+            nd.hi_x[c] = nd.hi_y[c] = nd.hi_z[c] = -INFINITY;
+            nd.child[c] = RT_CHILD_EMPTY;
+        }
+    }
+    nd.meta[0] = it.z;
+    nd.meta[1] = nd.meta[2] = nd.meta[3] = 0u;
+    nodes[it.y] = nd;
+    if (leaves) atomicAdd(&counters[CTR_LEAVES], leaves);
+}
+
+// Root of a scene with <= leaf_size triangles: one leaf child.
+__global__ void k_root_leaf(uint32_t n, const float* __restrict__ leafbox, const uint32_t* __restrict__ vals, float pad,
+                            rt_bvh4_node* nodes) {
+    rt_bvh4_node nd;
+    float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t j = 0; j < n; ++j) {
+        const float* s = leafbox + 6u * vals[j];
+        for (int k = 0; k < 3; ++k) { b[k] = fminf(b[k], s[k]); b[3 + k] = fmaxf(b[3 + k], s[3 + k]); }
+    }
+    for (int c = 0; c < 4; ++c) {
+        nd.lo_x[c] = nd.lo_y[c] = nd.lo_z[c] = INFINITY;
+        nd.hi_x[c] = nd.hi_y[c] = nd.hi_z[c] = -INFINITY;
+        nd.child[c] = RT_CHILD_EMPTY;
+    }
+    nd.lo_x[0] = b[0] - pad; nd.lo_y[0] = b[1] - pad; nd.lo_z[0] = b[2] - pad;
+    nd.hi_x[0] = b[3] + pad; nd.hi_y[0] = b[4] + pad; nd.hi_z[0] = b[5] + pad;
+    nd.child[0] = RT_LEAF_BIT | ((n - 1u) << RT_LEAF_COUNT_SHIFT);
+    nd.meta[0] = 0xFFFFFFFFu;
+    nd.meta[1] = nd.meta[2] = nd.meta[3] = 0u;
+    nodes[0] = nd;
+}
+
+// Leaf-ordered records: edge0 = v1 - v0, edge1 = v2 - v0 as the Triangle
+// constructor computes them (triangle.cpp:4-8).
+__global__ void k_tri_records(uint32_t n, const uint32_t* __restrict__ vals, const float* __restrict__ pos,
+                              const uint32_t* __restrict__ idx, rt_tri_rec* __restrict__ tris) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    uint32_t t = vals[j];
+    uint32_t a = idx[3 * t], b = idx[3 * t + 1], c = idx[3 * t + 2];
+    rt_tri_rec r;
+    for (int k = 0; k < 3; ++k) {
+        float p0 = pos[3 * a + k];
+        r.v0[k] = p0;
+        r.e0[k] = pos[3 * b + k] - p0;
+        r.e1[k] = pos[3 * c + k] - p0;
+    }
+    r.id = t;
+    r.pad0 = r.pad1 = 0u;
+    tris[j] = r;
+}
+
+inline uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
+
+}  // namespace
+
+int build_lbvh(rt_ctx* c, uint32_t leaf_size) {
+    const uint32_t n = c->T;
+    const size_t N = n;
+    int rc;
+    if ((rc = ensure(c, c->keys[0], 4 * N)) || (rc = ensure(c, c->keys[1], 4 * N)) ||
+        (rc = ensure(c, c->vals[0], 4 * N)) || (rc = ensure(c, c->vals[1], 4 * N)) ||
+        (rc = ensure(c, c->leafbox, 24 * N)) || (rc = ensure(c, c->inner, 6 * 4 * N)) ||
+        (rc = ensure(c, c->innerbox, 24 * N)) || (rc = ensure(c, c->visit, 4 * N)) ||
+        (rc = ensure(c, c->queue[0], 16 * (N / 2 + 2))) || (rc = ensure(c, c->queue[1], 16 * (N / 2 + 2))) ||
+        (rc = ensure(c, c->counters, 64)) || (rc = ensure(c, c->nodes, sizeof(rt_bvh4_node) * (N + 1))) ||
+        (rc = ensure(c, c->tris, sizeof(rt_tri_rec) * N)))
+        return rc;
+    hipStream_t st = c->stream;
+    RT_HIP(c, hipEventRecord(c->ev[0], st));
+
+    float ext[3], inv[3];
+    for (int k = 0; k < 3; ++k) {
+        ext[k] = c->scene_hi[k] - c->scene_lo[k];
+        inv[k] = ext[k] > 0.0f ? 1.0f / ext[k] : 0.0f;
+    }
+    float smax = 0.0f;
+    for (int k = 0; k < 3; ++k) smax = fmaxf(smax, fmaxf(fabsf(c->scene_lo[k]), fabsf(c->scene_hi[k])));
+    // outward padding of every box: 2^-16 of the largest coordinate magnitude
+    const float pad = smax * 1.52587890625e-05f + 1e-30f;
+
+    uint32_t* keys[2] = {(uint32_t*)c->keys[0].ptr, (uint32_t*)c->keys[1].ptr};
+    uint32_t* vals[2] = {(uint32_t*)c->vals[0].ptr, (uint32_t*)c->vals[1].ptr};
+    float* leafbox = (float*)c->leafbox.ptr;
+    hipLaunchKernelGGL(k_morton, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, (const float*)c->pos.ptr,
+                       (const uint32_t*)c->idx.ptr, c->scene_lo[0], c->scene_lo[1], c->scene_lo[2], inv[0], inv[1],
+                       inv[2], keys[0], vals[0], leafbox);
+
+    const uint32_t nblocks = blocks_for(n, RS_TILE);
+    if ((rc = ensure(c, c->hist, 4ull * 256 * nblocks))) return rc;
+    uint32_t* hist = (uint32_t*)c->hist.ptr;
+    int cur = 0;
+    for (int pass = 0; pass < 4; ++pass) {
+        int shift = pass * 8;
+        hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(RS_BLOCK), 0, st, keys[cur], n, shift, nblocks, hist);
+        hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, st, hist, 256u * nblocks);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nblocks), dim3(RS_BLOCK), 0, st, keys[cur], vals[cur], keys[cur ^ 1],
+                           vals[cur ^ 1], n, shift, nblocks, (const uint32_t*)hist);
+        cur ^= 1;
+    }
+    RT_HIP(c, hipGetLastError());
+
+    uint32_t* ib = (uint32_t*)c->inner.ptr;
+    Inner in{ib, ib + N, ib + 2 * N, ib + 3 * N, ib + 4 * N, ib + 5 * N};
+    float* innerbox = (float*)c->innerbox.ptr;
+    rt_bvh4_node* nodes = (rt_bvh4_node*)c->nodes.ptr;
+    uint32_t* counters = (uint32_t*)c->counters.ptr;
+
+    hipLaunchKernelGGL(k_tri_records, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, (const uint32_t*)vals[cur],
+                       (const float*)c->pos.ptr, (const uint32_t*)c->idx.ptr, (rt_tri_rec*)c->tris.ptr);
+
+    uint32_t depth = 1, num_nodes = 1, num_leaves = 1;
+    if (n <= leaf_size) {
+        hipLaunchKernelGGL(k_root_leaf, dim3(1), dim3(1), 0, st, n, (const float*)leafbox, (const uint32_t*)vals[cur],
+                           pad, nodes);
+    } else {
+        hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1, 256)), dim3(256), 0, st, (const uint32_t*)keys[cur], (int)n,
+                           in);
+        RT_HIP(c, hipMemsetAsync(c->visit.ptr, 0, 4 * N, st));
+        hipLaunchKernelGGL(k_bounds, dim3(blocks_for(n, 256)), dim3(256), 0, st, (int)n, in,
+                           (const uint32_t*)vals[cur], (const float*)leafbox, innerbox, (uint32_t*)c->visit.ptr);
+        uint32_t init[4] = {1u, 0u, 0u, 0u};
+        RT_HIP(c, hipMemcpyAsync(counters, init, sizeof(init), hipMemcpyHostToDevice, st));
+        uint4 root = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
+        RT_HIP(c, hipMemcpyAsync(c->queue[0].ptr, &root, sizeof(root), hipMemcpyHostToDevice, st));
+        uint32_t nq = 1;
+        int q = 0;
+        depth = 0;
+        const uint32_t max_nodes = (uint32_t)(N + 1), max_queue = (uint32_t)(N / 2 + 2);
+        while (nq > 0) {
+            if (depth >= 128) return fail(c, RT_E_DEVICE, "rt_build: BVH deeper than 128 levels");
+            ++depth;
+            RT_HIP(c, hipMemsetAsync(counters + CTR_QOUT, 0, 4, st));
+            hipLaunchKernelGGL(k_collapse, dim3(blocks_for(nq, 128)), dim3(128), 0, st, (const uint4*)c->queue[q].ptr, nq,
+                               (uint4*)c->queue[q ^ 1].ptr, in, (const uint32_t*)vals[cur], (const float*)leafbox,
+                               (const float*)innerbox, leaf_size, pad, nodes, max_nodes, max_queue, counters);
+            uint32_t h[4];
+            RT_HIP(c, hipMemcpyAsync(h, counters, sizeof(h), hipMemcpyDeviceToHost, st));
+            RT_HIP(c, hipStreamSynchronize(st));
+            nq = h[CTR_QOUT];
+            num_nodes = h[CTR_NODES];
+            num_leaves = h[CTR_LEAVES];
+            if (h[CTR_ERROR] || nq > max_queue || num_nodes > max_nodes)
+                return fail(c, RT_E_DEVICE, "rt_build: BVH collapse exceeded its structural bounds");
+            q ^= 1;
+        }
+    }
+    RT_HIP(c, hipGetLastError());
+    RT_HIP(c, hipEventRecord(c->ev[1], st));
+    RT_HIP(c, hipEventSynchronize(c->ev[1]));
+    float ms = 0.0f;
+    RT_HIP(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+
+    c->num_nodes = num_nodes;
+    c->bvh_depth = depth;
+    c->built = true;
+    c->info.num_nodes = num_nodes;
+    c->info.depth = depth;
+    c->info.num_triangles = n;
+    c->info.build_ms = ms;
+    c->info.leaf_size = leaf_size;
+    c->info.num_leaves = num_leaves;
+    return RT_OK;
+}
+
+}  // namespace rt

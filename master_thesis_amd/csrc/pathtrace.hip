@@ -1,0 +1,438 @@
+// pathtrace.hip — wavefront diffuse path tracer with persistent path slots.
+//
+// Replaces GraphicsEngine::render's thread pool (software_renderer/graphic/
+// graphics_engine.cpp:42-97) and PathTracer::renderPixel / traceSinglePath
+// (graphic/path_tracer.cpp:18-97).
+//
+// Estimator (the reference's, restated): the primary ray of a pixel is fixed
+// (no jitter) and shared by its N samples.  Per sample, throughput T = 1 and
+// for depth i < max_depth: a miss ends the path; while backface culling is on
+// and n.d > 0 the ray steps past the surface (o = p + 0.01 d) and the depth
+// slot is consumed; otherwise culling switches off, xi0 picks diffuse /
+// reflect / refract by the material thresholds, o = p + 0.01 d', a diffuse
+// bounce multiplies T by colour * (n.d'), and a light hit from the front
+// (n.d_prev <= 0) ends the path with contribution T.  The pixel is
+// 2*pi/N * sum of contributions, summed in sample order, the scale applied
+// in double.
+//
+// Scheduling: a slot owns one work item (pixel, chunk of samples) and runs
+// its samples one after another, so each pixel's sum has the reference's
+// order.  Every iteration is two kernels over the live slots:
+//   extend  closest hit of every queued ray (trace.hip, device-resident count)
+//   shade   the state machine above; a finished sample restarts the slot on
+//           the next sample; a finished item takes the next item from a
+//           global counter (the analogue of pixelCounter.fetch_add,
+//           graphics_engine.cpp:74); surviving slots are compacted into the
+//           next queue with one atomic per wave (ballot + popcount).
+// Random numbers: Philox-4x32-10 with counter (pixel, sample, depth, 0) and
+// the 64-bit seed as key, so a path does not depend on scheduling and the
+// adjoint pass replays it exactly.
+//
+// Adjoint (rt_render_adjoint): the same pass with dL/dI given.  A path's
+// contribution is the product of its diffuse factors colour_k * cos_k (the
+// light's own colour included), so d C / d colour_m sums, over the vertices k
+// with material m, the product of all other factors times cos_k.  Paths do not
+// depend on the colours (uniform hemisphere sampling, colour-independent
+// thresholds), which makes this gradient exact for the rendered estimate.
+#include <math.h>
+
+#include "rt_internal.hpp"
+#include "rt_core.hpp"
+
+namespace rt {
+namespace {
+
+constexpr int PT_BLOCK = 256;
+constexpr uint32_t ADJ_MAX_DEPTH = 16;
+constexpr uint32_t CULL_BIT = 0x10000u;
+
+// ctr slots
+constexpr int C_ITEM = 0;   // next work item
+constexpr int C_N0 = 1;     // live count, queue 0
+constexpr int C_N1 = 2;     // live count, queue 1
+
+struct PTArgs {
+    const ShadeGeom* geom;
+    const ShadeNrm* nrm;
+    const rt_material* mat;
+    float view_inv[16], proj_inv[16];
+    float ox, oy, oz;
+    uint32_t W, H, P;
+    uint32_t K, spc, items, S;
+    uint32_t max_depth, sample_offset, flags;
+    uint32_t seed_lo, seed_hi;
+    float sky[3];
+    double scale;
+    // slot state
+    uint32_t* s_item;
+    uint32_t* s_smp;
+    uint32_t* s_state;   // depth | CULL_BIT
+    float* s_T;          // 3 * S
+    float* s_sum;        // 3 * S
+    uint32_t* s_amat;    // adjoint: ADJ_MAX_DEPTH * S material ids
+    float* s_acos;       // adjoint: ADJ_MAX_DEPTH * S cosines
+    uint32_t* s_nv;      // adjoint: diffuse vertices so far
+    // queues
+    uint32_t* q_slot[2];
+    rt_ray* q_ray[2];
+    rt_hit* hits;
+    uint32_t* ctr;
+    float* partial;      // 3 * P * K
+    // outputs
+    float* rgb;
+    uint8_t* rgb8;
+    const float* dimg;
+    float* dcolor;
+};
+
+__device__ __forceinline__ rt_ray make_ray(f3 o, f3 d) {
+    rt_ray r;
+    r.ox = o.x; r.oy = o.y; r.oz = o.z; r.tmin = 0.0f;
+    r.dx = d.x; r.dy = d.y; r.dz = d.z; r.tmax = INFINITY;
+    return r;
+}
+
+__device__ __forceinline__ void start_sample(const PTArgs& a, uint32_t slot, uint32_t pix, rt_ray& r, bool adj) {
+    f3 d = primary_dir(a.view_inv, a.proj_inv, pix % a.W, pix / a.W, a.W, a.H);
+    r = make_ray(mk(a.ox, a.oy, a.oz), d);
+    a.s_state[slot] = CULL_BIT;
+    a.s_T[3 * slot] = 1.0f; a.s_T[3 * slot + 1] = 1.0f; a.s_T[3 * slot + 2] = 1.0f;
+    if (adj) a.s_nv[slot] = 0;
+}
+
+__device__ __forceinline__ void start_item(const PTArgs& a, uint32_t slot, uint32_t item, rt_ray& r, bool adj) {
+    a.s_item[slot] = item;
+    a.s_smp[slot] = 0;
+    a.s_sum[3 * slot] = 0.0f; a.s_sum[3 * slot + 1] = 0.0f; a.s_sum[3 * slot + 2] = 0.0f;
+    start_sample(a, slot, item % a.P, r, adj);
+}
+
+template <bool ADJ>
+__global__ void __launch_bounds__(PT_BLOCK) k_pt_init(PTArgs a) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        a.ctr[C_ITEM] = a.S;
+        a.ctr[C_N0] = a.S;
+        a.ctr[C_N1] = 0;
+    }
+    if (i >= a.S) return;
+    rt_ray r;
+    start_item(a, i, i, r, ADJ);
+    a.q_slot[0][i] = i;
+    a.q_ray[0][i] = r;
+}
+
+// Scatter dL/dI * dC/dcolour over the diffuse vertices of a finished path.
+// `term` is the factor after the last vertex (1 for a light hit, the sky
+// radiance for an escaped path).
+__device__ __forceinline__ void scatter_grad(const PTArgs& a, uint32_t slot, uint32_t pix, f3 term) {
+    const uint32_t nv = a.s_nv[slot];
+    const float w[3] = {a.dimg[3 * pix] * (float)a.scale, a.dimg[3 * pix + 1] * (float)a.scale,
+                        a.dimg[3 * pix + 2] * (float)a.scale};
+    const float tm[3] = {term.x, term.y, term.z};
+    uint32_t m[ADJ_MAX_DEPTH];
+    float cs[ADJ_MAX_DEPTH];
+    for (uint32_t k = 0; k < nv; ++k) {
+        m[k] = a.s_amat[k * a.S + slot];
+        cs[k] = a.s_acos[k * a.S + slot];
+    }
+    for (int ch = 0; ch < 3; ++ch) {
+        if (w[ch] == 0.0f) continue;
+        float f[ADJ_MAX_DEPTH];
+        for (uint32_t k = 0; k < nv; ++k) f[k] = a.mat[m[k]].color[ch] * cs[k];
+        // prefix products forward, suffix products backward
+        float pre = 1.0f;
+        float suf[ADJ_MAX_DEPTH + 1];
+        suf[nv] = tm[ch];
+        for (int k = (int)nv - 1; k >= 0; --k) suf[k] = f[k] * suf[k + 1];
+        for (uint32_t k = 0; k < nv; ++k) {
+            float g = pre * suf[k + 1] * cs[k];
+            atomicAdd(&a.dcolor[3 * m[k] + ch], w[ch] * g);
+            pre = pre * f[k];
+        }
+    }
+}
+
+template <bool ADJ>
+__global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
+    const uint32_t n = a.ctr[cur ? C_N1 : C_N0];
+    uint32_t* n_out = &a.ctr[cur ? C_N0 : C_N1];
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    bool alive = false, need_item = false;
+    uint32_t slot = 0;
+    rt_ray nr;
+    if (k < n) {
+        slot = a.q_slot[cur][k];
+        const rt_ray r = a.q_ray[cur][k];
+        const rt_hit h = a.hits[k];
+        const uint32_t item = a.s_item[slot];
+        const uint32_t pix = item % a.P;
+        const uint32_t chunk = item / a.P;
+        uint32_t smp = a.s_smp[slot];
+        uint32_t st = a.s_state[slot];
+        uint32_t depth = st & 0xFFFFu;
+        bool cull = (st & CULL_BIT) != 0;
+        f3 T = mk(a.s_T[3 * slot], a.s_T[3 * slot + 1], a.s_T[3 * slot + 2]);
+        f3 o = mk(r.ox, r.oy, r.oz), d = mk(r.dx, r.dy, r.dz);
+        bool end = false, contrib = false;
+        f3 term = mk(1.0f, 1.0f, 1.0f);
+        if (h.tri == RT_MISS) {
+            end = true;
+            if (a.flags & RT_RENDER_SKY) {
+                contrib = true;
+                term = mk(a.sky[0], a.sky[1], a.sky[2]);
+                T = mulv(T, term);
+            }
+        } else {
+            const f3 p = hit_point(o, d, h.t);
+            const ShadeGeom g = a.geom[h.tri];
+            const ShadeNrm nn = a.nrm[h.tri];
+            // Triangle::getBarycentricCoords (triangle.cpp:47-55)
+            const f3 v0 = mk(g.v0[0], g.v0[1], g.v0[2]);
+            const f3 e0 = mk(g.e0[0], g.e0[1], g.e0[2]), e1 = mk(g.e1[0], g.e1[1], g.e1[2]);
+            const f3 e2 = sub(p, v0);
+            const float d20 = dot(e2, e0), d21 = dot(e2, e1);
+            const float bv = (g.d11 * d20 - g.d01 * d21) / g.denom;
+            const float bw = (g.d00 * d21 - g.d01 * d20) / g.denom;
+            const float bu = 1.0f - bv - bw;
+            // Scene::traceRay normal (scene.cpp:49-53)
+            f3 nrm = mk(0.0f, 0.0f, 0.0f);
+            nrm = add(nrm, scale(bu, mk(nn.n0[0], nn.n0[1], nn.n0[2])));
+            nrm = add(nrm, scale(bv, mk(nn.n1[0], nn.n1[1], nn.n1[2])));
+            nrm = add(nrm, scale(bw, mk(nn.n2[0], nn.n2[1], nn.n2[2])));
+            nrm = normalize_ref(nrm);
+            const float ndotd = dot(nrm, d);
+            if (cull && ndotd > 0.0f) {
+                o = add(p, scale(0.01f, d));
+            } else {
+                cull = false;
+                const rt_material m = a.mat[g.mat];
+                const uint32_t gsmp = a.sample_offset + chunk * a.spc + smp;
+                u4 c4;
+                c4.x = pix; c4.y = gsmp; c4.z = depth; c4.w = 0u;
+                const u4 rnd = philox4x32_10(c4, a.seed_lo, a.seed_hi);
+                const float xi0 = u01(rnd.x);
+                const f3 prev = d;
+                const bool diffuse = xi0 <= m.diffuse_threshold;
+                if (diffuse) d = hemisphere_dir(u01(rnd.y), u01(rnd.z), nrm);
+                else if (xi0 <= m.reflect_threshold) d = reflect_ref(d, nrm);
+                else if (xi0 <= m.transparent_threshold) d = custom_refract_ref(d, nrm, m.refraction_index);
+                o = add(p, scale(0.01f, d));
+                if (diffuse) {
+                    const float cs = dot(nrm, d);
+                    T = mulv(T, scale(cs, mk(m.color[0], m.color[1], m.color[2])));
+                    if (ADJ) {
+                        uint32_t nv = a.s_nv[slot];
+                        if (nv < ADJ_MAX_DEPTH) {
+                            a.s_amat[nv * a.S + slot] = g.mat;
+                            a.s_acos[nv * a.S + slot] = cs;
+                        }
+                        a.s_nv[slot] = nv + 1;
+                    }
+                }
+                if (m.light_source && dot(nrm, prev) <= 0.0f) {
+                    end = true;
+                    contrib = true;
+                }
+            }
+            ++depth;
+            if (depth >= a.max_depth) end = true;
+        }
+        if (end) {
+            if (contrib) {
+                if (ADJ) {
+                    scatter_grad(a, slot, pix, term);
+                } else {
+                    a.s_sum[3 * slot] = a.s_sum[3 * slot] + T.x;
+                    a.s_sum[3 * slot + 1] = a.s_sum[3 * slot + 1] + T.y;
+                    a.s_sum[3 * slot + 2] = a.s_sum[3 * slot + 2] + T.z;
+                }
+            }
+            ++smp;
+            if (smp < a.spc) {
+                a.s_smp[slot] = smp;
+                start_sample(a, slot, pix, nr, ADJ);
+                alive = true;
+            } else {
+                if (!ADJ) {
+                    const size_t q = (size_t)chunk * a.P + pix;
+                    a.partial[3 * q] = a.s_sum[3 * slot];
+                    a.partial[3 * q + 1] = a.s_sum[3 * slot + 1];
+                    a.partial[3 * q + 2] = a.s_sum[3 * slot + 2];
+                }
+                need_item = true;
+            }
+        } else {
+            a.s_state[slot] = depth | (cull ? CULL_BIT : 0u);
+            a.s_T[3 * slot] = T.x; a.s_T[3 * slot + 1] = T.y; a.s_T[3 * slot + 2] = T.z;
+            nr = make_ray(o, d);
+            alive = true;
+        }
+    }
+    // next work item for slots whose item finished: one atomic per wave
+    const uint64_t want = __ballot(need_item);
+    if (want) {
+        const int leader = __ffsll((unsigned long long)want) - 1;
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(&a.ctr[C_ITEM], (uint32_t)__popcll(want));
+        base = __shfl(base, leader);
+        if (need_item) {
+            const uint32_t item = base + (uint32_t)__popcll(want & lt);
+            if (item < a.items) {
+                start_item(a, slot, item, nr, ADJ);
+                alive = true;
+            }
+        }
+    }
+    // compact the survivors into the next queue: one atomic per wave
+    const uint64_t live = __ballot(alive);
+    if (live) {
+        const int leader = __ffsll((unsigned long long)live) - 1;
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(n_out, (uint32_t)__popcll(live));
+        base = __shfl(base, leader);
+        if (alive) {
+            const uint32_t pos = base + (uint32_t)__popcll(live & lt);
+            a.q_slot[cur ^ 1][pos] = slot;
+            a.q_ray[cur ^ 1][pos] = nr;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(PT_BLOCK) k_pt_finish(PTArgs a) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.P) return;
+    float s[3] = {0.0f, 0.0f, 0.0f};
+    for (uint32_t c = 0; c < a.K; ++c)
+        for (int ch = 0; ch < 3; ++ch) s[ch] = s[ch] + a.partial[3 * ((size_t)c * a.P + p) + ch];
+    for (int ch = 0; ch < 3; ++ch) {
+        // finalColor *= 2.0f * M_PI * (1.0f / float(raysPerPixel))  (double scale)
+        float v = (float)((double)s[ch] * a.scale);
+        if (a.rgb) a.rgb[3 * (size_t)p + ch] = v;
+        if (a.rgb8) {
+            // clamp to [0, 1] then (char)(v * 255.0f) (graphics_engine.cpp:90-95)
+            float cv = fminf(fmaxf(v, 0.0f), 1.0f);
+            a.rgb8[3 * (size_t)p + ch] = (uint8_t)(int)(cv * 255.0f);
+        }
+    }
+}
+
+__global__ void k_zero(float* p, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+
+inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+}  // namespace
+
+int launch_render(rt_ctx* c, const rt_camera* cam, const rt_settings* s, float* d_rgb, uint8_t* d_rgb8,
+                  const float* d_dimage, float* d_dcolor) {
+    const bool adj = d_dcolor != nullptr;
+    if (adj && s->max_depth > ADJ_MAX_DEPTH)
+        return fail(c, RT_E_UNSUPPORTED, "rt_render_adjoint: max_depth above 16");
+    const uint64_t P = (uint64_t)cam->width * cam->height;
+    if (P > (1ull << 31)) return fail(c, RT_E_UNSUPPORTED, "rt_render: image above 2^31 pixels");
+    const uint32_t N = s->spp;
+    // samples of a pixel are split into K chunks (K | N) when there are too
+    // few pixels to fill the device; deterministic mode keeps one chunk so
+    // every pixel's sum has the reference's sample order.
+    const uint64_t target_slots = 1ull << 20;
+    uint32_t K = 1;
+    if (!(s->flags & RT_RENDER_DETERMINISTIC) && !adj) {
+        uint64_t want = (target_slots + P - 1) / P;
+        if (want > N) want = N;
+        for (uint32_t k = (uint32_t)want; k >= 1; --k)
+            if (N % k == 0) { K = k; break; }
+    }
+    const uint64_t items = P * K;
+    if (items > 0xFFFFFFFFull) return fail(c, RT_E_UNSUPPORTED, "rt_render: pixels * chunks above 2^32");
+    const uint32_t S = (uint32_t)(items < target_slots ? items : target_slots);
+    int rc;
+    const size_t szS = S;
+    if ((rc = ensure(c, c->pt_state, szS * (4 * 3 + 12 * 2) + (adj ? szS * (8 * ADJ_MAX_DEPTH + 4) : 0))) ||
+        (rc = ensure(c, c->pt_queue[0], 4 * szS + 16 + sizeof(rt_ray) * szS)) ||
+        (rc = ensure(c, c->pt_queue[1], 4 * szS + 16 + sizeof(rt_ray) * szS)) ||
+        (rc = ensure(c, c->hits, szS * sizeof(rt_hit))) || (rc = ensure(c, c->pt_ctr, 64)) ||
+        (rc = ensure(c, c->pt_partial, adj ? 4 : 12 * items)))
+        return rc;
+    PTArgs a;
+    a.geom = (const ShadeGeom*)c->shade_geom.ptr;
+    a.nrm = (const ShadeNrm*)c->shade_nrm.ptr;
+    a.mat = (const rt_material*)c->mat.ptr;
+    for (int k = 0; k < 16; ++k) { a.view_inv[k] = cam->view_inv[k]; a.proj_inv[k] = cam->proj_inv[k]; }
+    a.ox = cam->origin[0]; a.oy = cam->origin[1]; a.oz = cam->origin[2];
+    a.W = cam->width; a.H = cam->height; a.P = (uint32_t)P;
+    a.K = K; a.spc = N / K; a.items = (uint32_t)items; a.S = S;
+    a.max_depth = s->max_depth; a.sample_offset = s->sample_offset; a.flags = s->flags;
+    a.seed_lo = (uint32_t)s->seed; a.seed_hi = (uint32_t)(s->seed >> 32);
+    a.sky[0] = s->sky[0]; a.sky[1] = s->sky[1]; a.sky[2] = s->sky[2];
+    a.scale = 2.0 * M_PI * (double)(1.0f / (float)N);
+    char* st = (char*)c->pt_state.ptr;
+    a.s_item = (uint32_t*)st; st += 4 * szS;
+    a.s_smp = (uint32_t*)st; st += 4 * szS;
+    a.s_state = (uint32_t*)st; st += 4 * szS;
+    a.s_T = (float*)st; st += 12 * szS;
+    a.s_sum = (float*)st; st += 12 * szS;
+    a.s_amat = adj ? (uint32_t*)st : nullptr; st += adj ? 4 * ADJ_MAX_DEPTH * szS : 0;
+    a.s_acos = adj ? (float*)st : nullptr; st += adj ? 4 * ADJ_MAX_DEPTH * szS : 0;
+    a.s_nv = adj ? (uint32_t*)st : nullptr;
+    // queue buffers hold S slot ids then S rays (16-byte aligned)
+    for (int q = 0; q < 2; ++q) {
+        a.q_slot[q] = (uint32_t*)c->pt_queue[q].ptr;
+        a.q_ray[q] = (rt_ray*)((char*)c->pt_queue[q].ptr + ((4 * szS + 15) & ~(size_t)15));
+    }
+    a.hits = (rt_hit*)c->hits.ptr;
+    a.ctr = (uint32_t*)c->pt_ctr.ptr;
+    a.partial = (float*)c->pt_partial.ptr;
+    a.rgb = d_rgb;
+    a.rgb8 = d_rgb8;
+    a.dimg = d_dimage;
+    a.dcolor = d_dcolor;
+
+    hipStream_t stream = c->stream;
+    RT_HIP(c, hipEventRecord(c->ev[0], stream));
+    if (adj) hipLaunchKernelGGL(k_zero, dim3(div_up(3ull * c->M, 256)), dim3(256), 0, stream, d_dcolor, 3u * c->M);
+    if (adj) hipLaunchKernelGGL(k_pt_init<true>, dim3(div_up(S, PT_BLOCK)), dim3(PT_BLOCK), 0, stream, a);
+    else hipLaunchKernelGGL(k_pt_init<false>, dim3(div_up(S, PT_BLOCK)), dim3(PT_BLOCK), 0, stream, a);
+    RT_HIP(c, hipGetLastError());
+    // upper bound on iterations: every slot runs ceil(items / S) items of
+    // spc samples of at most max_depth extends
+    const uint64_t bound = (uint64_t)div_up(items, S) * a.spc * s->max_depth + 2;
+    int cur = 0;
+    uint64_t iter = 0;
+    uint32_t live = S;
+    uint32_t launches = 1;
+    while (live > 0) {
+        for (int rep = 0; rep < 8 && iter < bound; ++rep, ++iter) {
+            if ((rc = launch_trace(c, a.q_ray[cur], S, a.hits, false, &a.ctr[cur ? C_N1 : C_N0]))) return rc;
+            RT_HIP(c, hipMemsetAsync(&a.ctr[cur ? C_N0 : C_N1], 0, 4, stream));
+            if (adj) hipLaunchKernelGGL(k_pt_shade<true>, dim3(div_up(S, PT_BLOCK)), dim3(PT_BLOCK), 0, stream, a, cur);
+            else hipLaunchKernelGGL(k_pt_shade<false>, dim3(div_up(S, PT_BLOCK)), dim3(PT_BLOCK), 0, stream, a, cur);
+            RT_HIP(c, hipGetLastError());
+            launches += 2;
+            cur ^= 1;
+        }
+        RT_HIP(c, hipMemcpyAsync(&live, &a.ctr[cur ? C_N1 : C_N0], 4, hipMemcpyDeviceToHost, stream));
+        RT_HIP(c, hipStreamSynchronize(stream));
+        if (live > 0 && iter >= bound) return fail(c, RT_E_DEVICE, "rt_render: paths still live past the iteration bound");
+    }
+    if (!adj) {
+        hipLaunchKernelGGL(k_pt_finish, dim3(div_up(P, PT_BLOCK)), dim3(PT_BLOCK), 0, stream, a);
+        RT_HIP(c, hipGetLastError());
+        ++launches;
+    }
+    RT_HIP(c, hipEventRecord(c->ev[1], stream));
+    RT_HIP(c, hipEventSynchronize(c->ev[1]));
+    float ms = 0.0f;
+    RT_HIP(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    c->timing.last_call_ms = ms;
+    c->timing.launches = launches;
+    c->last_op = 2;
+    c->render_iters = (uint32_t)iter;
+    return RT_OK;
+}
+
+}  // namespace rt

@@ -1,0 +1,363 @@
+// rt_api.hip — the C ABI (include/rt.h): context, scene upload, build,
+// trace and render dispatch.  Every entry point validates its arguments on the
+// host before anything is launched, returns a status code and never throws.
+#include <math.h>
+#include <string.h>
+
+#include <new>
+#include <string>
+
+#include "rt_internal.hpp"
+#include "rt_core.hpp"
+
+namespace rt {
+
+static thread_local std::string g_create_err;
+
+int fail(rt_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg; else g_create_err = msg;
+    return code;
+}
+
+int hip_fail(rt_ctx* c, hipError_t e, const char* what) {
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    int code = (e == hipErrorOutOfMemory) ? RT_E_NOMEM : RT_E_DEVICE;
+    return fail(c, code, m);
+}
+
+int ensure(rt_ctx* c, DevBuf& b, size_t bytes) {
+    if (bytes == 0) bytes = 4;
+    if (b.bytes >= bytes) return RT_OK;
+    if (b.ptr) {
+        hipError_t e = hipFree(b.ptr);  // waits for work that may still use it
+        b.ptr = nullptr;
+        b.bytes = 0;
+        if (e != hipSuccess) return hip_fail(c, e, "hipFree");
+    }
+    hipError_t e = hipMalloc(&b.ptr, bytes);
+    if (e != hipSuccess) {
+        b.ptr = nullptr;
+        return fail(c, RT_E_NOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed: " +
+                                       hipGetErrorString(e));
+    }
+    b.bytes = bytes;
+    return RT_OK;
+}
+
+static void release(rt_ctx* c, DevBuf& b) {
+    if (b.ptr) (void)hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
+}
+
+namespace {
+
+__global__ void k_shade_records(uint32_t T, const float* __restrict__ pos, const float* __restrict__ nrm,
+                                const uint32_t* __restrict__ idx, const uint32_t* __restrict__ tri_mat,
+                                ShadeGeom* __restrict__ g, ShadeNrm* __restrict__ nr) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    uint32_t a = idx[3 * t], b = idx[3 * t + 1], c = idx[3 * t + 2];
+    ShadeGeom s;
+    ShadeNrm q;
+    for (int k = 0; k < 3; ++k) {
+        float p0 = pos[3 * a + k];
+        s.v0[k] = p0;
+        s.e0[k] = pos[3 * b + k] - p0;
+        s.e1[k] = pos[3 * c + k] - p0;
+        q.n0[k] = nrm[3 * a + k];
+        q.n1[k] = nrm[3 * b + k];
+        q.n2[k] = nrm[3 * c + k];
+    }
+    // Triangle constructor, triangle.cpp:10-13
+    f3 e0 = mk(s.e0[0], s.e0[1], s.e0[2]), e1 = mk(s.e1[0], s.e1[1], s.e1[2]);
+    s.d00 = dot(e0, e0);
+    s.d01 = dot(e0, e1);
+    s.d11 = dot(e1, e1);
+    s.denom = s.d00 * s.d11 - s.d01 * s.d01;
+    s.mat = tri_mat[t];
+    s.pad[0] = s.pad[1] = 0.0f;
+    q.pad[0] = q.pad[1] = q.pad[2] = 0.0f;
+    g[t] = s;
+    nr[t] = q;
+}
+
+struct DeviceGuard {
+    explicit DeviceGuard(int d) { (void)hipSetDevice(d); }
+};
+
+}  // namespace
+}  // namespace rt
+
+using namespace rt;
+
+extern "C" {
+
+int rt_create(rt_ctx** out, int device) {
+    if (!out) return fail(nullptr, RT_E_INVALID, "rt_create: out is NULL");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0)
+        return fail(nullptr, RT_E_DEVICE, std::string("rt_create: no HIP device (") + hipGetErrorString(e) + ")");
+    if (device < 0 || device >= n)
+        return fail(nullptr, RT_E_INVALID, "rt_create: device " + std::to_string(device) + " out of range");
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) return fail(nullptr, RT_E_DEVICE, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(nullptr, RT_E_DEVICE, std::string("libmtrt is built for gfx950; device reports ") + prop.gcnArchName);
+    e = hipSetDevice(device);
+    if (e != hipSuccess) return fail(nullptr, RT_E_DEVICE, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    rt_ctx* c = new (std::nothrow) rt_ctx();
+    if (!c) return fail(nullptr, RT_E_NOMEM, "rt_create: host allocation failed");
+    c->device = device;
+    c->num_cus = prop.multiProcessorCount;
+    e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(nullptr, RT_E_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    c->stream = c->own_stream;
+    for (int k = 0; k < 4; ++k) {
+        e = hipEventCreate(&c->ev[k]);
+        if (e != hipSuccess) {
+            rt_destroy(c);
+            return fail(nullptr, RT_E_DEVICE, std::string("hipEventCreate: ") + hipGetErrorString(e));
+        }
+    }
+    *out = c;
+    return RT_OK;
+}
+
+void rt_destroy(rt_ctx* c) {
+    if (!c) return;
+    DeviceGuard g(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    DevBuf* all[] = {&c->pos, &c->idx, &c->mat, &c->shade_geom, &c->shade_nrm, &c->nodes, &c->tris,
+                     &c->keys[0], &c->keys[1], &c->vals[0], &c->vals[1], &c->hist, &c->inner, &c->leafbox,
+                     &c->innerbox, &c->visit, &c->queue[0], &c->queue[1], &c->counters, &c->rays, &c->hits,
+                     &c->spill, &c->work, &c->stats, &c->pt_state, &c->pt_queue[0], &c->pt_queue[1],
+                     &c->pt_image, &c->pt_img8, &c->pt_dimg, &c->pt_dcolor, &c->pt_ctr, &c->pt_partial};
+    for (DevBuf* b : all) release(c, *b);
+    for (int k = 0; k < 4; ++k)
+        if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
+    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+    delete c;
+}
+
+const char* rt_last_error(const rt_ctx* c) {
+    return c ? c->err.c_str() : g_create_err.c_str();
+}
+
+int rt_set_stream(rt_ctx* c, void* s) {
+    if (!c) return RT_E_INVALID;
+    c->stream = s ? (hipStream_t)s : c->own_stream;
+    return RT_OK;
+}
+
+int rt_get_timing(const rt_ctx* cc, rt_timing* out) {
+    rt_ctx* c = const_cast<rt_ctx*>(cc);
+    if (!c || !out) return RT_E_INVALID;
+    DeviceGuard g(c->device);
+    if (c->last_op != 0) {
+        RT_HIP(c, hipEventSynchronize(c->ev[3]));
+        float ms = 0.0f;
+        RT_HIP(c, hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+        c->timing.last_kernel_ms = ms;
+        if (c->last_op == 1) c->timing.last_call_ms = ms;
+    }
+    *out = c->timing;
+    return RT_OK;
+}
+
+int rt_scene_upload(rt_ctx* c, const rt_scene_desc* s) {
+    if (!c) return RT_E_INVALID;
+    if (!s || !s->pos || !s->nrm || !s->idx || !s->tri_mat || !s->mat)
+        return fail(c, RT_E_INVALID, "rt_scene_upload: NULL array in scene description");
+    if (s->num_vertices == 0 || s->num_triangles == 0 || s->num_materials == 0)
+        return fail(c, RT_E_INVALID, "rt_scene_upload: empty scene");
+    if (s->num_triangles > RT_LEAF_FIRST_MASK)
+        return fail(c, RT_E_UNSUPPORTED, "rt_scene_upload: more than 2^27-1 triangles");
+    const uint64_t T = s->num_triangles, V = s->num_vertices, M = s->num_materials;
+    for (uint64_t i = 0; i < 3 * T; ++i)
+        if (s->idx[i] >= V)
+            return fail(c, RT_E_INVALID, "rt_scene_upload: vertex index out of range at triangle " + std::to_string(i / 3));
+    for (uint64_t i = 0; i < T; ++i)
+        if (s->tri_mat[i] >= M)
+            return fail(c, RT_E_INVALID, "rt_scene_upload: material index out of range at triangle " + std::to_string(i));
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint64_t v = 0; v < V; ++v)
+        for (int k = 0; k < 3; ++k) {
+            float x = s->pos[3 * v + k];
+            if (!(fabsf(x) < INFINITY)) return fail(c, RT_E_INVALID, "rt_scene_upload: non-finite vertex position");
+            lo[k] = fminf(lo[k], x);
+            hi[k] = fmaxf(hi[k], x);
+        }
+    DeviceGuard g(c->device);
+    int rc;
+    if ((rc = ensure(c, c->pos, 12 * V)) || (rc = ensure(c, c->idx, 12 * T)) ||
+        (rc = ensure(c, c->mat, sizeof(rt_material) * M)) || (rc = ensure(c, c->shade_geom, sizeof(ShadeGeom) * T)) ||
+        (rc = ensure(c, c->shade_nrm, sizeof(ShadeNrm) * T)))
+        return rc;
+    DevBuf nrm, tmat;
+    if ((rc = ensure(c, nrm, 12 * V)) || (rc = ensure(c, tmat, 4 * T))) return rc;
+    RT_HIP(c, hipMemcpyAsync(c->pos.ptr, s->pos, 12 * V, hipMemcpyHostToDevice, c->stream));
+    RT_HIP(c, hipMemcpyAsync(c->idx.ptr, s->idx, 12 * T, hipMemcpyHostToDevice, c->stream));
+    RT_HIP(c, hipMemcpyAsync(c->mat.ptr, s->mat, sizeof(rt_material) * M, hipMemcpyHostToDevice, c->stream));
+    RT_HIP(c, hipMemcpyAsync(nrm.ptr, s->nrm, 12 * V, hipMemcpyHostToDevice, c->stream));
+    RT_HIP(c, hipMemcpyAsync(tmat.ptr, s->tri_mat, 4 * T, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_shade_records, dim3((uint32_t)((T + 255) / 256)), dim3(256), 0, c->stream, (uint32_t)T,
+                       (const float*)c->pos.ptr, (const float*)nrm.ptr, (const uint32_t*)c->idx.ptr,
+                       (const uint32_t*)tmat.ptr, (ShadeGeom*)c->shade_geom.ptr, (ShadeNrm*)c->shade_nrm.ptr);
+    RT_HIP(c, hipGetLastError());
+    RT_HIP(c, hipStreamSynchronize(c->stream));
+    release(c, nrm);
+    release(c, tmat);
+    c->V = (uint32_t)V;
+    c->T = (uint32_t)T;
+    c->M = (uint32_t)M;
+    for (int k = 0; k < 3; ++k) { c->scene_lo[k] = lo[k]; c->scene_hi[k] = hi[k]; }
+    c->built = false;
+    return RT_OK;
+}
+
+int rt_set_material_colors(rt_ctx* c, const float* rgb, int ptr_kind) {
+    if (!c) return RT_E_INVALID;
+    if (!rgb) return fail(c, RT_E_INVALID, "rt_set_material_colors: NULL colours");
+    if (c->M == 0) return fail(c, RT_E_STATE, "rt_set_material_colors: no scene uploaded");
+    DeviceGuard g(c->device);
+    // colour is the first 12 bytes of each 32-byte rt_material record
+    RT_HIP(c, hipMemcpy2DAsync(c->mat.ptr, sizeof(rt_material), rgb, 12, 12, c->M,
+                               ptr_kind == RT_PTR_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
+    if (ptr_kind != RT_PTR_DEVICE) RT_HIP(c, hipStreamSynchronize(c->stream));
+    return RT_OK;
+}
+
+int rt_build(rt_ctx* c, const rt_build_opts* o, rt_bvh_info* info) {
+    if (!c) return RT_E_INVALID;
+    if (c->T == 0) return fail(c, RT_E_STATE, "rt_build: no scene uploaded");
+    uint32_t leaf = (o && o->leaf_size) ? o->leaf_size : 4u;
+    if (leaf > RT_MAX_LEAF_SIZE) return fail(c, RT_E_INVALID, "rt_build: leaf_size above 16");
+    DeviceGuard g(c->device);
+    int rc = build_lbvh(c, leaf);
+    if (rc) return rc;
+    if (info) *info = c->info;
+    return RT_OK;
+}
+
+int rt_bvh_export(rt_ctx* c, rt_bvh4_node* nodes, uint32_t* num_nodes, rt_tri_rec* tris, uint32_t* num_tris) {
+    if (!c || !num_nodes || !num_tris) return RT_E_INVALID;
+    if (!c->built) return fail(c, RT_E_STATE, "rt_bvh_export: no BVH built");
+    uint32_t cap_n = *num_nodes, cap_t = *num_tris;
+    *num_nodes = c->num_nodes;
+    *num_tris = c->T;
+    if (!nodes && !tris) return RT_OK;
+    if ((nodes && cap_n < c->num_nodes) || (tris && cap_t < c->T))
+        return fail(c, RT_E_INVALID, "rt_bvh_export: output capacity too small");
+    DeviceGuard g(c->device);
+    if (nodes)
+        RT_HIP(c, hipMemcpyAsync(nodes, c->nodes.ptr, sizeof(rt_bvh4_node) * c->num_nodes, hipMemcpyDeviceToHost,
+                                 c->stream));
+    if (tris) RT_HIP(c, hipMemcpyAsync(tris, c->tris.ptr, sizeof(rt_tri_rec) * c->T, hipMemcpyDeviceToHost, c->stream));
+    RT_HIP(c, hipStreamSynchronize(c->stream));
+    return RT_OK;
+}
+
+int rt_trace_closest(rt_ctx* c, const rt_ray* rays, uint32_t n, rt_hit* hits, int ptr_kind,
+                     rt_trace_counters* counters) {
+    if (!c) return RT_E_INVALID;
+    if (n > 0 && (!rays || !hits)) return fail(c, RT_E_INVALID, "rt_trace_closest: NULL rays or hits");
+    if (ptr_kind != RT_PTR_HOST && ptr_kind != RT_PTR_DEVICE)
+        return fail(c, RT_E_INVALID, "rt_trace_closest: bad ptr_kind");
+    if (!c->built) return fail(c, RT_E_STATE, "rt_trace_closest: call rt_build first");
+    if (counters) memset(counters, 0, sizeof(*counters));
+    if (n == 0) return RT_OK;
+    DeviceGuard g(c->device);
+    int rc;
+    const rt_ray* d_rays = rays;
+    rt_hit* d_hits = hits;
+    if (ptr_kind == RT_PTR_HOST) {
+        if ((rc = ensure(c, c->rays, sizeof(rt_ray) * (size_t)n)) || (rc = ensure(c, c->hits, sizeof(rt_hit) * (size_t)n)))
+            return rc;
+        RT_HIP(c, hipMemcpyAsync(c->rays.ptr, rays, sizeof(rt_ray) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+        d_rays = (const rt_ray*)c->rays.ptr;
+        d_hits = (rt_hit*)c->hits.ptr;
+    }
+    if ((rc = launch_trace(c, d_rays, n, d_hits, counters != nullptr))) return rc;
+    c->last_op = 1;
+    if (ptr_kind == RT_PTR_HOST)
+        RT_HIP(c, hipMemcpyAsync(hits, d_hits, sizeof(rt_hit) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    if (ptr_kind == RT_PTR_HOST || counters) {
+        uint32_t flags[2];
+        unsigned long long st[4];
+        RT_HIP(c, hipMemcpyAsync(flags, c->work.ptr, 8, hipMemcpyDeviceToHost, c->stream));
+        if (counters) RT_HIP(c, hipMemcpyAsync(st, c->stats.ptr, 32, hipMemcpyDeviceToHost, c->stream));
+        RT_HIP(c, hipStreamSynchronize(c->stream));
+        if (flags[1]) return fail(c, RT_E_UNSUPPORTED, "rt_trace_closest: traversal stack exceeded its bound");
+        if (counters) {
+            counters->rays = st[0];
+            counters->node_visits = st[1];
+            counters->tri_tests = st[2];
+            counters->hits = st[3];
+        }
+    }
+    return RT_OK;
+}
+
+int rt_render(rt_ctx* c, const rt_camera* cam, const rt_settings* s, float* rgb, uint8_t* rgb8, int ptr_kind) {
+    if (!c) return RT_E_INVALID;
+    if (!cam || !s) return fail(c, RT_E_INVALID, "rt_render: NULL camera or settings");
+    if (!c->built) return fail(c, RT_E_STATE, "rt_render: call rt_build first");
+    if (cam->width == 0 || cam->height == 0) return fail(c, RT_E_INVALID, "rt_render: empty image");
+    if (s->spp == 0) return fail(c, RT_E_INVALID, "rt_render: spp must be >= 1");
+    if (s->max_depth == 0 || s->max_depth > 64) return fail(c, RT_E_INVALID, "rt_render: max_depth must be 1..64");
+    if (ptr_kind != RT_PTR_HOST && ptr_kind != RT_PTR_DEVICE) return fail(c, RT_E_INVALID, "rt_render: bad ptr_kind");
+    DeviceGuard g(c->device);
+    const size_t P = (size_t)cam->width * cam->height;
+    float* d_rgb = rgb;
+    uint8_t* d_rgb8 = rgb8;
+    int rc;
+    if (ptr_kind == RT_PTR_HOST) {
+        if ((rc = ensure(c, c->pt_image, 12 * P)) || (rc = ensure(c, c->pt_img8, 3 * P))) return rc;
+        d_rgb = (float*)c->pt_image.ptr;
+        d_rgb8 = (uint8_t*)c->pt_img8.ptr;
+    }
+    if ((rc = launch_render(c, cam, s, d_rgb, d_rgb8, nullptr, nullptr))) return rc;
+    if (ptr_kind == RT_PTR_HOST) {
+        if (rgb) RT_HIP(c, hipMemcpyAsync(rgb, d_rgb, 12 * P, hipMemcpyDeviceToHost, c->stream));
+        if (rgb8) RT_HIP(c, hipMemcpyAsync(rgb8, d_rgb8, 3 * P, hipMemcpyDeviceToHost, c->stream));
+        RT_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    return RT_OK;
+}
+
+int rt_render_adjoint(rt_ctx* c, const rt_camera* cam, const rt_settings* s, const float* d_image, float* d_color,
+                      int ptr_kind) {
+    if (!c) return RT_E_INVALID;
+    if (!cam || !s || !d_image || !d_color) return fail(c, RT_E_INVALID, "rt_render_adjoint: NULL argument");
+    if (!c->built) return fail(c, RT_E_STATE, "rt_render_adjoint: call rt_build first");
+    if (cam->width == 0 || cam->height == 0) return fail(c, RT_E_INVALID, "rt_render_adjoint: empty image");
+    if (s->spp == 0) return fail(c, RT_E_INVALID, "rt_render_adjoint: spp must be >= 1");
+    if (s->max_depth == 0 || s->max_depth > 64) return fail(c, RT_E_INVALID, "rt_render_adjoint: max_depth must be 1..64");
+    if (ptr_kind != RT_PTR_HOST && ptr_kind != RT_PTR_DEVICE) return fail(c, RT_E_INVALID, "rt_render_adjoint: bad ptr_kind");
+    DeviceGuard g(c->device);
+    const size_t P = (size_t)cam->width * cam->height;
+    const float* dimg = d_image;
+    float* dcol = d_color;
+    int rc;
+    if (ptr_kind == RT_PTR_HOST) {
+        if ((rc = ensure(c, c->pt_dimg, 12 * P)) || (rc = ensure(c, c->pt_dcolor, 12 * (size_t)c->M))) return rc;
+        RT_HIP(c, hipMemcpyAsync(c->pt_dimg.ptr, d_image, 12 * P, hipMemcpyHostToDevice, c->stream));
+        dimg = (const float*)c->pt_dimg.ptr;
+        dcol = (float*)c->pt_dcolor.ptr;
+    }
+    if ((rc = launch_render(c, cam, s, nullptr, nullptr, dimg, dcol))) return rc;
+    if (ptr_kind == RT_PTR_HOST) {
+        RT_HIP(c, hipMemcpyAsync(d_color, dcol, 12 * (size_t)c->M, hipMemcpyDeviceToHost, c->stream));
+        RT_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    return RT_OK;
+}
+
+}  // extern "C"

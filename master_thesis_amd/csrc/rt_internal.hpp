@@ -1,0 +1,93 @@
+// rt_internal.hpp — context object and launcher declarations shared by the
+// HIP translation units of libmtrt.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/rt.h"
+
+namespace rt {
+
+// Per-triangle shading record in the caller's triangle order: what
+// Triangle::getBarycentricCoords (triangle.cpp:47-55) and Scene::traceRay's
+// normal interpolation (scene.cpp:43-53) read.  64 + 48 bytes.
+struct ShadeGeom {
+    float v0[3], e0[3], e1[3];
+    float d00, d01, d11, denom;
+    uint32_t mat;
+    float pad[2];
+};
+struct ShadeNrm {
+    float n0[3], n1[3], n2[3];
+    float pad[3];
+};
+
+// Device buffer that grows on demand and is freed with the context.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace rt
+
+struct rt_ctx {
+    int device = 0;
+    int num_cus = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // scene
+    uint32_t V = 0, T = 0, M = 0;
+    float scene_lo[3] = {0, 0, 0}, scene_hi[3] = {0, 0, 0};
+    rt::DevBuf pos, idx, mat, shade_geom, shade_nrm;
+    // BVH
+    bool built = false;
+    rt::DevBuf nodes, tris;
+    uint32_t num_nodes = 0, bvh_depth = 0;
+    rt_bvh_info info{};
+    // build scratch
+    rt::DevBuf keys[2], vals[2], hist, inner, leafbox, innerbox, visit, queue[2], counters;
+    // trace scratch
+    rt::DevBuf rays, hits, spill, work, stats;
+    // path tracer state
+    rt::DevBuf pt_state, pt_queue[2], pt_image, pt_img8, pt_dimg, pt_dcolor, pt_ctr, pt_partial;
+    uint32_t render_iters = 0;
+
+    // ev[0..1] bracket a whole call, ev[2..3] the last traversal kernel
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    rt_timing timing{};
+    int last_op = 0;  // 1 = trace, 2 = render
+};
+
+namespace rt {
+
+// error helpers (rt_api.hip)
+int fail(rt_ctx* c, int code, const std::string& msg);
+int hip_fail(rt_ctx* c, hipError_t e, const char* what);
+int ensure(rt_ctx* c, DevBuf& b, size_t bytes);
+
+#define RT_HIP(c, expr)                                      \
+    do {                                                     \
+        hipError_t _e = (expr);                              \
+        if (_e != hipSuccess) return rt::hip_fail(c, _e, #expr); \
+    } while (0)
+
+// lbvh.hip
+int build_lbvh(rt_ctx* c, uint32_t leaf_size);
+
+// trace.hip
+constexpr int TRACE_BLOCK = 256;
+constexpr int STACK_LDS = 32;
+// n: ray count, or an upper bound on it when n_dev (device-resident count) is set
+int launch_trace(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, bool stats,
+                 const uint32_t* n_dev = nullptr);
+
+// pathtrace.hip
+int launch_render(rt_ctx* c, const rt_camera* cam, const rt_settings* s, float* d_rgb, uint8_t* d_rgb8,
+                  const float* d_dimage, float* d_dcolor);
+
+}  // namespace rt

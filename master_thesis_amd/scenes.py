@@ -1,0 +1,218 @@
+"""Synthetic scenes and ray sets of the benchmark configurations.
+
+Everything is generated here, seed-fixed; nothing is read from the reference.
+
+  cornell_box()        Cornell-style box: 5 walls of a 5.5-unit cube (10 tris),
+                       two axis-aligned boxes (24 tris), one emissive quad (2 tris,
+                       colour 15), Lambertian albedos 0.73 / 0.65 / 0.12 triples.
+  triangle_soup()      T triangles, centroids uniform in the unit cube, vertex
+                       offsets of LogNormal(-5, 0.5) length in uniform directions.
+  microbench_rays()    8M coherent pinhole rays + 8M uniform-sphere incoherent rays.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import MATERIAL_DTYPE, RAY_DTYPE
+
+
+@dataclass
+class Scene:
+    pos: np.ndarray        # (V, 3) float32, world space
+    nrm: np.ndarray        # (V, 3) float32, world space
+    idx: np.ndarray        # (T, 3) uint32
+    tri_mat: np.ndarray    # (T,) uint32
+    materials: np.ndarray  # (M,) MATERIAL_DTYPE
+
+    @property
+    def num_triangles(self) -> int:
+        return int(self.idx.shape[0])
+
+    def validate(self) -> None:
+        assert self.pos.dtype == np.float32 and self.pos.ndim == 2 and self.pos.shape[1] == 3
+        assert self.nrm.dtype == np.float32 and self.nrm.shape == self.pos.shape
+        assert self.idx.dtype == np.uint32 and self.idx.ndim == 2 and self.idx.shape[1] == 3
+        assert self.tri_mat.dtype == np.uint32 and self.tri_mat.shape == (self.idx.shape[0],)
+        assert self.materials.dtype == MATERIAL_DTYPE
+
+
+def material(color, diffuse=1.0, reflect=0.0, transparent=0.0, refraction_index=1.0, light=False) -> np.ndarray:
+    """One material record with thresholds as GraphicsObject::init computes
+    them (graphics_object.cpp:48-51), in float."""
+    m = np.zeros((), MATERIAL_DTYPE)
+    m["color"] = np.asarray(color, np.float32)
+    wd, wr, wt = np.float32(diffuse), np.float32(reflect), np.float32(transparent)
+    total = np.float32(wd + wr) + wt
+    td = np.float32(wd / total)
+    tr = np.float32(td + np.float32(wr / total))
+    tt = np.float32(tr + np.float32(wt / total))
+    m["diffuse_threshold"] = td
+    m["reflect_threshold"] = tr
+    m["transparent_threshold"] = tt
+    m["refraction_index"] = np.float32(refraction_index)
+    m["light_source"] = 1 if light else 0
+    return m
+
+
+class _Builder:
+    def __init__(self):
+        self.pos, self.nrm, self.idx, self.mat = [], [], [], []
+        self.nv = 0
+
+    def quad(self, corner, ea, eb, normal, m):
+        """Two triangles (c, c+a, c+a+b), (c, c+a+b, c+b), flat normal."""
+        c = np.asarray(corner, np.float32)
+        a = np.asarray(ea, np.float32)
+        b = np.asarray(eb, np.float32)
+        vs = [c, c + a, c + a + b, c + b]
+        for v in vs:
+            self.pos.append(v)
+            self.nrm.append(np.asarray(normal, np.float32))
+        n = self.nv
+        self.idx += [(n, n + 1, n + 2), (n, n + 2, n + 3)]
+        self.mat += [m, m]
+        self.nv += 4
+
+    def box(self, lo, hi, m):
+        """Axis-aligned box with outward normals, 12 triangles."""
+        x0, y0, z0 = lo
+        x1, y1, z1 = hi
+        dx, dy, dz = x1 - x0, y1 - y0, z1 - z0
+        self.quad((x0, y0, z0), (0, 0, dz), (0, dy, 0), (-1, 0, 0), m)
+        self.quad((x1, y0, z0), (0, dy, 0), (0, 0, dz), (1, 0, 0), m)
+        self.quad((x0, y0, z0), (dx, 0, 0), (0, 0, dz), (0, -1, 0), m)
+        self.quad((x0, y1, z0), (0, 0, dz), (dx, 0, 0), (0, 1, 0), m)
+        self.quad((x0, y0, z0), (0, dy, 0), (dx, 0, 0), (0, 0, -1), m)
+        self.quad((x0, y0, z1), (dx, 0, 0), (0, dy, 0), (0, 0, 1), m)
+
+    def scene(self, materials) -> Scene:
+        s = Scene(
+            pos=np.asarray(self.pos, np.float32).reshape(-1, 3),
+            nrm=np.asarray(self.nrm, np.float32).reshape(-1, 3),
+            idx=np.asarray(self.idx, np.uint32).reshape(-1, 3),
+            tri_mat=np.asarray(self.mat, np.uint32),
+            materials=np.stack(materials),
+        )
+        s.validate()
+        return s
+
+
+WHITE = (0.73, 0.73, 0.73)
+RED = (0.65, 0.05, 0.05)
+GREEN = (0.12, 0.45, 0.15)
+CORNELL_HALF = 2.75
+
+
+def cornell_box(light_color=(15.0, 15.0, 15.0)) -> Scene:
+    """Cornell-style box seen from +x through its open side (camera on the +x
+    axis looking -x, as the reference's default camera looks).  Inward wall
+    normals.  Materials: 0 white, 1 red, 2 green, 3 light."""
+    h = CORNELL_HALF
+    b = _Builder()
+    W, R, G, L = 0, 1, 2, 3
+    s = 2 * h
+    b.quad((-h, -h, -h), (0, 0, s), (0, s, 0), (1, 0, 0), W)      # back wall x = -h
+    b.quad((-h, -h, -h), (s, 0, 0), (0, 0, s), (0, 1, 0), W)      # floor y = -h
+    b.quad((-h, h, -h), (0, 0, s), (s, 0, 0), (0, -1, 0), W)      # ceiling y = +h
+    b.quad((-h, -h, -h), (0, s, 0), (s, 0, 0), (0, 0, 1), G)      # wall z = -h (green)
+    b.quad((-h, -h, h), (s, 0, 0), (0, s, 0), (0, 0, -1), R)      # wall z = +h (red)
+    b.box((0.10, -h, 0.45), (1.75, -h + 1.65, 2.10), W)           # short box
+    b.box((-1.85, -h, -2.05), (-0.20, -h + 3.30, -0.40), W)       # tall box
+    lh = 0.70
+    b.quad((-lh, h - 0.01, -lh), (0, 0, 2 * lh), (2 * lh, 0, 0), (0, -1, 0), L)  # light
+    mats = [material(WHITE), material(RED), material(GREEN), material(light_color, light=True)]
+    return b.scene(mats)
+
+
+def closed_room(floor_mat: np.ndarray, wall_mat: np.ndarray, half: float = 1.0) -> Scene:
+    """Closed cube [-half, half]^3 seen from inside (inward normals): the
+    floor y = -half uses material 0, the other five faces material 1.  Used
+    by the estimator's closed-form checks."""
+    h = half
+    s = 2 * h
+    b = _Builder()
+    b.quad((-h, -h, -h), (s, 0, 0), (0, 0, s), (0, 1, 0), 0)      # floor
+    b.quad((-h, h, -h), (0, 0, s), (s, 0, 0), (0, -1, 0), 1)      # ceiling
+    b.quad((-h, -h, -h), (0, 0, s), (0, s, 0), (1, 0, 0), 1)      # x = -h
+    b.quad((h, -h, -h), (0, s, 0), (0, 0, s), (-1, 0, 0), 1)      # x = +h
+    b.quad((-h, -h, -h), (0, s, 0), (s, 0, 0), (0, 0, 1), 1)      # z = -h
+    b.quad((-h, -h, h), (s, 0, 0), (0, s, 0), (0, 0, -1), 1)      # z = +h
+    return b.scene([floor_mat, wall_mat])
+
+
+def cornell_camera_position() -> tuple[float, float, float]:
+    """Eye on the +x axis far enough for a 40-degree field of view to frame the
+    box's open face."""
+    return (CORNELL_HALF + CORNELL_HALF / np.tan(np.radians(20.0)) + 0.05, 0.0, 0.0)
+
+
+def _unit_vectors(rng: np.random.Generator, n: int) -> np.ndarray:
+    z = rng.uniform(-1.0, 1.0, n)
+    phi = rng.uniform(0.0, 2 * np.pi, n)
+    r = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    return np.stack([r * np.cos(phi), r * np.sin(phi), z], axis=1)
+
+
+def triangle_soup(T: int = 1 << 20, seed: int = 1234, mu: float = -5.0, sigma: float = 0.5) -> Scene:
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0.0, 1.0, (T, 1, 3))
+    length = rng.lognormal(mu, sigma, (T, 3, 1))
+    dirs = _unit_vectors(rng, 3 * T).reshape(T, 3, 3)
+    verts = (c + length * dirs).astype(np.float32)          # (T, 3, 3)
+    e0 = verts[:, 1] - verts[:, 0]
+    e1 = verts[:, 2] - verts[:, 0]
+    n = np.cross(e0, e1)
+    n /= np.maximum(np.linalg.norm(n, axis=1, keepdims=True), 1e-30)
+    pos = verts.reshape(-1, 3)
+    nrm = np.repeat(n.astype(np.float32), 3, axis=0)
+    idx = np.arange(3 * T, dtype=np.uint32).reshape(T, 3)
+    s = Scene(pos=np.ascontiguousarray(pos), nrm=np.ascontiguousarray(nrm), idx=idx,
+              tri_mat=np.zeros(T, np.uint32), materials=np.stack([material(WHITE)]))
+    s.validate()
+    return s
+
+
+def pinhole_rays(eye, target, fov_deg: float, width: int, height: int) -> np.ndarray:
+    """One ray per pixel centre of a pinhole camera (row-major, y down)."""
+    eye = np.asarray(eye, np.float64)
+    f = np.asarray(target, np.float64) - eye
+    f /= np.linalg.norm(f)
+    up = np.array([0.0, 1.0, 0.0])
+    s = np.cross(f, up)
+    s /= np.linalg.norm(s)
+    u = np.cross(s, f)
+    t = np.tan(np.radians(fov_deg) / 2)
+    aspect = width / height
+    px = (np.arange(width) + 0.5) / width * 2 - 1
+    py = 1 - (np.arange(height) + 0.5) / height * 2
+    X, Y = np.meshgrid(px * t * aspect, py * t)
+    d = X[..., None] * s + Y[..., None] * u + f
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    r = np.zeros(width * height, RAY_DTYPE)
+    r["o"] = eye.astype(np.float32)
+    r["d"] = d.reshape(-1, 3).astype(np.float32)
+    r["tmin"] = 0.0
+    r["tmax"] = np.inf
+    return r
+
+
+def incoherent_rays(n: int, seed: int = 4321) -> np.ndarray:
+    """Origins uniform in the unit cube, directions uniform on the sphere."""
+    rng = np.random.default_rng(seed)
+    r = np.zeros(n, RAY_DTYPE)
+    r["o"] = rng.uniform(0.0, 1.0, (n, 3)).astype(np.float32)
+    r["d"] = _unit_vectors(rng, n).astype(np.float32)
+    r["tmin"] = 0.0
+    r["tmax"] = np.inf
+    return r
+
+
+def microbench_rays(width: int = 4096, height: int = 2048, n_incoherent: int | None = None, seed: int = 4321):
+    """The ray-cast microbenchmark's two halves: coherent pinhole rays from
+    (0.5, 0.5, -1.5) at the cube centre, FOV 40 degrees, and as many
+    incoherent rays."""
+    coh = pinhole_rays((0.5, 0.5, -1.5), (0.5, 0.5, 0.5), 40.0, width, height)
+    inc = incoherent_rays(width * height if n_incoherent is None else n_incoherent, seed)
+    return coh, inc

@@ -1,0 +1,154 @@
+"""Python front end of the C ABI: a context that owns one scene, its BVH and
+the render / trace entry points.  Host arrays are numpy; device arrays are
+torch CUDA (HIP) tensors, passed by pointer (no copies)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .scenes import Scene
+
+
+class Tracer:
+    def __init__(self, device: int = 0):
+        self.lib = L.load()
+        h = C.c_void_p()
+        rc = self.lib.rt_create(C.byref(h), device)
+        if rc != L.RT_OK:
+            raise L.RTError(rc, self.lib.rt_last_error(None).decode())
+        self.h = h
+        self.scene: Scene | None = None
+        self.info = None
+
+    # -- lifetime -----------------------------------------------------------
+    def close(self) -> None:
+        if self.h:
+            self.lib.rt_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int) -> None:
+        if rc != L.RT_OK:
+            raise L.RTError(rc, self.lib.rt_last_error(self.h).decode())
+
+    # -- scene --------------------------------------------------------------
+    def upload(self, scene: Scene) -> None:
+        scene.validate()
+        self._keep = (np.ascontiguousarray(scene.pos), np.ascontiguousarray(scene.nrm),
+                      np.ascontiguousarray(scene.idx), np.ascontiguousarray(scene.tri_mat),
+                      np.ascontiguousarray(scene.materials))
+        pos, nrm, idx, tm, mat = self._keep
+        d = L.rt_scene_desc(pos.ctypes.data, nrm.ctypes.data, idx.ctypes.data, tm.ctypes.data, mat.ctypes.data,
+                            pos.shape[0], idx.shape[0], mat.shape[0])
+        self._check(self.lib.rt_scene_upload(self.h, C.byref(d)))
+        self.scene = scene
+
+    def set_stream(self, stream_ptr: int | None) -> None:
+        self._check(self.lib.rt_set_stream(self.h, stream_ptr))
+
+    def set_material_colors(self, rgb) -> None:
+        kind = L.RT_PTR_DEVICE if L._is_device(rgb) else L.RT_PTR_HOST
+        if kind == L.RT_PTR_HOST:
+            rgb = np.ascontiguousarray(rgb, np.float32)
+        self._check(self.lib.rt_set_material_colors(self.h, L._ptr(rgb), kind))
+
+    def build(self, leaf_size: int = 4) -> dict:
+        opts = L.rt_build_opts(leaf_size, 0)
+        info = L.rt_bvh_info()
+        self._check(self.lib.rt_build(self.h, C.byref(opts), C.byref(info)))
+        self.info = {f: getattr(info, f) for f, _ in L.rt_bvh_info._fields_}
+        return self.info
+
+    def export_bvh(self):
+        nn, nt = C.c_uint32(0), C.c_uint32(0)
+        self._check(self.lib.rt_bvh_export(self.h, None, C.byref(nn), None, C.byref(nt)))
+        nodes = np.zeros(nn.value, L.NODE_DTYPE)
+        tris = np.zeros(nt.value, L.TRI_DTYPE)
+        self._check(self.lib.rt_bvh_export(self.h, nodes.ctypes.data, C.byref(nn), tris.ctypes.data, C.byref(nt)))
+        return nodes, tris
+
+    # -- queries ------------------------------------------------------------
+    def trace(self, rays, hits=None, counters: bool = False):
+        """Closest hit.  rays: numpy RAY_DTYPE array (host) or a contiguous
+        CUDA tensor of n*8 float32 (device).  Returns hits (and counters)."""
+        dev = L._is_device(rays)
+        if dev:
+            n = rays.numel() // 8
+            if hits is None:
+                import torch
+                hits = torch.empty(n * 4, dtype=torch.int32, device=rays.device)
+        else:
+            rays = np.ascontiguousarray(rays)
+            assert rays.dtype == L.RAY_DTYPE
+            n = rays.shape[0]
+            if hits is None:
+                hits = np.zeros(n, L.HIT_DTYPE)
+        cnt = L.rt_trace_counters()
+        self._check(self.lib.rt_trace_closest(self.h, L._ptr(rays), n, L._ptr(hits),
+                                              L.RT_PTR_DEVICE if dev else L.RT_PTR_HOST,
+                                              C.byref(cnt) if counters else None))
+        if counters:
+            return hits, {f: getattr(cnt, f) for f, _ in L.rt_trace_counters._fields_}
+        return hits
+
+    def timing(self) -> dict:
+        t = L.rt_timing()
+        self._check(self.lib.rt_get_timing(self.h, C.byref(t)))
+        return {f: getattr(t, f) for f, _ in L.rt_timing._fields_}
+
+    # -- rendering ----------------------------------------------------------
+    def render(self, cam: L.rt_camera, settings: L.rt_settings, rgb=None, rgb8=None, want_rgb8: bool = False):
+        """Returns (rgb float32 HxWx3, rgb8 uint8 HxWx3 or None).  Pass torch
+        CUDA tensors as rgb/rgb8 to render straight into device memory."""
+        W, H = cam.width, cam.height
+        dev = rgb is not None and L._is_device(rgb)
+        if not dev:
+            rgb = np.zeros((H, W, 3), np.float32) if rgb is None else rgb
+            if want_rgb8 and rgb8 is None:
+                rgb8 = np.zeros((H, W, 3), np.uint8)
+        self._check(self.lib.rt_render(self.h, C.byref(cam), C.byref(settings), L._ptr(rgb),
+                                       L._ptr(rgb8) if rgb8 is not None else None,
+                                       L.RT_PTR_DEVICE if dev else L.RT_PTR_HOST))
+        return rgb, rgb8
+
+    def render_adjoint(self, cam: L.rt_camera, settings: L.rt_settings, d_image, d_color=None):
+        dev = L._is_device(d_image)
+        M = self.scene.materials.shape[0]
+        if d_color is None:
+            if dev:
+                import torch
+                d_color = torch.zeros((M, 3), dtype=torch.float32, device=d_image.device)
+            else:
+                d_color = np.zeros((M, 3), np.float32)
+        if not dev:
+            d_image = np.ascontiguousarray(d_image, np.float32)
+        self._check(self.lib.rt_render_adjoint(self.h, C.byref(cam), C.byref(settings), L._ptr(d_image),
+                                               L._ptr(d_color), L.RT_PTR_DEVICE if dev else L.RT_PTR_HOST))
+        return d_color
+
+
+def make_settings(spp: int, max_depth: int, seed: int = 1, flags: int = 0, sample_offset: int = 0,
+                  sky=(0.0, 0.0, 0.0)) -> L.rt_settings:
+    s = L.rt_settings()
+    s.max_depth = max_depth
+    s.spp = spp
+    s.seed = seed
+    s.flags = flags
+    s.sample_offset = sample_offset
+    for k in range(3):
+        s.sky[k] = sky[k]
+    s.num_views = 1
+    return s

@@ -1,0 +1,176 @@
+"""ctypes loader of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, always as the checker or the timed CPU baseline.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+REF_PATH = HERE / "_ref" / "libref.so"
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} not built (make -C oracle)")
+        lib = C.CDLL(str(LIB_PATH))
+        P = C.c_void_p
+        F = C.POINTER(C.c_float)
+        U = C.POINTER(C.c_uint32)
+        lib.or_philox.argtypes = [U, U, U]
+        lib.or_sincos_2pi.argtypes = [C.c_float, F, F]
+        lib.or_hemisphere.argtypes = [C.c_float, C.c_float, F, F]
+        lib.or_intersect.argtypes = [F, F, F, F, F, F]
+        lib.or_intersect.restype = C.c_int
+        lib.or_barycentric.argtypes = [F, F, F, F, F]
+        lib.or_trace_brute.argtypes = [P, P, C.c_uint32, P, C.c_uint32, P, C.c_int]
+        lib.or_bvh_build.argtypes = [P, P, C.c_uint32, C.c_int]
+        lib.or_bvh_build.restype = P
+        lib.or_bvh_free.argtypes = [P]
+        lib.or_bvh_num_nodes.argtypes = [P]
+        lib.or_bvh_num_nodes.restype = C.c_uint32
+        lib.or_trace_bvh.argtypes = [P, P, C.c_uint32, P, C.c_int, C.POINTER(C.c_uint64)]
+        lib.or_trace_bvh4.argtypes = [P, C.c_uint32, P, C.c_uint32, C.c_uint32, P, C.c_uint32, P, C.c_int,
+                                      C.POINTER(C.c_uint64)]
+        lib.or_check_bvh4.argtypes = [P, C.c_uint32, P, C.c_uint32, P, P, U]
+        lib.or_check_bvh4.restype = C.c_int
+        lib.or_vec4_mat.argtypes = [F, F, F]
+        lib.or_primary_dir.argtypes = [F, F, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, F]
+        lib.or_render.argtypes = [P, P, P, P, P, C.c_uint32, C.c_uint32, C.c_uint32, P, P, P, P, C.c_int]
+        lib.or_render.restype = C.c_int
+        lib.or_render_adjoint.argtypes = [P, P, P, P, P, C.c_uint32, C.c_uint32, C.c_uint32, P, P, P, P, C.c_int]
+        lib.or_render_adjoint.restype = C.c_int
+        _lib = lib
+    return _lib
+
+
+def _p(a: np.ndarray) -> int:
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+def fptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def philox(ctr, key) -> np.ndarray:
+    c = np.asarray(ctr, np.uint32)
+    k = np.asarray(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    load().or_philox(c.ctypes.data_as(C.POINTER(C.c_uint32)), k.ctypes.data_as(C.POINTER(C.c_uint32)),
+                     out.ctypes.data_as(C.POINTER(C.c_uint32)))
+    return out
+
+
+def sincos_2pi(x: float):
+    s, c = C.c_float(), C.c_float()
+    load().or_sincos_2pi(x, C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+def intersect(o, d, v0, v1, v2):
+    arrs = [np.asarray(a, np.float32) for a in (o, d, v0, v1, v2)]
+    out = np.zeros(3, np.float32)
+    hit = load().or_intersect(*[fptr(a) for a in arrs], fptr(out))
+    return bool(hit), out
+
+
+def barycentric(v0, v1, v2, p):
+    arrs = [np.asarray(a, np.float32) for a in (v0, v1, v2, p)]
+    out = np.zeros(3, np.float32)
+    load().or_barycentric(*[fptr(a) for a in arrs], fptr(out))
+    return out
+
+
+def trace_brute(scene, rays: np.ndarray, threads: int = 8) -> np.ndarray:
+    from master_thesis_amd._lib import HIT_DTYPE
+    hits = np.zeros(rays.shape[0], HIT_DTYPE)
+    pos = np.ascontiguousarray(scene.pos, np.float32)
+    idx = np.ascontiguousarray(scene.idx, np.uint32)
+    load().or_trace_brute(_p(pos), _p(idx), idx.shape[0], _p(np.ascontiguousarray(rays)), rays.shape[0], _p(hits),
+                          threads)
+    return hits
+
+
+class CpuBVH:
+    """The oracle's binned-SAH BVH (the CPU baseline's acceleration structure)."""
+
+    def __init__(self, scene, leaf_size: int = 4):
+        self.pos = np.ascontiguousarray(scene.pos, np.float32)
+        self.idx = np.ascontiguousarray(scene.idx, np.uint32)
+        self.h = load().or_bvh_build(_p(self.pos), _p(self.idx), self.idx.shape[0], leaf_size)
+
+    def trace(self, rays: np.ndarray, threads: int = 8):
+        from master_thesis_amd._lib import HIT_DTYPE
+        hits = np.zeros(rays.shape[0], HIT_DTYPE)
+        cnt = (C.c_uint64 * 4)()
+        load().or_trace_bvh(self.h, _p(np.ascontiguousarray(rays)), rays.shape[0], _p(hits), threads, cnt)
+        return hits, dict(rays=cnt[0], node_visits=cnt[1], tri_tests=cnt[2], hits=cnt[3])
+
+    def __del__(self):
+        try:
+            load().or_bvh_free(self.h)
+        except Exception:
+            pass
+
+
+def trace_bvh4(nodes, tris, depth: int, rays: np.ndarray, threads: int = 8):
+    from master_thesis_amd._lib import HIT_DTYPE
+    hits = np.zeros(rays.shape[0], HIT_DTYPE)
+    cnt = (C.c_uint64 * 4)()
+    load().or_trace_bvh4(_p(nodes), nodes.shape[0], _p(tris), tris.shape[0], depth, _p(np.ascontiguousarray(rays)),
+                         rays.shape[0], _p(hits), threads, cnt)
+    return hits, dict(rays=cnt[0], node_visits=cnt[1], tri_tests=cnt[2], hits=cnt[3])
+
+
+def check_bvh4(nodes, tris, scene):
+    d = C.c_uint32(0)
+    pos = np.ascontiguousarray(scene.pos, np.float32)
+    idx = np.ascontiguousarray(scene.idx, np.uint32)
+    rc = load().or_check_bvh4(_p(nodes), nodes.shape[0], _p(tris), tris.shape[0], _p(pos), _p(idx), C.byref(d))
+    return rc, d.value
+
+
+def primary_dir(cam, px: int, py: int) -> np.ndarray:
+    vi = np.array(cam.view_inv[:], np.float32)
+    pi = np.array(cam.proj_inv[:], np.float32)
+    out = np.zeros(3, np.float32)
+    load().or_primary_dir(fptr(vi), fptr(pi), px, py, cam.width, cam.height, fptr(out))
+    return out
+
+
+def _scene_ptrs(scene):
+    arrs = [np.ascontiguousarray(a) for a in (scene.pos, scene.nrm, scene.idx, scene.tri_mat, scene.materials)]
+    return arrs, [_p(a) for a in arrs]
+
+
+def render(scene, cam, settings, threads: int = 8):
+    """Returns (rgb float32 HxWx3, rgb8 uint8 HxWx3)."""
+    arrs, ps = _scene_ptrs(scene)
+    H, W = cam.height, cam.width
+    rgb = np.zeros((H, W, 3), np.float32)
+    rgb8 = np.zeros((H, W, 3), np.uint8)
+    rc = load().or_render(*ps, arrs[0].shape[0], arrs[2].shape[0], arrs[4].shape[0], C.addressof(cam),
+                          C.addressof(settings), _p(rgb), _p(rgb8), threads)
+    assert rc == 0, rc
+    return rgb, rgb8
+
+
+def render_adjoint(scene, cam, settings, d_image: np.ndarray, threads: int = 8) -> np.ndarray:
+    arrs, ps = _scene_ptrs(scene)
+    M = arrs[4].shape[0]
+    d_color = np.zeros((M, 3), np.float32)
+    d_image = np.ascontiguousarray(d_image, np.float32)
+    rc = load().or_render_adjoint(*ps, arrs[0].shape[0], arrs[2].shape[0], M, C.addressof(cam),
+                                  C.addressof(settings), _p(d_image), _p(d_color), threads)
+    assert rc == 0, rc
+    return d_color

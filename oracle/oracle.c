@@ -1,0 +1,949 @@
+/*
+ * oracle.c — CPU restatement of the reference software renderer's hot path.
+ *
+ * TEST INFRASTRUCTURE.  This is the checker: tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg load it; libmtrt.so never does.
+ *
+ * Build: oracle/Makefile (gcc -O2 -ffp-contract=off; plain IEEE float, no
+ * fast-math), output oracle/liboracle.so.
+ *
+ * Reference files restated here (paths under software_renderer/):
+ *   graphic/triangle.cpp:4-55        Triangle ctor, Moller-Trumbore, barycentrics
+ *   graphic/graphics_object.cpp:64-87 closest hit by squared distance, strict <
+ *   graphic/scene.cpp:22-56          two-level query, normal interpolation
+ *   graphic/path_tracer.cpp:18-97    the estimator
+ *   graphic/graphics_engine.cpp:59-97 per-pixel work loop, clamp, 8-bit store
+ *   math/vector.h:202-241,288-345    normalize (double), dot, cross, reflect, refract
+ *   math/matrix.h:245-253            row-vector product
+ *   math/random.cpp:16-35            sphere / hemisphere sampling
+ * Deliberate differences, each documented where it applies: the random
+ * numbers come from Philox-4x32-10 instead of a mutex-guarded mt19937 seeded
+ * from random_device (the reference is not reproducible run to run), sin/cos
+ * of the sampling angle use a fixed polynomial (identical on the GPU), and
+ * exact squared-distance ties go to the lowest triangle index (the reference
+ * keeps whichever its tree order meets first).
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdatomic.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/rt.h"
+
+typedef struct { float x, y, z; } v3;
+
+static inline v3 V3(float x, float y, float z) { v3 r; r.x = x; r.y = y; r.z = z; return r; }
+static inline v3 vld(const float* p) { return V3(p[0], p[1], p[2]); }
+static inline v3 vsub(v3 a, v3 b) { return V3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 vadd(v3 a, v3 b) { return V3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 vmul(v3 a, v3 b) { return V3(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline v3 vscl(float s, v3 a) { return V3(s * a.x, s * a.y, s * a.z); }
+/* vector.h:235-241 */
+static inline float vdot(v3 a, v3 b) {
+    float s = 0.0f;
+    s = s + a.x * b.x;
+    s = s + a.y * b.y;
+    s = s + a.z * b.z;
+    return s;
+}
+/* vector.h:288-297 */
+static inline v3 vcross(v3 a, v3 b) {
+    return V3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+/* vector.h:202-225 */
+static inline v3 vnormalize(v3 a) {
+    float m2 = vdot(a, a);
+    double m = sqrt((double)m2);
+    if (m != 0.0) {
+        double f = 1.0 / m;
+        a.x = (float)((double)a.x * f);
+        a.y = (float)((double)a.y * f);
+        a.z = (float)((double)a.z * f);
+    }
+    return a;
+}
+
+/* ------------------------------------------------------------------ */
+/* worker pool: threads pull chunks of [0, n) from an atomic counter   */
+typedef void (*range_fn)(void* arg, uint32_t b, uint32_t e, int tid);
+typedef struct {
+    range_fn fn;
+    void* arg;
+    uint32_t n, chunk;
+    atomic_uint next;
+    int tid;
+} pool_job;
+typedef struct { pool_job* job; int tid; } pool_slot;
+
+static void* pool_worker(void* p) {
+    pool_slot* s = (pool_slot*)p;
+    pool_job* j = s->job;
+    for (;;) {
+        uint32_t b = atomic_fetch_add(&j->next, j->chunk);
+        if (b >= j->n) break;
+        uint32_t e = b + j->chunk < j->n ? b + j->chunk : j->n;
+        j->fn(j->arg, b, e, s->tid);
+    }
+    return NULL;
+}
+
+static void run_parallel(int threads, uint32_t n, uint32_t chunk, range_fn fn, void* arg) {
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    pool_job job;
+    job.fn = fn; job.arg = arg; job.n = n; job.chunk = chunk ? chunk : 1;
+    atomic_init(&job.next, 0u);
+    pthread_t th[256];
+    pool_slot slot[256];
+    for (int t = 0; t < threads; ++t) {
+        slot[t].job = &job;
+        slot[t].tid = t;
+    }
+    for (int t = 1; t < threads; ++t) pthread_create(&th[t], NULL, pool_worker, &slot[t]);
+    pool_worker(&slot[0]);
+    for (int t = 1; t < threads; ++t) pthread_join(th[t], NULL);
+}
+
+/* ------------------------------------------------------------------ */
+/* Philox-4x32-10 */
+void or_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0, n1 = lo1, n2 = hi0 ^ c3 ^ k1, n3 = lo0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+static inline float u01(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777216.0f); }
+
+void or_sincos_2pi(float x, float* s, float* c) {
+    float e = x * 8.0f;
+    int o = (int)e;
+    float r = e - (float)o;
+    if (o & 1) r = 1.0f - r;
+    float a = r * 0x1.921fb6p-1f;
+    float a2 = a * a;
+    float ps = a * (1.0f + a2 * (-0x1.555556p-3f + a2 * (0x1.111112p-7f +
+               a2 * (-0x1.a01a02p-13f + a2 * 0x1.71de3ap-19f))));
+    float pc = 1.0f + a2 * (-0.5f + a2 * (0x1.555556p-5f + a2 * (-0x1.6c16c2p-10f +
+               a2 * (0x1.a01a02p-16f + a2 * -0x1.27e4fcp-22f))));
+    float sq = (o & 1) ? pc : ps;
+    float cq = (o & 1) ? ps : pc;
+    switch (o >> 1) {
+        case 0: *s = sq;  *c = cq;  break;
+        case 1: *s = cq;  *c = -sq; break;
+        case 2: *s = -sq; *c = -cq; break;
+        default: *s = -cq; *c = sq; break;
+    }
+}
+
+static inline v3 hemisphere(float xi1, float xi2, v3 n) {
+    float st, ct;
+    or_sincos_2pi(xi1, &st, &ct);
+    float z = 2.0f * xi2 - 1.0f;
+    float r = sqrtf(1.0f - z * z);
+    v3 d = V3(r * ct, r * st, z);
+    if (vdot(d, n) < 0.0f) d = V3(-d.x, -d.y, -d.z);
+    return d;
+}
+
+void or_hemisphere(float xi1, float xi2, const float n[3], float out[3]) {
+    v3 d = hemisphere(xi1, xi2, vld(n));
+    out[0] = d.x; out[1] = d.y; out[2] = d.z;
+}
+
+/* ------------------------------------------------------------------ */
+/* triangle.cpp:21-45 */
+static inline int mt(v3 o, v3 d, v3 v0, v3 e0, v3 e1, float* t, float* u, float* v) {
+    const float EPS = 1e-7f;
+    v3 h = vcross(d, e1);
+    float a = vdot(e0, h);
+    if (a > -EPS && a < EPS) return 0;
+    float f = 1.0f / a;
+    v3 s = vsub(o, v0);
+    *u = f * vdot(s, h);
+    if (*u < 0.0f || *u > 1.0f) return 0;
+    v3 q = vcross(s, e0);
+    *v = f * vdot(d, q);
+    if (*v < 0.0f || *u + *v > 1.0f) return 0;
+    *t = f * vdot(e1, q);
+    return *t > EPS;
+}
+
+int or_intersect(const float o[3], const float d[3], const float v0[3], const float v1[3], const float v2[3],
+                 float out[3]) {
+    v3 a = vld(v0);
+    float t, u, v;
+    if (!mt(vld(o), vld(d), a, vsub(vld(v1), a), vsub(vld(v2), a), &t, &u, &v)) return 0;
+    out[0] = t; out[1] = u; out[2] = v;
+    return 1;
+}
+
+/* triangle.cpp:47-55 with the ctor's d00/d01/d11/denom (triangle.cpp:10-13) */
+static inline void bary(v3 v0, v3 e0, v3 e1, v3 p, float* b0, float* b1, float* b2) {
+    float d00 = vdot(e0, e0), d01 = vdot(e0, e1), d11 = vdot(e1, e1);
+    float denom = d00 * d11 - d01 * d01;
+    v3 e2 = vsub(p, v0);
+    float d20 = vdot(e2, e0), d21 = vdot(e2, e1);
+    float v = (d11 * d20 - d01 * d21) / denom;
+    float w = (d00 * d21 - d01 * d20) / denom;
+    *b0 = 1.0f - v - w;
+    *b1 = v;
+    *b2 = w;
+}
+
+void or_barycentric(const float v0[3], const float v1[3], const float v2[3], const float p[3], float out[3]) {
+    v3 a = vld(v0);
+    bary(a, vsub(vld(v1), a), vsub(vld(v2), a), vld(p), &out[0], &out[1], &out[2]);
+}
+
+/* graphics_object.cpp:75-83: keep the hit with the smallest |o - p|^2,
+ * strict <; exact ties to the lowest index (the documented divergence). */
+static inline int closer(float d2a, uint32_t ida, float d2b, uint32_t idb) {
+    return d2a < d2b || (d2a == d2b && ida < idb);
+}
+
+typedef struct {
+    float d2, t, u, v;
+    uint32_t id;
+} best_t;
+
+static inline void consider(v3 o, v3 d, float tmin, float tmax, v3 v0, v3 e0, v3 e1, uint32_t id, best_t* b) {
+    float t, u, v;
+    if (mt(o, d, v0, e0, e1, &t, &u, &v) && t > tmin && t < tmax) {
+        v3 p = vadd(o, vscl(t, d));
+        v3 w = vsub(o, p);
+        float d2 = vdot(w, w);
+        if (closer(d2, id, b->d2, b->id)) {
+            b->d2 = d2; b->t = t; b->u = u; b->v = v; b->id = id;
+        }
+    }
+}
+
+static inline void store_hit(rt_hit* h, const best_t* b) {
+    if (b->id == RT_MISS) {
+        h->tri = RT_MISS; h->t = 0.0f; h->u = 0.0f; h->v = 0.0f;
+    } else {
+        h->tri = b->id; h->t = b->t; h->u = b->u; h->v = b->v;
+    }
+}
+
+typedef struct {
+    const float* pos;
+    const uint32_t* idx;
+    uint32_t T;
+    const rt_ray* rays;
+    rt_hit* hits;
+} brute_job;
+
+static void brute_range(void* arg, uint32_t b, uint32_t e, int tid) {
+    (void)tid;
+    brute_job* j = (brute_job*)arg;
+    for (uint32_t r = b; r < e; ++r) {
+        const rt_ray* ry = &j->rays[r];
+        v3 o = V3(ry->ox, ry->oy, ry->oz), d = V3(ry->dx, ry->dy, ry->dz);
+        best_t bt;
+        bt.d2 = INFINITY; bt.id = RT_MISS; bt.t = bt.u = bt.v = 0.0f;
+        for (uint32_t t = 0; t < j->T; ++t) {
+            v3 v0 = vld(j->pos + 3 * j->idx[3 * t]);
+            v3 v1 = vld(j->pos + 3 * j->idx[3 * t + 1]);
+            v3 v2 = vld(j->pos + 3 * j->idx[3 * t + 2]);
+            consider(o, d, ry->tmin, ry->tmax, v0, vsub(v1, v0), vsub(v2, v0), t, &bt);
+        }
+        store_hit(&j->hits[r], &bt);
+    }
+}
+
+void or_trace_brute(const float* pos, const uint32_t* idx, uint32_t T, const void* rays, uint32_t n, void* hits,
+                    int threads) {
+    brute_job j = {pos, idx, T, (const rt_ray*)rays, (rt_hit*)hits};
+    run_parallel(threads, n, 64, brute_range, &j);
+}
+
+/* ------------------------------------------------------------------ */
+/* culling shared by both BVH traversals (same rule as the GPU kernel) */
+static inline float safe_rcp(float d) {
+    const float tiny = 8.27180613e-25f; /* 2^-80 */
+    float a = fabsf(d) < tiny ? copysignf(tiny, d) : d;
+    return 1.0f / a;
+}
+
+typedef struct {
+    v3 o, d;
+    float tmin, tmax, tlo, sigma;
+    float ix, iy, iz, nox, noy, noz;
+    int valid;
+} ray_pre;
+
+static inline ray_pre prep(const rt_ray* r) {
+    ray_pre p;
+    p.o = V3(r->ox, r->oy, r->oz);
+    p.d = V3(r->dx, r->dy, r->dz);
+    float dmax = fmaxf(fabsf(p.d.x), fmaxf(fabsf(p.d.y), fabsf(p.d.z)));
+    float omax = fmaxf(fabsf(p.o.x), fmaxf(fabsf(p.o.y), fabsf(p.o.z)));
+    p.valid = (dmax > 0.0f) && (dmax < INFINITY) && (omax < INFINITY);
+    p.tmin = r->tmin;
+    p.tmax = r->tmax;
+    p.tlo = fmaxf(r->tmin, 0.0f);
+    p.sigma = p.valid ? 3.814697265625e-06f * omax / dmax : 0.0f;
+    p.ix = safe_rcp(p.d.x); p.iy = safe_rcp(p.d.y); p.iz = safe_rcp(p.d.z);
+    p.nox = -p.o.x * p.ix; p.noy = -p.o.y * p.iy; p.noz = -p.o.z * p.iz;
+    return p;
+}
+
+static inline float slab(const ray_pre* p, float lx, float ly, float lz, float hx, float hy, float hz, float tcull,
+                         int* hit) {
+    float x0 = fmaf(lx, p->ix, p->nox), x1 = fmaf(hx, p->ix, p->nox);
+    float y0 = fmaf(ly, p->iy, p->noy), y1 = fmaf(hy, p->iy, p->noy);
+    float z0 = fmaf(lz, p->iz, p->noz), z1 = fmaf(hz, p->iz, p->noz);
+    float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), p->tlo));
+    float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tcull));
+    *hit = tn <= tf;
+    return tn;
+}
+
+static inline float next_cull(const ray_pre* p, const best_t* b) {
+    return fminf(p->tmax, fmaf(b->t, 1.0009765625f, p->sigma));
+}
+
+/* ------------------------------------------------------------------ */
+/* binary BVH, binned SAH */
+typedef struct {
+    float lo[3], hi[3];
+    uint32_t a;   /* leaf: first triangle record; inner: left child (right = a + 1) */
+    uint32_t cnt; /* leaf: triangle count; inner: 0 */
+} bnode;
+
+struct or_bvh {
+    bnode* nodes;
+    uint32_t nn;
+    float* rec; /* per leaf-ordered triangle: v0[3], e0[3], e1[3], id (as bits) = 10 floats */
+    uint32_t T;
+};
+
+typedef struct {
+    float lo[3], hi[3];
+    float c[3];
+} tbox;
+
+uint32_t or_bvh_num_nodes(const or_bvh* b) { return b->nn; }
+
+static void box_empty(float lo[3], float hi[3]) {
+    for (int k = 0; k < 3; ++k) { lo[k] = INFINITY; hi[k] = -INFINITY; }
+}
+static void box_grow(float lo[3], float hi[3], const float l[3], const float h[3]) {
+    for (int k = 0; k < 3; ++k) { lo[k] = fminf(lo[k], l[k]); hi[k] = fmaxf(hi[k], h[k]); }
+}
+static float box_area(const float lo[3], const float hi[3]) {
+    float x = hi[0] - lo[0], y = hi[1] - lo[1], z = hi[2] - lo[2];
+    if (!(x >= 0.0f)) return 0.0f;
+    return x * y + y * z + z * x;
+}
+
+or_bvh* or_bvh_build(const float* pos, const uint32_t* idx, uint32_t T, int leaf_size) {
+    if (leaf_size < 1) leaf_size = 4;
+    or_bvh* b = (or_bvh*)calloc(1, sizeof(or_bvh));
+    tbox* tb = (tbox*)malloc(sizeof(tbox) * (T ? T : 1));
+    uint32_t* order = (uint32_t*)malloc(4 * (size_t)(T ? T : 1));
+    b->nodes = (bnode*)malloc(sizeof(bnode) * (2 * (size_t)T + 1));
+    float smax = 0.0f;
+    for (uint32_t t = 0; t < T; ++t) {
+        for (int k = 0; k < 3; ++k) {
+            float a = pos[3 * idx[3 * t] + k], bb = pos[3 * idx[3 * t + 1] + k], c = pos[3 * idx[3 * t + 2] + k];
+            tb[t].lo[k] = fminf(a, fminf(bb, c));
+            tb[t].hi[k] = fmaxf(a, fmaxf(bb, c));
+            tb[t].c[k] = 0.5f * (tb[t].lo[k] + tb[t].hi[k]);
+            smax = fmaxf(smax, fmaxf(fabsf(tb[t].lo[k]), fabsf(tb[t].hi[k])));
+        }
+        order[t] = t;
+    }
+    typedef struct { uint32_t node, first, count; } item;
+    item* stack = (item*)malloc(sizeof(item) * (2 * (size_t)T + 8));
+    int sp = 0;
+    b->nn = 1;
+    stack[sp++] = (item){0, 0, T};
+    enum { NB = 16 };
+    while (sp > 0) {
+        item it = stack[--sp];
+        bnode* nd = &b->nodes[it.node];
+        float lo[3], hi[3], clo[3], chi[3];
+        box_empty(lo, hi);
+        box_empty(clo, chi);
+        for (uint32_t i = it.first; i < it.first + it.count; ++i) {
+            box_grow(lo, hi, tb[order[i]].lo, tb[order[i]].hi);
+            box_grow(clo, chi, tb[order[i]].c, tb[order[i]].c);
+        }
+        memcpy(nd->lo, lo, sizeof(lo));
+        memcpy(nd->hi, hi, sizeof(hi));
+        if (it.count <= (uint32_t)leaf_size) {
+            nd->a = it.first; nd->cnt = it.count;
+            continue;
+        }
+        int axis = 0;
+        float ext = chi[0] - clo[0];
+        for (int k = 1; k < 3; ++k)
+            if (chi[k] - clo[k] > ext) { ext = chi[k] - clo[k]; axis = k; }
+        uint32_t mid = 0;
+        if (ext > 0.0f) {
+            uint32_t cnt[NB] = {0};
+            float blo[NB][3], bhi[NB][3];
+            for (int q = 0; q < NB; ++q) box_empty(blo[q], bhi[q]);
+            float scale = (float)NB / ext;
+            for (uint32_t i = it.first; i < it.first + it.count; ++i) {
+                int q = (int)((tb[order[i]].c[axis] - clo[axis]) * scale);
+                if (q >= NB) q = NB - 1;
+                if (q < 0) q = 0;
+                cnt[q]++;
+                box_grow(blo[q], bhi[q], tb[order[i]].lo, tb[order[i]].hi);
+            }
+            float best = INFINITY;
+            int bsplit = -1;
+            for (int s = 1; s < NB; ++s) {
+                float l0[3], h0[3], l1[3], h1[3];
+                uint32_t n0 = 0, n1 = 0;
+                box_empty(l0, h0);
+                box_empty(l1, h1);
+                for (int q = 0; q < s; ++q) { n0 += cnt[q]; if (cnt[q]) box_grow(l0, h0, blo[q], bhi[q]); }
+                for (int q = s; q < NB; ++q) { n1 += cnt[q]; if (cnt[q]) box_grow(l1, h1, blo[q], bhi[q]); }
+                if (!n0 || !n1) continue;
+                float cost = box_area(l0, h0) * (float)n0 + box_area(l1, h1) * (float)n1;
+                if (cost < best) { best = cost; bsplit = s; }
+            }
+            if (bsplit > 0) {
+                uint32_t i = it.first, j = it.first + it.count;
+                while (i < j) {
+                    int q = (int)((tb[order[i]].c[axis] - clo[axis]) * scale);
+                    if (q >= NB) q = NB - 1;
+                    if (q < 0) q = 0;
+                    if (q < bsplit) ++i;
+                    else { --j; uint32_t tmp = order[i]; order[i] = order[j]; order[j] = tmp; }
+                }
+                mid = i - it.first;
+            }
+        }
+        if (mid == 0 || mid == it.count) mid = it.count / 2; /* coincident centroids: split by index */
+        uint32_t l = b->nn;
+        b->nn += 2;
+        nd->a = l;
+        nd->cnt = 0;
+        stack[sp++] = (item){l + 1, it.first + mid, it.count - mid};
+        stack[sp++] = (item){l, it.first, mid};
+    }
+    free(stack);
+    /* pad every box outward by 2^-16 of the largest coordinate magnitude */
+    float pad = smax * 1.52587890625e-05f + 1e-30f;
+    for (uint32_t i = 0; i < b->nn; ++i)
+        for (int k = 0; k < 3; ++k) { b->nodes[i].lo[k] -= pad; b->nodes[i].hi[k] += pad; }
+    b->T = T;
+    b->rec = (float*)malloc(sizeof(float) * 10 * (size_t)(T ? T : 1));
+    for (uint32_t i = 0; i < T; ++i) {
+        uint32_t t = order[i];
+        float* r = b->rec + 10 * (size_t)i;
+        for (int k = 0; k < 3; ++k) {
+            float p0 = pos[3 * idx[3 * t] + k];
+            r[k] = p0;
+            r[3 + k] = pos[3 * idx[3 * t + 1] + k] - p0;
+            r[6 + k] = pos[3 * idx[3 * t + 2] + k] - p0;
+        }
+        memcpy(&r[9], &t, 4);
+    }
+    free(tb);
+    free(order);
+    return b;
+}
+
+void or_bvh_free(or_bvh* b) {
+    if (!b) return;
+    free(b->nodes);
+    free(b->rec);
+    free(b);
+}
+
+static best_t trace_bvh_one(const or_bvh* b, const rt_ray* r, uint64_t* c_nodes, uint64_t* c_tris) {
+    best_t bt;
+    bt.d2 = INFINITY; bt.id = RT_MISS; bt.t = bt.u = bt.v = 0.0f;
+    ray_pre p = prep(r);
+    if (!p.valid || b->T == 0) return bt;
+    float tcull = p.tmax;
+    uint32_t stack[128];
+    int sp = 0;
+    int h;
+    const bnode* root = &b->nodes[0];
+    slab(&p, root->lo[0], root->lo[1], root->lo[2], root->hi[0], root->hi[1], root->hi[2], tcull, &h);
+    if (!h) return bt;
+    uint32_t cur = 0;
+    for (;;) {
+        const bnode* nd = &b->nodes[cur];
+        (*c_nodes)++;
+        if (nd->cnt) {
+            for (uint32_t i = 0; i < nd->cnt; ++i) {
+                const float* rc = b->rec + 10 * (size_t)(nd->a + i);
+                uint32_t id;
+                memcpy(&id, &rc[9], 4);
+                (*c_tris)++;
+                float before = bt.d2;
+                uint32_t bid = bt.id;
+                consider(p.o, p.d, p.tmin, p.tmax, vld(rc), vld(rc + 3), vld(rc + 6), id, &bt);
+                if (bt.d2 != before || bt.id != bid) tcull = next_cull(&p, &bt);
+            }
+        } else {
+            const bnode* l = &b->nodes[nd->a];
+            const bnode* rr = &b->nodes[nd->a + 1];
+            int hl, hr;
+            float tl = slab(&p, l->lo[0], l->lo[1], l->lo[2], l->hi[0], l->hi[1], l->hi[2], tcull, &hl);
+            float tr = slab(&p, rr->lo[0], rr->lo[1], rr->lo[2], rr->hi[0], rr->hi[1], rr->hi[2], tcull, &hr);
+            if (hl && hr) {
+                uint32_t nearc = tl <= tr ? nd->a : nd->a + 1, farc = tl <= tr ? nd->a + 1 : nd->a;
+                stack[sp++] = farc;
+                cur = nearc;
+                continue;
+            }
+            if (hl) { cur = nd->a; continue; }
+            if (hr) { cur = nd->a + 1; continue; }
+        }
+        if (sp == 0) break;
+        cur = stack[--sp];
+    }
+    return bt;
+}
+
+typedef struct {
+    const or_bvh* b;
+    const rt_ray* rays;
+    rt_hit* hits;
+    uint64_t cnt[256][4];
+} bvh_job;
+
+static void bvh_range(void* arg, uint32_t b, uint32_t e, int tid) {
+    bvh_job* j = (bvh_job*)arg;
+    for (uint32_t r = b; r < e; ++r) {
+        best_t bt = trace_bvh_one(j->b, &j->rays[r], &j->cnt[tid][1], &j->cnt[tid][2]);
+        store_hit(&j->hits[r], &bt);
+        j->cnt[tid][0]++;
+        j->cnt[tid][3] += bt.id != RT_MISS;
+    }
+}
+
+void or_trace_bvh(const or_bvh* b, const void* rays, uint32_t n, void* hits, int threads, uint64_t counters[4]) {
+    bvh_job* j = (bvh_job*)calloc(1, sizeof(bvh_job));
+    j->b = b;
+    j->rays = (const rt_ray*)rays;
+    j->hits = (rt_hit*)hits;
+    run_parallel(threads, n, 256, bvh_range, j);
+    if (counters) {
+        for (int k = 0; k < 4; ++k) counters[k] = 0;
+        for (int t = 0; t < 256; ++t)
+            for (int k = 0; k < 4; ++k) counters[k] += j->cnt[t][k];
+    }
+    free(j);
+}
+
+/* ------------------------------------------------------------------ */
+/* BVH4 traversal in the GPU kernel's exact order (trace.hip) */
+typedef struct {
+    const rt_bvh4_node* nodes;
+    const rt_tri_rec* tris;
+    const rt_ray* rays;
+    rt_hit* hits;
+    uint32_t stack_cap;
+    uint64_t cnt[256][4];
+} bvh4_job;
+
+static best_t trace_bvh4_one(const bvh4_job* j, const rt_ray* r, uint64_t* c_nodes, uint64_t* c_tris,
+                             uint32_t* stack) {
+    best_t bt;
+    bt.d2 = INFINITY; bt.id = RT_MISS; bt.t = bt.u = bt.v = 0.0f;
+    ray_pre p = prep(r);
+    if (!p.valid) return bt;
+    float tcull = p.tmax;
+    uint32_t sp = 0, cur = 0;
+    for (;;) {
+        if (!(cur & RT_LEAF_BIT)) {
+            const rt_bvh4_node* nd = &j->nodes[cur];
+            (*c_nodes)++;
+            float key[4];
+            uint32_t ref[4];
+            for (int c = 0; c < 4; ++c) {
+                int h;
+                float tn = slab(&p, nd->lo_x[c], nd->lo_y[c], nd->lo_z[c], nd->hi_x[c], nd->hi_y[c], nd->hi_z[c],
+                                tcull, &h);
+                ref[c] = nd->child[c];
+                key[c] = (h && ref[c] != RT_CHILD_EMPTY) ? tn : INFINITY;
+            }
+#define CSWAP(i, k)                                                              \
+    if (key[k] < key[i]) {                                                       \
+        float tk = key[i]; key[i] = key[k]; key[k] = tk;                         \
+        uint32_t tr = ref[i]; ref[i] = ref[k]; ref[k] = tr;                      \
+    }
+            CSWAP(0, 1) CSWAP(2, 3) CSWAP(0, 2) CSWAP(1, 3) CSWAP(1, 2)
+#undef CSWAP
+            if (key[0] < INFINITY) {
+                for (int c = 3; c >= 1; --c)
+                    if (key[c] < INFINITY && sp < j->stack_cap) stack[sp++] = ref[c];
+                cur = ref[0];
+                continue;
+            }
+        } else {
+            uint32_t first = cur & RT_LEAF_FIRST_MASK;
+            uint32_t cnt = ((cur >> RT_LEAF_COUNT_SHIFT) & 15u) + 1u;
+            for (uint32_t i = 0; i < cnt; ++i) {
+                const rt_tri_rec* rc = &j->tris[first + i];
+                (*c_tris)++;
+                float before = bt.d2;
+                uint32_t bid = bt.id;
+                consider(p.o, p.d, p.tmin, p.tmax, vld(rc->v0), vld(rc->e0), vld(rc->e1), rc->id, &bt);
+                if (bt.d2 != before || bt.id != bid) tcull = next_cull(&p, &bt);
+            }
+        }
+        if (sp == 0) break;
+        cur = stack[--sp];
+    }
+    return bt;
+}
+
+static void bvh4_range(void* arg, uint32_t b, uint32_t e, int tid) {
+    bvh4_job* j = (bvh4_job*)arg;
+    uint32_t* stack = (uint32_t*)malloc(4 * (size_t)j->stack_cap);
+    for (uint32_t r = b; r < e; ++r) {
+        best_t bt = trace_bvh4_one(j, &j->rays[r], &j->cnt[tid][1], &j->cnt[tid][2], stack);
+        store_hit(&j->hits[r], &bt);
+        j->cnt[tid][0]++;
+        j->cnt[tid][3] += bt.id != RT_MISS;
+    }
+    free(stack);
+}
+
+void or_trace_bvh4(const void* nodes, uint32_t num_nodes, const void* tris, uint32_t T, uint32_t depth,
+                   const void* rays, uint32_t n, void* hits, int threads, uint64_t counters[4]) {
+    (void)num_nodes;
+    (void)T;
+    bvh4_job* j = (bvh4_job*)calloc(1, sizeof(bvh4_job));
+    j->nodes = (const rt_bvh4_node*)nodes;
+    j->tris = (const rt_tri_rec*)tris;
+    j->rays = (const rt_ray*)rays;
+    j->hits = (rt_hit*)hits;
+    j->stack_cap = 3 * depth + 1;
+    run_parallel(threads, n, 256, bvh4_range, j);
+    if (counters) {
+        for (int k = 0; k < 4; ++k) counters[k] = 0;
+        for (int t = 0; t < 256; ++t)
+            for (int k = 0; k < 4; ++k) counters[k] += j->cnt[t][k];
+    }
+    free(j);
+}
+
+/* structural check: returns 0 when sound */
+int or_check_bvh4(const void* nodes_v, uint32_t num_nodes, const void* tris_v, uint32_t T, const float* pos,
+                  const uint32_t* idx, uint32_t* out_depth) {
+    const rt_bvh4_node* nodes = (const rt_bvh4_node*)nodes_v;
+    const rt_tri_rec* tris = (const rt_tri_rec*)tris_v;
+    uint8_t* seen_tri = (uint8_t*)calloc(T ? T : 1, 1);
+    uint8_t* seen_rec = (uint8_t*)calloc(T ? T : 1, 1);
+    uint8_t* seen_node = (uint8_t*)calloc(num_nodes ? num_nodes : 1, 1);
+    typedef struct { uint32_t node, depth; } it;
+    it* st = (it*)malloc(sizeof(it) * ((size_t)num_nodes * 4 + 4));
+    int sp = 0, rc = 0;
+    uint32_t maxd = 0;
+    st[sp++] = (it){0, 1};
+    while (sp > 0 && !rc) {
+        it x = st[--sp];
+        if (x.node >= num_nodes) { rc = 1; break; }
+        if (seen_node[x.node]) { rc = 2; break; }
+        seen_node[x.node] = 1;
+        if (x.depth > maxd) maxd = x.depth;
+        const rt_bvh4_node* nd = &nodes[x.node];
+        for (int c = 0; c < 4 && !rc; ++c) {
+            uint32_t ch = nd->child[c];
+            if (ch == RT_CHILD_EMPTY) continue;
+            float lo[3] = {nd->lo_x[c], nd->lo_y[c], nd->lo_z[c]}, hi[3] = {nd->hi_x[c], nd->hi_y[c], nd->hi_z[c]};
+            if (ch & RT_LEAF_BIT) {
+                uint32_t first = ch & RT_LEAF_FIRST_MASK, cnt = ((ch >> RT_LEAF_COUNT_SHIFT) & 15u) + 1u;
+                if ((uint64_t)first + cnt > T) { rc = 3; break; }
+                for (uint32_t i = first; i < first + cnt; ++i) {
+                    if (seen_rec[i]) { rc = 4; break; }
+                    seen_rec[i] = 1;
+                    uint32_t t = tris[i].id;
+                    if (t >= T || seen_tri[t]) { rc = 5; break; }
+                    seen_tri[t] = 1;
+                    for (int v = 0; v < 3 && !rc; ++v)
+                        for (int k = 0; k < 3; ++k) {
+                            float x = pos[3 * idx[3 * t + v] + k];
+                            if (!(x >= lo[k] && x <= hi[k])) { rc = 6; break; }
+                        }
+                    /* record equals v0 / v1 - v0 / v2 - v0 */
+                    for (int k = 0; k < 3 && !rc; ++k) {
+                        float p0 = pos[3 * idx[3 * t] + k];
+                        if (tris[i].v0[k] != p0 || tris[i].e0[k] != pos[3 * idx[3 * t + 1] + k] - p0 ||
+                            tris[i].e1[k] != pos[3 * idx[3 * t + 2] + k] - p0)
+                            rc = 7;
+                    }
+                }
+            } else {
+                const rt_bvh4_node* cn = &nodes[ch];
+                for (int cc = 0; cc < 4 && !rc; ++cc) {
+                    if (cn->child[cc] == RT_CHILD_EMPTY) continue;
+                    if (cn->lo_x[cc] < lo[0] || cn->lo_y[cc] < lo[1] || cn->lo_z[cc] < lo[2] ||
+                        cn->hi_x[cc] > hi[0] || cn->hi_y[cc] > hi[1] || cn->hi_z[cc] > hi[2])
+                        rc = 8;
+                }
+                if (cn->meta[0] != x.node) rc = 9;
+                st[sp++] = (it){ch, x.depth + 1};
+            }
+        }
+    }
+    if (!rc)
+        for (uint32_t t = 0; t < T; ++t)
+            if (!seen_tri[t]) { rc = 10; break; }
+    if (!rc)
+        for (uint32_t i = 0; i < num_nodes; ++i)
+            if (!seen_node[i]) { rc = 11; break; }
+    if (out_depth) *out_depth = maxd;
+    free(seen_tri); free(seen_rec); free(seen_node); free(st);
+    return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* camera rays, math/matrix.h:245-253 and graphic/path_tracer.cpp:19-23 */
+void or_vec4_mat(const float m[16], const float v[4], float out[4]) {
+    for (int j = 0; j < 4; ++j) out[j] = 0.0f;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) out[j] = out[j] + m[4 * i + j] * v[i];
+}
+
+static v3 primary(const rt_camera* cam, uint32_t px, uint32_t py) {
+    float cx = (float)px + 0.5f, cy = (float)py + 0.5f;
+    float ux = cx / (float)cam->width, uy = cy / (float)cam->height;
+    float dx = 2.0f * ux - 1.0f, dy = 2.0f * uy - 1.0f;
+    float v4[4] = {dx, dy, 1.0f, 1.0f}, t4[4];
+    or_vec4_mat(cam->proj_inv, v4, t4);
+    v3 target = vnormalize(V3(t4[0], t4[1], t4[2]));
+    float w4[4] = {target.x, target.y, target.z, 0.0f}, d4[4];
+    or_vec4_mat(cam->view_inv, w4, d4);
+    return vnormalize(V3(d4[0], d4[1], d4[2]));
+}
+
+void or_primary_dir(const float view_inv[16], const float proj_inv[16], uint32_t px, uint32_t py, uint32_t w,
+                    uint32_t h, float out[3]) {
+    rt_camera cam;
+    memcpy(cam.view_inv, view_inv, sizeof(cam.view_inv));
+    memcpy(cam.proj_inv, proj_inv, sizeof(cam.proj_inv));
+    cam.width = w;
+    cam.height = h;
+    v3 d = primary(&cam, px, py);
+    out[0] = d.x; out[1] = d.y; out[2] = d.z;
+}
+
+/* ------------------------------------------------------------------ */
+/* vector.h:319-345 */
+static v3 reflect_ref(v3 I, v3 N) {
+    double k = 2.0 * (double)vdot(N, I);
+    v3 kn = V3((float)(k * (double)N.x), (float)(k * (double)N.y), (float)(k * (double)N.z));
+    return vsub(I, kn);
+}
+static v3 refract_ref(v3 I, v3 N, float eta) {
+    float ndi = vdot(N, I);
+    float k = (float)(1.0 - (double)(eta * eta) * (1.0 - (double)(ndi * ndi)));
+    if (k < 0.0f) return V3(0.0f, 0.0f, 0.0f);
+    v3 ei = vscl(eta, I);
+    double m = (double)(eta * ndi) + sqrt((double)k);
+    v3 mn = V3((float)(m * (double)N.x), (float)(m * (double)N.y), (float)(m * (double)N.z));
+    return vsub(ei, mn);
+}
+static v3 custom_refract_ref(v3 I, v3 N, float eta) {
+    if ((double)vdot(N, I) > 0.0) N = V3(-N.x, -N.y, -N.z);
+    else eta = (float)(1.0 / (double)eta);
+    float angle = (float)(sin(acos((double)vdot(N, I))) * (double)eta);
+    if (-1.0f < angle && angle < 1.0f) return refract_ref(I, N, eta);
+    return reflect_ref(I, N);
+}
+
+/* ------------------------------------------------------------------ */
+/* the path tracer */
+typedef struct {
+    const float* pos;
+    const float* nrm;
+    const uint32_t* idx;
+    const uint32_t* tri_mat;
+    const rt_material* mat;
+    uint32_t M;
+    const rt_camera* cam;
+    const rt_settings* s;
+    or_bvh* bvh;
+    double scale;
+    float* rgb;
+    uint8_t* rgb8;
+    const float* dimg;
+    double* dcol; /* [256 threads][3M] */
+} pt_job;
+
+#define OR_MAX_DEPTH 64
+
+/* One sample of graphic/path_tracer.cpp:42-97.  Returns 1 and the
+ * contribution when the path ends on a light (or escapes to the sky);
+ * nv / vm / vc receive the diffuse vertices for the adjoint. */
+static int trace_path(const pt_job* j, uint32_t pix, uint32_t gsmp, v3 dir0, v3* contrib, v3* term, uint32_t* nv,
+                      uint32_t* vm, float* vc) {
+    const rt_settings* s = j->s;
+    v3 o = V3(j->cam->origin[0], j->cam->origin[1], j->cam->origin[2]);
+    v3 d = dir0;
+    v3 T = V3(1.0f, 1.0f, 1.0f);
+    int cull = 1;
+    *nv = 0;
+    *term = V3(1.0f, 1.0f, 1.0f);
+    uint32_t key[2] = {(uint32_t)s->seed, (uint32_t)(s->seed >> 32)};
+    for (uint32_t depth = 0; depth < s->max_depth; ++depth) {
+        rt_ray r;
+        r.ox = o.x; r.oy = o.y; r.oz = o.z; r.tmin = 0.0f;
+        r.dx = d.x; r.dy = d.y; r.dz = d.z; r.tmax = INFINITY;
+        uint64_t cn = 0, ct = 0;
+        best_t h = trace_bvh_one(j->bvh, &r, &cn, &ct);
+        if (h.id == RT_MISS) {
+            if (s->flags & RT_RENDER_SKY) {
+                *term = V3(s->sky[0], s->sky[1], s->sky[2]);
+                *contrib = vmul(T, *term);
+                return 1;
+            }
+            return 0;
+        }
+        v3 p = vadd(o, vscl(h.t, d));
+        uint32_t t = h.id;
+        v3 v0 = vld(j->pos + 3 * j->idx[3 * t]);
+        v3 e0 = vsub(vld(j->pos + 3 * j->idx[3 * t + 1]), v0);
+        v3 e1 = vsub(vld(j->pos + 3 * j->idx[3 * t + 2]), v0);
+        float b0, b1, b2;
+        bary(v0, e0, e1, p, &b0, &b1, &b2);
+        /* scene.cpp:49-53 */
+        v3 n = V3(0.0f, 0.0f, 0.0f);
+        n = vadd(n, vscl(b0, vld(j->nrm + 3 * j->idx[3 * t])));
+        n = vadd(n, vscl(b1, vld(j->nrm + 3 * j->idx[3 * t + 1])));
+        n = vadd(n, vscl(b2, vld(j->nrm + 3 * j->idx[3 * t + 2])));
+        n = vnormalize(n);
+        float ndotd = vdot(n, d);
+        if (cull && ndotd > 0.0f) {
+            o = vadd(p, vscl(0.01f, d));
+            continue;
+        }
+        cull = 0;
+        const rt_material* m = &j->mat[j->tri_mat[t]];
+        uint32_t ctr[4] = {pix, gsmp, depth, 0u}, rnd[4];
+        or_philox(ctr, key, rnd);
+        float xi0 = u01(rnd[0]);
+        v3 prev = d;
+        int diffuse = xi0 <= m->diffuse_threshold;
+        if (diffuse) d = hemisphere(u01(rnd[1]), u01(rnd[2]), n);
+        else if (xi0 <= m->reflect_threshold) d = reflect_ref(d, n);
+        else if (xi0 <= m->transparent_threshold) d = custom_refract_ref(d, n, m->refraction_index);
+        o = vadd(p, vscl(0.01f, d));
+        if (diffuse) {
+            float cs = vdot(n, d);
+            T = vmul(T, vscl(cs, V3(m->color[0], m->color[1], m->color[2])));
+            if (*nv < OR_MAX_DEPTH) { vm[*nv] = j->tri_mat[t]; vc[*nv] = cs; }
+            (*nv)++;
+        }
+        if (m->light_source && vdot(n, prev) <= 0.0f) {
+            *contrib = T;
+            return 1;
+        }
+    }
+    return 0;
+}
+
+static void pt_range(void* arg, uint32_t b, uint32_t e, int tid) {
+    pt_job* j = (pt_job*)arg;
+    const rt_camera* cam = j->cam;
+    uint32_t vm[OR_MAX_DEPTH];
+    float vc[OR_MAX_DEPTH];
+    for (uint32_t pix = b; pix < e; ++pix) {
+        v3 dir = primary(cam, pix % cam->width, pix / cam->width);
+        float sum[3] = {0.0f, 0.0f, 0.0f};
+        for (uint32_t smp = 0; smp < j->s->spp; ++smp) {
+            v3 c, term;
+            uint32_t nv;
+            if (!trace_path(j, pix, j->s->sample_offset + smp, dir, &c, &term, &nv, vm, vc)) continue;
+            if (j->dimg) {
+                float w[3], tm[3] = {term.x, term.y, term.z};
+                for (int ch = 0; ch < 3; ++ch) w[ch] = j->dimg[3 * (size_t)pix + ch] * (float)j->scale;
+                for (int ch = 0; ch < 3; ++ch) {
+                    if (w[ch] == 0.0f) continue;
+                    float f[OR_MAX_DEPTH], suf[OR_MAX_DEPTH + 1], pre = 1.0f;
+                    for (uint32_t k = 0; k < nv; ++k) f[k] = j->mat[vm[k]].color[ch] * vc[k];
+                    suf[nv] = tm[ch];
+                    for (int k = (int)nv - 1; k >= 0; --k) suf[k] = f[k] * suf[k + 1];
+                    for (uint32_t k = 0; k < nv; ++k) {
+                        float g = pre * suf[k + 1] * vc[k];
+                        j->dcol[(size_t)tid * 3 * j->M + 3 * vm[k] + ch] += (double)(w[ch] * g);
+                        pre = pre * f[k];
+                    }
+                }
+            } else {
+                sum[0] = sum[0] + c.x;
+                sum[1] = sum[1] + c.y;
+                sum[2] = sum[2] + c.z;
+            }
+        }
+        if (!j->dimg) {
+            for (int ch = 0; ch < 3; ++ch) {
+                float v = (float)((double)sum[ch] * j->scale);
+                j->rgb[3 * (size_t)pix + ch] = v;
+                if (j->rgb8) {
+                    float cv = fminf(fmaxf(v, 0.0f), 1.0f);
+                    j->rgb8[3 * (size_t)pix + ch] = (uint8_t)(int)(cv * 255.0f);
+                }
+            }
+        }
+    }
+}
+
+static int pt_run(const float* pos, const float* nrm, const uint32_t* idx, const uint32_t* tri_mat, const void* mat,
+                  uint32_t V, uint32_t T, uint32_t M, const void* cam, const void* settings, float* rgb,
+                  uint8_t* rgb8, const float* dimg, float* dcol, int threads) {
+    (void)V;
+    const rt_settings* s = (const rt_settings*)settings;
+    if (s->spp == 0 || s->max_depth == 0 || s->max_depth > OR_MAX_DEPTH) return -1;
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    pt_job j;
+    memset(&j, 0, sizeof(j));
+    j.pos = pos; j.nrm = nrm; j.idx = idx; j.tri_mat = tri_mat; j.mat = (const rt_material*)mat; j.M = M;
+    j.cam = (const rt_camera*)cam;
+    j.s = s;
+    j.bvh = or_bvh_build(pos, idx, T, 4);
+    /* finalColor *= 2.0f * M_PI * (1.0f / float(raysPerPixel)), path_tracer.cpp:37 */
+    j.scale = 2.0 * 3.14159265358979323846 * (double)(1.0f / (float)s->spp);
+    j.rgb = rgb; j.rgb8 = rgb8; j.dimg = dimg;
+    if (dimg) j.dcol = (double*)calloc((size_t)threads * 3 * M, sizeof(double));
+    run_parallel(threads, j.cam->width * j.cam->height, 16, pt_range, &j);
+    if (dimg) {
+        for (uint32_t i = 0; i < 3 * M; ++i) {
+            double acc = 0.0;
+            for (int t = 0; t < threads; ++t) acc += j.dcol[(size_t)t * 3 * M + i];
+            dcol[i] = (float)acc;
+        }
+        free(j.dcol);
+    }
+    or_bvh_free(j.bvh);
+    return 0;
+}
+
+int or_render(const float* pos, const float* nrm, const uint32_t* idx, const uint32_t* tri_mat, const void* mat,
+              uint32_t V, uint32_t T, uint32_t M, const void* cam, const void* settings, float* rgb, uint8_t* rgb8,
+              int threads) {
+    return pt_run(pos, nrm, idx, tri_mat, mat, V, T, M, cam, settings, rgb, rgb8, NULL, NULL, threads);
+}
+
+int or_render_adjoint(const float* pos, const float* nrm, const uint32_t* idx, const uint32_t* tri_mat,
+                      const void* mat, uint32_t V, uint32_t T, uint32_t M, const void* cam, const void* settings,
+                      const float* d_image, float* d_color, int threads) {
+    return pt_run(pos, nrm, idx, tri_mat, mat, V, T, M, cam, settings, NULL, NULL, d_image, d_color, threads);
+}

@@ -1,0 +1,90 @@
+/*
+ * oracle.h — CPU restatement of the reference software renderer's hot path.
+ *
+ * TEST INFRASTRUCTURE.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load this library, and only as the checker (or the timed
+ * CPU baseline).  The product path (libmtrt.so) never calls it.
+ *
+ * Every function states the reference file:line it follows (paths relative
+ * to the reference's software_renderer/).  The arithmetic is plain float,
+ * compiled with -ffp-contract=off, in the reference's operation order.
+ *
+ * Pinning (what checks each function independently of this restatement) is
+ * listed per function; see tests/test_oracle.py.
+ */
+#ifndef MT_ORACLE_H
+#define MT_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Ray / hit / material / BVH4 layouts are those of include/rt.h. */
+
+/* Philox-4x32-10 (Salmon et al. SC'11).  Pinned by the published
+ * known-answer vectors (tests/test_oracle.py). */
+void or_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* sin/cos of 2*pi*x (x = k*2^-24).  Pinned against libm within 2 ulp and
+ * by sin^2 + cos^2 = 1. */
+void or_sincos_2pi(float x, float* s, float* c);
+/* math/random.cpp:16-35.  Pinned by distribution moments. */
+void or_hemisphere(float xi1, float xi2, const float n[3], float out[3]);
+
+/* Triangle ctor + rayIntersects, graphic/triangle.cpp:4-17,21-45.
+ * out = {t, u, v}; returns 1 on a hit.  Pinned by closed forms and the
+ * compiled reference (oracle/_ref). */
+int or_intersect(const float o[3], const float d[3], const float v0[3], const float v1[3], const float v2[3],
+                 float out[3]);
+/* Triangle::getBarycentricCoords, triangle.cpp:47-55. */
+void or_barycentric(const float v0[3], const float v1[3], const float v2[3], const float p[3], float out[3]);
+
+/* Closest hit by brute force over all triangles: the definition of the
+ * result (graphics_object.cpp:64-87 + scene.cpp:22-41 with the lowest-index
+ * tie rule).  rays: rt_ray[n], hits: rt_hit[n]. */
+void or_trace_brute(const float* pos, const uint32_t* idx, uint32_t T, const void* rays, uint32_t n, void* hits,
+                    int threads);
+
+/* CPU BVH (binary, binned SAH) over the same triangles; traversal with the
+ * same conservative culling as the GPU.  Used as the CPU baseline and by the
+ * path-tracer restatement.  counters (may be NULL): rays, nodes, tris, hits. */
+typedef struct or_bvh or_bvh;
+or_bvh* or_bvh_build(const float* pos, const uint32_t* idx, uint32_t T, int leaf_size);
+void or_bvh_free(or_bvh* b);
+uint32_t or_bvh_num_nodes(const or_bvh* b);
+void or_trace_bvh(const or_bvh* b, const void* rays, uint32_t n, void* hits, int threads, uint64_t counters[4]);
+
+/* Traversal of an exported GPU BVH4 (rt_bvh4_node[], rt_tri_rec[]) with the
+ * GPU kernel's exact visiting order: same hits and same node / triangle
+ * counts as the GPU.  counters: rays, nodes, tris, hits. */
+void or_trace_bvh4(const void* nodes, uint32_t num_nodes, const void* tris, uint32_t T, uint32_t depth,
+                   const void* rays, uint32_t n, void* hits, int threads, uint64_t counters[4]);
+/* Structural check of an exported BVH4: every triangle in exactly one leaf,
+ * every child box contains its subtree.  Returns 0 if sound, else a code. */
+int or_check_bvh4(const void* nodes, uint32_t num_nodes, const void* tris, uint32_t T, const float* pos,
+                  const uint32_t* idx, uint32_t* out_depth);
+
+/* Row-vector 4x4 product v * M, math/matrix.h:245-253. */
+void or_vec4_mat(const float m[16], const float v[4], float out[4]);
+/* Primary ray direction, graphic/path_tracer.cpp:19-23. */
+void or_primary_dir(const float view_inv[16], const float proj_inv[16], uint32_t px, uint32_t py, uint32_t w,
+                    uint32_t h, float out[3]);
+
+/* The diffuse path tracer, graphic/path_tracer.cpp:18-97 driven per pixel as
+ * graphic/graphics_engine.cpp:59-97 does (atomic pixel counter, `threads`
+ * workers), with Philox (pixel, sample, depth, 0) for random numbers.
+ * scene: rt_scene_desc fields.  cam: rt_camera.  s: rt_settings.
+ * rgb: W*H*3 floats, rgb8 (may be NULL): W*H*3 bytes. */
+int or_render(const float* pos, const float* nrm, const uint32_t* idx, const uint32_t* tri_mat, const void* mat,
+              uint32_t V, uint32_t T, uint32_t M, const void* cam, const void* settings, float* rgb, uint8_t* rgb8,
+              int threads);
+/* Path-replay gradient of sum(d_image * image) with respect to every
+ * material colour (3*M doubles accumulated, returned as float). */
+int or_render_adjoint(const float* pos, const float* nrm, const uint32_t* idx, const uint32_t* tri_mat,
+                      const void* mat, uint32_t V, uint32_t T, uint32_t M, const void* cam, const void* settings,
+                      const float* d_image, float* d_color, int threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,138 @@
+"""GPU closest-hit parity: every hit (triangle id, t, u, v) bit-identical to
+the oracle's brute force on the same rays, the GPU BVH structurally sound,
+and the GPU traversal's node / triangle counts equal to the oracle's replay
+of the same BVH4."""
+import numpy as np
+import pytest
+
+import oracle as O
+from master_thesis_amd import scenes
+from master_thesis_amd._lib import RAY_DTYPE, RT_MISS, RTError, RT_E_STATE
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits(h):
+    return np.ascontiguousarray(h).view(np.uint32).reshape(-1, 4)
+
+
+def _assert_same_hits(got, want, label=""):
+    g, w = _bits(got), _bits(want)
+    bad = np.nonzero(np.any(g != w, axis=1))[0]
+    assert bad.size == 0, f"{label}: {bad.size} of {len(g)} rays differ; first {bad[:5]}: got {got[bad[:3]]} want {want[bad[:3]]}"
+
+
+def _upload(tracer, sc, leaf=4):
+    tracer.upload(sc)
+    return tracer.build(leaf)
+
+
+def test_cornell_rays_bit_exact(tracer):
+    sc = scenes.cornell_box()
+    _upload(tracer, sc)
+    rng = np.random.default_rng(1)
+    n = 20000
+    r = np.zeros(n, RAY_DTYPE)
+    r["o"] = rng.uniform(-2.7, 2.7, (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    r["d"] = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    r["tmax"] = np.inf
+    got = tracer.trace(r)
+    want = O.trace_brute(sc, r)
+    _assert_same_hits(got, want, "cornell")
+    assert (got["tri"] != RT_MISS).mean() > 0.99  # closed on 5 sides + boxes: most rays hit
+
+
+@pytest.mark.parametrize("T,leaf", [(1, 4), (3, 1), (5, 4), (64, 2), (1000, 4), (1000, 16), (30000, 4)])
+def test_soup_bit_exact_vs_brute(tracer, T, leaf):
+    sc = scenes.triangle_soup(T, seed=T, mu=-2.0 if T < 100 else -4.5)
+    info = _upload(tracer, sc, leaf)
+    assert info["num_triangles"] == T
+    coh, inc = scenes.microbench_rays(64, 32, 3000, seed=T + 1)
+    rays = np.concatenate([coh, inc])
+    got = tracer.trace(rays)
+    want = O.trace_brute(sc, rays)
+    _assert_same_hits(got, want, f"soup T={T} leaf={leaf}")
+
+
+def test_bvh_structure_and_counters(tracer):
+    sc = scenes.triangle_soup(50000, seed=77)
+    info = _upload(tracer, sc, 4)
+    nodes, tris = tracer.export_bvh()
+    rc, depth = O.check_bvh4(nodes, tris, sc)
+    assert rc == 0, f"BVH4 check failed with code {rc}"
+    assert depth == info["depth"]
+    assert nodes.shape[0] == info["num_nodes"]
+    coh, inc = scenes.microbench_rays(128, 64, 8192, seed=3)
+    rays = np.concatenate([coh, inc])
+    got, cnt = tracer.trace(rays, counters=True)
+    want, ocnt = O.trace_bvh4(nodes, tris, info["depth"], rays)
+    _assert_same_hits(got, want, "bvh4 replay")
+    assert cnt == ocnt, (cnt, ocnt)
+    brute = O.trace_brute(sc, rays[::7].copy())
+    _assert_same_hits(got[::7].copy(), brute, "brute subset")
+
+
+def test_large_soup_vs_cpu_bvh(tracer):
+    """1M-triangle soup of the ray-cast benchmark: the GPU agrees with the
+    oracle's own (independently built) BVH on 200k rays."""
+    sc = scenes.triangle_soup(1 << 20, seed=1234)
+    _upload(tracer, sc, 4)
+    coh, inc = scenes.microbench_rays(256, 256, 1 << 17, seed=99)
+    rays = np.concatenate([coh, inc])
+    got = tracer.trace(rays)
+    want, _ = O.CpuBVH(sc).trace(rays)
+    _assert_same_hits(got, want, "1M soup")
+    assert (got["tri"] != RT_MISS).sum() > 1000
+
+
+def test_edge_case_rays(tracer):
+    sc = scenes.cornell_box()
+    _upload(tracer, sc)
+    r = np.zeros(8, RAY_DTYPE)
+    r["tmax"] = np.inf
+    r["o"][0] = (0, 0, 0); r["d"][0] = (0, 0, 0)                 # zero direction: miss
+    r["o"][1] = (0, 0, 0); r["d"][1] = (-1, 0, 0)                # axis aligned
+    r["o"][2] = (0, 0, 0); r["d"][2] = (0, -1, 0)
+    r["o"][3] = (0, -2.75, 0); r["d"][3] = (1, 0, 0)             # origin on the floor plane, grazing
+    r["o"][4] = (0, 0, 0); r["d"][4] = (-1, 0, 0); r["tmax"][4] = 1.0   # tmax before the wall
+    r["o"][5] = (0, 0, 0); r["d"][5] = (-1, 0, 0); r["tmin"][5] = 2.7   # tmin just before the wall
+    r["o"][6] = (100, 0, 0); r["d"][6] = (1, 0, 0)               # pointing away
+    r["o"][7] = (0, 0, 0); r["d"][7] = (-1e-30, -1, 1e-30)       # tiny components
+    got = tracer.trace(r)
+    want = O.trace_brute(sc, r)
+    _assert_same_hits(got, want, "edge")
+    assert got["tri"][0] == RT_MISS and got["tri"][4] == RT_MISS and got["tri"][6] == RT_MISS
+    assert got["tri"][1] != RT_MISS and got["tri"][2] != RT_MISS and got["tri"][5] != RT_MISS
+
+
+def test_empty_ray_batch(tracer):
+    sc = scenes.cornell_box()
+    _upload(tracer, sc)
+    got = tracer.trace(np.zeros(0, RAY_DTYPE))
+    assert got.shape == (0,)
+
+
+def test_device_pointer_path_matches_host(tracer):
+    import torch
+    sc = scenes.triangle_soup(20000, seed=5)
+    _upload(tracer, sc)
+    coh, inc = scenes.microbench_rays(64, 64, 4096, seed=6)
+    rays = np.concatenate([coh, inc])
+    host = tracer.trace(rays)
+    dr = torch.from_numpy(rays.view(np.float32).reshape(-1)).cuda()
+    tracer.set_stream(torch.cuda.current_stream().cuda_stream)
+    dh = tracer.trace(dr)
+    torch.cuda.synchronize()
+    tracer.set_stream(None)
+    got = dh.cpu().numpy().view(np.uint32).reshape(-1, 4)
+    assert np.array_equal(got, _bits(host))
+
+
+def test_trace_before_build_is_a_state_error():
+    from master_thesis_amd.tracer import Tracer
+    with Tracer(0) as t:
+        t.upload(scenes.cornell_box())
+        with pytest.raises(RTError) as e:
+            t.trace(scenes.incoherent_rays(4))
+        assert e.value.code == RT_E_STATE
