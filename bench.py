@@ -45,6 +45,9 @@ def main() -> int:
     ap.add_argument("--height", type=int, default=2048)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-halves", action="store_true", help="skip the per-half (coherent / incoherent) timing")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 512 MiB device copy (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (rays and pixels would be sharded "
@@ -66,9 +69,18 @@ def main() -> int:
     log(f"[bench] scene {sc.num_triangles} tris, {n} rays generated in {time.time() - t0:.1f}s")
 
     dev = torch.device("cuda:0")
-    stream = torch.cuda.current_stream(dev)
+    stream = torch.cuda.Stream(dev)  # one explicit stream for the library's launches and the timing events
+    torch.cuda.set_stream(stream)
     d_rays = torch.from_numpy(rays.view(np.float32).reshape(-1)).to(dev)
     d_hits = torch.empty(n * 4, dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+    if args.calibrate_copy:
+        src = torch.empty(1 << 27, dtype=torch.float32, device=dev)  # 512 MiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] calibration: 3 x 512 MiB device copy done")
     with Tracer(0) as t:
         t.set_stream(stream.cuda_stream)
         t.upload(sc)
@@ -82,8 +94,8 @@ def main() -> int:
         bytes_alg = 32 * cnt["rays"] + 16 * cnt["rays"] + 128 * cnt["node_visits"] + 40 * cnt["tri_tests"]
 
         # per-half rates (informational)
-        halves = {}
-        for name, lo, hi in (("coherent", 0, n_coh), ("incoherent", n_coh, n)):
+        halves = {"coherent": None, "incoherent": None}
+        for name, lo, hi in (("coherent", 0, n_coh), ("incoherent", n_coh, n)) if not args.no_halves else ():
             sub = d_rays[lo * 8:hi * 8]
             sub_h = d_hits[lo * 4:hi * 4]
             for _ in range(2):
@@ -153,8 +165,8 @@ def main() -> int:
                                f"{n} rays ({n_coh} coherent pinhole + {n - n_coh} incoherent), closest hit",
                    "triangles": sc.num_triangles, "rays": n, "bvh": "BVH4 LBVH leaf<=4",
                    "bvh_build_ms": round(info["build_ms"], 3),
-                   "mrays_per_s_coherent": round(halves["coherent"], 1),
-                   "mrays_per_s_incoherent": round(halves["incoherent"], 1),
+                   "mrays_per_s_coherent": halves["coherent"] and round(halves["coherent"], 1),
+                   "mrays_per_s_incoherent": halves["incoherent"] and round(halves["incoherent"], 1),
                    "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,

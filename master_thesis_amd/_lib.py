@@ -145,6 +145,14 @@ def load() -> C.CDLL:
     if _lib is None:
         if not LIB_PATH.exists():
             raise ImportError(f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'`")
+        # torch ships its own HIP runtime under the same SONAME
+        # (libamdhip64.so.7).  Loading torch first makes libmtrt bind to that
+        # one copy; loaded the other way round the process ends up with two
+        # runtimes and torch cannot open the device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = C.CDLL(str(LIB_PATH))
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib, name)

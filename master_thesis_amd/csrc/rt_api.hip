@@ -113,7 +113,9 @@ int rt_create(rt_ctx** out, int device) {
     if (!c) return fail(nullptr, RT_E_NOMEM, "rt_create: host allocation failed");
     c->device = device;
     c->num_cus = prop.multiProcessorCount;
-    e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+    // a blocking stream: work here is ordered after earlier work on the null
+    // stream (where e.g. torch's default stream writes the rays)
+    e = hipStreamCreateWithFlags(&c->own_stream, hipStreamDefault);
     if (e != hipSuccess) {
         delete c;
         return fail(nullptr, RT_E_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));

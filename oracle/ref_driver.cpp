@@ -1,0 +1,217 @@
+// ref_driver.cpp — C entry points over the reference software renderer,
+// compiled together with the reference's own sources where they lie
+// (oracle/Makefile target `ref`, output oracle/_ref/libref.so, this container
+// only).  TEST INFRASTRUCTURE: it exists to pin the oracle and the host layer
+// against the reference itself.  No reference source is copied; this file
+// only calls the reference's classes.
+#include <atomic>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "camera.h"
+#include "graphic/graphics_object.h"
+#include "graphic/mesh.h"
+#include "graphic/path_tracer.h"
+#include "graphic/scene.h"
+#include "graphic/triangle.h"
+#include "input_parser.h"
+#include "mesh_manager.h"
+
+namespace {
+
+Vector3f v3(const float* p) { return Vector3f({p[0], p[1], p[2]}); }
+
+struct RefScene {
+    std::vector<Mesh*> meshes;
+    std::vector<GraphicsObject*> objects;
+    std::vector<GraphicsObject*> lights;
+    Scene scene;
+    ~RefScene() {
+        for (auto* o : objects) delete o;
+        for (auto* m : meshes) delete m;
+    }
+};
+
+InputEntry entry(const std::string& name, const std::vector<std::pair<std::string, std::vector<std::string>>>& kv) {
+    InputEntry e(name);
+    for (const auto& p : kv) e.insert(p);
+    return e;
+}
+
+std::string fstr(float x) {
+    char b[64];
+    snprintf(b, sizeof(b), "%.9g", (double)x);
+    return b;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Triangle::rayIntersects + getBarycentricCoords (triangle.cpp:21-55).
+int ref_intersect(const float* v0, const float* v1, const float* v2, const float* o, const float* d, float* p,
+                  float* bary) {
+    Triangle t(Vector3u({0u, 1u, 2u}), v3(v0), v3(v1), v3(v2));
+    Ray r(v3(o), v3(d));
+    Vector3f hp;
+    if (!t.rayIntersects(r, hp)) return 0;
+    Vector3f b = t.getBarycentricCoords(hp);
+    for (int k = 0; k < 3; ++k) {
+        p[k] = hp[k];
+        bary[k] = b[k];
+    }
+    return 1;
+}
+
+// A reference Scene from world-space arrays: one Mesh + GraphicsObject per
+// material index (identity transform), material thresholds from weights.
+void* ref_scene_create(const float* pos, const float* nrm, const uint32_t* idx, const uint32_t* tri_mat, uint32_t T,
+                       const float* colors, const uint32_t* is_light, uint32_t M) {
+    auto* s = new RefScene();
+    for (uint32_t m = 0; m < M; ++m) {
+        Mesh* mesh = new Mesh();
+        std::vector<int> remap;
+        for (uint32_t t = 0; t < T; ++t) {
+            if (tri_mat[t] != m) continue;
+            uint32_t loc[3];
+            for (int k = 0; k < 3; ++k) {
+                uint32_t v = idx[3 * t + k];
+                mesh->addVertex(v3(pos + 3 * v), v3(nrm + 3 * v));
+                loc[k] = (uint32_t)mesh->vertices.size() - 1;
+            }
+            mesh->addIndex(Vector3u({loc[0], loc[1], loc[2]}));
+        }
+        if (mesh->indices.empty()) {
+            delete mesh;
+            continue;
+        }
+        mesh->init();
+        s->meshes.push_back(mesh);
+        auto* o = new GraphicsObject(mesh, Vector3f({0.0f, 0.0f, 0.0f}));
+        o->color = v3(colors + 3 * m);
+        if (is_light[m]) {
+            o->lightSource = true;
+            s->lights.push_back(o);
+        }
+        o->init();
+        s->objects.push_back(o);
+        s->scene.addObject(o);
+    }
+    s->scene.init();
+    return s;
+}
+
+void ref_scene_free(void* h) { delete (RefScene*)h; }
+
+// Scene::traceRay per ray (scene.cpp:22-56): hit flag, position, normal,
+// object index (in creation order).
+void ref_scene_trace(void* h, const float* rays /* 8 floats each */, uint32_t n, int* hit, float* p, float* nr,
+                     int* obj) {
+    auto* s = (RefScene*)h;
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* r = rays + 8 * i;
+        Ray ray(Vector3f({r[0], r[1], r[2]}), Vector3f({r[4], r[5], r[6]}));
+        Mesh::Vertex hv;
+        const GraphicsObject* o = nullptr;
+        hit[i] = s->scene.traceRay(ray, hv, o) ? 1 : 0;
+        obj[i] = -1;
+        if (hit[i]) {
+            for (int k = 0; k < 3; ++k) {
+                p[3 * i + k] = hv.pos[k];
+                nr[3 * i + k] = hv.normal[k];
+            }
+            for (size_t q = 0; q < s->objects.size(); ++q)
+                if (s->objects[q] == o) obj[i] = (int)q;
+        }
+    }
+}
+
+// Camera::parseInput + GraphicsEngine::render's matrices (camera.cpp:19-51,
+// graphics_engine.cpp:42-47).  The reference's fov is fixed at pi/4.
+void ref_camera(const float* position, float angle0, float angle1, uint32_t w, uint32_t h, float* view_inv,
+                float* proj_inv, float* origin) {
+    Camera cam;
+    cam.parseInput(entry("Camera", {{"position", {fstr(position[0]), fstr(position[1]), fstr(position[2])}},
+                                    {"angle", {fstr(angle0), fstr(angle1)}}}));
+    Matrix4f vi = cam.getViewMatrix().inverseMatrix();
+    Matrix4f pi = cam.getProjectionMatrix(float(w) / float(h)).inverseMatrix();
+    Vector3f o = cutVector(vi * Vector4f({0.0f, 0.0f, 0.0f, 1.0f}));
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            view_inv[4 * i + j] = vi[i][j];
+            proj_inv[4 * i + j] = pi[i][j];
+        }
+    for (int k = 0; k < 3; ++k) origin[k] = o[k];
+}
+
+// MeshManager::createObjectsFromFile + GraphicsObject::init: world-space
+// vertices of every object (flattened, object order), plus counts.  With
+// pos == NULL only the counts are returned.
+int ref_load_scene(const char* scene_path, const char* basepath, float* pos, float* nrm, uint32_t* idx,
+                   uint32_t* tri_obj, float* colors, uint32_t* counts /* V, T, O */) {
+    MeshManager mm(basepath);
+    mm.createObjectsFromFile(scene_path);
+    std::vector<GraphicsObject*> objs = mm.getCreatedObjects();
+    uint32_t V = 0, T = 0;
+    for (size_t oi = 0; oi < objs.size(); ++oi) {
+        GraphicsObject* o = objs[oi];
+        o->init();
+        if (pos) {
+            for (size_t v = 0; v < o->vertices.size(); ++v)
+                for (int k = 0; k < 3; ++k) {
+                    pos[3 * (V + v) + k] = o->vertices[v].pos[k];
+                    nrm[3 * (V + v) + k] = o->vertices[v].normal[k];
+                }
+            for (size_t i = 0; i < o->mesh->indices.size(); ++i) idx[3 * T + i] = V + (uint32_t)o->mesh->indices[i];
+            for (size_t t = 0; t < o->mesh->indices.size() / 3; ++t) tri_obj[T + t] = (uint32_t)oi;
+            for (int k = 0; k < 3; ++k) colors[3 * oi + k] = o->color[k];
+        }
+        V += (uint32_t)o->vertices.size();
+        T += (uint32_t)(o->mesh->indices.size() / 3);
+    }
+    counts[0] = V;
+    counts[1] = T;
+    counts[2] = (uint32_t)objs.size();
+    return 0;
+}
+
+// PathTracer::renderPixel over a whole image (path_tracer.cpp:18-40), driven
+// by `threads` workers pulling pixels from an atomic counter like
+// GraphicsEngine::render.  The float pixel values, unclamped.
+void ref_render(void* h, const float* view_inv, const float* proj_inv, const float* origin, uint32_t w,
+                uint32_t hgt, uint32_t spp, uint32_t depth, int threads, float* rgb) {
+    auto* s = (RefScene*)h;
+    PathTracer pt;
+    pt.parseInput(entry("PathTracer", {{"visionJumpCount", {std::to_string(depth)}},
+                                       {"raysPerPixel", {std::to_string(spp)}}}));
+    Renderer::PixelRenderData prd;
+    prd.scene = &s->scene;
+    prd.objects = &s->objects;
+    prd.lightSources = &s->lights;
+    prd.imageSize = Vector2u({w, hgt});
+    prd.origin = v3(origin);
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            prd.viewInverse[i][j] = view_inv[4 * i + j];
+            prd.projInverse[i][j] = proj_inv[4 * i + j];
+        }
+    std::atomic<uint32_t> next(0);
+    auto work = [&]() {
+        Renderer::PixelRenderData local = prd;
+        for (;;) {
+            uint32_t i = next.fetch_add(1);
+            if (i >= w * hgt) break;
+            local.pixel = Vector2u({i % w, i / w});
+            Vector3f c = pt.renderPixel(local);
+            for (int k = 0; k < 3; ++k) rgb[3 * i + k] = c[k];
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < threads; ++t) th.emplace_back(work);
+    work();
+    for (auto& t : th) t.join();
+}
+
+}  // extern "C"
