@@ -91,6 +91,18 @@ def barycentric(v0, v1, v2, p):
     return out
 
 
+def hit_shading(scene, rays: np.ndarray, hits: np.ndarray):
+    """(hit points, shading normals) of closest hits, as Scene::traceRay forms them."""
+    lib = load()
+    lib.or_hit_shading.argtypes = [C.c_void_p] * 5 + [C.c_uint32, C.c_void_p, C.c_void_p]
+    n = rays.shape[0]
+    p = np.zeros((n, 3), np.float32)
+    nr = np.zeros((n, 3), np.float32)
+    arrs = [np.ascontiguousarray(a) for a in (scene.pos, scene.nrm, scene.idx, rays, hits)]
+    lib.or_hit_shading(*[_p(a) for a in arrs], n, _p(p), _p(nr))
+    return p, nr
+
+
 def trace_brute(scene, rays: np.ndarray, threads: int = 8) -> np.ndarray:
     from master_thesis_amd._lib import HIT_DTYPE
     hits = np.zeros(rays.shape[0], HIT_DTYPE)

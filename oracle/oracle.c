@@ -208,6 +208,33 @@ void or_barycentric(const float v0[3], const float v1[3], const float v2[3], con
     bary(a, vsub(vld(v1), a), vsub(vld(v2), a), vld(p), &out[0], &out[1], &out[2]);
 }
 
+/* Hit point and shading normal of a closest hit, scene.cpp:43-53:
+ * p = o + t d, barycentrics from p, n = normalize(b0 n0 + b1 n1 + b2 n2). */
+void or_hit_shading(const float* pos, const float* nrm, const uint32_t* idx, const void* rays, const void* hits,
+                    uint32_t n, float* p_out, float* n_out) {
+    const rt_ray* rr = (const rt_ray*)rays;
+    const rt_hit* hh = (const rt_hit*)hits;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (hh[i].tri == RT_MISS) {
+            for (int k = 0; k < 3; ++k) { p_out[3 * i + k] = 0.0f; n_out[3 * i + k] = 0.0f; }
+            continue;
+        }
+        uint32_t t = hh[i].tri;
+        v3 o = V3(rr[i].ox, rr[i].oy, rr[i].oz), d = V3(rr[i].dx, rr[i].dy, rr[i].dz);
+        v3 p = vadd(o, vscl(hh[i].t, d));
+        v3 v0 = vld(pos + 3 * idx[3 * t]);
+        float b0, b1, b2;
+        bary(v0, vsub(vld(pos + 3 * idx[3 * t + 1]), v0), vsub(vld(pos + 3 * idx[3 * t + 2]), v0), p, &b0, &b1, &b2);
+        v3 nn = V3(0.0f, 0.0f, 0.0f);
+        nn = vadd(nn, vscl(b0, vld(nrm + 3 * idx[3 * t])));
+        nn = vadd(nn, vscl(b1, vld(nrm + 3 * idx[3 * t + 1])));
+        nn = vadd(nn, vscl(b2, vld(nrm + 3 * idx[3 * t + 2])));
+        nn = vnormalize(nn);
+        p_out[3 * i] = p.x; p_out[3 * i + 1] = p.y; p_out[3 * i + 2] = p.z;
+        n_out[3 * i] = nn.x; n_out[3 * i + 1] = nn.y; n_out[3 * i + 2] = nn.z;
+    }
+}
+
 /* graphics_object.cpp:75-83: keep the hit with the smallest |o - p|^2,
  * strict <; exact ties to the lowest index (the documented divergence). */
 static inline int closer(float d2a, uint32_t ida, float d2b, uint32_t idb) {

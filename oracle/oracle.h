@@ -39,6 +39,11 @@ int or_intersect(const float o[3], const float d[3], const float v0[3], const fl
 /* Triangle::getBarycentricCoords, triangle.cpp:47-55. */
 void or_barycentric(const float v0[3], const float v1[3], const float v2[3], const float p[3], float out[3]);
 
+/* Hit point o + t d and interpolated, normalised shading normal of each hit
+ * (graphic/scene.cpp:43-53); zeros for misses. */
+void or_hit_shading(const float* pos, const float* nrm, const uint32_t* idx, const void* rays, const void* hits,
+                    uint32_t n, float* p_out, float* n_out);
+
 /* Closest hit by brute force over all triangles: the definition of the
  * result (graphics_object.cpp:64-87 + scene.cpp:22-41 with the lowest-index
  * tie rule).  rays: rt_ray[n], hits: rt_hit[n]. */

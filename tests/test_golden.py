@@ -1,0 +1,57 @@
+"""Golden vectors from the compiled reference (tests/golden/make_golden.py):
+camera matrices (host layer, CPU) and Scene::traceRay answers on the
+Cornell-style box (GPU closest hit, hit point and shading normal)."""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle as O
+from master_thesis_amd import scenes, host_lib as H
+from master_thesis_amd._lib import RAY_DTYPE, RT_MISS
+
+G = Path(__file__).resolve().parent / "golden"
+
+
+def test_camera_matrices_match_reference_vectors():
+    g = np.load(G / "ref_cameras.npz")
+    for row in g["rows"]:
+        pos, a0, a1 = row[0:3], float(row[3]), float(row[4])
+        vi, pi, og = row[5:21], row[21:37], row[37:40]
+        cam = H.camera(tuple(float(x) for x in pos), a0, a1, float(g["fov"]), int(g["width"]), int(g["height"]))
+        assert np.array_equal(np.array(cam.view_inv[:], np.float32).view(np.uint32), vi.view(np.uint32))
+        assert np.array_equal(np.array(cam.proj_inv[:], np.float32).view(np.uint32), pi.view(np.uint32))
+        assert np.array_equal(np.array(cam.origin[:], np.float32).view(np.uint32), og.view(np.uint32))
+
+
+def _compare_with_reference(sc, rays, hits, g):
+    hit = hits["tri"] != RT_MISS
+    assert np.array_equal(hit, g["hit"])
+    p, nr = O.hit_shading(sc, rays, hits)
+    obj = sc.tri_mat[np.where(hit, hits["tri"], 0)]
+    diff = hit & (np.any(p != g["pos"], axis=1) | np.any(nr != g["normal"], axis=1) | (obj != g["obj"]))
+    # differences may only be exact squared-distance ties (floor / box bottom faces)
+    for i in np.nonzero(diff)[0]:
+        w = (rays["o"][i] - g["pos"][i]).astype(np.float32)
+        d2_ref = np.float32(np.float32(np.float32(np.float32(0) + w[0] * w[0]) + w[1] * w[1]) + w[2] * w[2])
+        w2 = (rays["o"][i] - p[i]).astype(np.float32)
+        d2_me = np.float32(np.float32(np.float32(np.float32(0) + w2[0] * w2[0]) + w2[1] * w2[1]) + w2[2] * w2[2])
+        assert d2_ref == d2_me, f"ray {i}: reference d2 {d2_ref} vs {d2_me}"
+    return int(diff.sum())
+
+
+def test_oracle_matches_reference_vectors():
+    g = np.load(G / "ref_cornell_trace.npz")
+    rays = np.ascontiguousarray(g["rays"]).view(RAY_DTYPE).reshape(-1)
+    sc = scenes.cornell_box()
+    assert _compare_with_reference(sc, rays, O.trace_brute(sc, rays), g) < 0.01 * rays.shape[0]
+
+
+@pytest.mark.gpu
+def test_gpu_matches_reference_vectors(tracer):
+    g = np.load(G / "ref_cornell_trace.npz")
+    rays = np.ascontiguousarray(g["rays"]).view(RAY_DTYPE).reshape(-1)
+    sc = scenes.cornell_box()
+    tracer.upload(sc)
+    tracer.build(4)
+    assert _compare_with_reference(sc, rays, tracer.trace(rays), g) < 0.01 * rays.shape[0]

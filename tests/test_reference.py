@@ -1,0 +1,116 @@
+"""Oracle and host layer against the reference itself, compiled from its own
+sources (oracle/_ref/libref.so).  Runs where the reference was available at
+build time; the GPU box checks the golden vectors made from it instead
+(tests/golden/)."""
+import glob
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import ref as R
+from master_thesis_amd import scenes, host_lib as H
+from master_thesis_amd._lib import RAY_DTYPE, RT_MISS
+from master_thesis_amd.tracer import make_settings
+
+REF_RES = "/root/reference/res"
+pytestmark = pytest.mark.skipif(not R.available() or not os.path.isdir(REF_RES),
+                                reason="compiled reference (oracle/_ref) or its resources not present here")
+
+
+def _random_rays(n, lo, hi, seed):
+    rng = np.random.default_rng(seed)
+    r = np.zeros(n, RAY_DTYPE)
+    r["o"] = rng.uniform(lo, hi, (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    r["d"] = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    r["tmax"] = np.inf
+    return r
+
+
+def test_moller_trumbore_matches_reference():
+    rng = np.random.default_rng(2)
+    hits = 0
+    for _ in range(4000):
+        v = rng.normal(size=(3, 3)).astype(np.float32)
+        o = rng.normal(size=3).astype(np.float32) * 3
+        target = (v.mean(axis=0) + rng.normal(size=3) * 0.7).astype(np.float32)
+        d = (target - o).astype(np.float32)
+        rh, rp, rb = R.intersect(v[0], v[1], v[2], o, d)
+        oh, tuv = O.intersect(o, d, v[0], v[1], v[2])
+        assert rh == oh
+        if oh:
+            hits += 1
+            p = (o + tuv[0] * d).astype(np.float32)
+            assert np.array_equal(p.view(np.uint32), rp.view(np.uint32))
+            assert np.array_equal(O.barycentric(v[0], v[1], v[2], p).view(np.uint32), rb.view(np.uint32))
+    assert hits > 500
+
+
+def test_scene_trace_matches_reference_up_to_exact_ties():
+    sc = scenes.cornell_box()
+    rays = _random_rays(20000, -2.7, 2.7, 1)
+    rh, rp, rn, robj = R.RefScene(sc).trace(rays)
+    hb = O.trace_brute(sc, rays)
+    oh = hb["tri"] != RT_MISS
+    assert np.array_equal(rh, oh)
+    p, nr = O.hit_shading(sc, rays, hb)
+    diff = oh & (np.any(p != rp, axis=1) | np.any(nr != rn, axis=1) | (sc.tri_mat[np.where(oh, hb["tri"], 0)] != robj))
+    # every difference must be an exact squared-distance tie between two triangles
+    for i in np.nonzero(diff)[0]:
+        d2 = []
+        for t in range(sc.idx.shape[0]):
+            v = sc.pos[sc.idx[t]]
+            ok, tuv = O.intersect(rays["o"][i], rays["d"][i], v[0], v[1], v[2])
+            if ok:
+                w = (rays["o"][i] - (rays["o"][i] + tuv[0] * rays["d"][i]).astype(np.float32)).astype(np.float32)
+                d2.append(np.float32(np.float32(np.float32(np.float32(0) + w[0] * w[0]) + w[1] * w[1]) + w[2] * w[2]))
+        d2 = np.sort(np.array(d2, np.float32))
+        assert len(d2) >= 2 and d2[0] == d2[1], f"ray {i}: not a tie"
+    assert diff.mean() < 0.01  # the boxes stand on the floor: their bottom faces tie with it
+
+
+@pytest.mark.parametrize("pos,a0,a1", [((3.4, 0, 0), 1, 0), ((4, 0, 0), 1, 0),
+                                       ((8.29397, 0, -1.6169), 0.783549, -0.0445634), ((0, 60, 0), 1, -1),
+                                       ((1, 2, 3), 1.7, 0.3), ((-2, 0.5, 1), -1.3, 1.4), ((0.1, 0.2, 0.3), 0.0, -0.7)])
+def test_camera_matches_reference(pos, a0, a1):
+    vi, pi, og = R.camera(pos, a0, a1, 320, 200)
+    cam = H.camera(pos, a0, a1, np.float32(math.pi / 4), 320, 200)
+    assert np.array_equal(vi.view(np.uint32), np.array(cam.view_inv[:], np.float32).view(np.uint32))
+    assert np.array_equal(pi.view(np.uint32), np.array(cam.proj_inv[:], np.float32).view(np.uint32))
+    assert np.array_equal(og.view(np.uint32), np.array(cam.origin[:], np.float32).view(np.uint32))
+
+
+@pytest.mark.parametrize("scene", sorted(glob.glob(f"{REF_RES}/scene/*.scene")))
+def test_scene_files_flatten_like_reference(scene):
+    pos, nrm, idx, tri_obj, colors = R.load_scene(scene, "/root/reference/build/")
+    mine = H.load_scene(scene, f"{REF_RES}/obj")
+    assert np.array_equal(mine.pos.view(np.uint32), pos.view(np.uint32))
+    assert np.array_equal(mine.nrm.view(np.uint32), nrm.view(np.uint32))
+    assert np.array_equal(mine.idx, idx)
+    assert np.array_equal(mine.tri_mat, tri_obj)
+    assert np.array_equal(mine.materials["color"], colors)
+
+
+def test_path_tracer_statistically_matches_reference():
+    """The reference's own PathTracer (mt19937 stream) and the oracle
+    (Philox stream) estimate the same image: block means agree within the
+    spread the oracle shows across seeds."""
+    sc = scenes.cornell_box()
+    cam = H.camera(scenes.cornell_camera_position(), 1.0, 0.0, np.float32(math.pi / 4), 32, 32)
+    spp, depth = 256, 5
+    ref = R.RefScene(sc).render(cam, spp, depth, threads=8)
+    runs = np.stack([O.render(sc, cam, make_settings(spp=spp, max_depth=depth, seed=s))[0] for s in range(1, 7)])
+
+    def blocks(img):
+        return img.reshape(4, 8, 4, 8, 3).mean(axis=(1, 3))
+
+    rb = blocks(ref)
+    ob = np.stack([blocks(x) for x in runs])
+    m, s = ob.mean(axis=0), ob.std(axis=0, ddof=1)
+    z = (rb - m) / np.maximum(s * math.sqrt(1 + 1 / len(runs)), 1e-6)
+    assert np.all(np.abs(z) < 6), np.abs(z).max()
+    zg = (ref.mean() - runs.mean()) / (runs.mean(axis=(1, 2, 3)).std(ddof=1) * math.sqrt(1 + 1 / len(runs)))
+    assert abs(zg) < 5, zg
