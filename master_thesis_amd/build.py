@@ -68,19 +68,20 @@ def build_librt(jobs: int = 8) -> Path:
 
 
 def build_host() -> list[Path]:
-    srcs = sorted(HOST.glob("*.cpp"))
-    lib_srcs = [s for s in srcs if s.name != "sw_render_main.cpp"]
+    # libmtrt_host.so has no GPU dependency (formats, transforms, camera);
+    # the engine (which calls libmtrt) is linked only into the command line
+    gpu_srcs = {"engine.cpp", "sw_render_main.cpp"}
+    lib_srcs = [s for s in sorted(HOST.glob("*.cpp")) if s.name not in gpu_srcs]
     out = []
-    if lib_srcs:
-        tmp = LIBHOST.with_suffix(".so.tmp")
-        _run([CXX, *CXX_FLAGS, "-shared", "-o", str(tmp), *map(str, lib_srcs), "-pthread"])
-        os.replace(tmp, LIBHOST)
-        out.append(LIBHOST)
-    main = HOST / "sw_render_main.cpp"
-    if main.exists():
-        _run([CXX, *CXX_FLAGS, "-o", str(CLI), str(main), *map(str, lib_srcs), "-L", str(PKG), "-lmtrt",
-              f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN", "-pthread"])
-        out.append(CLI)
+    tmp = LIBHOST.with_suffix(".so.tmp")
+    _run([CXX, *CXX_FLAGS, "-shared", "-o", str(tmp), *map(str, lib_srcs), "-pthread"])
+    os.replace(tmp, LIBHOST)
+    out.append(LIBHOST)
+    tmp = CLI.with_suffix(".tmp")
+    _run([CXX, *CXX_FLAGS, "-o", str(tmp), str(HOST / "sw_render_main.cpp"), str(HOST / "engine.cpp"),
+          *map(str, lib_srcs), "-L", str(PKG), "-lmtrt", "-Wl,-rpath,$ORIGIN", "-pthread"])
+    os.replace(tmp, CLI)
+    out.append(CLI)
     return out
 
 

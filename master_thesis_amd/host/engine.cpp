@@ -1,0 +1,95 @@
+// engine.cpp — Renderer factory, HipPathTracer and the frame loop.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <thread>
+
+namespace mth {
+
+Vec3 Renderer::renderPixel(const PixelRenderData&) const {
+    throw InitError("Renderer", "this renderer has no per-pixel implementation");
+}
+
+bool Renderer::renderFrame(const FrameRenderData&, float*) const { return false; }
+
+void HipPathTracer::parseInput(const InputEntry& e) {
+    settings = rt_settings{};
+    settings.max_depth = e.get_uint("visionJumpCount");
+    settings.spp = e.get_uint("raysPerPixel");
+    settings.seed = e.has("seed") ? (uint64_t)std::stoull(e.get_string("seed")) : 1ull;
+    settings.num_views = 1;
+}
+
+bool HipPathTracer::renderFrame(const FrameRenderData& frd, float* rgb) const {
+    int rc = rt_render(frd.ctx, frd.camera, &settings, rgb, nullptr, RT_PTR_HOST);
+    if (rc != RT_OK) throw InitError("HipPathTracer", rt_last_error(frd.ctx));
+    return true;
+}
+
+std::unique_ptr<Renderer> make_renderer(const std::string& name) {
+    if (name == "PathTracer") return std::make_unique<HipPathTracer>();
+    throw InitError("getRenderer not found", name);
+}
+
+GraphicsEngine::GraphicsEngine(int device) {
+    int rc = rt_create(&ctx_, device);
+    if (rc != RT_OK) throw InitError("GraphicsEngine", rt_last_error(nullptr));
+}
+
+GraphicsEngine::~GraphicsEngine() { rt_destroy(ctx_); }
+
+void GraphicsEngine::init(std::unique_ptr<Renderer> renderer, const FlatScene& scene, unsigned thread_count) {
+    renderer_ = std::move(renderer);
+    threads_ = std::max(1u, thread_count);
+    rt_scene_desc d = scene.desc();
+    if (rt_scene_upload(ctx_, &d) != RT_OK) throw InitError("GraphicsEngine", rt_last_error(ctx_));
+    rt_build_opts o{4u, 0u};
+    if (rt_build(ctx_, &o, nullptr) != RT_OK) throw InitError("GraphicsEngine", rt_last_error(ctx_));
+    rgb_.assign((size_t)width * height * 3, 0.0f);
+    rgb8_.assign((size_t)width * height * 3, 0);
+}
+
+void GraphicsEngine::render() {
+    if (!renderer_) throw InitError("GraphicsEngine", "render before init");
+    const rt_camera cam = camera.rt(width, height);
+    auto t0 = std::chrono::steady_clock::now();
+    FrameRenderData frd{ctx_, &cam};
+    if (!renderer_->renderFrame(frd, rgb_.data())) {
+        // per-pixel path: workers pull pixels from an atomic counter
+        std::atomic<uint32_t> next(0);
+        const uint32_t total = width * height;
+        auto work = [&]() {
+            for (;;) {
+                uint32_t i = next.fetch_add(1);
+                if (i >= total) break;
+                PixelRenderData prd{width, height, i % width, i / width, &cam};
+                Vec3 c = renderer_->renderPixel(prd);
+                for (int k = 0; k < 3; ++k) rgb_[3 * (size_t)i + k] = c[k];
+            }
+        };
+        std::vector<std::thread> th;
+        for (unsigned t = 1; t < threads_; ++t) th.emplace_back(work);
+        work();
+        for (auto& t : th) t.join();
+    }
+    last_render_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    // clamp to [0, 1], then (char)(v * 255.0f) (graphics_engine.cpp:90-95)
+    for (size_t i = 0; i < rgb_.size(); ++i) {
+        float v = std::clamp(rgb_[i], 0.0f, 1.0f);
+        rgb8_[i] = (uint8_t)(int)(v * 255.0f);
+    }
+    std::printf("100%% done\n");
+}
+
+void GraphicsEngine::save_image(const std::string& path) const {
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) throw InitError("GraphicsEngine", "cannot write \"" + path + "\"");
+    std::fprintf(f, "P6\n%u\n%u\n255\n", width, height);
+    std::fwrite(rgb8_.data(), 1, rgb8_.size(), f);
+    std::fclose(f);
+}
+
+}  // namespace mth

@@ -126,6 +126,47 @@ def cornell_box(light_color=(15.0, 15.0, 15.0)) -> Scene:
     return b.scene(mats)
 
 
+def cornell_groups(albedo: np.ndarray | None = None, light_color=(15.0, 15.0, 15.0), seed: int = 0) -> Scene:
+    """The Cornell-style box with 64 triangle groups, each its own material
+    with a learnable albedo: every wall is a 2 x 2 grid of quads (5 walls x 8
+    triangles = 40) plus the two boxes (24 triangles).  Material 64 is the
+    light.  albedo: (64, 3); default U(0.2, 0.8) drawn from `seed`."""
+    if albedo is None:
+        albedo = np.random.default_rng(seed).uniform(0.2, 0.8, (64, 3))
+    albedo = np.asarray(albedo, np.float32)
+    assert albedo.shape == (64, 3)
+    h = CORNELL_HALF
+    b = _Builder()
+    g = [0]
+
+    def grid(corner, ea, eb, normal):
+        c, a, bb = (np.asarray(x, np.float32) for x in (corner, ea, eb))
+        for i in range(2):
+            for j in range(2):
+                b.quad(c + a * (i / 2) + bb * (j / 2), a / 2, bb / 2, normal, 0)
+                b.mat[-2] = g[0]
+                b.mat[-1] = g[0] + 1
+                g[0] += 2
+
+    s = 2 * h
+    grid((-h, -h, -h), (0, 0, s), (0, s, 0), (1, 0, 0))
+    grid((-h, -h, -h), (s, 0, 0), (0, 0, s), (0, 1, 0))
+    grid((-h, h, -h), (0, 0, s), (s, 0, 0), (0, -1, 0))
+    grid((-h, -h, -h), (0, s, 0), (s, 0, 0), (0, 0, 1))
+    grid((-h, -h, h), (s, 0, 0), (0, s, 0), (0, 0, -1))
+    first_box = len(b.mat)
+    b.box((0.10, -h, 0.45), (1.75, -h + 1.65, 2.10), 0)
+    b.box((-1.85, -h, -2.05), (-0.20, -h + 3.30, -0.40), 0)
+    for k in range(first_box, len(b.mat)):
+        b.mat[k] = g[0]
+        g[0] += 1
+    assert g[0] == 64
+    lh = 0.70
+    b.quad((-lh, h - 0.01, -lh), (0, 0, 2 * lh), (2 * lh, 0, 0), (0, -1, 0), 64)
+    mats = [material(albedo[k]) for k in range(64)] + [material(light_color, light=True)]
+    return b.scene(mats)
+
+
 def closed_room(floor_mat: np.ndarray, wall_mat: np.ndarray, half: float = 1.0) -> Scene:
     """Closed cube [-half, half]^3 seen from inside (inward normals): the
     floor y = -half uses material 0, the other five faces material 1.  Used

@@ -104,3 +104,39 @@ def test_render_argument_errors(tracer):
         tracer.render(cam, make_settings(spp=0, max_depth=5))
     with pytest.raises(RTError):
         tracer.render(cam, make_settings(spp=1, max_depth=0))
+
+
+def test_sw_render_cli_matches_library(tracer, tmp_path):
+    """The SoftwareRenderer-compatible command line renders the same image as
+    the library for the same scene file, camera file and renderer file."""
+    import subprocess
+    from master_thesis_amd import build as B
+    (tmp_path / "obj").mkdir()
+    (tmp_path / "scene").mkdir()
+    (tmp_path / "obj" / "panel.obj").write_text(
+        "v 0.0 1.0 -1.0\nv 0.0 -1.0 -1.0\nv 0.0 1.0 1.0\nv 0.0 -1.0 1.0\nvn 1.0 0.0 0.0\nf 2//1 1//1 3//1 4//1\n")
+    (tmp_path / "scene" / "box.scene").write_text(
+        "panel.obj\n\tposition(-1, 0, 0)\n\n"
+        "panel.obj\n\tposition(0, -1, 0)\n\trotation(0, 0, 1, 1.57079)\n\n"
+        "panel.obj\n\tposition(0, 1, 0)\n\trotation(0, 0, 1, -1.57079)\n\n"
+        "panel.obj\n\tposition(0, 0, -1)\n\trotation(0, 1, 0, -1.57079)\n\tcolor(0, 1, 0)\n\n"
+        "panel.obj\n\tposition(0, 0, 1)\n\trotation(0, 1, 0, 1.57079)\n\tcolor(1, 0, 0)\n\n"
+        "panel.obj\n\tposition(0, 0.99, 0)\n\tscale(0.2, 0.2, 0.2)\n\trotation(0, 0, 1, -1.57079)\n\tlightSource(1.0)\n"
+        "\tcolor(15, 15, 15)\n")
+    (tmp_path / "pt.renderer").write_text("PathTracer\n\tvisionJumpCount(5)\n\traysPerPixel(16)\n\tseed(7)\n")
+    (tmp_path / "c.camera").write_text("Camera\n\tposition(3.4, 0, 0)\n\tangle(1, 0)\n")
+    out = tmp_path / "o.ppm"
+    p = subprocess.run([str(B.CLI), str(tmp_path / "pt.renderer"), str(tmp_path / "scene" / "box.scene"), "64", "48",
+                        "4", str(tmp_path / "c.camera"), str(out)], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    data = out.read_bytes()
+    head = b"P6\n64\n48\n255\n"
+    assert data.startswith(head)
+    img = np.frombuffer(data[len(head):], np.uint8).reshape(48, 64, 3)
+    sc = H.load_scene(str(tmp_path / "scene" / "box.scene"), str(tmp_path / "obj"))
+    _setup(tracer, sc)
+    cam = H.camera_from_file(str(tmp_path / "c.camera"), 64, 48)
+    _, s = H.renderer_from_file(str(tmp_path / "pt.renderer"))
+    _, rgb8 = tracer.render(cam, s, want_rgb8=True)
+    assert np.array_equal(img, rgb8)
+    assert img.mean() > 5
