@@ -296,7 +296,9 @@ int rt_trace_closest(rt_ctx* c, const rt_ray* rays, uint32_t n, rt_hit* hits, in
         RT_HIP(c, hipMemcpyAsync(flags, c->work.ptr, 8, hipMemcpyDeviceToHost, c->stream));
         if (counters) RT_HIP(c, hipMemcpyAsync(st, c->stats.ptr, 32, hipMemcpyDeviceToHost, c->stream));
         RT_HIP(c, hipStreamSynchronize(c->stream));
-        if (flags[1]) return fail(c, RT_E_UNSUPPORTED, "rt_trace_closest: traversal stack exceeded its bound");
+        if (flags[1])
+            return fail(c, RT_E_DEVICE, flags[1] == 1 ? "rt_trace_closest: traversal stack exceeded its bound"
+                                                      : "rt_trace_closest: traversal exceeded its step bound");
         if (counters) {
             counters->rays = st[0];
             counters->node_visits = st[1];

@@ -69,6 +69,44 @@ RT_HD bool intersect_mt(f3 o, f3 d, f3 v0, f3 e0, f3 e1, float& t, float& u, flo
     return t > EPS;
 }
 
+// The same test with division-free early rejection.  v_rcp_f32 (1 ulp)
+// gives approximate u, v, t; a candidate is rejected early only when the
+// approximation is outside the acceptance region by more than 2^-20 of its
+// magnitude, which bounds the difference to the correctly rounded values, or
+// when its t lies beyond `tcull` (no triangle there can beat the current
+// best, the same argument as the box culling).  Everything that survives runs
+// the exact test above, so the accepted set and the returned t, u, v are
+// those of intersect_mt.
+__device__ __forceinline__ bool intersect_mt_fast(f3 o, f3 d, f3 v0, f3 e0, f3 e1, float tmin, float tmax,
+                                                  float tcull, float& t, float& u, float& v) {
+    const float EPS = 1e-7f;
+    f3 h = cross(d, e1);
+    float a = dot(e0, h);
+    if (a > -EPS && a < EPS) return false;
+    f3 s = sub(o, v0);
+    const float su = dot(s, h);
+    f3 q = cross(s, e0);
+    const float sv = dot(d, q);
+    const float st = dot(e1, q);
+    if (fabsf(a) < 1e30f) {
+        const float r = __builtin_amdgcn_rcpf(a);
+        const float ua = r * su, va = r * sv, ta = r * st;
+        const float tu = fabsf(ua) * 0x1p-20f + 0x1p-100f;
+        const float tv = fabsf(va) * 0x1p-20f + 0x1p-100f;
+        const float tt = fabsf(ta) * 0x1p-20f + 0x1p-100f;
+        if (ua < -tu || ua > 1.0f + tu) return false;
+        if (va < -tv || ua + va > 1.0f + tu + tv + 0x1p-20f) return false;
+        if (ta + tt < EPS || ta + tt < tmin || ta - tt > tcull || ta - tt > tmax) return false;
+    }
+    const float f = 1.0f / a;
+    u = f * su;
+    if (u < 0.0f || u > 1.0f) return false;
+    v = f * sv;
+    if (v < 0.0f || u + v > 1.0f) return false;
+    t = f * st;
+    return t > EPS;
+}
+
 // Hit point o + t*d (triangle.cpp:39) and its squared distance to o
 // (graphics_object.cpp:77 -> vector.h:227-229, (o - p).magnitudeSquared()).
 RT_HD f3 hit_point(f3 o, f3 d, float t) { return add(o, scale(t, d)); }

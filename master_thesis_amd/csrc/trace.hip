@@ -17,10 +17,14 @@
 // max|o| / max|d| covers the cancellation in |(o + t d) - o|^2 for hits close
 // to a distant origin; boxes are padded outward at build time.
 //
-// Execution: persistent waves pull 64-ray batches from a global counter
-// (Aila & Laine, "Understanding the efficiency of ray traversal on GPUs",
-// HPG 2009).  Each lane keeps a 32-entry ring stack in LDS; deeper entries
-// spill to a per-lane global area sized from the BVH depth.
+// Execution: persistent waves; each lane fetches a new ray as soon as its ray
+// finishes (dynamic fetch, Aila & Laine, "Understanding the efficiency of ray
+// traversal on GPUs", HPG 2009) from a wave-local pool of consecutive rays.
+// One loop iteration is one traversal step.  Each lane keeps a 32-entry ring
+// stack in LDS; deeper entries spill to a per-lane global area sized from the
+// BVH depth.  The slab test reads each axis's near and far planes directly
+// (chosen by the sign of the direction), and the triangle test rejects most
+// candidates before its IEEE division (intersect_mt_fast).
 #include "rt_internal.hpp"
 #include "rt_core.hpp"
 
@@ -38,132 +42,202 @@ struct TraceArgs {
     uint32_t* spill;       // [spill_cap][grid threads]
     uint32_t spill_cap;
     uint32_t spill_stride;
+    uint32_t max_steps;    // bound on steps per ray (each node and leaf is visited at most once)
     unsigned long long* stats;  // rays, nodes, tris, hits
 };
 
+constexpr uint32_t POOL_CHUNK = 128;  // rays a wave takes from the global counter at a time
+
+// Per-lane traversal state of one ray.
+struct Lane {
+    f3 o, d;
+    float ix, iy, iz, nox, noy, noz;
+    float tmin, tmax, tlo, sigma, tcull;
+    float best_d2, best_t, best_u, best_v;
+    uint32_t best_id;
+    uint32_t offx, offy, offz;  // byte offset of the near plane (lo or hi) per axis
+    uint32_t cur, sp, steps;
+};
+
+__device__ __forceinline__ bool lane_start(Lane& L, const rt_ray& r) {
+    L.o = mk(r.ox, r.oy, r.oz);
+    L.d = mk(r.dx, r.dy, r.dz);
+    const float dmax = fmaxf(fabsf(L.d.x), fmaxf(fabsf(L.d.y), fabsf(L.d.z)));
+    const float omax = fmaxf(fabsf(L.o.x), fmaxf(fabsf(L.o.y), fabsf(L.o.z)));
+    L.tmin = r.tmin;
+    L.tmax = r.tmax;
+    L.tlo = fmaxf(r.tmin, 0.0f);
+    L.sigma = 3.814697265625e-06f * omax / dmax;  // 2^-18 * max|o| / max|d|
+    L.ix = safe_rcp(L.d.x);
+    L.iy = safe_rcp(L.d.y);
+    L.iz = safe_rcp(L.d.z);
+    L.nox = -L.o.x * L.ix;
+    L.noy = -L.o.y * L.iy;
+    L.noz = -L.o.z * L.iz;
+    // lo_x at byte 0, hi_x at 16, lo_y at 32 ...: for a positive direction the
+    // near plane is the low bound
+    L.offx = (__float_as_uint(L.ix) >> 31) * 16u;
+    L.offy = 32u + (__float_as_uint(L.iy) >> 31) * 16u;
+    L.offz = 64u + (__float_as_uint(L.iz) >> 31) * 16u;
+    L.best_d2 = INFINITY;
+    L.best_t = L.best_u = L.best_v = 0.0f;
+    L.best_id = RT_MISS;
+    L.tcull = r.tmax;
+    L.cur = 0;
+    L.sp = 0;
+    L.steps = 0;
+    return (dmax > 0.0f) && (dmax < INFINITY) && (omax < INFINITY);
+}
+
+// One traversal step (one inner node, or one leaf's triangles, then a pop).
+// Returns true when the ray is finished.
 template <bool STATS>
-__device__ __forceinline__ rt_hit trace_one(const TraceArgs& a, const rt_ray& r, uint32_t* stk, uint32_t tid,
-                                            uint32_t gtid, uint64_t& c_nodes, uint64_t& c_tris) {
-    rt_hit out;
-    out.tri = RT_MISS;
-    out.t = 0.0f; out.u = 0.0f; out.v = 0.0f;
-    const f3 o = mk(r.ox, r.oy, r.oz), d = mk(r.dx, r.dy, r.dz);
-    const float dmax = fmaxf(fabsf(d.x), fmaxf(fabsf(d.y), fabsf(d.z)));
-    const float omax = fmaxf(fabsf(o.x), fmaxf(fabsf(o.y), fabsf(o.z)));
-    if (!(dmax > 0.0f) || !(dmax < INFINITY) || !(omax < INFINITY)) return out;
-    const float tmin = r.tmin, tmax = r.tmax;
-    const float sigma = 3.814697265625e-06f * omax / dmax;  // 2^-18 * max|o| / max|d|
-    const float ix = safe_rcp(d.x), iy = safe_rcp(d.y), iz = safe_rcp(d.z);
-    const float nox = -o.x * ix, noy = -o.y * iy, noz = -o.z * iz;
-    const float tlo = fmaxf(tmin, 0.0f);
-
-    float best_d2 = INFINITY, best_t = 0.0f, best_u = 0.0f, best_v = 0.0f;
-    uint32_t best_id = RT_MISS;
-    float tcull = tmax;
-
-    uint32_t sp = 0;
-    uint32_t cur = 0;  // root: inner node 0
-    while (true) {
-        if (!(cur & RT_LEAF_BIT)) {
-            const rt_bvh4_node* nd = a.nodes + cur;
-            const float4 lx = *(const float4*)nd->lo_x, hx = *(const float4*)nd->hi_x;
-            const float4 ly = *(const float4*)nd->lo_y, hy = *(const float4*)nd->hi_y;
-            const float4 lz = *(const float4*)nd->lo_z, hz = *(const float4*)nd->hi_z;
-            const uint4 ch = *(const uint4*)nd->child;
-            if (STATS) ++c_nodes;
-            float key[4];
-            uint32_t ref[4] = {ch.x, ch.y, ch.z, ch.w};
-            const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
-            const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
-            const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+__device__ __forceinline__ bool lane_step(const TraceArgs& a, Lane& L, uint32_t* stk, uint32_t tid, uint32_t gtid,
+                                          uint64_t& c_nodes, uint64_t& c_tris) {
+    if (!(L.cur & RT_LEAF_BIT)) {
+        const char* nb = (const char*)(a.nodes + L.cur);
+        const float4 nx = *(const float4*)(nb + L.offx), fx = *(const float4*)(nb + (L.offx ^ 16u));
+        const float4 ny = *(const float4*)(nb + L.offy), fy = *(const float4*)(nb + (L.offy ^ 16u));
+        const float4 nz = *(const float4*)(nb + L.offz), fz = *(const float4*)(nb + (L.offz ^ 16u));
+        const uint4 ch = *(const uint4*)(nb + 96);
+        if (STATS) ++c_nodes;
+        const float nxa[4] = {nx.x, nx.y, nx.z, nx.w}, fxa[4] = {fx.x, fx.y, fx.z, fx.w};
+        const float nya[4] = {ny.x, ny.y, ny.z, ny.w}, fya[4] = {fy.x, fy.y, fy.z, fy.w};
+        const float nza[4] = {nz.x, nz.y, nz.z, nz.w}, fza[4] = {fz.x, fz.y, fz.z, fz.w};
+        uint32_t ref[4] = {ch.x, ch.y, ch.z, ch.w};
+        float key[4];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                float x0 = fmaf(lxa[c], ix, nox), x1 = fmaf(hxa[c], ix, nox);
-                float y0 = fmaf(lya[c], iy, noy), y1 = fmaf(hya[c], iy, noy);
-                float z0 = fmaf(lza[c], iz, noz), z1 = fmaf(hza[c], iz, noz);
-                float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tlo));
-                float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tcull));
-                bool hit = (tn <= tf) && (ref[c] != RT_CHILD_EMPTY);
-                key[c] = hit ? tn : INFINITY;
-            }
-            // sort the four (key, ref) pairs ascending: 5-comparator network
+        for (int c = 0; c < 4; ++c) {
+            const float tn = fmaxf(fmaxf(fmaf(nxa[c], L.ix, L.nox), fmaf(nya[c], L.iy, L.noy)),
+                                   fmaxf(fmaf(nza[c], L.iz, L.noz), L.tlo));
+            const float tf = fminf(fminf(fmaf(fxa[c], L.ix, L.nox), fmaf(fya[c], L.iy, L.noy)),
+                                   fminf(fmaf(fza[c], L.iz, L.noz), L.tcull));
+            key[c] = (tn <= tf && ref[c] != RT_CHILD_EMPTY) ? tn : INFINITY;
+        }
+        // sort the four (key, ref) pairs ascending: 5-comparator network
 #define RT_CSWAP(i, j)                                                        \
     if (key[j] < key[i]) {                                                    \
         float tk = key[i]; key[i] = key[j]; key[j] = tk;                      \
         uint32_t tr = ref[i]; ref[i] = ref[j]; ref[j] = tr;                   \
     }
-            RT_CSWAP(0, 1) RT_CSWAP(2, 3) RT_CSWAP(0, 2) RT_CSWAP(1, 3) RT_CSWAP(1, 2)
+        RT_CSWAP(0, 1) RT_CSWAP(2, 3) RT_CSWAP(0, 2) RT_CSWAP(1, 3) RT_CSWAP(1, 2)
 #undef RT_CSWAP
-            if (key[0] < INFINITY) {
+        if (key[0] < INFINITY) {
 #pragma unroll
-                for (int c = 3; c >= 1; --c) {
-                    if (key[c] < INFINITY) {
-                        const uint32_t slot = (sp & (STACK_LDS - 1)) * TRACE_BLOCK + tid;
-                        if (sp >= (uint32_t)STACK_LDS) {
-                            uint32_t deep = sp - STACK_LDS;
-                            if (deep < a.spill_cap) a.spill[(size_t)deep * a.spill_stride + gtid] = stk[slot];
-                            else a.work[1] = 1u;  // cannot happen for a stack sized from the depth
-                        }
-                        stk[slot] = ref[c];
-                        ++sp;
+            for (int c = 3; c >= 1; --c) {
+                if (key[c] < INFINITY) {
+                    const uint32_t slot = (L.sp & (STACK_LDS - 1)) * TRACE_BLOCK + tid;
+                    if (L.sp >= (uint32_t)STACK_LDS) {
+                        const uint32_t deep = L.sp - STACK_LDS;
+                        if (deep < a.spill_cap) a.spill[(size_t)deep * a.spill_stride + gtid] = stk[slot];
+                        else a.work[1] = 1u;  // cannot happen for a stack sized from the depth
                     }
+                    stk[slot] = ref[c];
+                    ++L.sp;
                 }
-                cur = ref[0];
-                continue;
             }
-        } else {
-            const uint32_t first = cur & RT_LEAF_FIRST_MASK;
-            const uint32_t cnt = ((cur >> RT_LEAF_COUNT_SHIFT) & 15u) + 1u;
-            for (uint32_t j = 0; j < cnt; ++j) {
-                const float4* rp = (const float4*)(a.tris + first + j);
-                const float4 p0 = rp[0], p1 = rp[1], p2 = rp[2];
-                if (STATS) ++c_tris;
-                float t, u, v;
-                if (intersect_mt(o, d, mk(p0.x, p0.y, p0.z), mk(p1.x, p1.y, p1.z), mk(p2.x, p2.y, p2.z), t, u, v) &&
-                    t > tmin && t < tmax) {
-                    const uint32_t id = __float_as_uint(p0.w);
-                    const float dd = dist2(o, hit_point(o, d, t));
-                    if (closer(dd, id, best_d2, best_id)) {
-                        best_d2 = dd; best_id = id; best_t = t; best_u = u; best_v = v;
-                        tcull = fminf(tmax, fmaf(best_t, 1.0009765625f, sigma));
-                    }
+            L.cur = ref[0];
+            return false;
+        }
+    } else {
+        const uint32_t first = L.cur & RT_LEAF_FIRST_MASK;
+        const uint32_t cnt = ((L.cur >> RT_LEAF_COUNT_SHIFT) & 15u) + 1u;
+        for (uint32_t j = 0; j < cnt; ++j) {
+            const float4* rp = (const float4*)(a.tris + first + j);
+            const float4 p0 = rp[0], p1 = rp[1], p2 = rp[2];
+            if (STATS) ++c_tris;
+            float t, u, v;
+            if (intersect_mt_fast(L.o, L.d, mk(p0.x, p0.y, p0.z), mk(p1.x, p1.y, p1.z), mk(p2.x, p2.y, p2.z),
+                                  L.tmin, L.tmax, L.tcull, t, u, v) &&
+                t > L.tmin && t < L.tmax) {
+                const uint32_t id = __float_as_uint(p0.w);
+                const float dd = dist2(L.o, hit_point(L.o, L.d, t));
+                if (closer(dd, id, L.best_d2, L.best_id)) {
+                    L.best_d2 = dd; L.best_id = id; L.best_t = t; L.best_u = u; L.best_v = v;
+                    L.tcull = fminf(L.tmax, fmaf(L.best_t, 1.0009765625f, L.sigma));
                 }
             }
         }
-        if (sp == 0) break;
-        --sp;
-        const uint32_t slot = (sp & (STACK_LDS - 1)) * TRACE_BLOCK + tid;
-        cur = stk[slot];
-        if (sp >= (uint32_t)STACK_LDS) {
-            uint32_t deep = sp - STACK_LDS;
-            if (deep < a.spill_cap) stk[slot] = a.spill[(size_t)deep * a.spill_stride + gtid];
-        }
     }
-    if (best_id != RT_MISS) {
-        out.tri = best_id; out.t = best_t; out.u = best_u; out.v = best_v;
+    if (L.sp == 0) return true;
+    if (++L.steps > a.max_steps) {  // cannot happen for a tree; ends the ray instead of looping
+        a.work[1] = 2u;
+        return true;
     }
-    return out;
+    --L.sp;
+    const uint32_t slot = (L.sp & (STACK_LDS - 1)) * TRACE_BLOCK + tid;
+    L.cur = stk[slot];
+    if (L.sp >= (uint32_t)STACK_LDS) {
+        const uint32_t deep = L.sp - STACK_LDS;
+        if (deep < a.spill_cap) stk[slot] = a.spill[(size_t)deep * a.spill_stride + gtid];
+    }
+    return false;
 }
 
+// Persistent waves; every lane takes a new ray as soon as its ray finishes
+// (dynamic fetch, Aila & Laine 2009), from a wave-local pool of POOL_CHUNK
+// consecutive rays refilled with one global atomic, so that a wave is not
+// held up by its slowest ray.
 template <bool STATS>
 __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceArgs a) {
     __shared__ uint32_t stk[STACK_LDS * TRACE_BLOCK];
     const uint32_t tid = threadIdx.x;
     const uint32_t gtid = blockIdx.x * TRACE_BLOCK + tid;
     const uint32_t lane = tid & 63u;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     uint64_t c_nodes = 0, c_tris = 0, c_hits = 0, c_rays = 0;
     const uint32_t n = a.n_dev ? *a.n_dev : a.n;
+    uint32_t pool_next = 0, pool_end = 0;  // wave-uniform
+    bool exhausted = false;                // wave-uniform
+    bool have = false, retired = false;
+    uint32_t ri = 0;
+    Lane L;
     while (true) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(a.work, 64u);
-        base = __shfl(base, 0);
-        if (base >= n) break;
-        const uint32_t ri = base + lane;
-        if (ri < n) {
-            const rt_ray r = a.rays[ri];
-            rt_hit h = trace_one<STATS>(a, r, stk, tid, gtid, c_nodes, c_tris);
+        const uint64_t need = __ballot(!have && !retired);
+        if (need) {
+            if (pool_next >= pool_end && !exhausted) {
+                const int leader = __ffsll((unsigned long long)need) - 1;
+                uint32_t g = 0;
+                if ((int)lane == leader) g = atomicAdd(a.work, POOL_CHUNK);
+                g = __shfl(g, leader);
+                if (g >= n) {
+                    exhausted = true;
+                } else {
+                    pool_next = g;
+                    pool_end = min(g + POOL_CHUNK, n);
+                }
+            }
+            if (pool_next < pool_end) {
+                if (!have && !retired) {
+                    const uint32_t idx = pool_next + (uint32_t)__popcll(need & lt);
+                    if (idx < pool_end) {
+                        ri = idx;
+                        have = true;
+                        if (!lane_start(L, a.rays[ri])) {
+                            rt_hit h;
+                            h.tri = RT_MISS; h.t = 0.0f; h.u = 0.0f; h.v = 0.0f;
+                            a.hits[ri] = h;
+                            have = false;
+                            if (STATS) ++c_rays;
+                        }
+                    }
+                }
+                pool_next = min(pool_end, pool_next + (uint32_t)__popcll(need));
+            } else if (exhausted && !have) {
+                retired = true;
+            }
+        }
+        if (__ballot(!retired) == 0ull) break;
+        if (have && lane_step<STATS>(a, L, stk, tid, gtid, c_nodes, c_tris)) {
+            rt_hit h;
+            h.tri = L.best_id;
+            const bool hit = L.best_id != RT_MISS;
+            h.t = hit ? L.best_t : 0.0f;
+            h.u = hit ? L.best_u : 0.0f;
+            h.v = hit ? L.best_v : 0.0f;
             a.hits[ri] = h;
-            if (STATS) { ++c_rays; c_hits += (h.tri != RT_MISS); }
+            have = false;
+            if (STATS) { ++c_rays; c_hits += hit; }
         }
     }
     if (STATS) {
@@ -205,6 +279,7 @@ int launch_trace(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, bo
     a.spill = (uint32_t*)c->spill.ptr;
     a.spill_cap = spill_cap;
     a.spill_stride = threads;
+    a.max_steps = 2u * c->num_nodes + c->T + 16u;
     a.stats = (unsigned long long*)c->stats.ptr;
     RT_HIP(c, hipEventRecord(c->ev[2], c->stream));
     if (stats) hipLaunchKernelGGL(k_trace<true>, dim3(grid), dim3(TRACE_BLOCK), 0, c->stream, a);
