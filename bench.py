@@ -46,6 +46,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-halves", action="store_true", help="skip the per-half (coherent / incoherent) timing")
+    ap.add_argument("--leaf", type=int, default=4, help="BVH leaf size (1..16)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 512 MiB device copy (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -84,8 +85,8 @@ def main() -> int:
     with Tracer(0) as t:
         t.set_stream(stream.cuda_stream)
         t.upload(sc)
-        info = t.build(4)
-        info = t.build(4)  # second build: steady-state build time
+        info = t.build(args.leaf)
+        info = t.build(args.leaf)  # second build: steady-state build time
         log(f"[bench] BVH4: {info['num_nodes']} nodes, depth {info['depth']}, build {info['build_ms']:.2f} ms")
 
         # counters pass (same kernel with per-wave counters): node / triangle fetches for the roofline
@@ -163,7 +164,7 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": f"ray-cast microbench: {sc.num_triangles}-triangle lognormal soup, "
                                f"{n} rays ({n_coh} coherent pinhole + {n - n_coh} incoherent), closest hit",
-                   "triangles": sc.num_triangles, "rays": n, "bvh": "BVH4 LBVH leaf<=4",
+                   "triangles": sc.num_triangles, "rays": n, "bvh": f"BVH4 LBVH leaf<={args.leaf}",
                    "bvh_build_ms": round(info["build_ms"], 3),
                    "mrays_per_s_coherent": halves["coherent"] and round(halves["coherent"], 1),
                    "mrays_per_s_incoherent": halves["incoherent"] and round(halves["incoherent"], 1),

@@ -2,6 +2,7 @@
 // trace and render dispatch.  Every entry point validates its arguments on the
 // host before anything is launched, returns a status code and never throws.
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <new>
@@ -113,6 +114,10 @@ int rt_create(rt_ctx** out, int device) {
     if (!c) return fail(nullptr, RT_E_NOMEM, "rt_create: host allocation failed");
     c->device = device;
     c->num_cus = prop.multiProcessorCount;
+    if (const char* r = getenv("MTRT_TRACE_REFILL")) {  // tuning knob, 1..64
+        long v = strtol(r, nullptr, 10);
+        c->trace_refill_min = (uint32_t)(v < 1 ? 1 : (v > 64 ? 64 : v));
+    }
     // a blocking stream: work here is ordered after earlier work on the null
     // stream (where e.g. torch's default stream writes the rays)
     e = hipStreamCreateWithFlags(&c->own_stream, hipStreamDefault);

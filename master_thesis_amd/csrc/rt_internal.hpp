@@ -53,6 +53,7 @@ struct rt_ctx {
     rt::DevBuf keys[2], vals[2], hist, inner, leafbox, innerbox, visit, queue[2], counters;
     // trace scratch
     rt::DevBuf rays, hits, spill, work, stats;
+    uint32_t trace_refill_min = 1;  // traversal: idle lanes that trigger a ray refill (MTRT_TRACE_REFILL)
     // path tracer state
     rt::DevBuf pt_state, pt_queue[2], pt_image, pt_img8, pt_dimg, pt_dcolor, pt_ctr, pt_partial;
     uint32_t render_iters = 0;

@@ -42,7 +42,8 @@ struct TraceArgs {
     uint32_t* spill;       // [spill_cap][grid threads]
     uint32_t spill_cap;
     uint32_t spill_stride;
-    uint32_t max_steps;    // bound on steps per ray (each node and leaf is visited at most once)
+    uint32_t max_steps;    // bound on steps per ray (each node and triangle is visited at most once)
+    uint32_t refill_min;   // idle lanes that trigger a refill (1..64)
     unsigned long long* stats;  // rays, nodes, tris, hits
 };
 
@@ -140,23 +141,28 @@ __device__ __forceinline__ bool lane_step(const TraceArgs& a, Lane& L, uint32_t*
             return false;
         }
     } else {
+        // one triangle per step: a lane in a leaf costs one triangle test per
+        // iteration, not the whole leaf, so lanes at inner nodes wait less;
+        // the leaf reference shrinks in place to the remaining triangles
         const uint32_t first = L.cur & RT_LEAF_FIRST_MASK;
-        const uint32_t cnt = ((L.cur >> RT_LEAF_COUNT_SHIFT) & 15u) + 1u;
-        for (uint32_t j = 0; j < cnt; ++j) {
-            const float4* rp = (const float4*)(a.tris + first + j);
-            const float4 p0 = rp[0], p1 = rp[1], p2 = rp[2];
-            if (STATS) ++c_tris;
-            float t, u, v;
-            if (intersect_mt_fast(L.o, L.d, mk(p0.x, p0.y, p0.z), mk(p1.x, p1.y, p1.z), mk(p2.x, p2.y, p2.z),
-                                  L.tmin, L.tmax, L.tcull, t, u, v) &&
-                t > L.tmin && t < L.tmax) {
-                const uint32_t id = __float_as_uint(p0.w);
-                const float dd = dist2(L.o, hit_point(L.o, L.d, t));
-                if (closer(dd, id, L.best_d2, L.best_id)) {
-                    L.best_d2 = dd; L.best_id = id; L.best_t = t; L.best_u = u; L.best_v = v;
-                    L.tcull = fminf(L.tmax, fmaf(L.best_t, 1.0009765625f, L.sigma));
-                }
+        const uint32_t rest = (L.cur >> RT_LEAF_COUNT_SHIFT) & 15u;  // triangles after this one
+        const float4* rp = (const float4*)(a.tris + first);
+        const float4 p0 = rp[0], p1 = rp[1], p2 = rp[2];
+        if (STATS) ++c_tris;
+        float t, u, v;
+        if (intersect_mt_fast(L.o, L.d, mk(p0.x, p0.y, p0.z), mk(p1.x, p1.y, p1.z), mk(p2.x, p2.y, p2.z), L.tmin,
+                              L.tmax, L.tcull, t, u, v) &&
+            t > L.tmin && t < L.tmax) {
+            const uint32_t id = __float_as_uint(p0.w);
+            const float dd = dist2(L.o, hit_point(L.o, L.d, t));
+            if (closer(dd, id, L.best_d2, L.best_id)) {
+                L.best_d2 = dd; L.best_id = id; L.best_t = t; L.best_u = u; L.best_v = v;
+                L.tcull = fminf(L.tmax, fmaf(L.best_t, 1.0009765625f, L.sigma));
             }
+        }
+        if (rest) {
+            L.cur = RT_LEAF_BIT | ((rest - 1u) << RT_LEAF_COUNT_SHIFT) | (first + 1u);
+            return false;
         }
     }
     if (L.sp == 0) return true;
@@ -194,7 +200,9 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceArgs a) {
     Lane L;
     while (true) {
         const uint64_t need = __ballot(!have && !retired);
-        if (need) {
+        // refill once enough lanes are idle (or none is busy), so lanes that
+        // start together stay in step
+        if (need && ((uint32_t)__popcll(need) >= a.refill_min || __ballot(have) == 0ull)) {
             if (pool_next >= pool_end && !exhausted) {
                 const int leader = __ffsll((unsigned long long)need) - 1;
                 uint32_t g = 0;
@@ -279,7 +287,8 @@ int launch_trace(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, bo
     a.spill = (uint32_t*)c->spill.ptr;
     a.spill_cap = spill_cap;
     a.spill_stride = threads;
-    a.max_steps = 2u * c->num_nodes + c->T + 16u;
+    a.max_steps = 2u * c->num_nodes + 2u * c->T + 16u;
+    a.refill_min = c->trace_refill_min;
     a.stats = (unsigned long long*)c->stats.ptr;
     RT_HIP(c, hipEventRecord(c->ev[2], c->stream));
     if (stats) hipLaunchKernelGGL(k_trace<true>, dim3(grid), dim3(TRACE_BLOCK), 0, c->stream, a);
