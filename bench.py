@@ -46,14 +46,22 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-halves", action="store_true", help="skip the per-half (coherent / incoherent) timing")
-    ap.add_argument("--leaf", type=int, default=4, help="BVH leaf size (1..16)")
+    ap.add_argument("--leaf", type=int, default=2, help="BVH leaf size (1..16)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 512 MiB device copy (reference kernel for HBM counters)")
+    ap.add_argument("--config", default="raycast", choices=["raycast", "cornell", "terrain", "diff"],
+                    help="raycast: the headline microbench (default); cornell / terrain / diff: the other workloads")
+    ap.add_argument("--views", type=int, default=16, help="terrain: camera views batched in one launch")
+    ap.add_argument("--spp", type=int, default=0, help="cornell / terrain / diff: override samples per pixel")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (rays and pixels would be sharded "
-              "by range with the BVH replicated per GPU; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+              "by range with the BVH replicated on every GPU). Run with --gpus 1.", file=sys.stderr)
         return 2
+    if args.config != "raycast":
+        out = {"cornell": bench_cornell, "terrain": bench_terrain, "diff": bench_diff}[args.config](args)
+        print(json.dumps(out), flush=True)
+        return 0
 
     import torch
 
@@ -202,6 +210,162 @@ def cpu_baseline(sc, rays, gpu_hits, target_s: float):
     return {"value": round(sample.shape[0] / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"every {stride}th ray of the same 16M ({sample.shape[0]} rays, both halves), oracle binned-SAH "
                       f"BVH built in {build_s:.1f}s; hits identical to GPU: {same}"}
+
+
+def _render_timed(t, cams, s, reps: int):
+    """Median over reps of the library's own device time for one render call."""
+    times, rays = [], 0
+    for _ in range(reps):
+        rgb, _ = t.render_views(cams, s)
+        tm = t.timing()
+        times.append(tm["last_call_ms"])
+        rays = tm["rays_traced"]
+    return rgb, float(np.median(times)), rays
+
+
+def bench_cornell(args):
+    """Cornell-style box, 256 x 256, 16 spp, depth 5 (the reference estimator):
+    spp/s and Mrays/s on the GPU, relative RMSE against a 4096-spp render
+    (the deterministic GPU mode, which equals the oracle bit for bit), and the
+    oracle on the host cores over the same frame."""
+    from master_thesis_amd import scenes, host_lib as H
+    from master_thesis_amd._lib import RT_RENDER_DETERMINISTIC
+    from master_thesis_amd.tracer import Tracer, make_settings
+    spp = args.spp or 16
+    sc = scenes.cornell_box()
+    cam = H.cornell_camera(256, 256)
+    s = make_settings(spp=spp, max_depth=5, seed=3)
+    with Tracer(0) as t:
+        t.upload(sc)
+        t.build(args.leaf)
+        _render_timed(t, [cam], s, 2)
+        rgb, ms, rays = _render_timed(t, [cam], s, 5)
+        ref, _ = t.render(cam, make_settings(spp=4096, max_depth=5, seed=99, flags=RT_RENDER_DETERMINISTIC))
+    rel_rmse = float(np.sqrt(np.mean((rgb[0] - ref) ** 2)) / np.sqrt(np.mean(ref ** 2)))
+    P = 256 * 256
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        threads = min(16, os.cpu_count() or 1)
+        t0 = time.time()
+        O.render(sc, cam, s, threads)
+        dt = time.time() - t0
+        cpu = {"value": round(P * spp / dt, 1), "unit": "spp/s", "cores": threads, "kind": "port",
+               "sample": f"the whole frame (256x256, {spp} spp, depth 5) with the oracle's CPU BVH"}
+    return {"metric": "spp/s (Cornell-style box, 256x256, 16 spp, depth 5)", "value": round(P * spp / (ms / 1e3), 1),
+            "unit": "spp/s", "ms_per_frame": round(ms, 3), "mrays_per_s": round(rays / (ms / 1e3) / 1e6, 1),
+            "rays_per_frame": int(rays), "rel_rmse_vs_4096spp": round(rel_rmse, 4), "higher_is_better": True,
+            "dtype": "f32", "data": "synthetic", "n_gpus": 1,
+            "config": {"workload": "Cornell-style box: 5 walls, 2 boxes, 1 light (36 triangles), FOV 40, 256x256, "
+                                   f"{spp} spp, depth 5", "triangles": int(sc.num_triangles)},
+            "cpu_baseline": cpu}
+
+
+def bench_terrain(args):
+    """fBm terrain (2048^2, 8.38M triangles) + 64 icospheres (1.31M), sun and
+    sky, 1920x1080, 64 spp, depth 8, 16 views in one launch sequence."""
+    from master_thesis_amd import scenes
+    from master_thesis_amd._lib import RT_RENDER_SKY, RT_RENDER_SUN
+    from master_thesis_amd.tracer import Tracer, make_settings
+    spp = args.spp or 64
+    t0 = time.time()
+    sc = scenes.terrain_scene()
+    cams = scenes.terrain_cameras(args.views, 1920, 1080)
+    log(f"[bench] terrain {sc.num_triangles} tris generated in {time.time() - t0:.1f}s")
+    s = make_settings(spp=spp, max_depth=8, seed=5, flags=RT_RENDER_SKY | RT_RENDER_SUN, sky=(0.45, 0.55, 0.75),
+                      sun_dir=(0.4, 0.8, 0.3), sun_radius_deg=5.0, sun_radiance=(40.0, 36.0, 30.0))
+    with Tracer(0) as t:
+        t.upload(sc)
+        info = t.build(args.leaf)
+        info = t.build(args.leaf)
+        log(f"[bench] BVH4 {info['num_nodes']} nodes depth {info['depth']} build {info['build_ms']:.1f} ms")
+        rgb, ms, rays = _render_timed(t, cams, s, 1)
+    P = 1920 * 1080 * len(cams)
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        from master_thesis_amd import host_lib as H  # noqa: F401
+        threads = min(16, os.cpu_count() or 1)
+        crop = scenes.terrain_cameras(1, 240, 135)
+        s1 = make_settings(spp=1, max_depth=8, seed=5, flags=s.flags, sky=(0.45, 0.55, 0.75),
+                           sun_dir=(0.4, 0.8, 0.3), sun_radius_deg=5.0, sun_radiance=(40.0, 36.0, 30.0))
+        t0 = time.time()
+        O.render(sc, crop[0], s1, threads)
+        dt = time.time() - t0
+        cpu = {"value": round(240 * 135 / dt, 1), "unit": "spp/s", "cores": threads, "kind": "port",
+               "sample": "one 240x135 view at 1 spp, depth 8, including the oracle's CPU BVH build over 9.69M "
+                         "triangles (the GPU figure excludes its build)"}
+    return {"metric": "Grays/s closest-hit extension rays (terrain, 16 views, 1080p, 64 spp, depth 8)",
+            "value": round(rays / (ms / 1e3) / 1e9, 3), "unit": "Grays/s", "spp_per_s": round(P * spp / (ms / 1e3), 1),
+            "ms_per_batch": round(ms, 1), "rays": int(rays), "views": len(cams), "higher_is_better": True,
+            "dtype": "f32", "data": "synthetic", "n_gpus": 1, "bvh_build_ms": round(info["build_ms"], 2),
+            "config": {"workload": "fBm heightfield 2048x2048 (6 octaves) + 64 level-5 icospheres, sun + sky, "
+                                   f"1920x1080, {spp} spp, depth 8, {len(cams)} views in one launch sequence",
+                       "triangles": int(sc.num_triangles)},
+            "image_mean": [round(float(x), 4) for x in rgb.reshape(-1, 3).mean(axis=0)],
+            "cpu_baseline": cpu}
+
+
+def bench_diff(args):
+    """Differentiable Cornell-style box, 64 albedo groups, 256x256, 32 spp,
+    depth 4: time of forward render + path-replay adjoint, agreement with
+    central differences (h = 1e-3, common random numbers), oracle adjoint on
+    the host cores."""
+    from master_thesis_amd import scenes, host_lib as H
+    from master_thesis_amd.tracer import Tracer, make_settings
+    spp = args.spp or 32
+    sc = scenes.cornell_groups(seed=0)
+    cam = H.cornell_camera(256, 256)
+    s = make_settings(spp=spp, max_depth=4, seed=7)
+    target_sc = scenes.cornell_groups(seed=1)
+    w = None
+    with Tracer(0) as t:
+        t.upload(target_sc)
+        t.build(args.leaf)
+        target, _ = t.render(cam, s)
+        t.upload(sc)
+        t.build(args.leaf)
+        times = []
+        for rep in range(4):
+            img, _ = t.render(cam, s)
+            f_ms = t.timing()["last_call_ms"]
+            w = (2.0 * (img - target)).astype(np.float32)  # dL/dI of L = sum (I - target)^2
+            g = t.render_adjoint(cam, s, w)
+            a_ms = t.timing()["last_call_ms"]
+            if rep:
+                times.append((f_ms, a_ms))
+        base = sc.materials["color"].copy()
+        checks = []
+        for m, ch in [(3, 0), (20, 1), (45, 2), (60, 0), (64, 1)]:
+            vals = []
+            for sign in (1, -1):
+                c = base.copy()
+                c[m, ch] += np.float32(sign * 1e-3)
+                t.set_material_colors(c)
+                im, _ = t.render(cam, s)
+                vals.append(float(np.sum(w.astype(np.float64) * im)))
+            t.set_material_colors(base)
+            fd = (vals[0] - vals[1]) / 2e-3
+            checks.append({"material": m, "channel": ch, "adjoint": float(g[m, ch]), "fd": fd,
+                           "rel_err": abs(float(g[m, ch]) - fd) / max(abs(fd), 1e-12)})
+    f_ms = float(np.median([x[0] for x in times]))
+    a_ms = float(np.median([x[1] for x in times]))
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        threads = min(16, os.cpu_count() or 1)
+        t0 = time.time()
+        gc = O.render_adjoint(sc, cam, s, w, threads)
+        dt = time.time() - t0
+        cpu = {"value": round(dt * 1e3, 1), "unit": "ms per adjoint pass", "cores": threads, "kind": "port",
+               "sample": "the whole adjoint pass (256x256, 32 spp, depth 4)",
+               "max_rel_diff_vs_gpu": float(np.max(np.abs(gc - g)) / max(float(np.max(np.abs(gc))), 1e-12))}
+    return {"metric": "gradient time, forward + adjoint (64 albedo groups, 256x256, 32 spp, depth 4)",
+            "value": round(f_ms + a_ms, 3), "unit": "ms", "forward_ms": round(f_ms, 3), "adjoint_ms": round(a_ms, 3),
+            "higher_is_better": False, "dtype": "f32", "data": "synthetic", "n_gpus": 1,
+            "fd_checks": checks, "config": {"workload": "Cornell-style box, 64 triangle groups with learnable RGB "
+                                                        "albedo, L2 loss against a render with other albedos"},
+            "cpu_baseline": cpu}
 
 
 if __name__ == "__main__":

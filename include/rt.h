@@ -121,7 +121,7 @@ typedef struct rt_tri_rec {
 } rt_tri_rec;
 
 typedef struct rt_build_opts {
-    uint32_t leaf_size;   /* max triangles per leaf, 1..16 (0 = default 4) */
+    uint32_t leaf_size;   /* max triangles per leaf, 1..16 (0 = default 2) */
     uint32_t flags;       /* reserved, 0 */
 } rt_build_opts;
 
@@ -155,6 +155,7 @@ typedef struct rt_camera {
 
 #define RT_RENDER_DETERMINISTIC 0x1u  /* one path slot per pixel, samples summed in order (bit-reproducible) */
 #define RT_RENDER_SKY           0x2u  /* extension: a miss returns throughput * sky (reference: black)     */
+#define RT_RENDER_SUN           0x4u  /* extension: plus sun_radiance inside the sun's cone              */
 
 typedef struct rt_settings {
     uint32_t max_depth;       /* PathTracer visionJumpCount (path_tracer.cpp:14)  */
@@ -163,13 +164,18 @@ typedef struct rt_settings {
     uint32_t flags;           /* RT_RENDER_*                                      */
     uint32_t sample_offset;   /* index of the first sample (progressive renders)  */
     float    sky[3];          /* RT_RENDER_SKY radiance                           */
-    uint32_t num_views;       /* reserved for batched views; 0 or 1              */
+    uint32_t num_views;       /* informational; rt_render_views takes the count   */
+    float    sun_dir[3];      /* RT_RENDER_SUN: unit direction towards the sun    */
+    float    sun_cos;         /* cosine of the sun's angular radius               */
+    float    sun_radiance[3];
 } rt_settings;
 
 typedef struct rt_timing {
     float last_kernel_ms;     /* device time of the dominant kernel of the last call   */
     float last_call_ms;       /* device time of the whole last call (first to last op) */
     uint32_t launches;        /* kernel launches the last call made                    */
+    uint32_t iterations;      /* render: extend/shade iterations of the last call      */
+    uint64_t rays_traced;     /* render: closest-hit queries of the last call          */
 } rt_timing;
 
 /* ---- context ------------------------------------------------------------- */
@@ -217,6 +223,14 @@ int  rt_trace_closest(rt_ctx* ctx, const rt_ray* rays, uint32_t n,
  * (graphics_engine.cpp:90-95), may be NULL. */
 int  rt_render(rt_ctx* ctx, const rt_camera* cam, const rt_settings* s,
                float* rgb, uint8_t* rgb8, int ptr_kind);
+
+/* Renders num_views frames of the same scene and settings (cams[v], all of
+ * one size) in one launch sequence whose path slots move freely between the
+ * views.  rgb / rgb8 hold the views one after another.  View v draws its
+ * random numbers with counter (pixel, sample, depth, v), so view 0 equals
+ * rt_render.  Extension: the reference renders one camera per process. */
+int  rt_render_views(rt_ctx* ctx, const rt_camera* cams, uint32_t num_views, const rt_settings* s,
+                     float* rgb, uint8_t* rgb8, int ptr_kind);
 
 /* Gradient of a scalar image loss with respect to every material colour,
  * by path replay.  d_image: dL/d(rgb) (width*height*3), d_color: output

@@ -29,6 +29,7 @@ RT_LEAF_COUNT_SHIFT = 27
 RT_LEAF_FIRST_MASK = 0x07FFFFFF
 RT_RENDER_DETERMINISTIC = 0x1
 RT_RENDER_SKY = 0x2
+RT_RENDER_SUN = 0x4
 
 _ERR_NAMES = {
     RT_E_INVALID: "RT_E_INVALID",
@@ -106,11 +107,13 @@ class rt_camera(C.Structure):
 
 class rt_settings(C.Structure):
     _fields_ = [("max_depth", C.c_uint32), ("spp", C.c_uint32), ("seed", C.c_uint64), ("flags", C.c_uint32),
-                ("sample_offset", C.c_uint32), ("sky", C.c_float * 3), ("num_views", C.c_uint32)]
+                ("sample_offset", C.c_uint32), ("sky", C.c_float * 3), ("num_views", C.c_uint32),
+                ("sun_dir", C.c_float * 3), ("sun_cos", C.c_float), ("sun_radiance", C.c_float * 3)]
 
 
 class rt_timing(C.Structure):
-    _fields_ = [("last_kernel_ms", C.c_float), ("last_call_ms", C.c_float), ("launches", C.c_uint32)]
+    _fields_ = [("last_kernel_ms", C.c_float), ("last_call_ms", C.c_float), ("launches", C.c_uint32),
+                ("iterations", C.c_uint32), ("rays_traced", C.c_uint64)]
 
 
 assert C.sizeof(rt_ray) == 32 and RAY_DTYPE.itemsize == 32
@@ -133,6 +136,7 @@ SIGNATURES = {
     "rt_bvh_export": (C.c_int, [_P, _P, C.POINTER(C.c_uint32), _P, C.POINTER(C.c_uint32)]),
     "rt_trace_closest": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_int, C.POINTER(rt_trace_counters)]),
     "rt_render": (C.c_int, [_P, C.POINTER(rt_camera), C.POINTER(rt_settings), _P, _P, C.c_int]),
+    "rt_render_views": (C.c_int, [_P, C.POINTER(rt_camera), C.c_uint32, C.POINTER(rt_settings), _P, _P, C.c_int]),
     "rt_render_adjoint": (C.c_int, [_P, C.POINTER(rt_camera), C.POINTER(rt_settings), _P, _P, C.c_int]),
 }
 

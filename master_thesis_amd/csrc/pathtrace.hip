@@ -24,9 +24,10 @@
 //           global counter (the analogue of pixelCounter.fetch_add,
 //           graphics_engine.cpp:74); surviving slots are compacted into the
 //           next queue with one atomic per wave (ballot + popcount).
-// Random numbers: Philox-4x32-10 with counter (pixel, sample, depth, 0) and
-// the 64-bit seed as key, so a path does not depend on scheduling and the
-// adjoint pass replays it exactly.
+// Random numbers: Philox-4x32-10 with counter (pixel, sample, depth, view)
+// and the 64-bit seed as key, so a path does not depend on scheduling and the
+// adjoint pass replays it exactly.  Several views (rt_render_views) share
+// one slot pool: a work item is (chunk, view, pixel).
 //
 // Adjoint (rt_render_adjoint): the same pass with dL/dI given.  A path's
 // contribution is the product of its diffuse factors colour_k * cos_k (the
@@ -50,18 +51,20 @@ constexpr uint32_t CULL_BIT = 0x10000u;
 constexpr int C_ITEM = 0;   // next work item
 constexpr int C_N0 = 1;     // live count, queue 0
 constexpr int C_N1 = 2;     // live count, queue 1
+constexpr int C_RAYS = 4;   // 64-bit count of traced rays (slots 4-5)
 
 struct PTArgs {
     const ShadeGeom* geom;
     const ShadeNrm* nrm;
     const rt_material* mat;
-    float view_inv[16], proj_inv[16];
-    float ox, oy, oz;
-    uint32_t W, H, P;
+    const rt_camera* cams;  // V cameras in device memory
+    uint32_t W, H, P;       // image size, pixels per view
+    uint32_t V, VP;         // views, V * P
     uint32_t K, spc, items, S;
     uint32_t max_depth, sample_offset, flags;
     uint32_t seed_lo, seed_hi;
     float sky[3];
+    float sun_dir[3], sun_cos, sun_rad[3];
     double scale;
     // slot state
     uint32_t* s_item;
@@ -77,7 +80,7 @@ struct PTArgs {
     rt_ray* q_ray[2];
     rt_hit* hits;
     uint32_t* ctr;
-    float* partial;      // 3 * P * K
+    float* partial;      // 3 * VP * K
     // outputs
     float* rgb;
     uint8_t* rgb8;
@@ -92,9 +95,12 @@ __device__ __forceinline__ rt_ray make_ray(f3 o, f3 d) {
     return r;
 }
 
-__device__ __forceinline__ void start_sample(const PTArgs& a, uint32_t slot, uint32_t pix, rt_ray& r, bool adj) {
-    f3 d = primary_dir(a.view_inv, a.proj_inv, pix % a.W, pix / a.W, a.W, a.H);
-    r = make_ray(mk(a.ox, a.oy, a.oz), d);
+// vp = view * P + pixel
+__device__ __forceinline__ void start_sample(const PTArgs& a, uint32_t slot, uint32_t vp, rt_ray& r, bool adj) {
+    const rt_camera& c = a.cams[vp / a.P];
+    const uint32_t pix = vp % a.P;
+    f3 d = primary_dir(c.view_inv, c.proj_inv, pix % a.W, pix / a.W, a.W, a.H);
+    r = make_ray(mk(c.origin[0], c.origin[1], c.origin[2]), d);
     a.s_state[slot] = CULL_BIT;
     a.s_T[3 * slot] = 1.0f; a.s_T[3 * slot + 1] = 1.0f; a.s_T[3 * slot + 2] = 1.0f;
     if (adj) a.s_nv[slot] = 0;
@@ -104,7 +110,17 @@ __device__ __forceinline__ void start_item(const PTArgs& a, uint32_t slot, uint3
     a.s_item[slot] = item;
     a.s_smp[slot] = 0;
     a.s_sum[3 * slot] = 0.0f; a.s_sum[3 * slot + 1] = 0.0f; a.s_sum[3 * slot + 2] = 0.0f;
-    start_sample(a, slot, item % a.P, r, adj);
+    start_sample(a, slot, item % a.VP, r, adj);
+}
+
+// Radiance of an escaped ray (extension; the reference's miss is black):
+// constant sky plus the sun inside its cone.
+__device__ __forceinline__ f3 miss_radiance(const PTArgs& a, f3 d) {
+    f3 L = mk(0.0f, 0.0f, 0.0f);
+    if (a.flags & RT_RENDER_SKY) L = mk(a.sky[0], a.sky[1], a.sky[2]);
+    if ((a.flags & RT_RENDER_SUN) && dot(d, mk(a.sun_dir[0], a.sun_dir[1], a.sun_dir[2])) >= a.sun_cos)
+        L = add(L, mk(a.sun_rad[0], a.sun_rad[1], a.sun_rad[2]));
+    return L;
 }
 
 template <bool ADJ>
@@ -114,6 +130,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_init(PTArgs a) {
         a.ctr[C_ITEM] = a.S;
         a.ctr[C_N0] = a.S;
         a.ctr[C_N1] = 0;
+        *(unsigned long long*)&a.ctr[C_RAYS] = 0ull;
     }
     if (i >= a.S) return;
     rt_ray r;
@@ -158,6 +175,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
     const uint32_t n = a.ctr[cur ? C_N1 : C_N0];
     uint32_t* n_out = &a.ctr[cur ? C_N0 : C_N1];
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) *(unsigned long long*)&a.ctr[C_RAYS] += n;  // rays the extend step just traced
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     bool alive = false, need_item = false;
@@ -168,8 +186,9 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
         const rt_ray r = a.q_ray[cur][k];
         const rt_hit h = a.hits[k];
         const uint32_t item = a.s_item[slot];
-        const uint32_t pix = item % a.P;
-        const uint32_t chunk = item / a.P;
+        const uint32_t vp = item % a.VP;   // view * P + pixel: index into images and partial sums
+        const uint32_t chunk = item / a.VP;
+        const uint32_t view = vp / a.P, pix = vp % a.P;
         uint32_t smp = a.s_smp[slot];
         uint32_t st = a.s_state[slot];
         uint32_t depth = st & 0xFFFFu;
@@ -180,9 +199,9 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
         f3 term = mk(1.0f, 1.0f, 1.0f);
         if (h.tri == RT_MISS) {
             end = true;
-            if (a.flags & RT_RENDER_SKY) {
+            if (a.flags & (RT_RENDER_SKY | RT_RENDER_SUN)) {
                 contrib = true;
-                term = mk(a.sky[0], a.sky[1], a.sky[2]);
+                term = miss_radiance(a, d);
                 T = mulv(T, term);
             }
         } else {
@@ -211,7 +230,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
                 const rt_material m = a.mat[g.mat];
                 const uint32_t gsmp = a.sample_offset + chunk * a.spc + smp;
                 u4 c4;
-                c4.x = pix; c4.y = gsmp; c4.z = depth; c4.w = 0u;
+                c4.x = pix; c4.y = gsmp; c4.z = depth; c4.w = view;
                 const u4 rnd = philox4x32_10(c4, a.seed_lo, a.seed_hi);
                 const float xi0 = u01(rnd.x);
                 const f3 prev = d;
@@ -243,7 +262,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
         if (end) {
             if (contrib) {
                 if (ADJ) {
-                    scatter_grad(a, slot, pix, term);
+                    scatter_grad(a, slot, vp, term);
                 } else {
                     a.s_sum[3 * slot] = a.s_sum[3 * slot] + T.x;
                     a.s_sum[3 * slot + 1] = a.s_sum[3 * slot + 1] + T.y;
@@ -253,11 +272,11 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
             ++smp;
             if (smp < a.spc) {
                 a.s_smp[slot] = smp;
-                start_sample(a, slot, pix, nr, ADJ);
+                start_sample(a, slot, vp, nr, ADJ);
                 alive = true;
             } else {
                 if (!ADJ) {
-                    const size_t q = (size_t)chunk * a.P + pix;
+                    const size_t q = (size_t)chunk * a.VP + vp;
                     a.partial[3 * q] = a.s_sum[3 * slot];
                     a.partial[3 * q + 1] = a.s_sum[3 * slot + 1];
                     a.partial[3 * q + 2] = a.s_sum[3 * slot + 2];
@@ -303,10 +322,10 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
 
 __global__ void __launch_bounds__(PT_BLOCK) k_pt_finish(PTArgs a) {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.P) return;
+    if (p >= a.VP) return;
     float s[3] = {0.0f, 0.0f, 0.0f};
     for (uint32_t c = 0; c < a.K; ++c)
-        for (int ch = 0; ch < 3; ++ch) s[ch] = s[ch] + a.partial[3 * ((size_t)c * a.P + p) + ch];
+        for (int ch = 0; ch < 3; ++ch) s[ch] = s[ch] + a.partial[3 * ((size_t)c * a.VP + p) + ch];
     for (int ch = 0; ch < 3; ++ch) {
         // finalColor *= 2.0f * M_PI * (1.0f / float(raysPerPixel))  (double scale)
         float v = (float)((double)s[ch] * a.scale);
@@ -328,13 +347,15 @@ inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) /
 
 }  // namespace
 
-int launch_render(rt_ctx* c, const rt_camera* cam, const rt_settings* s, float* d_rgb, uint8_t* d_rgb8,
-                  const float* d_dimage, float* d_dcolor) {
+int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_settings* s, float* d_rgb,
+                  uint8_t* d_rgb8, const float* d_dimage, float* d_dcolor) {
     const bool adj = d_dcolor != nullptr;
     if (adj && s->max_depth > ADJ_MAX_DEPTH)
         return fail(c, RT_E_UNSUPPORTED, "rt_render_adjoint: max_depth above 16");
-    const uint64_t P = (uint64_t)cam->width * cam->height;
-    if (P > (1ull << 31)) return fail(c, RT_E_UNSUPPORTED, "rt_render: image above 2^31 pixels");
+    const rt_camera* cam = &cams[0];
+    const uint64_t P1 = (uint64_t)cam->width * cam->height;
+    const uint64_t P = P1 * V;  // view-major (view, pixel) pairs
+    if (P > (1ull << 31)) return fail(c, RT_E_UNSUPPORTED, "rt_render: views * pixels above 2^31");
     const uint32_t N = s->spp;
     // samples of a pixel are split into K chunks (K | N) when there are too
     // few pixels to fill the device; deterministic mode keeps one chunk so
@@ -356,19 +377,25 @@ int launch_render(rt_ctx* c, const rt_camera* cam, const rt_settings* s, float* 
         (rc = ensure(c, c->pt_queue[0], 4 * szS + 16 + sizeof(rt_ray) * szS)) ||
         (rc = ensure(c, c->pt_queue[1], 4 * szS + 16 + sizeof(rt_ray) * szS)) ||
         (rc = ensure(c, c->hits, szS * sizeof(rt_hit))) || (rc = ensure(c, c->pt_ctr, 64)) ||
-        (rc = ensure(c, c->pt_partial, adj ? 4 : 12 * items)))
+        (rc = ensure(c, c->pt_partial, adj ? 4 : 12 * items)) || (rc = ensure(c, c->pt_cams, sizeof(rt_camera) * V)))
         return rc;
+    RT_HIP(c, hipMemcpyAsync(c->pt_cams.ptr, cams, sizeof(rt_camera) * V, hipMemcpyHostToDevice, c->stream));
     PTArgs a;
     a.geom = (const ShadeGeom*)c->shade_geom.ptr;
     a.nrm = (const ShadeNrm*)c->shade_nrm.ptr;
     a.mat = (const rt_material*)c->mat.ptr;
-    for (int k = 0; k < 16; ++k) { a.view_inv[k] = cam->view_inv[k]; a.proj_inv[k] = cam->proj_inv[k]; }
-    a.ox = cam->origin[0]; a.oy = cam->origin[1]; a.oz = cam->origin[2];
-    a.W = cam->width; a.H = cam->height; a.P = (uint32_t)P;
+    a.cams = (const rt_camera*)c->pt_cams.ptr;
+    a.W = cam->width; a.H = cam->height; a.P = (uint32_t)P1;
+    a.V = V; a.VP = (uint32_t)P;
     a.K = K; a.spc = N / K; a.items = (uint32_t)items; a.S = S;
     a.max_depth = s->max_depth; a.sample_offset = s->sample_offset; a.flags = s->flags;
     a.seed_lo = (uint32_t)s->seed; a.seed_hi = (uint32_t)(s->seed >> 32);
-    a.sky[0] = s->sky[0]; a.sky[1] = s->sky[1]; a.sky[2] = s->sky[2];
+    for (int k = 0; k < 3; ++k) {
+        a.sky[k] = s->sky[k];
+        a.sun_dir[k] = s->sun_dir[k];
+        a.sun_rad[k] = s->sun_radiance[k];
+    }
+    a.sun_cos = s->sun_cos;
     a.scale = 2.0 * M_PI * (double)(1.0f / (float)N);
     char* st = (char*)c->pt_state.ptr;
     a.s_item = (uint32_t*)st; st += 4 * szS;
@@ -425,11 +452,15 @@ int launch_render(rt_ctx* c, const rt_camera* cam, const rt_settings* s, float* 
         ++launches;
     }
     RT_HIP(c, hipEventRecord(c->ev[1], stream));
-    RT_HIP(c, hipEventSynchronize(c->ev[1]));
+    unsigned long long rays = 0;
+    RT_HIP(c, hipMemcpyAsync(&rays, &a.ctr[C_RAYS], 8, hipMemcpyDeviceToHost, stream));
+    RT_HIP(c, hipStreamSynchronize(stream));
     float ms = 0.0f;
     RT_HIP(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
     c->timing.last_call_ms = ms;
     c->timing.launches = launches;
+    c->timing.iterations = (uint32_t)iter;
+    c->timing.rays_traced = rays;
     c->last_op = 2;
     c->render_iters = (uint32_t)iter;
     return RT_OK;

@@ -145,7 +145,8 @@ void rt_destroy(rt_ctx* c) {
                      &c->keys[0], &c->keys[1], &c->vals[0], &c->vals[1], &c->hist, &c->inner, &c->leafbox,
                      &c->innerbox, &c->visit, &c->queue[0], &c->queue[1], &c->counters, &c->rays, &c->hits,
                      &c->spill, &c->work, &c->stats, &c->pt_state, &c->pt_queue[0], &c->pt_queue[1],
-                     &c->pt_image, &c->pt_img8, &c->pt_dimg, &c->pt_dcolor, &c->pt_ctr, &c->pt_partial};
+                     &c->pt_image, &c->pt_img8, &c->pt_dimg, &c->pt_dcolor, &c->pt_ctr, &c->pt_partial,
+                     &c->pt_cams};
     for (DevBuf* b : all) release(c, *b);
     for (int k = 0; k < 4; ++k)
         if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
@@ -244,7 +245,7 @@ int rt_set_material_colors(rt_ctx* c, const float* rgb, int ptr_kind) {
 int rt_build(rt_ctx* c, const rt_build_opts* o, rt_bvh_info* info) {
     if (!c) return RT_E_INVALID;
     if (c->T == 0) return fail(c, RT_E_STATE, "rt_build: no scene uploaded");
-    uint32_t leaf = (o && o->leaf_size) ? o->leaf_size : 4u;
+    uint32_t leaf = (o && o->leaf_size) ? o->leaf_size : 2u;
     if (leaf > RT_MAX_LEAF_SIZE) return fail(c, RT_E_INVALID, "rt_build: leaf_size above 16");
     DeviceGuard g(c->device);
     int rc = build_lbvh(c, leaf);
@@ -315,15 +316,24 @@ int rt_trace_closest(rt_ctx* c, const rt_ray* rays, uint32_t n, rt_hit* hits, in
 }
 
 int rt_render(rt_ctx* c, const rt_camera* cam, const rt_settings* s, float* rgb, uint8_t* rgb8, int ptr_kind) {
+    return rt_render_views(c, cam, 1, s, rgb, rgb8, ptr_kind);
+}
+
+int rt_render_views(rt_ctx* c, const rt_camera* cams, uint32_t num_views, const rt_settings* s, float* rgb,
+                    uint8_t* rgb8, int ptr_kind) {
     if (!c) return RT_E_INVALID;
-    if (!cam || !s) return fail(c, RT_E_INVALID, "rt_render: NULL camera or settings");
+    if (!cams || !s) return fail(c, RT_E_INVALID, "rt_render: NULL camera or settings");
+    if (num_views == 0) return fail(c, RT_E_INVALID, "rt_render_views: no views");
     if (!c->built) return fail(c, RT_E_STATE, "rt_render: call rt_build first");
-    if (cam->width == 0 || cam->height == 0) return fail(c, RT_E_INVALID, "rt_render: empty image");
+    if (cams[0].width == 0 || cams[0].height == 0) return fail(c, RT_E_INVALID, "rt_render: empty image");
+    for (uint32_t v = 1; v < num_views; ++v)
+        if (cams[v].width != cams[0].width || cams[v].height != cams[0].height)
+            return fail(c, RT_E_INVALID, "rt_render_views: views differ in size");
     if (s->spp == 0) return fail(c, RT_E_INVALID, "rt_render: spp must be >= 1");
     if (s->max_depth == 0 || s->max_depth > 64) return fail(c, RT_E_INVALID, "rt_render: max_depth must be 1..64");
     if (ptr_kind != RT_PTR_HOST && ptr_kind != RT_PTR_DEVICE) return fail(c, RT_E_INVALID, "rt_render: bad ptr_kind");
     DeviceGuard g(c->device);
-    const size_t P = (size_t)cam->width * cam->height;
+    const size_t P = (size_t)cams[0].width * cams[0].height * num_views;
     float* d_rgb = rgb;
     uint8_t* d_rgb8 = rgb8;
     int rc;
@@ -332,7 +342,7 @@ int rt_render(rt_ctx* c, const rt_camera* cam, const rt_settings* s, float* rgb,
         d_rgb = (float*)c->pt_image.ptr;
         d_rgb8 = (uint8_t*)c->pt_img8.ptr;
     }
-    if ((rc = launch_render(c, cam, s, d_rgb, d_rgb8, nullptr, nullptr))) return rc;
+    if ((rc = launch_render(c, cams, num_views, s, d_rgb, d_rgb8, nullptr, nullptr))) return rc;
     if (ptr_kind == RT_PTR_HOST) {
         if (rgb) RT_HIP(c, hipMemcpyAsync(rgb, d_rgb, 12 * P, hipMemcpyDeviceToHost, c->stream));
         if (rgb8) RT_HIP(c, hipMemcpyAsync(rgb8, d_rgb8, 3 * P, hipMemcpyDeviceToHost, c->stream));
@@ -361,7 +371,7 @@ int rt_render_adjoint(rt_ctx* c, const rt_camera* cam, const rt_settings* s, con
         dimg = (const float*)c->pt_dimg.ptr;
         dcol = (float*)c->pt_dcolor.ptr;
     }
-    if ((rc = launch_render(c, cam, s, nullptr, nullptr, dimg, dcol))) return rc;
+    if ((rc = launch_render(c, cam, 1, s, nullptr, nullptr, dimg, dcol))) return rc;
     if (ptr_kind == RT_PTR_HOST) {
         RT_HIP(c, hipMemcpyAsync(d_color, dcol, 12 * (size_t)c->M, hipMemcpyDeviceToHost, c->stream));
         RT_HIP(c, hipStreamSynchronize(c->stream));

@@ -53,9 +53,9 @@ struct rt_ctx {
     rt::DevBuf keys[2], vals[2], hist, inner, leafbox, innerbox, visit, queue[2], counters;
     // trace scratch
     rt::DevBuf rays, hits, spill, work, stats;
-    uint32_t trace_refill_min = 1;  // traversal: idle lanes that trigger a ray refill (MTRT_TRACE_REFILL)
+    uint32_t trace_refill_min = 8;  // traversal: idle lanes that trigger a ray refill (MTRT_TRACE_REFILL)
     // path tracer state
-    rt::DevBuf pt_state, pt_queue[2], pt_image, pt_img8, pt_dimg, pt_dcolor, pt_ctr, pt_partial;
+    rt::DevBuf pt_state, pt_queue[2], pt_image, pt_img8, pt_dimg, pt_dcolor, pt_ctr, pt_partial, pt_cams;
     uint32_t render_iters = 0;
 
     // ev[0..1] bracket a whole call, ev[2..3] the last traversal kernel
@@ -88,7 +88,8 @@ int launch_trace(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, bo
                  const uint32_t* n_dev = nullptr);
 
 // pathtrace.hip
-int launch_render(rt_ctx* c, const rt_camera* cam, const rt_settings* s, float* d_rgb, uint8_t* d_rgb8,
-                  const float* d_dimage, float* d_dcolor);
+// cams: V host cameras of one size; d_rgb / d_rgb8 / d_dimage view-major
+int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_settings* s, float* d_rgb,
+                  uint8_t* d_rgb8, const float* d_dimage, float* d_dcolor);
 
 }  // namespace rt

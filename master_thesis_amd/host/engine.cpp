@@ -46,7 +46,7 @@ void GraphicsEngine::init(std::unique_ptr<Renderer> renderer, const FlatScene& s
     threads_ = std::max(1u, thread_count);
     rt_scene_desc d = scene.desc();
     if (rt_scene_upload(ctx_, &d) != RT_OK) throw InitError("GraphicsEngine", rt_last_error(ctx_));
-    rt_build_opts o{4u, 0u};
+    rt_build_opts o{0u, 0u};  // library default leaf size
     if (rt_build(ctx_, &o, nullptr) != RT_OK) throw InitError("GraphicsEngine", rt_last_error(ctx_));
     rgb_.assign((size_t)width * height * 3, 0.0f);
     rgb8_.assign((size_t)width * height * 3, 0);

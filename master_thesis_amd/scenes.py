@@ -189,6 +189,138 @@ def cornell_camera_position() -> tuple[float, float, float]:
     return (CORNELL_HALF + CORNELL_HALF / np.tan(np.radians(20.0)) + 0.05, 0.0, 0.0)
 
 
+def _hash01(ix: np.ndarray, iz: np.ndarray, octave: int, seed: int) -> np.ndarray:
+    """Lattice value in [0, 1) from a splitmix64-style integer hash."""
+    with np.errstate(over="ignore"):
+        h = (ix.astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)) ^ (iz.astype(np.uint64) * np.uint64(0xC2B2AE3D27D4EB4F))
+        h ^= np.uint64((octave * 0x165667B19E3779F9 + seed * 0x27D4EB2F165667C5) & 0xFFFFFFFFFFFFFFFF)
+        h ^= h >> np.uint64(30)
+        h *= np.uint64(0xBF58476D1CE4E5B9)
+        h ^= h >> np.uint64(27)
+        h *= np.uint64(0x94D049BB133111EB)
+        h ^= h >> np.uint64(31)
+    return (h >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+
+
+def fbm_heightfield(n: int = 2048, octaves: int = 6, seed: int = 7, base_cells: float = 8.0) -> np.ndarray:
+    """n x n heights in [-1, 1]-ish: value noise (smoothstep-interpolated lattice
+    hashes), lacunarity 2, gain 0.5."""
+    u = np.arange(n, dtype=np.float64) / (n - 1)
+    h = np.zeros((n, n), np.float64)
+    amp, freq, total = 1.0, base_cells, 0.0
+    for o in range(octaves):
+        x = u * freq
+        i0 = np.floor(x).astype(np.int64)
+        f = x - i0
+        s = f * f * (3 - 2 * f)
+        X0, Z0 = np.meshgrid(i0, i0, indexing="xy")
+        SX, SZ = np.meshgrid(s, s, indexing="xy")
+        v00 = _hash01(X0, Z0, o, seed)
+        v10 = _hash01(X0 + 1, Z0, o, seed)
+        v01 = _hash01(X0, Z0 + 1, o, seed)
+        v11 = _hash01(X0 + 1, Z0 + 1, o, seed)
+        val = (v00 * (1 - SX) + v10 * SX) * (1 - SZ) + (v01 * (1 - SX) + v11 * SX) * SZ
+        h += amp * (2 * val - 1)
+        total += amp
+        amp *= 0.5
+        freq *= 2.0
+    return h / total
+
+
+def icosphere(level: int = 5):
+    """Unit icosphere: (vertices, triangles) after `level` midpoint subdivisions
+    (20 * 4^level triangles)."""
+    t = (1 + 5 ** 0.5) / 2
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(p, np.float64) / np.linalg.norm(p) for p in v]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(level):
+        cache = {}
+
+        def mid(a, b):
+            key = (a, b) if a < b else (b, a)
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    return np.array(verts, np.float64), np.array(faces, np.int64)
+
+
+def terrain_scene(n: int = 2048, spheres: int = 64, sphere_level: int = 5, extent: float = 100.0,
+                  height: float = 12.0, seed: int = 7) -> Scene:
+    """Procedural landscape: an n x n fBm heightfield (2 (n-1)^2 triangles,
+    6 octaves) over [-extent/2, extent/2]^2 plus `spheres` icospheres at
+    seed-drawn positions resting on it.  Lambertian albedos U(0.1, 0.9): one
+    for the terrain, one per sphere.  Lighting comes from the sky and sun
+    settings (RT_RENDER_SKY / RT_RENDER_SUN)."""
+    rng = np.random.default_rng(seed)
+    hf = fbm_heightfield(n, seed=seed) * height
+    xs = (np.arange(n) / (n - 1) - 0.5) * extent
+    X, Z = np.meshgrid(xs, xs, indexing="xy")
+    pos = np.stack([X, hf, Z], axis=-1).reshape(-1, 3)
+    # normals from central differences of the height field
+    dx = extent / (n - 1)
+    gx = np.gradient(hf, dx, axis=1)
+    gz = np.gradient(hf, dx, axis=0)
+    nrm = np.stack([-gx, np.ones_like(gx), -gz], axis=-1)
+    nrm /= np.linalg.norm(nrm, axis=-1, keepdims=True)
+    nrm = nrm.reshape(-1, 3)
+    i = np.arange(n - 1)
+    I, J = np.meshgrid(i, i, indexing="xy")
+    v00 = (J * n + I).reshape(-1)
+    v10, v01, v11 = v00 + 1, v00 + n, v00 + n + 1
+    tri = np.concatenate([np.stack([v00, v01, v10], 1), np.stack([v10, v01, v11], 1)]).astype(np.uint32)
+    tri_mat = np.zeros(tri.shape[0], np.uint32)
+    mats = [material(rng.uniform(0.1, 0.9, 3))]
+    sv, sf = icosphere(sphere_level)
+    all_pos, all_nrm, all_tri, all_mat = [pos], [nrm], [tri], [tri_mat]
+    base = pos.shape[0]
+    for k in range(spheres):
+        r = rng.uniform(1.0, 3.0)
+        cx, cz = rng.uniform(-0.4, 0.4, 2) * extent
+        gi = int(round((cx / extent + 0.5) * (n - 1)))
+        gj = int(round((cz / extent + 0.5) * (n - 1)))
+        cy = hf[gj, gi] + 0.8 * r
+        all_pos.append(sv * r + [cx, cy, cz])
+        all_nrm.append(sv)
+        all_tri.append((sf + base).astype(np.uint32))
+        all_mat.append(np.full(sf.shape[0], k + 1, np.uint32))
+        mats.append(material(rng.uniform(0.1, 0.9, 3)))
+        base += sv.shape[0]
+    s = Scene(pos=np.concatenate(all_pos).astype(np.float32), nrm=np.concatenate(all_nrm).astype(np.float32),
+              idx=np.ascontiguousarray(np.concatenate(all_tri)), tri_mat=np.concatenate(all_mat),
+              materials=np.stack(mats))
+    s.validate()
+    return s
+
+
+def terrain_cameras(num_views: int = 16, width: int = 1920, height: int = 1080, radius: float = 42.0,
+                    elevation: float = 22.0, fov_deg: float = 45.0):
+    """Cameras on a circle around the terrain looking at its centre (in the
+    reference's camera terms: angle0 = theta / pi, angle1 maps phi)."""
+    from . import host_lib as H
+    cams = []
+    for v in range(num_views):
+        a = 2 * np.pi * v / num_views
+        p = np.array([radius * np.cos(a), elevation, radius * np.sin(a)])
+        d = -p / np.linalg.norm(p)
+        theta = np.arctan2(d[2], d[0])
+        phi = np.arccos(np.clip(d[1], -1, 1))
+        cams.append(H.camera(tuple(p), float(theta / np.pi), float((np.pi / 2 - phi) / (np.pi / 2)),
+                             float(np.radians(fov_deg)), width, height))
+    return cams
+
+
 def _unit_vectors(rng: np.random.Generator, n: int) -> np.ndarray:
     z = rng.uniform(-1.0, 1.0, n)
     phi = rng.uniform(0.0, 2 * np.pi, n)

@@ -65,7 +65,7 @@ class Tracer:
             rgb = np.ascontiguousarray(rgb, np.float32)
         self._check(self.lib.rt_set_material_colors(self.h, L._ptr(rgb), kind))
 
-    def build(self, leaf_size: int = 4) -> dict:
+    def build(self, leaf_size: int = 2) -> dict:
         opts = L.rt_build_opts(leaf_size, 0)
         info = L.rt_bvh_info()
         self._check(self.lib.rt_build(self.h, C.byref(opts), C.byref(info)))
@@ -124,6 +124,23 @@ class Tracer:
                                        L.RT_PTR_DEVICE if dev else L.RT_PTR_HOST))
         return rgb, rgb8
 
+    def render_views(self, cams, settings: L.rt_settings, rgb=None, want_rgb8: bool = False):
+        """Renders len(cams) views in one launch sequence; returns
+        (rgb (V, H, W, 3), rgb8 or None).  rgb may be a CUDA tensor."""
+        V = len(cams)
+        arr = (L.rt_camera * V)(*cams)
+        W, H = cams[0].width, cams[0].height
+        dev = rgb is not None and L._is_device(rgb)
+        rgb8 = None
+        if not dev:
+            rgb = np.zeros((V, H, W, 3), np.float32) if rgb is None else rgb
+            if want_rgb8:
+                rgb8 = np.zeros((V, H, W, 3), np.uint8)
+        self._check(self.lib.rt_render_views(self.h, arr, V, C.byref(settings), L._ptr(rgb),
+                                             L._ptr(rgb8) if rgb8 is not None else None,
+                                             L.RT_PTR_DEVICE if dev else L.RT_PTR_HOST))
+        return rgb, rgb8
+
     def render_adjoint(self, cam: L.rt_camera, settings: L.rt_settings, d_image, d_color=None):
         dev = L._is_device(d_image)
         M = self.scene.materials.shape[0]
@@ -141,14 +158,20 @@ class Tracer:
 
 
 def make_settings(spp: int, max_depth: int, seed: int = 1, flags: int = 0, sample_offset: int = 0,
-                  sky=(0.0, 0.0, 0.0)) -> L.rt_settings:
+                  sky=(0.0, 0.0, 0.0), sun_dir=(0.0, 1.0, 0.0), sun_radius_deg: float = 0.0,
+                  sun_radiance=(0.0, 0.0, 0.0)) -> L.rt_settings:
     s = L.rt_settings()
     s.max_depth = max_depth
     s.spp = spp
     s.seed = seed
     s.flags = flags
     s.sample_offset = sample_offset
+    sd = np.asarray(sun_dir, np.float64)
+    sd = sd / np.linalg.norm(sd)
     for k in range(3):
         s.sky[k] = sky[k]
+        s.sun_dir[k] = sd[k]
+        s.sun_radiance[k] = sun_radiance[k]
+    s.sun_cos = float(np.cos(np.radians(sun_radius_deg)))
     s.num_views = 1
     return s

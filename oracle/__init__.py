@@ -177,6 +177,26 @@ def render(scene, cam, settings, threads: int = 8):
     return rgb, rgb8
 
 
+def render_views(scene, cams, settings, threads: int = 8):
+    """Returns (rgb (V, H, W, 3) float32, rgb8 uint8) for a list of cameras."""
+    lib = load()
+    P = C.c_void_p
+    lib.or_render_views.argtypes = [P, P, P, P, P, C.c_uint32, C.c_uint32, C.c_uint32, P, C.c_uint32, P, P, P,
+                                    C.c_int]
+    lib.or_render_views.restype = C.c_int
+    arrs, ps = _scene_ptrs(scene)
+    from master_thesis_amd._lib import rt_camera
+    V = len(cams)
+    carr = (rt_camera * V)(*cams)
+    H, W = cams[0].height, cams[0].width
+    rgb = np.zeros((V, H, W, 3), np.float32)
+    rgb8 = np.zeros((V, H, W, 3), np.uint8)
+    rc = lib.or_render_views(*ps, arrs[0].shape[0], arrs[2].shape[0], arrs[4].shape[0], C.addressof(carr), V,
+                             C.addressof(settings), _p(rgb), _p(rgb8), threads)
+    assert rc == 0, rc
+    return rgb, rgb8
+
+
 def render_adjoint(scene, cam, settings, d_image: np.ndarray, threads: int = 8) -> np.ndarray:
     arrs, ps = _scene_ptrs(scene)
     M = arrs[4].shape[0]

@@ -804,7 +804,8 @@ typedef struct {
     const uint32_t* tri_mat;
     const rt_material* mat;
     uint32_t M;
-    const rt_camera* cam;
+    const rt_camera* cams; /* num_views cameras of one size */
+    uint32_t num_views;
     const rt_settings* s;
     or_bvh* bvh;
     double scale;
@@ -819,10 +820,11 @@ typedef struct {
 /* One sample of graphic/path_tracer.cpp:42-97.  Returns 1 and the
  * contribution when the path ends on a light (or escapes to the sky);
  * nv / vm / vc receive the diffuse vertices for the adjoint. */
-static int trace_path(const pt_job* j, uint32_t pix, uint32_t gsmp, v3 dir0, v3* contrib, v3* term, uint32_t* nv,
-                      uint32_t* vm, float* vc) {
+static int trace_path(const pt_job* j, uint32_t view, uint32_t pix, uint32_t gsmp, v3 dir0, v3* contrib, v3* term,
+                      uint32_t* nv, uint32_t* vm, float* vc) {
     const rt_settings* s = j->s;
-    v3 o = V3(j->cam->origin[0], j->cam->origin[1], j->cam->origin[2]);
+    const rt_camera* cam = &j->cams[view];
+    v3 o = V3(cam->origin[0], cam->origin[1], cam->origin[2]);
     v3 d = dir0;
     v3 T = V3(1.0f, 1.0f, 1.0f);
     int cull = 1;
@@ -836,9 +838,14 @@ static int trace_path(const pt_job* j, uint32_t pix, uint32_t gsmp, v3 dir0, v3*
         uint64_t cn = 0, ct = 0;
         best_t h = trace_bvh_one(j->bvh, &r, &cn, &ct);
         if (h.id == RT_MISS) {
-            if (s->flags & RT_RENDER_SKY) {
-                *term = V3(s->sky[0], s->sky[1], s->sky[2]);
-                *contrib = vmul(T, *term);
+            /* extension: sky and sun radiance for escaped rays (the reference's miss is black) */
+            if (s->flags & (RT_RENDER_SKY | RT_RENDER_SUN)) {
+                v3 L = V3(0.0f, 0.0f, 0.0f);
+                if (s->flags & RT_RENDER_SKY) L = V3(s->sky[0], s->sky[1], s->sky[2]);
+                if ((s->flags & RT_RENDER_SUN) && vdot(d, vld(s->sun_dir)) >= s->sun_cos)
+                    L = vadd(L, vld(s->sun_radiance));
+                *term = L;
+                *contrib = vmul(T, L);
                 return 1;
             }
             return 0;
@@ -863,7 +870,7 @@ static int trace_path(const pt_job* j, uint32_t pix, uint32_t gsmp, v3 dir0, v3*
         }
         cull = 0;
         const rt_material* m = &j->mat[j->tri_mat[t]];
-        uint32_t ctr[4] = {pix, gsmp, depth, 0u}, rnd[4];
+        uint32_t ctr[4] = {pix, gsmp, depth, view}, rnd[4];
         or_philox(ctr, key, rnd);
         float xi0 = u01(rnd[0]);
         v3 prev = d;
@@ -888,19 +895,22 @@ static int trace_path(const pt_job* j, uint32_t pix, uint32_t gsmp, v3 dir0, v3*
 
 static void pt_range(void* arg, uint32_t b, uint32_t e, int tid) {
     pt_job* j = (pt_job*)arg;
-    const rt_camera* cam = j->cam;
+    const uint32_t P = j->cams[0].width * j->cams[0].height;
     uint32_t vm[OR_MAX_DEPTH];
     float vc[OR_MAX_DEPTH];
-    for (uint32_t pix = b; pix < e; ++pix) {
+    /* vp = view * P + pixel; images are view-major */
+    for (uint32_t vp = b; vp < e; ++vp) {
+        const uint32_t view = vp / P, pix = vp % P;
+        const rt_camera* cam = &j->cams[view];
         v3 dir = primary(cam, pix % cam->width, pix / cam->width);
         float sum[3] = {0.0f, 0.0f, 0.0f};
         for (uint32_t smp = 0; smp < j->s->spp; ++smp) {
             v3 c, term;
             uint32_t nv;
-            if (!trace_path(j, pix, j->s->sample_offset + smp, dir, &c, &term, &nv, vm, vc)) continue;
+            if (!trace_path(j, view, pix, j->s->sample_offset + smp, dir, &c, &term, &nv, vm, vc)) continue;
             if (j->dimg) {
                 float w[3], tm[3] = {term.x, term.y, term.z};
-                for (int ch = 0; ch < 3; ++ch) w[ch] = j->dimg[3 * (size_t)pix + ch] * (float)j->scale;
+                for (int ch = 0; ch < 3; ++ch) w[ch] = j->dimg[3 * (size_t)vp + ch] * (float)j->scale;
                 for (int ch = 0; ch < 3; ++ch) {
                     if (w[ch] == 0.0f) continue;
                     float f[OR_MAX_DEPTH], suf[OR_MAX_DEPTH + 1], pre = 1.0f;
@@ -922,10 +932,10 @@ static void pt_range(void* arg, uint32_t b, uint32_t e, int tid) {
         if (!j->dimg) {
             for (int ch = 0; ch < 3; ++ch) {
                 float v = (float)((double)sum[ch] * j->scale);
-                j->rgb[3 * (size_t)pix + ch] = v;
+                j->rgb[3 * (size_t)vp + ch] = v;
                 if (j->rgb8) {
                     float cv = fminf(fmaxf(v, 0.0f), 1.0f);
-                    j->rgb8[3 * (size_t)pix + ch] = (uint8_t)(int)(cv * 255.0f);
+                    j->rgb8[3 * (size_t)vp + ch] = (uint8_t)(int)(cv * 255.0f);
                 }
             }
         }
@@ -933,8 +943,8 @@ static void pt_range(void* arg, uint32_t b, uint32_t e, int tid) {
 }
 
 static int pt_run(const float* pos, const float* nrm, const uint32_t* idx, const uint32_t* tri_mat, const void* mat,
-                  uint32_t V, uint32_t T, uint32_t M, const void* cam, const void* settings, float* rgb,
-                  uint8_t* rgb8, const float* dimg, float* dcol, int threads) {
+                  uint32_t V, uint32_t T, uint32_t M, const void* cams, uint32_t num_views, const void* settings,
+                  float* rgb, uint8_t* rgb8, const float* dimg, float* dcol, int threads) {
     (void)V;
     const rt_settings* s = (const rt_settings*)settings;
     if (s->spp == 0 || s->max_depth == 0 || s->max_depth > OR_MAX_DEPTH) return -1;
@@ -943,14 +953,15 @@ static int pt_run(const float* pos, const float* nrm, const uint32_t* idx, const
     pt_job j;
     memset(&j, 0, sizeof(j));
     j.pos = pos; j.nrm = nrm; j.idx = idx; j.tri_mat = tri_mat; j.mat = (const rt_material*)mat; j.M = M;
-    j.cam = (const rt_camera*)cam;
+    j.cams = (const rt_camera*)cams;
+    j.num_views = num_views;
     j.s = s;
     j.bvh = or_bvh_build(pos, idx, T, 4);
     /* finalColor *= 2.0f * M_PI * (1.0f / float(raysPerPixel)), path_tracer.cpp:37 */
     j.scale = 2.0 * 3.14159265358979323846 * (double)(1.0f / (float)s->spp);
     j.rgb = rgb; j.rgb8 = rgb8; j.dimg = dimg;
     if (dimg) j.dcol = (double*)calloc((size_t)threads * 3 * M, sizeof(double));
-    run_parallel(threads, j.cam->width * j.cam->height, 16, pt_range, &j);
+    run_parallel(threads, j.cams[0].width * j.cams[0].height * num_views, 16, pt_range, &j);
     if (dimg) {
         for (uint32_t i = 0; i < 3 * M; ++i) {
             double acc = 0.0;
@@ -966,11 +977,17 @@ static int pt_run(const float* pos, const float* nrm, const uint32_t* idx, const
 int or_render(const float* pos, const float* nrm, const uint32_t* idx, const uint32_t* tri_mat, const void* mat,
               uint32_t V, uint32_t T, uint32_t M, const void* cam, const void* settings, float* rgb, uint8_t* rgb8,
               int threads) {
-    return pt_run(pos, nrm, idx, tri_mat, mat, V, T, M, cam, settings, rgb, rgb8, NULL, NULL, threads);
+    return pt_run(pos, nrm, idx, tri_mat, mat, V, T, M, cam, 1, settings, rgb, rgb8, NULL, NULL, threads);
+}
+
+int or_render_views(const float* pos, const float* nrm, const uint32_t* idx, const uint32_t* tri_mat, const void* mat,
+                    uint32_t V, uint32_t T, uint32_t M, const void* cams, uint32_t num_views, const void* settings,
+                    float* rgb, uint8_t* rgb8, int threads) {
+    return pt_run(pos, nrm, idx, tri_mat, mat, V, T, M, cams, num_views, settings, rgb, rgb8, NULL, NULL, threads);
 }
 
 int or_render_adjoint(const float* pos, const float* nrm, const uint32_t* idx, const uint32_t* tri_mat,
                       const void* mat, uint32_t V, uint32_t T, uint32_t M, const void* cam, const void* settings,
                       const float* d_image, float* d_color, int threads) {
-    return pt_run(pos, nrm, idx, tri_mat, mat, V, T, M, cam, settings, NULL, NULL, d_image, d_color, threads);
+    return pt_run(pos, nrm, idx, tri_mat, mat, V, T, M, cam, 1, settings, NULL, NULL, d_image, d_color, threads);
 }

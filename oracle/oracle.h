@@ -83,6 +83,11 @@ void or_primary_dir(const float view_inv[16], const float proj_inv[16], uint32_t
 int or_render(const float* pos, const float* nrm, const uint32_t* idx, const uint32_t* tri_mat, const void* mat,
               uint32_t V, uint32_t T, uint32_t M, const void* cam, const void* settings, float* rgb, uint8_t* rgb8,
               int threads);
+/* The same for num_views cameras (rt_camera[num_views], one size); view v
+ * draws with Philox counter (pixel, sample, depth, v); images view-major. */
+int or_render_views(const float* pos, const float* nrm, const uint32_t* idx, const uint32_t* tri_mat, const void* mat,
+                    uint32_t V, uint32_t T, uint32_t M, const void* cams, uint32_t num_views, const void* settings,
+                    float* rgb, uint8_t* rgb8, int threads);
 /* Path-replay gradient of sum(d_image * image) with respect to every
  * material colour (3*M doubles accumulated, returned as float). */
 int or_render_adjoint(const float* pos, const float* nrm, const uint32_t* idx, const uint32_t* tri_mat,

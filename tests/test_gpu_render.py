@@ -140,3 +140,23 @@ def test_sw_render_cli_matches_library(tracer, tmp_path):
     _, rgb8 = tracer.render(cam, s, want_rgb8=True)
     assert np.array_equal(img, rgb8)
     assert img.mean() > 5
+
+
+def test_multi_view_with_sun_matches_oracle(tracer):
+    """Four views in one launch sequence, with sky and sun (extension):
+    every view bit-identical to the oracle; view 0 equals rt_render."""
+    from master_thesis_amd._lib import RT_RENDER_SUN
+    sc = scenes.cornell_box()
+    _setup(tracer, sc)
+    pos = scenes.cornell_camera_position()
+    cams = [H.camera(pos, 1.0, 0.0, math.radians(40), 24, 20), H.camera((2.0, 1.0, 1.5), 1.1, 0.2, 1.0, 24, 20),
+            H.camera((-1.0, 0.0, 0.0), 0.5, -0.3, 1.2, 24, 20), H.camera((0.0, 2.0, 0.0), 0.0, -0.9, 0.9, 24, 20)]
+    s = make_settings(spp=4, max_depth=4, seed=17, flags=RT_RENDER_DETERMINISTIC | RT_RENDER_SKY | RT_RENDER_SUN,
+                      sky=(0.1, 0.2, 0.3), sun_dir=(1.0, 0.3, 0.2), sun_radius_deg=30.0, sun_radiance=(5.0, 4.0, 3.0))
+    rgb, rgb8 = tracer.render_views(cams, s, want_rgb8=True)
+    want, want8 = O.render_views(sc, cams, s)
+    assert np.array_equal(rgb.view(np.uint32), want.view(np.uint32))
+    assert np.array_equal(rgb8, want8)
+    one, _ = tracer.render(cams[0], s)
+    assert np.array_equal(one, rgb[0])
+    assert not np.array_equal(rgb[0], rgb[1])
