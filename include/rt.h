@@ -240,6 +240,39 @@ int  rt_render_views(rt_ctx* ctx, const rt_camera* cams, uint32_t num_views, con
 int  rt_render_adjoint(rt_ctx* ctx, const rt_camera* cam, const rt_settings* s,
                        const float* d_image, float* d_color, int ptr_kind);
 
+/* ---- learned radiance cache ---------------------------------------------
+ * No counterpart in the software renderer: it stands in for the Vulkan
+ * renderer's DDGI indirect term (res/shader/majercik2019_final_raygen.rgen:
+ * 43-91,124).  A sample is 9 floats: hit position, shading normal, incoming
+ * ray direction; the cache predicts the RGB expectation of the rest of the
+ * path's contribution from that vertex.  Network: 78 encoded inputs (12
+ * frequencies x sin/cos x 3 axes of the position in the normalisation box,
+ * normal, direction) -> 5 hidden layers of 64 (ReLU) -> 3, BF16 MFMA with
+ * FP32 accumulation, relative L2 loss, Adam. */
+typedef struct rt_nrc rt_nrc;
+typedef struct rt_nrc_config {
+    float lr, beta1, beta2, eps;   /* Adam; 0 selects 1e-2, 0.9, 0.99, 1e-8        */
+    uint64_t seed;                 /* weight initialisation                       */
+    float bounds_lo[3];            /* position normalisation box; lo == hi: the   */
+    float bounds_hi[3];            /* uploaded scene's bounds                      */
+} rt_nrc_config;
+
+#define RT_NRC_FEATURES 9          /* floats per sample: position, normal, direction */
+
+int  rt_nrc_create(rt_ctx* ctx, const rt_nrc_config* cfg, rt_nrc** out);
+void rt_nrc_destroy(rt_nrc* nrc);
+/* Gets (set = 0) or sets (set = 1, resets Adam) the FP32 parameters; *count
+ * is the in-out capacity (params NULL: query the count). */
+int  rt_nrc_params(rt_nrc* nrc, float* params, uint32_t* count, int set);
+/* Predictions for n samples: feats n*9, out n*3. */
+int  rt_nrc_infer(rt_nrc* nrc, const float* feats, uint32_t n, float* out, int ptr_kind);
+/* Loss gradient for n samples with targets n*3, without an update: grad is
+ * host memory of the parameter count; loss (may be NULL) the mean loss. */
+int  rt_nrc_gradient(rt_nrc* nrc, const float* feats, const float* targets, uint32_t n, float* grad,
+                     float* loss, int ptr_kind);
+/* One Adam step on n samples; loss (may be NULL) receives the mean loss. */
+int  rt_nrc_train(rt_nrc* nrc, const float* feats, const float* targets, uint32_t n, float* loss, int ptr_kind);
+
 #ifdef __cplusplus
 }
 #endif

@@ -122,6 +122,13 @@ assert C.sizeof(rt_material) == 32 and MATERIAL_DTYPE.itemsize == 32
 assert C.sizeof(rt_bvh4_node) == 128 and NODE_DTYPE.itemsize == 128
 assert C.sizeof(rt_tri_rec) == 48 and TRI_DTYPE.itemsize == 48
 
+class rt_nrc_config(C.Structure):
+    _fields_ = [("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
+                ("seed", C.c_uint64), ("bounds_lo", C.c_float * 3), ("bounds_hi", C.c_float * 3)]
+
+
+RT_NRC_FEATURES = 9
+
 # every exported symbol of include/rt.h with its signature
 _P = C.c_void_p
 SIGNATURES = {
@@ -138,6 +145,12 @@ SIGNATURES = {
     "rt_render": (C.c_int, [_P, C.POINTER(rt_camera), C.POINTER(rt_settings), _P, _P, C.c_int]),
     "rt_render_views": (C.c_int, [_P, C.POINTER(rt_camera), C.c_uint32, C.POINTER(rt_settings), _P, _P, C.c_int]),
     "rt_render_adjoint": (C.c_int, [_P, C.POINTER(rt_camera), C.POINTER(rt_settings), _P, _P, C.c_int]),
+    "rt_nrc_create": (C.c_int, [_P, C.POINTER(rt_nrc_config), C.POINTER(_P)]),
+    "rt_nrc_destroy": (None, [_P]),
+    "rt_nrc_params": (C.c_int, [_P, _P, C.POINTER(C.c_uint32), C.c_int]),
+    "rt_nrc_infer": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_int]),
+    "rt_nrc_gradient": (C.c_int, [_P, _P, _P, C.c_uint32, _P, C.POINTER(C.c_float), C.c_int]),
+    "rt_nrc_train": (C.c_int, [_P, _P, _P, C.c_uint32, C.POINTER(C.c_float), C.c_int]),
 }
 
 _lib = None

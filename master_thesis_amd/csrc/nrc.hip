@@ -1,0 +1,562 @@
+// nrc.hip — learned radiance cache: a small MLP on BF16 MFMA with FP32
+// accumulation, trained online (Mueller, Rousselle, Novak, Keller, "Real-time
+// neural radiance caching for path tracing", SIGGRAPH 2021).  It stands in
+// for the reference renderer's DDGI indirect term (res/shader/
+// majercik2019_final_raygen.rgen:43-91,124), which has no software-renderer
+// counterpart.
+//
+// Network: input = frequency encoding of the position normalised to the
+// scene box (12 frequencies x sin/cos x 3 axes = 72) + normal (3) + incoming
+// direction (3) = 78, padded to 96; layers 96->64, 4 x (64->64), 64->16 (3
+// used); ReLU on the hidden layers, linear output.  Loss: relative L2,
+// (y - t)^2 / (y_sg^2 + 0.01), averaged over samples (y_sg = y without
+// gradient).  Optimiser: Adam on FP32 master weights; the BF16 copy the
+// kernels read is refreshed after every step.
+//
+// Kernels (one workgroup = 4 waves = one tile of 64 samples):
+//   k_mlp_infer  weights -> LDS once, then tiles: encode, 6 layers
+//   k_mlp_train  one tile per workgroup: encode, forward (all activations
+//                kept in LDS), loss, backward; each workgroup writes its
+//                partial gradient (no atomics)
+//   k_adam       sums the partials in workgroup order (deterministic), Adam
+//                update, BF16 refresh
+// Every GEMM is a sum of 16x16 output tiles of v_mfma_f32_16x16x32_bf16.
+#include <math.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "rt_internal.hpp"
+#include "rt_core.hpp"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+namespace rt {
+namespace nrc {
+
+constexpr int L = 6;                                   // layers
+constexpr int KD[L] = {96, 64, 64, 64, 64, 64};        // input width (padded)
+constexpr int ND[L] = {64, 64, 64, 64, 64, 16};        // output width (padded)
+constexpr int K0_VALID = 78, N5_VALID = 3;
+constexpr int FREQS = 12;
+constexpr int TILE = 64;                               // samples per workgroup tile
+constexpr int NT = 256;                                // threads per workgroup
+
+// fp32 parameter layout: [W0 b0 W1 b1 ... W5 b5], W_l row-major [n][k]
+constexpr int offw(int l) { int o = 0; for (int i = 0; i < l; ++i) o += ND[i] * KD[i] + ND[i]; return o; }
+constexpr int offb(int l) { return offw(l) + ND[l] * KD[l]; }
+constexpr int NPARAM = offw(L);
+// bf16 weight layout: [W0 W1 ... W5]
+constexpr int offwb(int l) { int o = 0; for (int i = 0; i < l; ++i) o += ND[i] * KD[i]; return o; }
+constexpr int NWB = offwb(L);
+static_assert(NPARAM == 23888, "parameter count");
+static_assert(NWB == 23552, "bf16 weight count");
+
+struct Geo {
+    float lo[3], inv[3];
+};
+
+__device__ __forceinline__ __bf16 tobf(float x) { return (__bf16)x; }
+
+// Features of one sample, group g of four: g < 3 -> the 24 sin/cos features of
+// axis g; g = 3 -> normal, direction, zero padding (24 values).
+__device__ __forceinline__ void encode_group(const float* f9, int g, const Geo& G, __bf16* row) {
+    if (g < 3) {
+        const float x = (f9[g] - G.lo[g]) * G.inv[g];
+#pragma unroll
+        for (int k = 0; k < FREQS; ++k) {
+            const float a = ldexpf(x, k);
+            row[g * 24 + 2 * k] = tobf(sinpif(a));
+            row[g * 24 + 2 * k + 1] = tobf(cospif(a));
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 6; ++j) row[72 + j] = tobf(f9[3 + j]);
+#pragma unroll
+        for (int j = 78; j < 96; ++j) row[j] = tobf(0.0f);
+    }
+}
+
+// 16x16 tile of C = A * B over K (multiple of 32; terms k >= kvalid are zero).
+// A(m, k) = A[m * sAm + k * sAk], B(k, n) = B[k * sBk + n * sBn].
+// Lane l supplies A row m0 + l%16 and B column n0 + l%16, k = 8*(l/16) + j;
+// on return acc[i] = C(m0 + 4*(l/16) + i, n0 + l%16).
+__device__ __forceinline__ f32x4 tile_mm(const __bf16* A, int sAm, int sAk, const __bf16* B, int sBk, int sBn,
+                                         int m0, int n0, int K, int kvalid, int lane) {
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    const int mi = lane & 15, kg = lane >> 4;
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        bf16x8 a, b;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int k = k0 + kg * 8 + j;
+            if (k < kvalid) {
+                a[j] = A[(m0 + mi) * sAm + k * sAk];
+                b[j] = B[k * sBk + (n0 + mi) * sBn];
+            } else {
+                a[j] = tobf(0.0f);
+                b[j] = tobf(0.0f);
+            }
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+__device__ __forceinline__ void load_weights(const __bf16* __restrict__ Wb, __bf16* sW) {
+    const uint4* src = (const uint4*)Wb;
+    uint4* dst = (uint4*)sW;
+    for (int i = threadIdx.x; i < NWB / 8; i += NT) dst[i] = src[i];
+}
+
+__device__ __forceinline__ void encode_tile(const float* __restrict__ feats, uint32_t s0, uint32_t n, const Geo& G,
+                                            __bf16* sX) {
+    const int s = threadIdx.x >> 2, g = threadIdx.x & 3;
+    float f9[9];
+    const bool valid = s0 + s < n;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) f9[j] = valid ? feats[(size_t)(s0 + s) * 9 + j] : 0.0f;
+    if (valid) {
+        encode_group(f9, g, G, sX + s * 96);
+    } else {
+        for (int j = 0; j < 24; ++j) sX[s * 96 + g * 24 + j] = tobf(0.0f);
+    }
+}
+
+// Forward through the hidden layers: layer l reads `in` (width KD[l]) and
+// writes bf16 ReLU activations to out(l).
+template <typename OutFn>
+__device__ __forceinline__ void hidden_forward(const __bf16* sW, const float* __restrict__ Wf, const __bf16* sX,
+                                               OutFn out) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int mi = lane & 15, kg = lane >> 4;
+    const __bf16* in = sX;
+    for (int l = 0; l < L - 1; ++l) {
+        __bf16* o = out(l);
+        const int K = KD[l];
+        for (int nt = 0; nt < ND[l] / 16; ++nt) {
+            f32x4 acc = tile_mm(in, K, 1, sW + offwb(l), 1, K, 16 * w, 16 * nt, K, K, lane);
+            const int col = 16 * nt + mi;
+            const float b = Wf[offb(l) + col];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int row = 16 * w + 4 * kg + i;
+                o[row * 64 + col] = tobf(fmaxf(acc[i] + b, 0.0f));
+            }
+        }
+        __syncthreads();
+        in = o;
+    }
+}
+
+__global__ void __launch_bounds__(NT) k_mlp_infer(const float* __restrict__ feats, uint32_t n,
+                                                  const __bf16* __restrict__ Wb, const float* __restrict__ Wf, Geo G,
+                                                  float* __restrict__ out) {
+    __shared__ __attribute__((aligned(16))) __bf16 sW[NWB];
+    __shared__ __attribute__((aligned(16))) __bf16 sX[TILE * 96];
+    __shared__ __attribute__((aligned(16))) __bf16 sH[2][TILE * 64];
+    load_weights(Wb, sW);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int mi = lane & 15, kg = lane >> 4;
+    const uint32_t tiles = (n + TILE - 1) / TILE;
+    for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const uint32_t s0 = t * TILE;
+        __syncthreads();  // previous tile done with sX / sH (and weights loaded)
+        encode_tile(feats, s0, n, G, sX);
+        __syncthreads();
+        hidden_forward(sW, Wf, sX, [&](int l) { return sH[l & 1]; });
+        // output layer (reads the last hidden layer, sH[(L - 2) & 1]); each wave its 16 rows
+        f32x4 acc = tile_mm(sH[(L - 2) & 1], 64, 1, sW + offwb(L - 1), 1, 64, 16 * w, 0, 64, 64, lane);
+        const int col = mi;
+        if (col < N5_VALID) {
+            const float b = Wf[offb(L - 1) + col];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const uint32_t s = s0 + 16 * w + 4 * kg + i;
+                if (s < n) out[(size_t)s * 3 + col] = acc[i] + b;
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(NT) k_mlp_train(const float* __restrict__ feats, const float* __restrict__ targets,
+                                                  uint32_t n, const __bf16* __restrict__ Wb,
+                                                  const float* __restrict__ Wf, Geo G, float* __restrict__ partial,
+                                                  float* __restrict__ loss_part) {
+    __shared__ __attribute__((aligned(16))) __bf16 sW[NWB];
+    __shared__ __attribute__((aligned(16))) __bf16 sX[TILE * 96];
+    __shared__ __attribute__((aligned(16))) __bf16 sH[L - 1][TILE * 64];
+    __shared__ __attribute__((aligned(16))) __bf16 sD[2][TILE * 64];
+    __shared__ float sLoss[NT / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int mi = lane & 15, kg = lane >> 4;
+    const uint32_t s0 = blockIdx.x * TILE;
+    float* part = partial + (size_t)blockIdx.x * NPARAM;
+    load_weights(Wb, sW);
+    encode_tile(feats, s0, n, G, sX);
+    __syncthreads();
+    hidden_forward(sW, Wf, sX, [&](int l) { return sH[l]; });
+
+    // output layer and loss: y = H4 W5^T + b5; dL/dy = 2 (y - t) / (y^2 + 0.01) / n
+    float lsum = 0.0f;
+    {
+        f32x4 acc = tile_mm(sH[L - 2], 64, 1, sW + offwb(L - 1), 1, 64, 16 * w, 0, 64, 64, lane);
+        const int col = mi;
+        const float b = Wf[offb(L - 1) + col];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int row = 16 * w + 4 * kg + i;
+            const uint32_t s = s0 + row;
+            float dy = 0.0f;
+            if (col < N5_VALID && s < n) {
+                const float y = acc[i] + b;
+                const float t = targets[(size_t)s * 3 + col];
+                const float den = y * y + 0.01f;
+                const float r = y - t;
+                lsum += r * r / den;
+                dy = 2.0f * r / den / (float)n;
+            }
+            sD[0][row * 16 + col] = tobf(dy);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
+    if (lane == 0) sLoss[w] = lsum;
+    __syncthreads();
+    if (threadIdx.x == 0) loss_part[blockIdx.x] = sLoss[0] + sLoss[1] + sLoss[2] + sLoss[3];
+
+    // layer 5: dW5[n][k] = sum_s dY[s][n] H4[s][k] (16 x 64), one 16-column tile per wave
+    {
+        f32x4 acc = tile_mm(sD[0], 1, 16, sH[L - 2], 64, 1, 0, 16 * w, 64, 64, lane);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) part[offw(L - 1) + (4 * kg + i) * 64 + 16 * w + mi] = acc[i];
+    }
+    if (threadIdx.x < 16) {
+        float s = 0.0f;
+        for (int r = 0; r < TILE; ++r) s += (float)sD[0][r * 16 + threadIdx.x];
+        part[offb(L - 1) + threadIdx.x] = s;
+    }
+    // dH4[s][k] = sum_n dY[s][n] W5[n][k] (K = 16 valid of 32), masked by H4 > 0 -> sD[1]
+    for (int nt = 0; nt < 4; ++nt) {
+        f32x4 acc = tile_mm(sD[0], 16, 1, sW + offwb(L - 1), 64, 1, 16 * w, 16 * nt, 32, 16, lane);
+        const int col = 16 * nt + mi;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int row = 16 * w + 4 * kg + i;
+            const float h = (float)sH[L - 2][row * 64 + col];
+            sD[1][row * 64 + col] = tobf(h > 0.0f ? acc[i] : 0.0f);
+        }
+    }
+    __syncthreads();
+
+    int cur = 1;
+    for (int l = L - 2; l >= 0; --l) {
+        const __bf16* dZ = sD[cur];
+        const __bf16* in = (l == 0) ? sX : sH[l - 1];
+        const int K = KD[l];
+        // dW_l[n][k] = sum_s dZ[s][n] in[s][k]: rows n in 16-row tiles (one per wave), K / 16 column tiles
+        for (int kt = 0; kt < K / 16; ++kt) {
+            f32x4 acc = tile_mm(dZ, 1, 64, in, K, 1, 16 * w, 16 * kt, 64, 64, lane);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) part[offw(l) + (16 * w + 4 * kg + i) * K + 16 * kt + mi] = acc[i];
+        }
+        if (threadIdx.x < 64) {
+            float s = 0.0f;
+            for (int r = 0; r < TILE; ++r) s += (float)dZ[r * 64 + threadIdx.x];
+            part[offb(l) + threadIdx.x] = s;
+        }
+        if (l > 0) {
+            // dH_{l-1}[s][k] = sum_n dZ[s][n] W_l[n][k], masked by H_{l-1} > 0
+            __bf16* dst = sD[cur ^ 1];
+            for (int nt = 0; nt < 4; ++nt) {
+                f32x4 acc = tile_mm(dZ, 64, 1, sW + offwb(l), 64, 1, 16 * w, 16 * nt, 64, 64, lane);
+                const int col = 16 * nt + mi;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int row = 16 * w + 4 * kg + i;
+                    const float h = (float)sH[l - 1][row * 64 + col];
+                    dst[row * 64 + col] = tobf(h > 0.0f ? acc[i] : 0.0f);
+                }
+            }
+            __syncthreads();
+            cur ^= 1;
+        }
+    }
+}
+
+// Adam with bias correction on the summed partial gradients (workgroup order,
+// so the sum is deterministic); refreshes the bf16 weight copy.  With lr = 0
+// it only writes the summed gradient (gradient queries).
+__global__ void k_adam(const float* __restrict__ partial, uint32_t nparts, float* __restrict__ grad,
+                       float* __restrict__ Wf, float* __restrict__ m, float* __restrict__ v, __bf16* __restrict__ Wb,
+                       float lr, float b1, float b2, float eps, float bc1, float bc2, int update) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= NPARAM) return;
+    float g = 0.0f;
+    for (uint32_t w = 0; w < nparts; ++w) g += partial[(size_t)w * NPARAM + p];
+    grad[p] = g;
+    if (!update) return;
+    const float mm = b1 * m[p] + (1.0f - b1) * g;
+    const float vv = b2 * v[p] + (1.0f - b2) * g * g;
+    m[p] = mm;
+    v[p] = vv;
+    const float wnew = Wf[p] - lr * (mm / bc1) / (sqrtf(vv / bc2) + eps);
+    Wf[p] = wnew;
+    for (int l = 0; l < L; ++l) {
+        const int o = offw(l), nw = ND[l] * KD[l];
+        if (p >= o && p < o + nw) Wb[offwb(l) + (p - o)] = tobf(wnew);
+    }
+}
+
+__global__ void k_refresh_bf16(const float* __restrict__ Wf, __bf16* __restrict__ Wb) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= NPARAM) return;
+    for (int l = 0; l < L; ++l) {
+        const int o = offw(l), nw = ND[l] * KD[l];
+        if (p >= o && p < o + nw) Wb[offwb(l) + (p - o)] = tobf(Wf[p]);
+    }
+}
+
+}  // namespace nrc
+}  // namespace rt
+
+// ---------------------------------------------------------------- C ABI
+using namespace rt;
+using namespace rt::nrc;
+
+struct rt_nrc {
+    rt_ctx* ctx = nullptr;
+    rt_nrc_config cfg{};
+    Geo geo{};
+    DevBuf wf, wb, m, v, grad, partial, loss, feats, targets, out;
+    uint32_t steps = 0;
+    int infer_grid = 0;
+};
+
+namespace {
+
+int nrc_fail(rt_nrc* s, int code, const std::string& msg) { return fail(s->ctx, code, msg); }
+
+uint64_t splitmix(uint64_t& x) {
+    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+int stage(rt_nrc* s, DevBuf& b, const void* src, size_t bytes, int ptr_kind, const void** out) {
+    if (ptr_kind == RT_PTR_DEVICE) {
+        *out = src;
+        return RT_OK;
+    }
+    int rc = ensure(s->ctx, b, bytes);
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(b.ptr, src, bytes, hipMemcpyHostToDevice, s->ctx->stream);
+    if (e != hipSuccess) return hip_fail(s->ctx, e, "hipMemcpyAsync");
+    *out = b.ptr;
+    return RT_OK;
+}
+
+// one gradient evaluation over n samples (and an Adam step when update)
+int run_train(rt_nrc* s, const float* feats, const float* targets, uint32_t n, int ptr_kind, float* loss, float* grad_out,
+              bool update) {
+    rt_ctx* c = s->ctx;
+    if (n == 0) return nrc_fail(s, RT_E_INVALID, "rt_nrc: no samples");
+    if (!feats || !targets) return nrc_fail(s, RT_E_INVALID, "rt_nrc: NULL samples");
+    const void *df, *dt;
+    int rc;
+    if ((rc = stage(s, s->feats, feats, (size_t)n * 36, ptr_kind, &df)) ||
+        (rc = stage(s, s->targets, targets, (size_t)n * 12, ptr_kind, &dt)))
+        return rc;
+    const uint32_t wgs = (n + TILE - 1) / TILE;
+    if ((rc = ensure(c, s->partial, (size_t)wgs * NPARAM * 4)) || (rc = ensure(c, s->loss, (size_t)wgs * 4))) return rc;
+    RT_HIP(c, hipEventRecord(c->ev[2], c->stream));
+    hipLaunchKernelGGL(k_mlp_train, dim3(wgs), dim3(NT), 0, c->stream, (const float*)df, (const float*)dt, n,
+                       (const __bf16*)s->wb.ptr, (const float*)s->wf.ptr, s->geo, (float*)s->partial.ptr,
+                       (float*)s->loss.ptr);
+    RT_HIP(c, hipGetLastError());
+    RT_HIP(c, hipEventRecord(c->ev[3], c->stream));
+    float bc1 = 1.0f, bc2 = 1.0f;
+    if (update) {
+        s->steps += 1;
+        bc1 = 1.0f - powf(s->cfg.beta1, (float)s->steps);
+        bc2 = 1.0f - powf(s->cfg.beta2, (float)s->steps);
+    }
+    hipLaunchKernelGGL(k_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, c->stream, (const float*)s->partial.ptr, wgs,
+                       (float*)s->grad.ptr, (float*)s->wf.ptr, (float*)s->m.ptr, (float*)s->v.ptr,
+                       (__bf16*)s->wb.ptr, s->cfg.lr, s->cfg.beta1, s->cfg.beta2, s->cfg.eps, bc1, bc2,
+                       update ? 1 : 0);
+    RT_HIP(c, hipGetLastError());
+    c->last_op = 3;
+    if (grad_out) RT_HIP(c, hipMemcpyAsync(grad_out, s->grad.ptr, NPARAM * 4, hipMemcpyDeviceToHost, c->stream));
+    if (loss) {
+        std::vector<float> lp(wgs);
+        RT_HIP(c, hipMemcpyAsync(lp.data(), s->loss.ptr, (size_t)wgs * 4, hipMemcpyDeviceToHost, c->stream));
+        RT_HIP(c, hipStreamSynchronize(c->stream));
+        double acc = 0.0;
+        for (float x : lp) acc += x;
+        *loss = (float)(acc / ((double)n * N5_VALID));
+    } else if (grad_out || ptr_kind == RT_PTR_HOST) {
+        RT_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    return RT_OK;
+}
+
+}  // namespace
+
+namespace rt {
+// used by the path tracer's cache mode (pathtrace.hip)
+int nrc_infer_device(rt_nrc* s, const float* d_feats, uint32_t n, float* d_out) {
+    rt_ctx* c = s->ctx;
+    if (n == 0) return RT_OK;
+    const uint32_t tiles = (n + TILE - 1) / TILE;
+    const uint32_t grid = tiles < (uint32_t)s->infer_grid ? tiles : (uint32_t)s->infer_grid;
+    hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(NT), 0, c->stream, d_feats, n, (const __bf16*)s->wb.ptr,
+                       (const float*)s->wf.ptr, s->geo, d_out);
+    RT_HIP(c, hipGetLastError());
+    return RT_OK;
+}
+int nrc_train_device(rt_nrc* s, const float* d_feats, const float* d_targets, uint32_t n) {
+    return run_train(s, d_feats, d_targets, n, RT_PTR_DEVICE, nullptr, nullptr, true);
+}
+}  // namespace rt
+
+extern "C" {
+
+int rt_nrc_create(rt_ctx* c, const rt_nrc_config* cfg, rt_nrc** out) {
+    if (!c) return RT_E_INVALID;
+    if (!out) return fail(c, RT_E_INVALID, "rt_nrc_create: out is NULL");
+    *out = nullptr;
+    rt_nrc* s = new (std::nothrow) rt_nrc();
+    if (!s) return fail(c, RT_E_NOMEM, "rt_nrc_create: host allocation failed");
+    s->ctx = c;
+    if (cfg) s->cfg = *cfg;
+    if (s->cfg.lr == 0.0f) s->cfg.lr = 1e-2f;
+    if (s->cfg.beta1 == 0.0f) s->cfg.beta1 = 0.9f;
+    if (s->cfg.beta2 == 0.0f) s->cfg.beta2 = 0.99f;
+    if (s->cfg.eps == 0.0f) s->cfg.eps = 1e-8f;
+    float lo[3], hi[3];
+    bool given = false;
+    for (int k = 0; k < 3; ++k) given |= s->cfg.bounds_lo[k] != s->cfg.bounds_hi[k];
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = given ? s->cfg.bounds_lo[k] : c->scene_lo[k];
+        hi[k] = given ? s->cfg.bounds_hi[k] : c->scene_hi[k];
+        const float ext = hi[k] - lo[k];
+        s->geo.lo[k] = lo[k];
+        s->geo.inv[k] = ext > 0.0f ? 1.0f / ext : 0.0f;
+    }
+    (void)hipSetDevice(c->device);
+    int rc;
+    if ((rc = ensure(c, s->wf, NPARAM * 4)) || (rc = ensure(c, s->wb, NWB * 2)) || (rc = ensure(c, s->m, NPARAM * 4)) ||
+        (rc = ensure(c, s->v, NPARAM * 4)) || (rc = ensure(c, s->grad, NPARAM * 4))) {
+        rt_nrc_destroy(s);
+        return rc;
+    }
+    // He-uniform weights on the valid inputs / outputs, zero biases and padding
+    std::vector<float> w(NPARAM, 0.0f);
+    uint64_t st = s->cfg.seed ^ 0x5EEDull;
+    for (int l = 0; l < L; ++l) {
+        const int kv = (l == 0) ? K0_VALID : KD[l];
+        const int nv = (l == L - 1) ? N5_VALID : ND[l];
+        const float a = sqrtf(6.0f / (float)kv);
+        for (int nn = 0; nn < nv; ++nn)
+            for (int k = 0; k < kv; ++k) {
+                const double u = (double)(splitmix(st) >> 11) * (1.0 / 9007199254740992.0);
+                w[offw(l) + nn * KD[l] + k] = (float)((2.0 * u - 1.0) * a);
+            }
+    }
+    hipError_t e = hipMemcpy(s->wf.ptr, w.data(), NPARAM * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(s->m.ptr, 0, NPARAM * 4);
+    if (e == hipSuccess) e = hipMemset(s->v.ptr, 0, NPARAM * 4);
+    if (e != hipSuccess) {
+        rt_nrc_destroy(s);
+        return hip_fail(c, e, "rt_nrc_create");
+    }
+    hipLaunchKernelGGL(k_refresh_bf16, dim3((NPARAM + 255) / 256), dim3(256), 0, c->stream, (const float*)s->wf.ptr,
+                       (__bf16*)s->wb.ptr);
+    int per_cu = 0;
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mlp_infer, NT, 0);
+    s->infer_grid = c->num_cus * (per_cu > 0 ? per_cu : 1);
+    e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        rt_nrc_destroy(s);
+        return hip_fail(c, e, "rt_nrc_create");
+    }
+    *out = s;
+    return RT_OK;
+}
+
+void rt_nrc_destroy(rt_nrc* s) {
+    if (!s) return;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    DevBuf* all[] = {&s->wf, &s->wb, &s->m, &s->v, &s->grad, &s->partial, &s->loss, &s->feats, &s->targets, &s->out};
+    for (DevBuf* b : all)
+        if (b->ptr) (void)hipFree(b->ptr);
+    delete s;
+}
+
+int rt_nrc_params(rt_nrc* s, float* params, uint32_t* count, int set) {
+    if (!s || !count) return RT_E_INVALID;
+    const uint32_t cap = *count;
+    *count = NPARAM;
+    if (!params) return RT_OK;
+    if (cap < NPARAM) return nrc_fail(s, RT_E_INVALID, "rt_nrc_params: capacity below the parameter count");
+    rt_ctx* c = s->ctx;
+    (void)hipSetDevice(c->device);
+    if (set) {
+        RT_HIP(c, hipMemcpyAsync(s->wf.ptr, params, NPARAM * 4, hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_refresh_bf16, dim3((NPARAM + 255) / 256), dim3(256), 0, c->stream, (const float*)s->wf.ptr,
+                           (__bf16*)s->wb.ptr);
+        RT_HIP(c, hipGetLastError());
+        RT_HIP(c, hipMemsetAsync(s->m.ptr, 0, NPARAM * 4, c->stream));
+        RT_HIP(c, hipMemsetAsync(s->v.ptr, 0, NPARAM * 4, c->stream));
+        s->steps = 0;
+    } else {
+        RT_HIP(c, hipMemcpyAsync(params, s->wf.ptr, NPARAM * 4, hipMemcpyDeviceToHost, c->stream));
+    }
+    RT_HIP(c, hipStreamSynchronize(c->stream));
+    return RT_OK;
+}
+
+int rt_nrc_infer(rt_nrc* s, const float* feats, uint32_t n, float* out, int ptr_kind) {
+    if (!s) return RT_E_INVALID;
+    if (n > 0 && (!feats || !out)) return nrc_fail(s, RT_E_INVALID, "rt_nrc_infer: NULL argument");
+    if (ptr_kind != RT_PTR_HOST && ptr_kind != RT_PTR_DEVICE) return nrc_fail(s, RT_E_INVALID, "rt_nrc_infer: bad ptr_kind");
+    if (n == 0) return RT_OK;
+    rt_ctx* c = s->ctx;
+    (void)hipSetDevice(c->device);
+    const void* df;
+    int rc;
+    if ((rc = stage(s, s->feats, feats, (size_t)n * 36, ptr_kind, &df))) return rc;
+    float* dout = out;
+    if (ptr_kind == RT_PTR_HOST) {
+        if ((rc = ensure(c, s->out, (size_t)n * 12))) return rc;
+        dout = (float*)s->out.ptr;
+    }
+    RT_HIP(c, hipEventRecord(c->ev[2], c->stream));
+    if ((rc = nrc_infer_device(s, (const float*)df, n, dout))) return rc;
+    RT_HIP(c, hipEventRecord(c->ev[3], c->stream));
+    c->last_op = 3;
+    if (ptr_kind == RT_PTR_HOST) {
+        RT_HIP(c, hipMemcpyAsync(out, dout, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream));
+        RT_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    return RT_OK;
+}
+
+int rt_nrc_gradient(rt_nrc* s, const float* feats, const float* targets, uint32_t n, float* grad, float* loss,
+                    int ptr_kind) {
+    if (!s) return RT_E_INVALID;
+    if (!grad) return nrc_fail(s, RT_E_INVALID, "rt_nrc_gradient: NULL gradient output");
+    (void)hipSetDevice(s->ctx->device);
+    return run_train(s, feats, targets, n, ptr_kind, loss, grad, false);
+}
+
+int rt_nrc_train(rt_nrc* s, const float* feats, const float* targets, uint32_t n, float* loss, int ptr_kind) {
+    if (!s) return RT_E_INVALID;
+    (void)hipSetDevice(s->ctx->device);
+    return run_train(s, feats, targets, n, ptr_kind, loss, nullptr, true);
+}
+
+}  // extern "C"

@@ -88,6 +88,10 @@ int launch_trace(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, bo
                  const uint32_t* n_dev = nullptr);
 
 // pathtrace.hip
+// nrc.hip: cache inference / one training step on device-resident samples
+int nrc_infer_device(rt_nrc* s, const float* d_feats, uint32_t n, float* d_out);
+int nrc_train_device(rt_nrc* s, const float* d_feats, const float* d_targets, uint32_t n);
+
 // cams: V host cameras of one size; d_rgb / d_rgb8 / d_dimage view-major
 int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_settings* s, float* d_rgb,
                   uint8_t* d_rgb8, const float* d_dimage, float* d_dcolor);

@@ -1,0 +1,132 @@
+"""CPU checks of the radiance-cache reference (oracle/nrc.py): the BF16
+rounding against torch's bfloat16 conversion, the encoding against its closed
+form, the hand-written backward pass against torch autograd and central
+finite differences, and the parameter layout."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import nrc as N
+
+
+def _samples(n, seed=0):
+    rng = np.random.default_rng(seed)
+    f = np.zeros((n, 9), np.float32)
+    f[:, 0:3] = rng.uniform(-1, 1, (n, 3))
+    nr = rng.normal(size=(n, 3))
+    f[:, 3:6] = nr / np.linalg.norm(nr, axis=1, keepdims=True)
+    d = rng.normal(size=(n, 3))
+    f[:, 6:9] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    t = rng.uniform(0, 2, (n, 3)).astype(np.float32)
+    return f, t
+
+
+LO, INV = np.array([-1, -1, -1], np.float32), np.array([0.5, 0.5, 0.5], np.float32)
+
+
+def test_parameter_layout():
+    assert N.NPARAM == 23888
+    assert N.OFFW[0] == 0 and N.OFFB[0] == 96 * 64 and N.OFFW[1] == 96 * 64 + 64
+    assert N.OFFB[5] + 16 == N.NPARAM
+
+
+def test_bf16_matches_torch():
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.normal(size=20000).astype(np.float32) * 10.0 ** rng.integers(-30, 30, 20000),
+                        np.array([0.0, -0.0, 1.0, 1.00390625, 1.01171875, 3.0e38, 1e-40], np.float32)])
+    want = torch.from_numpy(x).to(torch.bfloat16).to(torch.float32).numpy()
+    np.testing.assert_array_equal(N.bf16(x).view(np.uint32), want.view(np.uint32))
+
+
+def test_encoding_closed_form():
+    f = np.zeros((3, 9), np.float32)
+    f[:, 0] = [-1.0, 0.0, -0.5]          # x = 0, 0.5, 0.25 in the unit box
+    f[:, 3:9] = np.arange(6, dtype=np.float32)
+    X = N.encode(f, LO, INV)
+    assert X.shape == (3, 96)
+    # x = 0: sin 0 = 0, cos 0 = 1 at every frequency
+    np.testing.assert_allclose(X[0, 0:24:2], 0.0, atol=1e-12)
+    np.testing.assert_allclose(X[0, 1:24:2], 1.0, atol=1e-12)
+    # x = 1/2: k = 0 -> sin(pi/2) = 1; k >= 1 -> sin(2^(k-1) pi) = 0, cos = 1 for k >= 2, cos(pi) = -1 for k = 1
+    assert abs(X[1, 0] - 1.0) < 1e-12 and abs(X[1, 2]) < 1e-12 and abs(X[1, 3] + 1.0) < 1e-12
+    # x = 1/4: k = 0 -> sin(pi/4) = cos(pi/4) = sqrt(1/2)
+    assert abs(X[2, 0] - np.sqrt(0.5)) < 1e-12 and abs(X[2, 1] - np.sqrt(0.5)) < 1e-12
+    np.testing.assert_array_equal(X[:, 72:78], f[:, 3:9])
+    assert np.all(X[:, 78:] == 0.0)
+
+
+def _torch_loss_grad(params, feats, targets):
+    """Independent gradient: torch autograd of the same network in float64."""
+    p = torch.tensor(params, dtype=torch.float64, requires_grad=True)
+    h = torch.from_numpy(N.encode(feats, LO, INV))
+    for l in range(6):
+        W = p[N.OFFW[l]:N.OFFW[l] + N.ND[l] * N.KD[l]].view(N.ND[l], N.KD[l])
+        b = p[N.OFFB[l]:N.OFFB[l] + N.ND[l]]
+        h = h @ W.T + b
+        if l < 5:
+            h = torch.relu(h)
+    y = h[:, :3]
+    t = torch.from_numpy(targets.astype(np.float64))
+    n = y.shape[0]
+    loss = (((y - t) ** 2) / (y.detach() ** 2 + 0.01)).sum() / n
+    loss.backward()
+    return float(loss.detach()) / 3, p.grad.numpy()
+
+
+def test_backward_matches_autograd():
+    f, t = _samples(200, 1)
+    p = N.init_params(5)
+    p[N.OFFB[0]:N.OFFB[0] + 64] = np.random.default_rng(2).normal(0, 0.1, 64)  # non-zero biases too
+    loss, g = N.loss_and_grad(p, f, t, LO, INV, emulate=False)
+    loss_t, g_t = _torch_loss_grad(p, f, t)
+    assert abs(loss - loss_t) <= 1e-12 * max(1.0, loss_t)
+    np.testing.assert_allclose(g, g_t, rtol=1e-9, atol=1e-12)
+    # padded inputs / outputs get no gradient
+    W0 = g[:N.ND[0] * N.KD[0]].reshape(64, 96)
+    assert np.all(W0[:, 78:] == 0.0)
+    W5 = g[N.OFFW[5]:N.OFFB[5]].reshape(16, 64)
+    assert np.all(W5[3:] == 0.0) and np.all(g[N.OFFB[5] + 3:] == 0.0)
+
+
+def test_backward_matches_finite_differences():
+    """Central differences of a float64 forward pass on 36 parameters (six per
+    layer).  The relative-L2 loss treats its denominator y^2 + 0.01 as a
+    constant, so the differenced loss holds it at the base point's value."""
+    f, t = _samples(16, 4)
+    p = N.init_params(7).astype(np.float64)
+    rng = np.random.default_rng(9)
+    _, g = N.loss_and_grad(p.astype(np.float32), f, t, LO, INV, emulate=False)
+
+    # the oracle's forward casts parameters to float32; differentiate a float64 copy
+    def loss64(pp, den):
+        W, b = N.unpack(pp)
+        h = N.encode(f, LO, INV)
+        for l in range(6):
+            h = h @ W[l].T + b[l]
+            if l < 5:
+                h = np.maximum(h, 0.0)
+        y = h[:, :3]
+        return float(np.sum((y - t) ** 2 / den) / f.shape[0])
+
+    y0, _ = N.forward(p.astype(np.float32), f, LO, INV, emulate=False)
+    den = y0 ** 2 + 0.01
+    p32 = p.astype(np.float32).astype(np.float64)
+    idx = np.concatenate([rng.integers(N.OFFW[l], N.OFFB[l] + N.ND[l], 6) for l in range(6)])
+    for i in idx:
+        if g[i] == 0.0:
+            continue
+        h = 1e-6
+        e = np.zeros_like(p32)
+        e[i] = h
+        fd = (loss64(p32 + e, den) - loss64(p32 - e, den)) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-5 * max(1.0, abs(g[i])), (i, fd, g[i])
+
+
+def test_emulated_forward_close_to_exact():
+    f, _ = _samples(300, 2)
+    p = N.init_params(1)
+    ye, _ = N.forward(p, f, LO, INV, emulate=True)
+    yx, _ = N.forward(p, f, LO, INV, emulate=False)
+    err = np.abs(ye - yx)
+    assert err.max() < 0.05 * max(1.0, np.abs(yx).max())
+    assert err.max() > 0.0  # the emulation rounds somewhere
