@@ -215,6 +215,19 @@ int  rt_bvh_export(rt_ctx* ctx, rt_bvh4_node* nodes, uint32_t* num_nodes,
 int  rt_trace_closest(rt_ctx* ctx, const rt_ray* rays, uint32_t n,
                       rt_hit* hits, int ptr_kind, rt_trace_counters* counters);
 
+/* Occlusion: occluded[i] = 1 when ray i hits any triangle with t in
+ * (max(tmin, 1e-7), tmax), else 0 (the same acceptance as rt_trace_closest,
+ * so occluded[i] == (closest hit != RT_MISS)).  The walk ends at the first
+ * accepted triangle.  Used for the shadow rays of rt_render_nrc. */
+int  rt_trace_any(rt_ctx* ctx, const rt_ray* rays, uint32_t n, uint8_t* occluded, int ptr_kind);
+
+/* Scene::isOccluded (graphic/scene.cpp:58-83) for n segments, 6 floats each
+ * (start, end): the ray from start along normalize(end - start) is occluded
+ * when an accepted triangle's hit point lies within squared distance
+ * |end - start|^2 of start (inclusive), over all triangles (the reference's
+ * own BVH can miss triangles on tight box faces; this query cannot). */
+int  rt_occluded(rt_ctx* ctx, const float* segments, uint32_t n, uint8_t* occluded, int ptr_kind);
+
 /* Renders one frame with the diffuse path tracer.  Replaces the frame loop
  * GraphicsEngine::render (graphics_engine.cpp:42-97) calling
  * PathTracer::renderPixel (graphic/path_tracer.cpp:18-97) per pixel.
@@ -260,6 +273,8 @@ typedef struct rt_nrc_config {
 #define RT_NRC_FEATURES 9          /* floats per sample: position, normal, direction */
 
 int  rt_nrc_create(rt_ctx* ctx, const rt_nrc_config* cfg, rt_nrc** out);
+/* rt_destroy of the context frees a live cache's device memory; its handle
+ * then only accepts rt_nrc_destroy (other calls return RT_E_STATE). */
 void rt_nrc_destroy(rt_nrc* nrc);
 /* Gets (set = 0) or sets (set = 1, resets Adam) the FP32 parameters; *count
  * is the in-out capacity (params NULL: query the count). */
@@ -272,6 +287,39 @@ int  rt_nrc_gradient(rt_nrc* nrc, const float* feats, const float* targets, uint
                      float* loss, int ptr_kind);
 /* One Adam step on n samples; loss (may be NULL) receives the mean loss. */
 int  rt_nrc_train(rt_nrc* nrc, const float* feats, const float* targets, uint32_t n, float* loss, int ptr_kind);
+
+/* Path tracing with the cache (rt_render_nrc).  A sample's path runs as in
+ * rt_render up to its query_vertex-th front-facing surface hit (1 = the hit
+ * after the first bounce); there, unless that hit ends the path on a light,
+ * the sample stops and adds throughput * cache(position, normal, direction)
+ * instead of tracing the rest.  One sample in train_every (chosen by a hash
+ * of pixel, sample and view) is a training sample: it traces its whole path
+ * as rt_render does, adds that to the pixel, and yields one training record
+ * at its query vertex whose target is the rest of its path's contribution.
+ * After the frame the cache takes up to max_steps Adam steps of `batch`
+ * records, each batch spread over the whole frame. */
+#define RT_NRC_NO_TRAIN 0x1        /* records are made but no step is taken    */
+#define RT_NRC_NO_QUERY 0x2        /* every sample traces its whole path        */
+#define RT_NRC_NO_RECORDS 0x4      /* no training samples (train_every ignored) */
+typedef struct rt_nrc_render {
+    uint32_t query_vertex;         /* >= 1; 0 selects 1                         */
+    uint32_t train_every;          /* 0 selects 16                              */
+    uint32_t batch;                /* multiple of 64; 0 selects 16384           */
+    uint32_t max_steps;            /* 0 selects 4                               */
+    uint32_t flags;                /* RT_NRC_*                                  */
+} rt_nrc_render;
+typedef struct rt_nrc_stats {
+    uint64_t queries;              /* samples that stopped at the cache         */
+    uint64_t records;              /* training records made                     */
+    uint32_t steps;                /* Adam steps taken                          */
+    float loss;                    /* mean loss of the last step (0 if none)    */
+    float render_ms, train_ms;     /* device time of the two phases             */
+} rt_nrc_stats;
+/* Renders num_views frames like rt_render_views with the cache (which must
+ * belong to ctx); stats may be NULL.  No counterpart in the software
+ * renderer (see the cache section above). */
+int  rt_render_nrc(rt_ctx* ctx, rt_nrc* nrc, const rt_camera* cams, uint32_t num_views, const rt_settings* s,
+                   const rt_nrc_render* opts, float* rgb, uint8_t* rgb8, int ptr_kind, rt_nrc_stats* stats);
 
 #ifdef __cplusplus
 }

@@ -128,6 +128,19 @@ class rt_nrc_config(C.Structure):
 
 
 RT_NRC_FEATURES = 9
+RT_NRC_NO_TRAIN = 0x1
+RT_NRC_NO_QUERY = 0x2
+RT_NRC_NO_RECORDS = 0x4
+
+
+class rt_nrc_render(C.Structure):
+    _fields_ = [("query_vertex", C.c_uint32), ("train_every", C.c_uint32), ("batch", C.c_uint32),
+                ("max_steps", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class rt_nrc_stats(C.Structure):
+    _fields_ = [("queries", C.c_uint64), ("records", C.c_uint64), ("steps", C.c_uint32), ("loss", C.c_float),
+                ("render_ms", C.c_float), ("train_ms", C.c_float)]
 
 # every exported symbol of include/rt.h with its signature
 _P = C.c_void_p
@@ -142,6 +155,8 @@ SIGNATURES = {
     "rt_build": (C.c_int, [_P, C.POINTER(rt_build_opts), C.POINTER(rt_bvh_info)]),
     "rt_bvh_export": (C.c_int, [_P, _P, C.POINTER(C.c_uint32), _P, C.POINTER(C.c_uint32)]),
     "rt_trace_closest": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_int, C.POINTER(rt_trace_counters)]),
+    "rt_trace_any": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_int]),
+    "rt_occluded": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_int]),
     "rt_render": (C.c_int, [_P, C.POINTER(rt_camera), C.POINTER(rt_settings), _P, _P, C.c_int]),
     "rt_render_views": (C.c_int, [_P, C.POINTER(rt_camera), C.c_uint32, C.POINTER(rt_settings), _P, _P, C.c_int]),
     "rt_render_adjoint": (C.c_int, [_P, C.POINTER(rt_camera), C.POINTER(rt_settings), _P, _P, C.c_int]),
@@ -151,6 +166,8 @@ SIGNATURES = {
     "rt_nrc_infer": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_int]),
     "rt_nrc_gradient": (C.c_int, [_P, _P, _P, C.c_uint32, _P, C.POINTER(C.c_float), C.c_int]),
     "rt_nrc_train": (C.c_int, [_P, _P, _P, C.c_uint32, C.POINTER(C.c_float), C.c_int]),
+    "rt_render_nrc": (C.c_int, [_P, _P, C.POINTER(rt_camera), C.c_uint32, C.POINTER(rt_settings),
+                                C.POINTER(rt_nrc_render), _P, _P, C.c_int, C.POINTER(rt_nrc_stats)]),
 }
 
 _lib = None

@@ -26,9 +26,10 @@ class RadianceCache:
         h = C.c_void_p()
         tracer._check(self.lib.rt_nrc_create(tracer.h, C.byref(cfg), C.byref(h)))
         self.h = h
+        tracer.caches.add(self)
 
     def close(self):
-        if self.h:
+        if getattr(self, "h", None):
             self.lib.rt_nrc_destroy(self.h)
             self.h = None
 
@@ -92,3 +93,28 @@ class RadianceCache:
                                           C.byref(loss) if want_loss else None,
                                           L.RT_PTR_DEVICE if dev else L.RT_PTR_HOST))
         return loss.value if want_loss else None
+
+    def render(self, cams, settings, query_vertex: int = 1, train_every: int = 16, batch: int = 16384,
+               max_steps: int = 4, train: bool = True, query: bool = True, out=None):
+        """One frame of rt_render_nrc for a camera or a list of cameras.
+        train_every = 0: no training samples.  Returns (rgb, stats): rgb
+        (H, W, 3) or (V, H, W, 3) float32 (numpy, or the torch tensor `out`
+        on the device), stats an rt_nrc_stats."""
+        single = not isinstance(cams, (list, tuple))
+        cl = [cams] if single else list(cams)
+        V = len(cl)
+        carr = (L.rt_camera * V)(*cl)
+        H, W = cl[0].height, cl[0].width
+        o = L.rt_nrc_render(query_vertex, train_every, batch, max_steps,
+                            (0 if train else L.RT_NRC_NO_TRAIN) | (0 if query else L.RT_NRC_NO_QUERY) |
+                            (L.RT_NRC_NO_RECORDS if train_every == 0 else 0))
+        st = L.rt_nrc_stats()
+        dev = out is not None and L._is_device(out)
+        if out is None:
+            out = np.zeros((V, H, W, 3), np.float32)
+        self._check(self.lib.rt_render_nrc(self.tracer.h, self.h, carr, V, C.byref(settings), C.byref(o),
+                                           L._ptr(out), None, L.RT_PTR_DEVICE if dev else L.RT_PTR_HOST,
+                                           C.byref(st)))
+        if single and not dev:
+            out = out[0]
+        return out, st

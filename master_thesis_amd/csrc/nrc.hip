@@ -150,9 +150,13 @@ __device__ __forceinline__ void hidden_forward(const __bf16* sW, const float* __
     }
 }
 
+// SCATTER: out[3 * idx[s] + c] += qw[3 * s + c] * max(y_c, 0) (atomically) in
+// place of out[3 * s + c] = y_c.
+template <bool SCATTER>
 __global__ void __launch_bounds__(NT) k_mlp_infer(const float* __restrict__ feats, uint32_t n,
                                                   const __bf16* __restrict__ Wb, const float* __restrict__ Wf, Geo G,
-                                                  float* __restrict__ out) {
+                                                  float* __restrict__ out, const float* __restrict__ qw,
+                                                  const uint32_t* __restrict__ idx) {
     __shared__ __attribute__((aligned(16))) __bf16 sW[NWB];
     __shared__ __attribute__((aligned(16))) __bf16 sX[TILE * 96];
     __shared__ __attribute__((aligned(16))) __bf16 sH[2][TILE * 64];
@@ -174,7 +178,11 @@ __global__ void __launch_bounds__(NT) k_mlp_infer(const float* __restrict__ feat
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const uint32_t s = s0 + 16 * w + 4 * kg + i;
-                if (s < n) out[(size_t)s * 3 + col] = acc[i] + b;
+                if (s < n) {
+                    const float y = acc[i] + b;
+                    if (SCATTER) atomicAdd(&out[3 * (size_t)idx[s] + col], qw[3 * (size_t)s + col] * fmaxf(y, 0.0f));
+                    else out[(size_t)s * 3 + col] = y;
+                }
             }
         }
     }
@@ -317,6 +325,18 @@ __global__ void k_refresh_bf16(const float* __restrict__ Wf, __bf16* __restrict_
     }
 }
 
+// batch sample j <- record b + stride * j
+__global__ void k_gather(const float* __restrict__ feats, const float* __restrict__ tgt, uint32_t b, uint32_t stride,
+                         uint32_t n, float* __restrict__ bf, float* __restrict__ bt) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const size_t r = (size_t)b + (size_t)stride * j;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) bf[9 * (size_t)j + k] = feats[9 * r + k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) bt[3 * (size_t)j + k] = tgt[3 * r + k];
+}
+
 }  // namespace nrc
 }  // namespace rt
 
@@ -411,10 +431,61 @@ int nrc_infer_device(rt_nrc* s, const float* d_feats, uint32_t n, float* d_out) 
     if (n == 0) return RT_OK;
     const uint32_t tiles = (n + TILE - 1) / TILE;
     const uint32_t grid = tiles < (uint32_t)s->infer_grid ? tiles : (uint32_t)s->infer_grid;
-    hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(NT), 0, c->stream, d_feats, n, (const __bf16*)s->wb.ptr,
-                       (const float*)s->wf.ptr, s->geo, d_out);
+    hipLaunchKernelGGL(k_mlp_infer<false>, dim3(grid), dim3(NT), 0, c->stream, d_feats, n, (const __bf16*)s->wb.ptr,
+                       (const float*)s->wf.ptr, s->geo, d_out, nullptr, nullptr);
     RT_HIP(c, hipGetLastError());
     return RT_OK;
+}
+int nrc_infer_scatter(rt_nrc* s, const float* d_feats, const float* d_w, const uint32_t* d_idx, uint32_t n,
+                      float* d_acc) {
+    rt_ctx* c = s->ctx;
+    if (n == 0) return RT_OK;
+    const uint32_t tiles = (n + TILE - 1) / TILE;
+    const uint32_t grid = tiles < (uint32_t)s->infer_grid ? tiles : (uint32_t)s->infer_grid;
+    hipLaunchKernelGGL(k_mlp_infer<true>, dim3(grid), dim3(NT), 0, c->stream, d_feats, n, (const __bf16*)s->wb.ptr,
+                       (const float*)s->wf.ptr, s->geo, d_acc, d_w, d_idx);
+    RT_HIP(c, hipGetLastError());
+    return RT_OK;
+}
+int nrc_train_records(rt_nrc* s, const float* d_feats, const float* d_targets, uint32_t count, uint32_t batch,
+                      uint32_t max_steps, uint32_t* steps, float* last_loss) {
+    rt_ctx* c = s->ctx;
+    *steps = 0;
+    if (count == 0 || max_steps == 0 || batch == 0) return RT_OK;
+    const uint32_t n = count < batch ? count : batch;
+    const uint32_t stride = count / n;
+    const uint32_t ns = stride < max_steps ? stride : max_steps;
+    int rc;
+    if ((rc = ensure(c, s->feats, (size_t)n * 36)) || (rc = ensure(c, s->targets, (size_t)n * 12))) return rc;
+    for (uint32_t b = 0; b < ns; ++b) {
+        hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, c->stream, d_feats, d_targets, b, stride, n,
+                           (float*)s->feats.ptr, (float*)s->targets.ptr);
+        RT_HIP(c, hipGetLastError());
+        if ((rc = run_train(s, (const float*)s->feats.ptr, (const float*)s->targets.ptr, n, RT_PTR_DEVICE,
+                            b + 1 == ns ? last_loss : nullptr, nullptr, true)))
+            return rc;
+    }
+    *steps = ns;
+    return RT_OK;
+}
+rt_ctx* nrc_ctx(const rt_nrc* s) { return s->ctx; }
+void nrc_detach(rt_nrc* s) {
+    rt_ctx* c = s->ctx;
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    DevBuf* all[] = {&s->wf, &s->wb, &s->m, &s->v, &s->grad, &s->partial, &s->loss, &s->feats, &s->targets, &s->out};
+    for (DevBuf* b : all) {
+        if (b->ptr) (void)hipFree(b->ptr);
+        b->ptr = nullptr;
+        b->bytes = 0;
+    }
+    for (size_t i = 0; i < c->caches.size(); ++i)
+        if (c->caches[i] == s) {
+            c->caches.erase(c->caches.begin() + (long)i);
+            break;
+        }
+    s->ctx = nullptr;
 }
 int nrc_train_device(rt_nrc* s, const float* d_feats, const float* d_targets, uint32_t n) {
     return run_train(s, d_feats, d_targets, n, RT_PTR_DEVICE, nullptr, nullptr, true);
@@ -475,29 +546,27 @@ int rt_nrc_create(rt_ctx* c, const rt_nrc_config* cfg, rt_nrc** out) {
     hipLaunchKernelGGL(k_refresh_bf16, dim3((NPARAM + 255) / 256), dim3(256), 0, c->stream, (const float*)s->wf.ptr,
                        (__bf16*)s->wb.ptr);
     int per_cu = 0;
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mlp_infer, NT, 0);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mlp_infer<false>, NT, 0);
     s->infer_grid = c->num_cus * (per_cu > 0 ? per_cu : 1);
     e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         rt_nrc_destroy(s);
         return hip_fail(c, e, "rt_nrc_create");
     }
+    c->caches.push_back(s);
     *out = s;
     return RT_OK;
 }
 
 void rt_nrc_destroy(rt_nrc* s) {
     if (!s) return;
-    (void)hipSetDevice(s->ctx->device);
-    (void)hipStreamSynchronize(s->ctx->stream);
-    DevBuf* all[] = {&s->wf, &s->wb, &s->m, &s->v, &s->grad, &s->partial, &s->loss, &s->feats, &s->targets, &s->out};
-    for (DevBuf* b : all)
-        if (b->ptr) (void)hipFree(b->ptr);
+    nrc_detach(s);
     delete s;
 }
 
 int rt_nrc_params(rt_nrc* s, float* params, uint32_t* count, int set) {
     if (!s || !count) return RT_E_INVALID;
+    if (!s->ctx) return RT_E_STATE;
     const uint32_t cap = *count;
     *count = NPARAM;
     if (!params) return RT_OK;
@@ -521,6 +590,7 @@ int rt_nrc_params(rt_nrc* s, float* params, uint32_t* count, int set) {
 
 int rt_nrc_infer(rt_nrc* s, const float* feats, uint32_t n, float* out, int ptr_kind) {
     if (!s) return RT_E_INVALID;
+    if (!s->ctx) return RT_E_STATE;
     if (n > 0 && (!feats || !out)) return nrc_fail(s, RT_E_INVALID, "rt_nrc_infer: NULL argument");
     if (ptr_kind != RT_PTR_HOST && ptr_kind != RT_PTR_DEVICE) return nrc_fail(s, RT_E_INVALID, "rt_nrc_infer: bad ptr_kind");
     if (n == 0) return RT_OK;
@@ -548,6 +618,7 @@ int rt_nrc_infer(rt_nrc* s, const float* feats, uint32_t n, float* out, int ptr_
 int rt_nrc_gradient(rt_nrc* s, const float* feats, const float* targets, uint32_t n, float* grad, float* loss,
                     int ptr_kind) {
     if (!s) return RT_E_INVALID;
+    if (!s->ctx) return RT_E_STATE;
     if (!grad) return nrc_fail(s, RT_E_INVALID, "rt_nrc_gradient: NULL gradient output");
     (void)hipSetDevice(s->ctx->device);
     return run_train(s, feats, targets, n, ptr_kind, loss, grad, false);
@@ -555,6 +626,7 @@ int rt_nrc_gradient(rt_nrc* s, const float* feats, const float* targets, uint32_
 
 int rt_nrc_train(rt_nrc* s, const float* feats, const float* targets, uint32_t n, float* loss, int ptr_kind) {
     if (!s) return RT_E_INVALID;
+    if (!s->ctx) return RT_E_STATE;
     (void)hipSetDevice(s->ctx->device);
     return run_train(s, feats, targets, n, ptr_kind, loss, nullptr, true);
 }

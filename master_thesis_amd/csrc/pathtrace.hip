@@ -29,6 +29,16 @@
 // adjoint pass replays it exactly.  Several views (rt_render_views) share
 // one slot pool: a work item is (chunk, view, pixel).
 //
+// Radiance cache (rt_render_nrc): at its query vertex (the nrc_q-th
+// front-facing surface hit, unless that hit ends the path on a light) a
+// sample either stops and enqueues (position, normal, direction, throughput,
+// pixel) for the cache, or, if it is a training sample, records the features
+// there, keeps a second throughput Tq from that vertex on, and at its end
+// enqueues a training record with target Tq * (end term).  Every 8 iterations
+// the host runs the cache over the queued queries, which add throughput *
+// prediction to their pixels' partial sums atomically; after the frame the
+// cache trains on the records.
+//
 // Adjoint (rt_render_adjoint): the same pass with dL/dI given.  A path's
 // contribution is the product of its diffuse factors colour_k * cos_k (the
 // light's own colour included), so d C / d colour_m sums, over the vertices k
@@ -46,12 +56,20 @@ namespace {
 constexpr int PT_BLOCK = 256;
 constexpr uint32_t ADJ_MAX_DEPTH = 16;
 constexpr uint32_t CULL_BIT = 0x10000u;
+// cache mode: front-facing hits so far (bits 17-23), training record open (bit 24)
+constexpr uint32_t FH_SHIFT = 17;
+constexpr uint32_t FH_MASK = 0x7Fu;
+constexpr uint32_t REC_BIT = 1u << 24;
 
 // ctr slots
 constexpr int C_ITEM = 0;   // next work item
 constexpr int C_N0 = 1;     // live count, queue 0
 constexpr int C_N1 = 2;     // live count, queue 1
 constexpr int C_RAYS = 4;   // 64-bit count of traced rays (slots 4-5)
+constexpr int C_QN = 6;     // cache queries queued since the last flush
+constexpr int C_TN = 7;     // training records made
+constexpr int C_QDROP = 8;  // queries dropped for want of room (stays 0: the host flushes in time)
+constexpr int C_WORDS = 9;
 
 struct PTArgs {
     const ShadeGeom* geom;
@@ -86,7 +104,29 @@ struct PTArgs {
     uint8_t* rgb8;
     const float* dimg;
     float* dcolor;
+    // radiance cache (nrc_q == 0: off)
+    uint32_t nrc_q, nrc_every, nrc_query;
+    float* q_feat;      // q_cap * 9
+    float* q_w;         // q_cap * 3: throughput before the query vertex
+    uint32_t* q_idx;    // q_cap: partial-sum index
+    uint32_t q_cap;
+    float* t_feat;      // t_cap * 9
+    float* t_tgt;       // t_cap * 3
+    uint32_t t_cap;
+    float* s_qf;        // 9 * S: training sample's features at its query vertex
+    float* s_Tq;        // 3 * S: its throughput from the query vertex on
 };
+
+// Training-sample choice: lowbias32 (C. Wellons) over pixel, sample, view, seed.
+__device__ __forceinline__ uint32_t train_hash(uint32_t pix, uint32_t smp, uint32_t view, uint32_t seed) {
+    uint32_t h = pix * 0x9E3779B1u ^ (smp + 0x7F4A7C15u) * 0x85EBCA77u ^ (view + 0x632BE5ABu) * 0xC2B2AE3Du ^ seed;
+    h ^= h >> 16;
+    h *= 0x7FEB352Du;
+    h ^= h >> 15;
+    h *= 0x846CA68Bu;
+    h ^= h >> 16;
+    return h;
+}
 
 __device__ __forceinline__ rt_ray make_ray(f3 o, f3 d) {
     rt_ray r;
@@ -131,6 +171,9 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_init(PTArgs a) {
         a.ctr[C_N0] = a.S;
         a.ctr[C_N1] = 0;
         *(unsigned long long*)&a.ctr[C_RAYS] = 0ull;
+        a.ctr[C_QN] = 0;
+        a.ctr[C_TN] = 0;
+        a.ctr[C_QDROP] = 0;
     }
     if (i >= a.S) return;
     rt_ray r;
@@ -170,7 +213,7 @@ __device__ __forceinline__ void scatter_grad(const PTArgs& a, uint32_t slot, uin
     }
 }
 
-template <bool ADJ>
+template <bool ADJ, bool NRC>
 __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
     const uint32_t n = a.ctr[cur ? C_N1 : C_N0];
     uint32_t* n_out = &a.ctr[cur ? C_N0 : C_N1];
@@ -181,6 +224,11 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
     bool alive = false, need_item = false;
     uint32_t slot = 0;
     rt_ray nr;
+    // cache mode: a query or a training record to enqueue
+    bool push_q = false, push_t = false;
+    float qf[9];
+    f3 qw = mk(0.0f, 0.0f, 0.0f);
+    uint32_t qidx = 0;
     if (k < n) {
         slot = a.q_slot[cur][k];
         const rt_ray r = a.q_ray[cur][k];
@@ -193,7 +241,11 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
         uint32_t st = a.s_state[slot];
         uint32_t depth = st & 0xFFFFu;
         bool cull = (st & CULL_BIT) != 0;
+        uint32_t fh = (st >> FH_SHIFT) & FH_MASK;
+        bool rec = NRC && (st & REC_BIT) != 0;
         f3 T = mk(a.s_T[3 * slot], a.s_T[3 * slot + 1], a.s_T[3 * slot + 2]);
+        f3 Tq = mk(1.0f, 1.0f, 1.0f);
+        if (rec) Tq = mk(a.s_Tq[3 * slot], a.s_Tq[3 * slot + 1], a.s_Tq[3 * slot + 2]);
         f3 o = mk(r.ox, r.oy, r.oz), d = mk(r.dx, r.dy, r.dz);
         bool end = false, contrib = false;
         f3 term = mk(1.0f, 1.0f, 1.0f);
@@ -203,6 +255,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
                 contrib = true;
                 term = miss_radiance(a, d);
                 T = mulv(T, term);
+                if (rec) Tq = mulv(Tq, term);
             }
         } else {
             const f3 p = hit_point(o, d, h.t);
@@ -229,11 +282,28 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
                 cull = false;
                 const rt_material m = a.mat[g.mat];
                 const uint32_t gsmp = a.sample_offset + chunk * a.spc + smp;
+                const bool light_end = m.light_source && dot(nrm, d) <= 0.0f;
+                if (NRC && fh == a.nrc_q && !light_end && depth + 1 < a.max_depth) {
+                    const bool train = a.nrc_every != 0 && train_hash(pix, gsmp, view, a.seed_lo) % a.nrc_every == 0;
+                    const float f9[9] = {p.x, p.y, p.z, nrm.x, nrm.y, nrm.z, d.x, d.y, d.z};
+                    if (train) {
+                        rec = true;
+                        Tq = mk(1.0f, 1.0f, 1.0f);
+                        for (int j = 0; j < 9; ++j) a.s_qf[9 * (size_t)slot + j] = f9[j];
+                    } else if (a.nrc_query) {
+                        push_q = true;
+                        for (int j = 0; j < 9; ++j) qf[j] = f9[j];
+                        qw = T;
+                        qidx = chunk * a.VP + vp;
+                        end = true;
+                    }
+                }
+                ++fh;
+                if (!push_q) {
                 u4 c4;
                 c4.x = pix; c4.y = gsmp; c4.z = depth; c4.w = view;
                 const u4 rnd = philox4x32_10(c4, a.seed_lo, a.seed_hi);
                 const float xi0 = u01(rnd.x);
-                const f3 prev = d;
                 const bool diffuse = xi0 <= m.diffuse_threshold;
                 if (diffuse) d = hemisphere_dir(u01(rnd.y), u01(rnd.z), nrm);
                 else if (xi0 <= m.reflect_threshold) d = reflect_ref(d, nrm);
@@ -242,6 +312,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
                 if (diffuse) {
                     const float cs = dot(nrm, d);
                     T = mulv(T, scale(cs, mk(m.color[0], m.color[1], m.color[2])));
+                    if (rec) Tq = mulv(Tq, scale(cs, mk(m.color[0], m.color[1], m.color[2])));
                     if (ADJ) {
                         uint32_t nv = a.s_nv[slot];
                         if (nv < ADJ_MAX_DEPTH) {
@@ -251,9 +322,10 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
                         a.s_nv[slot] = nv + 1;
                     }
                 }
-                if (m.light_source && dot(nrm, prev) <= 0.0f) {
+                if (light_end) {
                     end = true;
                     contrib = true;
+                }
                 }
             }
             ++depth;
@@ -269,13 +341,24 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
                     a.s_sum[3 * slot + 2] = a.s_sum[3 * slot + 2] + T.z;
                 }
             }
+            if (rec) {
+                push_t = true;
+                for (int j = 0; j < 9; ++j) qf[j] = a.s_qf[9 * (size_t)slot + j];
+                qw = contrib ? Tq : mk(0.0f, 0.0f, 0.0f);
+            }
             ++smp;
             if (smp < a.spc) {
                 a.s_smp[slot] = smp;
                 start_sample(a, slot, vp, nr, ADJ);
                 alive = true;
             } else {
-                if (!ADJ) {
+                if (NRC) {
+                    // cache predictions land in the same sums
+                    const size_t q = (size_t)chunk * a.VP + vp;
+                    atomicAdd(&a.partial[3 * q], a.s_sum[3 * slot]);
+                    atomicAdd(&a.partial[3 * q + 1], a.s_sum[3 * slot + 1]);
+                    atomicAdd(&a.partial[3 * q + 2], a.s_sum[3 * slot + 2]);
+                } else if (!ADJ) {
                     const size_t q = (size_t)chunk * a.VP + vp;
                     a.partial[3 * q] = a.s_sum[3 * slot];
                     a.partial[3 * q + 1] = a.s_sum[3 * slot + 1];
@@ -284,10 +367,47 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
                 need_item = true;
             }
         } else {
-            a.s_state[slot] = depth | (cull ? CULL_BIT : 0u);
+            a.s_state[slot] = depth | (cull ? CULL_BIT : 0u) | (NRC ? (fh << FH_SHIFT) | (rec ? REC_BIT : 0u) : 0u);
             a.s_T[3 * slot] = T.x; a.s_T[3 * slot + 1] = T.y; a.s_T[3 * slot + 2] = T.z;
+            if (rec) {
+                a.s_Tq[3 * slot] = Tq.x; a.s_Tq[3 * slot + 1] = Tq.y; a.s_Tq[3 * slot + 2] = Tq.z;
+            }
             nr = make_ray(o, d);
             alive = true;
+        }
+    }
+    if (NRC) {
+        // enqueue cache queries and training records: one atomic per wave each
+        const uint64_t qb = __ballot(push_q);
+        if (qb) {
+            const int leader = __ffsll((unsigned long long)qb) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(&a.ctr[C_QN], (uint32_t)__popcll(qb));
+            base = __shfl(base, leader);
+            if (push_q) {
+                const uint32_t pos = base + (uint32_t)__popcll(qb & lt);
+                if (pos < a.q_cap) {
+                    for (int j = 0; j < 9; ++j) a.q_feat[9 * (size_t)pos + j] = qf[j];
+                    a.q_w[3 * (size_t)pos] = qw.x; a.q_w[3 * (size_t)pos + 1] = qw.y; a.q_w[3 * (size_t)pos + 2] = qw.z;
+                    a.q_idx[pos] = qidx;
+                } else {
+                    atomicAdd(&a.ctr[C_QDROP], 1u);
+                }
+            }
+        }
+        const uint64_t tb = __ballot(push_t);
+        if (tb) {
+            const int leader = __ffsll((unsigned long long)tb) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(&a.ctr[C_TN], (uint32_t)__popcll(tb));
+            base = __shfl(base, leader);
+            if (push_t) {
+                const uint32_t pos = base + (uint32_t)__popcll(tb & lt);
+                if (pos < a.t_cap) {  // records past the capacity are dropped
+                    for (int j = 0; j < 9; ++j) a.t_feat[9 * (size_t)pos + j] = qf[j];
+                    a.t_tgt[3 * (size_t)pos] = qw.x; a.t_tgt[3 * (size_t)pos + 1] = qw.y; a.t_tgt[3 * (size_t)pos + 2] = qw.z;
+                }
+            }
         }
     }
     // next work item for slots whose item finished: one atomic per wave
@@ -348,8 +468,10 @@ inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) /
 }  // namespace
 
 int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_settings* s, float* d_rgb,
-                  uint8_t* d_rgb8, const float* d_dimage, float* d_dcolor) {
+                  uint8_t* d_rgb8, const float* d_dimage, float* d_dcolor, const NrcFrame* nf) {
     const bool adj = d_dcolor != nullptr;
+    const bool nrc = nf != nullptr;
+    if (adj && nrc) return fail(c, RT_E_INVALID, "launch_render: adjoint and cache modes exclude each other");
     if (adj && s->max_depth > ADJ_MAX_DEPTH)
         return fail(c, RT_E_UNSUPPORTED, "rt_render_adjoint: max_depth above 16");
     const rt_camera* cam = &cams[0];
@@ -370,10 +492,19 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
     }
     const uint64_t items = P * K;
     if (items > 0xFFFFFFFFull) return fail(c, RT_E_UNSUPPORTED, "rt_render: pixels * chunks above 2^32");
+    if (nrc && 3 * items > 0xFFFFFFFFull) return fail(c, RT_E_UNSUPPORTED, "rt_render_nrc: pixels * chunks above 2^32 / 3");
     const uint32_t S = (uint32_t)(items < target_slots ? items : target_slots);
     int rc;
     const size_t szS = S;
-    if ((rc = ensure(c, c->pt_state, szS * (4 * 3 + 12 * 2) + (adj ? szS * (8 * ADJ_MAX_DEPTH + 4) : 0))) ||
+    // cache mode: queries flushed every 8 iterations (at most S per
+    // iteration), training records capped at 4 frames' worth of steps
+    const size_t q_cap = nrc ? 8 * szS : 0;
+    const size_t t_cap = nrc ? (size_t)4 * nf->opt.batch * nf->opt.max_steps : 0;
+    if (nrc && ((rc = ensure(c, c->pt_nrc_query, q_cap * (36 + 12 + 4))) ||
+                (rc = ensure(c, c->pt_nrc_train, (t_cap ? t_cap : 1) * (36 + 12)))))
+        return rc;
+    if ((rc = ensure(c, c->pt_state, szS * (4 * 3 + 12 * 2) + (adj ? szS * (8 * ADJ_MAX_DEPTH + 4) : 0) +
+                                         (nrc ? szS * 48 : 0))) ||
         (rc = ensure(c, c->pt_queue[0], 4 * szS + 16 + sizeof(rt_ray) * szS)) ||
         (rc = ensure(c, c->pt_queue[1], 4 * szS + 16 + sizeof(rt_ray) * szS)) ||
         (rc = ensure(c, c->hits, szS * sizeof(rt_hit))) || (rc = ensure(c, c->pt_ctr, 64)) ||
@@ -405,7 +536,19 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
     a.s_sum = (float*)st; st += 12 * szS;
     a.s_amat = adj ? (uint32_t*)st : nullptr; st += adj ? 4 * ADJ_MAX_DEPTH * szS : 0;
     a.s_acos = adj ? (float*)st : nullptr; st += adj ? 4 * ADJ_MAX_DEPTH * szS : 0;
-    a.s_nv = adj ? (uint32_t*)st : nullptr;
+    a.s_nv = adj ? (uint32_t*)st : nullptr; st += adj ? 4 * szS : 0;
+    a.s_qf = nrc ? (float*)st : nullptr; st += nrc ? 36 * szS : 0;
+    a.s_Tq = nrc ? (float*)st : nullptr;
+    a.nrc_q = nrc ? nf->opt.query_vertex : 0;
+    a.nrc_every = nrc && !(nf->opt.flags & RT_NRC_NO_RECORDS) ? nf->opt.train_every : 0;
+    a.nrc_query = nrc && !(nf->opt.flags & RT_NRC_NO_QUERY);
+    a.q_feat = nrc ? (float*)c->pt_nrc_query.ptr : nullptr;
+    a.q_w = nrc ? a.q_feat + 9 * q_cap : nullptr;
+    a.q_idx = nrc ? (uint32_t*)(a.q_w + 3 * q_cap) : nullptr;
+    a.q_cap = (uint32_t)q_cap;
+    a.t_feat = nrc ? (float*)c->pt_nrc_train.ptr : nullptr;
+    a.t_tgt = nrc ? a.t_feat + 9 * t_cap : nullptr;
+    a.t_cap = (uint32_t)t_cap;
     // queue buffers hold S slot ids then S rays (16-byte aligned)
     for (int q = 0; q < 2; ++q) {
         a.q_slot[q] = (uint32_t*)c->pt_queue[q].ptr;
@@ -422,6 +565,7 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
     hipStream_t stream = c->stream;
     RT_HIP(c, hipEventRecord(c->ev[0], stream));
     if (adj) hipLaunchKernelGGL(k_zero, dim3(div_up(3ull * c->M, 256)), dim3(256), 0, stream, d_dcolor, 3u * c->M);
+    if (nrc) hipLaunchKernelGGL(k_zero, dim3(div_up(3ull * items, 256)), dim3(256), 0, stream, a.partial, (uint32_t)(3 * items));
     if (adj) hipLaunchKernelGGL(k_pt_init<true>, dim3(div_up(S, PT_BLOCK)), dim3(PT_BLOCK), 0, stream, a);
     else hipLaunchKernelGGL(k_pt_init<false>, dim3(div_up(S, PT_BLOCK)), dim3(PT_BLOCK), 0, stream, a);
     RT_HIP(c, hipGetLastError());
@@ -436,14 +580,25 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
         for (int rep = 0; rep < 8 && iter < bound; ++rep, ++iter) {
             if ((rc = launch_trace(c, a.q_ray[cur], S, a.hits, false, &a.ctr[cur ? C_N1 : C_N0]))) return rc;
             RT_HIP(c, hipMemsetAsync(&a.ctr[cur ? C_N0 : C_N1], 0, 4, stream));
-            if (adj) hipLaunchKernelGGL(k_pt_shade<true>, dim3(div_up(S, PT_BLOCK)), dim3(PT_BLOCK), 0, stream, a, cur);
-            else hipLaunchKernelGGL(k_pt_shade<false>, dim3(div_up(S, PT_BLOCK)), dim3(PT_BLOCK), 0, stream, a, cur);
+            if (adj) hipLaunchKernelGGL((k_pt_shade<true, false>), dim3(div_up(S, PT_BLOCK)), dim3(PT_BLOCK), 0, stream, a, cur);
+            else if (nrc) hipLaunchKernelGGL((k_pt_shade<false, true>), dim3(div_up(S, PT_BLOCK)), dim3(PT_BLOCK), 0, stream, a, cur);
+            else hipLaunchKernelGGL((k_pt_shade<false, false>), dim3(div_up(S, PT_BLOCK)), dim3(PT_BLOCK), 0, stream, a, cur);
             RT_HIP(c, hipGetLastError());
             launches += 2;
             cur ^= 1;
         }
-        RT_HIP(c, hipMemcpyAsync(&live, &a.ctr[cur ? C_N1 : C_N0], 4, hipMemcpyDeviceToHost, stream));
+        uint32_t hc[C_WORDS];
+        RT_HIP(c, hipMemcpyAsync(hc, a.ctr, sizeof(hc), hipMemcpyDeviceToHost, stream));
         RT_HIP(c, hipStreamSynchronize(stream));
+        live = hc[cur ? C_N1 : C_N0];
+        if (nrc && hc[C_QDROP] != 0) return fail(c, RT_E_DEVICE, "rt_render_nrc: cache query buffer overflow");
+        if (nrc && hc[C_QN] > 0) {
+            // the queued samples' cache predictions, added to their pixels
+            if ((rc = nrc_infer_scatter(nf->cache, a.q_feat, a.q_w, a.q_idx, hc[C_QN], a.partial))) return rc;
+            RT_HIP(c, hipMemsetAsync(&a.ctr[C_QN], 0, 4, stream));
+            nf->stats->queries += hc[C_QN];
+            ++launches;
+        }
         if (live > 0 && iter >= bound) return fail(c, RT_E_DEVICE, "rt_render: paths still live past the iteration bound");
     }
     if (!adj) {
@@ -451,12 +606,30 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
         RT_HIP(c, hipGetLastError());
         ++launches;
     }
+    if (nrc) {
+        RT_HIP(c, hipEventRecord(c->ev[4], stream));
+        uint32_t tn = 0;
+        RT_HIP(c, hipMemcpyAsync(&tn, &a.ctr[C_TN], 4, hipMemcpyDeviceToHost, stream));
+        RT_HIP(c, hipStreamSynchronize(stream));
+        nf->stats->records = tn;
+        if (!(nf->opt.flags & RT_NRC_NO_TRAIN)) {
+            const uint32_t avail = tn < a.t_cap ? tn : a.t_cap;
+            if ((rc = nrc_train_records(nf->cache, a.t_feat, a.t_tgt, avail, nf->opt.batch, nf->opt.max_steps,
+                                        &nf->stats->steps, &nf->stats->loss)))
+                return rc;
+            launches += 2 * nf->stats->steps;
+        }
+    }
     RT_HIP(c, hipEventRecord(c->ev[1], stream));
     unsigned long long rays = 0;
     RT_HIP(c, hipMemcpyAsync(&rays, &a.ctr[C_RAYS], 8, hipMemcpyDeviceToHost, stream));
     RT_HIP(c, hipStreamSynchronize(stream));
     float ms = 0.0f;
     RT_HIP(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    if (nrc) {
+        RT_HIP(c, hipEventElapsedTime(&nf->stats->render_ms, c->ev[0], c->ev[4]));
+        nf->stats->train_ms = ms - nf->stats->render_ms;
+    }
     c->timing.last_call_ms = ms;
     c->timing.launches = launches;
     c->timing.iterations = (uint32_t)iter;

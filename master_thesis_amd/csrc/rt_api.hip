@@ -126,7 +126,7 @@ int rt_create(rt_ctx** out, int device) {
         return fail(nullptr, RT_E_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     }
     c->stream = c->own_stream;
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < rt_ctx::NUM_EV; ++k) {
         e = hipEventCreate(&c->ev[k]);
         if (e != hipSuccess) {
             rt_destroy(c);
@@ -141,14 +141,17 @@ void rt_destroy(rt_ctx* c) {
     if (!c) return;
     DeviceGuard g(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    // caches outliving their context keep only their host part
+    std::vector<rt_nrc*> caches = c->caches;
+    for (rt_nrc* n : caches) nrc_detach(n);
     DevBuf* all[] = {&c->pos, &c->idx, &c->mat, &c->shade_geom, &c->shade_nrm, &c->nodes, &c->tris,
                      &c->keys[0], &c->keys[1], &c->vals[0], &c->vals[1], &c->hist, &c->inner, &c->leafbox,
                      &c->innerbox, &c->visit, &c->queue[0], &c->queue[1], &c->counters, &c->rays, &c->hits,
                      &c->spill, &c->work, &c->stats, &c->pt_state, &c->pt_queue[0], &c->pt_queue[1],
                      &c->pt_image, &c->pt_img8, &c->pt_dimg, &c->pt_dcolor, &c->pt_ctr, &c->pt_partial,
-                     &c->pt_cams};
+                     &c->pt_cams, &c->pt_nrc_query, &c->pt_nrc_train};
     for (DevBuf* b : all) release(c, *b);
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < rt_ctx::NUM_EV; ++k)
         if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
@@ -315,6 +318,46 @@ int rt_trace_closest(rt_ctx* c, const rt_ray* rays, uint32_t n, rt_hit* hits, in
     return RT_OK;
 }
 
+// any-hit queries: rays (32 B each) or segments (24 B each) -> one byte per query
+static int occlusion_query(rt_ctx* c, const char* what, const void* in, size_t in_bytes, uint32_t n, uint8_t* out,
+                           int ptr_kind, bool segments) {
+    if (!c) return RT_E_INVALID;
+    if (n > 0 && (!in || !out)) return fail(c, RT_E_INVALID, std::string(what) + ": NULL argument");
+    if (ptr_kind != RT_PTR_HOST && ptr_kind != RT_PTR_DEVICE) return fail(c, RT_E_INVALID, std::string(what) + ": bad ptr_kind");
+    if (!c->built) return fail(c, RT_E_STATE, std::string(what) + ": call rt_build first");
+    if (n == 0) return RT_OK;
+    DeviceGuard g(c->device);
+    int rc;
+    const void* d_in = in;
+    uint8_t* d_out = out;
+    if (ptr_kind == RT_PTR_HOST) {
+        if ((rc = ensure(c, c->rays, in_bytes * n)) || (rc = ensure(c, c->hits, n))) return rc;
+        RT_HIP(c, hipMemcpyAsync(c->rays.ptr, in, in_bytes * n, hipMemcpyHostToDevice, c->stream));
+        d_in = c->rays.ptr;
+        d_out = (uint8_t*)c->hits.ptr;
+    }
+    if (segments) rc = launch_occluded(c, (const float*)d_in, n, d_out);
+    else rc = launch_trace_any(c, (const rt_ray*)d_in, n, d_out);
+    if (rc) return rc;
+    c->last_op = 1;
+    if (ptr_kind == RT_PTR_HOST) {
+        uint32_t flags[2];
+        RT_HIP(c, hipMemcpyAsync(out, d_out, n, hipMemcpyDeviceToHost, c->stream));
+        RT_HIP(c, hipMemcpyAsync(flags, c->work.ptr, 8, hipMemcpyDeviceToHost, c->stream));
+        RT_HIP(c, hipStreamSynchronize(c->stream));
+        if (flags[1]) return fail(c, RT_E_DEVICE, std::string(what) + ": traversal exceeded its stack or step bound");
+    }
+    return RT_OK;
+}
+
+int rt_trace_any(rt_ctx* c, const rt_ray* rays, uint32_t n, uint8_t* occluded, int ptr_kind) {
+    return occlusion_query(c, "rt_trace_any", rays, sizeof(rt_ray), n, occluded, ptr_kind, false);
+}
+
+int rt_occluded(rt_ctx* c, const float* segments, uint32_t n, uint8_t* occluded, int ptr_kind) {
+    return occlusion_query(c, "rt_occluded", segments, 24, n, occluded, ptr_kind, true);
+}
+
 int rt_render(rt_ctx* c, const rt_camera* cam, const rt_settings* s, float* rgb, uint8_t* rgb8, int ptr_kind) {
     return rt_render_views(c, cam, 1, s, rgb, rgb8, ptr_kind);
 }
@@ -343,6 +386,54 @@ int rt_render_views(rt_ctx* c, const rt_camera* cams, uint32_t num_views, const 
         d_rgb8 = (uint8_t*)c->pt_img8.ptr;
     }
     if ((rc = launch_render(c, cams, num_views, s, d_rgb, d_rgb8, nullptr, nullptr))) return rc;
+    if (ptr_kind == RT_PTR_HOST) {
+        if (rgb) RT_HIP(c, hipMemcpyAsync(rgb, d_rgb, 12 * P, hipMemcpyDeviceToHost, c->stream));
+        if (rgb8) RT_HIP(c, hipMemcpyAsync(rgb8, d_rgb8, 3 * P, hipMemcpyDeviceToHost, c->stream));
+        RT_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    return RT_OK;
+}
+
+int rt_render_nrc(rt_ctx* c, rt_nrc* nrc, const rt_camera* cams, uint32_t num_views, const rt_settings* s,
+                  const rt_nrc_render* opts, float* rgb, uint8_t* rgb8, int ptr_kind, rt_nrc_stats* stats) {
+    if (!c) return RT_E_INVALID;
+    if (!nrc || !cams || !s) return fail(c, RT_E_INVALID, "rt_render_nrc: NULL cache, camera or settings");
+    if (!nrc_ctx(nrc)) return fail(c, RT_E_STATE, "rt_render_nrc: the cache's context was destroyed");
+    if (nrc_ctx(nrc) != c) return fail(c, RT_E_INVALID, "rt_render_nrc: the cache belongs to another context");
+    if (num_views == 0) return fail(c, RT_E_INVALID, "rt_render_nrc: no views");
+    if (!c->built) return fail(c, RT_E_STATE, "rt_render_nrc: call rt_build first");
+    if (cams[0].width == 0 || cams[0].height == 0) return fail(c, RT_E_INVALID, "rt_render_nrc: empty image");
+    for (uint32_t v = 1; v < num_views; ++v)
+        if (cams[v].width != cams[0].width || cams[v].height != cams[0].height)
+            return fail(c, RT_E_INVALID, "rt_render_nrc: views differ in size");
+    if (s->spp == 0) return fail(c, RT_E_INVALID, "rt_render_nrc: spp must be >= 1");
+    if (s->max_depth == 0 || s->max_depth > 64) return fail(c, RT_E_INVALID, "rt_render_nrc: max_depth must be 1..64");
+    if (ptr_kind != RT_PTR_HOST && ptr_kind != RT_PTR_DEVICE) return fail(c, RT_E_INVALID, "rt_render_nrc: bad ptr_kind");
+    NrcFrame f;
+    f.cache = nrc;
+    f.opt = opts ? *opts : rt_nrc_render{};
+    if (f.opt.query_vertex == 0) f.opt.query_vertex = 1;
+    if (f.opt.train_every == 0) f.opt.train_every = 16;
+    if (f.opt.batch == 0) f.opt.batch = 16384;
+    if (f.opt.max_steps == 0) f.opt.max_steps = 4;
+    if (f.opt.query_vertex > 63) return fail(c, RT_E_INVALID, "rt_render_nrc: query_vertex must be 1..63");
+    if (f.opt.batch % 64 != 0 || f.opt.batch > (1u << 22))
+        return fail(c, RT_E_INVALID, "rt_render_nrc: batch must be a multiple of 64 up to 2^22");
+    if (f.opt.max_steps > 64) return fail(c, RT_E_INVALID, "rt_render_nrc: max_steps must be <= 64");
+    rt_nrc_stats local{};
+    f.stats = stats ? stats : &local;
+    *f.stats = rt_nrc_stats{};
+    DeviceGuard g(c->device);
+    const size_t P = (size_t)cams[0].width * cams[0].height * num_views;
+    float* d_rgb = rgb;
+    uint8_t* d_rgb8 = rgb8;
+    int rc;
+    if (ptr_kind == RT_PTR_HOST) {
+        if ((rc = ensure(c, c->pt_image, 12 * P)) || (rc = ensure(c, c->pt_img8, 3 * P))) return rc;
+        d_rgb = (float*)c->pt_image.ptr;
+        d_rgb8 = (uint8_t*)c->pt_img8.ptr;
+    }
+    if ((rc = launch_render(c, cams, num_views, s, d_rgb, d_rgb8, nullptr, nullptr, &f))) return rc;
     if (ptr_kind == RT_PTR_HOST) {
         if (rgb) RT_HIP(c, hipMemcpyAsync(rgb, d_rgb, 12 * P, hipMemcpyDeviceToHost, c->stream));
         if (rgb8) RT_HIP(c, hipMemcpyAsync(rgb8, d_rgb8, 3 * P, hipMemcpyDeviceToHost, c->stream));

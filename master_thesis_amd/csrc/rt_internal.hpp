@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/rt.h"
 
@@ -56,10 +57,14 @@ struct rt_ctx {
     uint32_t trace_refill_min = 8;  // traversal: idle lanes that trigger a ray refill (MTRT_TRACE_REFILL)
     // path tracer state
     rt::DevBuf pt_state, pt_queue[2], pt_image, pt_img8, pt_dimg, pt_dcolor, pt_ctr, pt_partial, pt_cams;
+    rt::DevBuf pt_nrc_query, pt_nrc_train;  // cache queries (features, weights, pixel) and training records
+    std::vector<rt_nrc*> caches;            // live radiance caches; rt_destroy detaches them
     uint32_t render_iters = 0;
 
-    // ev[0..1] bracket a whole call, ev[2..3] the last traversal kernel
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // ev[0..1] bracket a whole call, ev[2..3] the last traversal kernel,
+    // ev[4] the end of a cached render's path phase
+    static constexpr int NUM_EV = 5;
+    hipEvent_t ev[NUM_EV] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     rt_timing timing{};
     int last_op = 0;  // 1 = trace, 2 = render
 };
@@ -86,14 +91,36 @@ constexpr int STACK_LDS = 32;
 // n: ray count, or an upper bound on it when n_dev (device-resident count) is set
 int launch_trace(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, bool stats,
                  const uint32_t* n_dev = nullptr);
+// occlusion: d_occluded[i] = 1 when ray i hits any triangle in (tmin, tmax)
+int launch_trace_any(rt_ctx* c, const rt_ray* d_rays, uint32_t n, uint8_t* d_occluded,
+                     const uint32_t* n_dev = nullptr);
+// Scene::isOccluded for n segments (start, end: 6 floats each)
+int launch_occluded(rt_ctx* c, const float* d_segs, uint32_t n, uint8_t* d_occluded);
+
+// nrc.hip: cache inference / training on device-resident samples
+int nrc_infer_device(rt_nrc* s, const float* d_feats, uint32_t n, float* d_out);
+// acc[3 * idx[i] + ch] += w[3 * i + ch] * max(prediction_i[ch], 0) (atomically)
+int nrc_infer_scatter(rt_nrc* s, const float* d_feats, const float* d_w, const uint32_t* d_idx, uint32_t n,
+                      float* d_acc);
+// up to max_steps Adam steps over `count` records (feats 9 + targets 3
+// floats each, in two arrays); batch b of `batch` records takes records
+// b + stride * j with stride = count / batch.  Returns the steps taken.
+int nrc_train_records(rt_nrc* s, const float* d_feats, const float* d_targets, uint32_t count, uint32_t batch,
+                      uint32_t max_steps, uint32_t* steps, float* last_loss);
+rt_ctx* nrc_ctx(const rt_nrc* s);
+// frees a cache's device memory and cuts it from its context (rt_destroy)
+void nrc_detach(rt_nrc* s);
 
 // pathtrace.hip
-// nrc.hip: cache inference / one training step on device-resident samples
-int nrc_infer_device(rt_nrc* s, const float* d_feats, uint32_t n, float* d_out);
-int nrc_train_device(rt_nrc* s, const float* d_feats, const float* d_targets, uint32_t n);
+// Radiance-cache options of one frame (rt_render_nrc)
+struct NrcFrame {
+    rt_nrc* cache;
+    rt_nrc_render opt;
+    rt_nrc_stats* stats;
+};
 
 // cams: V host cameras of one size; d_rgb / d_rgb8 / d_dimage view-major
 int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_settings* s, float* d_rgb,
-                  uint8_t* d_rgb8, const float* d_dimage, float* d_dcolor);
+                  uint8_t* d_rgb8, const float* d_dimage, float* d_dcolor, const NrcFrame* nrc = nullptr);
 
 }  // namespace rt

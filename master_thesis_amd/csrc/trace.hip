@@ -25,6 +25,14 @@
 // BVH depth.  The slab test reads each axis's near and far planes directly
 // (chosen by the sign of the direction), and the triangle test rejects most
 // candidates before its IEEE division (intersect_mt_fast).
+//
+// Any-hit mode (occlusion, k_trace<.., true>): the same walk, ending a ray at
+// the first triangle the test accepts; the output is one flag per ray.  As
+// the box tests never reject a triangle the triangle test accepts, a ray is
+// occluded exactly when the closest-hit walk finds a hit.  Segment queries
+// (Scene::isOccluded, graphic/scene.cpp:58-83) build the ray from (start,
+// end) as the reference does and accept a triangle when its hit point's
+// squared distance from start is <= |end - start|^2.
 #include "rt_internal.hpp"
 #include "rt_core.hpp"
 
@@ -34,6 +42,9 @@ namespace {
 struct TraceArgs {
     const rt_ray* rays;
     rt_hit* hits;
+    uint8_t* occluded;     // any-hit mode output
+    const float* segs;     // any-hit mode: n * 6 floats (start, end) in place of rays
+
     uint32_t n;
     const uint32_t* n_dev;  // when set, the ray count is read from device memory
     const rt_bvh4_node* nodes;
@@ -55,6 +66,7 @@ struct Lane {
     float ix, iy, iz, nox, noy, noz;
     float tmin, tmax, tlo, sigma, tcull;
     float best_d2, best_t, best_u, best_v;
+    float seg_d2;  // segment query: accept hits with squared distance <= seg_d2
     uint32_t best_id;
     uint32_t offx, offy, offz;  // byte offset of the near plane (lo or hi) per axis
     uint32_t cur, sp, steps;
@@ -92,7 +104,7 @@ __device__ __forceinline__ bool lane_start(Lane& L, const rt_ray& r) {
 
 // One traversal step (one inner node, or one leaf's triangles, then a pop).
 // Returns true when the ray is finished.
-template <bool STATS>
+template <bool STATS, bool ANY>
 __device__ __forceinline__ bool lane_step(const TraceArgs& a, Lane& L, uint32_t* stk, uint32_t tid, uint32_t gtid,
                                           uint64_t& c_nodes, uint64_t& c_tris) {
     if (!(L.cur & RT_LEAF_BIT)) {
@@ -154,10 +166,17 @@ __device__ __forceinline__ bool lane_step(const TraceArgs& a, Lane& L, uint32_t*
                               L.tmax, L.tcull, t, u, v) &&
             t > L.tmin && t < L.tmax) {
             const uint32_t id = __float_as_uint(p0.w);
+            if (ANY) {
+                if (!a.segs || dist2(L.o, hit_point(L.o, L.d, t)) <= L.seg_d2) {
+                    L.best_id = id;
+                    return true;
+                }
+            } else {
             const float dd = dist2(L.o, hit_point(L.o, L.d, t));
             if (closer(dd, id, L.best_d2, L.best_id)) {
                 L.best_d2 = dd; L.best_id = id; L.best_t = t; L.best_u = u; L.best_v = v;
                 L.tcull = fminf(L.tmax, fmaf(L.best_t, 1.0009765625f, L.sigma));
+            }
             }
         }
         if (rest) {
@@ -184,7 +203,7 @@ __device__ __forceinline__ bool lane_step(const TraceArgs& a, Lane& L, uint32_t*
 // (dynamic fetch, Aila & Laine 2009), from a wave-local pool of POOL_CHUNK
 // consecutive rays refilled with one global atomic, so that a wave is not
 // held up by its slowest ray.
-template <bool STATS>
+template <bool STATS, bool ANY>
 __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceArgs a) {
     __shared__ uint32_t stk[STACK_LDS * TRACE_BLOCK];
     const uint32_t tid = threadIdx.x;
@@ -221,10 +240,31 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceArgs a) {
                     if (idx < pool_end) {
                         ri = idx;
                         have = true;
-                        if (!lane_start(L, a.rays[ri])) {
-                            rt_hit h;
-                            h.tri = RT_MISS; h.t = 0.0f; h.u = 0.0f; h.v = 0.0f;
-                            a.hits[ri] = h;
+                        bool ok;
+                        if (ANY && a.segs) {
+                            // Scene::isOccluded: direction = end - start, dist2, normalize
+                            const float* sg = a.segs + 6 * (size_t)ri;
+                            f3 dir = mk(sg[3] - sg[0], sg[4] - sg[1], sg[5] - sg[2]);
+                            const float d2 = dot(dir, dir);
+                            dir = normalize_ref(dir);
+                            rt_ray r;
+                            r.ox = sg[0]; r.oy = sg[1]; r.oz = sg[2]; r.tmin = 0.0f;
+                            r.dx = dir.x; r.dy = dir.y; r.dz = dir.z; r.tmax = INFINITY;
+                            ok = lane_start(L, r);
+                            L.seg_d2 = d2;
+                            // conservative: a hit within the segment has t <= |end - start| (1 + 2^-10) + sigma
+                            L.tcull = fmaf(sqrtf(d2), 1.0009765625f, L.sigma);
+                        } else {
+                            ok = lane_start(L, a.rays[ri]);
+                        }
+                        if (!ok) {
+                            if (ANY) {
+                                a.occluded[ri] = 0;
+                            } else {
+                                rt_hit h;
+                                h.tri = RT_MISS; h.t = 0.0f; h.u = 0.0f; h.v = 0.0f;
+                                a.hits[ri] = h;
+                            }
                             have = false;
                             if (STATS) ++c_rays;
                         }
@@ -236,14 +276,18 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceArgs a) {
             }
         }
         if (__ballot(!retired) == 0ull) break;
-        if (have && lane_step<STATS>(a, L, stk, tid, gtid, c_nodes, c_tris)) {
-            rt_hit h;
-            h.tri = L.best_id;
+        if (have && lane_step<STATS, ANY>(a, L, stk, tid, gtid, c_nodes, c_tris)) {
             const bool hit = L.best_id != RT_MISS;
-            h.t = hit ? L.best_t : 0.0f;
-            h.u = hit ? L.best_u : 0.0f;
-            h.v = hit ? L.best_v : 0.0f;
-            a.hits[ri] = h;
+            if (ANY) {
+                a.occluded[ri] = hit ? 1 : 0;
+            } else {
+                rt_hit h;
+                h.tri = L.best_id;
+                h.t = hit ? L.best_t : 0.0f;
+                h.u = hit ? L.best_u : 0.0f;
+                h.v = hit ? L.best_v : 0.0f;
+                a.hits[ri] = h;
+            }
             have = false;
             if (STATS) { ++c_rays; c_hits += hit; }
         }
@@ -258,11 +302,15 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceArgs a) {
 
 }  // namespace
 
-int launch_trace(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, bool stats, const uint32_t* n_dev) {
+namespace {
+int launch(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, uint8_t* d_occl, bool stats,
+           const uint32_t* n_dev, const float* d_segs = nullptr) {
     if (n == 0) return RT_OK;
+    const bool any = d_occl != nullptr;
     int per_cu = 0;
-    if (stats) RT_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<true>, TRACE_BLOCK, 0));
-    else RT_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false>, TRACE_BLOCK, 0));
+    if (any) RT_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false, true>, TRACE_BLOCK, 0));
+    else if (stats) RT_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<true, false>, TRACE_BLOCK, 0));
+    else RT_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false, false>, TRACE_BLOCK, 0));
     if (per_cu < 1) per_cu = 1;
     uint32_t grid = (uint32_t)(c->num_cus * per_cu);
     uint32_t need_blocks = (n + TRACE_BLOCK - 1) / TRACE_BLOCK;
@@ -279,6 +327,8 @@ int launch_trace(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, bo
     TraceArgs a;
     a.rays = d_rays;
     a.hits = d_hits;
+    a.occluded = d_occl;
+    a.segs = d_segs;
     a.n = n;
     a.n_dev = n_dev;
     a.nodes = (const rt_bvh4_node*)c->nodes.ptr;
@@ -291,12 +341,24 @@ int launch_trace(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, bo
     a.refill_min = c->trace_refill_min;
     a.stats = (unsigned long long*)c->stats.ptr;
     RT_HIP(c, hipEventRecord(c->ev[2], c->stream));
-    if (stats) hipLaunchKernelGGL(k_trace<true>, dim3(grid), dim3(TRACE_BLOCK), 0, c->stream, a);
-    else hipLaunchKernelGGL(k_trace<false>, dim3(grid), dim3(TRACE_BLOCK), 0, c->stream, a);
+    if (any) hipLaunchKernelGGL((k_trace<false, true>), dim3(grid), dim3(TRACE_BLOCK), 0, c->stream, a);
+    else if (stats) hipLaunchKernelGGL((k_trace<true, false>), dim3(grid), dim3(TRACE_BLOCK), 0, c->stream, a);
+    else hipLaunchKernelGGL((k_trace<false, false>), dim3(grid), dim3(TRACE_BLOCK), 0, c->stream, a);
     RT_HIP(c, hipGetLastError());
     RT_HIP(c, hipEventRecord(c->ev[3], c->stream));
     c->timing.launches = 1;
     return RT_OK;
+}
+}  // namespace
+
+int launch_trace(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, bool stats, const uint32_t* n_dev) {
+    return launch(c, d_rays, n, d_hits, nullptr, stats, n_dev);
+}
+int launch_trace_any(rt_ctx* c, const rt_ray* d_rays, uint32_t n, uint8_t* d_occluded, const uint32_t* n_dev) {
+    return launch(c, d_rays, n, nullptr, d_occluded, false, n_dev);
+}
+int launch_occluded(rt_ctx* c, const float* d_segs, uint32_t n, uint8_t* d_occluded) {
+    return launch(c, nullptr, n, nullptr, d_occluded, false, nullptr, d_segs);
 }
 
 }  // namespace rt

@@ -4,6 +4,7 @@ torch CUDA (HIP) tensors, passed by pointer (no copies)."""
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -21,9 +22,12 @@ class Tracer:
         self.h = h
         self.scene: Scene | None = None
         self.info = None
+        self.caches = weakref.WeakSet()  # radiance caches on this context, closed first
 
     # -- lifetime -----------------------------------------------------------
     def close(self) -> None:
+        for c in list(self.caches):
+            c.close()
         if self.h:
             self.lib.rt_destroy(self.h)
             self.h = None
@@ -103,6 +107,33 @@ class Tracer:
         if counters:
             return hits, {f: getattr(cnt, f) for f, _ in L.rt_trace_counters._fields_}
         return hits
+
+    def trace_any(self, rays, out=None):
+        """Occlusion flags (uint8, 1 = some triangle within (tmin, tmax)).
+        rays: numpy RAY_DTYPE array or a CUDA tensor of n*8 float32."""
+        dev = L._is_device(rays)
+        if dev:
+            n = rays.numel() // 8
+            if out is None:
+                import torch
+                out = torch.empty(n, dtype=torch.uint8, device=rays.device)
+        else:
+            rays = np.ascontiguousarray(rays)
+            assert rays.dtype == L.RAY_DTYPE
+            n = rays.shape[0]
+            if out is None:
+                out = np.zeros(n, np.uint8)
+        self._check(self.lib.rt_trace_any(self.h, L._ptr(rays), n, L._ptr(out),
+                                          L.RT_PTR_DEVICE if dev else L.RT_PTR_HOST))
+        return out
+
+    def occluded(self, start, end):
+        """Scene::isOccluded for segment arrays start, end of shape (n, 3)."""
+        seg = np.ascontiguousarray(np.concatenate([np.asarray(start, np.float32).reshape(-1, 3),
+                                                   np.asarray(end, np.float32).reshape(-1, 3)], axis=1))
+        out = np.zeros(seg.shape[0], np.uint8)
+        self._check(self.lib.rt_occluded(self.h, L._ptr(seg), seg.shape[0], L._ptr(out), L.RT_PTR_HOST))
+        return out
 
     def timing(self) -> dict:
         t = L.rt_timing()

@@ -113,6 +113,20 @@ def trace_brute(scene, rays: np.ndarray, threads: int = 8) -> np.ndarray:
     return hits
 
 
+def occluded_brute(scene, segs: np.ndarray, threads: int = 8) -> np.ndarray:
+    """Scene::isOccluded by brute force for (n, 6) segments (start, end)."""
+    lib = load()
+    P = C.c_void_p
+    lib.or_occluded_brute.argtypes = [P, P, C.c_uint32, P, C.c_uint32, P, C.c_int]
+    lib.or_occluded_brute.restype = None
+    segs = np.ascontiguousarray(segs, np.float32)
+    out = np.zeros(segs.shape[0], np.uint8)
+    pos = np.ascontiguousarray(scene.pos, np.float32)
+    idx = np.ascontiguousarray(scene.idx, np.uint32)
+    lib.or_occluded_brute(_p(pos), _p(idx), idx.shape[0], _p(segs), segs.shape[0], _p(out), threads)
+    return out
+
+
 class CpuBVH:
     """The oracle's binned-SAH BVH (the CPU baseline's acceleration structure)."""
 
@@ -206,3 +220,29 @@ def render_adjoint(scene, cam, settings, d_image: np.ndarray, threads: int = 8) 
                                   C.addressof(settings), _p(d_image), _p(d_color), threads)
     assert rc == 0, rc
     return d_color
+
+
+def render_cached(scene, cams, settings, cache_rgb, q_vertex: int = 1, train_every: int = 16, query: bool = True,
+                  threads: int = 8):
+    """The cache mode of rt_render_nrc with a constant cache prediction.
+    Returns (rgb (V, H, W, 3) float32, queries, records)."""
+    lib = load()
+    P = C.c_void_p
+    lib.or_render_cached.argtypes = [P, P, P, P, P, C.c_uint32, C.c_uint32, C.c_uint32, P, C.c_uint32, P,
+                                     C.c_uint32, C.c_uint32, C.c_int, P, P, P, C.c_int]
+    lib.or_render_cached.restype = C.c_int
+    arrs, ps = _scene_ptrs(scene)
+    from master_thesis_amd._lib import rt_camera
+    if not isinstance(cams, (list, tuple)):
+        cams = [cams]
+    V = len(cams)
+    carr = (rt_camera * V)(*cams)
+    H, W = cams[0].height, cams[0].width
+    rgb = np.zeros((V, H, W, 3), np.float32)
+    c = np.ascontiguousarray(cache_rgb, np.float32)
+    counts = np.zeros(2, np.uint64)
+    rc = lib.or_render_cached(*ps, arrs[0].shape[0], arrs[2].shape[0], arrs[4].shape[0], C.addressof(carr), V,
+                              C.addressof(settings), q_vertex, train_every, int(query), _p(c), _p(rgb), _p(counts),
+                              threads)
+    assert rc == 0, rc
+    return rgb, int(counts[0]), int(counts[1])

@@ -298,6 +298,48 @@ void or_trace_brute(const float* pos, const uint32_t* idx, uint32_t T, const voi
     run_parallel(threads, n, 64, brute_range, &j);
 }
 
+/* Scene::isOccluded (graphic/scene.cpp:58-83) by brute force: direction =
+ * end - start, dist2 = |direction|^2 (float), direction normalised as
+ * Vector::normalize does (vector.h:214-225, double factor); occluded when an
+ * accepted triangle's hit point lies within squared distance dist2. */
+typedef struct {
+    const float* pos;
+    const uint32_t* idx;
+    uint32_t T;
+    const float* segs;
+    uint8_t* out;
+} occl_job;
+
+static void occl_range(void* arg, uint32_t b, uint32_t e, int tid) {
+    (void)tid;
+    occl_job* j = (occl_job*)arg;
+    for (uint32_t r = b; r < e; ++r) {
+        const float* sg = j->segs + 6 * (size_t)r;
+        v3 o = vld(sg);
+        v3 d = vsub(vld(sg + 3), o);
+        float dist2 = vdot(d, d);
+        d = vnormalize(d);
+        uint8_t occ = 0;
+        for (uint32_t t = 0; t < j->T && !occ; ++t) {
+            v3 v0 = vld(j->pos + 3 * j->idx[3 * t]);
+            v3 v1 = vld(j->pos + 3 * j->idx[3 * t + 1]);
+            v3 v2 = vld(j->pos + 3 * j->idx[3 * t + 2]);
+            float th, u, v;
+            if (mt(o, d, v0, vsub(v1, v0), vsub(v2, v0), &th, &u, &v) && th > 0.0f) {
+                v3 w = vsub(o, vadd(o, vscl(th, d)));
+                if (vdot(w, w) <= dist2) occ = 1;
+            }
+        }
+        j->out[r] = occ;
+    }
+}
+
+void or_occluded_brute(const float* pos, const uint32_t* idx, uint32_t T, const float* segs, uint32_t n,
+                       uint8_t* out, int threads) {
+    occl_job j = {pos, idx, T, segs, out};
+    run_parallel(threads, n, 64, occl_range, &j);
+}
+
 /* ------------------------------------------------------------------ */
 /* culling shared by both BVH traversals (same rule as the GPU kernel) */
 static inline float safe_rcp(float d) {
@@ -813,7 +855,24 @@ typedef struct {
     uint8_t* rgb8;
     const float* dimg;
     double* dcol; /* [256 threads][3M] */
+    /* radiance-cache mode (cache_rgb != NULL): a constant cache */
+    const float* cache_rgb;
+    uint32_t q_vertex, train_every, query_on;
+    uint64_t* n_queries; /* [256 threads] */
+    uint64_t* n_records; /* [256 threads] */
 } pt_job;
+
+/* Training-sample choice of the cache mode: lowbias32 (C. Wellons) over
+ * pixel, sample, view and the low seed word. */
+static uint32_t train_hash(uint32_t pix, uint32_t smp, uint32_t view, uint32_t seed) {
+    uint32_t h = pix * 0x9E3779B1u ^ (smp + 0x7F4A7C15u) * 0x85EBCA77u ^ (view + 0x632BE5ABu) * 0xC2B2AE3Du ^ seed;
+    h ^= h >> 16;
+    h *= 0x7FEB352Du;
+    h ^= h >> 15;
+    h *= 0x846CA68Bu;
+    h ^= h >> 16;
+    return h;
+}
 
 #define OR_MAX_DEPTH 64
 
@@ -821,7 +880,7 @@ typedef struct {
  * contribution when the path ends on a light (or escapes to the sky);
  * nv / vm / vc receive the diffuse vertices for the adjoint. */
 static int trace_path(const pt_job* j, uint32_t view, uint32_t pix, uint32_t gsmp, v3 dir0, v3* contrib, v3* term,
-                      uint32_t* nv, uint32_t* vm, float* vc) {
+                      uint32_t* nv, uint32_t* vm, float* vc, int tid) {
     const rt_settings* s = j->s;
     const rt_camera* cam = &j->cams[view];
     v3 o = V3(cam->origin[0], cam->origin[1], cam->origin[2]);
@@ -831,6 +890,7 @@ static int trace_path(const pt_job* j, uint32_t view, uint32_t pix, uint32_t gsm
     *nv = 0;
     *term = V3(1.0f, 1.0f, 1.0f);
     uint32_t key[2] = {(uint32_t)s->seed, (uint32_t)(s->seed >> 32)};
+    uint32_t fh = 0; /* front-facing hits so far (cache mode) */
     for (uint32_t depth = 0; depth < s->max_depth; ++depth) {
         rt_ray r;
         r.ox = o.x; r.oy = o.y; r.oz = o.z; r.tmin = 0.0f;
@@ -870,6 +930,19 @@ static int trace_path(const pt_job* j, uint32_t view, uint32_t pix, uint32_t gsm
         }
         cull = 0;
         const rt_material* m = &j->mat[j->tri_mat[t]];
+        const int light_end = m->light_source && vdot(n, d) <= 0.0f;
+        if (j->cache_rgb && fh == j->q_vertex && !light_end && depth + 1 < s->max_depth) {
+            const int train = j->train_every != 0 && train_hash(pix, gsmp, view, key[0]) % j->train_every == 0;
+            if (train) {
+                j->n_records[tid]++; /* the record is made when the path ends; its target is not checked here */
+            } else if (j->query_on) {
+                j->n_queries[tid]++;
+                *contrib = vmul(T, V3(fmaxf(j->cache_rgb[0], 0.0f), fmaxf(j->cache_rgb[1], 0.0f),
+                                      fmaxf(j->cache_rgb[2], 0.0f)));
+                return 1;
+            }
+        }
+        ++fh;
         uint32_t ctr[4] = {pix, gsmp, depth, view}, rnd[4];
         or_philox(ctr, key, rnd);
         float xi0 = u01(rnd[0]);
@@ -907,7 +980,7 @@ static void pt_range(void* arg, uint32_t b, uint32_t e, int tid) {
         for (uint32_t smp = 0; smp < j->s->spp; ++smp) {
             v3 c, term;
             uint32_t nv;
-            if (!trace_path(j, view, pix, j->s->sample_offset + smp, dir, &c, &term, &nv, vm, vc)) continue;
+            if (!trace_path(j, view, pix, j->s->sample_offset + smp, dir, &c, &term, &nv, vm, vc, tid)) continue;
             if (j->dimg) {
                 float w[3], tm[3] = {term.x, term.y, term.z};
                 for (int ch = 0; ch < 3; ++ch) w[ch] = j->dimg[3 * (size_t)vp + ch] * (float)j->scale;
@@ -942,9 +1015,15 @@ static void pt_range(void* arg, uint32_t b, uint32_t e, int tid) {
     }
 }
 
+typedef struct {
+    const float* rgb;
+    uint32_t q_vertex, train_every, query_on;
+    uint64_t queries, records;
+} cache_cfg;
+
 static int pt_run(const float* pos, const float* nrm, const uint32_t* idx, const uint32_t* tri_mat, const void* mat,
                   uint32_t V, uint32_t T, uint32_t M, const void* cams, uint32_t num_views, const void* settings,
-                  float* rgb, uint8_t* rgb8, const float* dimg, float* dcol, int threads) {
+                  float* rgb, uint8_t* rgb8, const float* dimg, float* dcol, int threads, cache_cfg* cc) {
     (void)V;
     const rt_settings* s = (const rt_settings*)settings;
     if (s->spp == 0 || s->max_depth == 0 || s->max_depth > OR_MAX_DEPTH) return -1;
@@ -961,7 +1040,25 @@ static int pt_run(const float* pos, const float* nrm, const uint32_t* idx, const
     j.scale = 2.0 * 3.14159265358979323846 * (double)(1.0f / (float)s->spp);
     j.rgb = rgb; j.rgb8 = rgb8; j.dimg = dimg;
     if (dimg) j.dcol = (double*)calloc((size_t)threads * 3 * M, sizeof(double));
+    uint64_t nq[256], nr[256];
+    memset(nq, 0, sizeof(nq));
+    memset(nr, 0, sizeof(nr));
+    j.n_queries = nq;
+    j.n_records = nr;
+    if (cc) {
+        j.cache_rgb = cc->rgb;
+        j.q_vertex = cc->q_vertex;
+        j.train_every = cc->train_every;
+        j.query_on = cc->query_on;
+    }
     run_parallel(threads, j.cams[0].width * j.cams[0].height * num_views, 16, pt_range, &j);
+    if (cc) {
+        cc->queries = cc->records = 0;
+        for (int t = 0; t < 256; ++t) {
+            cc->queries += nq[t];
+            cc->records += nr[t];
+        }
+    }
     if (dimg) {
         for (uint32_t i = 0; i < 3 * M; ++i) {
             double acc = 0.0;
@@ -977,17 +1074,36 @@ static int pt_run(const float* pos, const float* nrm, const uint32_t* idx, const
 int or_render(const float* pos, const float* nrm, const uint32_t* idx, const uint32_t* tri_mat, const void* mat,
               uint32_t V, uint32_t T, uint32_t M, const void* cam, const void* settings, float* rgb, uint8_t* rgb8,
               int threads) {
-    return pt_run(pos, nrm, idx, tri_mat, mat, V, T, M, cam, 1, settings, rgb, rgb8, NULL, NULL, threads);
+    return pt_run(pos, nrm, idx, tri_mat, mat, V, T, M, cam, 1, settings, rgb, rgb8, NULL, NULL, threads, NULL);
 }
 
 int or_render_views(const float* pos, const float* nrm, const uint32_t* idx, const uint32_t* tri_mat, const void* mat,
                     uint32_t V, uint32_t T, uint32_t M, const void* cams, uint32_t num_views, const void* settings,
                     float* rgb, uint8_t* rgb8, int threads) {
-    return pt_run(pos, nrm, idx, tri_mat, mat, V, T, M, cams, num_views, settings, rgb, rgb8, NULL, NULL, threads);
+    return pt_run(pos, nrm, idx, tri_mat, mat, V, T, M, cams, num_views, settings, rgb, rgb8, NULL, NULL, threads, NULL);
 }
 
 int or_render_adjoint(const float* pos, const float* nrm, const uint32_t* idx, const uint32_t* tri_mat,
                       const void* mat, uint32_t V, uint32_t T, uint32_t M, const void* cam, const void* settings,
                       const float* d_image, float* d_color, int threads) {
-    return pt_run(pos, nrm, idx, tri_mat, mat, V, T, M, cam, 1, settings, NULL, NULL, d_image, d_color, threads);
+    return pt_run(pos, nrm, idx, tri_mat, mat, V, T, M, cam, 1, settings, NULL, NULL, d_image, d_color, threads, NULL);
+}
+
+int or_render_cached(const float* pos, const float* nrm, const uint32_t* idx, const uint32_t* tri_mat,
+                     const void* mat, uint32_t V, uint32_t T, uint32_t M, const void* cams, uint32_t num_views,
+                     const void* settings, uint32_t q_vertex, uint32_t train_every, int query_on,
+                     const float cache_rgb[3], float* rgb, uint64_t counts[2], int threads) {
+    cache_cfg cc;
+    memset(&cc, 0, sizeof(cc));
+    cc.rgb = cache_rgb;
+    cc.q_vertex = q_vertex;
+    cc.train_every = train_every;
+    cc.query_on = (uint32_t)(query_on != 0);
+    int rc = pt_run(pos, nrm, idx, tri_mat, mat, V, T, M, cams, num_views, settings, rgb, NULL, NULL, NULL, threads,
+                    &cc);
+    if (counts) {
+        counts[0] = cc.queries;
+        counts[1] = cc.records;
+    }
+    return rc;
 }

@@ -94,6 +94,22 @@ int or_render_adjoint(const float* pos, const float* nrm, const uint32_t* idx, c
                       const void* mat, uint32_t V, uint32_t T, uint32_t M, const void* cam, const void* settings,
                       const float* d_image, float* d_color, int threads);
 
+/* Scene::isOccluded (graphic/scene.cpp:58-83) for n segments (start, end:
+ * 6 floats each) by brute force over all T triangles; out[i] = 0 / 1. */
+void or_occluded_brute(const float* pos, const uint32_t* idx, uint32_t T, const float* segs, uint32_t n,
+                       uint8_t* out, int threads);
+
+/* The cache mode of rt_render_nrc with a cache that predicts the constant
+ * cache_rgb everywhere: a sample stops at its q_vertex-th front-facing hit
+ * (unless that hit ends the path on a light, or is at the last depth) and
+ * adds throughput * max(cache_rgb, 0), except training samples
+ * (train_hash(pixel, sample, view, seed) % train_every == 0), which trace on.
+ * counts (may be NULL): queries, training records. */
+int or_render_cached(const float* pos, const float* nrm, const uint32_t* idx, const uint32_t* tri_mat,
+                     const void* mat, uint32_t V, uint32_t T, uint32_t M, const void* cams, uint32_t num_views,
+                     const void* settings, uint32_t q_vertex, uint32_t train_every, int query_on,
+                     const float cache_rgb[3], float* rgb, uint64_t counts[2], int threads);
+
 #ifdef __cplusplus
 }
 #endif

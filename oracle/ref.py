@@ -30,6 +30,7 @@ def load() -> C.CDLL:
         lib.ref_scene_create.restype = P
         lib.ref_scene_free.argtypes = [P]
         lib.ref_scene_trace.argtypes = [P, P, C.c_uint32, P, P, P, P]
+        lib.ref_scene_occluded.argtypes = [P, P, C.c_uint32, P]
         lib.ref_camera.argtypes = [P, C.c_float, C.c_float, C.c_uint32, C.c_uint32, P, P, P]
         lib.ref_load_scene.argtypes = [C.c_char_p, C.c_char_p, P, P, P, P, P, P]
         lib.ref_load_scene.restype = C.c_int
@@ -69,6 +70,13 @@ class RefScene:
         load().ref_scene_trace(self.h, rays.ctypes.data, n, hit.ctypes.data, p.ctypes.data, nr.ctypes.data,
                                obj.ctypes.data)
         return hit.astype(bool), p, nr, obj
+
+    def occluded(self, segs: np.ndarray) -> np.ndarray:
+        """Scene::isOccluded per (n, 6) segment (start, end)."""
+        segs = np.ascontiguousarray(segs, np.float32)
+        out = np.zeros(segs.shape[0], np.uint8)
+        load().ref_scene_occluded(self.h, segs.ctypes.data, segs.shape[0], out.ctypes.data)
+        return out
 
     def render(self, cam, spp: int, depth: int, threads: int = 8) -> np.ndarray:
         vi = np.array(cam.view_inv[:], np.float32)

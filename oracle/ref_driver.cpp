@@ -128,6 +128,15 @@ void ref_scene_trace(void* h, const float* rays /* 8 floats each */, uint32_t n,
     }
 }
 
+// Scene::isOccluded per segment (scene.cpp:58-83), 6 floats each (start, end).
+void ref_scene_occluded(void* h, const float* segs, uint32_t n, uint8_t* out) {
+    auto* s = (RefScene*)h;
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* g = segs + 6 * i;
+        out[i] = s->scene.isOccluded(Vector3f({g[0], g[1], g[2]}), Vector3f({g[3], g[4], g[5]})) ? 1 : 0;
+    }
+}
+
 // Camera::parseInput + GraphicsEngine::render's matrices (camera.cpp:19-51,
 // graphics_engine.cpp:42-47).  The reference's fov is fixed at pi/4.
 void ref_camera(const float* position, float angle0, float angle1, uint32_t w, uint32_t h, float* view_inv,

@@ -3,7 +3,8 @@ its own sources (oracle/_ref/libref.so; needs the reference present, so it
 runs in the build container only).  The vectors are data: rays, the
 reference's answers, and camera matrices.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            (all vectors)
+    python tests/golden/make_golden.py occlusion  (the occlusion vectors only)
 """
 import math
 import sys
@@ -42,7 +43,35 @@ def main():
     np.savez_compressed(HERE / "ref_cameras.npz", rows=np.stack(mats), width=320, height=200,
                         fov=np.float32(math.pi / 4))
     print("wrote", HERE / "ref_cornell_trace.npz", HERE / "ref_cameras.npz")
+    occlusion()
+
+
+def occlusion():
+    """Scene::isOccluded on the Cornell box: random segments, segments ending
+    exactly at the reference's own hit points (the inclusive bound), and
+    segments stopping short of them."""
+    sc = scenes.cornell_box()
+    ref = R.RefScene(sc)
+    rng = np.random.default_rng(20231204)
+    n = 2048
+    segs = [np.concatenate([rng.uniform(-2.7, 2.7, (n, 3)), rng.uniform(-2.7, 2.7, (n, 3))], axis=1)]
+    rays = np.zeros(n, RAY_DTYPE)
+    rays["o"] = rng.uniform(-2.7, 2.7, (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    rays["d"] = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    rays["tmax"] = np.inf
+    hit, p, _, _ = ref.trace(rays)
+    o = rays["o"][hit]
+    segs.append(np.concatenate([o, p[hit]], axis=1))
+    segs.append(np.concatenate([o, o + np.float32(0.999) * (p[hit] - o)], axis=1))
+    segs = np.concatenate(segs).astype(np.float32)
+    occ = ref.occluded(segs)
+    np.savez_compressed(HERE / "ref_cornell_occluded.npz", segs=segs, occluded=occ)
+    print("wrote", HERE / "ref_cornell_occluded.npz", segs.shape[0], "segments,", int(occ.sum()), "occluded")
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["occlusion"]:
+        occlusion()
+    else:
+        main()

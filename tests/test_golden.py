@@ -55,3 +55,22 @@ def test_gpu_matches_reference_vectors(tracer):
     tracer.upload(sc)
     tracer.build(4)
     assert _compare_with_reference(sc, rays, tracer.trace(rays), g) < 0.01 * rays.shape[0]
+
+
+def test_oracle_occlusion_matches_reference_vectors():
+    """Scene::isOccluded answers of the compiled reference, including segments
+    ending exactly on a surface (the bound is inclusive)."""
+    g = np.load(G / "ref_cornell_occluded.npz")
+    got = O.occluded_brute(scenes.cornell_box(), g["segs"])
+    np.testing.assert_array_equal(got, g["occluded"])
+    assert 0 < g["occluded"].sum() < g["occluded"].size
+
+
+@pytest.mark.gpu
+def test_gpu_occlusion_matches_reference_vectors(tracer):
+    g = np.load(G / "ref_cornell_occluded.npz")
+    sc = scenes.cornell_box()
+    tracer.upload(sc)
+    tracer.build(2)
+    segs = g["segs"]
+    np.testing.assert_array_equal(tracer.occluded(segs[:, :3], segs[:, 3:]), g["occluded"])

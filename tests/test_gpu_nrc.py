@@ -1,12 +1,19 @@
 """GPU radiance cache: MFMA inference and gradient against the CPU
 reference network with the same BF16 roundings (oracle/nrc.py), parameter
-round trip, and online training that lowers the loss."""
+round trip, online training that lowers the loss; path tracing with the
+cache against the oracle's constant-cache render, a closed form, the plain
+render, and a converged reference (the cache's gain)."""
+import math
+
 import numpy as np
 import pytest
 
+import oracle as O
 from oracle import nrc as N
+from master_thesis_amd import host_lib as H
 from master_thesis_amd import scenes
 from master_thesis_amd.cache import RadianceCache
+from master_thesis_amd.tracer import make_settings
 
 pytestmark = pytest.mark.gpu
 
@@ -131,3 +138,94 @@ def test_training_lowers_loss(cache):
     y = cache.infer(f_eval)
     rel = np.sqrt(np.mean((y - t_eval) ** 2)) / np.sqrt(np.mean(t_eval ** 2))
     assert rel < 0.15, rel
+
+
+# ---- path tracing with the cache (rt_render_nrc)
+
+
+def _constant_params(c):
+    """All weights zero, output bias c: the network predicts c everywhere."""
+    p = np.zeros(N.NPARAM, np.float32)
+    p[N.OFFB[5]:N.OFFB[5] + 3] = c
+    return p
+
+
+def _scene_cache(tracer, sc, seed=0):
+    tracer.upload(sc)
+    tracer.build(2)
+    return RadianceCache(tracer, lr=1e-2, seed=seed)
+
+
+def test_render_without_queries_is_the_plain_render(tracer):
+    sc = scenes.cornell_box()
+    cache = _scene_cache(tracer, sc)
+    cam = H.cornell_camera(48, 40)
+    s = make_settings(spp=8, max_depth=5, seed=3)
+    plain, _ = tracer.render(cam, s)
+    got, st = cache.render(cam, s, train=False, query=False)
+    assert st.queries == 0 and st.records > 0 and st.steps == 0  # records are made, no step taken
+    np.testing.assert_array_equal(got, plain)
+    cache.close()
+
+
+@pytest.mark.parametrize("q_vertex,every", [(1, 16), (2, 4), (1, 0)])
+def test_constant_cache_render_matches_oracle(tracer, q_vertex, every):
+    """Same queries, records and image as the oracle's constant-cache render
+    (images to float summation order: the GPU adds cache terms atomically)."""
+    sc = scenes.cornell_box()
+    cache = _scene_cache(tracer, sc)
+    c = np.array([0.4, 0.25, 0.1], np.float32)
+    cache.set_params(_constant_params(c))
+    cam = H.cornell_camera(40, 32)
+    s = make_settings(spp=8, max_depth=6, seed=9, sample_offset=5)
+    got, st = cache.render(cam, s, query_vertex=q_vertex, train_every=every, train=False)
+    want, nq, nr = O.render_cached(sc, cam, s, c, q_vertex, every)
+    assert (st.queries, st.records) == (nq, nr)
+    assert nq > 0
+    np.testing.assert_allclose(got, want[0], rtol=2e-5, atol=1e-6)
+    cache.close()
+
+
+def test_constant_cache_closed_room_is_pi_rho_c(tracer):
+    rho = 0.6
+    sc = scenes.closed_room(scenes.material((rho, rho, rho)), scenes.material(scenes.WHITE))
+    cache = _scene_cache(tracer, sc)
+    c = np.array([0.3, 0.5, 0.7], np.float32)
+    cache.set_params(_constant_params(c))
+    cam = H.camera((0.0, 0.0, 0.0), 0.0, -1.0, math.pi / 4, 64, 64)
+    N_ = 16
+    got, st = cache.render(cam, make_settings(spp=N_, max_depth=4, seed=2), train_every=0, train=False)
+    P = 64 * 64
+    assert st.queries == N_ * P
+    for ch in range(3):
+        sd = 2 * math.pi * rho * c[ch] / math.sqrt(12 * N_ * P)
+        assert abs(got[..., ch].mean() - math.pi * rho * c[ch]) < 4 * sd
+    cache.close()
+
+
+def test_online_training_beats_plain_path_tracing(tracer):
+    """Cornell box, 64 x 64: after online training on plain frames, cached
+    frames at 4 spp have a lower RMSE against a converged plain render than
+    plain frames at 4 spp (the cache's bias included)."""
+    sc = scenes.cornell_box()
+    cache = _scene_cache(tracer, sc, seed=1)
+    cam = H.cornell_camera(64, 64)
+    ref, _ = tracer.render(cam, make_settings(spp=8192, max_depth=6, seed=100))
+    losses = []
+    for f in range(150):
+        s = make_settings(spp=4, max_depth=6, seed=7, sample_offset=4 * f)
+        _, st = cache.render(cam, s, train_every=1, batch=2048, max_steps=4, query=False)
+        assert st.steps == min(4, st.records // 2048)
+        losses.append(st.loss)
+    assert np.mean(losses[-10:]) < 0.5 * np.mean(losses[:10])
+    err_plain, err_nrc = [], []
+    for f in range(8):
+        s = make_settings(spp=4, max_depth=6, seed=7, sample_offset=100000 + 4 * f)
+        plain, _ = tracer.render(cam, s)
+        cached, _ = cache.render(cam, s, train=False)
+        err_plain.append(np.sqrt(np.mean((plain - ref) ** 2)))
+        err_nrc.append(np.sqrt(np.mean((cached - ref) ** 2)))
+    print(f"rmse plain {np.mean(err_plain):.4f} cached {np.mean(err_nrc):.4f} "
+          f"loss {np.mean(losses[:10]):.4f} -> {np.mean(losses[-10:]):.4f}")
+    assert np.mean(err_nrc) < 0.7 * np.mean(err_plain)
+    cache.close()

@@ -136,3 +136,36 @@ def test_trace_before_build_is_a_state_error():
         with pytest.raises(RTError) as e:
             t.trace(scenes.incoherent_rays(4))
         assert e.value.code == RT_E_STATE
+
+
+def test_any_hit_equals_closest_hit_presence(tracer):
+    """Occlusion is exactly 'the closest-hit walk finds a hit', here with
+    finite tmax and tmin on a 200k-triangle soup."""
+    sc = scenes.triangle_soup(200_000, seed=5)
+    _upload(tracer, sc, leaf=2)
+    rng = np.random.default_rng(8)
+    n = 300_000
+    r = np.zeros(n, RAY_DTYPE)
+    r["o"] = rng.uniform(-0.2, 1.2, (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    r["d"] = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    r["tmin"] = rng.uniform(0.0, 0.05, n).astype(np.float32)
+    r["tmax"] = rng.choice(np.array([0.01, 0.05, 0.2, np.inf], np.float32), n)
+    occ = tracer.trace_any(r)
+    hits = tracer.trace(r)
+    np.testing.assert_array_equal(occ, (hits["tri"] != RT_MISS).astype(np.uint8))
+    assert 0.05 < occ.mean() < 0.95
+
+
+def test_occluded_segments_match_oracle(tracer):
+    sc = scenes.cornell_box()
+    _upload(tracer, sc, leaf=2)
+    rng = np.random.default_rng(9)
+    n = 50_000
+    a = rng.uniform(-2.7, 2.7, (n, 3)).astype(np.float32)
+    b = rng.uniform(-2.7, 2.7, (n, 3)).astype(np.float32)
+    b[: n // 10] = a[: n // 10]  # zero-length segments are never occluded
+    got = tracer.occluded(a, b)
+    want = O.occluded_brute(sc, np.concatenate([a, b], axis=1))
+    np.testing.assert_array_equal(got, want)
+    assert not got[: n // 10].any()

@@ -1,12 +1,19 @@
 """CPU checks of the radiance-cache reference (oracle/nrc.py): the BF16
 rounding against torch's bfloat16 conversion, the encoding against its closed
 form, the hand-written backward pass against torch autograd and central
-finite differences, and the parameter layout."""
+finite differences, and the parameter layout; the oracle's constant-cache
+render (the checker of rt_render_nrc) against closed forms and the plain
+render."""
+import math
+
 import numpy as np
-import pytest
 import torch
 
+import oracle as O
 from oracle import nrc as N
+from master_thesis_amd import host_lib as H
+from master_thesis_amd import scenes
+from master_thesis_amd.tracer import make_settings
 
 
 def _samples(n, seed=0):
@@ -130,3 +137,71 @@ def test_emulated_forward_close_to_exact():
     err = np.abs(ye - yx)
     assert err.max() < 0.05 * max(1.0, np.abs(yx).max())
     assert err.max() > 0.0  # the emulation rounds somewhere
+
+
+# ---- the constant-cache render of the oracle (the checker of rt_render_nrc)
+
+
+def _down_camera(w=32, h=32):
+    # centre of the closed room, looking straight down at the floor
+    return H.camera((0.0, 0.0, 0.0), 0.0, -1.0, math.pi / 4, w, h)
+
+
+def test_cached_render_is_pi_rho_c():
+    """Camera rays hit the floor (albedo rho); every bounce from it hits a
+    non-emitting wall, the first-bounce vertex, where the sample adds
+    throughput rho * cos * c: pixel mean pi * rho * c (cos ~ U[0, 1])."""
+    rho = 0.6
+    sc = scenes.closed_room(scenes.material((rho, rho, rho)), scenes.material(scenes.WHITE))
+    cam = _down_camera()
+    c = np.array([0.3, 0.5, 0.7], np.float32)
+    N = 16
+    rgb, nq, nr = O.render_cached(sc, cam, make_settings(spp=N, max_depth=4, seed=5), c, q_vertex=1, train_every=0)
+    P = cam.width * cam.height
+    assert nq == N * P and nr == 0
+    for ch in range(3):
+        sd = 2 * math.pi * rho * c[ch] / math.sqrt(12 * N * P)
+        assert abs(rgb[..., ch].mean() - math.pi * rho * c[ch]) < 4 * sd
+    # a zero cache leaves nothing (the room has no light)
+    rgb0, _, _ = O.render_cached(sc, cam, make_settings(spp=N, max_depth=4, seed=5), np.zeros(3), 1, 0)
+    assert not rgb0.any()
+    # negative predictions count as zero
+    rgbn, _, _ = O.render_cached(sc, cam, make_settings(spp=N, max_depth=4, seed=5), -c, 1, 0)
+    assert not rgbn.any()
+
+
+def test_cached_render_skips_lights_and_last_depth():
+    """A first-bounce vertex on a light ends the path as in the plain render
+    (no query); so does a vertex at the last depth."""
+    rho, E = 0.6, 2.0
+    sc = scenes.closed_room(scenes.material((rho, rho, rho)), scenes.material((E, E, E), light=True))
+    cam = _down_camera(16, 16)
+    s = make_settings(spp=8, max_depth=4, seed=6)
+    plain, _ = O.render(sc, cam, s)
+    cached, nq, nr = O.render_cached(sc, cam, s, np.array([9.0, 9.0, 9.0]), 1, 0)
+    assert nq == 0 and nr == 0
+    np.testing.assert_array_equal(cached[0], plain)
+    sc2 = scenes.closed_room(scenes.material((rho, rho, rho)), scenes.material(scenes.WHITE))
+    s2 = make_settings(spp=8, max_depth=2, seed=6)
+    cached2, nq2, _ = O.render_cached(sc2, cam, s2, np.array([9.0, 9.0, 9.0]), 1, 0)
+    assert nq2 == 0 and not cached2.any()
+
+
+def test_cached_render_training_samples_trace_on():
+    """Training samples follow the plain render's paths; with train_every = 1
+    every sample is one and the image is the plain image bit for bit."""
+    sc = scenes.cornell_box()
+    cam = H.cornell_camera(24, 24)
+    s = make_settings(spp=4, max_depth=5, seed=7)
+    plain, _ = O.render(sc, cam, s)
+    cached, nq, nr = O.render_cached(sc, cam, s, np.array([1.0, 1.0, 1.0]), 1, 1)
+    assert nq == 0 and nr > 0
+    np.testing.assert_array_equal(cached[0], plain)
+    # train_every = 4: about a quarter of the samples that reach the vertex train
+    _, nq4, nr4 = O.render_cached(sc, cam, s, np.array([1.0, 1.0, 1.0]), 1, 4)
+    assert nq4 + nr4 == nr
+    assert abs(nr4 / nr - 0.25) < 0.05
+    # query off: every sample traces its whole path
+    off, nq0, _ = O.render_cached(sc, cam, s, np.array([1.0, 1.0, 1.0]), 1, 4, query=False)
+    assert nq0 == 0
+    np.testing.assert_array_equal(off[0], plain)

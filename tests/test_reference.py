@@ -114,3 +114,19 @@ def test_path_tracer_statistically_matches_reference():
     assert np.all(np.abs(z) < 6), np.abs(z).max()
     zg = (ref.mean() - runs.mean()) / (runs.mean(axis=(1, 2, 3)).std(ddof=1) * math.sqrt(1 + 1 / len(runs)))
     assert abs(zg) < 5, zg
+
+
+def test_occluded_matches_reference():
+    """Scene::isOccluded of the compiled reference against the oracle's brute
+    force on fresh segments, some ending exactly at hit points."""
+    sc = scenes.cornell_box()
+    ref = R.RefScene(sc)
+    rng = np.random.default_rng(77)
+    n = 3000
+    a = rng.uniform(-2.7, 2.7, (n, 3)).astype(np.float32)
+    b = rng.uniform(-2.7, 2.7, (n, 3)).astype(np.float32)
+    rays = _random_rays(n, -2.7, 2.7, 78)
+    hit, p, _, _ = ref.trace(rays)
+    segs = np.concatenate([np.concatenate([a, b], axis=1),
+                           np.concatenate([rays["o"][hit], p[hit]], axis=1)]).astype(np.float32)
+    np.testing.assert_array_equal(O.occluded_brute(sc, segs), ref.occluded(segs))
