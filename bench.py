@@ -49,8 +49,15 @@ def main() -> int:
     ap.add_argument("--leaf", type=int, default=2, help="BVH leaf size (1..16)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 512 MiB device copy (reference kernel for HBM counters)")
-    ap.add_argument("--config", default="raycast", choices=["raycast", "cornell", "terrain", "diff"],
-                    help="raycast: the headline microbench (default); cornell / terrain / diff: the other workloads")
+    ap.add_argument("--config", default="raycast", choices=["raycast", "cornell", "terrain", "diff", "nrc"],
+                    help="raycast: the headline microbench (default); cornell / terrain / diff / nrc: the other "
+                         "workloads")
+    ap.add_argument("--nrc-scene", default="terrain",
+                    choices=["terrain", "cornell_closed", "cornell_indirect", "cornell_box"],
+                    help="nrc: the scene")
+    ap.add_argument("--frames", type=int, default=300, help="nrc: online training frames")
+    ap.add_argument("--ref-spp", type=int, default=256, help="nrc: samples of the light-sampled reference")
+    ap.add_argument("--res", default="1920x1080", help="nrc: image size")
     ap.add_argument("--views", type=int, default=16, help="terrain: camera views batched in one launch")
     ap.add_argument("--spp", type=int, default=0, help="cornell / terrain / diff: override samples per pixel")
     args = ap.parse_args()
@@ -59,7 +66,8 @@ def main() -> int:
               "by range with the BVH replicated on every GPU). Run with --gpus 1.", file=sys.stderr)
         return 2
     if args.config != "raycast":
-        out = {"cornell": bench_cornell, "terrain": bench_terrain, "diff": bench_diff}[args.config](args)
+        out = {"cornell": bench_cornell, "terrain": bench_terrain, "diff": bench_diff,
+               "nrc": bench_nrc}[args.config](args)
         print(json.dumps(out), flush=True)
         return 0
 
@@ -368,5 +376,175 @@ def bench_diff(args):
             "cpu_baseline": cpu}
 
 
+
+
+# floating-point work of the cache network per sample, on its valid (unpadded)
+# inputs and outputs: 78*64 + 4*64*64 + 64*3 = 21,568 multiply-adds forward
+MLP_FWD_FLOP = 2 * (78 * 64 + 4 * 64 * 64 + 64 * 3)
+# training: forward, weight gradients (the same multiply-adds) and input
+# gradients of layers 1..5 (4*64*64 + 64*3)
+MLP_TRAIN_FLOP = 2 * MLP_FWD_FLOP + 2 * (4 * 64 * 64 + 64 * 3)
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def bench_nrc(args):
+    """Learned radiance cache, equal-RMSE protocol.  Scene (default: the
+    terrain, sun + sky, depth 8, 1920x1080), a light-sampled reference of
+    --ref-spp samples (same expectation as the plain estimator; the plain
+    render's mean is reported beside it as the bias judge), then:
+      plain PT          the reference estimator, RMSE at 4 and 16 spp
+      light sampling    the cache mode without the network, 1 and 4 spp
+      cached            --frames frames of online training at 1 spp (about
+                        2^16 training records per frame, 4 Adam steps of
+                        16384), then 1-spp frames with training on
+    Each estimator's RMSE scales as c / sqrt(spp); the time a method needs to
+    reach the cached frame's RMSE is (c / rmse_cached)^2 * (time per spp).
+    The MLP kernels are timed separately on 1920*1080 queries and 65536
+    training samples (MFMA throughput against the BF16 dense peak)."""
+    import torch
+
+    from master_thesis_amd import scenes, host_lib as H
+    from master_thesis_amd._lib import RT_RENDER_SKY, RT_RENDER_SUN
+    from master_thesis_amd.cache import RadianceCache
+    from master_thesis_amd.tracer import Tracer, make_settings
+    W, Hh = (int(x) for x in args.res.split("x"))
+    if args.nrc_scene == "terrain":
+        sc = scenes.terrain_scene()
+        cam = scenes.terrain_cameras(1, W, Hh)[0]
+        depth = 8
+
+        def S(spp, off=0, seed=5):
+            return make_settings(spp=spp, max_depth=depth, seed=seed, flags=RT_RENDER_SKY | RT_RENDER_SUN,
+                                 sample_offset=off, sky=(0.45, 0.55, 0.75), sun_dir=(0.4, 0.8, 0.3),
+                                 sun_radius_deg=5.0, sun_radiance=(40.0, 36.0, 30.0))
+    else:
+        sc = getattr(scenes, args.nrc_scene)()
+        closed = args.nrc_scene == "cornell_closed"
+        cam = H.closed_camera(W, Hh) if closed else H.cornell_camera(W, Hh)
+        depth = 8 if closed else 6
+
+        def S(spp, off=0, seed=5):
+            return make_settings(spp=spp, max_depth=depth, seed=seed, sample_offset=off)
+    P = W * Hh
+    every = max(1, P // 65536)
+    res = {}
+    with Tracer(0) as t:
+        t.upload(sc)
+        t.build(args.leaf)
+        cache = RadianceCache(t, lr=1e-2, seed=1)
+        t0 = time.time()
+        ref, _ = cache.render(cam, S(args.ref_spp, 0, 99), train=False, query=False, train_every=0)
+        log(f"[bench] nrc: light-sampled reference {args.ref_spp} spp in {time.time() - t0:.1f}s")
+
+        def rmse(img):
+            return float(np.sqrt(np.mean((img - ref) ** 2)))
+
+        def run(fn, spp, reps=3):
+            e, ms = [], []
+            for r in range(reps + 1):
+                img, tm = fn(spp, 1000000 + spp * r)
+                if r:
+                    e.append(rmse(img))
+                    ms.append(tm)
+            return float(np.mean(e)), float(np.median(ms))
+
+        def plain(spp, off):
+            img, _ = t.render(cam, S(spp, off))
+            return img, t.timing()["last_call_ms"]
+
+        def light(spp, off):
+            img, st = cache.render(cam, S(spp, off), train=False, query=False, train_every=0)
+            return img, st.render_ms
+
+        for name, fn, spps in (("plain", plain, (4, 16)), ("light_sampling", light, (1, 4))):
+            rows = {}
+            for spp in spps:
+                e, ms = run(fn, spp)
+                rows[spp] = (e, ms)
+                log(f"[bench] nrc: {name} {spp} spp rmse {e:.4f} {ms:.2f} ms")
+            c = float(np.mean([e * np.sqrt(spp) for spp, (e, _) in rows.items()]))
+            ms_spp = rows[spps[-1]][1] / spps[-1]
+            res[name] = {"rmse": {str(k): round(v[0], 5) for k, v in rows.items()},
+                         "ms": {str(k): round(v[1], 3) for k, v in rows.items()},
+                         "c": round(c, 5), "ms_per_spp": round(ms_spp, 4)}
+        plain_hi, _ = t.render(cam, S(max(16, args.ref_spp // 4), 0, 98))
+        # online training
+        cache.close()
+        cache = RadianceCache(t, lr=1e-2, seed=1)
+        t0 = time.time()
+        tms = []
+        for f in range(args.frames):
+            _, st = cache.render(cam, S(1, f), train_every=every, batch=16384, max_steps=4)
+            tms.append(st.render_ms + st.train_ms)
+        log(f"[bench] nrc: {args.frames} training frames in {time.time() - t0:.1f}s, last loss {st.loss:.3f}, "
+            f"{st.records} records, {st.steps} steps")
+
+        def cached(spp, off):
+            img, st = cache.render(cam, S(spp, off), train_every=every, batch=16384, max_steps=4)
+            return img, st.render_ms + st.train_ms
+
+        e1, ms1 = run(cached, 1, reps=4)
+        log(f"[bench] nrc: cached 1 spp rmse {e1:.4f} {ms1:.2f} ms")
+        big, _ = cache.render(cam, S(64, 5000000), train=False, train_every=0)
+        res["cached"] = {"rmse": {"1": round(e1, 5)}, "ms": {"1": round(ms1, 3)},
+                         "frames_ms_median": round(float(np.median(tms)), 3),
+                         "bias": {"mean_diff_vs_reference": round(float((big - ref).mean()), 6),
+                                  "rmse_64spp_vs_reference": round(rmse(big), 5),
+                                  "light_sampling_expected_rmse_64spp": round(
+                                      float(np.hypot(res["light_sampling"]["c"] / 8.0,
+                                                     res["light_sampling"]["c"] / np.sqrt(args.ref_spp))), 5)}}
+        # plain PT as the judge of the reference (and so of the cache's bias): frame means
+        res["judge"] = {"reference_mean": round(float(ref.mean()), 6), "plain_mean": round(float(plain_hi.mean()), 6),
+                        "plain_spp": max(16, args.ref_spp // 4)}
+        # time for each method to reach the cached frame's RMSE
+        ratio = {}
+        for name in ("plain", "light_sampling"):
+            need_spp = (res[name]["c"] / e1) ** 2
+            ratio[name] = need_spp * res[name]["ms_per_spp"] / ms1
+            res[name]["spp_for_equal_rmse"] = round(need_spp, 2)
+        # MLP kernels on the device, timed with events on the context stream
+        stream = torch.cuda.Stream()
+        torch.cuda.set_stream(stream)
+        t.set_stream(stream.cuda_stream)
+        lo, hi = np.asarray(sc.pos.min(axis=0)), np.asarray(sc.pos.max(axis=0))
+        g = torch.Generator(device="cuda").manual_seed(0)
+        feats = torch.rand((P, 9), device="cuda", generator=g)
+        feats[:, :3] = feats[:, :3] * torch.tensor(hi - lo, device="cuda") + torch.tensor(lo, device="cuda")
+        out = torch.empty((P, 3), device="cuda")
+        tgt = torch.rand((65536, 3), device="cuda", generator=g)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        for _ in range(3):
+            cache.infer(feats, out)
+        ev[0].record(stream)
+        for _ in range(20):
+            cache.infer(feats, out)
+        ev[1].record(stream)
+        tf = feats[:65536].contiguous()
+        for _ in range(3):
+            cache.train(tf, tgt, want_loss=False)
+        ev[2].record(stream)
+        for _ in range(20):
+            cache.train(tf, tgt, want_loss=False)
+        ev[3].record(stream)
+        torch.cuda.synchronize()
+        t.set_stream(None)
+        inf_ms = ev[0].elapsed_time(ev[1]) / 20
+        trn_ms = ev[2].elapsed_time(ev[3]) / 20
+        inf_tf = MLP_FWD_FLOP * P / (inf_ms * 1e-3) / 1e12
+        trn_tf = MLP_TRAIN_FLOP * 65536 / (trn_ms * 1e-3) / 1e12
+        cache.close()
+    return {"metric": f"equal-RMSE wall time, plain GPU path tracer / cached form ({args.nrc_scene}, {W}x{Hh})",
+            "value": round(ratio["plain"], 2), "unit": "x", "higher_is_better": True, "n_gpus": 1,
+            "dtype": "f32 (paths), bf16 MFMA with f32 accumulation (network)", "data": "synthetic",
+            "network_gain_over_light_sampling": round(ratio["light_sampling"], 3),
+            "estimators": res, "training_frames": args.frames, "train_every": every,
+            "mlp": {"infer_ms_per_frame": round(inf_ms, 4), "infer_tflops": round(inf_tf, 1),
+                    "train_ms_per_step_65536": round(trn_ms, 4), "train_tflops": round(trn_tf, 1)},
+            "roofline": {"bound": "mfma", "achieved": round(inf_tf, 1), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(inf_tf / BF16_PEAK_TFLOPS, 4), "traffic": None,
+                         "kernel": "k_mlp_infer", "flop_per_sample": MLP_FWD_FLOP},
+            "config": {"workload": f"{args.nrc_scene}, {W}x{Hh}, depth {depth}, {args.frames} frames of online "
+                                   f"training at 1 spp, reference {args.ref_spp} spp light-sampled",
+                       "triangles": int(sc.num_triangles)}}
 if __name__ == "__main__":
     sys.exit(main())

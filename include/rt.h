@@ -301,6 +301,7 @@ int  rt_nrc_train(rt_nrc* nrc, const float* feats, const float* targets, uint32_
 #define RT_NRC_NO_TRAIN 0x1        /* records are made but no step is taken    */
 #define RT_NRC_NO_QUERY 0x2        /* every sample traces its whole path        */
 #define RT_NRC_NO_RECORDS 0x4      /* no training samples (train_every ignored) */
+#define RT_NRC_NO_NEE 0x8          /* no light sampling: the reference estimator */
 typedef struct rt_nrc_render {
     uint32_t query_vertex;         /* >= 1; 0 selects 1                         */
     uint32_t train_every;          /* 0 selects 16                              */
@@ -314,12 +315,17 @@ typedef struct rt_nrc_stats {
     uint32_t steps;                /* Adam steps taken                          */
     float loss;                    /* mean loss of the last step (0 if none)    */
     float render_ms, train_ms;     /* device time of the two phases             */
+    uint64_t shadow_rays;          /* light-sample occlusion rays traced        */
 } rt_nrc_stats;
 /* Renders num_views frames like rt_render_views with the cache (which must
  * belong to ctx); stats may be NULL.  No counterpart in the software
  * renderer (see the cache section above). */
 int  rt_render_nrc(rt_ctx* ctx, rt_nrc* nrc, const rt_camera* cams, uint32_t num_views, const rt_settings* s,
                    const rt_nrc_render* opts, float* rgb, uint8_t* rgb8, int ptr_kind, rt_nrc_stats* stats);
+/* Copies the training records of the last rt_render_nrc (in no particular
+ * order) to host memory: feats 9 and targets 3 floats per record; *count is
+ * the in-out capacity (both NULL: query the count). */
+int  rt_render_nrc_records(rt_ctx* ctx, float* feats, float* targets, uint32_t* count);
 
 #ifdef __cplusplus
 }

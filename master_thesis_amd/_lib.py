@@ -131,6 +131,7 @@ RT_NRC_FEATURES = 9
 RT_NRC_NO_TRAIN = 0x1
 RT_NRC_NO_QUERY = 0x2
 RT_NRC_NO_RECORDS = 0x4
+RT_NRC_NO_NEE = 0x8
 
 
 class rt_nrc_render(C.Structure):
@@ -140,7 +141,7 @@ class rt_nrc_render(C.Structure):
 
 class rt_nrc_stats(C.Structure):
     _fields_ = [("queries", C.c_uint64), ("records", C.c_uint64), ("steps", C.c_uint32), ("loss", C.c_float),
-                ("render_ms", C.c_float), ("train_ms", C.c_float)]
+                ("render_ms", C.c_float), ("train_ms", C.c_float), ("shadow_rays", C.c_uint64)]
 
 # every exported symbol of include/rt.h with its signature
 _P = C.c_void_p
@@ -166,6 +167,7 @@ SIGNATURES = {
     "rt_nrc_infer": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_int]),
     "rt_nrc_gradient": (C.c_int, [_P, _P, _P, C.c_uint32, _P, C.POINTER(C.c_float), C.c_int]),
     "rt_nrc_train": (C.c_int, [_P, _P, _P, C.c_uint32, C.POINTER(C.c_float), C.c_int]),
+    "rt_render_nrc_records": (C.c_int, [_P, _P, _P, C.POINTER(C.c_uint32)]),
     "rt_render_nrc": (C.c_int, [_P, _P, C.POINTER(rt_camera), C.c_uint32, C.POINTER(rt_settings),
                                 C.POINTER(rt_nrc_render), _P, _P, C.c_int, C.POINTER(rt_nrc_stats)]),
 }

@@ -94,8 +94,17 @@ class RadianceCache:
                                           L.RT_PTR_DEVICE if dev else L.RT_PTR_HOST))
         return loss.value if want_loss else None
 
+    def last_records(self):
+        """(feats (n, 9), targets (n, 3)) of the last render's training records."""
+        n = C.c_uint32(0)
+        self._check(self.lib.rt_render_nrc_records(self.tracer.h, None, None, C.byref(n)))
+        f = np.zeros((n.value, 9), np.float32)
+        t = np.zeros((n.value, 3), np.float32)
+        self._check(self.lib.rt_render_nrc_records(self.tracer.h, L._ptr(f), L._ptr(t), C.byref(n)))
+        return f, t
+
     def render(self, cams, settings, query_vertex: int = 1, train_every: int = 16, batch: int = 16384,
-               max_steps: int = 4, train: bool = True, query: bool = True, out=None):
+               max_steps: int = 4, train: bool = True, query: bool = True, nee: bool = True, out=None):
         """One frame of rt_render_nrc for a camera or a list of cameras.
         train_every = 0: no training samples.  Returns (rgb, stats): rgb
         (H, W, 3) or (V, H, W, 3) float32 (numpy, or the torch tensor `out`
@@ -107,7 +116,7 @@ class RadianceCache:
         H, W = cl[0].height, cl[0].width
         o = L.rt_nrc_render(query_vertex, train_every, batch, max_steps,
                             (0 if train else L.RT_NRC_NO_TRAIN) | (0 if query else L.RT_NRC_NO_QUERY) |
-                            (L.RT_NRC_NO_RECORDS if train_every == 0 else 0))
+                            (L.RT_NRC_NO_RECORDS if train_every == 0 else 0) | (0 if nee else L.RT_NRC_NO_NEE))
         st = L.rt_nrc_stats()
         dev = out is not None and L._is_device(out)
         if out is None:

@@ -56,10 +56,13 @@ namespace {
 constexpr int PT_BLOCK = 256;
 constexpr uint32_t ADJ_MAX_DEPTH = 16;
 constexpr uint32_t CULL_BIT = 0x10000u;
-// cache mode: front-facing hits so far (bits 17-23), training record open (bit 24)
+// cache mode: front-facing hits so far (bits 17-23), throughput-from-the-
+// query-vertex tracked (bit 24), last bounce diffuse (bit 25)
 constexpr uint32_t FH_SHIFT = 17;
 constexpr uint32_t FH_MASK = 0x7Fu;
 constexpr uint32_t REC_BIT = 1u << 24;
+constexpr uint32_t DIFF_BIT = 1u << 25;
+constexpr uint32_t NO_REC = 0xFFFFFFFFu;
 
 // ctr slots
 constexpr int C_ITEM = 0;   // next work item
@@ -69,7 +72,9 @@ constexpr int C_RAYS = 4;   // 64-bit count of traced rays (slots 4-5)
 constexpr int C_QN = 6;     // cache queries queued since the last flush
 constexpr int C_TN = 7;     // training records made
 constexpr int C_QDROP = 8;  // queries dropped for want of room (stays 0: the host flushes in time)
-constexpr int C_WORDS = 9;
+constexpr int C_SN = 9;     // shadow rays queued this iteration
+constexpr int C_SHRAYS = 10;  // 64-bit count of shadow rays (slots 10-11)
+constexpr int C_WORDS = 12;
 
 struct PTArgs {
     const ShadeGeom* geom;
@@ -113,8 +118,17 @@ struct PTArgs {
     float* t_feat;      // t_cap * 9
     float* t_tgt;       // t_cap * 3
     uint32_t t_cap;
-    float* s_qf;        // 9 * S: training sample's features at its query vertex
+    uint32_t* s_ridx;   // S: training record of the slot's sample (NO_REC: none)
     float* s_Tq;        // 3 * S: its throughput from the query vertex on
+    // light sampling (cache mode): shadow rays to emissive triangles and the sun
+    uint32_t nee, sun_on;
+    const LightTri* lights;
+    uint32_t num_lights;
+    float light_area;
+    rt_ray* sh_ray;     // S
+    float* sh_w;        // 6 * S: pixel weight, record weight
+    uint32_t* sh_idx;   // 2 * S: partial-sum index, record index
+    uint8_t* sh_occ;    // S
 };
 
 // Training-sample choice: lowbias32 (C. Wellons) over pixel, sample, view, seed.
@@ -154,13 +168,35 @@ __device__ __forceinline__ void start_item(const PTArgs& a, uint32_t slot, uint3
 }
 
 // Radiance of an escaped ray (extension; the reference's miss is black):
-// constant sky plus the sun inside its cone.
-__device__ __forceinline__ f3 miss_radiance(const PTArgs& a, f3 d) {
+// constant sky plus, when `sun` is set, the sun inside its cone.
+__device__ __forceinline__ f3 miss_radiance(const PTArgs& a, f3 d, bool sun) {
     f3 L = mk(0.0f, 0.0f, 0.0f);
     if (a.flags & RT_RENDER_SKY) L = mk(a.sky[0], a.sky[1], a.sky[2]);
-    if ((a.flags & RT_RENDER_SUN) && dot(d, mk(a.sun_dir[0], a.sun_dir[1], a.sun_dir[2])) >= a.sun_cos)
+    if (sun && (a.flags & RT_RENDER_SUN) && dot(d, mk(a.sun_dir[0], a.sun_dir[1], a.sun_dir[2])) >= a.sun_cos)
         L = add(L, mk(a.sun_rad[0], a.sun_rad[1], a.sun_rad[2]));
     return L;
+}
+
+// Hit point, interpolated shading normal and material of a hit:
+// Triangle::getBarycentricCoords (triangle.cpp:47-55) and Scene::traceRay's
+// normal (scene.cpp:49-53).
+__device__ __forceinline__ void surface(const PTArgs& a, const rt_hit& h, f3 o, f3 d, f3& p, f3& nrm, uint32_t& mat) {
+    p = hit_point(o, d, h.t);
+    const ShadeGeom g = a.geom[h.tri];
+    const ShadeNrm nn = a.nrm[h.tri];
+    const f3 v0 = mk(g.v0[0], g.v0[1], g.v0[2]);
+    const f3 e0 = mk(g.e0[0], g.e0[1], g.e0[2]), e1 = mk(g.e1[0], g.e1[1], g.e1[2]);
+    const f3 e2 = sub(p, v0);
+    const float d20 = dot(e2, e0), d21 = dot(e2, e1);
+    const float bv = (g.d11 * d20 - g.d01 * d21) / g.denom;
+    const float bw = (g.d00 * d21 - g.d01 * d20) / g.denom;
+    const float bu = 1.0f - bv - bw;
+    nrm = mk(0.0f, 0.0f, 0.0f);
+    nrm = add(nrm, scale(bu, mk(nn.n0[0], nn.n0[1], nn.n0[2])));
+    nrm = add(nrm, scale(bv, mk(nn.n1[0], nn.n1[1], nn.n1[2])));
+    nrm = add(nrm, scale(bw, mk(nn.n2[0], nn.n2[1], nn.n2[2])));
+    nrm = normalize_ref(nrm);
+    mat = g.mat;
 }
 
 template <bool ADJ>
@@ -171,6 +207,8 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_init(PTArgs a) {
         a.ctr[C_N0] = a.S;
         a.ctr[C_N1] = 0;
         *(unsigned long long*)&a.ctr[C_RAYS] = 0ull;
+        *(unsigned long long*)&a.ctr[C_SHRAYS] = 0ull;
+        a.ctr[C_SN] = 0;
         a.ctr[C_QN] = 0;
         a.ctr[C_TN] = 0;
         a.ctr[C_QDROP] = 0;
@@ -213,7 +251,7 @@ __device__ __forceinline__ void scatter_grad(const PTArgs& a, uint32_t slot, uin
     }
 }
 
-template <bool ADJ, bool NRC>
+template <bool ADJ>
 __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
     const uint32_t n = a.ctr[cur ? C_N1 : C_N0];
     uint32_t* n_out = &a.ctr[cur ? C_N0 : C_N1];
@@ -224,11 +262,6 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
     bool alive = false, need_item = false;
     uint32_t slot = 0;
     rt_ray nr;
-    // cache mode: a query or a training record to enqueue
-    bool push_q = false, push_t = false;
-    float qf[9];
-    f3 qw = mk(0.0f, 0.0f, 0.0f);
-    uint32_t qidx = 0;
     if (k < n) {
         slot = a.q_slot[cur][k];
         const rt_ray r = a.q_ray[cur][k];
@@ -241,11 +274,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
         uint32_t st = a.s_state[slot];
         uint32_t depth = st & 0xFFFFu;
         bool cull = (st & CULL_BIT) != 0;
-        uint32_t fh = (st >> FH_SHIFT) & FH_MASK;
-        bool rec = NRC && (st & REC_BIT) != 0;
         f3 T = mk(a.s_T[3 * slot], a.s_T[3 * slot + 1], a.s_T[3 * slot + 2]);
-        f3 Tq = mk(1.0f, 1.0f, 1.0f);
-        if (rec) Tq = mk(a.s_Tq[3 * slot], a.s_Tq[3 * slot + 1], a.s_Tq[3 * slot + 2]);
         f3 o = mk(r.ox, r.oy, r.oz), d = mk(r.dx, r.dy, r.dz);
         bool end = false, contrib = false;
         f3 term = mk(1.0f, 1.0f, 1.0f);
@@ -253,53 +282,21 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
             end = true;
             if (a.flags & (RT_RENDER_SKY | RT_RENDER_SUN)) {
                 contrib = true;
-                term = miss_radiance(a, d);
+                term = miss_radiance(a, d, true);
                 T = mulv(T, term);
-                if (rec) Tq = mulv(Tq, term);
             }
         } else {
-            const f3 p = hit_point(o, d, h.t);
-            const ShadeGeom g = a.geom[h.tri];
-            const ShadeNrm nn = a.nrm[h.tri];
-            // Triangle::getBarycentricCoords (triangle.cpp:47-55)
-            const f3 v0 = mk(g.v0[0], g.v0[1], g.v0[2]);
-            const f3 e0 = mk(g.e0[0], g.e0[1], g.e0[2]), e1 = mk(g.e1[0], g.e1[1], g.e1[2]);
-            const f3 e2 = sub(p, v0);
-            const float d20 = dot(e2, e0), d21 = dot(e2, e1);
-            const float bv = (g.d11 * d20 - g.d01 * d21) / g.denom;
-            const float bw = (g.d00 * d21 - g.d01 * d20) / g.denom;
-            const float bu = 1.0f - bv - bw;
-            // Scene::traceRay normal (scene.cpp:49-53)
-            f3 nrm = mk(0.0f, 0.0f, 0.0f);
-            nrm = add(nrm, scale(bu, mk(nn.n0[0], nn.n0[1], nn.n0[2])));
-            nrm = add(nrm, scale(bv, mk(nn.n1[0], nn.n1[1], nn.n1[2])));
-            nrm = add(nrm, scale(bw, mk(nn.n2[0], nn.n2[1], nn.n2[2])));
-            nrm = normalize_ref(nrm);
+            f3 p, nrm;
+            uint32_t mid;
+            surface(a, h, o, d, p, nrm, mid);
             const float ndotd = dot(nrm, d);
             if (cull && ndotd > 0.0f) {
                 o = add(p, scale(0.01f, d));
             } else {
                 cull = false;
-                const rt_material m = a.mat[g.mat];
+                const rt_material m = a.mat[mid];
                 const uint32_t gsmp = a.sample_offset + chunk * a.spc + smp;
-                const bool light_end = m.light_source && dot(nrm, d) <= 0.0f;
-                if (NRC && fh == a.nrc_q && !light_end && depth + 1 < a.max_depth) {
-                    const bool train = a.nrc_every != 0 && train_hash(pix, gsmp, view, a.seed_lo) % a.nrc_every == 0;
-                    const float f9[9] = {p.x, p.y, p.z, nrm.x, nrm.y, nrm.z, d.x, d.y, d.z};
-                    if (train) {
-                        rec = true;
-                        Tq = mk(1.0f, 1.0f, 1.0f);
-                        for (int j = 0; j < 9; ++j) a.s_qf[9 * (size_t)slot + j] = f9[j];
-                    } else if (a.nrc_query) {
-                        push_q = true;
-                        for (int j = 0; j < 9; ++j) qf[j] = f9[j];
-                        qw = T;
-                        qidx = chunk * a.VP + vp;
-                        end = true;
-                    }
-                }
-                ++fh;
-                if (!push_q) {
+                const bool light_end = m.light_source && ndotd <= 0.0f;
                 u4 c4;
                 c4.x = pix; c4.y = gsmp; c4.z = depth; c4.w = view;
                 const u4 rnd = philox4x32_10(c4, a.seed_lo, a.seed_hi);
@@ -312,11 +309,10 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
                 if (diffuse) {
                     const float cs = dot(nrm, d);
                     T = mulv(T, scale(cs, mk(m.color[0], m.color[1], m.color[2])));
-                    if (rec) Tq = mulv(Tq, scale(cs, mk(m.color[0], m.color[1], m.color[2])));
                     if (ADJ) {
                         uint32_t nv = a.s_nv[slot];
                         if (nv < ADJ_MAX_DEPTH) {
-                            a.s_amat[nv * a.S + slot] = g.mat;
+                            a.s_amat[nv * a.S + slot] = mid;
                             a.s_acos[nv * a.S + slot] = cs;
                         }
                         a.s_nv[slot] = nv + 1;
@@ -325,7 +321,6 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
                 if (light_end) {
                     end = true;
                     contrib = true;
-                }
                 }
             }
             ++depth;
@@ -341,24 +336,13 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
                     a.s_sum[3 * slot + 2] = a.s_sum[3 * slot + 2] + T.z;
                 }
             }
-            if (rec) {
-                push_t = true;
-                for (int j = 0; j < 9; ++j) qf[j] = a.s_qf[9 * (size_t)slot + j];
-                qw = contrib ? Tq : mk(0.0f, 0.0f, 0.0f);
-            }
             ++smp;
             if (smp < a.spc) {
                 a.s_smp[slot] = smp;
                 start_sample(a, slot, vp, nr, ADJ);
                 alive = true;
             } else {
-                if (NRC) {
-                    // cache predictions land in the same sums
-                    const size_t q = (size_t)chunk * a.VP + vp;
-                    atomicAdd(&a.partial[3 * q], a.s_sum[3 * slot]);
-                    atomicAdd(&a.partial[3 * q + 1], a.s_sum[3 * slot + 1]);
-                    atomicAdd(&a.partial[3 * q + 2], a.s_sum[3 * slot + 2]);
-                } else if (!ADJ) {
+                if (!ADJ) {
                     const size_t q = (size_t)chunk * a.VP + vp;
                     a.partial[3 * q] = a.s_sum[3 * slot];
                     a.partial[3 * q + 1] = a.s_sum[3 * slot + 1];
@@ -367,47 +351,10 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
                 need_item = true;
             }
         } else {
-            a.s_state[slot] = depth | (cull ? CULL_BIT : 0u) | (NRC ? (fh << FH_SHIFT) | (rec ? REC_BIT : 0u) : 0u);
+            a.s_state[slot] = depth | (cull ? CULL_BIT : 0u);
             a.s_T[3 * slot] = T.x; a.s_T[3 * slot + 1] = T.y; a.s_T[3 * slot + 2] = T.z;
-            if (rec) {
-                a.s_Tq[3 * slot] = Tq.x; a.s_Tq[3 * slot + 1] = Tq.y; a.s_Tq[3 * slot + 2] = Tq.z;
-            }
             nr = make_ray(o, d);
             alive = true;
-        }
-    }
-    if (NRC) {
-        // enqueue cache queries and training records: one atomic per wave each
-        const uint64_t qb = __ballot(push_q);
-        if (qb) {
-            const int leader = __ffsll((unsigned long long)qb) - 1;
-            uint32_t base = 0;
-            if ((int)lane == leader) base = atomicAdd(&a.ctr[C_QN], (uint32_t)__popcll(qb));
-            base = __shfl(base, leader);
-            if (push_q) {
-                const uint32_t pos = base + (uint32_t)__popcll(qb & lt);
-                if (pos < a.q_cap) {
-                    for (int j = 0; j < 9; ++j) a.q_feat[9 * (size_t)pos + j] = qf[j];
-                    a.q_w[3 * (size_t)pos] = qw.x; a.q_w[3 * (size_t)pos + 1] = qw.y; a.q_w[3 * (size_t)pos + 2] = qw.z;
-                    a.q_idx[pos] = qidx;
-                } else {
-                    atomicAdd(&a.ctr[C_QDROP], 1u);
-                }
-            }
-        }
-        const uint64_t tb = __ballot(push_t);
-        if (tb) {
-            const int leader = __ffsll((unsigned long long)tb) - 1;
-            uint32_t base = 0;
-            if ((int)lane == leader) base = atomicAdd(&a.ctr[C_TN], (uint32_t)__popcll(tb));
-            base = __shfl(base, leader);
-            if (push_t) {
-                const uint32_t pos = base + (uint32_t)__popcll(tb & lt);
-                if (pos < a.t_cap) {  // records past the capacity are dropped
-                    for (int j = 0; j < 9; ++j) a.t_feat[9 * (size_t)pos + j] = qf[j];
-                    a.t_tgt[3 * (size_t)pos] = qw.x; a.t_tgt[3 * (size_t)pos + 1] = qw.y; a.t_tgt[3 * (size_t)pos + 2] = qw.z;
-                }
-            }
         }
     }
     // next work item for slots whose item finished: one atomic per wave
@@ -438,6 +385,354 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
             a.q_ray[cur ^ 1][pos] = nr;
         }
     }
+}
+
+// ---------------------------------------------------------------- cache mode
+//
+// Light sampling.  At a vertex x (shading normal n, material m with diffuse
+// probability td and colour c) the reference estimator's diffuse branch
+// draws w uniformly on the hemisphere (pdf 1/(2 pi)) and multiplies the
+// throughput by c (n.w); a next hit y on a light from the front then ends the
+// path with the light's own factor, whose expectation is
+//   R_L = td_L * c_L / 2 + (1 - td_L)
+// (a uniform hemisphere cosine averages 1/2; the other branches multiply by
+// 1), and an escape inside the sun's cone adds the sun radiance.  The
+// expected value of those terms is estimated here instead:
+//   light: y uniform on the emissive area A (triangle by area, point by
+//          sqrt-warped barycentrics), w = (y - x)/|y - x|
+//          C = td c (n.w) R_L (-n_y.w) A / (2 pi |y - x|^2)
+//   sun:   w uniform in the cone (solid angle 2 pi (1 - cos_sun))
+//          C = td c (n.w) sun_radiance (1 - cos_sun)
+// times the number of strategies present when one is picked at random, and
+// an occlusion test from x + 0.01 w (the estimator's own offset).  The BSDF
+// path then skips what was estimated: a front light hit after a diffuse
+// bounce ends the path with no contribution, and an escape after a diffuse
+// bounce gets the sky without the sun.  A light hit that is not estimated
+// (seen directly, or after a mirror or refraction) contributes T * R_L, the
+// expectation of the light's own random factor.  Both forms have the same
+// expected pixel value, so the plain render stays the judge of the cache's
+// bias.
+__device__ __forceinline__ f3 light_factor(const rt_material& lm) {
+    const float tdl = lm.diffuse_threshold;
+    return mk(tdl * lm.color[0] * 0.5f + (1.0f - tdl), tdl * lm.color[1] * 0.5f + (1.0f - tdl),
+              tdl * lm.color[2] * 0.5f + (1.0f - tdl));
+}
+
+__device__ __forceinline__ bool nee_sample(const PTArgs& a, f3 p, f3 nrm, const rt_material& m, u4 rnd, rt_ray& sh,
+                                           f3& C) {
+    const uint32_t nls = (a.num_lights > 0 ? 1u : 0u) + (a.sun_on ? 1u : 0u);
+    if (nls == 0 || !(m.diffuse_threshold > 0.0f)) return false;
+    const bool use_light = a.num_lights > 0 && (!a.sun_on || u01(rnd.x) < 0.5f);
+    const f3 col = mk(m.color[0], m.color[1], m.color[2]);
+    if (use_light) {
+        // smallest k with cdf[k] > u
+        const float u = u01(rnd.y);
+        uint32_t lo = 0, hi = a.num_lights - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (a.lights[mid].cdf > u) hi = mid;
+            else lo = mid + 1;
+        }
+        const uint32_t t = a.lights[lo].tri;
+        const ShadeGeom g = a.geom[t];
+        const ShadeNrm nn = a.nrm[t];
+        const float su = sqrtf(u01(rnd.z)), u2 = u01(rnd.w);
+        const float b1 = su * (1.0f - u2), b2 = su * u2, b0 = 1.0f - su;
+        const f3 y = add(add(mk(g.v0[0], g.v0[1], g.v0[2]), scale(b1, mk(g.e0[0], g.e0[1], g.e0[2]))),
+                         scale(b2, mk(g.e1[0], g.e1[1], g.e1[2])));
+        f3 ny = scale(b0, mk(nn.n0[0], nn.n0[1], nn.n0[2]));
+        ny = add(ny, scale(b1, mk(nn.n1[0], nn.n1[1], nn.n1[2])));
+        ny = add(ny, scale(b2, mk(nn.n2[0], nn.n2[1], nn.n2[2])));
+        ny = normalize_ref(ny);
+        const f3 wv = sub(y, p);
+        const float d2 = dot(wv, wv);
+        const float dist = sqrtf(d2);
+        if (!(dist > 0.02f)) return false;
+        const f3 w = scale(1.0f / dist, wv);
+        const float cx = dot(nrm, w), cy = -dot(ny, w);
+        if (!(cx > 0.0f) || !(cy > 0.0f)) return false;
+        const f3 RL = light_factor(a.mat[g.mat]);
+        const float f = m.diffuse_threshold * cx * cy * a.light_area * (float)nls / (6.28318530717958647f * d2);
+        C = scale(f, mulv(col, RL));
+        sh = make_ray(add(p, scale(0.01f, w)), w);
+        sh.tmax = (dist - 0.01f) * 0.9990234375f;  // stop short of the light itself
+    } else {
+        const f3 s = mk(a.sun_dir[0], a.sun_dir[1], a.sun_dir[2]);
+        const float ct = 1.0f - u01(rnd.y) * (1.0f - a.sun_cos);
+        const float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+        float sp, cp;
+        sincos_2pi(u01(rnd.z), sp, cp);
+        // orthonormal basis around s (Duff et al., "Building an Orthonormal
+        // Basis, Revisited", JCGT 2017)
+        const float sg = copysignf(1.0f, s.z);
+        const float ia = -1.0f / (sg + s.z);
+        const float ib = s.x * s.y * ia;
+        const f3 t1 = mk(1.0f + sg * s.x * s.x * ia, sg * ib, -sg * s.x);
+        const f3 t2 = mk(ib, sg + s.y * s.y * ia, -s.y);
+        const f3 w = add(scale(ct, s), scale(st, add(scale(cp, t1), scale(sp, t2))));
+        const float cx = dot(nrm, w);
+        if (!(cx > 0.0f)) return false;
+        const float f = m.diffuse_threshold * cx * (1.0f - a.sun_cos) * (float)nls;
+        C = scale(f, mulv(col, mk(a.sun_rad[0], a.sun_rad[1], a.sun_rad[2])));
+        sh = make_ray(add(p, scale(0.01f, w)), w);
+    }
+    return true;
+}
+
+__device__ __forceinline__ void add3(float* dst, f3 v) {
+    atomicAdd(dst, v.x);
+    atomicAdd(dst + 1, v.y);
+    atomicAdd(dst + 2, v.z);
+}
+
+// Shade step of rt_render_nrc: the plain state machine plus the query
+// vertex, training records and light sampling.
+__global__ void __launch_bounds__(PT_BLOCK) k_pt_shade_nrc(PTArgs a, int cur) {
+    const uint32_t n = a.ctr[cur ? C_N1 : C_N0];
+    uint32_t* n_out = &a.ctr[cur ? C_N0 : C_N1];
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) *(unsigned long long*)&a.ctr[C_RAYS] += n;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    bool alive = false, need_item = false;
+    uint32_t slot = 0;
+    rt_ray nr;
+    // work this step enqueues
+    bool push_q = false, want_rec = false, push_sh = false;
+    float f9[9];
+    f3 qw = mk(0.0f, 0.0f, 0.0f), shw_pix = mk(0.0f, 0.0f, 0.0f), shw_rec = mk(0.0f, 0.0f, 0.0f);
+    uint32_t pidx = 0, ridx = NO_REC;
+    bool sh_to_rec = false;
+    rt_ray sh;
+    if (k < n) {
+        slot = a.q_slot[cur][k];
+        const rt_ray r = a.q_ray[cur][k];
+        const rt_hit h = a.hits[k];
+        const uint32_t item = a.s_item[slot];
+        const uint32_t vp = item % a.VP;
+        const uint32_t chunk = item / a.VP;
+        const uint32_t view = vp / a.P, pix = vp % a.P;
+        pidx = chunk * a.VP + vp;
+        uint32_t smp = a.s_smp[slot];
+        const uint32_t st = a.s_state[slot];
+        uint32_t depth = st & 0xFFFFu;
+        bool cull = (st & CULL_BIT) != 0;
+        uint32_t fh = (st >> FH_SHIFT) & FH_MASK;
+        bool rec = (st & REC_BIT) != 0;
+        const bool prev_diff = (st & DIFF_BIT) != 0;
+        bool diff_now = false;
+        f3 T = mk(a.s_T[3 * slot], a.s_T[3 * slot + 1], a.s_T[3 * slot + 2]);
+        f3 Tq = mk(1.0f, 1.0f, 1.0f);
+        if (rec) {
+            Tq = mk(a.s_Tq[3 * slot], a.s_Tq[3 * slot + 1], a.s_Tq[3 * slot + 2]);
+            ridx = a.s_ridx[slot];
+        }
+        f3 o = mk(r.ox, r.oy, r.oz), d = mk(r.dx, r.dy, r.dz);
+        bool end = false, contrib = false;
+        if (h.tri == RT_MISS) {
+            end = true;
+            if (a.flags & (RT_RENDER_SKY | RT_RENDER_SUN)) {
+                contrib = true;
+                const f3 L = miss_radiance(a, d, !(a.nee && prev_diff));
+                T = mulv(T, L);
+                Tq = mulv(Tq, L);
+            }
+        } else {
+            f3 p, nrm;
+            uint32_t mid;
+            surface(a, h, o, d, p, nrm, mid);
+            const float ndotd = dot(nrm, d);
+            if (cull && ndotd > 0.0f) {
+                o = add(p, scale(0.01f, d));
+            } else {
+                cull = false;
+                const rt_material m = a.mat[mid];
+                const uint32_t gsmp = a.sample_offset + chunk * a.spc + smp;
+                const bool light_end = m.light_source && ndotd <= 0.0f;
+                bool stop = false;
+                if (light_end && a.nee) {
+                    end = true;
+                    stop = true;
+                    if (!prev_diff) {  // not estimated by a light sample: the light's expected factor
+                        const f3 RL = light_factor(m);
+                        T = mulv(T, RL);
+                        Tq = mulv(Tq, RL);
+                        contrib = true;
+                    }
+                } else if (!light_end && fh == a.nrc_q && depth + 1 < a.max_depth) {
+                    const bool train = a.nrc_every != 0 && train_hash(pix, gsmp, view, a.seed_lo) % a.nrc_every == 0;
+                    f9[0] = p.x; f9[1] = p.y; f9[2] = p.z;
+                    f9[3] = nrm.x; f9[4] = nrm.y; f9[5] = nrm.z;
+                    f9[6] = d.x; f9[7] = d.y; f9[8] = d.z;
+                    if (train) {
+                        want_rec = true;
+                        rec = true;
+                        Tq = mk(1.0f, 1.0f, 1.0f);
+                    } else if (a.nrc_query) {
+                        push_q = true;
+                        qw = T;
+                        end = true;
+                        stop = true;
+                    }
+                }
+                ++fh;
+                if (!stop) {
+                    u4 c4;
+                    c4.x = pix; c4.y = gsmp; c4.z = depth; c4.w = view;
+                    if (a.nee && !light_end && depth + 1 < a.max_depth) {
+                        u4 c5 = c4;
+                        c5.w = view | 0x80000000u;
+                        f3 Cn;
+                        if (nee_sample(a, p, nrm, m, philox4x32_10(c5, a.seed_lo, a.seed_hi), sh, Cn)) {
+                            push_sh = true;
+                            shw_pix = mulv(T, Cn);
+                            sh_to_rec = rec;
+                            shw_rec = mulv(Tq, Cn);
+                        }
+                    }
+                    const u4 rnd = philox4x32_10(c4, a.seed_lo, a.seed_hi);
+                    const float xi0 = u01(rnd.x);
+                    const bool diffuse = xi0 <= m.diffuse_threshold;
+                    if (diffuse) d = hemisphere_dir(u01(rnd.y), u01(rnd.z), nrm);
+                    else if (xi0 <= m.reflect_threshold) d = reflect_ref(d, nrm);
+                    else if (xi0 <= m.transparent_threshold) d = custom_refract_ref(d, nrm, m.refraction_index);
+                    o = add(p, scale(0.01f, d));
+                    if (diffuse) {
+                        const f3 f = scale(dot(nrm, d), mk(m.color[0], m.color[1], m.color[2]));
+                        T = mulv(T, f);
+                        Tq = mulv(Tq, f);
+                    }
+                    diff_now = diffuse;
+                    if (light_end) {
+                        end = true;
+                        contrib = true;
+                    }
+                }
+            }
+            ++depth;
+            if (depth >= a.max_depth) end = true;
+        }
+        if (end) {
+            if (contrib) {
+                a.s_sum[3 * slot] = a.s_sum[3 * slot] + T.x;
+                a.s_sum[3 * slot + 1] = a.s_sum[3 * slot + 1] + T.y;
+                a.s_sum[3 * slot + 2] = a.s_sum[3 * slot + 2] + T.z;
+                if (rec && !want_rec && ridx != NO_REC) add3(&a.t_tgt[3 * (size_t)ridx], Tq);
+            }
+            ++smp;
+            if (smp < a.spc) {
+                a.s_smp[slot] = smp;
+                start_sample(a, slot, vp, nr, false);
+                alive = true;
+            } else {
+                // cache predictions and light samples land in the same sums
+                add3(&a.partial[3 * (size_t)pidx], mk(a.s_sum[3 * slot], a.s_sum[3 * slot + 1], a.s_sum[3 * slot + 2]));
+                need_item = true;
+            }
+        } else {
+            a.s_state[slot] = depth | (cull ? CULL_BIT : 0u) | (fh << FH_SHIFT) | (rec ? REC_BIT : 0u) |
+                              (diff_now ? DIFF_BIT : 0u);
+            a.s_T[3 * slot] = T.x; a.s_T[3 * slot + 1] = T.y; a.s_T[3 * slot + 2] = T.z;
+            if (rec) {
+                a.s_Tq[3 * slot] = Tq.x; a.s_Tq[3 * slot + 1] = Tq.y; a.s_Tq[3 * slot + 2] = Tq.z;
+            }
+            nr = make_ray(o, d);
+            alive = true;
+        }
+    }
+    // cache queries: one atomic per wave
+    const uint64_t qb = __ballot(push_q);
+    if (qb) {
+        const int leader = __ffsll((unsigned long long)qb) - 1;
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(&a.ctr[C_QN], (uint32_t)__popcll(qb));
+        base = __shfl(base, leader);
+        if (push_q) {
+            const uint32_t pos = base + (uint32_t)__popcll(qb & lt);
+            if (pos < a.q_cap) {
+                for (int j = 0; j < 9; ++j) a.q_feat[9 * (size_t)pos + j] = f9[j];
+                a.q_w[3 * (size_t)pos] = qw.x; a.q_w[3 * (size_t)pos + 1] = qw.y; a.q_w[3 * (size_t)pos + 2] = qw.z;
+                a.q_idx[pos] = pidx;
+            } else {
+                atomicAdd(&a.ctr[C_QDROP], 1u);
+            }
+        }
+    }
+    // training records opened at the query vertex: features now, target
+    // accumulated from here on (records past the capacity are not kept)
+    const uint64_t tb = __ballot(want_rec);
+    if (tb) {
+        const int leader = __ffsll((unsigned long long)tb) - 1;
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(&a.ctr[C_TN], (uint32_t)__popcll(tb));
+        base = __shfl(base, leader);
+        if (want_rec) {
+            const uint32_t pos = base + (uint32_t)__popcll(tb & lt);
+            ridx = pos < a.t_cap ? pos : NO_REC;
+            if (ridx != NO_REC) {
+                for (int j = 0; j < 9; ++j) a.t_feat[9 * (size_t)pos + j] = f9[j];
+                a.t_tgt[3 * (size_t)pos] = 0.0f; a.t_tgt[3 * (size_t)pos + 1] = 0.0f; a.t_tgt[3 * (size_t)pos + 2] = 0.0f;
+            }
+            a.s_ridx[slot] = ridx;
+        }
+    }
+    // shadow rays: one atomic per wave
+    const uint64_t sb = __ballot(push_sh);
+    if (sb) {
+        const int leader = __ffsll((unsigned long long)sb) - 1;
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(&a.ctr[C_SN], (uint32_t)__popcll(sb));
+        base = __shfl(base, leader);
+        if (push_sh) {
+            const uint32_t pos = base + (uint32_t)__popcll(sb & lt);  // < S: one per live slot
+            a.sh_ray[pos] = sh;
+            float* w = a.sh_w + 6 * (size_t)pos;
+            w[0] = shw_pix.x; w[1] = shw_pix.y; w[2] = shw_pix.z;
+            w[3] = shw_rec.x; w[4] = shw_rec.y; w[5] = shw_rec.z;
+            a.sh_idx[2 * (size_t)pos] = pidx;
+            a.sh_idx[2 * (size_t)pos + 1] = sh_to_rec ? ridx : NO_REC;
+        }
+    }
+    // next work item for slots whose item finished: one atomic per wave
+    const uint64_t want = __ballot(need_item);
+    if (want) {
+        const int leader = __ffsll((unsigned long long)want) - 1;
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(&a.ctr[C_ITEM], (uint32_t)__popcll(want));
+        base = __shfl(base, leader);
+        if (need_item) {
+            const uint32_t item = base + (uint32_t)__popcll(want & lt);
+            if (item < a.items) {
+                start_item(a, slot, item, nr, false);
+                alive = true;
+            }
+        }
+    }
+    // compact the survivors into the next queue: one atomic per wave
+    const uint64_t live = __ballot(alive);
+    if (live) {
+        const int leader = __ffsll((unsigned long long)live) - 1;
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(n_out, (uint32_t)__popcll(live));
+        base = __shfl(base, leader);
+        if (alive) {
+            const uint32_t pos = base + (uint32_t)__popcll(live & lt);
+            a.q_slot[cur ^ 1][pos] = slot;
+            a.q_ray[cur ^ 1][pos] = nr;
+        }
+    }
+}
+
+// Unoccluded light samples add their weights to the pixel sums and records.
+__global__ void __launch_bounds__(PT_BLOCK) k_shadow_resolve(PTArgs a) {
+    const uint32_t n = a.ctr[C_SN];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *(unsigned long long*)&a.ctr[C_SHRAYS] += n;
+    if (i >= n || a.sh_occ[i]) return;
+    const float* w = a.sh_w + 6 * (size_t)i;
+    add3(&a.partial[3 * (size_t)a.sh_idx[2 * (size_t)i]], mk(w[0], w[1], w[2]));
+    const uint32_t r = a.sh_idx[2 * (size_t)i + 1];
+    if (r != NO_REC) add3(&a.t_tgt[3 * (size_t)r], mk(w[3], w[4], w[5]));
 }
 
 __global__ void __launch_bounds__(PT_BLOCK) k_pt_finish(PTArgs a) {
@@ -500,11 +795,14 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
     // iteration), training records capped at 4 frames' worth of steps
     const size_t q_cap = nrc ? 8 * szS : 0;
     const size_t t_cap = nrc ? (size_t)4 * nf->opt.batch * nf->opt.max_steps : 0;
+    // shadow rays (one per live slot and iteration): ray, 6 weights, 2 indices, flag
+    const size_t sh_bytes = szS * (sizeof(rt_ray) + 24 + 8 + 1);
     if (nrc && ((rc = ensure(c, c->pt_nrc_query, q_cap * (36 + 12 + 4))) ||
-                (rc = ensure(c, c->pt_nrc_train, (t_cap ? t_cap : 1) * (36 + 12)))))
+                (rc = ensure(c, c->pt_nrc_train, (t_cap ? t_cap : 1) * (36 + 12))) ||
+                (rc = ensure(c, c->pt_shadow, sh_bytes))))
         return rc;
     if ((rc = ensure(c, c->pt_state, szS * (4 * 3 + 12 * 2) + (adj ? szS * (8 * ADJ_MAX_DEPTH + 4) : 0) +
-                                         (nrc ? szS * 48 : 0))) ||
+                                         (nrc ? szS * 16 : 0))) ||
         (rc = ensure(c, c->pt_queue[0], 4 * szS + 16 + sizeof(rt_ray) * szS)) ||
         (rc = ensure(c, c->pt_queue[1], 4 * szS + 16 + sizeof(rt_ray) * szS)) ||
         (rc = ensure(c, c->hits, szS * sizeof(rt_hit))) || (rc = ensure(c, c->pt_ctr, 64)) ||
@@ -537,8 +835,20 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
     a.s_amat = adj ? (uint32_t*)st : nullptr; st += adj ? 4 * ADJ_MAX_DEPTH * szS : 0;
     a.s_acos = adj ? (float*)st : nullptr; st += adj ? 4 * ADJ_MAX_DEPTH * szS : 0;
     a.s_nv = adj ? (uint32_t*)st : nullptr; st += adj ? 4 * szS : 0;
-    a.s_qf = nrc ? (float*)st : nullptr; st += nrc ? 36 * szS : 0;
+    a.s_ridx = nrc ? (uint32_t*)st : nullptr; st += nrc ? 4 * szS : 0;
     a.s_Tq = nrc ? (float*)st : nullptr;
+    a.nee = nrc && !(nf->opt.flags & RT_NRC_NO_NEE) && (c->num_lights > 0 || ((s->flags & RT_RENDER_SUN) && s->sun_cos < 1.0f));
+    a.sun_on = (s->flags & RT_RENDER_SUN) && s->sun_cos < 1.0f;
+    a.lights = (const LightTri*)c->lights.ptr;
+    a.num_lights = c->num_lights;
+    a.light_area = c->light_area;
+    {
+        char* sp = (char*)c->pt_shadow.ptr;
+        a.sh_ray = nrc ? (rt_ray*)sp : nullptr; sp += sizeof(rt_ray) * szS;
+        a.sh_w = nrc ? (float*)sp : nullptr; sp += 24 * szS;
+        a.sh_idx = nrc ? (uint32_t*)sp : nullptr; sp += 8 * szS;
+        a.sh_occ = nrc ? (uint8_t*)sp : nullptr;
+    }
     a.nrc_q = nrc ? nf->opt.query_vertex : 0;
     a.nrc_every = nrc && !(nf->opt.flags & RT_NRC_NO_RECORDS) ? nf->opt.train_every : 0;
     a.nrc_query = nrc && !(nf->opt.flags & RT_NRC_NO_QUERY);
@@ -580,9 +890,22 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
         for (int rep = 0; rep < 8 && iter < bound; ++rep, ++iter) {
             if ((rc = launch_trace(c, a.q_ray[cur], S, a.hits, false, &a.ctr[cur ? C_N1 : C_N0]))) return rc;
             RT_HIP(c, hipMemsetAsync(&a.ctr[cur ? C_N0 : C_N1], 0, 4, stream));
-            if (adj) hipLaunchKernelGGL((k_pt_shade<true, false>), dim3(div_up(S, PT_BLOCK)), dim3(PT_BLOCK), 0, stream, a, cur);
-            else if (nrc) hipLaunchKernelGGL((k_pt_shade<false, true>), dim3(div_up(S, PT_BLOCK)), dim3(PT_BLOCK), 0, stream, a, cur);
-            else hipLaunchKernelGGL((k_pt_shade<false, false>), dim3(div_up(S, PT_BLOCK)), dim3(PT_BLOCK), 0, stream, a, cur);
+            if (nrc) {
+                if (a.nee) RT_HIP(c, hipMemsetAsync(&a.ctr[C_SN], 0, 4, stream));
+                hipLaunchKernelGGL(k_pt_shade_nrc, dim3(div_up(S, PT_BLOCK)), dim3(PT_BLOCK), 0, stream, a, cur);
+                RT_HIP(c, hipGetLastError());
+                if (a.nee) {
+                    // light samples: occlusion of the queued shadow rays, then their weights
+                    if ((rc = launch_trace_any(c, a.sh_ray, S, a.sh_occ, &a.ctr[C_SN]))) return rc;
+                    hipLaunchKernelGGL(k_shadow_resolve, dim3(div_up(S, PT_BLOCK)), dim3(PT_BLOCK), 0, stream, a);
+                    RT_HIP(c, hipGetLastError());
+                    launches += 2;
+                }
+            } else if (adj) {
+                hipLaunchKernelGGL(k_pt_shade<true>, dim3(div_up(S, PT_BLOCK)), dim3(PT_BLOCK), 0, stream, a, cur);
+            } else {
+                hipLaunchKernelGGL(k_pt_shade<false>, dim3(div_up(S, PT_BLOCK)), dim3(PT_BLOCK), 0, stream, a, cur);
+            }
             RT_HIP(c, hipGetLastError());
             launches += 2;
             cur ^= 1;
@@ -608,10 +931,14 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
     }
     if (nrc) {
         RT_HIP(c, hipEventRecord(c->ev[4], stream));
-        uint32_t tn = 0;
-        RT_HIP(c, hipMemcpyAsync(&tn, &a.ctr[C_TN], 4, hipMemcpyDeviceToHost, stream));
+        uint32_t hc[C_WORDS];
+        RT_HIP(c, hipMemcpyAsync(hc, a.ctr, sizeof(hc), hipMemcpyDeviceToHost, stream));
         RT_HIP(c, hipStreamSynchronize(stream));
+        const uint32_t tn = hc[C_TN];
         nf->stats->records = tn;
+        nf->stats->shadow_rays = (uint64_t)hc[C_SHRAYS] | ((uint64_t)hc[C_SHRAYS + 1] << 32);
+        c->nrc_records = tn < a.t_cap ? tn : a.t_cap;
+        c->nrc_tcap = a.t_cap;
         if (!(nf->opt.flags & RT_NRC_NO_TRAIN)) {
             const uint32_t avail = tn < a.t_cap ? tn : a.t_cap;
             if ((rc = nrc_train_records(nf->cache, a.t_feat, a.t_tgt, avail, nf->opt.batch, nf->opt.max_steps,

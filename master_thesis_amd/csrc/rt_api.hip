@@ -149,7 +149,7 @@ void rt_destroy(rt_ctx* c) {
                      &c->innerbox, &c->visit, &c->queue[0], &c->queue[1], &c->counters, &c->rays, &c->hits,
                      &c->spill, &c->work, &c->stats, &c->pt_state, &c->pt_queue[0], &c->pt_queue[1],
                      &c->pt_image, &c->pt_img8, &c->pt_dimg, &c->pt_dcolor, &c->pt_ctr, &c->pt_partial,
-                     &c->pt_cams, &c->pt_nrc_query, &c->pt_nrc_train};
+                     &c->pt_cams, &c->pt_nrc_query, &c->pt_nrc_train, &c->pt_shadow, &c->lights};
     for (DevBuf* b : all) release(c, *b);
     for (int k = 0; k < rt_ctx::NUM_EV; ++k)
         if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
@@ -205,8 +205,29 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* s) {
             lo[k] = fminf(lo[k], x);
             hi[k] = fmaxf(hi[k], x);
         }
+    // emissive triangles and their cumulative area (light sampling of the
+    // cached render); float arithmetic in triangle order
+    std::vector<LightTri> lights;
+    float area_sum = 0.0f;
+    for (uint64_t t = 0; t < T; ++t) {
+        if (!s->mat[s->tri_mat[t]].light_source) continue;
+        const float* p0 = s->pos + 3 * s->idx[3 * t];
+        const float* p1 = s->pos + 3 * s->idx[3 * t + 1];
+        const float* p2 = s->pos + 3 * s->idx[3 * t + 2];
+        const float area = tri_area(mk(p0[0], p0[1], p0[2]), mk(p1[0], p1[1], p1[2]), mk(p2[0], p2[1], p2[2]));
+        if (!(area > 0.0f)) continue;
+        area_sum = area_sum + area;
+        lights.push_back(LightTri{(uint32_t)t, area_sum});
+    }
+    for (LightTri& l : lights) l.cdf = l.cdf / area_sum;
+    if (!lights.empty()) lights.back().cdf = 1.0f;
     DeviceGuard g(c->device);
     int rc;
+    if (!lights.empty()) {
+        if ((rc = ensure(c, c->lights, sizeof(LightTri) * lights.size()))) return rc;
+        RT_HIP(c, hipMemcpyAsync(c->lights.ptr, lights.data(), sizeof(LightTri) * lights.size(), hipMemcpyHostToDevice,
+                                 c->stream));
+    }
     if ((rc = ensure(c, c->pos, 12 * V)) || (rc = ensure(c, c->idx, 12 * T)) ||
         (rc = ensure(c, c->mat, sizeof(rt_material) * M)) || (rc = ensure(c, c->shade_geom, sizeof(ShadeGeom) * T)) ||
         (rc = ensure(c, c->shade_nrm, sizeof(ShadeNrm) * T)))
@@ -229,6 +250,8 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* s) {
     c->T = (uint32_t)T;
     c->M = (uint32_t)M;
     for (int k = 0; k < 3; ++k) { c->scene_lo[k] = lo[k]; c->scene_hi[k] = hi[k]; }
+    c->num_lights = (uint32_t)lights.size();
+    c->light_area = area_sum;
     c->built = false;
     return RT_OK;
 }
@@ -439,6 +462,24 @@ int rt_render_nrc(rt_ctx* c, rt_nrc* nrc, const rt_camera* cams, uint32_t num_vi
         if (rgb8) RT_HIP(c, hipMemcpyAsync(rgb8, d_rgb8, 3 * P, hipMemcpyDeviceToHost, c->stream));
         RT_HIP(c, hipStreamSynchronize(c->stream));
     }
+    return RT_OK;
+}
+
+int rt_render_nrc_records(rt_ctx* c, float* feats, float* targets, uint32_t* count) {
+    if (!c || !count) return RT_E_INVALID;
+    const uint32_t cap = *count;
+    *count = c->nrc_records;
+    if (!feats && !targets) return RT_OK;
+    if (cap < c->nrc_records) return fail(c, RT_E_INVALID, "rt_render_nrc_records: capacity below the record count");
+    if (c->nrc_records == 0) return RT_OK;
+    DeviceGuard g(c->device);
+    const float* base = (const float*)c->pt_nrc_train.ptr;
+    if (feats)
+        RT_HIP(c, hipMemcpyAsync(feats, base, 36 * (size_t)c->nrc_records, hipMemcpyDeviceToHost, c->stream));
+    if (targets)
+        RT_HIP(c, hipMemcpyAsync(targets, base + 9 * (size_t)c->nrc_tcap, 12 * (size_t)c->nrc_records,
+                                 hipMemcpyDeviceToHost, c->stream));
+    RT_HIP(c, hipStreamSynchronize(c->stream));
     return RT_OK;
 }
 

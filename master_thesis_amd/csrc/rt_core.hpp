@@ -248,6 +248,12 @@ RT_HD f3 primary_dir(const float* view_inv, const float* proj_inv, uint32_t px, 
     return normalize_ref(mk(d4[0], d4[1], d4[2]));
 }
 
+// Triangle area 0.5 |(p1 - p0) x (p2 - p0)| (light sampling).
+RT_HD float tri_area(f3 p0, f3 p1, f3 p2) {
+    const f3 c = cross(sub(p1, p0), sub(p2, p0));
+    return 0.5f * sqrtf(dot(c, c));
+}
+
 // Reciprocal used only for box culling: zero components are replaced by a
 // tiny signed value so the slab test never forms 0 * inf.
 RT_HD float safe_rcp(float d) {

@@ -26,6 +26,13 @@ struct ShadeNrm {
     float pad[3];
 };
 
+// Emissive triangle for light sampling: caller's triangle index and the
+// cumulative area fraction up to and including it (last = 1).
+struct LightTri {
+    uint32_t tri;
+    float cdf;
+};
+
 // Device buffer that grows on demand and is freed with the context.
 struct DevBuf {
     void* ptr = nullptr;
@@ -45,6 +52,9 @@ struct rt_ctx {
     uint32_t V = 0, T = 0, M = 0;
     float scene_lo[3] = {0, 0, 0}, scene_hi[3] = {0, 0, 0};
     rt::DevBuf pos, idx, mat, shade_geom, shade_nrm;
+    rt::DevBuf lights;  // LightTri per emissive triangle of positive area
+    uint32_t num_lights = 0;
+    float light_area = 0.0f;
     // BVH
     bool built = false;
     rt::DevBuf nodes, tris;
@@ -58,6 +68,8 @@ struct rt_ctx {
     // path tracer state
     rt::DevBuf pt_state, pt_queue[2], pt_image, pt_img8, pt_dimg, pt_dcolor, pt_ctr, pt_partial, pt_cams;
     rt::DevBuf pt_nrc_query, pt_nrc_train;  // cache queries (features, weights, pixel) and training records
+    rt::DevBuf pt_shadow;                   // shadow rays of light sampling (rays, weights, indices, flags)
+    uint32_t nrc_records = 0, nrc_tcap = 0; // training records of the last cached render, record capacity
     std::vector<rt_nrc*> caches;            // live radiance caches; rt_destroy detaches them
     uint32_t render_iters = 0;
 

@@ -104,3 +104,10 @@ def cornell_camera(width: int = 256, height: int = 256, fov_deg: float = 40.0) -
     in the reference's camera terms), 40-degree field of view."""
     from .scenes import cornell_camera_position
     return camera(cornell_camera_position(), 1.0, 0.0, float(np.radians(fov_deg)), width, height)
+
+
+def closed_camera(width: int = 256, height: int = 256, fov_deg: float = 75.0) -> L.rt_camera:
+    """Camera inside the closed Cornell-style box, just in front of its
+    closing wall, looking -x."""
+    from .scenes import CORNELL_HALF
+    return camera((CORNELL_HALF - 0.05, 0.0, 0.0), 1.0, 0.0, float(np.radians(fov_deg)), width, height)

@@ -126,6 +126,47 @@ def cornell_box(light_color=(15.0, 15.0, 15.0)) -> Scene:
     return b.scene(mats)
 
 
+def cornell_closed(light_color=(15.0, 15.0, 15.0)) -> Scene:
+    """The Cornell-style box with its open side closed by a white wall
+    (x = +h, facing in): no path escapes, so paths run to the depth limit.
+    Seen from inside (closed_camera).  Materials as cornell_box."""
+    sc = cornell_box(light_color)
+    h = CORNELL_HALF
+    s = 2 * h
+    b = _Builder()
+    b.quad((h, -h, -h), (0, s, 0), (0, 0, s), (-1, 0, 0), 0)
+    n0 = sc.pos.shape[0]
+    return Scene(pos=np.concatenate([sc.pos, np.asarray(b.pos, np.float32)]),
+                 nrm=np.concatenate([sc.nrm, np.asarray(b.nrm, np.float32)]),
+                 idx=np.concatenate([sc.idx, np.asarray(b.idx, np.uint32) + n0]),
+                 tri_mat=np.concatenate([sc.tri_mat, np.asarray(b.mat, np.uint32)]),
+                 materials=sc.materials)
+
+
+def cornell_indirect(light_color=(15.0, 15.0, 15.0)) -> Scene:
+    """The Cornell-style box lit only indirectly: the light is a quad facing
+    the ceiling on top of a white slab (the lamp) 0.8 below it, so the
+    visible surfaces other than the ceiling see the light only after a
+    bounce.  Materials as cornell_box."""
+    h = CORNELL_HALF
+    b = _Builder()
+    W, R, G, L = 0, 1, 2, 3
+    s = 2 * h
+    b.quad((-h, -h, -h), (0, 0, s), (0, s, 0), (1, 0, 0), W)      # back wall x = -h
+    b.quad((-h, -h, -h), (s, 0, 0), (0, 0, s), (0, 1, 0), W)      # floor y = -h
+    b.quad((-h, h, -h), (0, 0, s), (s, 0, 0), (0, -1, 0), W)      # ceiling y = +h
+    b.quad((-h, -h, -h), (0, s, 0), (s, 0, 0), (0, 0, 1), G)      # wall z = -h (green)
+    b.quad((-h, -h, h), (s, 0, 0), (0, s, 0), (0, 0, -1), R)      # wall z = +h (red)
+    b.box((0.10, -h, 0.45), (1.75, -h + 1.65, 2.10), W)           # short box
+    b.box((-1.85, -h, -2.05), (-0.20, -h + 3.30, -0.40), W)       # tall box
+    y0 = h - 0.9
+    b.box((-0.8, y0, -0.8), (0.8, y0 + 0.1, 0.8), W)              # lamp slab
+    lh = 0.70
+    b.quad((-lh, y0 + 0.102, -lh), (0, 0, 2 * lh), (2 * lh, 0, 0), (0, 1, 0), L)  # light, facing up
+    mats = [material(WHITE), material(RED), material(GREEN), material(light_color, light=True)]
+    return b.scene(mats)
+
+
 def cornell_groups(albedo: np.ndarray | None = None, light_color=(15.0, 15.0, 15.0), seed: int = 0) -> Scene:
     """The Cornell-style box with 64 triangle groups, each its own material
     with a learnable albedo: every wall is a 2 x 2 grid of quads (5 walls x 8
@@ -181,6 +222,15 @@ def closed_room(floor_mat: np.ndarray, wall_mat: np.ndarray, half: float = 1.0) 
     b.quad((-h, -h, -h), (0, s, 0), (s, 0, 0), (0, 0, 1), 1)      # z = -h
     b.quad((-h, -h, h), (s, 0, 0), (0, s, 0), (0, 0, -1), 1)      # z = +h
     return b.scene([floor_mat, wall_mat])
+
+
+def ground_plane(albedo, half: float = 100.0) -> Scene:
+    """One upward-facing square y = 0 of side 2 * half (material 0): open
+    sky above it.  Used by the sun's closed-form checks."""
+    b = _Builder()
+    s = 2 * half
+    b.quad((-half, 0.0, -half), (0, 0, s), (s, 0, 0), (0, 1, 0), 0)
+    return b.scene([material(albedo)])
 
 
 def cornell_camera_position() -> tuple[float, float, float]:

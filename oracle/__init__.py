@@ -223,13 +223,16 @@ def render_adjoint(scene, cam, settings, d_image: np.ndarray, threads: int = 8) 
 
 
 def render_cached(scene, cams, settings, cache_rgb, q_vertex: int = 1, train_every: int = 16, query: bool = True,
-                  threads: int = 8):
+                  nee: bool = False, records: bool = False, threads: int = 8):
     """The cache mode of rt_render_nrc with a constant cache prediction.
-    Returns (rgb (V, H, W, 3) float32, queries, records)."""
+    Returns (rgb (V, H, W, 3) float32, queries, records) and, with
+    records=True, also (feats (n, 9), targets (n, 3), keys (n,)) of the
+    training records in (view, pixel, sample) order, key = (view * H * W +
+    pixel) * spp + sample."""
     lib = load()
     P = C.c_void_p
     lib.or_render_cached.argtypes = [P, P, P, P, P, C.c_uint32, C.c_uint32, C.c_uint32, P, C.c_uint32, P,
-                                     C.c_uint32, C.c_uint32, C.c_int, P, P, P, C.c_int]
+                                     C.c_uint32, C.c_uint32, C.c_int, C.c_int, P, P, P, P, C.c_int]
     lib.or_render_cached.restype = C.c_int
     arrs, ps = _scene_ptrs(scene)
     from master_thesis_amd._lib import rt_camera
@@ -241,8 +244,12 @@ def render_cached(scene, cams, settings, cache_rgb, q_vertex: int = 1, train_eve
     rgb = np.zeros((V, H, W, 3), np.float32)
     c = np.ascontiguousarray(cache_rgb, np.float32)
     counts = np.zeros(2, np.uint64)
+    rec = np.zeros((V * H * W * settings.spp, 13), np.float32) if records else None
     rc = lib.or_render_cached(*ps, arrs[0].shape[0], arrs[2].shape[0], arrs[4].shape[0], C.addressof(carr), V,
-                              C.addressof(settings), q_vertex, train_every, int(query), _p(c), _p(rgb), _p(counts),
-                              threads)
+                              C.addressof(settings), q_vertex, train_every, int(query), int(nee), _p(c), _p(rgb),
+                              _p(rec) if records else None, _p(counts), threads)
     assert rc == 0, rc
+    if records:
+        ok = rec[:, 12] == 1.0
+        return rgb, int(counts[0]), int(counts[1]), rec[ok, :9], rec[ok, 9:12], np.nonzero(ok)[0]
     return rgb, int(counts[0]), int(counts[1])

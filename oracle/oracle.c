@@ -857,9 +857,15 @@ typedef struct {
     double* dcol; /* [256 threads][3M] */
     /* radiance-cache mode (cache_rgb != NULL): a constant cache */
     const float* cache_rgb;
-    uint32_t q_vertex, train_every, query_on;
+    uint32_t q_vertex, train_every, query_on, nee;
     uint64_t* n_queries; /* [256 threads] */
     uint64_t* n_records; /* [256 threads] */
+    float* rec;          /* per (vp, sample): 9 features, 3 targets, valid flag (13 floats), or NULL */
+    /* light sampling: emissive triangles of positive area, cumulative area fraction */
+    uint32_t num_lights;
+    uint32_t* light_tri;
+    float* light_cdf;
+    float light_area;
 } pt_job;
 
 /* Training-sample choice of the cache mode: lowbias32 (C. Wellons) over
@@ -880,7 +886,7 @@ static uint32_t train_hash(uint32_t pix, uint32_t smp, uint32_t view, uint32_t s
  * contribution when the path ends on a light (or escapes to the sky);
  * nv / vm / vc receive the diffuse vertices for the adjoint. */
 static int trace_path(const pt_job* j, uint32_t view, uint32_t pix, uint32_t gsmp, v3 dir0, v3* contrib, v3* term,
-                      uint32_t* nv, uint32_t* vm, float* vc, int tid) {
+                      uint32_t* nv, uint32_t* vm, float* vc) {
     const rt_settings* s = j->s;
     const rt_camera* cam = &j->cams[view];
     v3 o = V3(cam->origin[0], cam->origin[1], cam->origin[2]);
@@ -890,7 +896,6 @@ static int trace_path(const pt_job* j, uint32_t view, uint32_t pix, uint32_t gsm
     *nv = 0;
     *term = V3(1.0f, 1.0f, 1.0f);
     uint32_t key[2] = {(uint32_t)s->seed, (uint32_t)(s->seed >> 32)};
-    uint32_t fh = 0; /* front-facing hits so far (cache mode) */
     for (uint32_t depth = 0; depth < s->max_depth; ++depth) {
         rt_ray r;
         r.ox = o.x; r.oy = o.y; r.oz = o.z; r.tmin = 0.0f;
@@ -930,19 +935,6 @@ static int trace_path(const pt_job* j, uint32_t view, uint32_t pix, uint32_t gsm
         }
         cull = 0;
         const rt_material* m = &j->mat[j->tri_mat[t]];
-        const int light_end = m->light_source && vdot(n, d) <= 0.0f;
-        if (j->cache_rgb && fh == j->q_vertex && !light_end && depth + 1 < s->max_depth) {
-            const int train = j->train_every != 0 && train_hash(pix, gsmp, view, key[0]) % j->train_every == 0;
-            if (train) {
-                j->n_records[tid]++; /* the record is made when the path ends; its target is not checked here */
-            } else if (j->query_on) {
-                j->n_queries[tid]++;
-                *contrib = vmul(T, V3(fmaxf(j->cache_rgb[0], 0.0f), fmaxf(j->cache_rgb[1], 0.0f),
-                                      fmaxf(j->cache_rgb[2], 0.0f)));
-                return 1;
-            }
-        }
-        ++fh;
         uint32_t ctr[4] = {pix, gsmp, depth, view}, rnd[4];
         or_philox(ctr, key, rnd);
         float xi0 = u01(rnd[0]);
@@ -966,6 +958,198 @@ static int trace_path(const pt_job* j, uint32_t view, uint32_t pix, uint32_t gsm
     return 0;
 }
 
+/* Light sample of the cache mode at vertex p (the expected value of the
+ * diffuse branch's next-vertex light hit, or of its escape inside the sun's
+ * cone): the GPU's nee_sample restated with the same float operations. */
+static int nee_sample(const pt_job* j, v3 p, v3 nrm, const rt_material* m, const uint32_t rnd[4], rt_ray* sh, v3* C) {
+    const rt_settings* s = j->s;
+    const int sun_on = (s->flags & RT_RENDER_SUN) && s->sun_cos < 1.0f;
+    const uint32_t nls = (j->num_lights > 0 ? 1u : 0u) + (sun_on ? 1u : 0u);
+    if (nls == 0 || !(m->diffuse_threshold > 0.0f)) return 0;
+    const int use_light = j->num_lights > 0 && (!sun_on || u01(rnd[0]) < 0.5f);
+    const v3 col = V3(m->color[0], m->color[1], m->color[2]);
+    v3 w;
+    if (use_light) {
+        const float u = u01(rnd[1]);
+        uint32_t lo = 0, hi = j->num_lights - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (j->light_cdf[mid] > u) hi = mid;
+            else lo = mid + 1;
+        }
+        const uint32_t t = j->light_tri[lo];
+        const v3 v0 = vld(j->pos + 3 * j->idx[3 * t]);
+        const v3 e0 = vsub(vld(j->pos + 3 * j->idx[3 * t + 1]), v0);
+        const v3 e1 = vsub(vld(j->pos + 3 * j->idx[3 * t + 2]), v0);
+        const float su = sqrtf(u01(rnd[2])), u2 = u01(rnd[3]);
+        const float b1 = su * (1.0f - u2), b2 = su * u2, b0 = 1.0f - su;
+        const v3 y = vadd(vadd(v0, vscl(b1, e0)), vscl(b2, e1));
+        v3 ny = vscl(b0, vld(j->nrm + 3 * j->idx[3 * t]));
+        ny = vadd(ny, vscl(b1, vld(j->nrm + 3 * j->idx[3 * t + 1])));
+        ny = vadd(ny, vscl(b2, vld(j->nrm + 3 * j->idx[3 * t + 2])));
+        ny = vnormalize(ny);
+        const v3 wv = vsub(y, p);
+        const float d2 = vdot(wv, wv);
+        const float dist = sqrtf(d2);
+        if (!(dist > 0.02f)) return 0;
+        w = vscl(1.0f / dist, wv);
+        const float cx = vdot(nrm, w), cy = -vdot(ny, w);
+        if (!(cx > 0.0f) || !(cy > 0.0f)) return 0;
+        const rt_material* lm = &j->mat[j->tri_mat[t]];
+        const float tdl = lm->diffuse_threshold;
+        const v3 RL = V3(tdl * lm->color[0] * 0.5f + (1.0f - tdl), tdl * lm->color[1] * 0.5f + (1.0f - tdl),
+                         tdl * lm->color[2] * 0.5f + (1.0f - tdl));
+        const float f = m->diffuse_threshold * cx * cy * j->light_area * (float)nls / (6.28318530717958647f * d2);
+        *C = vscl(f, vmul(col, RL));
+        sh->tmax = (dist - 0.01f) * 0.9990234375f;
+    } else {
+        const v3 sd = vld(s->sun_dir);
+        const float ct = 1.0f - u01(rnd[1]) * (1.0f - s->sun_cos);
+        const float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+        float sp, cp;
+        or_sincos_2pi(u01(rnd[2]), &sp, &cp);
+        /* orthonormal basis around the sun direction (Duff et al., JCGT 2017) */
+        const float sg = copysignf(1.0f, sd.z);
+        const float ia = -1.0f / (sg + sd.z);
+        const float ib = sd.x * sd.y * ia;
+        const v3 t1 = V3(1.0f + sg * sd.x * sd.x * ia, sg * ib, -sg * sd.x);
+        const v3 t2 = V3(ib, sg + sd.y * sd.y * ia, -sd.y);
+        w = vadd(vscl(ct, sd), vscl(st, vadd(vscl(cp, t1), vscl(sp, t2))));
+        const float cx = vdot(nrm, w);
+        if (!(cx > 0.0f)) return 0;
+        const float f = m->diffuse_threshold * cx * (1.0f - s->sun_cos) * (float)nls;
+        *C = vscl(f, vmul(col, vld(s->sun_radiance)));
+        sh->tmax = INFINITY;
+    }
+    const v3 o = vadd(p, vscl(0.01f, w));
+    sh->ox = o.x; sh->oy = o.y; sh->oz = o.z; sh->tmin = 0.0f;
+    sh->dx = w.x; sh->dy = w.y; sh->dz = w.z;
+    return 1;
+}
+
+/* One sample of the cache mode (rt_render_nrc with a constant cache):
+ * returns its summed pixel contribution; a training sample fills rec
+ * (9 features, 3 targets, flag 1) when rec is not NULL. */
+static v3 trace_path_cached(const pt_job* j, uint32_t view, uint32_t pix, uint32_t gsmp, v3 dir0, int tid, float* rec) {
+    const rt_settings* s = j->s;
+    const rt_camera* cam = &j->cams[view];
+    v3 o = V3(cam->origin[0], cam->origin[1], cam->origin[2]);
+    v3 d = dir0;
+    v3 T = V3(1.0f, 1.0f, 1.0f), Tq = V3(1.0f, 1.0f, 1.0f), sum = V3(0.0f, 0.0f, 0.0f), tgt = V3(0.0f, 0.0f, 0.0f);
+    int cull = 1, has_rec = 0, prev_diff = 0;
+    uint32_t fh = 0;
+    float feats[9];
+    uint32_t key[2] = {(uint32_t)s->seed, (uint32_t)(s->seed >> 32)};
+    if (rec) memset(rec, 0, 13 * sizeof(float));
+    for (uint32_t depth = 0; depth < s->max_depth; ++depth) {
+        rt_ray r;
+        r.ox = o.x; r.oy = o.y; r.oz = o.z; r.tmin = 0.0f;
+        r.dx = d.x; r.dy = d.y; r.dz = d.z; r.tmax = INFINITY;
+        uint64_t cn = 0, ct = 0;
+        best_t h = trace_bvh_one(j->bvh, &r, &cn, &ct);
+        if (h.id == RT_MISS) {
+            if (s->flags & (RT_RENDER_SKY | RT_RENDER_SUN)) {
+                v3 L = V3(0.0f, 0.0f, 0.0f);
+                if (s->flags & RT_RENDER_SKY) L = V3(s->sky[0], s->sky[1], s->sky[2]);
+                if (!(j->nee && prev_diff) && (s->flags & RT_RENDER_SUN) && vdot(d, vld(s->sun_dir)) >= s->sun_cos)
+                    L = vadd(L, vld(s->sun_radiance));
+                sum = vadd(sum, vmul(T, L));
+                if (has_rec) tgt = vadd(tgt, vmul(Tq, L));
+            }
+            break;
+        }
+        v3 p = vadd(o, vscl(h.t, d));
+        uint32_t t = h.id;
+        v3 v0 = vld(j->pos + 3 * j->idx[3 * t]);
+        v3 e0 = vsub(vld(j->pos + 3 * j->idx[3 * t + 1]), v0);
+        v3 e1 = vsub(vld(j->pos + 3 * j->idx[3 * t + 2]), v0);
+        float b0, b1, b2;
+        bary(v0, e0, e1, p, &b0, &b1, &b2);
+        v3 n = V3(0.0f, 0.0f, 0.0f);
+        n = vadd(n, vscl(b0, vld(j->nrm + 3 * j->idx[3 * t])));
+        n = vadd(n, vscl(b1, vld(j->nrm + 3 * j->idx[3 * t + 1])));
+        n = vadd(n, vscl(b2, vld(j->nrm + 3 * j->idx[3 * t + 2])));
+        n = vnormalize(n);
+        const float ndotd = vdot(n, d);
+        if (cull && ndotd > 0.0f) {
+            o = vadd(p, vscl(0.01f, d));
+            continue;
+        }
+        cull = 0;
+        const rt_material* m = &j->mat[j->tri_mat[t]];
+        const int light_end = m->light_source && ndotd <= 0.0f;
+        if (light_end && j->nee) {
+            /* after a diffuse bounce: estimated by the previous vertex's light sample; else the light's
+             * expected factor td_L * c_L / 2 + (1 - td_L) */
+            if (!prev_diff) {
+                const float tdl = m->diffuse_threshold;
+                const v3 RL = V3(tdl * m->color[0] * 0.5f + (1.0f - tdl), tdl * m->color[1] * 0.5f + (1.0f - tdl),
+                                 tdl * m->color[2] * 0.5f + (1.0f - tdl));
+                sum = vadd(sum, vmul(T, RL));
+                if (has_rec) tgt = vadd(tgt, vmul(Tq, RL));
+            }
+            break;
+        }
+        if (!light_end && fh == j->q_vertex && depth + 1 < s->max_depth) {
+            const int train = j->train_every != 0 && train_hash(pix, gsmp, view, key[0]) % j->train_every == 0;
+            if (train) {
+                j->n_records[tid]++;
+                has_rec = 1;
+                Tq = V3(1.0f, 1.0f, 1.0f);
+                tgt = V3(0.0f, 0.0f, 0.0f);
+                feats[0] = p.x; feats[1] = p.y; feats[2] = p.z;
+                feats[3] = n.x; feats[4] = n.y; feats[5] = n.z;
+                feats[6] = d.x; feats[7] = d.y; feats[8] = d.z;
+            } else if (j->query_on) {
+                j->n_queries[tid]++;
+                sum = vadd(sum, vmul(T, V3(fmaxf(j->cache_rgb[0], 0.0f), fmaxf(j->cache_rgb[1], 0.0f),
+                                            fmaxf(j->cache_rgb[2], 0.0f))));
+                break;
+            }
+        }
+        ++fh;
+        uint32_t ctr[4] = {pix, gsmp, depth, view}, rnd[4];
+        if (j->nee && !light_end && depth + 1 < s->max_depth) {
+            uint32_t c5[4] = {pix, gsmp, depth, view | 0x80000000u}, r5[4];
+            or_philox(c5, key, r5);
+            rt_ray sh;
+            v3 C;
+            if (nee_sample(j, p, n, m, r5, &sh, &C)) {
+                uint64_t a1 = 0, a2 = 0;
+                best_t oh = trace_bvh_one(j->bvh, &sh, &a1, &a2);
+                if (oh.id == RT_MISS) {
+                    sum = vadd(sum, vmul(T, C));
+                    if (has_rec) tgt = vadd(tgt, vmul(Tq, C));
+                }
+            }
+        }
+        or_philox(ctr, key, rnd);
+        float xi0 = u01(rnd[0]);
+        int diffuse = xi0 <= m->diffuse_threshold;
+        if (diffuse) d = hemisphere(u01(rnd[1]), u01(rnd[2]), n);
+        else if (xi0 <= m->reflect_threshold) d = reflect_ref(d, n);
+        else if (xi0 <= m->transparent_threshold) d = custom_refract_ref(d, n, m->refraction_index);
+        o = vadd(p, vscl(0.01f, d));
+        if (diffuse) {
+            const v3 f = vscl(vdot(n, d), V3(m->color[0], m->color[1], m->color[2]));
+            T = vmul(T, f);
+            Tq = vmul(Tq, f);
+        }
+        prev_diff = diffuse;
+        if (light_end) {
+            sum = vadd(sum, T);
+            if (has_rec) tgt = vadd(tgt, Tq);
+            break;
+        }
+    }
+    if (rec && has_rec) {
+        memcpy(rec, feats, sizeof(feats));
+        rec[9] = tgt.x; rec[10] = tgt.y; rec[11] = tgt.z;
+        rec[12] = 1.0f;
+    }
+    return sum;
+}
+
 static void pt_range(void* arg, uint32_t b, uint32_t e, int tid) {
     pt_job* j = (pt_job*)arg;
     const uint32_t P = j->cams[0].width * j->cams[0].height;
@@ -978,9 +1162,15 @@ static void pt_range(void* arg, uint32_t b, uint32_t e, int tid) {
         v3 dir = primary(cam, pix % cam->width, pix / cam->width);
         float sum[3] = {0.0f, 0.0f, 0.0f};
         for (uint32_t smp = 0; smp < j->s->spp; ++smp) {
-            v3 c, term;
-            uint32_t nv;
-            if (!trace_path(j, view, pix, j->s->sample_offset + smp, dir, &c, &term, &nv, vm, vc, tid)) continue;
+            v3 c, term = V3(0.0f, 0.0f, 0.0f);
+            uint32_t nv = 0;
+            if (j->cache_rgb) {
+                /* cache mode: every sample's summed contribution */
+                c = trace_path_cached(j, view, pix, j->s->sample_offset + smp, dir, tid,
+                                      j->rec ? j->rec + 13 * ((size_t)vp * j->s->spp + smp) : NULL);
+            } else if (!trace_path(j, view, pix, j->s->sample_offset + smp, dir, &c, &term, &nv, vm, vc)) {
+                continue;
+            }
             if (j->dimg) {
                 float w[3], tm[3] = {term.x, term.y, term.z};
                 for (int ch = 0; ch < 3; ++ch) w[ch] = j->dimg[3 * (size_t)vp + ch] * (float)j->scale;
@@ -1017,7 +1207,8 @@ static void pt_range(void* arg, uint32_t b, uint32_t e, int tid) {
 
 typedef struct {
     const float* rgb;
-    uint32_t q_vertex, train_every, query_on;
+    uint32_t q_vertex, train_every, query_on, nee;
+    float* rec;
     uint64_t queries, records;
 } cache_cfg;
 
@@ -1050,9 +1241,31 @@ static int pt_run(const float* pos, const float* nrm, const uint32_t* idx, const
         j.q_vertex = cc->q_vertex;
         j.train_every = cc->train_every;
         j.query_on = cc->query_on;
+        j.nee = cc->nee;
+        j.rec = cc->rec;
+        /* emissive triangles by cumulative area, float, in triangle order */
+        j.light_tri = (uint32_t*)malloc(sizeof(uint32_t) * (T ? T : 1));
+        j.light_cdf = (float*)malloc(sizeof(float) * (T ? T : 1));
+        float acc = 0.0f;
+        for (uint32_t t = 0; t < T; ++t) {
+            if (!j.mat[tri_mat[t]].light_source) continue;
+            const v3 p0 = vld(pos + 3 * idx[3 * t]), p1 = vld(pos + 3 * idx[3 * t + 1]), p2 = vld(pos + 3 * idx[3 * t + 2]);
+            const v3 c = vcross(vsub(p1, p0), vsub(p2, p0));
+            const float area = 0.5f * sqrtf(vdot(c, c));
+            if (!(area > 0.0f)) continue;
+            acc = acc + area;
+            j.light_tri[j.num_lights] = t;
+            j.light_cdf[j.num_lights] = acc;
+            j.num_lights++;
+        }
+        for (uint32_t k = 0; k < j.num_lights; ++k) j.light_cdf[k] = j.light_cdf[k] / acc;
+        if (j.num_lights) j.light_cdf[j.num_lights - 1] = 1.0f;
+        j.light_area = acc;
     }
     run_parallel(threads, j.cams[0].width * j.cams[0].height * num_views, 16, pt_range, &j);
     if (cc) {
+        free(j.light_tri);
+        free(j.light_cdf);
         cc->queries = cc->records = 0;
         for (int t = 0; t < 256; ++t) {
             cc->queries += nq[t];
@@ -1091,14 +1304,16 @@ int or_render_adjoint(const float* pos, const float* nrm, const uint32_t* idx, c
 
 int or_render_cached(const float* pos, const float* nrm, const uint32_t* idx, const uint32_t* tri_mat,
                      const void* mat, uint32_t V, uint32_t T, uint32_t M, const void* cams, uint32_t num_views,
-                     const void* settings, uint32_t q_vertex, uint32_t train_every, int query_on,
-                     const float cache_rgb[3], float* rgb, uint64_t counts[2], int threads) {
+                     const void* settings, uint32_t q_vertex, uint32_t train_every, int query_on, int nee,
+                     const float cache_rgb[3], float* rgb, float* records, uint64_t counts[2], int threads) {
     cache_cfg cc;
     memset(&cc, 0, sizeof(cc));
     cc.rgb = cache_rgb;
     cc.q_vertex = q_vertex;
     cc.train_every = train_every;
     cc.query_on = (uint32_t)(query_on != 0);
+    cc.nee = (uint32_t)(nee != 0);
+    cc.rec = records;
     int rc = pt_run(pos, nrm, idx, tri_mat, mat, V, T, M, cams, num_views, settings, rgb, NULL, NULL, NULL, threads,
                     &cc);
     if (counts) {

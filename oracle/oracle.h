@@ -103,12 +103,17 @@ void or_occluded_brute(const float* pos, const uint32_t* idx, uint32_t T, const 
  * cache_rgb everywhere: a sample stops at its q_vertex-th front-facing hit
  * (unless that hit ends the path on a light, or is at the last depth) and
  * adds throughput * max(cache_rgb, 0), except training samples
- * (train_hash(pixel, sample, view, seed) % train_every == 0), which trace on.
- * counts (may be NULL): queries, training records. */
+ * (train_hash(pixel, sample, view, seed) % train_every == 0), which trace on
+ * and record the rest of their path's contribution from that vertex.  nee:
+ * light sampling at every vertex (emissive triangles by area, the sun's
+ * cone) in place of the diffuse bounce's light hits and sun escapes.
+ * records (may be NULL): 13 floats per (view-major pixel, sample) - 9
+ * features, 3 targets, 1 if a record was made.  counts (may be NULL):
+ * queries, training records. */
 int or_render_cached(const float* pos, const float* nrm, const uint32_t* idx, const uint32_t* tri_mat,
                      const void* mat, uint32_t V, uint32_t T, uint32_t M, const void* cams, uint32_t num_views,
-                     const void* settings, uint32_t q_vertex, uint32_t train_every, int query_on,
-                     const float cache_rgb[3], float* rgb, uint64_t counts[2], int threads);
+                     const void* settings, uint32_t q_vertex, uint32_t train_every, int query_on, int nee,
+                     const float cache_rgb[3], float* rgb, float* records, uint64_t counts[2], int threads);
 
 #ifdef __cplusplus
 }

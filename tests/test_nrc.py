@@ -7,6 +7,7 @@ render."""
 import math
 
 import numpy as np
+import pytest
 import torch
 
 import oracle as O
@@ -205,3 +206,115 @@ def test_cached_render_training_samples_trace_on():
     off, nq0, _ = O.render_cached(sc, cam, s, np.array([1.0, 1.0, 1.0]), 1, 4, query=False)
     assert nq0 == 0
     np.testing.assert_array_equal(off[0], plain)
+
+
+# ---- light sampling of the cache mode (oracle)
+
+def _sun_settings(spp, theta_deg=30.0, alpha_deg=5.0, S=40.0, seed=3, max_depth=3):
+    t = math.radians(theta_deg)
+    return make_settings(spp=spp, max_depth=max_depth, seed=seed, flags=0x4,
+                         sun_dir=(math.sin(t), math.cos(t), 0.0), sun_radius_deg=alpha_deg,
+                         sun_radiance=(S, S, S))
+
+
+def test_sun_sampling_matches_closed_form():
+    """Open ground (albedo rho) under a sun of half-angle alpha at zenith
+    angle theta: pixel = rho * S * pi * sin^2(alpha) * cos(theta) for the
+    reference estimator; light sampling has the same mean at far lower
+    variance."""
+    rho, S, th, al = 0.5, 40.0, 30.0, 5.0
+    sc = scenes.ground_plane((rho, rho, rho))
+    cam = H.camera((0.0, 1.0, 0.0), 0.0, -1.0, math.pi / 4, 32, 32)
+    want = rho * S * math.pi * math.sin(math.radians(al)) ** 2 * math.cos(math.radians(th))
+    nee, nq, nr = O.render_cached(sc, cam, _sun_settings(4), np.zeros(3), q_vertex=1, train_every=0,
+                                  query=False, nee=True)
+    assert nq == 0 and nr == 0
+    assert abs(nee.mean() - want) < 2e-3 * want
+    plain, _ = O.render(sc, cam, _sun_settings(256))
+    # plain per-sample value 2 pi rho S cos, non-zero with probability 1 - cos(alpha) (the cone's share of
+    # the hemisphere): its standard deviation is below 2 pi rho S sqrt(1 - cos(alpha))
+    p_hit = 1.0 - math.cos(math.radians(al))
+    sd = 2 * math.pi * rho * S * math.sqrt(p_hit) / math.sqrt(256 * 32 * 32)
+    assert abs(plain.mean() - want) < 4 * sd
+    assert nee.std() < 0.2 * plain.std()
+
+
+def test_light_sampling_lit_floor_is_half_pi_rho_E():
+    """Closed room, floor albedo rho, emissive walls E: the reference
+    estimator's mean pi rho E / 2 (cosines at the floor and at the light),
+    reproduced by sampling the walls' area."""
+    rho, E = 0.6, 2.0
+    sc = scenes.closed_room(scenes.material((rho, rho, rho)), scenes.material((E, E, E), light=True))
+    cam = _down_camera()
+    N = 16
+    nee, _, _ = O.render_cached(sc, cam, make_settings(spp=N, max_depth=4, seed=4), np.zeros(3), 1, 0,
+                                query=False, nee=True)
+    want = math.pi * rho * E / 2
+    P = cam.width * cam.height
+    # the estimate per sample is bounded by a few times its mean; a 1% band is > 6 sigma here
+    assert abs(nee.mean() - want) < 0.01 * want
+    plain, _ = O.render(sc, cam, make_settings(spp=N, max_depth=4, seed=4))
+    sd = 2 * math.pi * rho * E * math.sqrt(1 / 9 - 1 / 16) / math.sqrt(N * P)
+    assert abs(plain.mean() - want) < 4 * sd
+
+
+def test_light_sampling_cornell_matches_plain_mean():
+    """Cornell box, whole frame: the light-sampled image converges to the
+    plain image (same expectation), checked on the frame mean."""
+    sc = scenes.cornell_box()
+    cam = H.cornell_camera(24, 24)
+    plain, _ = O.render(sc, cam, make_settings(spp=1024, max_depth=5, seed=1))
+    nee, _, _ = O.render_cached(sc, cam, make_settings(spp=256, max_depth=5, seed=2), np.zeros(3), 1, 0,
+                                query=False, nee=True)
+    assert abs(nee.mean() - plain.mean()) < 0.03 * plain.mean()
+    # and it is the lower-variance estimate: its per-pixel error against the plain 1024-spp image is smaller
+    plain2, _ = O.render(sc, cam, make_settings(spp=256, max_depth=5, seed=3))
+    assert np.sqrt(np.mean((nee[0] - plain) ** 2)) < np.sqrt(np.mean((plain2 - plain) ** 2))
+
+
+def test_training_records_hold_the_rest_of_the_path():
+    """A record's target is the rest of its path's contribution from the
+    query vertex on.  With the query vertex at the primary hit (index 0, an
+    oracle-only setting) and every sample a training sample, the targets
+    summed per pixel times 2 pi / N reproduce the plain image; no record is
+    made at a vertex that is a light."""
+    rho, E = 0.6, 2.0
+    sc = scenes.closed_room(scenes.material((rho, rho, rho)), scenes.material((E, E, E), light=True))
+    cam = _down_camera(8, 8)
+    s = make_settings(spp=4, max_depth=4, seed=4)
+    # vertex 1 is always a light here: no records
+    _, _, nr, f, t, _ = O.render_cached(sc, cam, s, np.zeros(3), 1, 1, records=True)
+    assert nr == 0 and f.shape[0] == 0
+    # q = 0: every sample records at the floor, its target is its whole contribution
+    rgb, _, nr0, f0, t0, _ = O.render_cached(sc, cam, s, np.zeros(3), 0, 1, records=True)
+    assert nr0 == 8 * 8 * 4
+    plain, _ = O.render(sc, cam, s)
+    per_pixel = t0.reshape(8 * 8, 4, 3).sum(axis=1).reshape(8, 8, 3)
+    scale = np.float32(2 * math.pi * (1.0 / 4))
+    np.testing.assert_allclose(per_pixel * scale, plain, rtol=1e-5)
+    np.testing.assert_array_equal(rgb[0], plain)
+
+
+@pytest.mark.parametrize("nee", [False, True])
+def test_record_target_is_the_rest_of_the_path(nee):
+    """One sample per pixel, query vertex 1.  The random numbers do not
+    depend on whether a sample queries or trains, so per pixel, with I0 the
+    image with a zero cache, I1 with a cache of 1 and It with every sample
+    training: I1 - I0 = s * T1 (s = 2 pi, T1 the throughput up to the query
+    vertex) and It - I0 = s * T1 * target.  The records must satisfy that."""
+    sc = scenes.cornell_box()
+    cam = H.cornell_camera(24, 24)
+    st = make_settings(spp=1, max_depth=6, seed=21)
+    I0, _, _ = O.render_cached(sc, cam, st, np.zeros(3), 1, 0, nee=nee)
+    I1, nq, _ = O.render_cached(sc, cam, st, np.ones(3), 1, 0, nee=nee)
+    It, _, nr, f, t, keys = O.render_cached(sc, cam, st, np.ones(3), 1, 1, nee=nee, records=True)
+    assert nr == nq > 100
+    w = (I1[0] - I0[0]).reshape(-1, 3)[keys].astype(np.float64)
+    dt = (It[0] - I0[0]).reshape(-1, 3)[keys].astype(np.float64)
+    # pixels that did not reach the query vertex have no record and identical images
+    rest = np.setdiff1d(np.arange(24 * 24), keys)
+    np.testing.assert_array_equal(I1[0].reshape(-1, 3)[rest], I0[0].reshape(-1, 3)[rest])
+    np.testing.assert_array_equal(It[0].reshape(-1, 3)[rest], I0[0].reshape(-1, 3)[rest])
+    scale = 1e-6 * (1.0 + np.abs(dt) + np.abs(I0[0].reshape(-1, 3)[keys]))
+    assert np.all(np.abs(w * t - dt) <= scale + 1e-5 * np.abs(w * t)), np.max(np.abs(w * t - dt))
+    assert (t > 0).any()
