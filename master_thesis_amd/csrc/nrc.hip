@@ -59,17 +59,35 @@ struct Geo {
 
 __device__ __forceinline__ __bf16 tobf(float x) { return (__bf16)x; }
 
+// sin and cos of pi * x * 2^k for k = 0..11, interleaved: exact anchors at
+// k = 0, 4, 8 (sincos_2pi of the fractional part of x * 2^(k-1); the scaling
+// and the subtraction of the floor are exact), then three angle doublings
+// from each anchor (error below 1e-6, far under the BF16 step).
+__device__ __forceinline__ void encode_axis(float x, float* sc) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float y = ldexpf(x, 4 * a - 1);
+        float sn, cs;
+        sincos_2pi(y - floorf(y), sn, cs);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            sc[8 * a + 2 * j] = sn;
+            sc[8 * a + 2 * j + 1] = cs;
+            const float s2 = 2.0f * sn * cs, c2 = (cs - sn) * (cs + sn);
+            sn = s2;
+            cs = c2;
+        }
+    }
+}
+
 // Features of one sample, group g of four: g < 3 -> the 24 sin/cos features of
 // axis g; g = 3 -> normal, direction, zero padding (24 values).
 __device__ __forceinline__ void encode_group(const float* f9, int g, const Geo& G, __bf16* row) {
     if (g < 3) {
-        const float x = (f9[g] - G.lo[g]) * G.inv[g];
+        float sc[24];
+        encode_axis((f9[g] - G.lo[g]) * G.inv[g], sc);
 #pragma unroll
-        for (int k = 0; k < FREQS; ++k) {
-            const float a = ldexpf(x, k);
-            row[g * 24 + 2 * k] = tobf(sinpif(a));
-            row[g * 24 + 2 * k + 1] = tobf(cospif(a));
-        }
+        for (int j = 0; j < 24; ++j) row[g * 24 + j] = tobf(sc[j]);
     } else {
 #pragma unroll
         for (int j = 0; j < 6; ++j) row[72 + j] = tobf(f9[3 + j]);
@@ -150,41 +168,149 @@ __device__ __forceinline__ void hidden_forward(const __bf16* sW, const float* __
     }
 }
 
+// ---- inference ------------------------------------------------------------
+//
+// 512 threads (8 waves) share one copy of the BF16 weights in LDS; each wave
+// owns 64 samples of a 512-sample tile and its own activation buffer, so the
+// waves never wait for each other after the weights are loaded.  LDS layout
+// of a matrix with K columns: K/32 blocks of [rows][32], so that a lane's
+// operand fragment (8 consecutive k of one row) is one 16-byte load and a
+// fragment of 16 rows is 1 KB contiguous.  Every product is computed
+// transposed, C^T = W X^T (weights as the row operand), so a lane ends with
+// four consecutive output features of one sample: ReLU, BF16 and one 8-byte
+// store per 16x16 tile.
+constexpr int INF_NT = 512;                       // threads per workgroup
+constexpr int INF_TILE = INF_NT;                  // samples per workgroup tile
+constexpr int ABUF = 3 * 64 * 32;                 // bf16 per wave: 64 rows x 96 columns, blocked
+
+// blocked index of element (row, k) in a matrix of `rows` rows
+__device__ __forceinline__ int blk(int row, int k, int rows) { return (k >> 5) * rows * 32 + row * 32 + (k & 31); }
+
+// weights, row-major [n][k] per layer in global memory -> blocked in LDS
+__device__ __forceinline__ void load_weights_blocked(const __bf16* __restrict__ Wb, __bf16* sW) {
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        const int K = KD[l], N = ND[l];
+        const int chunks = N * K / 8;  // 16-byte chunks
+        for (int c = threadIdx.x; c < chunks; c += INF_NT) {
+            const int n = c / (K / 8), k8 = (c % (K / 8)) * 8;
+            *(uint4*)(sW + offwb(l) + blk(n, k8, N)) = *(const uint4*)(Wb + offwb(l) + n * K + k8);
+        }
+    }
+}
+
+// One layer for a wave's 64 samples: acc[mt][nt] = W[16nt.., :] . X[16mt.., :]^T
+// over K (in the wave's blocked buffer), NT_ n-tiles.
+template <int K, int N, int NT_>
+__device__ __forceinline__ void layer_mm(const __bf16* sWl, const __bf16* sA, int lane, f32x4 (&acc)[4][4]) {
+    const int mi = lane & 15, kg = lane >> 4;
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT_; ++nt) acc[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int ks = 0; ks < K / 32; ++ks) {
+        bf16x8 xa[4], wb[NT_];
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) xa[mt] = *(const bf16x8*)(sA + ks * 64 * 32 + (16 * mt + mi) * 32 + 8 * kg);
+#pragma unroll
+        for (int nt = 0; nt < NT_; ++nt) wb[nt] = *(const bf16x8*)(sWl + ks * N * 32 + (16 * nt + mi) * 32 + 8 * kg);
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < NT_; ++nt)
+                acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wb[nt], xa[mt], acc[mt][nt], 0, 0, 0);
+    }
+}
+
 // SCATTER: out[3 * idx[s] + c] += qw[3 * s + c] * max(y_c, 0) (atomically) in
 // place of out[3 * s + c] = y_c.
 template <bool SCATTER>
-__global__ void __launch_bounds__(NT) k_mlp_infer(const float* __restrict__ feats, uint32_t n,
-                                                  const __bf16* __restrict__ Wb, const float* __restrict__ Wf, Geo G,
-                                                  float* __restrict__ out, const float* __restrict__ qw,
-                                                  const uint32_t* __restrict__ idx) {
+__global__ void __launch_bounds__(INF_NT) k_mlp_infer(const float* __restrict__ feats, uint32_t n,
+                                                      const __bf16* __restrict__ Wb, const float* __restrict__ Wf,
+                                                      Geo G, float* __restrict__ out, const float* __restrict__ qw,
+                                                      const uint32_t* __restrict__ idx) {
     __shared__ __attribute__((aligned(16))) __bf16 sW[NWB];
-    __shared__ __attribute__((aligned(16))) __bf16 sX[TILE * 96];
-    __shared__ __attribute__((aligned(16))) __bf16 sH[2][TILE * 64];
-    load_weights(Wb, sW);
+    __shared__ __attribute__((aligned(16))) __bf16 sAct[INF_NT / 64][ABUF];
+    load_weights_blocked(Wb, sW);
+    __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int mi = lane & 15, kg = lane >> 4;
-    const uint32_t tiles = (n + TILE - 1) / TILE;
+    __bf16* sA = sAct[w];
+    const uint32_t tiles = (n + INF_TILE - 1) / INF_TILE;
     for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
-        const uint32_t s0 = t * TILE;
-        __syncthreads();  // previous tile done with sX / sH (and weights loaded)
-        encode_tile(feats, s0, n, G, sX);
-        __syncthreads();
-        hidden_forward(sW, Wf, sX, [&](int l) { return sH[l & 1]; });
-        // output layer (reads the last hidden layer, sH[(L - 2) & 1]); each wave its 16 rows
-        f32x4 acc = tile_mm(sH[(L - 2) & 1], 64, 1, sW + offwb(L - 1), 1, 64, 16 * w, 0, 64, 64, lane);
-        const int col = mi;
-        if (col < N5_VALID) {
-            const float b = Wf[offb(L - 1) + col];
+        const uint32_t s0 = t * INF_TILE + 64 * w;  // this wave's first sample
+        // encode one sample per lane into its row
+        {
+            const uint32_t sm = s0 + lane;
+            float f9[9];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const uint32_t s = s0 + 16 * w + 4 * kg + i;
-                if (s < n) {
-                    const float y = acc[i] + b;
-                    if (SCATTER) atomicAdd(&out[3 * (size_t)idx[s] + col], qw[3 * (size_t)s + col] * fmaxf(y, 0.0f));
-                    else out[(size_t)s * 3 + col] = y;
+            for (int j = 0; j < 9; ++j) f9[j] = sm < n ? feats[(size_t)sm * 9 + j] : 0.0f;
+            float v[96];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) encode_axis((f9[a] - G.lo[a]) * G.inv[a], v + 24 * a);
+#pragma unroll
+            for (int j = 0; j < 6; ++j) v[72 + j] = f9[3 + j];
+#pragma unroll
+            for (int j = 78; j < 96; ++j) v[j] = 0.0f;
+            if (sm >= n) {
+#pragma unroll
+                for (int j = 0; j < 96; ++j) v[j] = 0.0f;
+            }
+#pragma unroll
+            for (int c = 0; c < 12; ++c) {
+                bf16x8 pk;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) pk[j] = tobf(v[8 * c + j]);
+                *(bf16x8*)(sA + blk(lane, 8 * c, 64)) = pk;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // hidden layers, in place in the wave's buffer
+        f32x4 acc[4][4];
+#pragma unroll
+        for (int l = 0; l < L - 1; ++l) {
+            if (l == 0) layer_mm<96, 64, 4>(sW + offwb(0), sA, lane, acc);
+            else layer_mm<64, 64, 4>(sW + offwb(l), sA, lane, acc);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                const float4 b = *(const float4*)(Wf + offb(l) + 16 * nt + 4 * kg);
+#pragma unroll
+                for (int mt = 0; mt < 4; ++mt) {
+                    typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+                    bf16x4 h;
+                    h[0] = tobf(fmaxf(acc[mt][nt][0] + b.x, 0.0f));
+                    h[1] = tobf(fmaxf(acc[mt][nt][1] + b.y, 0.0f));
+                    h[2] = tobf(fmaxf(acc[mt][nt][2] + b.z, 0.0f));
+                    h[3] = tobf(fmaxf(acc[mt][nt][3] + b.w, 0.0f));
+                    *(bf16x4*)(sA + blk(16 * mt + mi, 16 * nt + 4 * kg, 64)) = h;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        // output layer: one n-tile; lanes with kg == 0 hold outputs 0..3 of sample 16 mt + mi
+        f32x4 o5[4][4];
+        layer_mm<64, 16, 1>(sW + offwb(L - 1), sA, lane, o5);
+        if (kg == 0) {
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+                const uint32_t sm = s0 + 16 * mt + mi;
+                if (sm < n) {
+#pragma unroll
+                    for (int c = 0; c < N5_VALID; ++c) {
+                        const float y = o5[mt][0][c] + Wf[offb(L - 1) + c];
+                        if (SCATTER) atomicAdd(&out[3 * (size_t)idx[sm] + c], qw[3 * (size_t)sm + c] * fmaxf(y, 0.0f));
+                        else out[(size_t)sm * 3 + c] = y;
+                    }
                 }
             }
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
@@ -429,9 +555,9 @@ namespace rt {
 int nrc_infer_device(rt_nrc* s, const float* d_feats, uint32_t n, float* d_out) {
     rt_ctx* c = s->ctx;
     if (n == 0) return RT_OK;
-    const uint32_t tiles = (n + TILE - 1) / TILE;
+    const uint32_t tiles = (n + INF_TILE - 1) / INF_TILE;
     const uint32_t grid = tiles < (uint32_t)s->infer_grid ? tiles : (uint32_t)s->infer_grid;
-    hipLaunchKernelGGL(k_mlp_infer<false>, dim3(grid), dim3(NT), 0, c->stream, d_feats, n, (const __bf16*)s->wb.ptr,
+    hipLaunchKernelGGL(k_mlp_infer<false>, dim3(grid), dim3(INF_NT), 0, c->stream, d_feats, n, (const __bf16*)s->wb.ptr,
                        (const float*)s->wf.ptr, s->geo, d_out, nullptr, nullptr);
     RT_HIP(c, hipGetLastError());
     return RT_OK;
@@ -440,9 +566,9 @@ int nrc_infer_scatter(rt_nrc* s, const float* d_feats, const float* d_w, const u
                       float* d_acc) {
     rt_ctx* c = s->ctx;
     if (n == 0) return RT_OK;
-    const uint32_t tiles = (n + TILE - 1) / TILE;
+    const uint32_t tiles = (n + INF_TILE - 1) / INF_TILE;
     const uint32_t grid = tiles < (uint32_t)s->infer_grid ? tiles : (uint32_t)s->infer_grid;
-    hipLaunchKernelGGL(k_mlp_infer<true>, dim3(grid), dim3(NT), 0, c->stream, d_feats, n, (const __bf16*)s->wb.ptr,
+    hipLaunchKernelGGL(k_mlp_infer<true>, dim3(grid), dim3(INF_NT), 0, c->stream, d_feats, n, (const __bf16*)s->wb.ptr,
                        (const float*)s->wf.ptr, s->geo, d_acc, d_w, d_idx);
     RT_HIP(c, hipGetLastError());
     return RT_OK;
@@ -546,7 +672,7 @@ int rt_nrc_create(rt_ctx* c, const rt_nrc_config* cfg, rt_nrc** out) {
     hipLaunchKernelGGL(k_refresh_bf16, dim3((NPARAM + 255) / 256), dim3(256), 0, c->stream, (const float*)s->wf.ptr,
                        (__bf16*)s->wb.ptr);
     int per_cu = 0;
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mlp_infer<false>, NT, 0);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mlp_infer<false>, INF_NT, 0);
     s->infer_grid = c->num_cus * (per_cu > 0 ? per_cu : 1);
     e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
