@@ -13,11 +13,13 @@
 // gradient).  Optimiser: Adam on FP32 master weights; the BF16 copy the
 // kernels read is refreshed after every step.
 //
-// Kernels (one workgroup = 4 waves = one tile of 64 samples):
-//   k_mlp_infer  weights -> LDS once, then tiles: encode, 6 layers
-//   k_mlp_train  one tile per workgroup: encode, forward (all activations
-//                kept in LDS), loss, backward; each workgroup writes its
-//                partial gradient (no atomics)
+// Kernels:
+//   k_mlp_infer  8 waves, weights -> LDS once, then persistent tiles of 512
+//                samples (64 per wave): encode, 6 layers
+//   k_mlp_train  4 waves, tiles of 64 samples, at most one workgroup per CU
+//                striding over the tiles: encode, forward (all activations
+//                kept in LDS), loss, backward; each workgroup accumulates
+//                its own partial gradient (no atomics)
 //   k_adam       sums the partials in workgroup order (deterministic), Adam
 //                update, BF16 refresh
 // Every GEMM is a sum of 16x16 output tiles of v_mfma_f32_16x16x32_bf16.
@@ -325,9 +327,17 @@ __global__ void __launch_bounds__(NT) k_mlp_train(const float* __restrict__ feat
     __shared__ float sLoss[NT / 64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int mi = lane & 15, kg = lane >> 4;
-    const uint32_t s0 = blockIdx.x * TILE;
     float* part = partial + (size_t)blockIdx.x * NPARAM;
     load_weights(Wb, sW);
+    // tiles blockIdx.x, + gridDim.x, ...: the workgroup's partial gradient is
+    // the sum over its tiles in that order (each element is always written by
+    // the same thread, so the read-modify-write needs no synchronisation)
+    float lacc = 0.0f;
+    const uint32_t tiles = (n + TILE - 1) / TILE;
+    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    const bool first = tile == blockIdx.x;
+    const uint32_t s0 = tile * TILE;
+    __syncthreads();  // the previous tile's backward pass is done with sX / sH / sD
     encode_tile(feats, s0, n, G, sX);
     __syncthreads();
     hidden_forward(sW, Wf, sX, [&](int l) { return sH[l]; });
@@ -357,18 +367,22 @@ __global__ void __launch_bounds__(NT) k_mlp_train(const float* __restrict__ feat
     for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
     if (lane == 0) sLoss[w] = lsum;
     __syncthreads();
-    if (threadIdx.x == 0) loss_part[blockIdx.x] = sLoss[0] + sLoss[1] + sLoss[2] + sLoss[3];
+    if (threadIdx.x == 0) lacc += sLoss[0] + sLoss[1] + sLoss[2] + sLoss[3];
 
     // layer 5: dW5[n][k] = sum_s dY[s][n] H4[s][k] (16 x 64), one 16-column tile per wave
     {
         f32x4 acc = tile_mm(sD[0], 1, 16, sH[L - 2], 64, 1, 0, 16 * w, 64, 64, lane);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) part[offw(L - 1) + (4 * kg + i) * 64 + 16 * w + mi] = acc[i];
+        for (int i = 0; i < 4; ++i) {
+            float* q = part + offw(L - 1) + (4 * kg + i) * 64 + 16 * w + mi;
+            *q = first ? acc[i] : *q + acc[i];
+        }
     }
     if (threadIdx.x < 16) {
         float s = 0.0f;
         for (int r = 0; r < TILE; ++r) s += (float)sD[0][r * 16 + threadIdx.x];
-        part[offb(L - 1) + threadIdx.x] = s;
+        float* q = part + offb(L - 1) + threadIdx.x;
+        *q = first ? s : *q + s;
     }
     // dH4[s][k] = sum_n dY[s][n] W5[n][k] (K = 16 valid of 32), masked by H4 > 0 -> sD[1]
     for (int nt = 0; nt < 4; ++nt) {
@@ -392,12 +406,16 @@ __global__ void __launch_bounds__(NT) k_mlp_train(const float* __restrict__ feat
         for (int kt = 0; kt < K / 16; ++kt) {
             f32x4 acc = tile_mm(dZ, 1, 64, in, K, 1, 16 * w, 16 * kt, 64, 64, lane);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) part[offw(l) + (16 * w + 4 * kg + i) * K + 16 * kt + mi] = acc[i];
+            for (int i = 0; i < 4; ++i) {
+                float* q = part + offw(l) + (16 * w + 4 * kg + i) * K + 16 * kt + mi;
+                *q = first ? acc[i] : *q + acc[i];
+            }
         }
         if (threadIdx.x < 64) {
             float s = 0.0f;
             for (int r = 0; r < TILE; ++r) s += (float)dZ[r * 64 + threadIdx.x];
-            part[offb(l) + threadIdx.x] = s;
+            float* q = part + offb(l) + threadIdx.x;
+            *q = first ? s : *q + s;
         }
         if (l > 0) {
             // dH_{l-1}[s][k] = sum_n dZ[s][n] W_l[n][k], masked by H_{l-1} > 0
@@ -416,6 +434,8 @@ __global__ void __launch_bounds__(NT) k_mlp_train(const float* __restrict__ feat
             cur ^= 1;
         }
     }
+    }  // tiles
+    if (threadIdx.x == 0) loss_part[blockIdx.x] = lacc;
 }
 
 // Adam with bias correction on the summed partial gradients (workgroup order,
@@ -427,7 +447,15 @@ __global__ void k_adam(const float* __restrict__ partial, uint32_t nparts, float
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= NPARAM) return;
     float g = 0.0f;
-    for (uint32_t w = 0; w < nparts; ++w) g += partial[(size_t)w * NPARAM + p];
+    uint32_t w = 0;
+    for (; w + 8 <= nparts; w += 8) {  // eight loads in flight, summed in order
+        float v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = partial[(size_t)(w + j) * NPARAM + p];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) g += v[j];
+    }
+    for (; w < nparts; ++w) g += partial[(size_t)w * NPARAM + p];
     grad[p] = g;
     if (!update) return;
     const float mm = b1 * m[p] + (1.0f - b1) * g;
@@ -514,7 +542,10 @@ int run_train(rt_nrc* s, const float* feats, const float* targets, uint32_t n, i
     if ((rc = stage(s, s->feats, feats, (size_t)n * 36, ptr_kind, &df)) ||
         (rc = stage(s, s->targets, targets, (size_t)n * 12, ptr_kind, &dt)))
         return rc;
-    const uint32_t wgs = (n + TILE - 1) / TILE;
+    const uint32_t tiles = (n + TILE - 1) / TILE;
+    // one workgroup per CU at most (the kernel holds 116 KB of LDS); each
+    // strides over tiles, so k_adam sums at most num_cus partial gradients
+    const uint32_t wgs = tiles < (uint32_t)c->num_cus ? tiles : (uint32_t)c->num_cus;
     if ((rc = ensure(c, s->partial, (size_t)wgs * NPARAM * 4)) || (rc = ensure(c, s->loss, (size_t)wgs * 4))) return rc;
     RT_HIP(c, hipEventRecord(c->ev[2], c->stream));
     hipLaunchKernelGGL(k_mlp_train, dim3(wgs), dim3(NT), 0, c->stream, (const float*)df, (const float*)dt, n,
