@@ -4,10 +4,12 @@
 // bounding_volume_hierachy.cpp:9-27,47-75: repeatedly merge the closest pair of
 // centroids, O(n^3)) with a linear BVH:
 //   1. 30-bit Morton code of each triangle's box centre (k_morton)
-//   2. LSD radix sort, 4 passes of 8 bits, stable (k_rs_hist/k_rs_scan/k_rs_scatter)
+//   2. LSD radix sort, 4 passes of 8 bits, stable (k_rs_hist, k_scan_*,
+//      k_rs_scatter)
 //   3. binary hierarchy from the sorted codes (Karras, "Maximizing parallelism
 //      in the construction of BVHs, octrees, and k-d trees", HPG 2012) (k_karras)
-//   4. bounds bottom-up, second arrival at each node unions its children (k_bounds)
+//   4. node boxes as unions of their sorted leaf ranges: block unions, a
+//      sparse table over blocks, one range query per node (k_node_boxes)
 //   5. top-down collapse to 4-wide nodes: a node opens its largest-area inner
 //      child until it has four children; subtrees of <= leaf_size triangles
 //      become leaves over a contiguous range of the sorted order (k_collapse)
@@ -82,28 +84,68 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_hist(const uint32_t* __restrict
     hist[(uint32_t)t * nblocks + blockIdx.x] = cnt[t];
 }
 
-// Exclusive scan of `total` counters in place, one block of 1024 threads.
-__global__ void __launch_bounds__(1024) k_rs_scan(uint32_t* __restrict__ h, uint32_t total) {
-    __shared__ uint32_t part[1024];
+// Exclusive scan of `total` counters in place, in three coalesced passes:
+// every 1024-counter tile is scanned by one block (k_scan_tiles), the tile
+// totals by one block (k_scan_totals), and the totals' prefixes added back
+// (k_scan_add).
+constexpr uint32_t SCAN_TILE = 1024;
+
+// exclusive scan of 256 values held one per thread; returns the total
+__device__ __forceinline__ uint32_t block_scan_256(uint32_t& v, uint32_t* sh) {
     const uint32_t t = threadIdx.x;
-    const uint32_t chunk = (total + 1023u) / 1024u;
-    const uint32_t b = t * chunk, e = min(total, b + chunk);
-    uint32_t s = 0;
-    for (uint32_t i = b; i < e; ++i) s += h[i];
-    part[t] = s;
+    sh[t] = v;
     __syncthreads();
-    for (uint32_t off = 1; off < 1024; off <<= 1) {
-        uint32_t v = t >= off ? part[t - off] : 0u;
+    for (uint32_t off = 1; off < 256; off <<= 1) {
+        const uint32_t x = t >= off ? sh[t - off] : 0u;
         __syncthreads();
-        part[t] += v;
+        sh[t] += x;
         __syncthreads();
     }
-    uint32_t run = part[t] - s;  // exclusive prefix of this chunk
-    for (uint32_t i = b; i < e; ++i) {
-        uint32_t x = h[i];
-        h[i] = run;
-        run += x;
+    const uint32_t incl = sh[t], total = sh[255];
+    __syncthreads();
+    v = incl - v;
+    return total;
+}
+
+__global__ void __launch_bounds__(256) k_scan_tiles(uint32_t* __restrict__ h, uint32_t total,
+                                                    uint32_t* __restrict__ tsum) {
+    __shared__ uint32_t sh[256];
+    const uint32_t base = blockIdx.x * SCAN_TILE + 4u * threadIdx.x;
+    uint32_t x[4], s = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        x[j] = base + j < total ? h[base + j] : 0u;
+        s += x[j];
     }
+    uint32_t pre = s;
+    const uint32_t tot = block_scan_256(pre, sh);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (base + j < total) h[base + j] = pre;
+        pre += x[j];
+    }
+    if (threadIdx.x == 0) tsum[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(256) k_scan_totals(uint32_t* __restrict__ tsum, uint32_t ntiles) {
+    __shared__ uint32_t sh[256];
+    uint32_t carry = 0;
+    for (uint32_t b = 0; b < ntiles; b += 256) {
+        const uint32_t i = b + threadIdx.x;
+        uint32_t v = i < ntiles ? tsum[i] : 0u;
+        const uint32_t tot = block_scan_256(v, sh);
+        if (i < ntiles) tsum[i] = v + carry;
+        carry += tot;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_scan_add(uint32_t* __restrict__ h, uint32_t total,
+                                                  const uint32_t* __restrict__ tsum) {
+    const uint32_t add = tsum[blockIdx.x];
+    const uint32_t base = blockIdx.x * SCAN_TILE + 4u * threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        if (base + j < total) h[base + j] += add;
 }
 
 // Stable scatter: each of the RS_ITEMS rounds ranks 256 consecutive keys.
@@ -199,26 +241,70 @@ __device__ __forceinline__ void load_box(uint32_t ref, const uint32_t* __restric
     for (int k = 0; k < 6; ++k) b[k] = __builtin_nontemporal_load(src + k);
 }
 
-__global__ void k_bounds(int n, Inner in, const uint32_t* __restrict__ vals, const float* leafbox,
-                         float* innerbox, uint32_t* visit) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    uint32_t p = in.lparent[j];
-    // a binary tree over n <= 2^27 leaves built from 30-bit codes + index
-    // bits is at most 64 levels deep; the bound only guards against a bug
-    for (int level = 0; level < 96; ++level) {
-        uint32_t old = __hip_atomic_fetch_add(&visit[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old == 0) return;  // the sibling subtree is not finished; its thread continues
-        float a[6], b[6];
-        load_box(in.left[p], vals, leafbox, innerbox, a);
-        load_box(in.right[p], vals, leafbox, innerbox, b);
-        for (int k = 0; k < 3; ++k) {
-            innerbox[6u * p + k] = fminf(a[k], b[k]);
-            innerbox[6u * p + 3 + k] = fmaxf(a[3 + k], b[3 + k]);
-        }
-        if (p == 0) return;
-        p = in.iparent[p];
+// Node boxes as range unions.  A Karras node covers the contiguous sorted
+// leaves [first, last], so its box is the union of those leaves' boxes; min
+// and max are exact, so this equals the union of its children's boxes
+// without a bottom-up walk (whose per-node device-scope synchronisation
+// dominated the build).  Leaves are grouped in blocks of 64; a sparse table
+// over block boxes answers any run of whole blocks with two lookups, and a
+// node adds at most 126 single leaves at the ends.
+constexpr uint32_t BOX_BLOCK = 64;
+
+__device__ __forceinline__ void box_union(float a[6], const float* b) {
+    for (int k = 0; k < 3; ++k) {
+        a[k] = fminf(a[k], b[k]);
+        a[3 + k] = fmaxf(a[3 + k], b[3 + k]);
     }
+}
+
+// sorted leaf boxes and the unions of each block of 64 (table level 0)
+__global__ void k_sorted_boxes(uint32_t n, const uint32_t* __restrict__ vals, const float* __restrict__ leafbox,
+                               float* __restrict__ sbox) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* src = leafbox + 6u * vals[i];
+    for (int k = 0; k < 6; ++k) sbox[6u * i + k] = src[k];
+}
+
+__global__ void k_block_boxes(uint32_t n, uint32_t nb, const float* __restrict__ sbox, float* __restrict__ t0) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nb) return;
+    float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    const uint32_t e = min(n, (j + 1) * BOX_BLOCK);
+    for (uint32_t i = j * BOX_BLOCK; i < e; ++i) box_union(b, sbox + 6u * i);
+    for (int k = 0; k < 6; ++k) t0[6u * j + k] = b[k];
+}
+
+// table level k: union of level k-1 at j and j + half (where that exists)
+__global__ void k_box_table_level(uint32_t nb, uint32_t half, const float* __restrict__ prev, float* __restrict__ cur) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nb) return;
+    float b[6];
+    for (int k = 0; k < 6; ++k) b[k] = prev[6u * j + k];
+    if (j + half < nb) box_union(b, prev + 6u * (j + half));
+    for (int k = 0; k < 6; ++k) cur[6u * j + k] = b[k];
+}
+
+__global__ void k_node_boxes(uint32_t ninner, Inner in, const float* __restrict__ sbox, const float* __restrict__ table,
+                             uint32_t nb, float* __restrict__ innerbox) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= ninner) return;
+    const uint32_t a = in.first[p], z = in.last[p];
+    float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    const uint32_t ja = (a + BOX_BLOCK - 1) / BOX_BLOCK;   // first whole block
+    const uint32_t jz1 = (z + 1) / BOX_BLOCK;               // one past the last whole block
+    if (ja >= jz1) {
+        for (uint32_t i = a; i <= z; ++i) box_union(b, sbox + 6u * i);
+    } else {
+        for (uint32_t i = a; i < ja * BOX_BLOCK; ++i) box_union(b, sbox + 6u * i);
+        for (uint32_t i = jz1 * BOX_BLOCK; i <= z; ++i) box_union(b, sbox + 6u * i);
+        const uint32_t len = jz1 - ja;
+        const uint32_t k = 31u - (uint32_t)__builtin_clz(len);
+        const float* lvl = table + (size_t)k * nb * 6u;
+        box_union(b, lvl + 6u * ja);
+        box_union(b, lvl + 6u * (jz1 - (1u << k)));
+    }
+    for (int k = 0; k < 6; ++k) innerbox[6u * p + k] = b[k];
 }
 
 struct Entry {
@@ -357,7 +443,7 @@ int build_lbvh(rt_ctx* c, uint32_t leaf_size) {
     if ((rc = ensure(c, c->keys[0], 4 * N)) || (rc = ensure(c, c->keys[1], 4 * N)) ||
         (rc = ensure(c, c->vals[0], 4 * N)) || (rc = ensure(c, c->vals[1], 4 * N)) ||
         (rc = ensure(c, c->leafbox, 24 * N)) || (rc = ensure(c, c->inner, 6 * 4 * N)) ||
-        (rc = ensure(c, c->innerbox, 24 * N)) || (rc = ensure(c, c->visit, 4 * N)) ||
+        (rc = ensure(c, c->innerbox, 24 * N)) ||
         (rc = ensure(c, c->queue[0], 16 * (N / 2 + 2))) || (rc = ensure(c, c->queue[1], 16 * (N / 2 + 2))) ||
         (rc = ensure(c, c->counters, 64)) || (rc = ensure(c, c->nodes, sizeof(rt_bvh4_node) * (N + 1))) ||
         (rc = ensure(c, c->tris, sizeof(rt_tri_rec) * N)))
@@ -383,13 +469,17 @@ int build_lbvh(rt_ctx* c, uint32_t leaf_size) {
                        inv[2], keys[0], vals[0], leafbox);
 
     const uint32_t nblocks = blocks_for(n, RS_TILE);
-    if ((rc = ensure(c, c->hist, 4ull * 256 * nblocks))) return rc;
+    const uint32_t scan_tiles = (256u * nblocks + SCAN_TILE - 1) / SCAN_TILE;
+    if ((rc = ensure(c, c->hist, 4ull * 256 * nblocks + 4ull * scan_tiles))) return rc;
     uint32_t* hist = (uint32_t*)c->hist.ptr;
+    uint32_t* tsum = hist + 256u * nblocks;
     int cur = 0;
     for (int pass = 0; pass < 4; ++pass) {
         int shift = pass * 8;
         hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(RS_BLOCK), 0, st, keys[cur], n, shift, nblocks, hist);
-        hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, st, hist, 256u * nblocks);
+        hipLaunchKernelGGL(k_scan_tiles, dim3(scan_tiles), dim3(256), 0, st, hist, 256u * nblocks, tsum);
+        hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(256), 0, st, tsum, scan_tiles);
+        hipLaunchKernelGGL(k_scan_add, dim3(scan_tiles), dim3(256), 0, st, hist, 256u * nblocks, (const uint32_t*)tsum);
         hipLaunchKernelGGL(k_rs_scatter, dim3(nblocks), dim3(RS_BLOCK), 0, st, keys[cur], vals[cur], keys[cur ^ 1],
                            vals[cur ^ 1], n, shift, nblocks, (const uint32_t*)hist);
         cur ^= 1;
@@ -412,9 +502,20 @@ int build_lbvh(rt_ctx* c, uint32_t leaf_size) {
     } else {
         hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1, 256)), dim3(256), 0, st, (const uint32_t*)keys[cur], (int)n,
                            in);
-        RT_HIP(c, hipMemsetAsync(c->visit.ptr, 0, 4 * N, st));
-        hipLaunchKernelGGL(k_bounds, dim3(blocks_for(n, 256)), dim3(256), 0, st, (int)n, in,
-                           (const uint32_t*)vals[cur], (const float*)leafbox, innerbox, (uint32_t*)c->visit.ptr);
+        const uint32_t nb = (n + BOX_BLOCK - 1) / BOX_BLOCK;
+        uint32_t levels = 1;
+        while ((1u << levels) <= nb) ++levels;  // table levels 0 .. floor(log2(nb))
+        if ((rc = ensure(c, c->boxtab, 24ull * N + 24ull * nb * levels))) return rc;
+        float* sbox = (float*)c->boxtab.ptr;
+        float* table = sbox + 6ull * N;
+        hipLaunchKernelGGL(k_sorted_boxes, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, (const uint32_t*)vals[cur],
+                           (const float*)leafbox, sbox);
+        hipLaunchKernelGGL(k_block_boxes, dim3(blocks_for(nb, 256)), dim3(256), 0, st, n, nb, (const float*)sbox, table);
+        for (uint32_t k = 1; k < levels; ++k)
+            hipLaunchKernelGGL(k_box_table_level, dim3(blocks_for(nb, 256)), dim3(256), 0, st, nb, 1u << (k - 1),
+                               (const float*)(table + 6ull * nb * (k - 1)), table + 6ull * nb * k);
+        hipLaunchKernelGGL(k_node_boxes, dim3(blocks_for(n - 1, 256)), dim3(256), 0, st, n - 1, in, (const float*)sbox,
+                           (const float*)table, nb, innerbox);
         uint32_t init[4] = {1u, 0u, 0u, 0u};
         RT_HIP(c, hipMemcpyAsync(counters, init, sizeof(init), hipMemcpyHostToDevice, st));
         uint4 root = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);

@@ -146,7 +146,7 @@ void rt_destroy(rt_ctx* c) {
     for (rt_nrc* n : caches) nrc_detach(n);
     DevBuf* all[] = {&c->pos, &c->idx, &c->mat, &c->shade_geom, &c->shade_nrm, &c->nodes, &c->tris,
                      &c->keys[0], &c->keys[1], &c->vals[0], &c->vals[1], &c->hist, &c->inner, &c->leafbox,
-                     &c->innerbox, &c->visit, &c->queue[0], &c->queue[1], &c->counters, &c->rays, &c->hits,
+                     &c->innerbox, &c->boxtab, &c->queue[0], &c->queue[1], &c->counters, &c->rays, &c->hits,
                      &c->spill, &c->work, &c->stats, &c->pt_state, &c->pt_queue[0], &c->pt_queue[1],
                      &c->pt_image, &c->pt_img8, &c->pt_dimg, &c->pt_dcolor, &c->pt_ctr, &c->pt_partial,
                      &c->pt_cams, &c->pt_nrc_query, &c->pt_nrc_train, &c->pt_shadow, &c->lights};

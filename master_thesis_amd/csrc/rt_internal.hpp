@@ -61,7 +61,7 @@ struct rt_ctx {
     uint32_t num_nodes = 0, bvh_depth = 0;
     rt_bvh_info info{};
     // build scratch
-    rt::DevBuf keys[2], vals[2], hist, inner, leafbox, innerbox, visit, queue[2], counters;
+    rt::DevBuf keys[2], vals[2], hist, inner, leafbox, innerbox, boxtab, queue[2], counters;
     // trace scratch
     rt::DevBuf rays, hits, spill, work, stats;
     uint32_t trace_refill_min = 8;  // traversal: idle lanes that trigger a ray refill (MTRT_TRACE_REFILL)
