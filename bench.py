@@ -50,7 +50,7 @@ def main() -> int:
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 512 MiB device copy (reference kernel for HBM counters)")
     ap.add_argument("--config", default="raycast",
-                    choices=["raycast", "terrain_raycast", "cornell", "terrain", "diff", "nrc"],
+                    choices=["raycast", "terrain_raycast", "cornell", "terrain", "diff", "nrc", "bdpt"],
                     help="raycast: the headline microbench (default); cornell / terrain / diff / nrc: the other "
                          "workloads")
     ap.add_argument("--nrc-scene", default="terrain",
@@ -68,7 +68,7 @@ def main() -> int:
         return 2
     if args.config not in ("raycast", "terrain_raycast"):
         out = {"cornell": bench_cornell, "terrain": bench_terrain, "diff": bench_diff,
-               "nrc": bench_nrc}[args.config](args)
+               "nrc": bench_nrc, "bdpt": bench_bdpt}[args.config](args)
         print(json.dumps(out), flush=True)
         return 0
     if args.config == "terrain_raycast":
@@ -327,6 +327,44 @@ def bench_cornell(args):
             "dtype": "f32", "data": "synthetic", "n_gpus": 1,
             "config": {"workload": "Cornell-style box: 5 walls, 2 boxes, 1 light (36 triangles), FOV 40, 256x256, "
                                    f"{spp} spp, depth 5", "triangles": int(sc.num_triangles)},
+            "cpu_baseline": cpu}
+
+
+def bench_bdpt(args):
+    """The bidirectional tracer on the Cornell-style box, 256 x 256, with the
+    reference's SoftwareRendererBidirectionalPathTracer settings (vision 5,
+    light 5, maxDepth 5; 16 spp here): spp/s on the GPU, the oracle's
+    restatement on the host cores over a 64 x 64 crop of the same frame."""
+    from master_thesis_amd import scenes, host_lib as H
+    from master_thesis_amd.tracer import Tracer, make_settings
+    spp = args.spp or 16
+    sc = scenes.cornell_box()
+    cam = H.cornell_camera(256, 256)
+    with Tracer(0) as t:
+        t.upload(sc)
+        t.build(args.leaf)
+        t.render_bdpt(cam, spp, 5, 5, 5, seed=3)
+        ms = []
+        for r in range(5):
+            t.render_bdpt(cam, spp, 5, 5, 5, seed=3, sample_offset=spp * r)
+            ms.append(t.timing()["last_call_ms"])
+        ms = float(np.median(ms))
+    P = 256 * 256
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        threads = min(16, os.cpu_count() or 1)
+        crop = H.cornell_camera(64, 64)
+        t0 = time.time()
+        O.render_bdpt(sc, crop, make_settings(spp=spp, max_depth=5, seed=3), 5, 5, 5, threads)
+        dt = time.time() - t0
+        cpu = {"value": round(64 * 64 * spp / dt, 1), "unit": "spp/s", "cores": threads, "kind": "port",
+               "sample": f"a 64x64 view of the same box at {spp} spp with the oracle's CPU BVH"}
+    return {"metric": f"spp/s (bidirectional tracer, Cornell-style box, 256x256, {spp} spp, 5/5/5)",
+            "value": round(P * spp / (ms / 1e3), 1), "unit": "spp/s", "ms_per_frame": round(ms, 3),
+            "higher_is_better": True, "dtype": "f32", "data": "synthetic", "n_gpus": 1,
+            "config": {"workload": f"Cornell-style box (36 triangles), FOV 40, 256x256, {spp} spp, visionJumpCount 5, "
+                                   "lightJumpCount 5, maxDepth 5", "triangles": int(sc.num_triangles)},
             "cpu_baseline": cpu}
 
 

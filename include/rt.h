@@ -245,6 +245,29 @@ int  rt_render(rt_ctx* ctx, const rt_camera* cam, const rt_settings* s,
 int  rt_render_views(rt_ctx* ctx, const rt_camera* cams, uint32_t num_views, const rt_settings* s,
                      float* rgb, uint8_t* rgb8, int ptr_kind);
 
+/* BidirectionalPathTracer (graphic/bidirectional_path_tracer.cpp:13-168):
+ * per sample a vision path of up to vision_jumps vertices (as the path
+ * tracer's, recording every vertex), and unless it ends on a light, a light
+ * path from a random point of a random light object (object, then triangle
+ * uniformly, sqrt-warped barycentrics) of up to light_jumps vertices, then
+ * every vision/light vertex pair with vi + li <= max_depth joined by an
+ * occlusion test between the points offset 0.01 along their normals (as
+ * Scene::isOccluded).  Culled vision slots hold no vertex (the reference
+ * reads whatever a previous sample left there).  Random numbers: Philox
+ * with counter (pixel, sample, depth, 0) on the vision side, (pixel, sample,
+ * k, 2^30) on the light side. */
+typedef struct rt_bdpt_settings {
+    uint32_t vision_jumps;         /* visionJumpCount, 1..16                    */
+    uint32_t light_jumps;          /* lightJumpCount, 1..16                     */
+    uint32_t max_depth;            /* maxDepth, 1..32                           */
+    uint32_t spp;                  /* raysPerPixel                              */
+    uint64_t seed;
+    uint32_t sample_offset;
+    uint32_t flags;                /* reserved, 0                               */
+} rt_bdpt_settings;
+int  rt_render_bdpt(rt_ctx* ctx, const rt_camera* cam, const rt_bdpt_settings* s, float* rgb, uint8_t* rgb8,
+                    int ptr_kind);
+
 /* Gradient of a scalar image loss with respect to every material colour,
  * by path replay.  d_image: dL/d(rgb) (width*height*3), d_color: output
  * 3*num_materials (overwritten).  Same camera/settings/seed as the forward

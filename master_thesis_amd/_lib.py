@@ -111,6 +111,11 @@ class rt_settings(C.Structure):
                 ("sun_dir", C.c_float * 3), ("sun_cos", C.c_float), ("sun_radiance", C.c_float * 3)]
 
 
+class rt_bdpt_settings(C.Structure):
+    _fields_ = [("vision_jumps", C.c_uint32), ("light_jumps", C.c_uint32), ("max_depth", C.c_uint32),
+                ("spp", C.c_uint32), ("seed", C.c_uint64), ("sample_offset", C.c_uint32), ("flags", C.c_uint32)]
+
+
 class rt_timing(C.Structure):
     _fields_ = [("last_kernel_ms", C.c_float), ("last_call_ms", C.c_float), ("launches", C.c_uint32),
                 ("iterations", C.c_uint32), ("rays_traced", C.c_uint64)]
@@ -160,6 +165,7 @@ SIGNATURES = {
     "rt_occluded": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_int]),
     "rt_render": (C.c_int, [_P, C.POINTER(rt_camera), C.POINTER(rt_settings), _P, _P, C.c_int]),
     "rt_render_views": (C.c_int, [_P, C.POINTER(rt_camera), C.c_uint32, C.POINTER(rt_settings), _P, _P, C.c_int]),
+    "rt_render_bdpt": (C.c_int, [_P, C.POINTER(rt_camera), C.POINTER(rt_bdpt_settings), _P, _P, C.c_int]),
     "rt_render_adjoint": (C.c_int, [_P, C.POINTER(rt_camera), C.POINTER(rt_settings), _P, _P, C.c_int]),
     "rt_nrc_create": (C.c_int, [_P, C.POINTER(rt_nrc_config), C.POINTER(_P)]),
     "rt_nrc_destroy": (None, [_P]),

@@ -144,12 +144,14 @@ void rt_destroy(rt_ctx* c) {
     // caches outliving their context keep only their host part
     std::vector<rt_nrc*> caches = c->caches;
     for (rt_nrc* n : caches) nrc_detach(n);
-    DevBuf* all[] = {&c->pos, &c->idx, &c->mat, &c->shade_geom, &c->shade_nrm, &c->nodes, &c->tris,
+    DevBuf* all[] = {&c->pos, &c->idx, &c->mat, &c->shade_geom, &c->shade_nrm, &c->vnrm, &c->nodes, &c->tris,
                      &c->keys[0], &c->keys[1], &c->vals[0], &c->vals[1], &c->hist, &c->inner, &c->leafbox,
                      &c->innerbox, &c->boxtab, &c->queue[0], &c->queue[1], &c->counters, &c->rays, &c->hits,
                      &c->spill, &c->work, &c->stats, &c->pt_state, &c->pt_queue[0], &c->pt_queue[1],
                      &c->pt_image, &c->pt_img8, &c->pt_dimg, &c->pt_dcolor, &c->pt_ctr, &c->pt_partial,
-                     &c->pt_cams, &c->pt_nrc_query, &c->pt_nrc_train, &c->pt_shadow, &c->lights};
+                     &c->pt_cams, &c->pt_nrc_query, &c->pt_nrc_train, &c->pt_shadow, &c->lights,
+                     &c->light_objs, &c->light_obj_tris, &c->bd_vert, &c->bd_state, &c->bd_queue[0],
+                     &c->bd_queue[1], &c->bd_seg, &c->bd_occ, &c->bd_val};
     for (DevBuf* b : all) release(c, *b);
     for (int k = 0; k < rt_ctx::NUM_EV; ++k)
         if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
@@ -221,6 +223,22 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* s) {
     }
     for (LightTri& l : lights) l.cdf = l.cdf / area_sum;
     if (!lights.empty()) lights.back().cdf = 1.0f;
+    // light objects of the bidirectional tracer: every emissive material with
+    // its triangles in triangle order (the reference's lightSources list, one
+    // GraphicsObject per material, graphic/graphics_engine.cpp:20-30)
+    std::vector<uint32_t> lobj_tris;
+    std::vector<uint32_t> lobj;  // pairs {first, count}
+    {
+        std::vector<std::vector<uint32_t>> per(M);
+        for (uint64_t t = 0; t < T; ++t)
+            if (s->mat[s->tri_mat[t]].light_source) per[s->tri_mat[t]].push_back((uint32_t)t);
+        for (uint64_t m = 0; m < M; ++m)
+            if (s->mat[m].light_source && !per[m].empty()) {
+                lobj.push_back((uint32_t)lobj_tris.size());
+                lobj.push_back((uint32_t)per[m].size());
+                lobj_tris.insert(lobj_tris.end(), per[m].begin(), per[m].end());
+            }
+    }
     DeviceGuard g(c->device);
     int rc;
     if (!lights.empty()) {
@@ -228,23 +246,29 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* s) {
         RT_HIP(c, hipMemcpyAsync(c->lights.ptr, lights.data(), sizeof(LightTri) * lights.size(), hipMemcpyHostToDevice,
                                  c->stream));
     }
+    if (!lobj.empty()) {
+        if ((rc = ensure(c, c->light_objs, 4 * lobj.size())) || (rc = ensure(c, c->light_obj_tris, 4 * lobj_tris.size())))
+            return rc;
+        RT_HIP(c, hipMemcpyAsync(c->light_objs.ptr, lobj.data(), 4 * lobj.size(), hipMemcpyHostToDevice, c->stream));
+        RT_HIP(c, hipMemcpyAsync(c->light_obj_tris.ptr, lobj_tris.data(), 4 * lobj_tris.size(), hipMemcpyHostToDevice,
+                                 c->stream));
+    }
     if ((rc = ensure(c, c->pos, 12 * V)) || (rc = ensure(c, c->idx, 12 * T)) ||
         (rc = ensure(c, c->mat, sizeof(rt_material) * M)) || (rc = ensure(c, c->shade_geom, sizeof(ShadeGeom) * T)) ||
         (rc = ensure(c, c->shade_nrm, sizeof(ShadeNrm) * T)))
         return rc;
-    DevBuf nrm, tmat;
-    if ((rc = ensure(c, nrm, 12 * V)) || (rc = ensure(c, tmat, 4 * T))) return rc;
+    DevBuf tmat;
+    if ((rc = ensure(c, c->vnrm, 12 * V)) || (rc = ensure(c, tmat, 4 * T))) return rc;
     RT_HIP(c, hipMemcpyAsync(c->pos.ptr, s->pos, 12 * V, hipMemcpyHostToDevice, c->stream));
     RT_HIP(c, hipMemcpyAsync(c->idx.ptr, s->idx, 12 * T, hipMemcpyHostToDevice, c->stream));
     RT_HIP(c, hipMemcpyAsync(c->mat.ptr, s->mat, sizeof(rt_material) * M, hipMemcpyHostToDevice, c->stream));
-    RT_HIP(c, hipMemcpyAsync(nrm.ptr, s->nrm, 12 * V, hipMemcpyHostToDevice, c->stream));
+    RT_HIP(c, hipMemcpyAsync(c->vnrm.ptr, s->nrm, 12 * V, hipMemcpyHostToDevice, c->stream));
     RT_HIP(c, hipMemcpyAsync(tmat.ptr, s->tri_mat, 4 * T, hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_shade_records, dim3((uint32_t)((T + 255) / 256)), dim3(256), 0, c->stream, (uint32_t)T,
-                       (const float*)c->pos.ptr, (const float*)nrm.ptr, (const uint32_t*)c->idx.ptr,
+                       (const float*)c->pos.ptr, (const float*)c->vnrm.ptr, (const uint32_t*)c->idx.ptr,
                        (const uint32_t*)tmat.ptr, (ShadeGeom*)c->shade_geom.ptr, (ShadeNrm*)c->shade_nrm.ptr);
     RT_HIP(c, hipGetLastError());
     RT_HIP(c, hipStreamSynchronize(c->stream));
-    release(c, nrm);
     release(c, tmat);
     c->V = (uint32_t)V;
     c->T = (uint32_t)T;
@@ -252,6 +276,7 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* s) {
     for (int k = 0; k < 3; ++k) { c->scene_lo[k] = lo[k]; c->scene_hi[k] = hi[k]; }
     c->num_lights = (uint32_t)lights.size();
     c->light_area = area_sum;
+    c->num_light_objs = (uint32_t)(lobj.size() / 2);
     c->built = false;
     return RT_OK;
 }
@@ -480,6 +505,37 @@ int rt_render_nrc_records(rt_ctx* c, float* feats, float* targets, uint32_t* cou
         RT_HIP(c, hipMemcpyAsync(targets, base + 9 * (size_t)c->nrc_tcap, 12 * (size_t)c->nrc_records,
                                  hipMemcpyDeviceToHost, c->stream));
     RT_HIP(c, hipStreamSynchronize(c->stream));
+    return RT_OK;
+}
+
+int rt_render_bdpt(rt_ctx* c, const rt_camera* cam, const rt_bdpt_settings* s, float* rgb, uint8_t* rgb8,
+                   int ptr_kind) {
+    if (!c) return RT_E_INVALID;
+    if (!cam || !s) return fail(c, RT_E_INVALID, "rt_render_bdpt: NULL camera or settings");
+    if (!c->built) return fail(c, RT_E_STATE, "rt_render_bdpt: call rt_build first");
+    if (cam->width == 0 || cam->height == 0) return fail(c, RT_E_INVALID, "rt_render_bdpt: empty image");
+    if ((uint64_t)cam->width * cam->height > (1ull << 31)) return fail(c, RT_E_UNSUPPORTED, "rt_render_bdpt: image above 2^31 pixels");
+    if (s->spp == 0 || s->spp > (1u << 20)) return fail(c, RT_E_INVALID, "rt_render_bdpt: spp must be 1..2^20");
+    if (s->vision_jumps == 0 || s->vision_jumps > 16 || s->light_jumps == 0 || s->light_jumps > 16)
+        return fail(c, RT_E_INVALID, "rt_render_bdpt: vision_jumps and light_jumps must be 1..16");
+    if (s->max_depth == 0 || s->max_depth > 32) return fail(c, RT_E_INVALID, "rt_render_bdpt: max_depth must be 1..32");
+    if (ptr_kind != RT_PTR_HOST && ptr_kind != RT_PTR_DEVICE) return fail(c, RT_E_INVALID, "rt_render_bdpt: bad ptr_kind");
+    DeviceGuard g(c->device);
+    const size_t P = (size_t)cam->width * cam->height;
+    float* d_rgb = rgb;
+    uint8_t* d_rgb8 = rgb8;
+    int rc;
+    if (ptr_kind == RT_PTR_HOST) {
+        if ((rc = ensure(c, c->pt_image, 12 * P)) || (rc = ensure(c, c->pt_img8, 3 * P))) return rc;
+        d_rgb = (float*)c->pt_image.ptr;
+        d_rgb8 = (uint8_t*)c->pt_img8.ptr;
+    }
+    if ((rc = launch_bdpt(c, cam, s, d_rgb, d_rgb8))) return rc;
+    if (ptr_kind == RT_PTR_HOST) {
+        if (rgb) RT_HIP(c, hipMemcpyAsync(rgb, d_rgb, 12 * P, hipMemcpyDeviceToHost, c->stream));
+        if (rgb8) RT_HIP(c, hipMemcpyAsync(rgb8, d_rgb8, 3 * P, hipMemcpyDeviceToHost, c->stream));
+        RT_HIP(c, hipStreamSynchronize(c->stream));
+    }
     return RT_OK;
 }
 

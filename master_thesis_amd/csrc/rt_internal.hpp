@@ -51,10 +51,15 @@ struct rt_ctx {
     // scene
     uint32_t V = 0, T = 0, M = 0;
     float scene_lo[3] = {0, 0, 0}, scene_hi[3] = {0, 0, 0};
-    rt::DevBuf pos, idx, mat, shade_geom, shade_nrm;
+    rt::DevBuf pos, idx, mat, shade_geom, shade_nrm, vnrm;  // vnrm: per-vertex world normals
     rt::DevBuf lights;  // LightTri per emissive triangle of positive area
     uint32_t num_lights = 0;
     float light_area = 0.0f;
+    // light objects (one per emissive material, in material order) for the
+    // bidirectional tracer: uint2 {first, count} into a triangle-id list
+    // grouped by material in triangle order
+    rt::DevBuf light_objs, light_obj_tris;
+    uint32_t num_light_objs = 0;
     // BVH
     bool built = false;
     rt::DevBuf nodes, tris;
@@ -69,6 +74,7 @@ struct rt_ctx {
     rt::DevBuf pt_state, pt_queue[2], pt_image, pt_img8, pt_dimg, pt_dcolor, pt_ctr, pt_partial, pt_cams;
     rt::DevBuf pt_nrc_query, pt_nrc_train;  // cache queries (features, weights, pixel) and training records
     rt::DevBuf pt_shadow;                   // shadow rays of light sampling (rays, weights, indices, flags)
+    rt::DevBuf bd_vert, bd_state, bd_queue[2], bd_seg, bd_occ, bd_val;  // bidirectional tracer
     uint32_t nrc_records = 0, nrc_tcap = 0; // training records of the last cached render, record capacity
     std::vector<rt_nrc*> caches;            // live radiance caches; rt_destroy detaches them
     uint32_t render_iters = 0;
@@ -130,6 +136,9 @@ struct NrcFrame {
     rt_nrc_render opt;
     rt_nrc_stats* stats;
 };
+
+// bdpt.hip: one frame of the bidirectional tracer into device rgb / rgb8
+int launch_bdpt(rt_ctx* c, const rt_camera* cam, const rt_bdpt_settings* s, float* d_rgb, uint8_t* d_rgb8);
 
 // cams: V host cameras of one size; d_rgb / d_rgb8 / d_dimage view-major
 int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_settings* s, float* d_rgb,

@@ -1,4 +1,4 @@
-// engine.cpp — Renderer factory, HipPathTracer and the frame loop.
+// engine.cpp — Renderer factory, the GPU renderers and the frame loop.
 #include "engine.hpp"
 
 #include <algorithm>
@@ -29,8 +29,24 @@ bool HipPathTracer::renderFrame(const FrameRenderData& frd, float* rgb) const {
     return true;
 }
 
+void HipBidirectionalPathTracer::parseInput(const InputEntry& e) {
+    settings = rt_bdpt_settings{};
+    settings.vision_jumps = e.get_uint("visionJumpCount");
+    settings.light_jumps = e.get_uint("lightJumpCount");
+    settings.max_depth = e.get_uint("maxDepth");
+    settings.spp = e.get_uint("raysPerPixel");
+    settings.seed = e.has("seed") ? (uint64_t)std::stoull(e.get_string("seed")) : 1ull;
+}
+
+bool HipBidirectionalPathTracer::renderFrame(const FrameRenderData& frd, float* rgb) const {
+    int rc = rt_render_bdpt(frd.ctx, frd.camera, &settings, rgb, nullptr, RT_PTR_HOST);
+    if (rc != RT_OK) throw InitError("HipBidirectionalPathTracer", rt_last_error(frd.ctx));
+    return true;
+}
+
 std::unique_ptr<Renderer> make_renderer(const std::string& name) {
     if (name == "PathTracer") return std::make_unique<HipPathTracer>();
+    if (name == "BidirectionalPathTracer") return std::make_unique<HipBidirectionalPathTracer>();
     throw InitError("getRenderer not found", name);
 }
 

@@ -7,6 +7,10 @@
 //                   a renderer implements it (a GPU renders whole frames).
 //   HipPathTracer   PathTracer (graphic/path_tracer.h:9-32) on libmtrt:
 //                   same keys visionJumpCount / raysPerPixel, optional seed.
+//   HipBidirectionalPathTracer  BidirectionalPathTracer
+//                   (graphic/bidirectional_path_tracer.h:9-46): keys
+//                   visionJumpCount / lightJumpCount / maxDepth /
+//                   raysPerPixel, optional seed.
 //   make_renderer   getRenderer (main.cpp:22-26): by the .renderer entry name.
 //   GraphicsEngine  graphic/graphics_engine.h:17-63: objects -> scene upload
 //                   and BVH build (init), render(), saveImage (clamp, 8-bit
@@ -50,6 +54,13 @@ class HipPathTracer : public Renderer {
     void parseInput(const InputEntry& e) override;
     bool renderFrame(const FrameRenderData& frd, float* rgb) const override;
     rt_settings settings{};
+};
+
+class HipBidirectionalPathTracer : public Renderer {
+  public:
+    void parseInput(const InputEntry& e) override;
+    bool renderFrame(const FrameRenderData& frd, float* rgb) const override;
+    rt_bdpt_settings settings{};
 };
 
 // Throws InitError("getRenderer not found", name) for unknown names.

@@ -4,7 +4,8 @@
 //   sw_render renderer scene image_width image_height thread_count camera resultimage
 //
 // The renderer is chosen by the first entry name of the .renderer file
-// ("PathTracer" -> the GPU path tracer).  Meshes are read from
+// ("PathTracer" -> the GPU path tracer, "BidirectionalPathTracer" -> the GPU
+// bidirectional tracer).  Meshes are read from
 // $MTH_MESH_DIR, else from <scene dir>/../obj (the reference's res/ layout).
 #include <cstdio>
 #include <cstdlib>

@@ -155,6 +155,16 @@ class Tracer:
                                        L.RT_PTR_DEVICE if dev else L.RT_PTR_HOST))
         return rgb, rgb8
 
+    def render_bdpt(self, cam: L.rt_camera, spp: int, vision_jumps: int = 5, light_jumps: int = 5,
+                    max_depth: int = 5, seed: int = 1, sample_offset: int = 0, want_rgb8: bool = False):
+        """BidirectionalPathTracer frame; returns (rgb HxWx3, rgb8 or None)."""
+        s = L.rt_bdpt_settings(vision_jumps, light_jumps, max_depth, spp, seed, sample_offset, 0)
+        rgb = np.zeros((cam.height, cam.width, 3), np.float32)
+        rgb8 = np.zeros((cam.height, cam.width, 3), np.uint8) if want_rgb8 else None
+        self._check(self.lib.rt_render_bdpt(self.h, C.byref(cam), C.byref(s), L._ptr(rgb),
+                                            L._ptr(rgb8) if rgb8 is not None else None, L.RT_PTR_HOST))
+        return rgb, rgb8
+
     def render_views(self, cams, settings: L.rt_settings, rgb=None, want_rgb8: bool = False):
         """Renders len(cams) views in one launch sequence; returns
         (rgb (V, H, W, 3), rgb8 or None).  rgb may be a CUDA tensor."""

@@ -253,3 +253,20 @@ def render_cached(scene, cams, settings, cache_rgb, q_vertex: int = 1, train_eve
         ok = rec[:, 12] == 1.0
         return rgb, int(counts[0]), int(counts[1]), rec[ok, :9], rec[ok, 9:12], np.nonzero(ok)[0]
     return rgb, int(counts[0]), int(counts[1])
+
+
+def render_bdpt(scene, cam, settings, vision_jumps: int = 5, light_jumps: int = 5, max_depth: int = 5,
+                threads: int = 8) -> np.ndarray:
+    """The bidirectional path tracer (rt_render_bdpt's checker); rgb (H, W, 3)."""
+    lib = load()
+    P = C.c_void_p
+    lib.or_render_bdpt.argtypes = [P, P, P, P, P, C.c_uint32, C.c_uint32, C.c_uint32, P, P, C.c_uint32, C.c_uint32,
+                                   C.c_uint32, P, C.c_int]
+    lib.or_render_bdpt.restype = C.c_int
+    arrs, ps = _scene_ptrs(scene)
+    rgb = np.zeros((cam.height, cam.width, 3), np.float32)
+    rc = lib.or_render_bdpt(*ps, arrs[0].shape[0], arrs[2].shape[0], arrs[4].shape[0], C.addressof(cam),
+                            C.addressof(settings), vision_jumps, light_jumps, max_depth, _p(rgb), threads)
+    assert rc == 0, rc
+    return rgb
+

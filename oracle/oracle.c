@@ -1322,3 +1322,226 @@ int or_render_cached(const float* pos, const float* nrm, const uint32_t* idx, co
     }
     return rc;
 }
+
+/* ------------------------------------------------------------------ */
+/* BidirectionalPathTracer (graphic/bidirectional_path_tracer.cpp:20-168) */
+
+#define OR_LIGHT_SIDE 0x40000000u
+#define OR_BD_MAX 16
+
+typedef struct {
+    v3 p, n, c;
+    int valid;
+} bd_vert;
+
+typedef struct {
+    const pt_job* j;
+    uint32_t vj, lj, maxd;
+    uint32_t num_lobj;
+    const uint32_t* lobj;      /* {first, count} per light object */
+    const uint32_t* lobj_tris;
+} bd_job;
+
+/* surface at the closest hit: hit point, normalised interpolated normal */
+static void bd_surface(const pt_job* j, v3 o, v3 d, const best_t* h, v3* p, v3* n) {
+    *p = vadd(o, vscl(h->t, d));
+    uint32_t t = h->id;
+    v3 v0 = vld(j->pos + 3 * j->idx[3 * t]);
+    v3 e0 = vsub(vld(j->pos + 3 * j->idx[3 * t + 1]), v0);
+    v3 e1 = vsub(vld(j->pos + 3 * j->idx[3 * t + 2]), v0);
+    float b0, b1, b2;
+    bary(v0, e0, e1, *p, &b0, &b1, &b2);
+    v3 nn = V3(0.0f, 0.0f, 0.0f);
+    nn = vadd(nn, vscl(b0, vld(j->nrm + 3 * j->idx[3 * t])));
+    nn = vadd(nn, vscl(b1, vld(j->nrm + 3 * j->idx[3 * t + 1])));
+    nn = vadd(nn, vscl(b2, vld(j->nrm + 3 * j->idx[3 * t + 2])));
+    *n = vnormalize(nn);
+}
+
+/* Scene::isOccluded (scene.cpp:58-83) on the oracle's BVH: the closest hit's
+ * squared distance against |end - start|^2 */
+static int bd_occluded(const pt_job* j, v3 st, v3 en) {
+    v3 d = vsub(en, st);
+    float dist2 = vdot(d, d);
+    d = vnormalize(d);
+    rt_ray r;
+    r.ox = st.x; r.oy = st.y; r.oz = st.z; r.tmin = 0.0f;
+    r.dx = d.x; r.dy = d.y; r.dz = d.z; r.tmax = INFINITY;
+    uint64_t a1 = 0, a2 = 0;
+    best_t h = trace_bvh_one(j->bvh, &r, &a1, &a2);
+    return h.id != RT_MISS && h.d2 <= dist2;
+}
+
+static v3 bdpt_sample(const bd_job* b, uint32_t pix, uint32_t gsmp, v3 dir0) {
+    const pt_job* j = b->j;
+    const rt_settings* s = j->s;
+    const rt_camera* cam = &j->cams[0];
+    uint32_t key[2] = {(uint32_t)s->seed, (uint32_t)(s->seed >> 32)};
+    bd_vert V[OR_BD_MAX], L[OR_BD_MAX];
+    memset(V, 0, sizeof(V));
+    memset(L, 0, sizeof(L));
+    /* vision path (traceSinglePath, isLightRay = false) */
+    v3 o = V3(cam->origin[0], cam->origin[1], cam->origin[2]), d = dir0, T = V3(1.0f, 1.0f, 1.0f);
+    int cull = 1, lasthit = 0;
+    uint32_t vdepth = 0, ldepth = 0;
+    for (uint32_t i = 0; i < b->vj; ++i) {
+        rt_ray r;
+        r.ox = o.x; r.oy = o.y; r.oz = o.z; r.tmin = 0.0f;
+        r.dx = d.x; r.dy = d.y; r.dz = d.z; r.tmax = INFINITY;
+        uint64_t a1 = 0, a2 = 0;
+        best_t h = trace_bvh_one(j->bvh, &r, &a1, &a2);
+        if (h.id == RT_MISS) break;
+        v3 p, n;
+        bd_surface(j, o, d, &h, &p, &n);
+        if (cull && vdot(n, d) > 0.0f) {
+            o = vadd(p, vscl(0.01f, d));
+            continue;
+        }
+        cull = 0;
+        vdepth = i + 1;
+        const rt_material* m = &j->mat[j->tri_mat[h.id]];
+        uint32_t ctr[4] = {pix, gsmp, i, 0u}, rnd[4];
+        or_philox(ctr, key, rnd);
+        float xi0 = u01(rnd[0]);
+        v3 prev = d;
+        int diffuse = xi0 <= m->diffuse_threshold;
+        if (diffuse) d = hemisphere(u01(rnd[1]), u01(rnd[2]), n);
+        else if (xi0 <= m->reflect_threshold) d = reflect_ref(d, n);
+        else if (xi0 <= m->transparent_threshold) d = custom_refract_ref(d, n, m->refraction_index);
+        o = vadd(p, vscl(0.01f, d));
+        if (diffuse) T = vmul(T, vscl(vdot(n, d), V3(m->color[0], m->color[1], m->color[2])));
+        V[i].p = p; V[i].n = n; V[i].c = T; V[i].valid = 1;
+        if (m->light_source && vdot(n, prev) <= 0.0f) {
+            lasthit = 1;
+            break;
+        }
+    }
+    if (vdepth == 0) return V3(0.0f, 0.0f, 0.0f);
+    if (lasthit) return V[vdepth - 1].c;
+    /* light point (getRandomLightSourcePoint) and light path (isLightRay = true) */
+    if (b->num_lobj > 0) {
+        uint32_t ctr[4] = {pix, gsmp, 0u, OR_LIGHT_SIDE}, rnd[4];
+        or_philox(ctr, key, rnd);
+        uint32_t lo = (uint32_t)(u01(rnd[0]) * (float)b->num_lobj);
+        if (lo > b->num_lobj - 1u) lo = b->num_lobj - 1u;
+        const float sq = sqrtf(u01(rnd[1])), r2 = u01(rnd[2]);
+        const float b0 = 1.0f - sq, b1 = sq * (1.0f - r2), b2 = sq * r2;
+        const uint32_t first = b->lobj[2 * lo], count = b->lobj[2 * lo + 1];
+        uint32_t ti = (uint32_t)(u01(rnd[3]) * (float)count);
+        if (ti > count - 1u) ti = count - 1u;
+        const uint32_t t = b->lobj_tris[first + ti];
+        const uint32_t i0 = j->idx[3 * t], i1 = j->idx[3 * t + 1], i2 = j->idx[3 * t + 2];
+        v3 lp = vadd(vadd(vscl(b0, vld(j->pos + 3 * i0)), vscl(b1, vld(j->pos + 3 * i1))), vscl(b2, vld(j->pos + 3 * i2)));
+        v3 ln = V3(0.0f, 0.0f, 0.0f);
+        ln = vadd(ln, vscl(b0, vld(j->nrm + 3 * i0)));
+        ln = vadd(ln, vscl(b1, vld(j->nrm + 3 * i1)));
+        ln = vadd(ln, vscl(b2, vld(j->nrm + 3 * i2)));
+        ln = vnormalize(ln);
+        const rt_material* lm = &j->mat[j->tri_mat[t]];
+        L[0].p = lp; L[0].n = ln; L[0].c = V3(lm->color[0], lm->color[1], lm->color[2]); L[0].valid = 1;
+        ctr[2] = 1u;
+        or_philox(ctr, key, rnd);
+        o = lp;
+        d = hemisphere(u01(rnd[1]), u01(rnd[2]), ln);
+        T = V3(1.0f, 1.0f, 1.0f);
+        for (uint32_t i = 1; i < b->lj; ++i) {
+            rt_ray r;
+            r.ox = o.x; r.oy = o.y; r.oz = o.z; r.tmin = 0.0f;
+            r.dx = d.x; r.dy = d.y; r.dz = d.z; r.tmax = INFINITY;
+            uint64_t a1 = 0, a2 = 0;
+            best_t h = trace_bvh_one(j->bvh, &r, &a1, &a2);
+            if (h.id == RT_MISS) break;
+            v3 p, n;
+            bd_surface(j, o, d, &h, &p, &n);
+            ldepth = i + 1;
+            const rt_material* m = &j->mat[j->tri_mat[h.id]];
+            uint32_t c2[4] = {pix, gsmp, i + 1u, OR_LIGHT_SIDE}, rn[4];
+            or_philox(c2, key, rn);
+            float xi0 = u01(rn[0]);
+            v3 prev = d;
+            int diffuse = xi0 <= m->diffuse_threshold;
+            if (diffuse) d = hemisphere(u01(rn[1]), u01(rn[2]), n);
+            else if (xi0 <= m->reflect_threshold) d = reflect_ref(d, n);
+            else if (xi0 <= m->transparent_threshold) d = custom_refract_ref(d, n, m->refraction_index);
+            o = vadd(p, vscl(0.01f, d));
+            if (diffuse) T = vmul(T, vscl(vdot(n, prev), V3(m->color[0], m->color[1], m->color[2])));
+            L[i].p = p; L[i].n = n; L[i].c = T; L[i].valid = 1;
+            if (m->light_source && vdot(n, prev) <= 0.0f) break;
+        }
+    }
+    /* connections, in the reference's loop order */
+    v3 color = V3(0.0f, 0.0f, 0.0f);
+    int done = 0;
+    for (uint32_t vi = 0; vi < vdepth; ++vi)
+        for (uint32_t li = 0; li < ldepth; ++li) {
+            if (vi + li > b->maxd || !V[vi].valid) continue;
+            v3 st = vadd(V[vi].p, vscl(0.01f, V[vi].n));
+            v3 en = vadd(L[li].p, vscl(0.01f, L[li].n));
+            if (bd_occluded(j, st, en)) continue;
+            v3 dir = vnormalize(vsub(en, st));
+            color = vadd(color, vscl(vdot(dir, V[vi].n), vmul(V[vi].c, L[li].c)));
+            done = 1;
+        }
+    return done ? vscl(1.0f / (float)b->maxd, color) : V3(0.0f, 0.0f, 0.0f);
+}
+
+static void bd_range(void* arg, uint32_t lo, uint32_t hi, int tid) {
+    (void)tid;
+    const bd_job* b = (const bd_job*)arg;
+    const pt_job* j = b->j;
+    const rt_camera* cam = &j->cams[0];
+    for (uint32_t pix = lo; pix < hi; ++pix) {
+        v3 dir = primary(cam, pix % cam->width, pix / cam->width);
+        float sum[3] = {0.0f, 0.0f, 0.0f};
+        for (uint32_t smp = 0; smp < j->s->spp; ++smp) {
+            v3 c = bdpt_sample(b, pix, j->s->sample_offset + smp, dir);
+            sum[0] = sum[0] + c.x;
+            sum[1] = sum[1] + c.y;
+            sum[2] = sum[2] + c.z;
+        }
+        for (int ch = 0; ch < 3; ++ch) j->rgb[3 * (size_t)pix + ch] = (float)((double)sum[ch] * j->scale);
+    }
+}
+
+int or_render_bdpt(const float* pos, const float* nrm, const uint32_t* idx, const uint32_t* tri_mat, const void* mat,
+                   uint32_t V, uint32_t T, uint32_t M, const void* cam, const void* settings, uint32_t vision_jumps,
+                   uint32_t light_jumps, uint32_t max_depth, float* rgb, int threads) {
+    (void)V;
+    const rt_settings* s = (const rt_settings*)settings;
+    if (s->spp == 0 || vision_jumps == 0 || vision_jumps > OR_BD_MAX || light_jumps == 0 || light_jumps > OR_BD_MAX ||
+        max_depth == 0)
+        return -1;
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    pt_job j;
+    memset(&j, 0, sizeof(j));
+    j.pos = pos; j.nrm = nrm; j.idx = idx; j.tri_mat = tri_mat; j.mat = (const rt_material*)mat; j.M = M;
+    j.cams = (const rt_camera*)cam;
+    j.num_views = 1;
+    j.s = s;
+    j.bvh = or_bvh_build(pos, idx, T, 4);
+    j.scale = 2.0 * 3.14159265358979323846 * (double)(1.0f / (float)s->spp);
+    j.rgb = rgb;
+    /* light objects: every emissive material with its triangles, in order */
+    uint32_t* lobj = (uint32_t*)malloc(sizeof(uint32_t) * 2 * (M ? M : 1));
+    uint32_t* ltris = (uint32_t*)malloc(sizeof(uint32_t) * (T ? T : 1));
+    uint32_t nl = 0, nt = 0;
+    for (uint32_t m = 0; m < M; ++m) {
+        if (!j.mat[m].light_source) continue;
+        uint32_t first = nt;
+        for (uint32_t t = 0; t < T; ++t)
+            if (tri_mat[t] == m) ltris[nt++] = t;
+        if (nt > first) {
+            lobj[2 * nl] = first;
+            lobj[2 * nl + 1] = nt - first;
+            ++nl;
+        }
+    }
+    bd_job b = {&j, vision_jumps, light_jumps, max_depth, nl, lobj, ltris};
+    run_parallel(threads, j.cams[0].width * j.cams[0].height, 4, bd_range, &b);
+    free(lobj);
+    free(ltris);
+    or_bvh_free(j.bvh);
+    return 0;
+}
+

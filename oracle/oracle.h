@@ -115,6 +115,15 @@ int or_render_cached(const float* pos, const float* nrm, const uint32_t* idx, co
                      const void* settings, uint32_t q_vertex, uint32_t train_every, int query_on, int nee,
                      const float cache_rgb[3], float* rgb, float* records, uint64_t counts[2], int threads);
 
+/* BidirectionalPathTracer (graphic/bidirectional_path_tracer.cpp:20-168)
+ * for one camera: settings supply spp, seed and sample_offset; Philox
+ * counters (pixel, sample, depth, 0) on the vision side and (pixel, sample,
+ * k, 2^30) on the light side (k = 0 light point, 1 its direction, i + 1 light
+ * vertex i).  Culled vision slots hold no vertex. */
+int or_render_bdpt(const float* pos, const float* nrm, const uint32_t* idx, const uint32_t* tri_mat, const void* mat,
+                   uint32_t V, uint32_t T, uint32_t M, const void* cam, const void* settings, uint32_t vision_jumps,
+                   uint32_t light_jumps, uint32_t max_depth, float* rgb, int threads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,6 +31,8 @@ def load() -> C.CDLL:
         lib.ref_scene_free.argtypes = [P]
         lib.ref_scene_trace.argtypes = [P, P, C.c_uint32, P, P, P, P]
         lib.ref_scene_occluded.argtypes = [P, P, C.c_uint32, P]
+        lib.ref_render_bdpt.argtypes = [P, P, P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                        C.c_uint32, C.c_int, P]
         lib.ref_camera.argtypes = [P, C.c_float, C.c_float, C.c_uint32, C.c_uint32, P, P, P]
         lib.ref_load_scene.argtypes = [C.c_char_p, C.c_char_p, P, P, P, P, P, P]
         lib.ref_load_scene.restype = C.c_int
@@ -85,6 +87,15 @@ class RefScene:
         rgb = np.zeros((cam.height, cam.width, 3), np.float32)
         load().ref_render(self.h, vi.ctypes.data, pi.ctypes.data, og.ctypes.data, cam.width, cam.height, spp,
                           depth, threads, rgb.ctypes.data)
+        return rgb
+
+    def render_bdpt(self, cam, spp: int, vision: int, light: int, max_depth: int, threads: int = 8) -> np.ndarray:
+        vi = np.array(cam.view_inv[:], np.float32)
+        pi = np.array(cam.proj_inv[:], np.float32)
+        og = np.array(cam.origin[:], np.float32)
+        rgb = np.zeros((cam.height, cam.width, 3), np.float32)
+        load().ref_render_bdpt(self.h, vi.ctypes.data, pi.ctypes.data, og.ctypes.data, cam.width, cam.height, spp,
+                               vision, light, max_depth, threads, rgb.ctypes.data)
         return rgb
 
     def __del__(self):

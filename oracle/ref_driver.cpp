@@ -13,6 +13,7 @@
 #include "camera.h"
 #include "graphic/graphics_object.h"
 #include "graphic/mesh.h"
+#include "graphic/bidirectional_path_tracer.h"
 #include "graphic/path_tracer.h"
 #include "graphic/scene.h"
 #include "graphic/triangle.h"
@@ -214,6 +215,45 @@ void ref_render(void* h, const float* view_inv, const float* proj_inv, const flo
             if (i >= w * hgt) break;
             local.pixel = Vector2u({i % w, i / w});
             Vector3f c = pt.renderPixel(local);
+            for (int k = 0; k < 3; ++k) rgb[3 * i + k] = c[k];
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < threads; ++t) th.emplace_back(work);
+    work();
+    for (auto& t : th) t.join();
+}
+
+// BidirectionalPathTracer::renderPixel over the image (graphic/
+// bidirectional_path_tracer.cpp:13-18 keys), `threads` workers.
+void ref_render_bdpt(void* h, const float* view_inv, const float* proj_inv, const float* origin, uint32_t w,
+                     uint32_t hgt, uint32_t spp, uint32_t vision, uint32_t light, uint32_t maxd, int threads,
+                     float* rgb) {
+    auto* s = (RefScene*)h;
+    BidirectionalPathTracer bt;
+    bt.parseInput(entry("BidirectionalPathTracer", {{"visionJumpCount", {std::to_string(vision)}},
+                                                    {"lightJumpCount", {std::to_string(light)}},
+                                                    {"maxDepth", {std::to_string(maxd)}},
+                                                    {"raysPerPixel", {std::to_string(spp)}}}));
+    Renderer::PixelRenderData prd;
+    prd.scene = &s->scene;
+    prd.objects = &s->objects;
+    prd.lightSources = &s->lights;
+    prd.imageSize = Vector2u({w, hgt});
+    prd.origin = v3(origin);
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            prd.viewInverse[i][j] = view_inv[4 * i + j];
+            prd.projInverse[i][j] = proj_inv[4 * i + j];
+        }
+    std::atomic<uint32_t> next(0);
+    auto work = [&]() {
+        Renderer::PixelRenderData local = prd;
+        for (;;) {
+            uint32_t i = next.fetch_add(1);
+            if (i >= w * hgt) break;
+            local.pixel = Vector2u({i % w, i / w});
+            Vector3f c = bt.renderPixel(local);
             for (int k = 0; k < 3; ++k) rgb[3 * i + k] = c[k];
         }
     };

@@ -140,6 +140,18 @@ def test_sw_render_cli_matches_library(tracer, tmp_path):
     _, rgb8 = tracer.render(cam, s, want_rgb8=True)
     assert np.array_equal(img, rgb8)
     assert img.mean() > 5
+    # the reference's BidirectionalPathTracer renderer file drives the GPU bidirectional tracer
+    (tmp_path / "bd.renderer").write_text("BidirectionalPathTracer\n\tvisionJumpCount(5)\n\tlightJumpCount(4)\n"
+                                          "\tmaxDepth(5)\n\traysPerPixel(8)\n\tseed(7)\n")
+    out2 = tmp_path / "bd.ppm"
+    p = subprocess.run([str(B.CLI), str(tmp_path / "bd.renderer"), str(tmp_path / "scene" / "box.scene"), "64", "48",
+                        "4", str(tmp_path / "c.camera"), str(out2)], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    img2 = np.frombuffer(out2.read_bytes()[len(head):], np.uint8).reshape(48, 64, 3)
+    _setup(tracer, sc)
+    _, bd8 = tracer.render_bdpt(cam, 8, 5, 4, 5, seed=7, want_rgb8=True)
+    assert np.array_equal(img2, bd8)
+    assert img2.mean() > 5
 
 
 def test_multi_view_with_sun_matches_oracle(tracer):
@@ -160,3 +172,33 @@ def test_multi_view_with_sun_matches_oracle(tracer):
     one, _ = tracer.render(cams[0], s)
     assert np.array_equal(one, rgb[0])
     assert not np.array_equal(rgb[0], rgb[1])
+
+
+# ---- the bidirectional tracer
+
+@pytest.mark.parametrize("vj,lj,md", [(5, 5, 5), (3, 4, 2), (1, 2, 1)])
+def test_bdpt_matches_oracle(tracer, vj, lj, md):
+    """rt_render_bdpt against the oracle's restatement on the same Philox
+    counters: the same paths, light points and occlusion answers, so the
+    images agree to float summation order (the GPU sums a sample's pairs in
+    the same order; only the pixel scale's rounding path is shared)."""
+    sc = scenes.cornell_box()
+    tracer.upload(sc)
+    tracer.build(2)
+    cam = H.cornell_camera(40, 32)
+    s = make_settings(spp=8, max_depth=5, seed=17, sample_offset=3)
+    got, _ = tracer.render_bdpt(cam, 8, vj, lj, md, seed=17, sample_offset=3)
+    want = O.render_bdpt(sc, cam, s, vj, lj, md)
+    np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-6)
+    assert (got != 0).mean() > 0.5
+
+
+def test_bdpt_equals_path_tracer_when_paths_end_on_lights(tracer):
+    E, rho = 2.0, 0.6
+    sc = scenes.closed_room(scenes.material((rho, rho, rho)), scenes.material((E, E, E), light=True))
+    tracer.upload(sc)
+    tracer.build(2)
+    cam = H.camera((0.0, 0.0, 0.0), 0.0, -1.0, math.pi / 4, 24, 24)
+    pt, _ = tracer.render(cam, make_settings(spp=16, max_depth=5, seed=12, flags=RT_RENDER_DETERMINISTIC))
+    bd, _ = tracer.render_bdpt(cam, 16, 5, 5, 5, seed=12)
+    np.testing.assert_array_equal(bd, pt)

@@ -140,7 +140,7 @@ def test_cli_argument_count():
 def test_cli_unknown_renderer(tmp_path):
     sp, od = _files(tmp_path, "tri.obj\n\tposition(0, 0, 0)\n", {"tri.obj": TRI_OBJ})
     r = tmp_path / "x.renderer"
-    r.write_text("BidirectionalPathTracer\n\tvisionJumpCount(5)\n\tlightJumpCount(5)\n\tmaxDepth(5)\n\traysPerPixel(1)\n")
+    r.write_text("PhotonMapper\n\tvisionJumpCount(5)\n\traysPerPixel(1)\n")  # a Vulkan-only renderer
     c = tmp_path / "c.camera"
     c.write_text("Camera\n\tposition(4, 0, 0)\n\tangle(1, 0)\n")
     p = subprocess.run([str(B.CLI), str(r), str(sp), "8", "8", "1", str(c), str(tmp_path / "o.ppm")],

@@ -237,3 +237,18 @@ def test_rgb8_is_clamped_truncation():
     rgb, rgb8 = O.render(sc, _room_camera(8, 8), make_settings(spp=2, max_depth=3, seed=2))
     expect = (np.clip(rgb, 0, 1) * np.float32(255.0)).astype(np.int32).astype(np.uint8)
     assert np.array_equal(rgb8, expect)
+
+
+def test_bidirectional_equals_path_tracer_when_paths_end_on_lights():
+    """Floor under emissive walls: every vision path ends on a light at its
+    second vertex, so the bidirectional tracer adds exactly the path
+    tracer's contribution (same Philox counters on the vision side): the
+    images are identical, and their mean is pi * rho * E / 2."""
+    E, rho = 2.0, 0.6
+    sc = scenes.closed_room(scenes.material((rho, rho, rho)), scenes.material((E, E, E), light=True))
+    cam = _room_camera(24, 24)
+    s = make_settings(spp=16, max_depth=5, seed=12)
+    pt, _ = O.render(sc, cam, s)
+    bd = O.render_bdpt(sc, cam, s, vision_jumps=5, light_jumps=5, max_depth=5)
+    np.testing.assert_array_equal(bd, pt)
+

@@ -130,3 +130,25 @@ def test_occluded_matches_reference():
     segs = np.concatenate([np.concatenate([a, b], axis=1),
                            np.concatenate([rays["o"][hit], p[hit]], axis=1)]).astype(np.float32)
     np.testing.assert_array_equal(O.occluded_brute(sc, segs), ref.occluded(segs))
+
+
+def test_bidirectional_tracer_statistically_matches_reference():
+    """The reference's own BidirectionalPathTracer (its mt19937 stream) and
+    the oracle's restatement (Philox) estimate the same image: block means
+    agree within the spread the oracle shows across seeds."""
+    sc = scenes.cornell_box()
+    cam = H.camera(scenes.cornell_camera_position(), 1.0, 0.0, np.float32(math.pi / 4), 32, 32)
+    spp = 64
+    ref = R.RefScene(sc).render_bdpt(cam, spp, 5, 5, 5, threads=8)
+    runs = np.stack([O.render_bdpt(sc, cam, make_settings(spp=spp, max_depth=5, seed=s)) for s in range(1, 7)])
+
+    def blocks(img):
+        return img.reshape(4, 8, 4, 8, 3).mean(axis=(1, 3))
+
+    rb = blocks(ref)
+    ob = np.stack([blocks(x) for x in runs])
+    m, s = ob.mean(axis=0), ob.std(axis=0, ddof=1)
+    z = (rb - m) / np.maximum(s * math.sqrt(1 + 1 / len(runs)), 1e-6)
+    assert np.all(np.abs(z) < 6), np.abs(z).max()
+    zg = (ref.mean() - runs.mean()) / (runs.mean(axis=(1, 2, 3)).std(ddof=1) * math.sqrt(1 + 1 / len(runs)))
+    assert abs(zg) < 5, zg
