@@ -277,7 +277,9 @@ def cpu_baseline(sc, rays, gpu_hits, target_s: float):
     log(f"[bench] CPU oracle: {sample.shape[0]} rays in {dt:.2f}s on {threads} threads (BVH build {build_s:.1f}s); "
         f"hits identical to GPU: {same}")
     return {"value": round(sample.shape[0] / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"every {stride}th ray of the same {rays.shape[0]} ({sample.shape[0]} rays, both halves), oracle binned-SAH "
+            "sample": (f"all {rays.shape[0]} rays" if stride == 1 else
+                       f"every {stride}th ray of the same {rays.shape[0]} ({sample.shape[0]} rays, both halves)") +
+                      f", oracle binned-SAH "
                       f"BVH built in {build_s:.1f}s; hits identical to GPU: {same}"}
 
 

@@ -7,12 +7,14 @@ silently run on another code path.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
 
 PKG = Path(__file__).resolve().parent
-LIB_PATH = PKG / "libmtrt.so"
+# MTRT_LIB: another build of the library (for comparing kernel variants)
+LIB_PATH = Path(os.environ["MTRT_LIB"]) if os.environ.get("MTRT_LIB") else PKG / "libmtrt.so"
 
 RT_OK = 0
 RT_E_INVALID = -1

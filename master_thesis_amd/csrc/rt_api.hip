@@ -118,6 +118,11 @@ int rt_create(rt_ctx** out, int device) {
         long v = strtol(r, nullptr, 10);
         c->trace_refill_min = (uint32_t)(v < 1 ? 1 : (v > 64 ? 64 : v));
     }
+    if (const char* r = getenv("MTRT_TRACE_SLICE")) {  // tuning knob, rounded to 128..2^24
+        long v = strtol(r, nullptr, 10);
+        v = v < 128 ? 128 : (v > (1l << 24) ? (1l << 24) : v);
+        c->trace_slice = (uint32_t)(v & ~127l);
+    }
     // a blocking stream: work here is ordered after earlier work on the null
     // stream (where e.g. torch's default stream writes the rays)
     e = hipStreamCreateWithFlags(&c->own_stream, hipStreamDefault);

@@ -49,16 +49,26 @@ struct TraceArgs {
     const uint32_t* n_dev;  // when set, the ray count is read from device memory
     const rt_bvh4_node* nodes;
     const rt_tri_rec* tris;
-    uint32_t* work;        // [0] = next ray batch, [1] = stack overflow flag
+    uint32_t* work;        // [1] = stack overflow flag; [32 (1 + x)] = next ray of XCD x's slices
     uint32_t* spill;       // [spill_cap][grid threads]
     uint32_t spill_cap;
     uint32_t spill_stride;
     uint32_t max_steps;    // bound on steps per ray (each node and triangle is visited at most once)
     uint32_t refill_min;   // idle lanes that trigger a refill (1..64)
+    uint32_t slice;        // rays per XCD slice (a multiple of POOL_CHUNK)
     unsigned long long* stats;  // rays, nodes, tris, hits
 };
 
-constexpr uint32_t POOL_CHUNK = 128;  // rays a wave takes from the global counter at a time
+constexpr uint32_t POOL_CHUNK = 128;  // rays a wave takes from a counter at a time
+// The ray range is cut into slices of TraceArgs::slice rays, dealt round-robin
+// to the 8 XCDs (slice k to XCD k mod 8), each with its own counter.  A
+// persistent grid's workgroup b runs on XCD b mod 8 (workgroups are dispatched
+// to the XCDs in turn), so the rays in flight on one XCD, and the nodes its
+// own L2 holds for them, come from a few consecutive slices instead of
+// interleaving with the other seven.  A wave whose XCD has no rays left takes
+// them from the next XCD's counter.
+constexpr uint32_t XCDS = 8;
+constexpr uint32_t XCD_COUNTER_STRIDE = 32;  // counters 128 B apart
 
 // Per-lane traversal state of one ray.
 struct Lane {
@@ -214,6 +224,8 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceArgs a) {
     const uint32_t n = a.n_dev ? *a.n_dev : a.n;
     uint32_t pool_next = 0, pool_end = 0;  // wave-uniform
     bool exhausted = false;                // wave-uniform
+    uint32_t part = blockIdx.x % XCDS;     // wave-uniform: the XCD whose slices this wave takes
+    uint32_t drained = 0;                  // wave-uniform: XCD counters seen past the end
     bool have = false, retired = false;
     uint32_t ri = 0;
     Lane L;
@@ -224,14 +236,24 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceArgs a) {
         if (need && ((uint32_t)__popcll(need) >= a.refill_min || __ballot(have) == 0ull)) {
             if (pool_next >= pool_end && !exhausted) {
                 const int leader = __ffsll((unsigned long long)need) - 1;
-                uint32_t g = 0;
-                if ((int)lane == leader) g = atomicAdd(a.work, POOL_CHUNK);
-                g = __shfl(g, leader);
-                if (g >= n) {
-                    exhausted = true;
-                } else {
-                    pool_next = g;
-                    pool_end = min(g + POOL_CHUNK, n);
+                while (true) {
+                    uint32_t v = 0;
+                    if ((int)lane == leader) v = atomicAdd(a.work + XCD_COUNTER_STRIDE * (1u + part), POOL_CHUNK);
+                    v = __shfl(v, leader);
+                    // XCD part's v-th ray: ray v mod slice of its (v / slice)-th slice;
+                    // increasing in v, so the first v past n ends this counter
+                    const uint64_t g = (uint64_t)(v / a.slice) * (XCDS * (uint64_t)a.slice) +
+                                       (uint64_t)part * a.slice + v % a.slice;
+                    if (g < n) {
+                        pool_next = (uint32_t)g;
+                        pool_end = min(pool_next + POOL_CHUNK, n);
+                        break;
+                    }
+                    if (++drained == XCDS) {
+                        exhausted = true;
+                        break;
+                    }
+                    part = (part + 1u) % XCDS;
                 }
             }
             if (pool_next < pool_end) {
@@ -320,9 +342,10 @@ int launch(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, uint8_t*
     const uint32_t spill_cap = max_stack > (uint32_t)STACK_LDS ? max_stack - STACK_LDS : 1u;
     int rc;
     if ((rc = ensure(c, c->spill, 4ull * spill_cap * threads))) return rc;
-    if ((rc = ensure(c, c->work, 64))) return rc;
+    const size_t work_bytes = 4ull * XCD_COUNTER_STRIDE * (1 + XCDS);
+    if ((rc = ensure(c, c->work, work_bytes))) return rc;
     if ((rc = ensure(c, c->stats, 64))) return rc;
-    RT_HIP(c, hipMemsetAsync(c->work.ptr, 0, 8, c->stream));
+    RT_HIP(c, hipMemsetAsync(c->work.ptr, 0, work_bytes, c->stream));
     if (stats) RT_HIP(c, hipMemsetAsync(c->stats.ptr, 0, 32, c->stream));
     TraceArgs a;
     a.rays = d_rays;
@@ -339,6 +362,7 @@ int launch(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, uint8_t*
     a.spill_stride = threads;
     a.max_steps = 2u * c->num_nodes + 2u * c->T + 16u;
     a.refill_min = c->trace_refill_min;
+    a.slice = c->trace_slice;
     a.stats = (unsigned long long*)c->stats.ptr;
     RT_HIP(c, hipEventRecord(c->ev[2], c->stream));
     if (any) hipLaunchKernelGGL((k_trace<false, true>), dim3(grid), dim3(TRACE_BLOCK), 0, c->stream, a);
