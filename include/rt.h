@@ -350,6 +350,84 @@ int  rt_render_nrc(rt_ctx* ctx, rt_nrc* nrc, const rt_camera* cams, uint32_t num
  * the in-out capacity (both NULL: query the count). */
 int  rt_render_nrc_records(rt_ctx* ctx, float* feats, float* targets, uint32_t* count);
 
+/* ---- irradiance probes (DDGI) --------------------------------------------
+ * The Vulkan renderer's Majercik2019 pipeline (src/graphic/renderer/
+ * majercik2019_renderer.cpp:35-47; res/shader/majercik2019_probe_raygen.rgen,
+ * majercik2019_shading_update_comp.comp, majercik2019_final_raygen.rgen)
+ * restated as a cache beside rt_nrc, in the units of rt_render_nrc:
+ *  - update: every probe of a regular grid (the scene file's Probes entry,
+ *    mesh_manager.cpp:36-40) casts rays_per_probe rays in spherical
+ *    Fibonacci directions (helper.glsl sphericalFibonacci), optionally under
+ *    a random rotation per update; a ray's radiance is what a path leaving a
+ *    diffuse vertex in that direction adds beyond the vertex's own light
+ *    sample: sky without the sun on a miss, 0 on a light or a back face, and
+ *    on a surface the mean of shadow_count light samples (plus, with
+ *    RT_DDGI_MULTI_BOUNCE, the field's own diffuse term there);
+ *  - atlases: per probe a (side + 2)^2 tile of octahedral texels with a
+ *    one-texel wrapped border; irradiance = sum max(0, n_t . dir) radiance /
+ *    sum of weights, depth moments (r, r^2) weighted by that cosine to the
+ *    power depth_sharpness, r = min(hit distance, max_ray_distance)
+ *    (majercik2019_shading_update_comp.comp:67-93); hysteresis blends the
+ *    previous atlases in;
+ *  - lookup (getIrradiance, majercik2019_final_raygen.rgen:44-91): the 8
+ *    probes of the cell around the point, trilinear x wrap-shading weight x
+ *    Chebyshev visibility from the depth moments, square-root blending
+ *    unless RT_DDGI_LINEAR_BLENDING, times energy_preservation.
+ * rt_render_ddgi runs rt_render_nrc's path loop with the field in place of
+ * the network: at the query vertex (0 = the first surface hit) of a purely
+ * diffuse material the sample takes a light sample and adds throughput *
+ * albedo / 2 * lookup, the diffuse bounce's expectation in the path
+ * tracer's units (it has no training records). */
+typedef struct rt_ddgi rt_ddgi;
+typedef struct rt_ddgi_config {
+    uint32_t probe_count[3];       /* probeCount, each >= 1                     */
+    float probe_start[3];          /* probeStartCorner                          */
+    float probe_spacing[3];        /* betweenProbeDistance, each > 0            */
+    uint32_t rays_per_probe;       /* perProbeRayCount, 1..1024                 */
+    uint32_t side;                 /* probeSampleSideLength, 2..32              */
+    float max_ray_distance;        /* maxProbeRayDistance, > 0                  */
+    float depth_sharpness;         /* depthSharpness                            */
+    float normal_bias;             /* normalBias                                */
+    float energy_preservation;     /* energyPreservation; 0 selects 1           */
+    float hysteresis;              /* previous field's weight in an update, 0..1 */
+    uint32_t shadow_count;         /* shadowCountProbe, 1..64; 0 selects 1      */
+    uint32_t flags;                /* RT_DDGI_*                                 */
+} rt_ddgi_config;
+#define RT_DDGI_LINEAR_BLENDING 0x1      /* linearBlending(1)                   */
+#define RT_DDGI_MULTI_BOUNCE 0x2         /* hits add the field's diffuse term   */
+#define RT_DDGI_ROTATE 0x4               /* random ray-set rotation per update  */
+/* Chebyshev weight where the reference's final pass applies it (distance <=
+ * mean, majercik2019_final_raygen.rgen:75); default: distance > mean, as in
+ * Majercik et al. 2019 */
+#define RT_DDGI_REFERENCE_VISIBILITY 0x8
+#define RT_DDGI_SURFEL_FLOATS 8          /* direction, radiance, distance, hit */
+
+int  rt_ddgi_create(rt_ctx* ctx, const rt_ddgi_config* cfg, rt_ddgi** out);
+/* rt_destroy of the context frees a live field's device memory; its handle
+ * then only accepts rt_ddgi_destroy (other calls return RT_E_STATE). */
+void rt_ddgi_destroy(rt_ddgi* g);
+/* atlas size in texels: width (side + 2) * count_x * count_y, height
+ * (side + 2) * count_z; irradiance 4 floats (RGB, 0), depth 2 per texel */
+int  rt_ddgi_atlas_size(const rt_ddgi* g, uint32_t* width, uint32_t* height);
+/* One update with the scene of the field's context (built): probe rays,
+ * light samples at their hits, atlases.  s supplies seed, sky and sun. */
+int  rt_ddgi_update(rt_ddgi* g, const rt_settings* s);
+/* Atlas update from caller surfels (probes * rays * 8 floats, probe-major). */
+int  rt_ddgi_update_surfels(rt_ddgi* g, const float* surfels, int ptr_kind);
+/* The last update's surfels and ray-set rotation (row-major 3x3, applied as
+ * rot * direction); either may be NULL. */
+int  rt_ddgi_surfels(rt_ddgi* g, float* surfels, float* rotation, int ptr_kind);
+int  rt_ddgi_get_atlas(rt_ddgi* g, float* irradiance, float* depth, int ptr_kind);
+/* Sets both atlases; later updates blend into them. */
+int  rt_ddgi_set_atlas(rt_ddgi* g, const float* irradiance, const float* depth, int ptr_kind);
+/* Lookups for n samples (9 floats each: position, normal, incoming ray
+ * direction) into out (n * 3), energy_preservation included. */
+int  rt_ddgi_eval(rt_ddgi* g, const float* samples, uint32_t n, float* out, int ptr_kind);
+/* Frames of rt_render_views with the field (which must belong to ctx) at
+ * query_vertex (0..63); stats may be NULL (records and steps stay 0). */
+int  rt_render_ddgi(rt_ctx* ctx, rt_ddgi* g, const rt_camera* cams, uint32_t num_views, const rt_settings* s,
+                    uint32_t query_vertex, float* rgb, uint8_t* rgb8, int ptr_kind, rt_nrc_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -150,6 +150,19 @@ class rt_nrc_stats(C.Structure):
     _fields_ = [("queries", C.c_uint64), ("records", C.c_uint64), ("steps", C.c_uint32), ("loss", C.c_float),
                 ("render_ms", C.c_float), ("train_ms", C.c_float), ("shadow_rays", C.c_uint64)]
 
+class rt_ddgi_config(C.Structure):
+    _fields_ = [("probe_count", C.c_uint32 * 3), ("probe_start", C.c_float * 3), ("probe_spacing", C.c_float * 3),
+                ("rays_per_probe", C.c_uint32), ("side", C.c_uint32), ("max_ray_distance", C.c_float),
+                ("depth_sharpness", C.c_float), ("normal_bias", C.c_float), ("energy_preservation", C.c_float),
+                ("hysteresis", C.c_float), ("shadow_count", C.c_uint32), ("flags", C.c_uint32)]
+
+
+RT_DDGI_LINEAR_BLENDING = 0x1
+RT_DDGI_MULTI_BOUNCE = 0x2
+RT_DDGI_ROTATE = 0x4
+RT_DDGI_REFERENCE_VISIBILITY = 0x8
+RT_DDGI_SURFEL_FLOATS = 8
+
 # every exported symbol of include/rt.h with its signature
 _P = C.c_void_p
 SIGNATURES = {
@@ -178,6 +191,17 @@ SIGNATURES = {
     "rt_render_nrc_records": (C.c_int, [_P, _P, _P, C.POINTER(C.c_uint32)]),
     "rt_render_nrc": (C.c_int, [_P, _P, C.POINTER(rt_camera), C.c_uint32, C.POINTER(rt_settings),
                                 C.POINTER(rt_nrc_render), _P, _P, C.c_int, C.POINTER(rt_nrc_stats)]),
+    "rt_ddgi_create": (C.c_int, [_P, C.POINTER(rt_ddgi_config), C.POINTER(_P)]),
+    "rt_ddgi_destroy": (None, [_P]),
+    "rt_ddgi_atlas_size": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "rt_ddgi_update": (C.c_int, [_P, C.POINTER(rt_settings)]),
+    "rt_ddgi_update_surfels": (C.c_int, [_P, _P, C.c_int]),
+    "rt_ddgi_surfels": (C.c_int, [_P, _P, _P, C.c_int]),
+    "rt_ddgi_get_atlas": (C.c_int, [_P, _P, _P, C.c_int]),
+    "rt_ddgi_set_atlas": (C.c_int, [_P, _P, _P, C.c_int]),
+    "rt_ddgi_eval": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_int]),
+    "rt_render_ddgi": (C.c_int, [_P, _P, C.POINTER(rt_camera), C.c_uint32, C.POINTER(rt_settings), C.c_uint32,
+                                 _P, _P, C.c_int, C.POINTER(rt_nrc_stats)]),
 }
 
 _lib = None

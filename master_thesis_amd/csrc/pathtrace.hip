@@ -46,6 +46,7 @@
 // depend on the colours (uniform hemisphere sampling, colour-independent
 // thresholds), which makes this gradient exact for the rendered estimate.
 #include <math.h>
+#include <string.h>
 
 #include "rt_internal.hpp"
 #include "rt_core.hpp"
@@ -129,6 +130,9 @@ struct PTArgs {
     float* sh_w;        // 6 * S: pixel weight, record weight
     uint32_t* sh_idx;   // 2 * S: partial-sum index, record index
     uint8_t* sh_occ;    // S
+    // probe field in place of the network (rt_render_ddgi): queries only at
+    // purely diffuse vertices, which also take their light sample
+    uint32_t ddgi_mode;
 };
 
 // Training-sample choice: lowbias32 (C. Wellons) over pixel, sample, view, seed.
@@ -559,7 +563,8 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade_nrc(PTArgs a, int cur) {
                         Tq = mulv(Tq, RL);
                         contrib = true;
                     }
-                } else if (!light_end && fh == a.nrc_q && depth + 1 < a.max_depth) {
+                } else if (!light_end && fh == a.nrc_q && depth + 1 < a.max_depth &&
+                           (!a.ddgi_mode || m.diffuse_threshold >= 1.0f)) {
                     const bool train = a.nrc_every != 0 && train_hash(pix, gsmp, view, a.seed_lo) % a.nrc_every == 0;
                     f9[0] = p.x; f9[1] = p.y; f9[2] = p.z;
                     f9[3] = nrm.x; f9[4] = nrm.y; f9[5] = nrm.z;
@@ -570,26 +575,29 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade_nrc(PTArgs a, int cur) {
                         Tq = mk(1.0f, 1.0f, 1.0f);
                     } else if (a.nrc_query) {
                         push_q = true;
-                        qw = T;
+                        // probe field: the diffuse bounce's expectation, albedo / 2 * lookup
+                        qw = a.ddgi_mode ? mulv(T, scale(0.5f, mk(m.color[0], m.color[1], m.color[2]))) : T;
                         end = true;
                         stop = true;
                     }
                 }
                 ++fh;
-                if (!stop) {
-                    u4 c4;
-                    c4.x = pix; c4.y = gsmp; c4.z = depth; c4.w = view;
-                    if (a.nee && !light_end && depth + 1 < a.max_depth) {
-                        u4 c5 = c4;
-                        c5.w = view | 0x80000000u;
-                        f3 Cn;
-                        if (nee_sample(a, p, nrm, m, philox4x32_10(c5, a.seed_lo, a.seed_hi), sh, Cn)) {
-                            push_sh = true;
-                            shw_pix = mulv(T, Cn);
-                            sh_to_rec = rec;
-                            shw_rec = mulv(Tq, Cn);
-                        }
+                u4 c4;
+                c4.x = pix; c4.y = gsmp; c4.z = depth; c4.w = view;
+                // a probe-field query holds the indirect part only: its vertex
+                // takes the light sample too
+                if (a.nee && !light_end && depth + 1 < a.max_depth && (!stop || (push_q && a.ddgi_mode))) {
+                    u4 c5 = c4;
+                    c5.w = view | 0x80000000u;
+                    f3 Cn;
+                    if (nee_sample(a, p, nrm, m, philox4x32_10(c5, a.seed_lo, a.seed_hi), sh, Cn)) {
+                        push_sh = true;
+                        shw_pix = mulv(T, Cn);
+                        sh_to_rec = rec;
+                        shw_rec = mulv(Tq, Cn);
                     }
+                }
+                if (!stop) {
                     const u4 rnd = philox4x32_10(c4, a.seed_lo, a.seed_hi);
                     const float xi0 = u01(rnd.x);
                     const bool diffuse = xi0 <= m.diffuse_threshold;
@@ -758,6 +766,68 @@ __global__ void k_zero(float* p, uint32_t n) {
     if (i < n) p[i] = 0.0f;
 }
 
+// Radiance of the probe rays of a field update (rt_ddgi_update): what a path
+// leaving a purely diffuse vertex along the ray adds beyond that vertex's own
+// light sample.  A miss sees the sky without the sun; a light seen from the
+// front, or any back face (the probe is inside geometry), adds 0; a surface
+// adds the mean of `count` light samples (queued here, tested by the any-hit
+// kernel, summed by k_ddgi_resolve) and, with multi-bounce, td * albedo / 2 *
+// the previous field's lookup there.  The distance runs from the probe to the
+// hit moved normal_bias along its normal (majercik2019_probe_raygen.rgen:40-47).
+// Light sample k of ray j draws Philox (j, update, k, 0x50000000).
+__global__ void __launch_bounds__(PT_BLOCK) k_ddgi_shade(PTArgs a, DdgiShade b) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= b.n) return;
+    const rt_ray r = b.rays[j];
+    const rt_hit h = b.hits[j];
+    const f3 o = mk(r.ox, r.oy, r.oz), d = mk(r.dx, r.dy, r.dz);
+    f3 rad = mk(0.0f, 0.0f, 0.0f);
+    float dist = b.grid.max_dist, hit = 0.0f;
+    bool sampled = false;
+    if (h.tri == RT_MISS) {
+        rad = miss_radiance(a, d, false);
+    } else {
+        f3 p, nrm;
+        uint32_t mid;
+        surface(a, h, o, d, p, nrm, mid);
+        const f3 dv = sub(o, add(p, scale(b.grid.normal_bias, nrm)));
+        dist = sqrtf(dot(dv, dv));
+        hit = 1.0f;
+        const rt_material m = a.mat[mid];
+        if (!(dot(nrm, d) > 0.0f) && !m.light_source) {
+            sampled = true;
+            for (uint32_t k = 0; k < b.count; ++k) {
+                const size_t q = (size_t)j * b.count + k;
+                u4 c;
+                c.x = j; c.y = b.update; c.z = k; c.w = 0x50000000u;
+                rt_ray sr;
+                f3 C = mk(0.0f, 0.0f, 0.0f);
+                if (!(a.nee && nee_sample(a, p, nrm, m, philox4x32_10(c, a.seed_lo, a.seed_hi), sr, C))) {
+                    sr = make_ray(p, mk(0.0f, 0.0f, 0.0f));  // no direction: never occluded, weight 0
+                    C = mk(0.0f, 0.0f, 0.0f);
+                }
+                b.sh_ray[q] = sr;
+                b.sh_w[3 * q] = C.x; b.sh_w[3 * q + 1] = C.y; b.sh_w[3 * q + 2] = C.z;
+            }
+            if (b.multi) {
+                const f3 e = ddgi_lookup(b.grid, b.irr, b.dep, p, nrm, d);
+                rad = mulv(scale(0.5f * m.diffuse_threshold, mk(m.color[0], m.color[1], m.color[2])), e);
+            }
+        }
+    }
+    if (!sampled) {
+        for (uint32_t k = 0; k < b.count; ++k) {
+            const size_t q = (size_t)j * b.count + k;
+            b.sh_ray[q] = make_ray(o, mk(0.0f, 0.0f, 0.0f));
+            b.sh_w[3 * q] = 0.0f; b.sh_w[3 * q + 1] = 0.0f; b.sh_w[3 * q + 2] = 0.0f;
+        }
+    }
+    float* sf = b.surf + 8 * (size_t)j;
+    sf[3] = rad.x; sf[4] = rad.y; sf[5] = rad.z;
+    sf[6] = dist;
+    sf[7] = hit;
+}
+
 inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 }  // namespace
@@ -852,6 +922,7 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
     a.nrc_q = nrc ? nf->opt.query_vertex : 0;
     a.nrc_every = nrc && !(nf->opt.flags & RT_NRC_NO_RECORDS) ? nf->opt.train_every : 0;
     a.nrc_query = nrc && !(nf->opt.flags & RT_NRC_NO_QUERY);
+    a.ddgi_mode = nrc && nf->ddgi != nullptr;
     a.q_feat = nrc ? (float*)c->pt_nrc_query.ptr : nullptr;
     a.q_w = nrc ? a.q_feat + 9 * q_cap : nullptr;
     a.q_idx = nrc ? (uint32_t*)(a.q_w + 3 * q_cap) : nullptr;
@@ -917,7 +988,9 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
         if (nrc && hc[C_QDROP] != 0) return fail(c, RT_E_DEVICE, "rt_render_nrc: cache query buffer overflow");
         if (nrc && hc[C_QN] > 0) {
             // the queued samples' cache predictions, added to their pixels
-            if ((rc = nrc_infer_scatter(nf->cache, a.q_feat, a.q_w, a.q_idx, hc[C_QN], a.partial))) return rc;
+            if (nf->ddgi) rc = ddgi_query_scatter(nf->ddgi, a.q_feat, a.q_w, a.q_idx, hc[C_QN], a.partial);
+            else rc = nrc_infer_scatter(nf->cache, a.q_feat, a.q_w, a.q_idx, hc[C_QN], a.partial);
+            if (rc) return rc;
             RT_HIP(c, hipMemsetAsync(&a.ctr[C_QN], 0, 4, stream));
             nf->stats->queries += hc[C_QN];
             ++launches;
@@ -939,7 +1012,7 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
         nf->stats->shadow_rays = (uint64_t)hc[C_SHRAYS] | ((uint64_t)hc[C_SHRAYS + 1] << 32);
         c->nrc_records = tn < a.t_cap ? tn : a.t_cap;
         c->nrc_tcap = a.t_cap;
-        if (!(nf->opt.flags & RT_NRC_NO_TRAIN)) {
+        if (!(nf->opt.flags & RT_NRC_NO_TRAIN) && nf->cache) {
             const uint32_t avail = tn < a.t_cap ? tn : a.t_cap;
             if ((rc = nrc_train_records(nf->cache, a.t_feat, a.t_tgt, avail, nf->opt.batch, nf->opt.max_steps,
                                         &nf->stats->steps, &nf->stats->loss)))
@@ -963,6 +1036,32 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
     c->timing.rays_traced = rays;
     c->last_op = 2;
     c->render_iters = (uint32_t)iter;
+    return RT_OK;
+}
+
+int launch_ddgi_shade(rt_ctx* c, const rt_settings* s, const DdgiShade& b) {
+    if (b.n == 0) return RT_OK;
+    PTArgs a;
+    memset(&a, 0, sizeof(a));
+    a.geom = (const ShadeGeom*)c->shade_geom.ptr;
+    a.nrm = (const ShadeNrm*)c->shade_nrm.ptr;
+    a.mat = (const rt_material*)c->mat.ptr;
+    a.flags = s->flags;
+    a.seed_lo = (uint32_t)s->seed;
+    a.seed_hi = (uint32_t)(s->seed >> 32);
+    for (int k = 0; k < 3; ++k) {
+        a.sky[k] = s->sky[k];
+        a.sun_dir[k] = s->sun_dir[k];
+        a.sun_rad[k] = s->sun_radiance[k];
+    }
+    a.sun_cos = s->sun_cos;
+    a.sun_on = (s->flags & RT_RENDER_SUN) && s->sun_cos < 1.0f;
+    a.nee = c->num_lights > 0 || a.sun_on;
+    a.lights = (const LightTri*)c->lights.ptr;
+    a.num_lights = c->num_lights;
+    a.light_area = c->light_area;
+    hipLaunchKernelGGL(k_ddgi_shade, dim3(div_up(b.n, PT_BLOCK)), dim3(PT_BLOCK), 0, c->stream, a, b);
+    RT_HIP(c, hipGetLastError());
     return RT_OK;
 }
 

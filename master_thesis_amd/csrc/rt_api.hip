@@ -149,6 +149,8 @@ void rt_destroy(rt_ctx* c) {
     // caches outliving their context keep only their host part
     std::vector<rt_nrc*> caches = c->caches;
     for (rt_nrc* n : caches) nrc_detach(n);
+    std::vector<rt_ddgi*> fields = c->probe_fields;
+    for (rt_ddgi* g : fields) ddgi_detach(g);
     DevBuf* all[] = {&c->pos, &c->idx, &c->mat, &c->shade_geom, &c->shade_nrm, &c->vnrm, &c->nodes, &c->tris,
                      &c->keys[0], &c->keys[1], &c->vals[0], &c->vals[1], &c->hist, &c->inner, &c->leafbox,
                      &c->innerbox, &c->boxtab, &c->queue[0], &c->queue[1], &c->counters, &c->rays, &c->hits,
@@ -477,6 +479,48 @@ int rt_render_nrc(rt_ctx* c, rt_nrc* nrc, const rt_camera* cams, uint32_t num_vi
     f.stats = stats ? stats : &local;
     *f.stats = rt_nrc_stats{};
     DeviceGuard g(c->device);
+    const size_t P = (size_t)cams[0].width * cams[0].height * num_views;
+    float* d_rgb = rgb;
+    uint8_t* d_rgb8 = rgb8;
+    int rc;
+    if (ptr_kind == RT_PTR_HOST) {
+        if ((rc = ensure(c, c->pt_image, 12 * P)) || (rc = ensure(c, c->pt_img8, 3 * P))) return rc;
+        d_rgb = (float*)c->pt_image.ptr;
+        d_rgb8 = (uint8_t*)c->pt_img8.ptr;
+    }
+    if ((rc = launch_render(c, cams, num_views, s, d_rgb, d_rgb8, nullptr, nullptr, &f))) return rc;
+    if (ptr_kind == RT_PTR_HOST) {
+        if (rgb) RT_HIP(c, hipMemcpyAsync(rgb, d_rgb, 12 * P, hipMemcpyDeviceToHost, c->stream));
+        if (rgb8) RT_HIP(c, hipMemcpyAsync(rgb8, d_rgb8, 3 * P, hipMemcpyDeviceToHost, c->stream));
+        RT_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    return RT_OK;
+}
+
+int rt_render_ddgi(rt_ctx* c, rt_ddgi* g, const rt_camera* cams, uint32_t num_views, const rt_settings* s,
+                   uint32_t query_vertex, float* rgb, uint8_t* rgb8, int ptr_kind, rt_nrc_stats* stats) {
+    if (!c) return RT_E_INVALID;
+    if (!g || !cams || !s) return fail(c, RT_E_INVALID, "rt_render_ddgi: NULL field, camera or settings");
+    if (!ddgi_ctx(g)) return fail(c, RT_E_STATE, "rt_render_ddgi: the field's context was destroyed");
+    if (ddgi_ctx(g) != c) return fail(c, RT_E_INVALID, "rt_render_ddgi: the field belongs to another context");
+    if (num_views == 0) return fail(c, RT_E_INVALID, "rt_render_ddgi: no views");
+    if (!c->built) return fail(c, RT_E_STATE, "rt_render_ddgi: call rt_build first");
+    if (cams[0].width == 0 || cams[0].height == 0) return fail(c, RT_E_INVALID, "rt_render_ddgi: empty image");
+    for (uint32_t v = 1; v < num_views; ++v)
+        if (cams[v].width != cams[0].width || cams[v].height != cams[0].height)
+            return fail(c, RT_E_INVALID, "rt_render_ddgi: views differ in size");
+    if (s->spp == 0) return fail(c, RT_E_INVALID, "rt_render_ddgi: spp must be >= 1");
+    if (s->max_depth == 0 || s->max_depth > 64) return fail(c, RT_E_INVALID, "rt_render_ddgi: max_depth must be 1..64");
+    if (query_vertex > 63) return fail(c, RT_E_INVALID, "rt_render_ddgi: query_vertex must be 0..63");
+    if (ptr_kind != RT_PTR_HOST && ptr_kind != RT_PTR_DEVICE) return fail(c, RT_E_INVALID, "rt_render_ddgi: bad ptr_kind");
+    NrcFrame f;
+    f.ddgi = g;
+    f.opt.query_vertex = query_vertex;
+    f.opt.flags = RT_NRC_NO_RECORDS | RT_NRC_NO_TRAIN;
+    rt_nrc_stats local{};
+    f.stats = stats ? stats : &local;
+    *f.stats = rt_nrc_stats{};
+    DeviceGuard dg(c->device);
     const size_t P = (size_t)cams[0].width * cams[0].height * num_views;
     float* d_rgb = rgb;
     uint8_t* d_rgb8 = rgb8;

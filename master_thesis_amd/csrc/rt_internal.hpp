@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/rt.h"
+#include "ddgi_field.hpp"
 
 namespace rt {
 
@@ -78,6 +79,7 @@ struct rt_ctx {
     rt::DevBuf bd_vert, bd_state, bd_queue[2], bd_seg, bd_occ, bd_val;  // bidirectional tracer
     uint32_t nrc_records = 0, nrc_tcap = 0; // training records of the last cached render, record capacity
     std::vector<rt_nrc*> caches;            // live radiance caches; rt_destroy detaches them
+    std::vector<rt_ddgi*> probe_fields;     // live irradiance-probe fields; rt_destroy detaches them
     uint32_t render_iters = 0;
 
     // ev[0..1] bracket a whole call, ev[2..3] the last traversal kernel,
@@ -133,10 +135,35 @@ void nrc_detach(rt_nrc* s);
 // pathtrace.hip
 // Radiance-cache options of one frame (rt_render_nrc)
 struct NrcFrame {
-    rt_nrc* cache;
-    rt_nrc_render opt;
-    rt_nrc_stats* stats;
+    rt_nrc* cache = nullptr;
+    rt_nrc_render opt{};
+    rt_nrc_stats* stats = nullptr;
+    rt_ddgi* ddgi = nullptr;  // set: the probe field answers the queries (rt_render_ddgi)
 };
+// ddgi.hip: probe-field lookups of queued cache queries, added to their
+// pixels: acc[3 * idx[i] + ch] += w[3 * i + ch] * lookup_i[ch]
+int ddgi_query_scatter(rt_ddgi* g, const float* d_feats, const float* d_w, const uint32_t* d_idx, uint32_t n,
+                       float* d_acc);
+rt_ctx* ddgi_ctx(const rt_ddgi* g);
+// frees a field's device memory and cuts it from its context (rt_destroy)
+void ddgi_detach(rt_ddgi* g);
+// pathtrace.hip: radiance of a probe update's rays (surfel floats 3..7) and
+// their light samples (count per ray; unused slots get a zero direction)
+struct DdgiShade {
+    DdgiGrid grid;
+    const rt_ray* rays;
+    const rt_hit* hits;
+    float* surf;              // n * 8
+    const float* irr;         // the field before this update (multi-bounce term)
+    const float* dep;
+    uint32_t multi;           // add the field's diffuse term at surface hits
+    uint32_t update;          // update index: Philox counter word
+    uint32_t count;           // light samples per ray
+    rt_ray* sh_ray;           // n * count
+    float* sh_w;              // n * count * 3
+    uint32_t n;               // probes * rays
+};
+int launch_ddgi_shade(rt_ctx* c, const rt_settings* s, const DdgiShade& d);
 
 // bdpt.hip: one frame of the bidirectional tracer into device rgb / rgb8
 int launch_bdpt(rt_ctx* c, const rt_camera* cam, const rt_bdpt_settings* s, float* d_rgb, uint8_t* d_rgb8);

@@ -22,7 +22,7 @@ class Tracer:
         self.h = h
         self.scene: Scene | None = None
         self.info = None
-        self.caches = weakref.WeakSet()  # radiance caches on this context, closed first
+        self.caches = weakref.WeakSet()  # radiance caches and probe fields on this context, closed first
 
     # -- lifetime -----------------------------------------------------------
     def close(self) -> None:

@@ -270,3 +270,107 @@ def render_bdpt(scene, cam, settings, vision_jumps: int = 5, light_jumps: int = 
     assert rc == 0, rc
     return rgb
 
+
+
+# ---- irradiance probes (rt_ddgi_*): cfg is a master_thesis_amd._lib.rt_ddgi_config
+
+def ddgi_atlas_shape(cfg):
+    ts = cfg.side + 2
+    return cfg.probe_count[2] * ts, cfg.probe_count[0] * cfg.probe_count[1] * ts
+
+
+def ddgi_fib(i: int, n: int) -> np.ndarray:
+    lib = load()
+    lib.or_ddgi_fib.argtypes = [C.c_uint32, C.c_uint32, C.c_void_p]
+    out = np.zeros(3, np.float32)
+    lib.or_ddgi_fib(i, n, _p(out))
+    return out
+
+
+def ddgi_oct_encode(v) -> np.ndarray:
+    lib = load()
+    lib.or_ddgi_oct_encode.argtypes = [C.c_void_p, C.c_void_p]
+    v = np.ascontiguousarray(v, np.float32)
+    o = np.zeros(2, np.float32)
+    lib.or_ddgi_oct_encode(_p(v), _p(o))
+    return o
+
+
+def ddgi_oct_decode(o) -> np.ndarray:
+    lib = load()
+    lib.or_ddgi_oct_decode.argtypes = [C.c_void_p, C.c_void_p]
+    o = np.ascontiguousarray(o, np.float32)
+    v = np.zeros(3, np.float32)
+    lib.or_ddgi_oct_decode(_p(o), _p(v))
+    return v
+
+
+def ddgi_update(cfg, surfels: np.ndarray, irr=None, dep=None):
+    """Atlases (irr (H, W, 4), dep (H, W, 2)) after an update from surfels
+    (probes * rays, 8); irr / dep: the previous atlases (None: first update)."""
+    lib = load()
+    lib.or_ddgi_update.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+    H, W = ddgi_atlas_shape(cfg)
+    first = irr is None
+    irr = np.zeros((H, W, 4), np.float32) if first else np.array(irr, np.float32, copy=True)
+    dep = np.zeros((H, W, 2), np.float32) if first else np.array(dep, np.float32, copy=True)
+    s = np.ascontiguousarray(surfels, np.float32)
+    lib.or_ddgi_update(C.addressof(cfg), _p(s), int(first), _p(irr), _p(dep))
+    return irr, dep
+
+
+def ddgi_eval(cfg, irr, dep, samples) -> np.ndarray:
+    lib = load()
+    lib.or_ddgi_eval.argtypes = [C.c_void_p] * 4 + [C.c_uint32, C.c_void_p]
+    irr = np.ascontiguousarray(irr, np.float32)
+    dep = np.ascontiguousarray(dep, np.float32)
+    s = np.ascontiguousarray(samples, np.float32).reshape(-1, 9)
+    out = np.zeros((s.shape[0], 3), np.float32)
+    lib.or_ddgi_eval(C.addressof(cfg), _p(irr), _p(dep), _p(s), s.shape[0], _p(out))
+    return out
+
+
+def ddgi_surfels(scene, cfg, settings, rot, update: int = 0, irr=None, dep=None, multi: bool = False,
+                 threads: int = 8) -> np.ndarray:
+    """Surfels (probes * rays, 8) of one field update on the CPU."""
+    lib = load()
+    P = C.c_void_p
+    lib.or_ddgi_surfels.argtypes = [P, P, P, P, P, C.c_uint32, C.c_uint32, C.c_uint32, P, P, P, C.c_uint32, P, P,
+                                    C.c_int, P, C.c_int]
+    lib.or_ddgi_surfels.restype = C.c_int
+    arrs, ps = _scene_ptrs(scene)
+    n = cfg.probe_count[0] * cfg.probe_count[1] * cfg.probe_count[2] * cfg.rays_per_probe
+    out = np.zeros((n, 8), np.float32)
+    r = np.ascontiguousarray(rot, np.float32).reshape(9)
+    H, W = ddgi_atlas_shape(cfg)
+    irr = np.zeros((H, W, 4), np.float32) if irr is None else np.ascontiguousarray(irr, np.float32)
+    dep = np.zeros((H, W, 2), np.float32) if dep is None else np.ascontiguousarray(dep, np.float32)
+    rc = lib.or_ddgi_surfels(*ps, arrs[0].shape[0], arrs[2].shape[0], arrs[4].shape[0], C.addressof(cfg),
+                             C.addressof(settings), _p(r), update, _p(irr), _p(dep), int(multi), _p(out), threads)
+    assert rc == 0, rc
+    return out
+
+
+def render_ddgi(scene, cams, settings, cfg, irr, dep, q_vertex: int = 0, threads: int = 8):
+    """rt_render_ddgi's checker: (rgb (V, H, W, 3), queries)."""
+    lib = load()
+    P = C.c_void_p
+    lib.or_render_ddgi.argtypes = [P, P, P, P, P, C.c_uint32, C.c_uint32, C.c_uint32, P, C.c_uint32, P, C.c_uint32,
+                                   P, P, P, P, P, C.c_int]
+    lib.or_render_ddgi.restype = C.c_int
+    arrs, ps = _scene_ptrs(scene)
+    from master_thesis_amd._lib import rt_camera
+    if not isinstance(cams, (list, tuple)):
+        cams = [cams]
+    V = len(cams)
+    carr = (rt_camera * V)(*cams)
+    H, W = cams[0].height, cams[0].width
+    rgb = np.zeros((V, H, W, 3), np.float32)
+    q = np.zeros(1, np.uint64)
+    irr = np.ascontiguousarray(irr, np.float32)
+    dep = np.ascontiguousarray(dep, np.float32)
+    rc = lib.or_render_ddgi(*ps, arrs[0].shape[0], arrs[2].shape[0], arrs[4].shape[0], C.addressof(carr), V,
+                            C.addressof(settings), q_vertex, C.addressof(cfg), _p(irr), _p(dep), _p(rgb), _p(q),
+                            threads)
+    assert rc == 0, rc
+    return rgb, int(q[0])

@@ -858,6 +858,10 @@ typedef struct {
     /* radiance-cache mode (cache_rgb != NULL): a constant cache */
     const float* cache_rgb;
     uint32_t q_vertex, train_every, query_on, nee;
+    /* probe-field mode (rt_render_ddgi): the field answers the queries */
+    const rt_ddgi_config* ddgi;
+    const float* ddgi_irr;
+    const float* ddgi_dep;
     uint64_t* n_queries; /* [256 threads] */
     uint64_t* n_records; /* [256 threads] */
     float* rec;          /* per (vp, sample): 9 features, 3 targets, valid flag (13 floats), or NULL */
@@ -1027,6 +1031,8 @@ static int nee_sample(const pt_job* j, v3 p, v3 nrm, const rt_material* m, const
     return 1;
 }
 
+static v3 ddgi_lookup(const rt_ddgi_config* g, const float* irr, const float* dep, v3 x, v3 n, v3 vd);
+
 /* One sample of the cache mode (rt_render_nrc with a constant cache):
  * returns its summed pixel contribution; a training sample fills rec
  * (9 features, 3 targets, flag 1) when rec is not NULL. */
@@ -1090,7 +1096,8 @@ static v3 trace_path_cached(const pt_job* j, uint32_t view, uint32_t pix, uint32
             }
             break;
         }
-        if (!light_end && fh == j->q_vertex && depth + 1 < s->max_depth) {
+        if (!light_end && fh == j->q_vertex && depth + 1 < s->max_depth &&
+            (!j->ddgi || m->diffuse_threshold >= 1.0f)) {
             const int train = j->train_every != 0 && train_hash(pix, gsmp, view, key[0]) % j->train_every == 0;
             if (train) {
                 j->n_records[tid]++;
@@ -1102,8 +1109,26 @@ static v3 trace_path_cached(const pt_job* j, uint32_t view, uint32_t pix, uint32
                 feats[6] = d.x; feats[7] = d.y; feats[8] = d.z;
             } else if (j->query_on) {
                 j->n_queries[tid]++;
-                sum = vadd(sum, vmul(T, V3(fmaxf(j->cache_rgb[0], 0.0f), fmaxf(j->cache_rgb[1], 0.0f),
-                                            fmaxf(j->cache_rgb[2], 0.0f))));
+                if (j->ddgi) {
+                    /* probe field: the vertex's own light sample, then albedo / 2 * lookup */
+                    if (j->nee) {
+                        uint32_t c5[4] = {pix, gsmp, depth, view | 0x80000000u}, r5[4];
+                        or_philox(c5, key, r5);
+                        rt_ray sh;
+                        v3 C;
+                        if (nee_sample(j, p, n, m, r5, &sh, &C)) {
+                            uint64_t a1 = 0, a2 = 0;
+                            best_t oh = trace_bvh_one(j->bvh, &sh, &a1, &a2);
+                            if (oh.id == RT_MISS) sum = vadd(sum, vmul(T, C));
+                        }
+                    }
+                    const v3 e = ddgi_lookup(j->ddgi, j->ddgi_irr, j->ddgi_dep, p, n, d);
+                    const v3 qw = vmul(T, vscl(0.5f, V3(m->color[0], m->color[1], m->color[2])));
+                    sum = vadd(sum, vmul(qw, e));
+                } else {
+                    sum = vadd(sum, vmul(T, V3(fmaxf(j->cache_rgb[0], 0.0f), fmaxf(j->cache_rgb[1], 0.0f),
+                                                fmaxf(j->cache_rgb[2], 0.0f))));
+                }
                 break;
             }
         }
@@ -1210,7 +1235,32 @@ typedef struct {
     uint32_t q_vertex, train_every, query_on, nee;
     float* rec;
     uint64_t queries, records;
+    const rt_ddgi_config* ddgi;
+    const float* ddgi_irr;
+    const float* ddgi_dep;
 } cache_cfg;
+
+/* emissive triangles by cumulative area, float, in triangle order */
+static void setup_lights(pt_job* j, const float* pos, const uint32_t* idx, const uint32_t* tri_mat, uint32_t T) {
+    j->light_tri = (uint32_t*)malloc(sizeof(uint32_t) * (T ? T : 1));
+    j->light_cdf = (float*)malloc(sizeof(float) * (T ? T : 1));
+    j->num_lights = 0;
+    float acc = 0.0f;
+    for (uint32_t t = 0; t < T; ++t) {
+        if (!j->mat[tri_mat[t]].light_source) continue;
+        const v3 p0 = vld(pos + 3 * idx[3 * t]), p1 = vld(pos + 3 * idx[3 * t + 1]), p2 = vld(pos + 3 * idx[3 * t + 2]);
+        const v3 c = vcross(vsub(p1, p0), vsub(p2, p0));
+        const float area = 0.5f * sqrtf(vdot(c, c));
+        if (!(area > 0.0f)) continue;
+        acc = acc + area;
+        j->light_tri[j->num_lights] = t;
+        j->light_cdf[j->num_lights] = acc;
+        j->num_lights++;
+    }
+    for (uint32_t k = 0; k < j->num_lights; ++k) j->light_cdf[k] = j->light_cdf[k] / acc;
+    if (j->num_lights) j->light_cdf[j->num_lights - 1] = 1.0f;
+    j->light_area = acc;
+}
 
 static int pt_run(const float* pos, const float* nrm, const uint32_t* idx, const uint32_t* tri_mat, const void* mat,
                   uint32_t V, uint32_t T, uint32_t M, const void* cams, uint32_t num_views, const void* settings,
@@ -1243,24 +1293,10 @@ static int pt_run(const float* pos, const float* nrm, const uint32_t* idx, const
         j.query_on = cc->query_on;
         j.nee = cc->nee;
         j.rec = cc->rec;
-        /* emissive triangles by cumulative area, float, in triangle order */
-        j.light_tri = (uint32_t*)malloc(sizeof(uint32_t) * (T ? T : 1));
-        j.light_cdf = (float*)malloc(sizeof(float) * (T ? T : 1));
-        float acc = 0.0f;
-        for (uint32_t t = 0; t < T; ++t) {
-            if (!j.mat[tri_mat[t]].light_source) continue;
-            const v3 p0 = vld(pos + 3 * idx[3 * t]), p1 = vld(pos + 3 * idx[3 * t + 1]), p2 = vld(pos + 3 * idx[3 * t + 2]);
-            const v3 c = vcross(vsub(p1, p0), vsub(p2, p0));
-            const float area = 0.5f * sqrtf(vdot(c, c));
-            if (!(area > 0.0f)) continue;
-            acc = acc + area;
-            j.light_tri[j.num_lights] = t;
-            j.light_cdf[j.num_lights] = acc;
-            j.num_lights++;
-        }
-        for (uint32_t k = 0; k < j.num_lights; ++k) j.light_cdf[k] = j.light_cdf[k] / acc;
-        if (j.num_lights) j.light_cdf[j.num_lights - 1] = 1.0f;
-        j.light_area = acc;
+        j.ddgi = cc->ddgi;
+        j.ddgi_irr = cc->ddgi_irr;
+        j.ddgi_dep = cc->ddgi_dep;
+        setup_lights(&j, pos, idx, tri_mat, T);
     }
     run_parallel(threads, j.cams[0].width * j.cams[0].height * num_views, 16, pt_range, &j);
     if (cc) {
@@ -1320,6 +1356,386 @@ int or_render_cached(const float* pos, const float* nrm, const uint32_t* idx, co
         counts[0] = cc.queries;
         counts[1] = cc.records;
     }
+    return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* Irradiance probes: the Vulkan renderer's Majercik2019 pipeline
+ * (res/shader/majercik2019_probe_raygen.rgen, majercik2019_shading_update_comp.comp,
+ * majercik2019_final_raygen.rgen, helper.glsl) in the path tracer's units.
+ * Atlas: probe p owns the (side + 2)^2 tile at tile column p mod (cx cy),
+ * tile row p div (cx cy); texels 1..side hold the octahedral map, the border
+ * the texels across its folded edges. */
+
+/* helper.glsl:79-86 */
+static void oct_enc(v3 v, float* ox, float* oy) {
+    const float l1 = fabsf(v.x) + fabsf(v.y) + fabsf(v.z);
+    const float inv = 1.0f / l1;
+    float rx = v.x * inv, ry = v.y * inv;
+    if (v.z < 0.0f) {
+        const float nx = (1.0f - fabsf(ry)) * (rx >= 0.0f ? 1.0f : -1.0f);
+        const float ny = (1.0f - fabsf(rx)) * (ry >= 0.0f ? 1.0f : -1.0f);
+        rx = nx;
+        ry = ny;
+    }
+    *ox = rx;
+    *oy = ry;
+}
+
+/* helper.glsl:88-94 */
+static v3 oct_dec(float ox, float oy) {
+    v3 v = V3(ox, oy, 1.0f - fabsf(ox) - fabsf(oy));
+    if (v.z < 0.0f) {
+        const float nx = (1.0f - fabsf(oy)) * (ox >= 0.0f ? 1.0f : -1.0f);
+        const float ny = (1.0f - fabsf(ox)) * (oy >= 0.0f ? 1.0f : -1.0f);
+        v.x = nx;
+        v.y = ny;
+    }
+    return vnormalize(v);
+}
+
+void or_ddgi_oct_encode(const float v[3], float o[2]) { oct_enc(vld(v), &o[0], &o[1]); }
+void or_ddgi_oct_decode(const float o[2], float v[3]) {
+    const v3 r = oct_dec(o[0], o[1]);
+    v[0] = r.x; v[1] = r.y; v[2] = r.z;
+}
+
+/* helper.glsl:59-70 sphericalFibonacci */
+static v3 fib(uint32_t i, uint32_t n) {
+    const float fi = (float)i;
+    const float iphi = fi * 2.2360680f * 0.5f + 0.5f - 1.0f;
+    float fr = iphi - floorf(iphi);
+    if (!(fr < 1.0f)) fr = 0.0f;
+    float sn, cs;
+    or_sincos_2pi(fr, &sn, &cs);
+    const float ct = 1.0f - (2.0f * fi + 1.0f) * (1.0f / (float)n);
+    const float st = sqrtf(1.0f - ct * ct);
+    return V3(cs * st, sn * st, ct);
+}
+
+void or_ddgi_fib(uint32_t i, uint32_t n, float out[3]) {
+    const v3 r = fib(i, n);
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+static uint32_t dg_tiles_x(const rt_ddgi_config* g) { return g->probe_count[0] * g->probe_count[1]; }
+static size_t dg_texel(const rt_ddgi_config* g, uint32_t p, uint32_t tx, uint32_t ty) {
+    const uint32_t ts = g->side + 2, tiles = dg_tiles_x(g);
+    return (size_t)((p / tiles) * ts + ty) * (tiles * ts) + (p % tiles) * ts + tx;
+}
+
+/* the interior texel a tile texel holds (mirrored across a folded edge,
+ * opposite corner at a corner) */
+static void dg_source(uint32_t side, uint32_t tx, uint32_t ty, uint32_t* ix, uint32_t* iy) {
+    const uint32_t e = side + 1;
+    const int bx = tx == 0 || tx == e, by = ty == 0 || ty == e;
+    if (bx && by) {
+        *ix = tx == 0 ? side : 1;
+        *iy = ty == 0 ? side : 1;
+    } else if (bx) {
+        *ix = tx == 0 ? 1 : side;
+        *iy = e - ty;
+    } else if (by) {
+        *ix = e - tx;
+        *iy = ty == 0 ? 1 : side;
+    } else {
+        *ix = tx;
+        *iy = ty;
+    }
+}
+
+/* Atlas update (majercik2019_shading_update_comp.comp:25-94) from surfels
+ * (8 floats: direction, radiance, distance, hit); irr / dep hold the
+ * previous atlases and receive the new ones (blended unless first). */
+void or_ddgi_update(const void* cfg, const float* surfels, int first, float* irr, float* dep) {
+    const rt_ddgi_config* g = (const rt_ddgi_config*)cfg;
+    const uint32_t P = g->probe_count[0] * g->probe_count[1] * g->probe_count[2];
+    const uint32_t R = g->rays_per_probe, ts = g->side + 2;
+    const float sc = 2.0f / (float)g->side;
+    for (uint32_t p = 0; p < P; ++p) {
+        const float* sp = surfels + (size_t)p * 8 * R;
+        for (uint32_t ty = 0; ty < ts; ++ty) {
+            for (uint32_t tx = 0; tx < ts; ++tx) {
+                uint32_t ix, iy;
+                dg_source(g->side, tx, ty, &ix, &iy);
+                const v3 nt = oct_dec(((float)(ix - 1) + 0.5f) * sc - 1.0f, ((float)(iy - 1) + 0.5f) * sc - 1.0f);
+                float sr = 0.0f, sg = 0.0f, sb = 0.0f, tw = 0.0f, d0 = 0.0f, d1 = 0.0f, tdw = 0.0f;
+                for (uint32_t i = 0; i < R; ++i) {
+                    const float* q = sp + 8 * i;
+                    float rd = g->max_ray_distance;
+                    if (q[7] != 0.0f) rd = fminf(rd, q[6]);
+                    const float w = fmaxf(0.0f, vdot(nt, V3(q[0], q[1], q[2])));
+                    const float dw = powf(w, g->depth_sharpness);
+                    if (w > 1e-6f) {
+                        sr = sr + w * q[3];
+                        sg = sg + w * q[4];
+                        sb = sb + w * q[5];
+                        tw = tw + w;
+                    }
+                    if (dw > 1e-6f) {
+                        d0 = d0 + dw * rd;
+                        d1 = d1 + dw * (rd * rd);
+                        tdw = tdw + dw;
+                    }
+                }
+                float nr = 0.0f, ng = 0.0f, nb = 0.0f, nd0 = 0.0f, nd1 = 0.0f;
+                if (tw > 0.0f) { nr = sr / tw; ng = sg / tw; nb = sb / tw; }
+                if (tdw > 0.0f) { nd0 = d0 / tdw; nd1 = d1 / tdw; }
+                const size_t t = dg_texel(g, p, tx, ty);
+                if (!first) {
+                    const float h = g->hysteresis, h1 = 1.0f - g->hysteresis;
+                    nr = h * irr[4 * t] + h1 * nr;
+                    ng = h * irr[4 * t + 1] + h1 * ng;
+                    nb = h * irr[4 * t + 2] + h1 * nb;
+                    nd0 = h * dep[2 * t] + h1 * nd0;
+                    nd1 = h * dep[2 * t + 1] + h1 * nd1;
+                }
+                irr[4 * t] = nr; irr[4 * t + 1] = ng; irr[4 * t + 2] = nb; irr[4 * t + 3] = 0.0f;
+                dep[2 * t] = nd0; dep[2 * t + 1] = nd1;
+            }
+        }
+    }
+}
+
+/* bilinear read of the octahedral coordinate (ox, oy) in probe p's tile,
+ * texel centres at k + 0.5, the map over tile coordinates [1, side + 1] */
+static void dg_bilinear(const rt_ddgi_config* g, uint32_t p, float ox, float oy, size_t t[4], float* ax, float* ay) {
+    const float fs = (float)g->side;
+    float fx = (ox + 1.0f) * 0.5f * fs + 0.5f;
+    float fy = (oy + 1.0f) * 0.5f * fs + 0.5f;
+    fx = fminf(fmaxf(fx, 0.0f), fs + 0.5f);
+    fy = fminf(fmaxf(fy, 0.0f), fs + 0.5f);
+    const float x0 = floorf(fx), y0 = floorf(fy);
+    *ax = fx - x0;
+    *ay = fy - y0;
+    const uint32_t ix = (uint32_t)x0, iy = (uint32_t)y0;
+    const uint32_t ix1 = ix + 1 > g->side + 1 ? g->side + 1 : ix + 1;
+    const uint32_t iy1 = iy + 1 > g->side + 1 ? g->side + 1 : iy + 1;
+    t[0] = dg_texel(g, p, ix, iy);
+    t[1] = dg_texel(g, p, ix1, iy);
+    t[2] = dg_texel(g, p, ix, iy1);
+    t[3] = dg_texel(g, p, ix1, iy1);
+}
+static float dg_blend(const float* a, int stride, int c, const size_t t[4], float ax, float ay) {
+    const float top = (1.0f - ax) * a[stride * t[0] + c] + ax * a[stride * t[1] + c];
+    const float bot = (1.0f - ax) * a[stride * t[2] + c] + ax * a[stride * t[3] + c];
+    return (1.0f - ay) * top + ay * bot;
+}
+
+/* getIrradiance (majercik2019_final_raygen.rgen:43-91) at x with normal n,
+ * reached along vd; energy_preservation applied */
+static v3 ddgi_lookup(const rt_ddgi_config* g, const float* irr, const float* dep, v3 x, v3 n, v3 vd) {
+    const float xs[3] = {x.x, x.y, x.z};
+    uint32_t bc[3];
+    float al[3];
+    for (int k = 0; k < 3; ++k) {
+        const float q = (xs[k] - g->probe_start[k]) / g->probe_spacing[k];
+        const float top = (float)(g->probe_count[k] - 1);
+        bc[k] = q > 0.0f ? (uint32_t)fminf(q, top) : 0u;
+        const float bp = g->probe_spacing[k] * (float)bc[k] + g->probe_start[k];
+        al[k] = fminf(fmaxf(fabsf(xs[k] - bp) / g->probe_spacing[k], 0.0f), 1.0f);
+    }
+    const v3 od = V3(-vd.x, -vd.y, -vd.z);
+    const v3 bias = vscl(g->normal_bias, vadd(n, vscl(3.0f, od)));
+    const int linear = (g->flags & RT_DDGI_LINEAR_BLENDING) != 0;
+    const int ref_vis = (g->flags & RT_DDGI_REFERENCE_VISIBILITY) != 0;
+    float sr = 0.0f, sg = 0.0f, sb = 0.0f, sw = 0.0f;
+    for (uint32_t i = 0; i < 8; ++i) {
+        const uint32_t off[3] = {i & 1u, (i >> 1) & 1u, (i >> 2) & 1u};
+        uint32_t pc[3];
+        for (int k = 0; k < 3; ++k)
+            pc[k] = bc[k] + off[k] > g->probe_count[k] - 1 ? g->probe_count[k] - 1 : bc[k] + off[k];
+        const uint32_t pidx = pc[0] + pc[1] * g->probe_count[0] + pc[2] * g->probe_count[0] * g->probe_count[1];
+        const v3 pp = V3(g->probe_spacing[0] * (float)pc[0] + g->probe_start[0],
+                         g->probe_spacing[1] * (float)pc[1] + g->probe_start[1],
+                         g->probe_spacing[2] * (float)pc[2] + g->probe_start[2]);
+        const v3 ptp = vadd(vsub(x, pp), bias);
+        const v3 dtp = vnormalize(V3(-ptp.x, -ptp.y, -ptp.z));
+        float w = (vdot(dtp, n) + 1.0f) * 0.5f;
+        const float tx = off[0] ? al[0] : 1.0f - al[0];
+        const float ty = off[1] ? al[1] : 1.0f - al[1];
+        const float tz = off[2] ? al[2] : 1.0f - al[2];
+        w = w * (tx * ty * tz);
+        float ox, oy, ax, ay;
+        size_t t[4];
+        oct_enc(vnormalize(V3(-dtp.x, -dtp.y, -dtp.z)), &ox, &oy);
+        dg_bilinear(g, pidx, ox, oy, t, &ax, &ay);
+        const float dist = sqrtf(vdot(ptp, ptp));
+        const float mean = dg_blend(dep, 2, 0, t, ax, ay);
+        const float m2 = dg_blend(dep, 2, 1, t, ax, ay);
+        const float var = fabsf(mean * mean - m2);
+        const float dd = fmaxf(dist - mean, 0.0f);
+        const float den = var + dd * dd;
+        float cheb = den > 0.0f ? var / den : 1.0f;
+        cheb = fmaxf(cheb * cheb * cheb, 0.0f);
+        if (ref_vis ? dist <= mean : dist > mean) w = w * cheb;
+        w = fmaxf(0.0001f, w);
+        float ir = dg_blend(irr, 4, 0, t, ax, ay), ig = dg_blend(irr, 4, 1, t, ax, ay), ib = dg_blend(irr, 4, 2, t, ax, ay);
+        if (!linear) {
+            ir = sqrtf(fmaxf(ir, 0.0f));
+            ig = sqrtf(fmaxf(ig, 0.0f));
+            ib = sqrtf(fmaxf(ib, 0.0f));
+        }
+        sr = sr + w * ir;
+        sg = sg + w * ig;
+        sb = sb + w * ib;
+        sw = sw + w;
+    }
+    v3 net = V3(sr / sw, sg / sw, sb / sw);
+    if (!linear) net = vmul(net, net);
+    return vscl(g->energy_preservation, net);
+}
+
+void or_ddgi_eval(const void* cfg, const float* irr, const float* dep, const float* samples, uint32_t n, float* out) {
+    const rt_ddgi_config* g = (const rt_ddgi_config*)cfg;
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* q = samples + 9 * (size_t)i;
+        const v3 e = ddgi_lookup(g, irr, dep, vld(q), vld(q + 3), vld(q + 6));
+        out[3 * (size_t)i] = e.x; out[3 * (size_t)i + 1] = e.y; out[3 * (size_t)i + 2] = e.z;
+    }
+}
+
+typedef struct {
+    pt_job j;
+    const rt_ddgi_config* g;
+    const float* rot;
+    uint32_t update, multi;
+    const float* irr;
+    const float* dep;
+    float* surf;
+} dg_job;
+
+/* One probe ray (majercik2019_probe_raygen.rgen:8-48 in the path tracer's
+ * units): miss -> sky without the sun; a light seen from the front or any
+ * back face -> 0; a surface -> td albedo / 2 * lookup (multi-bounce) plus the
+ * mean of shadow_count light samples, sample k drawing Philox (ray, update,
+ * k, 0x50000000); distance from the probe to the hit moved normal_bias along
+ * its normal. */
+static void dg_range(void* arg, uint32_t b, uint32_t e, int tid) {
+    (void)tid;
+    const dg_job* d = (const dg_job*)arg;
+    const pt_job* j = &d->j;
+    const rt_ddgi_config* g = d->g;
+    const uint32_t R = g->rays_per_probe, cx = g->probe_count[0], cy = g->probe_count[1];
+    const uint32_t key[2] = {(uint32_t)j->s->seed, (uint32_t)(j->s->seed >> 32)};
+    const uint32_t count = g->shadow_count ? g->shadow_count : 1;
+    for (uint32_t r = b; r < e; ++r) {
+        const uint32_t p = r / R, i = r % R;
+        const v3 f = fib(i, R);
+        const float* m9 = d->rot;
+        const v3 dir = V3(m9[0] * f.x + m9[1] * f.y + m9[2] * f.z, m9[3] * f.x + m9[4] * f.y + m9[5] * f.z,
+                          m9[6] * f.x + m9[7] * f.y + m9[8] * f.z);
+        const uint32_t px = p % cx, py = (p / cx) % cy, pz = p / (cx * cy);
+        const v3 o = V3(g->probe_spacing[0] * (float)px + g->probe_start[0],
+                        g->probe_spacing[1] * (float)py + g->probe_start[1],
+                        g->probe_spacing[2] * (float)pz + g->probe_start[2]);
+        rt_ray ray;
+        ray.ox = o.x; ray.oy = o.y; ray.oz = o.z; ray.tmin = 0.0f;
+        ray.dx = dir.x; ray.dy = dir.y; ray.dz = dir.z; ray.tmax = INFINITY;
+        uint64_t a1 = 0, a2 = 0;
+        const best_t h = trace_bvh_one(j->bvh, &ray, &a1, &a2);
+        v3 rad = V3(0.0f, 0.0f, 0.0f);
+        float dist = g->max_ray_distance, hit = 0.0f;
+        if (h.id == RT_MISS) {
+            if (j->s->flags & RT_RENDER_SKY) rad = vld(j->s->sky);
+        } else {
+            const v3 pt = vadd(o, vscl(h.t, dir));
+            const uint32_t t = h.id;
+            const v3 v0 = vld(j->pos + 3 * j->idx[3 * t]);
+            const v3 e0 = vsub(vld(j->pos + 3 * j->idx[3 * t + 1]), v0);
+            const v3 e1 = vsub(vld(j->pos + 3 * j->idx[3 * t + 2]), v0);
+            float b0, b1, b2;
+            bary(v0, e0, e1, pt, &b0, &b1, &b2);
+            v3 n = V3(0.0f, 0.0f, 0.0f);
+            n = vadd(n, vscl(b0, vld(j->nrm + 3 * j->idx[3 * t])));
+            n = vadd(n, vscl(b1, vld(j->nrm + 3 * j->idx[3 * t + 1])));
+            n = vadd(n, vscl(b2, vld(j->nrm + 3 * j->idx[3 * t + 2])));
+            n = vnormalize(n);
+            const v3 dv = vsub(o, vadd(pt, vscl(g->normal_bias, n)));
+            dist = sqrtf(vdot(dv, dv));
+            hit = 1.0f;
+            const rt_material* m = &j->mat[j->tri_mat[t]];
+            if (!(vdot(n, dir) > 0.0f) && !m->light_source) {
+                if (d->multi) {
+                    const v3 lk = ddgi_lookup(g, d->irr, d->dep, pt, n, dir);
+                    rad = vmul(vscl(0.5f * m->diffuse_threshold, V3(m->color[0], m->color[1], m->color[2])), lk);
+                }
+                float ar = 0.0f, ag = 0.0f, ab = 0.0f;
+                for (uint32_t k = 0; k < count; ++k) {
+                    const uint32_t ctr[4] = {r, d->update, k, 0x50000000u};
+                    uint32_t rnd[4];
+                    or_philox(ctr, key, rnd);
+                    rt_ray sh;
+                    v3 C;
+                    if (!j->nee || !nee_sample(j, pt, n, m, rnd, &sh, &C)) continue;
+                    uint64_t c1 = 0, c2 = 0;
+                    if (trace_bvh_one(j->bvh, &sh, &c1, &c2).id != RT_MISS) continue;
+                    ar = ar + C.x;
+                    ag = ag + C.y;
+                    ab = ab + C.z;
+                }
+                const float fc = (float)count;
+                rad = V3(rad.x + ar / fc, rad.y + ag / fc, rad.z + ab / fc);
+            }
+        }
+        float* q = d->surf + 8 * (size_t)r;
+        q[0] = dir.x; q[1] = dir.y; q[2] = dir.z;
+        q[3] = rad.x; q[4] = rad.y; q[5] = rad.z;
+        q[6] = dist; q[7] = hit;
+    }
+}
+
+int or_ddgi_surfels(const float* pos, const float* nrm, const uint32_t* idx, const uint32_t* tri_mat, const void* mat,
+                    uint32_t V, uint32_t T, uint32_t M, const void* cfg, const void* settings, const float rot[9],
+                    uint32_t update, const float* irr, const float* dep, int multi, float* surfels, int threads) {
+    (void)V;
+    const rt_ddgi_config* g = (const rt_ddgi_config*)cfg;
+    const rt_settings* s = (const rt_settings*)settings;
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    dg_job d;
+    memset(&d, 0, sizeof(d));
+    d.j.pos = pos; d.j.nrm = nrm; d.j.idx = idx; d.j.tri_mat = tri_mat; d.j.mat = (const rt_material*)mat; d.j.M = M;
+    d.j.s = s;
+    d.j.bvh = or_bvh_build(pos, idx, T, 4);
+    setup_lights(&d.j, pos, idx, tri_mat, T);
+    d.j.nee = d.j.num_lights > 0 || ((s->flags & RT_RENDER_SUN) && s->sun_cos < 1.0f);
+    d.g = g;
+    d.rot = rot;
+    d.update = update;
+    d.multi = (uint32_t)(multi != 0);
+    d.irr = irr;
+    d.dep = dep;
+    d.surf = surfels;
+    const uint32_t n = g->probe_count[0] * g->probe_count[1] * g->probe_count[2] * g->rays_per_probe;
+    run_parallel(threads, n, 64, dg_range, &d);
+    free(d.j.light_tri);
+    free(d.j.light_cdf);
+    or_bvh_free(d.j.bvh);
+    return 0;
+}
+
+int or_render_ddgi(const float* pos, const float* nrm, const uint32_t* idx, const uint32_t* tri_mat, const void* mat,
+                   uint32_t V, uint32_t T, uint32_t M, const void* cams, uint32_t num_views, const void* settings,
+                   uint32_t q_vertex, const void* cfg, const float* irr, const float* dep, float* rgb,
+                   uint64_t* queries, int threads) {
+    const rt_settings* s = (const rt_settings*)settings;
+    const float zero[3] = {0.0f, 0.0f, 0.0f};
+    cache_cfg cc;
+    memset(&cc, 0, sizeof(cc));
+    cc.rgb = zero;
+    cc.q_vertex = q_vertex;
+    cc.query_on = 1;
+    cc.ddgi = (const rt_ddgi_config*)cfg;
+    cc.ddgi_irr = irr;
+    cc.ddgi_dep = dep;
+    /* light sampling whenever there is something to sample */
+    int nl = 0;
+    for (uint32_t t = 0; t < T && !nl; ++t) nl = ((const rt_material*)mat)[tri_mat[t]].light_source != 0;
+    cc.nee = (uint32_t)(nl || ((s->flags & RT_RENDER_SUN) && s->sun_cos < 1.0f));
+    int rc = pt_run(pos, nrm, idx, tri_mat, mat, V, T, M, cams, num_views, settings, rgb, NULL, NULL, NULL, threads,
+                    &cc);
+    if (queries) *queries = cc.queries;
     return rc;
 }
 

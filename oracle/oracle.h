@@ -115,6 +115,26 @@ int or_render_cached(const float* pos, const float* nrm, const uint32_t* idx, co
                      const void* settings, uint32_t q_vertex, uint32_t train_every, int query_on, int nee,
                      const float cache_rgb[3], float* rgb, float* records, uint64_t counts[2], int threads);
 
+/* Irradiance probes (rt_ddgi_*, the Vulkan renderer's Majercik2019
+ * pipeline in the path tracer's units); cfg is an rt_ddgi_config. */
+void or_ddgi_oct_encode(const float v[3], float o[2]);
+void or_ddgi_oct_decode(const float o[2], float v[3]);
+void or_ddgi_fib(uint32_t i, uint32_t n, float out[3]);
+/* atlas update from surfels (8 floats each); irr / dep: previous atlases in,
+ * new ones out (blended by the hysteresis unless first) */
+void or_ddgi_update(const void* cfg, const float* surfels, int first, float* irr, float* dep);
+void or_ddgi_eval(const void* cfg, const float* irr, const float* dep, const float* samples, uint32_t n, float* out);
+/* surfels of one update: ray directions rot * fib, radiance with the field
+ * irr / dep before the update (multi-bounce), update index for Philox */
+int or_ddgi_surfels(const float* pos, const float* nrm, const uint32_t* idx, const uint32_t* tri_mat, const void* mat,
+                    uint32_t V, uint32_t T, uint32_t M, const void* cfg, const void* settings, const float rot[9],
+                    uint32_t update, const float* irr, const float* dep, int multi, float* surfels, int threads);
+/* rt_render_ddgi: the cache mode with the field answering the queries */
+int or_render_ddgi(const float* pos, const float* nrm, const uint32_t* idx, const uint32_t* tri_mat, const void* mat,
+                   uint32_t V, uint32_t T, uint32_t M, const void* cams, uint32_t num_views, const void* settings,
+                   uint32_t q_vertex, const void* cfg, const float* irr, const float* dep, float* rgb,
+                   uint64_t* queries, int threads);
+
 /* BidirectionalPathTracer (graphic/bidirectional_path_tracer.cpp:20-168)
  * for one camera: settings supply spp, seed and sample_offset; Philox
  * counters (pixel, sample, depth, 0) on the vision side and (pixel, sample,
