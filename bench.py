@@ -498,6 +498,10 @@ def bench_nrc(args):
       cached            --frames frames of online training at 1 spp (about
                         2^16 training records per frame, 4 Adam steps of
                         16384), then 1-spp frames with training on
+      probes            the irradiance-probe field (DDGI) in the cache's
+                        place: one field update per frame for --frames
+                        frames, then 1-spp frames (update + render), at the
+                        cache's query vertex and at the first hit
     Each estimator's RMSE scales as c / sqrt(spp); the time a method needs to
     reach the cached frame's RMSE is (c / rmse_cached)^2 * (time per spp).
     The MLP kernels are timed separately on 1920*1080 queries and 65536
@@ -597,6 +601,38 @@ def bench_nrc(args):
         # plain PT as the judge of the reference (and so of the cache's bias): frame means
         res["judge"] = {"reference_mean": round(float(ref.mean()), 6), "plain_mean": round(float(plain_hi.mean()), 6),
                         "plain_spp": max(16, args.ref_spp // 4)}
+        # irradiance probes (the Vulkan renderer's DDGI term) in the cache's place: a grid over the scene's
+        # bounds, one field update per frame (random rotation, multi-bounce, hysteresis), then the same
+        # 1-spp frames with the field answering the queries
+        from master_thesis_amd.probes import ProbeField
+        lo, hi = np.asarray(sc.pos.min(axis=0), np.float64), np.asarray(sc.pos.max(axis=0), np.float64)
+        count = (32, 4, 32) if args.nrc_scene == "terrain" else (4, 4, 4)
+        inset = 0.05 * (hi - lo)
+        spacing = (hi - lo - 2 * inset) / (np.asarray(count) - 1)
+        field = ProbeField(t, count, lo + inset, spacing, rays=256, side=8,
+                           max_distance=float(np.linalg.norm(hi - lo)), sharpness=50.0,
+                           normal_bias=float(0.1 * spacing.min()), hysteresis=0.95, shadow_count=1,
+                           multi_bounce=True, rotate=True)
+        t0 = time.time()
+        ums = []
+        for f in range(args.frames):
+            ums.append(field.update(S(1, f, 7 + f)))
+        log(f"[bench] nrc: {args.frames} probe-field updates in {time.time() - t0:.1f}s "
+            f"({float(np.median(ums)):.3f} ms each, {field.num_probes} probes x {field.rays} rays)")
+        pr = {"probes": list(count), "rays_per_probe": field.rays, "update_ms_median": round(float(np.median(ums)), 3)}
+        for qv in (1, 0):
+            def probes(spp, off, qv=qv):
+                um = field.update(S(1, 2000000 + off, 3))
+                img, st = field.render(cam, S(spp, off), query_vertex=qv)
+                return img, st.render_ms + um
+            e, ms = run(probes, 1, reps=4)
+            bigp, _ = field.render(cam, S(64, 6000000), query_vertex=qv)
+            pr[f"query_vertex_{qv}"] = {"rmse": {"1": round(e, 5)}, "ms": {"1": round(ms, 3)},
+                                        "rmse_64spp_vs_reference": round(rmse(bigp), 5),
+                                        "mean_diff_vs_reference": round(float((bigp - ref).mean()), 6)}
+            log(f"[bench] nrc: probes (query vertex {qv}) 1 spp rmse {e:.4f} {ms:.2f} ms, 64 spp rmse {rmse(bigp):.4f}")
+        field.close()
+        res["probes"] = pr
         # time for each method to reach the cached frame's RMSE
         ratio = {}
         for name in ("plain", "light_sampling"):
@@ -638,6 +674,8 @@ def bench_nrc(args):
             "value": round(ratio["plain"], 2), "unit": "x", "higher_is_better": True, "n_gpus": 1,
             "dtype": "f32 (paths), bf16 MFMA with f32 accumulation (network)", "data": "synthetic",
             "network_gain_over_light_sampling": round(ratio["light_sampling"], 3),
+            "cache_vs_probes": {"rmse_ratio_1spp": round(res["probes"]["query_vertex_1"]["rmse"]["1"] / e1, 3),
+                                "ms_ratio_1spp": round(res["probes"]["query_vertex_1"]["ms"]["1"] / ms1, 3)},
             "estimators": res, "training_frames": args.frames, "train_every": every,
             "mlp": {"infer_ms_per_frame": round(inf_ms, 4), "infer_tflops": round(inf_tf, 1),
                     "train_ms_per_step_65536": round(trn_ms, 4), "train_tflops": round(trn_tf, 1)},

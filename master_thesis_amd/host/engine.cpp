@@ -44,9 +44,52 @@ bool HipBidirectionalPathTracer::renderFrame(const FrameRenderData& frd, float* 
     return true;
 }
 
+HipMajercik2019::~HipMajercik2019() { rt_ddgi_destroy(field_); }
+
+void HipMajercik2019::parseInput(const InputEntry& e) {
+    config = rt_ddgi_config{};
+    config.rays_per_probe = e.get_uint("perProbeRayCount");
+    config.max_ray_distance = e.get_float("maxProbeRayDistance");
+    config.side = e.get_uint("probeSampleSideLength");
+    config.depth_sharpness = e.get_float("depthSharpness");
+    config.normal_bias = e.get_float("normalBias");
+    config.flags = e.get_uint("linearBlending") ? RT_DDGI_LINEAR_BLENDING : 0u;
+    config.energy_preservation = e.get_float("energyPreservation");
+    config.shadow_count = e.get_uint("shadowCountProbe");
+    if (e.has("hysteresis")) config.hysteresis = e.get_float("hysteresis");
+    settings = rt_settings{};
+    settings.max_depth = e.get_uint("visionJumpCount");
+    settings.spp = e.get_uint("shadowCountVision");
+    settings.seed = e.has("seed") ? (uint64_t)std::stoull(e.get_string("seed")) : 1ull;
+    settings.num_views = 1;
+    field_updates = e.has("fieldUpdates") ? e.get_uint("fieldUpdates") : 1u;
+}
+
+bool HipMajercik2019::renderFrame(const FrameRenderData& frd, float* rgb) const {
+    if (!field_) {
+        if (!frd.probes || !frd.probes->present) throw InitError("HipMajercik2019", "the scene has no Probes entry");
+        rt_ddgi_config c = config;
+        for (int k = 0; k < 3; ++k) {
+            c.probe_count[k] = frd.probes->count[k];
+            c.probe_start[k] = frd.probes->start[k];
+            c.probe_spacing[k] = frd.probes->spacing[k];
+        }
+        if (rt_ddgi_create(frd.ctx, &c, &field_) != RT_OK) throw InitError("HipMajercik2019", rt_last_error(frd.ctx));
+    }
+    for (uint32_t k = 0; k < field_updates; ++k) {
+        rt_settings u = settings;
+        u.seed = settings.seed + frame_++;
+        if (rt_ddgi_update(field_, &u) != RT_OK) throw InitError("HipMajercik2019", rt_last_error(frd.ctx));
+    }
+    if (rt_render_ddgi(frd.ctx, field_, frd.camera, 1, &settings, 0, rgb, nullptr, RT_PTR_HOST, nullptr) != RT_OK)
+        throw InitError("HipMajercik2019", rt_last_error(frd.ctx));
+    return true;
+}
+
 std::unique_ptr<Renderer> make_renderer(const std::string& name) {
     if (name == "PathTracer") return std::make_unique<HipPathTracer>();
     if (name == "BidirectionalPathTracer") return std::make_unique<HipBidirectionalPathTracer>();
+    if (name == "Majercik2019") return std::make_unique<HipMajercik2019>();
     throw InitError("getRenderer not found", name);
 }
 
@@ -72,7 +115,7 @@ void GraphicsEngine::render() {
     if (!renderer_) throw InitError("GraphicsEngine", "render before init");
     const rt_camera cam = camera.rt(width, height);
     auto t0 = std::chrono::steady_clock::now();
-    FrameRenderData frd{ctx_, &cam};
+    FrameRenderData frd{ctx_, &cam, &probes};
     if (!renderer_->renderFrame(frd, rgb_.data())) {
         // per-pixel path: workers pull pixels from an atomic counter
         std::atomic<uint32_t> next(0);

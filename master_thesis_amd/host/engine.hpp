@@ -11,6 +11,11 @@
 //                   (graphic/bidirectional_path_tracer.h:9-46): keys
 //                   visionJumpCount / lightJumpCount / maxDepth /
 //                   raysPerPixel, optional seed.
+//   HipMajercik2019 the Vulkan renderer's Majercik2019 (renderer/
+//                   majercik2019_renderer.cpp:75-91) on libmtrt's irradiance
+//                   probes: same keys; each frame updates the field
+//                   (fieldUpdates times, default 1) and renders with it at
+//                   the first hit, shadowCountVision samples per pixel.
 //   make_renderer   getRenderer (main.cpp:22-26): by the .renderer entry name.
 //   GraphicsEngine  graphic/graphics_engine.h:17-63: objects -> scene upload
 //                   and BVH build (init), render(), saveImage (clamp, 8-bit
@@ -36,6 +41,7 @@ struct PixelRenderData {
 struct FrameRenderData {
     rt_ctx* ctx = nullptr;
     const rt_camera* camera = nullptr;
+    const ProbeGrid* probes = nullptr;  // the scene's Probes entry
 };
 
 class Renderer {
@@ -63,6 +69,20 @@ class HipBidirectionalPathTracer : public Renderer {
     rt_bdpt_settings settings{};
 };
 
+class HipMajercik2019 : public Renderer {
+  public:
+    ~HipMajercik2019() override;
+    void parseInput(const InputEntry& e) override;
+    bool renderFrame(const FrameRenderData& frd, float* rgb) const override;
+    rt_ddgi_config config{};  // probe grid filled from the scene at the first frame
+    rt_settings settings{};   // max_depth visionJumpCount, spp shadowCountVision, seed
+    uint32_t field_updates = 1;
+
+  private:
+    mutable rt_ddgi* field_ = nullptr;
+    mutable uint64_t frame_ = 0;
+};
+
 // Throws InitError("getRenderer not found", name) for unknown names.
 std::unique_ptr<Renderer> make_renderer(const std::string& name);
 
@@ -75,6 +95,7 @@ class GraphicsEngine {
 
     uint32_t width = 0, height = 0;
     CameraDesc camera;
+    ProbeGrid probes;  // the scene's Probes entry, for renderers that use it
 
     void init(std::unique_ptr<Renderer> renderer, const FlatScene& scene, unsigned thread_count);
     void render();

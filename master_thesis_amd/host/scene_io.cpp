@@ -129,7 +129,15 @@ void SceneDesc::add(const SceneObject& o, std::shared_ptr<MeshData> mesh) {
 
 void SceneDesc::load(const std::string& scene_path, const std::string& mesh_dir) {
     for (const InputEntry& e : parse_config_file(scene_path)) {
-        if (e.name == "Probes") continue;  // DDGI probe grid, not geometry (mesh_manager.cpp:36)
+        if (e.name == "Probes") {  // the probe grid, not geometry (mesh_manager.cpp:36-40)
+            for (int k = 0; k < 3; ++k) {
+                probes_.count[k] = e.get_uint("probeCount", (size_t)k);
+                probes_.start[k] = e.get_float("probeStartCorner", (size_t)k);
+                probes_.spacing[k] = e.get_float("betweenProbeDistance", (size_t)k);
+            }
+            probes_.present = true;
+            continue;
+        }
         SceneObject o;
         o.mesh = e.name;
         o.position = e.get_vec3("position");

@@ -7,7 +7,8 @@
 //                     v / vn / f lines, unique (pos, tex, normal) vertices in
 //                     first-seen order, faces fan-triangulated
 //   SceneObject       GraphicsObject's scene-file fields (graphics_object.h:28-52)
-//   SceneDesc::load   MeshManager::createObjectsFromFile (mesh_manager.cpp:30-60)
+//   SceneDesc::load   MeshManager::createObjectsFromFile (mesh_manager.cpp:30-60),
+//                     the Probes entry included (ProbeGrid)
 //   SceneDesc::flatten GraphicsObject::init (graphics_object.cpp:12-52): v * (S R T)
 //                     for positions (w = 1) and normals (w = 0), thresholds
 //   CameraDesc        Camera (camera.h:9-27, camera.cpp:13-51) + the matrices
@@ -49,6 +50,15 @@ struct SceneObject {
     rt_material material() const;
 };
 
+// The scene file's Probes entry (mesh_manager.cpp:36-40): the irradiance
+// probe grid of the Majercik2019 renderer.
+struct ProbeGrid {
+    bool present = false;
+    uint32_t count[3] = {0, 0, 0};   // probeCount
+    float start[3] = {0, 0, 0};      // probeStartCorner
+    float spacing[3] = {0, 0, 0};    // betweenProbeDistance
+};
+
 struct FlatScene {
     std::vector<float> pos, nrm;
     std::vector<uint32_t> idx, tri_mat;
@@ -63,8 +73,10 @@ class SceneDesc {
     void add(const SceneObject& o, std::shared_ptr<MeshData> mesh);
     FlatScene flatten() const;
     const std::vector<SceneObject>& objects() const { return objects_; }
+    const ProbeGrid& probes() const { return probes_; }
 
   private:
+    ProbeGrid probes_;
     std::shared_ptr<MeshData> mesh(const std::string& name, const std::string& mesh_dir);
     std::vector<SceneObject> objects_;
     std::vector<std::shared_ptr<MeshData>> object_mesh_;

@@ -5,7 +5,8 @@
 //
 // The renderer is chosen by the first entry name of the .renderer file
 // ("PathTracer" -> the GPU path tracer, "BidirectionalPathTracer" -> the GPU
-// bidirectional tracer).  Meshes are read from
+// bidirectional tracer, "Majercik2019" -> the GPU irradiance probes with the
+// scene's Probes grid).  Meshes are read from
 // $MTH_MESH_DIR, else from <scene dir>/../obj (the reference's res/ layout).
 #include <cstdio>
 #include <cstdlib>
@@ -49,6 +50,7 @@ int main(int argc, char* argv[]) {
         engine.width = width;
         engine.height = height;
         engine.camera.parse(centries[0]);
+        engine.probes = scene.probes();
         engine.init(std::move(renderer), scene.flatten(), threads);
         engine.render();
         engine.save_image(out_path);

@@ -161,3 +161,56 @@ def test_field_outliving_its_context():
     assert lib.rt_ddgi_update(h, C.byref(make_settings(1, 1))) == L.RT_E_STATE
     lib.rt_ddgi_destroy(h)
     f.h = None
+
+
+def test_sw_render_cli_majercik2019(tracer, tmp_path):
+    """The Vulkan renderer's Majercik2019 renderer file and a scene file with a
+    Probes entry drive the GPU probes through the SoftwareRenderer-compatible
+    command line: one field update, then shadowCountVision samples per pixel
+    at the first hit; the same image as the library's (8-bit, up to the float
+    sums' order)."""
+    import subprocess
+    from master_thesis_amd import build as B
+    (tmp_path / "obj").mkdir()
+    (tmp_path / "scene").mkdir()
+    (tmp_path / "obj" / "panel.obj").write_text(
+        "v 0.0 1.0 -1.0\nv 0.0 -1.0 -1.0\nv 0.0 1.0 1.0\nv 0.0 -1.0 1.0\nvn 1.0 0.0 0.0\nf 2//1 1//1 3//1 4//1\n")
+    (tmp_path / "scene" / "box.scene").write_text(
+        "panel.obj\n\tposition(-1, 0, 0)\n\n"
+        "panel.obj\n\tposition(0, -1, 0)\n\trotation(0, 0, 1, 1.57079)\n\n"
+        "panel.obj\n\tposition(0, 1, 0)\n\trotation(0, 0, 1, -1.57079)\n\n"
+        "panel.obj\n\tposition(0, 0, -1)\n\trotation(0, 1, 0, -1.57079)\n\tcolor(0, 1, 0)\n\n"
+        "panel.obj\n\tposition(0, 0, 1)\n\trotation(0, 1, 0, 1.57079)\n\tcolor(1, 0, 0)\n\n"
+        "panel.obj\n\tposition(0, 0.99, 0)\n\tscale(0.2, 0.2, 0.2)\n\trotation(0, 0, 1, -1.57079)\n\tlightSource(1.0)\n"
+        "\tcolor(15, 15, 15)\n\n"
+        "Probes\n\tprobeCount(3, 3, 3)\n\tprobeStartCorner(-0.8, -0.8, -0.8)\n\tbetweenProbeDistance(0.8, 0.8, 0.8)\n")
+    (tmp_path / "dd.renderer").write_text(
+        "Majercik2019\n\tvisionJumpCount(3)\n\tperProbeRayCount(128)\n\tmaxProbeRayDistance(100)\n"
+        "\tprobeSampleSideLength(6)\n\tdepthSharpness(4)\n\tnormalBias(0.02)\n\tlinearBlending(0)\n"
+        "\tenergyPreservation(1)\n\tshadowCountProbe(2)\n\tshadowCountVision(8)\n\tseed(7)\n")
+    (tmp_path / "c.camera").write_text("Camera\n\tposition(3.4, 0, 0)\n\tangle(1, 0)\n")
+    out = tmp_path / "dd.ppm"
+    p = subprocess.run([str(B.CLI), str(tmp_path / "dd.renderer"), str(tmp_path / "scene" / "box.scene"), "64", "48",
+                        "4", str(tmp_path / "c.camera"), str(out)], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    head = b"P6\n64\n48\n255\n"
+    img = np.frombuffer(out.read_bytes()[len(head):], np.uint8).reshape(48, 64, 3).astype(int)
+    sc = H.load_scene(str(tmp_path / "scene" / "box.scene"), str(tmp_path / "obj"))
+    tracer.upload(sc)
+    tracer.build()
+    f = ProbeField(tracer, (3, 3, 3), (-0.8, -0.8, -0.8), (0.8, 0.8, 0.8), rays=128, side=6, max_distance=100.0,
+                   sharpness=4.0, normal_bias=0.02, energy=1.0, shadow_count=2)
+    f.update(make_settings(spp=8, max_depth=3, seed=7))
+    cam = H.camera_from_file(str(tmp_path / "c.camera"), 64, 48)
+    rgb, st = f.render(cam, make_settings(spp=8, max_depth=3, seed=7), query_vertex=0)
+    f.close()
+    want = (np.clip(rgb, 0.0, 1.0) * np.float32(255.0)).astype(int)
+    d = np.abs(img - want)
+    assert d.max() <= 1 and (d > 0).mean() < 0.02
+    assert st.queries > 0 and img.mean() > 5
+    # a scene without a Probes entry is an error for this renderer
+    (tmp_path / "scene" / "np.scene").write_text("panel.obj\n\tposition(-1, 0, 0)\n")
+    p = subprocess.run([str(B.CLI), str(tmp_path / "dd.renderer"), str(tmp_path / "scene" / "np.scene"), "8", "8",
+                        "1", str(tmp_path / "c.camera"), str(tmp_path / "x.ppm")], capture_output=True, text=True,
+                       timeout=120)
+    assert p.returncode == 1 and "no Probes entry" in p.stderr

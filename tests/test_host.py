@@ -53,7 +53,7 @@ def test_scene_keys_and_thresholds(tmp_path):
     text = ("# comment line\n"
             "tri.obj\n\tposition(1, 2, 3)\n\tcolor(0.5, 0.25, 1)\n\tdiffuse(2)\n\treflect(1)\n\ttransparent(1)\n"
             "\trefractionIndex(1.5)\n\n"
-            "Probes\n\tprobeCount(2, 2, 2)\n\n"
+            "Probes\n\tprobeCount(2, 2, 2)\n\tprobeStartCorner(-1, -1, -1)\n\tbetweenProbeDistance(2, 2, 2)\n\n"
             "tri.obj\n\tposition(0, 0, 0)\n\tlightSource(1.0)\n\tcolor(15, 15, 15)\n")
     sp, od = _files(tmp_path, text, {"tri.obj": TRI_OBJ})
     s = H.load_scene(str(sp), str(od))
