@@ -21,12 +21,21 @@ extern "C" {
 
 typedef struct mth_scene mth_scene;
 
-/* Reads a .scene file and its OBJ meshes (from mesh_dir) and flattens every
+/* Reads a .scene file and its meshes (OBJ from mesh_dir, binary STL from
+ * mesh_dir/../stl, as the reference's res/obj and res/stl) and flattens every
  * object to world space.  Replaces MeshManager::createObjectsFromFile
  * (software_renderer/mesh_manager.cpp:30-60) + GraphicsObject::init
  * (graphic/graphics_object.cpp:12-52).  One material per object, in file
- * order; the "Probes" entry is skipped as in the reference. */
+ * order; the "Probes" entry (the probe grid) adds no geometry. */
 int  mth_scene_load(const char* scene_path, const char* mesh_dir, mth_scene** out);
+/* One mesh file as loaded, in its own coordinates (no transform), one
+ * default material: OBJ (ObjLoader, obj_loader.cpp:19-56,156-184) or binary
+ * STL (STLLoader::load / get_mesh, stl_loader.cpp:18-123: y and z swapped,
+ * corners at one position merged into one vertex per group of facets whose
+ * normals lie within pi/3 of the group's first, normal = mean facet
+ * normal).  Vertex numbering is first-seen; each triangle corner's position
+ * and normal equal the reference's.  Free with mth_scene_free. */
+int  mth_mesh_load(const char* mesh_path, mth_scene** out);
 /* Pointers into the loaded scene, valid until mth_scene_free. */
 int  mth_scene_desc(const mth_scene* s, rt_scene_desc* out);
 void mth_scene_free(mth_scene* s);

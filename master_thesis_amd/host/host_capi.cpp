@@ -42,6 +42,32 @@ int mth_scene_load(const char* scene_path, const char* mesh_dir, mth_scene** out
     }
 }
 
+int mth_mesh_load(const char* mesh_path, mth_scene** out) {
+    if (!mesh_path || !out) return fail(RT_E_INVALID, "mth_mesh_load: NULL argument");
+    *out = nullptr;
+    try {
+        const std::string p(mesh_path);
+        auto ends = [&](const char* suf) {
+            const std::string x(suf);
+            return p.size() >= x.size() && p.compare(p.size() - x.size(), x.size(), x) == 0;
+        };
+        mth::MeshData m;
+        if (ends(".stl")) m = mth::load_stl(p);
+        else if (ends(".obj")) m = mth::load_obj(p);
+        else return fail(RT_E_INVALID, "mth_mesh_load: unsupported mesh format \"" + p + "\"");
+        auto* s = new mth_scene();
+        s->flat.pos = std::move(m.pos);
+        s->flat.nrm = std::move(m.nrm);
+        s->flat.idx = std::move(m.idx);
+        s->flat.tri_mat.assign(s->flat.idx.size() / 3, 0u);
+        s->flat.mat.push_back(mth::SceneObject{}.material());
+        *out = s;
+        return RT_OK;
+    } catch (const std::exception& e) {
+        return fail(RT_E_INVALID, e.what());
+    }
+}
+
 int mth_scene_desc(const mth_scene* s, rt_scene_desc* out) {
     if (!s || !out) return fail(RT_E_INVALID, "mth_scene_desc: NULL argument");
     *out = s->flat.desc();

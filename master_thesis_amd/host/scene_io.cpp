@@ -1,11 +1,15 @@
-// scene_io.cpp — OBJ reader, scene-file objects, world-space flattening and
+// scene_io.cpp — OBJ and binary STL readers, scene-file objects, world-space flattening and
 // the camera.
 #include "scene_io.hpp"
 
+#include <algorithm>
+#include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <fstream>
 #include <sstream>
 #include <tuple>
+#include <unordered_map>
 
 namespace mth {
 
@@ -109,6 +113,109 @@ rt_scene_desc FlatScene::desc() const {
     return d;
 }
 
+// Binary STL (STLLoader::load / get_mesh, software_renderer/stl_loader.cpp:
+// 18-123): 80-byte header, uint32 count, 50-byte records (facet normal,
+// three vertices, attribute), y and z swapped.  The corners meeting at one
+// exact position are split into groups: the last corner opens the first
+// group, every other corner (in triangle order) joins the first group whose
+// opening corner's facet normal is within pi/3 of its own, else opens a new
+// one; a group becomes one vertex with the mean of its facet normals.  The
+// vertices are numbered in first-seen position order (the reference numbers
+// them in its hash map's order); every triangle corner gets the same
+// position and normal as in the reference.
+MeshData load_stl(const std::string& path) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in.is_open()) throw InitError("STLLoader", "Could not load STL file \"" + path + "\"!");
+    char header[80];
+    uint32_t count = 0;
+    in.read(header, 80);
+    in.read(reinterpret_cast<char*>(&count), 4);
+    if (!in) throw InitError("STLLoader", "truncated STL header in \"" + path + "\"");
+    in.seekg(0, std::ios::end);
+    const uint64_t size = (uint64_t)in.tellg();
+    if (size < 84 + 50ull * count) throw InitError("STLLoader", "truncated STL file \"" + path + "\"");
+    in.seekg(84);
+    struct Tri {
+        Vec3 n;
+        Vec3 v[3];
+    };
+    std::vector<Tri> tris(count);
+    std::vector<char> rec(50);
+    for (uint32_t t = 0; t < count; ++t) {
+        in.read(rec.data(), 50);
+        float f[12];
+        std::memcpy(f, rec.data(), 48);
+        for (float x : f)
+            if (std::isnan(x)) throw InitError("STLLoader", "NaN coordinate in \"" + path + "\"");
+        tris[t].n = Vec3{f[0], f[2], f[1]};
+        for (int k = 0; k < 3; ++k) tris[t].v[k] = Vec3{f[3 + 3 * k], f[3 + 3 * k + 2], f[3 + 3 * k + 1]};
+    }
+    // corners by exact position (float equality: -0 and +0 meet), first-seen order
+    struct Key {
+        float x, y, z;
+        bool operator==(const Key& o) const { return x == o.x && y == o.y && z == o.z; }
+    };
+    struct KeyHash {
+        size_t operator()(const Key& k) const {
+            auto h = [](float f) { return f == 0.0f ? (size_t)0 : std::hash<float>()(f); };
+            return h(k.x) ^ (h(k.y) << 1) ^ (h(k.z) << 2);
+        }
+    };
+    std::unordered_map<Key, uint32_t, KeyHash> slot;
+    std::vector<std::vector<uint32_t>> corners;  // per position: corner ids 3 t + k in triangle order
+    std::vector<Vec3> where;
+    for (uint32_t t = 0; t < count; ++t)
+        for (int k = 0; k < 3; ++k) {
+            const Vec3& v = tris[t].v[k];
+            auto it = slot.emplace(Key{v[0], v[1], v[2]}, (uint32_t)corners.size());
+            if (it.second) {
+                corners.emplace_back();
+                where.push_back(v);
+            }
+            corners[it.first->second].push_back(3 * t + (uint32_t)k);
+        }
+    // within pi/3 (stl_loader.cpp:13-16): Vector::angle = acos(clamp(dot / (|a| |b|))) in double, kept as float
+    auto fits = [](const Vec3& a, const Vec3& b) {
+        float d = 0.0f, ma = 0.0f, mb = 0.0f;
+        for (int k = 0; k < 3; ++k) d += a[k] * b[k];
+        for (int k = 0; k < 3; ++k) ma += a[k] * a[k];
+        for (int k = 0; k < 3; ++k) mb += b[k] * b[k];
+        const double c = (double)d / (std::sqrt((double)ma) * std::sqrt((double)mb));
+        const float angle = (float)std::acos(std::clamp(c, -1.0, 1.0));
+        return angle < 1.0 / 3.0 * M_PI;
+    };
+    MeshData m;
+    m.idx.assign(3 * (size_t)count, 0);
+    uint32_t next = 0;
+    for (size_t g = 0; g < corners.size(); ++g) {
+        std::vector<uint32_t> pre = corners[g];
+        std::vector<std::vector<uint32_t>> groups;
+        groups.push_back({pre.back()});
+        pre.pop_back();
+        for (uint32_t c : pre) {
+            bool added = false;
+            for (auto& grp : groups)
+                if (fits(tris[grp[0] / 3].n, tris[c / 3].n)) {
+                    grp.push_back(c);
+                    added = true;
+                    break;
+                }
+            if (!added) groups.push_back({c});
+        }
+        for (const auto& grp : groups) {
+            Vec3 n{0.0f, 0.0f, 0.0f};
+            for (uint32_t c : grp)
+                for (int k = 0; k < 3; ++k) n[k] += tris[c / 3].n[k];
+            for (int k = 0; k < 3; ++k) n[k] /= (float)grp.size();
+            for (int k = 0; k < 3; ++k) m.pos.push_back(where[g][k]);
+            for (int k = 0; k < 3; ++k) m.nrm.push_back(n[k]);
+            for (uint32_t c : grp) m.idx[c] = next;
+            ++next;
+        }
+    }
+    return m;
+}
+
 std::shared_ptr<MeshData> SceneDesc::mesh(const std::string& name, const std::string& mesh_dir) {
     auto it = meshes_.find(name);
     if (it != meshes_.end()) return it->second;
@@ -116,8 +223,12 @@ std::shared_ptr<MeshData> SceneDesc::mesh(const std::string& name, const std::st
         std::string s(suf);
         return name.size() >= s.size() && name.compare(name.size() - s.size(), s.size(), s) == 0;
     };
-    if (!ends(".obj")) throw InitError("MeshManager", "unsupported mesh format \"" + name + "\"");
-    auto m = std::make_shared<MeshData>(load_obj(mesh_dir + "/" + name));
+    std::shared_ptr<MeshData> m;
+    // MeshManager::getMesh (mesh_manager.cpp:17-28): .obj from the mesh directory, .stl from its
+    // sibling stl directory (the reference's res/obj and res/stl)
+    if (ends(".obj")) m = std::make_shared<MeshData>(load_obj(mesh_dir + "/" + name));
+    else if (ends(".stl")) m = std::make_shared<MeshData>(load_stl(mesh_dir + "/../stl/" + name));
+    else throw InitError("MeshManager", "unsupported mesh format \"" + name + "\"");
     meshes_[name] = m;
     return m;
 }

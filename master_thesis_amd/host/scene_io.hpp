@@ -6,6 +6,7 @@
 //   load_obj          ObjLoader::load / get_mesh (obj_loader.cpp:19-56,156-184):
 //                     v / vn / f lines, unique (pos, tex, normal) vertices in
 //                     first-seen order, faces fan-triangulated
+//   load_stl          STLLoader::load / get_mesh (stl_loader.cpp:18-123)
 //   SceneObject       GraphicsObject's scene-file fields (graphics_object.h:28-52)
 //   SceneDesc::load   MeshManager::createObjectsFromFile (mesh_manager.cpp:30-60),
 //                     the Probes entry included (ProbeGrid)
@@ -34,6 +35,9 @@ struct MeshData {
 };
 
 MeshData load_obj(const std::string& path);
+// binary STL (stl_loader.cpp:18-123): y and z swapped, normals smoothed
+// across facets within pi/3 of each other
+MeshData load_stl(const std::string& path);
 
 struct SceneObject {
     std::string mesh;            // mesh file name as written in the scene file

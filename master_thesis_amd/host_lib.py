@@ -23,6 +23,8 @@ def load() -> C.CDLL:
         P = C.c_void_p
         lib.mth_scene_load.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(P)]
         lib.mth_scene_load.restype = C.c_int
+        lib.mth_mesh_load.argtypes = [C.c_char_p, C.POINTER(P)]
+        lib.mth_mesh_load.restype = C.c_int
         lib.mth_scene_desc.argtypes = [P, C.POINTER(L.rt_scene_desc)]
         lib.mth_scene_desc.restype = C.c_int
         lib.mth_scene_free.argtypes = [P]
@@ -51,6 +53,18 @@ def load_scene(scene_path: str, mesh_dir: str) -> Scene:
     lib = load()
     h = C.c_void_p()
     _check(lib.mth_scene_load(str(scene_path).encode(), str(mesh_dir).encode(), C.byref(h)))
+    return _take_scene(lib, h)
+
+
+def load_mesh(mesh_path: str) -> Scene:
+    """One OBJ or binary STL mesh in its own coordinates (mth_mesh_load)."""
+    lib = load()
+    h = C.c_void_p()
+    _check(lib.mth_mesh_load(str(mesh_path).encode(), C.byref(h)))
+    return _take_scene(lib, h)
+
+
+def _take_scene(lib, h) -> Scene:
     try:
         d = L.rt_scene_desc()
         _check(lib.mth_scene_desc(h, C.byref(d)))

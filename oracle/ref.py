@@ -35,6 +35,8 @@ def load() -> C.CDLL:
                                         C.c_uint32, C.c_int, P]
         lib.ref_camera.argtypes = [P, C.c_float, C.c_float, C.c_uint32, C.c_uint32, P, P, P]
         lib.ref_load_scene.argtypes = [C.c_char_p, C.c_char_p, P, P, P, P, P, P]
+        lib.ref_stl_load.argtypes = [C.c_char_p, C.c_char_p, P, P, P, P]
+        lib.ref_stl_load.restype = C.c_int
         lib.ref_load_scene.restype = C.c_int
         lib.ref_render.argtypes = [P, P, P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, P]
         _lib = lib
@@ -128,3 +130,20 @@ def load_scene(scene_path: str, basepath: str):
     lib.ref_load_scene(scene_path.encode(), basepath.encode(), pos.ctypes.data, nrm.ctypes.data, idx.ctypes.data,
                        tri_obj.ctypes.data, colors.ctypes.data, cnt.ctypes.data)
     return pos, nrm, idx, tri_obj, colors
+
+
+def stl_load(workdir: str, name: str):
+    """STLLoader on <workdir>/../res/stl/<name>: (pos (V, 3), nrm (V, 3),
+    idx (T, 3)) in the loader's vertex order."""
+    lib = load()
+    cnt = np.zeros(2, np.uint32)
+    rc = lib.ref_stl_load(str(workdir).encode(), name.encode(), None, None, None, cnt.ctypes.data)
+    assert rc == 0, rc
+    V, T = (int(x) for x in cnt)
+    pos = np.zeros((V, 3), np.float32)
+    nrm = np.zeros((V, 3), np.float32)
+    idx = np.zeros((T, 3), np.uint32)
+    rc = lib.ref_stl_load(str(workdir).encode(), name.encode(), pos.ctypes.data, nrm.ctypes.data, idx.ctypes.data,
+                          cnt.ctypes.data)
+    assert rc == 0, rc
+    return pos, nrm, idx

@@ -10,6 +10,8 @@
 #include <thread>
 #include <vector>
 
+#include <unistd.h>
+
 #include "camera.h"
 #include "graphic/graphics_object.h"
 #include "graphic/mesh.h"
@@ -19,6 +21,7 @@
 #include "graphic/triangle.h"
 #include "input_parser.h"
 #include "mesh_manager.h"
+#include "stl_loader.h"
 
 namespace {
 
@@ -184,6 +187,38 @@ int ref_load_scene(const char* scene_path, const char* basepath, float* pos, flo
     counts[0] = V;
     counts[1] = T;
     counts[2] = (uint32_t)objs.size();
+    return 0;
+}
+
+// STLLoader::load + get_mesh (stl_loader.cpp:18-123) on <workdir>/../res/stl/<name>
+// (the loader opens "../res/stl/" + name relative to the working directory,
+// which is switched to workdir for the call).  Vertices in the loader's
+// order; with pos == NULL only the counts {V, T}.
+int ref_stl_load(const char* workdir, const char* name, float* pos, float* nrm, uint32_t* idx, uint32_t* counts) {
+    char prev[4096];
+    if (!getcwd(prev, sizeof(prev))) return -1;
+    if (chdir(workdir) != 0) return -2;
+    Mesh* m = nullptr;
+    try {
+        STLLoader loader("");
+        loader.load(name);
+        m = loader.get_mesh();
+    } catch (...) {
+        m = nullptr;
+    }
+    if (chdir(prev) != 0) return -3;
+    if (!m) return -4;
+    counts[0] = (uint32_t)m->vertices.size();
+    counts[1] = (uint32_t)(m->indices.size() / 3);
+    if (pos) {
+        for (size_t v = 0; v < m->vertices.size(); ++v)
+            for (int k = 0; k < 3; ++k) {
+                pos[3 * v + k] = m->vertices[v].pos[k];
+                nrm[3 * v + k] = m->vertices[v].normal[k];
+            }
+        for (size_t i = 0; i < m->indices.size(); ++i) idx[i] = (uint32_t)m->indices[i];
+    }
+    delete m;
     return 0;
 }
 
