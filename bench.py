@@ -50,7 +50,7 @@ def main() -> int:
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 512 MiB device copy (reference kernel for HBM counters)")
     ap.add_argument("--config", default="raycast",
-                    choices=["raycast", "terrain_raycast", "cornell", "terrain", "diff", "nrc", "bdpt"],
+                    choices=["raycast", "terrain_raycast", "cornell", "terrain", "diff", "nrc", "bdpt", "denoise"],
                     help="raycast: the headline microbench (default); cornell / terrain / diff / nrc: the other "
                          "workloads")
     ap.add_argument("--nrc-scene", default="terrain",
@@ -68,7 +68,7 @@ def main() -> int:
         return 2
     if args.config not in ("raycast", "terrain_raycast"):
         out = {"cornell": bench_cornell, "terrain": bench_terrain, "diff": bench_diff,
-               "nrc": bench_nrc, "bdpt": bench_bdpt}[args.config](args)
+               "nrc": bench_nrc, "bdpt": bench_bdpt, "denoise": bench_denoise}[args.config](args)
         print(json.dumps(out), flush=True)
         return 0
     if args.config == "terrain_raycast":
@@ -486,6 +486,54 @@ MLP_FWD_FLOP = 2 * (78 * 64 + 4 * 64 * 64 + 64 * 3)
 # gradients of layers 1..5 (4*64*64 + 64*3)
 MLP_TRAIN_FLOP = 2 * MLP_FWD_FLOP + 2 * (4 * 64 * 64 + 64 * 3)
 BF16_PEAK_TFLOPS = 2500.0
+
+
+def bench_denoise(args):
+    """Post-filters on a 3840x2160 float RGB image resident in HBM: Gauss
+    (kernelSize 5) and the 3x3 median, device time of each kernel from HIP
+    events on the context's stream.  Algorithmic bytes: the image read once
+    and written once (24 B per pixel).  CPU baseline: the oracle's filters on
+    a 480x270 crop, one thread, scaled per pixel."""
+    import torch
+
+    from master_thesis_amd.tracer import Tracer
+    W, Hh = 3840, 2160
+    g = torch.Generator(device="cuda").manual_seed(0)
+    img = torch.rand((Hh, W, 3), device="cuda", generator=g)
+    res = {}
+    with Tracer(0) as t:
+        for name, fn in (("gauss5", lambda: t.denoise_gauss(img, 5, 1.5)), ("median3", lambda: t.denoise_median(img))):
+            for _ in range(args.warmup):
+                fn()
+            ms = []
+            for _ in range(args.steps):
+                fn()
+                ms.append(t.timing()["last_kernel_ms"])
+            k = float(np.median(ms))
+            res[name] = {"kernel_ms": round(k, 4), "gb_per_s": round(24.0 * W * Hh / (k * 1e-3) / 1e9, 1)}
+            log(f"[bench] denoise: {name} {k:.4f} ms per 3840x2160 image")
+    crop = img[:270, :480].cpu().numpy()
+    cpu = {}
+    if not args.no_cpu:
+        import oracle as O
+        for name, fn in (("gauss5", lambda: O.denoise_gauss(crop, 5, 1.5)), ("median3", lambda: O.denoise_median(crop))):
+            t0 = time.time()
+            fn()
+            dt = time.time() - t0
+            cpu[name] = round(480 * 270 / dt / 1e6, 2)
+    gk = res["gauss5"]["kernel_ms"]
+    achieved = res["gauss5"]["gb_per_s"]
+    return {"metric": "Mpixels/s, Gauss 5x5 post-filter (3840x2160 float RGB)",
+            "value": round(W * Hh / (gk * 1e-3) / 1e6, 1), "unit": "Mpixels/s", "higher_is_better": True,
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "dtype": "f32", "data": "synthetic",
+            "filters": res,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_gauss",
+                         "bytes_alg_per_launch": 24 * W * Hh},
+            "cpu_baseline": ({"value": cpu["gauss5"], "unit": "Mpixels/s", "cores": 1, "kind": "port",
+                              "sample": "oracle filters on a 480x270 crop of the same image",
+                              "median3": cpu["median3"]} if cpu else None),
+            "config": {"workload": "GaussDenoiser kernelSize 5 and MedianDenoiser on a 3840x2160 float RGB image"}}
 
 
 def bench_nrc(args):

@@ -350,6 +350,23 @@ int  rt_render_nrc(rt_ctx* ctx, rt_nrc* nrc, const rt_camera* cams, uint32_t num
  * the in-out capacity (both NULL: query the count). */
 int  rt_render_nrc_records(rt_ctx* ctx, float* feats, float* targets, uint32_t* count);
 
+/* ---- post-filters ----------------------------------------------------------
+ * The Vulkan renderer's denoisers (src/graphic/denoiser/, chained after the
+ * renderer by the extra entries of a .renderer file, src/main.cpp:163-168) on
+ * float RGB images (width * height * 3, row-major), in and out distinct.
+ * Gauss (GaussDenoiser, gauss_denoiser.cpp:31-47 and
+ * res/shader/denoise_gauss_comp.comp:17-33): weights g[i] = exp(-(i - (k -
+ * 1))^2 / (2 sigma^2)) normalised to sum 1 (centred on the last index, as the
+ * reference computes them), out(p) = sum over x, y in [-k/2, k/2] of
+ * in(p + (x, y)) g[x + k/2] g[y + k/2], taps outside the image skipped.
+ * kernel_size odd, 1..19.  Median (MedianDenoiser, denoise_median_comp.comp:
+ * 22-46): the median of the 3x3 neighbourhood per channel by the shader's
+ * exchange network, taps outside the image 0. */
+int  rt_denoise_gauss_weights(uint32_t kernel_size, float sigma, float* weights);
+int  rt_denoise_gauss(rt_ctx* ctx, const float* in, float* out, uint32_t width, uint32_t height,
+                      uint32_t kernel_size, float sigma, int ptr_kind);
+int  rt_denoise_median(rt_ctx* ctx, const float* in, float* out, uint32_t width, uint32_t height, int ptr_kind);
+
 /* ---- irradiance probes (DDGI) --------------------------------------------
  * The Vulkan renderer's Majercik2019 pipeline (src/graphic/renderer/
  * majercik2019_renderer.cpp:35-47; res/shader/majercik2019_probe_raygen.rgen,

@@ -191,6 +191,9 @@ SIGNATURES = {
     "rt_render_nrc_records": (C.c_int, [_P, _P, _P, C.POINTER(C.c_uint32)]),
     "rt_render_nrc": (C.c_int, [_P, _P, C.POINTER(rt_camera), C.c_uint32, C.POINTER(rt_settings),
                                 C.POINTER(rt_nrc_render), _P, _P, C.c_int, C.POINTER(rt_nrc_stats)]),
+    "rt_denoise_gauss_weights": (C.c_int, [C.c_uint32, C.c_float, _P]),
+    "rt_denoise_gauss": (C.c_int, [_P, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_int]),
+    "rt_denoise_median": (C.c_int, [_P, _P, _P, C.c_uint32, C.c_uint32, C.c_int]),
     "rt_ddgi_create": (C.c_int, [_P, C.POINTER(rt_ddgi_config), C.POINTER(_P)]),
     "rt_ddgi_destroy": (None, [_P]),
     "rt_ddgi_atlas_size": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),

@@ -86,6 +86,18 @@ bool HipMajercik2019::renderFrame(const FrameRenderData& frd, float* rgb) const 
     return true;
 }
 
+DenoiserDesc make_denoiser(const InputEntry& e) {
+    DenoiserDesc d;
+    d.name = e.name;
+    if (e.name == "GaussDenoiser") {
+        d.kernel_size = e.get_uint("kernelSize");
+        d.sigma = e.get_float("sigma");
+    } else if (e.name != "MedianDenoiser") {
+        throw InitError("getDenoiser not found", e.name);
+    }
+    return d;
+}
+
 std::unique_ptr<Renderer> make_renderer(const std::string& name) {
     if (name == "PathTracer") return std::make_unique<HipPathTracer>();
     if (name == "BidirectionalPathTracer") return std::make_unique<HipBidirectionalPathTracer>();
@@ -133,6 +145,17 @@ void GraphicsEngine::render() {
         for (unsigned t = 1; t < threads_; ++t) th.emplace_back(work);
         work();
         for (auto& t : th) t.join();
+    }
+    if (!denoisers.empty()) {
+        std::vector<float> tmp(rgb_.size());
+        for (const DenoiserDesc& d : denoisers) {
+            int rc = d.name == "GaussDenoiser"
+                         ? rt_denoise_gauss(ctx_, rgb_.data(), tmp.data(), width, height, d.kernel_size, d.sigma,
+                                            RT_PTR_HOST)
+                         : rt_denoise_median(ctx_, rgb_.data(), tmp.data(), width, height, RT_PTR_HOST);
+            if (rc != RT_OK) throw InitError(d.name, rt_last_error(ctx_));
+            rgb_.swap(tmp);
+        }
     }
     last_render_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     // clamp to [0, 1], then (char)(v * 255.0f) (graphics_engine.cpp:90-95)

@@ -17,6 +17,9 @@
 //                   (fieldUpdates times, default 1) and renders with it at
 //                   the first hit, shadowCountVision samples per pixel.
 //   make_renderer   getRenderer (main.cpp:22-26): by the .renderer entry name.
+//   make_denoiser   getDenoiser (src/main.cpp:73-77): GaussDenoiser /
+//                   MedianDenoiser entries after the renderer's, run on the
+//                   GPU (rt_denoise_*) in order after each frame.
 //   GraphicsEngine  graphic/graphics_engine.h:17-63: objects -> scene upload
 //                   and BVH build (init), render(), saveImage (clamp, 8-bit
 //                   truncation, binary PPM).
@@ -83,6 +86,17 @@ class HipMajercik2019 : public Renderer {
     mutable uint64_t frame_ = 0;
 };
 
+// A post-filter of the Vulkan renderer (src/main.cpp:73-77,163-168): the
+// .renderer file's entries after the first, applied in file order to the
+// float image before the clamp and 8-bit truncation.
+struct DenoiserDesc {
+    std::string name;          // "GaussDenoiser" (kernelSize, sigma) or "MedianDenoiser"
+    uint32_t kernel_size = 0;
+    float sigma = 0.0f;
+};
+// Throws InitError("getDenoiser not found", name) for unknown names.
+DenoiserDesc make_denoiser(const InputEntry& e);
+
 // Throws InitError("getRenderer not found", name) for unknown names.
 std::unique_ptr<Renderer> make_renderer(const std::string& name);
 
@@ -96,6 +110,7 @@ class GraphicsEngine {
     uint32_t width = 0, height = 0;
     CameraDesc camera;
     ProbeGrid probes;  // the scene's Probes entry, for renderers that use it
+    std::vector<DenoiserDesc> denoisers;  // post-filters, in order
 
     void init(std::unique_ptr<Renderer> renderer, const FlatScene& scene, unsigned thread_count);
     void render();

@@ -6,7 +6,8 @@
 // The renderer is chosen by the first entry name of the .renderer file
 // ("PathTracer" -> the GPU path tracer, "BidirectionalPathTracer" -> the GPU
 // bidirectional tracer, "Majercik2019" -> the GPU irradiance probes with the
-// scene's Probes grid).  Meshes are read from
+// scene's Probes grid); later entries (GaussDenoiser, MedianDenoiser) filter
+// the image in order, as the Vulkan renderer chains them.  Meshes are read from
 // $MTH_MESH_DIR, else from <scene dir>/../obj (the reference's res/ layout).
 #include <cstdio>
 #include <cstdlib>
@@ -51,6 +52,7 @@ int main(int argc, char* argv[]) {
         engine.height = height;
         engine.camera.parse(centries[0]);
         engine.probes = scene.probes();
+        for (size_t i = 1; i < rentries.size(); ++i) engine.denoisers.push_back(mth::make_denoiser(rentries[i]));
         engine.init(std::move(renderer), scene.flatten(), threads);
         engine.render();
         engine.save_image(out_path);

@@ -155,6 +155,28 @@ class Tracer:
                                        L.RT_PTR_DEVICE if dev else L.RT_PTR_HOST))
         return rgb, rgb8
 
+    # -- post-filters ---------------------------------------------------
+    def _filter(self, fn, img, *args):
+        dev = L._is_device(img)
+        if dev:
+            import torch
+            img = img.contiguous()
+            out = torch.empty_like(img)
+        else:
+            img = np.ascontiguousarray(img, np.float32)
+            out = np.empty_like(img)
+        H, W = img.shape[0], img.shape[1]
+        self._check(fn(self.h, L._ptr(img), L._ptr(out), W, H, *args, L.RT_PTR_DEVICE if dev else L.RT_PTR_HOST))
+        return out
+
+    def denoise_gauss(self, img, kernel_size: int, sigma: float):
+        """GaussDenoiser on an (H, W, 3) float image (numpy or device tensor)."""
+        return self._filter(self.lib.rt_denoise_gauss, img, kernel_size, sigma)
+
+    def denoise_median(self, img):
+        """MedianDenoiser (3x3 median per channel) on an (H, W, 3) float image."""
+        return self._filter(self.lib.rt_denoise_median, img)
+
     def render_bdpt(self, cam: L.rt_camera, spp: int, vision_jumps: int = 5, light_jumps: int = 5,
                     max_depth: int = 5, seed: int = 1, sample_offset: int = 0, want_rgb8: bool = False):
         """BidirectionalPathTracer frame; returns (rgb HxWx3, rgb8 or None)."""

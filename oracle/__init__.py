@@ -374,3 +374,31 @@ def render_ddgi(scene, cams, settings, cfg, irr, dep, q_vertex: int = 0, threads
                             threads)
     assert rc == 0, rc
     return rgb, int(q[0])
+
+
+# ---- post-filters
+
+def gauss_weights(k: int, sigma: float) -> np.ndarray:
+    lib = load()
+    lib.or_gauss_weights.argtypes = [C.c_uint32, C.c_float, C.c_void_p]
+    g = np.zeros(20, np.float32)
+    assert lib.or_gauss_weights(k, sigma, _p(g)) == 0
+    return g[:k]
+
+
+def denoise_gauss(img: np.ndarray, k: int, sigma: float) -> np.ndarray:
+    lib = load()
+    lib.or_denoise_gauss.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float]
+    a = np.ascontiguousarray(img, np.float32)
+    out = np.empty_like(a)
+    assert lib.or_denoise_gauss(_p(a), _p(out), a.shape[1], a.shape[0], k, sigma) == 0
+    return out
+
+
+def denoise_median(img: np.ndarray) -> np.ndarray:
+    lib = load()
+    lib.or_denoise_median.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]
+    a = np.ascontiguousarray(img, np.float32)
+    out = np.empty_like(a)
+    assert lib.or_denoise_median(_p(a), _p(out), a.shape[1], a.shape[0]) == 0
+    return out

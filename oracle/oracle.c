@@ -1961,3 +1961,73 @@ int or_render_bdpt(const float* pos, const float* nrm, const uint32_t* idx, cons
     return 0;
 }
 
+/* ------------------------------------------------------------------ */
+/* Post-filters of the Vulkan renderer (src/graphic/denoiser/). */
+
+/* GaussDenoiser::updateUniforms (gauss_denoiser.cpp:31-47): exp(-(i - (k - 1))^2
+ * / (2 sigma^2)), scaled by 1 / sum; exp in double, rounded to float. */
+int or_gauss_weights(uint32_t k, float sigma, float* g) {
+    if (k == 0 || k > 19 || k % 2 == 0) return -1;
+    float sum = 0.0f;
+    for (uint32_t i = 0; i < k; ++i) {
+        float a = (float)i - ((float)k - 1.0f);
+        a = a * a;
+        const float b = 2.0f * sigma * sigma;
+        const float r = (float)exp((double)(-a / b));
+        g[i] = r;
+        sum += r;
+    }
+    sum = 1.0f / sum;
+    for (uint32_t i = 0; i < k; ++i) g[i] *= sum;
+    return 0;
+}
+
+/* denoise_gauss_comp.comp:17-33 on a float RGB image (h rows of w pixels) */
+int or_denoise_gauss(const float* in, float* out, uint32_t w, uint32_t h, uint32_t k, float sigma) {
+    float g[20];
+    if (or_gauss_weights(k, sigma, g)) return -1;
+    const int off = (int)(k / 2);
+    for (uint32_t py = 0; py < h; ++py)
+        for (uint32_t px = 0; px < w; ++px) {
+            float c[3] = {0.0f, 0.0f, 0.0f};
+            for (int x = -off; x <= off; ++x)
+                for (int y = -off; y <= off; ++y) {
+                    const int qx = (int)px + x, qy = (int)py + y;
+                    if (qx < 0 || qx >= (int)w || qy < 0 || qy >= (int)h) continue;
+                    const float* v = in + 3 * ((size_t)qy * w + (size_t)qx);
+                    for (int ch = 0; ch < 3; ++ch) c[ch] = c[ch] + v[ch] * g[x + off] * g[y + off];
+                }
+            for (int ch = 0; ch < 3; ++ch) out[3 * ((size_t)py * w + px) + ch] = c[ch];
+        }
+    return 0;
+}
+
+/* denoise_median_comp.comp:22-46: the shader's exchange network on the 3x3
+ * taps v[(dx + 1) * 3 + (dy + 1)] of each channel, taps outside the image 0 */
+#define OR_S2(a, b) do { const float t_ = (a); (a) = fminf((a), (b)); (b) = fmaxf(t_, (b)); } while (0)
+int or_denoise_median(const float* in, float* out, uint32_t w, uint32_t h) {
+    for (uint32_t py = 0; py < h; ++py)
+        for (uint32_t px = 0; px < w; ++px)
+            for (int ch = 0; ch < 3; ++ch) {
+                float v[9];
+                for (int dx = -1; dx <= 1; ++dx)
+                    for (int dy = -1; dy <= 1; ++dy) {
+                        const int qx = (int)px + dx, qy = (int)py + dy;
+                        const int inside = qx >= 0 && qx < (int)w && qy >= 0 && qy < (int)h;
+                        v[(dx + 1) * 3 + (dy + 1)] = inside ? in[3 * ((size_t)qy * w + (size_t)qx) + ch] : 0.0f;
+                    }
+                /* mnmx6(v0..v5) */
+                OR_S2(v[0], v[3]); OR_S2(v[1], v[4]); OR_S2(v[2], v[5]);
+                OR_S2(v[0], v[1]); OR_S2(v[0], v[2]); OR_S2(v[4], v[5]); OR_S2(v[3], v[5]);
+                /* mnmx5(v1, v2, v3, v4, v6) */
+                OR_S2(v[1], v[2]); OR_S2(v[3], v[4]); OR_S2(v[1], v[3]); OR_S2(v[1], v[6]);
+                OR_S2(v[4], v[6]); OR_S2(v[2], v[6]);
+                /* mnmx4(v2, v3, v4, v7) */
+                OR_S2(v[2], v[3]); OR_S2(v[4], v[7]); OR_S2(v[2], v[4]); OR_S2(v[3], v[7]);
+                /* mnmx3(v3, v4, v8) */
+                OR_S2(v[4], v[8]); OR_S2(v[3], v[8]); OR_S2(v[3], v[4]);
+                out[3 * ((size_t)py * w + px) + ch] = v[4];
+            }
+    return 0;
+}
+

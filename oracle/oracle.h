@@ -135,6 +135,12 @@ int or_render_ddgi(const float* pos, const float* nrm, const uint32_t* idx, cons
                    uint32_t q_vertex, const void* cfg, const float* irr, const float* dep, float* rgb,
                    uint64_t* queries, int threads);
 
+/* Post-filters of the Vulkan renderer on float RGB images: GaussDenoiser
+ * weights and filter, MedianDenoiser's exchange network. */
+int or_gauss_weights(uint32_t k, float sigma, float* g);
+int or_denoise_gauss(const float* in, float* out, uint32_t w, uint32_t h, uint32_t k, float sigma);
+int or_denoise_median(const float* in, float* out, uint32_t w, uint32_t h);
+
 /* BidirectionalPathTracer (graphic/bidirectional_path_tracer.cpp:20-168)
  * for one camera: settings supply spp, seed and sample_offset; Philox
  * counters (pixel, sample, depth, 0) on the vision side and (pixel, sample,
