@@ -46,7 +46,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-halves", action="store_true", help="skip the per-half (coherent / incoherent) timing")
-    ap.add_argument("--leaf", type=int, default=2, help="BVH leaf size (1..16)")
+    ap.add_argument("--leaf", type=int, default=4, help="BVH leaf size (1..16)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 512 MiB device copy (reference kernel for HBM counters)")
     ap.add_argument("--config", default="raycast",
@@ -242,7 +242,7 @@ def raycast(args, sc, rays, n_coh, workload, pmc_path):
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": workload,
-                   "triangles": sc.num_triangles, "rays": n, "bvh": f"BVH4 LBVH leaf<={args.leaf}",
+                   "triangles": sc.num_triangles, "rays": n, "bvh": f"BVH4 LBVH, SAH leaves of <= {args.leaf}",
                    "bvh_build_ms": round(info["build_ms"], 3),
                    "mrays_per_s_coherent": halves["coherent"] and round(halves["coherent"], 1),
                    "mrays_per_s_incoherent": halves["incoherent"] and round(halves["incoherent"], 1),

@@ -121,7 +121,7 @@ typedef struct rt_tri_rec {
 } rt_tri_rec;
 
 typedef struct rt_build_opts {
-    uint32_t leaf_size;   /* max triangles per leaf, 1..16 (0 = default 2) */
+    uint32_t leaf_size;   /* max triangles per leaf, 1..16 (0 = default 4); a subtree this small becomes a leaf when its surface-area cost says so */
     uint32_t flags;       /* reserved, 0 */
 } rt_build_opts;
 

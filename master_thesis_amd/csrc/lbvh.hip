@@ -11,8 +11,9 @@
 //   4. node boxes as unions of their sorted leaf ranges: block unions, a
 //      sparse table over blocks, one range query per node (k_node_boxes)
 //   5. top-down collapse to 4-wide nodes: a node opens its largest-area inner
-//      child until it has four children; subtrees of <= leaf_size triangles
-//      become leaves over a contiguous range of the sorted order (k_collapse)
+//      child until it has four children; a subtree of <= leaf_size triangles
+//      becomes a leaf over a contiguous range of the sorted order when its
+//      surface-area cost is no more than splitting it once (k_collapse)
 //   6. leaf-ordered triangle records v0 / edge0 / edge1 / id (k_tri_records)
 // The topology does not affect results: the traversal's answer is defined by
 // the closest-hit rule alone (see trace.hip).  Boxes are padded outward so the
@@ -309,11 +310,28 @@ __global__ void k_node_boxes(uint32_t ninner, Inner in, const float* __restrict_
 
 struct Entry {
     uint32_t ref, first, count;
+    bool leaf;
     float b[6];
 };
 
+__device__ __forceinline__ float half_area(const float b[6]) {
+    float x = b[3] - b[0], y = b[4] - b[1], z = b[5] - b[2];
+    return x * y + y * z + z * x;
+}
+
+__device__ __forceinline__ uint32_t ref_count(uint32_t ref, const Inner& in) {
+    return (ref & RT_LEAF_BIT) ? 1u : in.last[ref] - in.first[ref] + 1u;
+}
+
+// A subtree becomes a leaf when it has at most leaf_size triangles and the
+// surface-area cost of testing them all is no more than that of one more
+// node step plus its two halves as leaves:
+//   count * A  <=  alpha * A + count_l * A_l + count_r * A_r
+// (alpha = cost of a node step over a triangle step; a large alpha makes
+// every small subtree a leaf).  Triangles that overlap little and sit apart
+// (a soup) split; the two triangles of a heightfield quad stay together.
 __device__ __forceinline__ void make_entry(uint32_t ref, const Inner& in, const uint32_t* vals, const float* leafbox,
-                                           const float* innerbox, Entry& e) {
+                                           const float* innerbox, uint32_t leaf_size, float alpha, Entry& e) {
     e.ref = ref;
     if (ref & RT_LEAF_BIT) {
         e.first = ref & ~RT_LEAF_BIT;
@@ -323,38 +341,46 @@ __device__ __forceinline__ void make_entry(uint32_t ref, const Inner& in, const 
         e.count = in.last[ref] - e.first + 1;
     }
     load_box(ref, vals, leafbox, innerbox, e.b);
-}
-
-__device__ __forceinline__ float half_area(const float b[6]) {
-    float x = b[3] - b[0], y = b[4] - b[1], z = b[5] - b[2];
-    return x * y + y * z + z * x;
+    if (ref & RT_LEAF_BIT) {
+        e.leaf = true;
+    } else if (e.count > leaf_size) {
+        e.leaf = false;
+    } else {
+        float bl[6], br[6];
+        const uint32_t l = in.left[ref], r = in.right[ref];
+        load_box(l, vals, leafbox, innerbox, bl);
+        load_box(r, vals, leafbox, innerbox, br);
+        const float a = half_area(e.b);
+        const float split = alpha * a + (float)ref_count(l, in) * half_area(bl) + (float)ref_count(r, in) * half_area(br);
+        e.leaf = (float)e.count * a <= split;
+    }
 }
 
 // One item per thread: (binary inner node, 4-wide slot, parent slot).
 __global__ void k_collapse(const uint4* __restrict__ qin, uint32_t nq, uint4* __restrict__ qout, Inner in,
                            const uint32_t* __restrict__ vals, const float* leafbox, const float* innerbox,
-                           uint32_t leaf_size, float pad, rt_bvh4_node* __restrict__ nodes, uint32_t max_nodes,
-                           uint32_t max_queue, uint32_t* counters) {
+                           uint32_t leaf_size, float alpha, float pad, rt_bvh4_node* __restrict__ nodes,
+                           uint32_t max_nodes, uint32_t max_queue, uint32_t* counters) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nq) return;
     uint4 it = qin[k];
     Entry e[4];
     int cnt = 2;
-    make_entry(in.left[it.x], in, vals, leafbox, innerbox, e[0]);
-    make_entry(in.right[it.x], in, vals, leafbox, innerbox, e[1]);
+    make_entry(in.left[it.x], in, vals, leafbox, innerbox, leaf_size, alpha, e[0]);
+    make_entry(in.right[it.x], in, vals, leafbox, innerbox, leaf_size, alpha, e[1]);
     while (cnt < 4) {
         int best = -1;
         float best_a = -1.0f;
         for (int c = 0; c < cnt; ++c) {
-            if (!(e[c].ref & RT_LEAF_BIT) && e[c].count > leaf_size) {
+            if (!e[c].leaf) {
                 float a = half_area(e[c].b);
                 if (a > best_a) { best_a = a; best = c; }
             }
         }
         if (best < 0) break;
         uint32_t r = e[best].ref;
-        make_entry(in.left[r], in, vals, leafbox, innerbox, e[best]);
-        make_entry(in.right[r], in, vals, leafbox, innerbox, e[cnt]);
+        make_entry(in.left[r], in, vals, leafbox, innerbox, leaf_size, alpha, e[best]);
+        make_entry(in.right[r], in, vals, leafbox, innerbox, leaf_size, alpha, e[cnt]);
         ++cnt;
     }
     rt_bvh4_node nd;
@@ -364,7 +390,7 @@ __global__ void k_collapse(const uint4* __restrict__ qin, uint32_t nq, uint4* __
             nd.lo_x[c] = e[c].b[0] - pad; nd.hi_x[c] = e[c].b[3] + pad;
             nd.lo_y[c] = e[c].b[1] - pad; nd.hi_y[c] = e[c].b[4] + pad;
             nd.lo_z[c] = e[c].b[2] - pad; nd.hi_z[c] = e[c].b[5] + pad;
-            if (e[c].count <= leaf_size) {
+            if (e[c].leaf) {
                 nd.child[c] = RT_LEAF_BIT | ((e[c].count - 1u) << RT_LEAF_COUNT_SHIFT) | e[c].first;
                 ++leaves;
             } else {
@@ -530,7 +556,8 @@ int build_lbvh(rt_ctx* c, uint32_t leaf_size) {
             RT_HIP(c, hipMemsetAsync(counters + CTR_QOUT, 0, 4, st));
             hipLaunchKernelGGL(k_collapse, dim3(blocks_for(nq, 128)), dim3(128), 0, st, (const uint4*)c->queue[q].ptr, nq,
                                (uint4*)c->queue[q ^ 1].ptr, in, (const uint32_t*)vals[cur], (const float*)leafbox,
-                               (const float*)innerbox, leaf_size, pad, nodes, max_nodes, max_queue, counters);
+                               (const float*)innerbox, leaf_size, c->bvh_leaf_alpha, pad, nodes, max_nodes, max_queue,
+                               counters);
             uint32_t h[4];
             RT_HIP(c, hipMemcpyAsync(h, counters, sizeof(h), hipMemcpyDeviceToHost, st));
             RT_HIP(c, hipStreamSynchronize(st));

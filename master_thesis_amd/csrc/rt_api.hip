@@ -118,6 +118,10 @@ int rt_create(rt_ctx** out, int device) {
         long v = strtol(r, nullptr, 10);
         c->trace_refill_min = (uint32_t)(v < 1 ? 1 : (v > 64 ? 64 : v));
     }
+    if (const char* r = getenv("MTRT_BVH_ALPHA")) {  // tuning knob, >= 0
+        const float v = strtof(r, nullptr);
+        c->bvh_leaf_alpha = v >= 0.0f ? v : 0.0f;
+    }
     if (const char* r = getenv("MTRT_TRACE_SLICE")) {  // tuning knob, rounded to 128..2^24
         long v = strtol(r, nullptr, 10);
         v = v < 128 ? 128 : (v > (1l << 24) ? (1l << 24) : v);
@@ -303,7 +307,7 @@ int rt_set_material_colors(rt_ctx* c, const float* rgb, int ptr_kind) {
 int rt_build(rt_ctx* c, const rt_build_opts* o, rt_bvh_info* info) {
     if (!c) return RT_E_INVALID;
     if (c->T == 0) return fail(c, RT_E_STATE, "rt_build: no scene uploaded");
-    uint32_t leaf = (o && o->leaf_size) ? o->leaf_size : 2u;
+    uint32_t leaf = (o && o->leaf_size) ? o->leaf_size : 4u;
     if (leaf > RT_MAX_LEAF_SIZE) return fail(c, RT_E_INVALID, "rt_build: leaf_size above 16");
     DeviceGuard g(c->device);
     int rc = build_lbvh(c, leaf);

@@ -72,6 +72,7 @@ struct rt_ctx {
     rt::DevBuf rays, hits, spill, work, stats;
     uint32_t trace_refill_min = 8;  // traversal: idle lanes that trigger a ray refill (MTRT_TRACE_REFILL)
     uint32_t trace_slice = 16384;   // traversal: rays per XCD slice, a multiple of 128 (MTRT_TRACE_SLICE)
+    float bvh_leaf_alpha = 1.0f;    // build: node-step / triangle-step cost of the leaf test (MTRT_BVH_ALPHA)
     // path tracer state
     rt::DevBuf pt_state, pt_queue[2], pt_image, pt_img8, pt_dimg, pt_dcolor, pt_ctr, pt_partial, pt_cams;
     rt::DevBuf pt_nrc_query, pt_nrc_train;  // cache queries (features, weights, pixel) and training records

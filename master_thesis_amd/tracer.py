@@ -69,7 +69,7 @@ class Tracer:
             rgb = np.ascontiguousarray(rgb, np.float32)
         self._check(self.lib.rt_set_material_colors(self.h, L._ptr(rgb), kind))
 
-    def build(self, leaf_size: int = 2) -> dict:
+    def build(self, leaf_size: int = 4) -> dict:
         opts = L.rt_build_opts(leaf_size, 0)
         info = L.rt_bvh_info()
         self._check(self.lib.rt_build(self.h, C.byref(opts), C.byref(info)))
