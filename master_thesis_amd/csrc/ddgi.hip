@@ -285,12 +285,12 @@ int rt_ddgi_create(rt_ctx* c, const rt_ddgi_config* cfg, rt_ddgi** out) {
     d.hysteresis = k.hysteresis;
     d.flags = k.flags;
     (void)hipSetDevice(c->device);
+    c->probe_fields.push_back(g);  // from here on rt_ddgi_destroy releases whatever was allocated
     const size_t texels = (size_t)W * H;
     int rc;
     if ((rc = ensure(c, g->irr, 16 * texels)) || (rc = ensure(c, g->dep, 8 * texels)) ||
         (rc = ensure(c, g->surf, 32 * (size_t)P * g->R))) {
-        g->ctx = nullptr;
-        delete g;
+        rt_ddgi_destroy(g);
         return rc;
     }
     hipError_t e = hipMemsetAsync(g->irr.ptr, 0, 16 * texels, c->stream);
@@ -298,11 +298,9 @@ int rt_ddgi_create(rt_ctx* c, const rt_ddgi_config* cfg, rt_ddgi** out) {
     if (e == hipSuccess) e = hipMemsetAsync(g->surf.ptr, 0, 32 * (size_t)P * g->R, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
-        c->probe_fields.push_back(g);
         rt_ddgi_destroy(g);
         return hip_fail(c, e, "rt_ddgi_create");
     }
-    c->probe_fields.push_back(g);
     *out = g;
     return RT_OK;
 }
