@@ -18,8 +18,9 @@
 //                samples (64 per wave): encode, 6 layers
 //   k_mlp_train  4 waves, tiles of 64 samples, at most one workgroup per CU
 //                striding over the tiles: encode, forward (all activations
-//                kept in LDS), loss, backward; each workgroup accumulates
-//                its own partial gradient (no atomics)
+//                kept in LDS), loss, backward; each workgroup sums its own
+//                partial gradient over its tiles in registers and writes it
+//                once (no atomics, no read-modify-write of global memory)
 //   k_adam       sums the partials in workgroup order (deterministic), Adam
 //                update, BF16 refresh
 // Every GEMM is a sum of 16x16 output tiles of v_mfma_f32_16x16x32_bf16.
@@ -52,6 +53,10 @@ constexpr int NPARAM = offw(L);
 // bf16 weight layout: [W0 W1 ... W5]
 constexpr int offwb(int l) { int o = 0; for (int i = 0; i < l; ++i) o += ND[i] * KD[i]; return o; }
 constexpr int NWB = offwb(L);
+// training: 16 x 16 weight-gradient tiles a wave accumulates per layer (K / 16
+// for the hidden layers, one for the 16-row output layer) and their offsets
+constexpr int GW_OFF[L] = {0, 6, 10, 14, 18, 22};
+constexpr int GW_TILES = 23;
 static_assert(NPARAM == 23888, "parameter count");
 static_assert(NWB == 23552, "bf16 weight count");
 
@@ -333,9 +338,19 @@ __global__ void __launch_bounds__(NT) k_mlp_train(const float* __restrict__ feat
     // the sum over its tiles in that order (each element is always written by
     // the same thread, so the read-modify-write needs no synchronisation)
     float lacc = 0.0f;
+    // this thread's share of the workgroup's gradient, summed over its tiles
+    // in tile order in registers and written once at the end: the weight
+    // tiles of every layer (wave w owns rows 16 w .. 16 w + 15 of each dW,
+    // GW_OFF[l] + kt indexes layer l's column tile kt) and, for threads < 64,
+    // one bias element per layer
+    f32x4 gw[GW_TILES];
+#pragma unroll
+    for (int i = 0; i < GW_TILES; ++i) gw[i] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    float gb[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) gb[l] = 0.0f;
     const uint32_t tiles = (n + TILE - 1) / TILE;
     for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-    const bool first = tile == blockIdx.x;
     const uint32_t s0 = tile * TILE;
     __syncthreads();  // the previous tile's backward pass is done with sX / sH / sD
     encode_tile(feats, s0, n, G, sX);
@@ -371,18 +386,14 @@ __global__ void __launch_bounds__(NT) k_mlp_train(const float* __restrict__ feat
 
     // layer 5: dW5[n][k] = sum_s dY[s][n] H4[s][k] (16 x 64), one 16-column tile per wave
     {
-        f32x4 acc = tile_mm(sD[0], 1, 16, sH[L - 2], 64, 1, 0, 16 * w, 64, 64, lane);
+        const f32x4 acc = tile_mm(sD[0], 1, 16, sH[L - 2], 64, 1, 0, 16 * w, 64, 64, lane);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            float* q = part + offw(L - 1) + (4 * kg + i) * 64 + 16 * w + mi;
-            *q = first ? acc[i] : *q + acc[i];
-        }
+        for (int i = 0; i < 4; ++i) gw[GW_OFF[L - 1]][i] += acc[i];
     }
     if (threadIdx.x < 16) {
         float s = 0.0f;
         for (int r = 0; r < TILE; ++r) s += (float)sD[0][r * 16 + threadIdx.x];
-        float* q = part + offb(L - 1) + threadIdx.x;
-        *q = first ? s : *q + s;
+        gb[L - 1] += s;
     }
     // dH4[s][k] = sum_n dY[s][n] W5[n][k] (K = 16 valid of 32), masked by H4 > 0 -> sD[1]
     for (int nt = 0; nt < 4; ++nt) {
@@ -398,24 +409,22 @@ __global__ void __launch_bounds__(NT) k_mlp_train(const float* __restrict__ feat
     __syncthreads();
 
     int cur = 1;
+#pragma unroll
     for (int l = L - 2; l >= 0; --l) {
         const __bf16* dZ = sD[cur];
         const __bf16* in = (l == 0) ? sX : sH[l - 1];
         const int K = KD[l];
         // dW_l[n][k] = sum_s dZ[s][n] in[s][k]: rows n in 16-row tiles (one per wave), K / 16 column tiles
-        for (int kt = 0; kt < K / 16; ++kt) {
-            f32x4 acc = tile_mm(dZ, 1, 64, in, K, 1, 16 * w, 16 * kt, 64, 64, lane);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                float* q = part + offw(l) + (16 * w + 4 * kg + i) * K + 16 * kt + mi;
-                *q = first ? acc[i] : *q + acc[i];
-            }
+        for (int kt = 0; kt < K / 16; ++kt) {
+            const f32x4 acc = tile_mm(dZ, 1, 64, in, K, 1, 16 * w, 16 * kt, 64, 64, lane);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) gw[GW_OFF[l] + kt][i] += acc[i];
         }
         if (threadIdx.x < 64) {
             float s = 0.0f;
             for (int r = 0; r < TILE; ++r) s += (float)dZ[r * 64 + threadIdx.x];
-            float* q = part + offb(l) + threadIdx.x;
-            *q = first ? s : *q + s;
+            gb[l] += s;
         }
         if (l > 0) {
             // dH_{l-1}[s][k] = sum_n dZ[s][n] W_l[n][k], masked by H_{l-1} > 0
@@ -435,6 +444,24 @@ __global__ void __launch_bounds__(NT) k_mlp_train(const float* __restrict__ feat
         }
     }
     }  // tiles
+    // the workgroup's partial gradient, written once
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        const int K = KD[l];
+        const int nkt = (l == L - 1) ? 1 : K / 16;
+#pragma unroll
+        for (int kt = 0; kt < nkt; ++kt) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                // layer 5 (16 x 64): wave w holds columns 16 w .. 16 w + 15 of all 16 rows
+                const int row = (l == L - 1) ? 4 * kg + i : 16 * w + 4 * kg + i;
+                const int col = (l == L - 1) ? 16 * w + mi : 16 * kt + mi;
+                part[offw(l) + row * K + col] = gw[GW_OFF[l] + kt][i];
+            }
+        }
+        const int nb = (l == L - 1) ? 16 : 64;
+        if ((int)threadIdx.x < nb) part[offb(l) + threadIdx.x] = gb[l];
+    }
     if (threadIdx.x == 0) loss_part[blockIdx.x] = lacc;
 }
 

@@ -97,12 +97,16 @@ def test_scene_files_flatten_like_reference(scene):
 def test_path_tracer_statistically_matches_reference():
     """The reference's own PathTracer (mt19937 stream) and the oracle
     (Philox stream) estimate the same image: block means agree within the
-    spread the oracle shows across seeds."""
+    spread the oracle shows across seeds.  The reference seeds its streams
+    afresh on every call, so its image differs run to run; the spread is taken
+    over 16 oracle seeds, which keeps the t statistic's tails (15 degrees of
+    freedom) far below the threshold for an unbiased estimator while a dropped
+    term or a wrong weight moves block means by tens of sigma."""
     sc = scenes.cornell_box()
     cam = H.camera(scenes.cornell_camera_position(), 1.0, 0.0, np.float32(math.pi / 4), 32, 32)
     spp, depth = 256, 5
     ref = R.RefScene(sc).render(cam, spp, depth, threads=8)
-    runs = np.stack([O.render(sc, cam, make_settings(spp=spp, max_depth=depth, seed=s))[0] for s in range(1, 7)])
+    runs = np.stack([O.render(sc, cam, make_settings(spp=spp, max_depth=depth, seed=s))[0] for s in range(1, 17)])
 
     def blocks(img):
         return img.reshape(4, 8, 4, 8, 3).mean(axis=(1, 3))
@@ -111,7 +115,7 @@ def test_path_tracer_statistically_matches_reference():
     ob = np.stack([blocks(x) for x in runs])
     m, s = ob.mean(axis=0), ob.std(axis=0, ddof=1)
     z = (rb - m) / np.maximum(s * math.sqrt(1 + 1 / len(runs)), 1e-6)
-    assert np.all(np.abs(z) < 6), np.abs(z).max()
+    assert np.all(np.abs(z) < 7), np.abs(z).max()
     zg = (ref.mean() - runs.mean()) / (runs.mean(axis=(1, 2, 3)).std(ddof=1) * math.sqrt(1 + 1 / len(runs)))
     assert abs(zg) < 5, zg
 
@@ -140,7 +144,7 @@ def test_bidirectional_tracer_statistically_matches_reference():
     cam = H.camera(scenes.cornell_camera_position(), 1.0, 0.0, np.float32(math.pi / 4), 32, 32)
     spp = 64
     ref = R.RefScene(sc).render_bdpt(cam, spp, 5, 5, 5, threads=8)
-    runs = np.stack([O.render_bdpt(sc, cam, make_settings(spp=spp, max_depth=5, seed=s)) for s in range(1, 7)])
+    runs = np.stack([O.render_bdpt(sc, cam, make_settings(spp=spp, max_depth=5, seed=s)) for s in range(1, 17)])
 
     def blocks(img):
         return img.reshape(4, 8, 4, 8, 3).mean(axis=(1, 3))
@@ -149,6 +153,6 @@ def test_bidirectional_tracer_statistically_matches_reference():
     ob = np.stack([blocks(x) for x in runs])
     m, s = ob.mean(axis=0), ob.std(axis=0, ddof=1)
     z = (rb - m) / np.maximum(s * math.sqrt(1 + 1 / len(runs)), 1e-6)
-    assert np.all(np.abs(z) < 6), np.abs(z).max()
+    assert np.all(np.abs(z) < 7), np.abs(z).max()
     zg = (ref.mean() - runs.mean()) / (runs.mean(axis=(1, 2, 3)).std(ddof=1) * math.sqrt(1 + 1 / len(runs)))
     assert abs(zg) < 5, zg
