@@ -24,6 +24,10 @@
 //           global counter (the analogue of pixelCounter.fetch_add,
 //           graphics_engine.cpp:74); surviving slots are compacted into the
 //           next queue with one atomic per wave (ballot + popcount).
+// The primary ray of a pixel is fixed and the culling steps after it draw no
+// random numbers, so only sample 0 of an item traces that prefix: the slot
+// caches its first accepted vertex (or the escape, or the depth running out)
+// and later samples of the item start there (plain and adjoint modes).
 // Random numbers: Philox-4x32-10 with counter (pixel, sample, depth, view)
 // and the 64-bit seed as key, so a path does not depend on scheduling and the
 // adjoint pass replays it exactly.  Several views (rt_render_views) share
@@ -99,6 +103,7 @@ struct PTArgs {
     uint32_t* s_amat;    // adjoint: ADJ_MAX_DEPTH * S material ids
     float* s_acos;       // adjoint: ADJ_MAX_DEPTH * S cosines
     uint32_t* s_nv;      // adjoint: diffuse vertices so far
+    float* s_fv;         // plain and adjoint: 11 * S first-vertex cache of the slot's item
     // queues
     uint32_t* q_slot[2];
     rt_ray* q_ray[2];
@@ -255,6 +260,71 @@ __device__ __forceinline__ void scatter_grad(const PTArgs& a, uint32_t slot, uin
     }
 }
 
+// First-vertex cache of a work item (plain and adjoint modes).  A pixel's
+// primary ray is fixed and the culling steps before the first accepted hit
+// draw no random numbers, so every sample of an item reaches the same first
+// vertex along the same direction; only sample 0 traces that prefix and the
+// slot keeps where it ended.  Later samples start at the cached vertex and
+// draw their own numbers there, which gives the same paths, the same values
+// and the same summation order as tracing the prefix again.
+// s_fv layout, one array of S words each: 0-2 p, 3-5 shading normal,
+// 6-8 direction, 9 material, 10 depth | kind << 16.
+constexpr uint32_t FV_HIT = 1;   // accepted vertex at `depth`
+constexpr uint32_t FV_MISS = 2;  // escaped before any accepted vertex
+constexpr uint32_t FV_END = 3;   // depth ran out in culling steps
+
+__device__ __forceinline__ void fv_put(const PTArgs& a, uint32_t slot, uint32_t kind, uint32_t depth, f3 p, f3 n,
+                                       f3 d, uint32_t mid) {
+    const size_t S = a.S;
+    float* f = a.s_fv;
+    if (kind == FV_HIT) {
+        f[0 * S + slot] = p.x; f[1 * S + slot] = p.y; f[2 * S + slot] = p.z;
+        f[3 * S + slot] = n.x; f[4 * S + slot] = n.y; f[5 * S + slot] = n.z;
+        ((uint32_t*)f)[9 * S + slot] = mid;
+    }
+    f[6 * S + slot] = d.x; f[7 * S + slot] = d.y; f[8 * S + slot] = d.z;
+    ((uint32_t*)f)[10 * S + slot] = depth | (kind << 16);
+}
+
+// The accepted-vertex step of the estimator at (p, nrm, material mid) reached
+// along d: pick the branch, bounce, weight a diffuse bounce, end on a light
+// seen from the front, then count the depth slot.
+template <bool ADJ>
+__device__ __forceinline__ void vertex_step(const PTArgs& a, uint32_t slot, uint32_t pix, uint32_t view,
+                                            uint32_t gsmp, uint32_t& depth, f3 p, f3 nrm, uint32_t mid, f3& o,
+                                            f3& d, f3& T, bool& end, bool& contrib) {
+    const float ndotd = dot(nrm, d);
+    const rt_material m = a.mat[mid];
+    const bool light_end = m.light_source && ndotd <= 0.0f;
+    u4 c4;
+    c4.x = pix; c4.y = gsmp; c4.z = depth; c4.w = view;
+    const u4 rnd = philox4x32_10(c4, a.seed_lo, a.seed_hi);
+    const float xi0 = u01(rnd.x);
+    const bool diffuse = xi0 <= m.diffuse_threshold;
+    if (diffuse) d = hemisphere_dir(u01(rnd.y), u01(rnd.z), nrm);
+    else if (xi0 <= m.reflect_threshold) d = reflect_ref(d, nrm);
+    else if (xi0 <= m.transparent_threshold) d = custom_refract_ref(d, nrm, m.refraction_index);
+    o = add(p, scale(0.01f, d));
+    if (diffuse) {
+        const float cs = dot(nrm, d);
+        T = mulv(T, scale(cs, mk(m.color[0], m.color[1], m.color[2])));
+        if (ADJ) {
+            uint32_t nv = a.s_nv[slot];
+            if (nv < ADJ_MAX_DEPTH) {
+                a.s_amat[nv * a.S + slot] = mid;
+                a.s_acos[nv * a.S + slot] = cs;
+            }
+            a.s_nv[slot] = nv + 1;
+        }
+    }
+    if (light_end) {
+        end = true;
+        contrib = true;
+    }
+    ++depth;
+    if (depth >= a.max_depth) end = true;
+}
+
 template <bool ADJ>
 __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
     const uint32_t n = a.ctr[cur ? C_N1 : C_N0];
@@ -280,8 +350,10 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
         bool cull = (st & CULL_BIT) != 0;
         f3 T = mk(a.s_T[3 * slot], a.s_T[3 * slot + 1], a.s_T[3 * slot + 2]);
         f3 o = mk(r.ox, r.oy, r.oz), d = mk(r.dx, r.dy, r.dz);
-        bool end = false, contrib = false;
+        bool end = false, contrib = false, at_vertex = false;
         f3 term = mk(1.0f, 1.0f, 1.0f);
+        f3 p = o, nrm = o;
+        uint32_t mid = 0;
         if (h.tri == RT_MISS) {
             end = true;
             if (a.flags & (RT_RENDER_SKY | RT_RENDER_SUN)) {
@@ -289,70 +361,115 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
                 term = miss_radiance(a, d, true);
                 T = mulv(T, term);
             }
+            if (cull) fv_put(a, slot, FV_MISS, depth, o, o, d, 0);
         } else {
-            f3 p, nrm;
-            uint32_t mid;
             surface(a, h, o, d, p, nrm, mid);
-            const float ndotd = dot(nrm, d);
-            if (cull && ndotd > 0.0f) {
+            if (cull && dot(nrm, d) > 0.0f) {
                 o = add(p, scale(0.01f, d));
-            } else {
-                cull = false;
-                const rt_material m = a.mat[mid];
-                const uint32_t gsmp = a.sample_offset + chunk * a.spc + smp;
-                const bool light_end = m.light_source && ndotd <= 0.0f;
-                u4 c4;
-                c4.x = pix; c4.y = gsmp; c4.z = depth; c4.w = view;
-                const u4 rnd = philox4x32_10(c4, a.seed_lo, a.seed_hi);
-                const float xi0 = u01(rnd.x);
-                const bool diffuse = xi0 <= m.diffuse_threshold;
-                if (diffuse) d = hemisphere_dir(u01(rnd.y), u01(rnd.z), nrm);
-                else if (xi0 <= m.reflect_threshold) d = reflect_ref(d, nrm);
-                else if (xi0 <= m.transparent_threshold) d = custom_refract_ref(d, nrm, m.refraction_index);
-                o = add(p, scale(0.01f, d));
-                if (diffuse) {
-                    const float cs = dot(nrm, d);
-                    T = mulv(T, scale(cs, mk(m.color[0], m.color[1], m.color[2])));
-                    if (ADJ) {
-                        uint32_t nv = a.s_nv[slot];
-                        if (nv < ADJ_MAX_DEPTH) {
-                            a.s_amat[nv * a.S + slot] = mid;
-                            a.s_acos[nv * a.S + slot] = cs;
-                        }
-                        a.s_nv[slot] = nv + 1;
-                    }
-                }
-                if (light_end) {
+                ++depth;
+                if (depth >= a.max_depth) {
                     end = true;
-                    contrib = true;
+                    fv_put(a, slot, FV_END, depth, p, nrm, d, mid);
                 }
+            } else {
+                if (cull) fv_put(a, slot, FV_HIT, depth, p, nrm, d, mid);
+                cull = false;
+                at_vertex = true;
             }
-            ++depth;
-            if (depth >= a.max_depth) end = true;
         }
+        if (at_vertex)
+            vertex_step<ADJ>(a, slot, pix, view, a.sample_offset + chunk * a.spc + smp, depth, p, nrm, mid, o, d, T,
+                             end, contrib);
         if (end) {
+            // the sample ended: add it, then start the next one at the item's
+            // cached first vertex (at most one such step per call; a sample
+            // that ends there too is added and the next traces its primary ray)
+            float sum[3] = {0.0f, 0.0f, 0.0f};
+            if (!ADJ) {
+                sum[0] = a.s_sum[3 * slot];
+                sum[1] = a.s_sum[3 * slot + 1];
+                sum[2] = a.s_sum[3 * slot + 2];
+            }
             if (contrib) {
                 if (ADJ) {
                     scatter_grad(a, slot, vp, term);
                 } else {
-                    a.s_sum[3 * slot] = a.s_sum[3 * slot] + T.x;
-                    a.s_sum[3 * slot + 1] = a.s_sum[3 * slot + 1] + T.y;
-                    a.s_sum[3 * slot + 2] = a.s_sum[3 * slot + 2] + T.z;
+                    sum[0] = sum[0] + T.x;
+                    sum[1] = sum[1] + T.y;
+                    sum[2] = sum[2] + T.z;
                 }
             }
             ++smp;
-            if (smp < a.spc) {
-                a.s_smp[slot] = smp;
-                start_sample(a, slot, vp, nr, ADJ);
-                alive = true;
-            } else {
+            bool item_done = smp >= a.spc;
+            if (!item_done) {
+                const size_t S = a.S;
+                const uint32_t w10 = ((const uint32_t*)a.s_fv)[10 * S + slot];
+                const uint32_t kind = w10 >> 16;
+                const bool sky = (a.flags & (RT_RENDER_SKY | RT_RENDER_SUN)) != 0;
+                d = mk(a.s_fv[6 * S + slot], a.s_fv[7 * S + slot], a.s_fv[8 * S + slot]);
+                if (kind == FV_END || (kind == FV_MISS && !sky)) {
+                    item_done = true;  // every remaining sample contributes nothing
+                } else if (kind == FV_MISS) {
+                    // every remaining sample escapes along the primary ray: 1 times the miss radiance
+                    const f3 t0 = miss_radiance(a, d, true);
+                    if (!ADJ)
+                        for (; smp < a.spc; ++smp) {
+                            sum[0] = sum[0] + t0.x;
+                            sum[1] = sum[1] + t0.y;
+                            sum[2] = sum[2] + t0.z;
+                        }
+                    item_done = true;
+                } else {
+                    p = mk(a.s_fv[0 * S + slot], a.s_fv[1 * S + slot], a.s_fv[2 * S + slot]);
+                    nrm = mk(a.s_fv[3 * S + slot], a.s_fv[4 * S + slot], a.s_fv[5 * S + slot]);
+                    mid = ((const uint32_t*)a.s_fv)[9 * S + slot];
+                    depth = w10 & 0xFFFFu;
+                    T = mk(1.0f, 1.0f, 1.0f);
+                    end = false;
+                    contrib = false;
+                    if (ADJ) a.s_nv[slot] = 0;
+                    vertex_step<ADJ>(a, slot, pix, view, a.sample_offset + chunk * a.spc + smp, depth, p, nrm, mid,
+                                     o, d, T, end, contrib);
+                    if (end) {
+                        if (contrib) {
+                            if (ADJ) {
+                                scatter_grad(a, slot, vp, mk(1.0f, 1.0f, 1.0f));
+                            } else {
+                                sum[0] = sum[0] + T.x;
+                                sum[1] = sum[1] + T.y;
+                                sum[2] = sum[2] + T.z;
+                            }
+                        }
+                        ++smp;
+                        if (smp >= a.spc) {
+                            item_done = true;
+                        } else {
+                            start_sample(a, slot, vp, nr, ADJ);
+                            alive = true;
+                        }
+                    } else {
+                        a.s_state[slot] = depth;
+                        a.s_T[3 * slot] = T.x; a.s_T[3 * slot + 1] = T.y; a.s_T[3 * slot + 2] = T.z;
+                        nr = make_ray(o, d);
+                        alive = true;
+                    }
+                }
+            }
+            if (item_done) {
                 if (!ADJ) {
                     const size_t q = (size_t)chunk * a.VP + vp;
-                    a.partial[3 * q] = a.s_sum[3 * slot];
-                    a.partial[3 * q + 1] = a.s_sum[3 * slot + 1];
-                    a.partial[3 * q + 2] = a.s_sum[3 * slot + 2];
+                    a.partial[3 * q] = sum[0];
+                    a.partial[3 * q + 1] = sum[1];
+                    a.partial[3 * q + 2] = sum[2];
                 }
                 need_item = true;
+            } else {
+                if (!ADJ) {
+                    a.s_sum[3 * slot] = sum[0];
+                    a.s_sum[3 * slot + 1] = sum[1];
+                    a.s_sum[3 * slot + 2] = sum[2];
+                }
+                a.s_smp[slot] = smp;
             }
         } else {
             a.s_state[slot] = depth | (cull ? CULL_BIT : 0u);
@@ -872,7 +989,7 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
                 (rc = ensure(c, c->pt_shadow, sh_bytes))))
         return rc;
     if ((rc = ensure(c, c->pt_state, szS * (4 * 3 + 12 * 2) + (adj ? szS * (8 * ADJ_MAX_DEPTH + 4) : 0) +
-                                         (nrc ? szS * 16 : 0))) ||
+                                         (nrc ? szS * 16 : szS * 44))) ||
         (rc = ensure(c, c->pt_queue[0], 4 * szS + 16 + sizeof(rt_ray) * szS)) ||
         (rc = ensure(c, c->pt_queue[1], 4 * szS + 16 + sizeof(rt_ray) * szS)) ||
         (rc = ensure(c, c->hits, szS * sizeof(rt_hit))) || (rc = ensure(c, c->pt_ctr, 64)) ||
@@ -907,6 +1024,7 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
     a.s_nv = adj ? (uint32_t*)st : nullptr; st += adj ? 4 * szS : 0;
     a.s_ridx = nrc ? (uint32_t*)st : nullptr; st += nrc ? 4 * szS : 0;
     a.s_Tq = nrc ? (float*)st : nullptr;
+    a.s_fv = nrc ? nullptr : (float*)st;
     a.nee = nrc && !(nf->opt.flags & RT_NRC_NO_NEE) && (c->num_lights > 0 || ((s->flags & RT_RENDER_SUN) && s->sun_cos < 1.0f));
     a.sun_on = (s->flags & RT_RENDER_SUN) && s->sun_cos < 1.0f;
     a.lights = (const LightTri*)c->lights.ptr;

@@ -96,6 +96,29 @@ def test_sky_extension_matches_oracle(tracer):
     assert np.array_equal(rgb.view(np.uint32), want.view(np.uint32))
 
 
+@pytest.mark.parametrize("depth,flags", [(1, RT_RENDER_SKY), (2, RT_RENDER_SKY), (5, RT_RENDER_SKY), (4, 0)])
+def test_first_vertex_cache_outside_closed_room(tracer, depth, flags):
+    """Camera outside a closed room (inward normals, light walls, a diffuse
+    floor): primary rays hit walls from behind and step through them,
+    consuming depth slots (with depth 1 the depth runs out there), or escape
+    past the room; seen from inside, a light wall ends the sample at its first
+    vertex.  Later samples start at the item's cached first vertex; the image
+    equals the oracle's, which traces every sample from the camera, bit for
+    bit, and the throughput mode agrees within summation error."""
+    E, rho = 1.5, 0.6
+    sc = scenes.closed_room(scenes.material((rho, rho, rho)), scenes.material((E, E, E), light=True))
+    _setup(tracer, sc)
+    cam = H.camera((3.0, 0.3, 0.2), 1.0, 0.05, math.pi / 2, 32, 24)
+    s = make_settings(spp=6, max_depth=depth, seed=31, flags=RT_RENDER_DETERMINISTIC | flags, sky=(0.3, 0.4, 0.5))
+    rgb, _ = tracer.render(cam, s)
+    want, _ = O.render(sc, cam, s)
+    assert np.array_equal(rgb.view(np.uint32), want.view(np.uint32))
+    if depth > 1:
+        assert want.max() > 0.5
+    fast, _ = tracer.render(cam, make_settings(spp=6, max_depth=depth, seed=31, flags=flags, sky=(0.3, 0.4, 0.5)))
+    assert np.allclose(fast, want, rtol=2e-5, atol=1e-5)
+
+
 def test_render_argument_errors(tracer):
     from master_thesis_amd._lib import RTError
     _setup(tracer, scenes.cornell_box())
