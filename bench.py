@@ -395,21 +395,25 @@ def bench_terrain(args):
         import oracle as O
         from master_thesis_amd import host_lib as H  # noqa: F401
         threads = min(16, os.cpu_count() or 1)
-        crop = scenes.terrain_cameras(1, 240, 135)
-        s1 = make_settings(spp=1, max_depth=8, seed=5, flags=s.flags, sky=(0.45, 0.55, 0.75),
+        one = scenes.terrain_cameras(1, 1920, 1080)
+        s1 = make_settings(spp=4, max_depth=8, seed=5, flags=s.flags, sky=(0.45, 0.55, 0.75),
                            sun_dir=(0.4, 0.8, 0.3), sun_radius_deg=5.0, sun_radiance=(40.0, 36.0, 30.0))
         t0 = time.time()
-        O.render(sc, crop[0], s1, threads)
-        dt = time.time() - t0
-        cpu = {"value": round(240 * 135 / dt, 1), "unit": "spp/s", "cores": threads, "kind": "port",
-               "sample": "one 240x135 view at 1 spp, depth 8, including the oracle's CPU BVH build over 9.69M "
-                         "triangles (the GPU figure excludes its build)"}
-    return {"metric": "Grays/s closest-hit extension rays (terrain, 16 views, 1080p, 64 spp, depth 8)",
-            "value": round(rays / (ms / 1e3) / 1e9, 3), "unit": "Grays/s", "spp_per_s": round(P * spp / (ms / 1e3), 1),
-            "ms_per_batch": round(ms, 1), "rays": int(rays), "views": len(cams), "higher_is_better": True,
+        O.render(sc, one[0], s1, threads)
+        dt = O.last_render_seconds()
+        log(f"[bench] terrain oracle: {time.time() - t0:.1f}s with its BVH build, {dt:.1f}s path tracing")
+        cpu = {"value": round(1920 * 1080 * 4 / dt, 1), "unit": "spp/s", "cores": threads, "kind": "port",
+               "sample": "view 0 at 1920x1080, 4 spp, depth 8 (the oracle traces every sample from the camera); "
+                         "path-tracing time only, its CPU BVH build over 9.69M triangles excluded"}
+    return {"metric": f"spp/s path tracing (terrain, {len(cams)} views at 1920x1080, {spp} spp, depth 8)",
+            "value": round(P * spp / (ms / 1e3), 1), "unit": "spp/s",
+            "grays_per_s": round(rays / (ms / 1e3) / 1e9, 3), "rays": int(rays),
+            "rays_per_sample": round(rays / (P * spp), 3),
+            "ms_per_batch": round(ms, 1), "views": len(cams), "higher_is_better": True,
             "dtype": "f32", "data": "synthetic", "n_gpus": 1, "bvh_build_ms": round(info["build_ms"], 2),
             "config": {"workload": "fBm heightfield 2048x2048 (6 octaves) + 64 level-5 icospheres, sun + sky, "
-                                   f"1920x1080, {spp} spp, depth 8, {len(cams)} views in one launch sequence",
+                                   f"1920x1080, {spp} spp, depth 8, {len(cams)} views in one launch sequence; "
+                                   "each pixel's fixed primary ray traced once per work item",
                        "triangles": int(sc.num_triangles)},
             "image_mean": [round(float(x), 4) for x in rgb.reshape(-1, 3).mean(axis=0)],
             "cpu_baseline": cpu}

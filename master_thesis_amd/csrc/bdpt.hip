@@ -96,20 +96,19 @@ __device__ __forceinline__ void put_vert(BdVert& v, f3 p, f3 n, f3 c, uint32_t f
     v.flags = flags;
 }
 
-// one append per wave into queue q
+// append the live lanes to queue q: one atomic per workgroup; every thread
+// of the workgroup calls it
 __device__ __forceinline__ void enqueue(const BdArgs& a, int q, bool alive, uint32_t s, const rt_ray& r) {
+    __shared__ uint32_t lds_alloc[BD_BLOCK / 64 + 1];
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const uint64_t live = __ballot(alive);
-    if (!live) return;
-    const int leader = __ffsll((unsigned long long)live) - 1;
-    uint32_t base = 0;
-    if ((int)lane == leader) base = atomicAdd(&a.ctr[q ? BC_N1 : BC_N0], (uint32_t)__popcll(live));
-    base = __shfl(base, leader);
+    const bool f[1] = {alive};
+    uint32_t* const c[1] = {&a.ctr[q ? BC_N1 : BC_N0]};
+    uint32_t pos[1];
+    block_alloc<BD_BLOCK, 1>(f, c, lds_alloc, lane, lt, pos);
     if (alive) {
-        const uint32_t pos = base + (uint32_t)__popcll(live & lt);
-        a.q_id[q][pos] = s;
-        a.q_ray[q][pos] = r;
+        a.q_id[q][pos[0]] = s;
+        a.q_ray[q][pos[0]] = r;
     }
 }
 

@@ -478,33 +478,26 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
             alive = true;
         }
     }
-    // next work item for slots whose item finished: one atomic per wave
-    const uint64_t want = __ballot(need_item);
-    if (want) {
-        const int leader = __ffsll((unsigned long long)want) - 1;
-        uint32_t base = 0;
-        if ((int)lane == leader) base = atomicAdd(&a.ctr[C_ITEM], (uint32_t)__popcll(want));
-        base = __shfl(base, leader);
-        if (need_item) {
-            const uint32_t item = base + (uint32_t)__popcll(want & lt);
-            if (item < a.items) {
-                start_item(a, slot, item, nr, ADJ);
-                alive = true;
-            }
+    // next work item for slots whose item finished, then the survivors
+    // compacted into the next queue: one atomic per workgroup each
+    __shared__ uint32_t lds_alloc[PT_BLOCK / 64 + 1];
+    {
+        const bool f[1] = {need_item};
+        uint32_t* const c[1] = {&a.ctr[C_ITEM]};
+        uint32_t item[1];
+        block_alloc<PT_BLOCK, 1>(f, c, lds_alloc, lane, lt, item);
+        if (need_item && item[0] < a.items) {
+            start_item(a, slot, item[0], nr, ADJ);
+            alive = true;
         }
     }
-    // compact the survivors into the next queue: one atomic per wave
-    const uint64_t live = __ballot(alive);
-    if (live) {
-        const int leader = __ffsll((unsigned long long)live) - 1;
-        uint32_t base = 0;
-        if ((int)lane == leader) base = atomicAdd(n_out, (uint32_t)__popcll(live));
-        base = __shfl(base, leader);
-        if (alive) {
-            const uint32_t pos = base + (uint32_t)__popcll(live & lt);
-            a.q_slot[cur ^ 1][pos] = slot;
-            a.q_ray[cur ^ 1][pos] = nr;
-        }
+    const bool f[1] = {alive};
+    uint32_t* const c[1] = {n_out};
+    uint32_t pos[1];
+    block_alloc<PT_BLOCK, 1>(f, c, lds_alloc, lane, lt, pos);
+    if (alive) {
+        a.q_slot[cur ^ 1][pos[0]] = slot;
+        a.q_ray[cur ^ 1][pos[0]] = nr;
     }
 }
 
@@ -765,86 +758,57 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade_nrc(PTArgs a, int cur) {
             alive = true;
         }
     }
-    // cache queries: one atomic per wave
-    const uint64_t qb = __ballot(push_q);
-    if (qb) {
-        const int leader = __ffsll((unsigned long long)qb) - 1;
-        uint32_t base = 0;
-        if ((int)lane == leader) base = atomicAdd(&a.ctr[C_QN], (uint32_t)__popcll(qb));
-        base = __shfl(base, leader);
+    // cache queries, training records opened at the query vertex, shadow
+    // rays and next work items: one atomic per counter and workgroup
+    __shared__ uint32_t lds_alloc[4 * (PT_BLOCK / 64 + 1)];
+    {
+        const bool f[4] = {push_q, want_rec, push_sh, need_item};
+        uint32_t* const c[4] = {&a.ctr[C_QN], &a.ctr[C_TN], &a.ctr[C_SN], &a.ctr[C_ITEM]};
+        uint32_t pos[4];
+        block_alloc<PT_BLOCK, 4>(f, c, lds_alloc, lane, lt, pos);
         if (push_q) {
-            const uint32_t pos = base + (uint32_t)__popcll(qb & lt);
-            if (pos < a.q_cap) {
-                for (int j = 0; j < 9; ++j) a.q_feat[9 * (size_t)pos + j] = f9[j];
-                a.q_w[3 * (size_t)pos] = qw.x; a.q_w[3 * (size_t)pos + 1] = qw.y; a.q_w[3 * (size_t)pos + 2] = qw.z;
-                a.q_idx[pos] = pidx;
+            if (pos[0] < a.q_cap) {
+                for (int j = 0; j < 9; ++j) a.q_feat[9 * (size_t)pos[0] + j] = f9[j];
+                a.q_w[3 * (size_t)pos[0]] = qw.x; a.q_w[3 * (size_t)pos[0] + 1] = qw.y;
+                a.q_w[3 * (size_t)pos[0] + 2] = qw.z;
+                a.q_idx[pos[0]] = pidx;
             } else {
                 atomicAdd(&a.ctr[C_QDROP], 1u);
             }
         }
-    }
-    // training records opened at the query vertex: features now, target
-    // accumulated from here on (records past the capacity are not kept)
-    const uint64_t tb = __ballot(want_rec);
-    if (tb) {
-        const int leader = __ffsll((unsigned long long)tb) - 1;
-        uint32_t base = 0;
-        if ((int)lane == leader) base = atomicAdd(&a.ctr[C_TN], (uint32_t)__popcll(tb));
-        base = __shfl(base, leader);
+        // a training record: features now, target accumulated from here on
+        // (records past the capacity are not kept)
         if (want_rec) {
-            const uint32_t pos = base + (uint32_t)__popcll(tb & lt);
-            ridx = pos < a.t_cap ? pos : NO_REC;
+            ridx = pos[1] < a.t_cap ? pos[1] : NO_REC;
             if (ridx != NO_REC) {
-                for (int j = 0; j < 9; ++j) a.t_feat[9 * (size_t)pos + j] = f9[j];
-                a.t_tgt[3 * (size_t)pos] = 0.0f; a.t_tgt[3 * (size_t)pos + 1] = 0.0f; a.t_tgt[3 * (size_t)pos + 2] = 0.0f;
+                for (int j = 0; j < 9; ++j) a.t_feat[9 * (size_t)ridx + j] = f9[j];
+                a.t_tgt[3 * (size_t)ridx] = 0.0f; a.t_tgt[3 * (size_t)ridx + 1] = 0.0f;
+                a.t_tgt[3 * (size_t)ridx + 2] = 0.0f;
             }
             a.s_ridx[slot] = ridx;
         }
-    }
-    // shadow rays: one atomic per wave
-    const uint64_t sb = __ballot(push_sh);
-    if (sb) {
-        const int leader = __ffsll((unsigned long long)sb) - 1;
-        uint32_t base = 0;
-        if ((int)lane == leader) base = atomicAdd(&a.ctr[C_SN], (uint32_t)__popcll(sb));
-        base = __shfl(base, leader);
         if (push_sh) {
-            const uint32_t pos = base + (uint32_t)__popcll(sb & lt);  // < S: one per live slot
-            a.sh_ray[pos] = sh;
-            float* w = a.sh_w + 6 * (size_t)pos;
+            const uint32_t q = pos[2];  // < S: one per live slot
+            a.sh_ray[q] = sh;
+            float* w = a.sh_w + 6 * (size_t)q;
             w[0] = shw_pix.x; w[1] = shw_pix.y; w[2] = shw_pix.z;
             w[3] = shw_rec.x; w[4] = shw_rec.y; w[5] = shw_rec.z;
-            a.sh_idx[2 * (size_t)pos] = pidx;
-            a.sh_idx[2 * (size_t)pos + 1] = sh_to_rec ? ridx : NO_REC;
+            a.sh_idx[2 * (size_t)q] = pidx;
+            a.sh_idx[2 * (size_t)q + 1] = sh_to_rec ? ridx : NO_REC;
+        }
+        if (need_item && pos[3] < a.items) {
+            start_item(a, slot, pos[3], nr, false);
+            alive = true;
         }
     }
-    // next work item for slots whose item finished: one atomic per wave
-    const uint64_t want = __ballot(need_item);
-    if (want) {
-        const int leader = __ffsll((unsigned long long)want) - 1;
-        uint32_t base = 0;
-        if ((int)lane == leader) base = atomicAdd(&a.ctr[C_ITEM], (uint32_t)__popcll(want));
-        base = __shfl(base, leader);
-        if (need_item) {
-            const uint32_t item = base + (uint32_t)__popcll(want & lt);
-            if (item < a.items) {
-                start_item(a, slot, item, nr, false);
-                alive = true;
-            }
-        }
-    }
-    // compact the survivors into the next queue: one atomic per wave
-    const uint64_t live = __ballot(alive);
-    if (live) {
-        const int leader = __ffsll((unsigned long long)live) - 1;
-        uint32_t base = 0;
-        if ((int)lane == leader) base = atomicAdd(n_out, (uint32_t)__popcll(live));
-        base = __shfl(base, leader);
-        if (alive) {
-            const uint32_t pos = base + (uint32_t)__popcll(live & lt);
-            a.q_slot[cur ^ 1][pos] = slot;
-            a.q_ray[cur ^ 1][pos] = nr;
-        }
+    // compact the survivors into the next queue
+    const bool f[1] = {alive};
+    uint32_t* const c[1] = {n_out};
+    uint32_t pos[1];
+    block_alloc<PT_BLOCK, 1>(f, c, lds_alloc, lane, lt, pos);
+    if (alive) {
+        a.q_slot[cur ^ 1][pos[0]] = slot;
+        a.q_ray[cur ^ 1][pos[0]] = nr;
     }
 }
 

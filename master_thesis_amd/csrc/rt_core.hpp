@@ -262,4 +262,43 @@ RT_HD float safe_rcp(float d) {
     return 1.0f / a;
 }
 
+// Dense positions for the flagged lanes of a workgroup, for K counters at
+// once, with one atomic per counter and workgroup: same-address atomics from
+// every wave of a launch over 2^20 slots queue up at the memory-side atomic
+// unit, and the K atomics here are independent, so they overlap.  pos[k] is
+// meaningful for lanes with flag[k] set; lanes keep their order inside the
+// workgroup.  Every thread of the workgroup (BLOCK threads) must call it; lds
+// holds K * (BLOCK / 64 + 1) words.
+template <uint32_t BLOCK, int K>
+__device__ __forceinline__ void block_alloc(const bool (&flag)[K], uint32_t* const (&counter)[K], uint32_t* lds,
+                                            uint32_t lane, uint64_t lt, uint32_t (&pos)[K]) {
+    constexpr uint32_t NW = BLOCK / 64;
+    const uint32_t w = threadIdx.x >> 6;
+    uint64_t m[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        m[k] = __ballot(flag[k]);
+        if (lane == 0) lds[k * (NW + 1) + w] = (uint32_t)__popcll(m[k]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            tot[k] = 0;
+            for (uint32_t i = 0; i < NW; ++i) {
+                const uint32_t c = lds[k * (NW + 1) + i];
+                lds[k * (NW + 1) + i] = tot[k];
+                tot[k] += c;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) lds[k * (NW + 1) + NW] = tot[k] ? atomicAdd(counter[k], tot[k]) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) pos[k] = lds[k * (NW + 1) + NW] + lds[k * (NW + 1) + w] + (uint32_t)__popcll(m[k] & lt);
+    __syncthreads();  // lds is reused by the next call
+}
+
 }  // namespace rt

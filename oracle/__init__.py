@@ -211,6 +211,14 @@ def render_views(scene, cams, settings, threads: int = 8):
     return rgb, rgb8
 
 
+def last_render_seconds() -> float:
+    """Wall time of the last render / render_views pass, BVH build excluded."""
+    lib = load()
+    lib.or_last_render_seconds.restype = C.c_double
+    lib.or_last_render_seconds.argtypes = []
+    return float(lib.or_last_render_seconds())
+
+
 def render_adjoint(scene, cam, settings, d_image: np.ndarray, threads: int = 8) -> np.ndarray:
     arrs, ps = _scene_ptrs(scene)
     M = arrs[4].shape[0]

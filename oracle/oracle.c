@@ -23,6 +23,7 @@
  * exact squared-distance ties go to the lowest triangle index (the reference
  * keeps whichever its tree order meets first).
  */
+#define _POSIX_C_SOURCE 199309L  /* clock_gettime */
 #include "oracle.h"
 
 #include <math.h>
@@ -30,6 +31,7 @@
 #include <stdatomic.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "../include/rt.h"
 
@@ -1262,6 +1264,18 @@ static void setup_lights(pt_job* j, const float* pos, const uint32_t* idx, const
     j->light_area = acc;
 }
 
+/* Wall time of the last path-tracing pass, BVH build excluded (for timing
+ * the oracle as a CPU baseline). */
+static double g_render_seconds;
+
+double or_last_render_seconds(void) { return g_render_seconds; }
+
+static double now_seconds(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
 static int pt_run(const float* pos, const float* nrm, const uint32_t* idx, const uint32_t* tri_mat, const void* mat,
                   uint32_t V, uint32_t T, uint32_t M, const void* cams, uint32_t num_views, const void* settings,
                   float* rgb, uint8_t* rgb8, const float* dimg, float* dcol, int threads, cache_cfg* cc) {
@@ -1298,7 +1312,9 @@ static int pt_run(const float* pos, const float* nrm, const uint32_t* idx, const
         j.ddgi_dep = cc->ddgi_dep;
         setup_lights(&j, pos, idx, tri_mat, T);
     }
+    const double t0 = now_seconds();
     run_parallel(threads, j.cams[0].width * j.cams[0].height * num_views, 16, pt_range, &j);
+    g_render_seconds = now_seconds() - t0;
     if (cc) {
         free(j.light_tri);
         free(j.light_cdf);

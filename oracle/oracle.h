@@ -88,6 +88,8 @@ int or_render(const float* pos, const float* nrm, const uint32_t* idx, const uin
 int or_render_views(const float* pos, const float* nrm, const uint32_t* idx, const uint32_t* tri_mat, const void* mat,
                     uint32_t V, uint32_t T, uint32_t M, const void* cams, uint32_t num_views, const void* settings,
                     float* rgb, uint8_t* rgb8, int threads);
+/* Wall time of the last or_render* pass in seconds, BVH build excluded. */
+double or_last_render_seconds(void);
 /* Path-replay gradient of sum(d_image * image) with respect to every
  * material colour (3*M doubles accumulated, returned as float). */
 int or_render_adjoint(const float* pos, const float* nrm, const uint32_t* idx, const uint32_t* tri_mat,
