@@ -356,35 +356,57 @@ __device__ __forceinline__ void make_entry(uint32_t ref, const Inner& in, const 
     }
 }
 
-// One item per thread: (binary inner node, 4-wide slot, parent slot).
-__global__ void k_collapse(const uint4* __restrict__ qin, uint32_t nq, uint4* __restrict__ qout, Inner in,
-                           const uint32_t* __restrict__ vals, const float* leafbox, const float* innerbox,
-                           uint32_t leaf_size, float alpha, float pad, rt_bvh4_node* __restrict__ nodes,
-                           uint32_t max_nodes, uint32_t max_queue, uint32_t* counters) {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nq) return;
-    uint4 it = qin[k];
+// One item per thread: (binary inner node, 4-wide slot, parent slot).  New
+// node slots, next-level queue entries and the leaf count are taken with one
+// atomic per counter and workgroup.
+constexpr uint32_t COLLAPSE_BLOCK = 128;
+
+__global__ void __launch_bounds__(COLLAPSE_BLOCK) k_collapse(const uint4* __restrict__ qin, uint32_t nq,
+                                                             uint4* __restrict__ qout, Inner in,
+                                                             const uint32_t* __restrict__ vals, const float* leafbox,
+                                                             const float* innerbox, uint32_t leaf_size, float alpha,
+                                                             float pad, rt_bvh4_node* __restrict__ nodes,
+                                                             uint32_t max_nodes, uint32_t max_queue,
+                                                             uint32_t* counters) {
+    __shared__ uint32_t lds_alloc[3 * (COLLAPSE_BLOCK / 64 + 1)];
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = k < nq;
+    uint4 it = make_uint4(0u, 0u, 0u, 0u);
     Entry e[4];
-    int cnt = 2;
-    make_entry(in.left[it.x], in, vals, leafbox, innerbox, leaf_size, alpha, e[0]);
-    make_entry(in.right[it.x], in, vals, leafbox, innerbox, leaf_size, alpha, e[1]);
-    while (cnt < 4) {
-        int best = -1;
-        float best_a = -1.0f;
-        for (int c = 0; c < cnt; ++c) {
-            if (!e[c].leaf) {
-                float a = half_area(e[c].b);
-                if (a > best_a) { best_a = a; best = c; }
+    int cnt = 0;
+    if (active) {
+        it = qin[k];
+        cnt = 2;
+        make_entry(in.left[it.x], in, vals, leafbox, innerbox, leaf_size, alpha, e[0]);
+        make_entry(in.right[it.x], in, vals, leafbox, innerbox, leaf_size, alpha, e[1]);
+        while (cnt < 4) {
+            int best = -1;
+            float best_a = -1.0f;
+            for (int c = 0; c < cnt; ++c) {
+                if (!e[c].leaf) {
+                    float a = half_area(e[c].b);
+                    if (a > best_a) { best_a = a; best = c; }
+                }
             }
+            if (best < 0) break;
+            uint32_t r = e[best].ref;
+            make_entry(in.left[r], in, vals, leafbox, innerbox, leaf_size, alpha, e[best]);
+            make_entry(in.right[r], in, vals, leafbox, innerbox, leaf_size, alpha, e[cnt]);
+            ++cnt;
         }
-        if (best < 0) break;
-        uint32_t r = e[best].ref;
-        make_entry(in.left[r], in, vals, leafbox, innerbox, leaf_size, alpha, e[best]);
-        make_entry(in.right[r], in, vals, leafbox, innerbox, leaf_size, alpha, e[cnt]);
-        ++cnt;
     }
+    uint32_t inner = 0, leaves = 0;
+    for (int c = 0; c < cnt; ++c) {
+        if (e[c].leaf) ++leaves;
+        else ++inner;
+    }
+    const uint32_t counts[3] = {inner, inner, leaves};
+    uint32_t* const ctr[3] = {&counters[CTR_NODES], &counters[CTR_QOUT], &counters[CTR_LEAVES]};
+    uint32_t base[3];
+    block_alloc_counts<COLLAPSE_BLOCK, 3>(counts, ctr, lds_alloc, base);
+    if (!active) return;
     rt_bvh4_node nd;
-    uint32_t leaves = 0;
+    uint32_t j = 0;
     for (int c = 0; c < 4; ++c) {
         if (c < cnt) {
             nd.lo_x[c] = e[c].b[0] - pad; nd.hi_x[c] = e[c].b[3] + pad;
@@ -392,10 +414,9 @@ __global__ void k_collapse(const uint4* __restrict__ qin, uint32_t nq, uint4* __
             nd.lo_z[c] = e[c].b[2] - pad; nd.hi_z[c] = e[c].b[5] + pad;
             if (e[c].leaf) {
                 nd.child[c] = RT_LEAF_BIT | ((e[c].count - 1u) << RT_LEAF_COUNT_SHIFT) | e[c].first;
-                ++leaves;
             } else {
-                uint32_t slot = atomicAdd(&counters[CTR_NODES], 1u);
-                uint32_t q = atomicAdd(&counters[CTR_QOUT], 1u);
+                const uint32_t slot = base[0] + j, q = base[1] + j;
+                ++j;
                 if (slot < max_nodes && q < max_queue) {
                     qout[q] = make_uint4(e[c].ref, slot, it.y, 0u);
                     nd.child[c] = slot;
@@ -413,7 +434,6 @@ __global__ void k_collapse(const uint4* __restrict__ qin, uint32_t nq, uint4* __
     nd.meta[0] = it.z;
     nd.meta[1] = nd.meta[2] = nd.meta[3] = 0u;
     nodes[it.y] = nd;
-    if (leaves) atomicAdd(&counters[CTR_LEAVES], leaves);
 }
 
 // Root of a scene with <= leaf_size triangles: one leaf child.
@@ -554,7 +574,7 @@ int build_lbvh(rt_ctx* c, uint32_t leaf_size) {
             if (depth >= 128) return fail(c, RT_E_DEVICE, "rt_build: BVH deeper than 128 levels");
             ++depth;
             RT_HIP(c, hipMemsetAsync(counters + CTR_QOUT, 0, 4, st));
-            hipLaunchKernelGGL(k_collapse, dim3(blocks_for(nq, 128)), dim3(128), 0, st, (const uint4*)c->queue[q].ptr, nq,
+            hipLaunchKernelGGL(k_collapse, dim3(blocks_for(nq, COLLAPSE_BLOCK)), dim3(COLLAPSE_BLOCK), 0, st, (const uint4*)c->queue[q].ptr, nq,
                                (uint4*)c->queue[q ^ 1].ptr, in, (const uint32_t*)vals[cur], (const float*)leafbox,
                                (const float*)innerbox, leaf_size, c->bvh_leaf_alpha, pad, nodes, max_nodes, max_queue,
                                counters);

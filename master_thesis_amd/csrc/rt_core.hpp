@@ -301,4 +301,43 @@ __device__ __forceinline__ void block_alloc(const bool (&flag)[K], uint32_t* con
     __syncthreads();  // lds is reused by the next call
 }
 
+// As block_alloc, for per-lane counts: pos[k] is the first of the cnt[k]
+// consecutive positions the lane gets from *counter[k].
+template <uint32_t BLOCK, int K>
+__device__ __forceinline__ void block_alloc_counts(const uint32_t (&cnt)[K], uint32_t* const (&counter)[K],
+                                                   uint32_t* lds, uint32_t (&pos)[K]) {
+    constexpr uint32_t NW = BLOCK / 64;
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    uint32_t x[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        x[k] = cnt[k];  // inclusive prefix over the wave
+#pragma unroll
+        for (uint32_t o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x[k], o);
+            if (lane >= o) x[k] += y;
+        }
+        if (lane == 63u) lds[k * (NW + 1) + w] = x[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            tot[k] = 0;
+            for (uint32_t i = 0; i < NW; ++i) {
+                const uint32_t c = lds[k * (NW + 1) + i];
+                lds[k * (NW + 1) + i] = tot[k];
+                tot[k] += c;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) lds[k * (NW + 1) + NW] = tot[k] ? atomicAdd(counter[k], tot[k]) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) pos[k] = lds[k * (NW + 1) + NW] + lds[k * (NW + 1) + w] + x[k] - cnt[k];
+    __syncthreads();  // lds is reused by the next call
+}
+
 }  // namespace rt
