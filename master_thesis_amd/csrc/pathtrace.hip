@@ -60,6 +60,7 @@ namespace {
 
 constexpr int PT_BLOCK = 256;
 constexpr uint32_t ADJ_MAX_DEPTH = 16;
+constexpr uint32_t ADJ_LDS_MAX = 3072;  // 3 * materials summed in LDS (12 KB)
 constexpr uint32_t CULL_BIT = 0x10000u;
 // cache mode: front-facing hits so far (bits 17-23), throughput-from-the-
 // query-vertex tracked (bit 24), last bounce diffuse (bit 25)
@@ -115,6 +116,11 @@ struct PTArgs {
     uint8_t* rgb8;
     const float* dimg;
     float* dcolor;
+    // adjoint with few materials: each workgroup sums its gradient in LDS and
+    // adds it to its own column of adj_rows ([adj_cols][grid]); 0 = atomics
+    // straight into dcolor
+    float* adj_rows;
+    uint32_t adj_cols;
     // radiance cache (nrc_q == 0: off)
     uint32_t nrc_q, nrc_every, nrc_query;
     float* q_feat;      // q_cap * 9
@@ -232,7 +238,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_init(PTArgs a) {
 // Scatter dL/dI * dC/dcolour over the diffuse vertices of a finished path.
 // `term` is the factor after the last vertex (1 for a light hit, the sky
 // radiance for an escaped path).
-__device__ __forceinline__ void scatter_grad(const PTArgs& a, uint32_t slot, uint32_t pix, f3 term) {
+__device__ __forceinline__ void scatter_grad(const PTArgs& a, uint32_t slot, uint32_t pix, f3 term, float* acc) {
     const uint32_t nv = a.s_nv[slot];
     const float w[3] = {a.dimg[3 * pix] * (float)a.scale, a.dimg[3 * pix + 1] * (float)a.scale,
                         a.dimg[3 * pix + 2] * (float)a.scale};
@@ -254,7 +260,7 @@ __device__ __forceinline__ void scatter_grad(const PTArgs& a, uint32_t slot, uin
         for (int k = (int)nv - 1; k >= 0; --k) suf[k] = f[k] * suf[k + 1];
         for (uint32_t k = 0; k < nv; ++k) {
             float g = pre * suf[k + 1] * cs[k];
-            atomicAdd(&a.dcolor[3 * m[k] + ch], w[ch] * g);
+            atomicAdd(a.adj_cols ? &acc[3 * m[k] + ch] : &a.dcolor[3 * m[k] + ch], w[ch] * g);
             pre = pre * f[k];
         }
     }
@@ -327,6 +333,11 @@ __device__ __forceinline__ void vertex_step(const PTArgs& a, uint32_t slot, uint
 
 template <bool ADJ>
 __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
+    __shared__ float adj_acc[ADJ ? ADJ_LDS_MAX : 1];
+    if (ADJ && a.adj_cols) {
+        for (uint32_t i = threadIdx.x; i < a.adj_cols; i += PT_BLOCK) adj_acc[i] = 0.0f;
+        __syncthreads();
+    }
     const uint32_t n = a.ctr[cur ? C_N1 : C_N0];
     uint32_t* n_out = &a.ctr[cur ? C_N0 : C_N1];
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -392,7 +403,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
             }
             if (contrib) {
                 if (ADJ) {
-                    scatter_grad(a, slot, vp, term);
+                    scatter_grad(a, slot, vp, term, adj_acc);
                 } else {
                     sum[0] = sum[0] + T.x;
                     sum[1] = sum[1] + T.y;
@@ -433,7 +444,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
                     if (end) {
                         if (contrib) {
                             if (ADJ) {
-                                scatter_grad(a, slot, vp, mk(1.0f, 1.0f, 1.0f));
+                                scatter_grad(a, slot, vp, mk(1.0f, 1.0f, 1.0f), adj_acc);
                             } else {
                                 sum[0] = sum[0] + T.x;
                                 sum[1] = sum[1] + T.y;
@@ -494,11 +505,33 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
     const bool f[1] = {alive};
     uint32_t* const c[1] = {n_out};
     uint32_t pos[1];
-    block_alloc<PT_BLOCK, 1>(f, c, lds_alloc, lane, lt, pos);
+    block_alloc<PT_BLOCK, 1>(f, c, lds_alloc, lane, lt, pos);  // also orders the LDS gradient sums
     if (alive) {
         a.q_slot[cur ^ 1][pos[0]] = slot;
         a.q_ray[cur ^ 1][pos[0]] = nr;
     }
+    if (ADJ && a.adj_cols) {
+        // the workgroup's gradient into its own column: launches on the
+        // stream run one after another and the grid is the same every
+        // iteration, so no other workgroup touches it
+        for (uint32_t i = threadIdx.x; i < a.adj_cols; i += PT_BLOCK) {
+            const float v = adj_acc[i];
+            if (v != 0.0f) a.adj_rows[(size_t)i * gridDim.x + blockIdx.x] += v;
+        }
+    }
+}
+
+// dcolor[i] = sum over workgroups of adj_rows[i][*] (one block per entry)
+__global__ void __launch_bounds__(256) k_adj_reduce(const float* __restrict__ rows, uint32_t nblocks,
+                                                    float* __restrict__ dcolor) {
+    __shared__ float part[4];
+    const float* r = rows + (size_t)blockIdx.x * nblocks;
+    float v = 0.0f;
+    for (uint32_t j = threadIdx.x; j < nblocks; j += 256) v = v + r[j];
+    for (int o = 32; o >= 1; o >>= 1) v = v + __shfl_xor(v, o);
+    if ((threadIdx.x & 63u) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) dcolor[blockIdx.x] = (part[0] + part[1]) + (part[2] + part[3]);
 }
 
 // ---------------------------------------------------------------- cache mode
@@ -927,10 +960,11 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
     const uint32_t N = s->spp;
     // samples of a pixel are split into K chunks (K | N) when there are too
     // few pixels to fill the device; deterministic mode keeps one chunk so
-    // every pixel's sum has the reference's sample order.
+    // every pixel's sum has the reference's sample order.  (The adjoint
+    // scatters per path, so chunks change only its summation order.)
     const uint64_t target_slots = 1ull << 20;
     uint32_t K = 1;
-    if (!(s->flags & RT_RENDER_DETERMINISTIC) && !adj) {
+    if (!(s->flags & RT_RENDER_DETERMINISTIC)) {
         uint64_t want = (target_slots + P - 1) / P;
         if (want > N) want = N;
         for (uint32_t k = (uint32_t)want; k >= 1; --k)
@@ -940,6 +974,8 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
     if (items > 0xFFFFFFFFull) return fail(c, RT_E_UNSUPPORTED, "rt_render: pixels * chunks above 2^32");
     if (nrc && 3 * items > 0xFFFFFFFFull) return fail(c, RT_E_UNSUPPORTED, "rt_render_nrc: pixels * chunks above 2^32 / 3");
     const uint32_t S = (uint32_t)(items < target_slots ? items : target_slots);
+    // adjoint: gradient sums in LDS per workgroup when 3 * materials fit
+    const uint32_t adj_cols = adj && 3 * (uint64_t)c->M <= ADJ_LDS_MAX ? 3u * c->M : 0u;
     int rc;
     const size_t szS = S;
     // cache mode: queries flushed every 8 iterations (at most S per
@@ -957,7 +993,8 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
         (rc = ensure(c, c->pt_queue[0], 4 * szS + 16 + sizeof(rt_ray) * szS)) ||
         (rc = ensure(c, c->pt_queue[1], 4 * szS + 16 + sizeof(rt_ray) * szS)) ||
         (rc = ensure(c, c->hits, szS * sizeof(rt_hit))) || (rc = ensure(c, c->pt_ctr, 64)) ||
-        (rc = ensure(c, c->pt_partial, adj ? 4 : 12 * items)) || (rc = ensure(c, c->pt_cams, sizeof(rt_camera) * V)))
+        (rc = ensure(c, c->pt_partial, adj ? 4 * adj_cols * (size_t)div_up(S, PT_BLOCK) + 4 : 12 * items)) ||
+        (rc = ensure(c, c->pt_cams, sizeof(rt_camera) * V)))
         return rc;
     RT_HIP(c, hipMemcpyAsync(c->pt_cams.ptr, cams, sizeof(rt_camera) * V, hipMemcpyHostToDevice, c->stream));
     PTArgs a;
@@ -1024,10 +1061,16 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
     a.rgb8 = d_rgb8;
     a.dimg = d_dimage;
     a.dcolor = d_dcolor;
+    a.adj_rows = adj_cols ? (float*)c->pt_partial.ptr : nullptr;
+    a.adj_cols = adj_cols;
 
     hipStream_t stream = c->stream;
     RT_HIP(c, hipEventRecord(c->ev[0], stream));
-    if (adj) hipLaunchKernelGGL(k_zero, dim3(div_up(3ull * c->M, 256)), dim3(256), 0, stream, d_dcolor, 3u * c->M);
+    if (adj && !adj_cols)
+        hipLaunchKernelGGL(k_zero, dim3(div_up(3ull * c->M, 256)), dim3(256), 0, stream, d_dcolor, 3u * c->M);
+    if (adj_cols)
+        hipLaunchKernelGGL(k_zero, dim3(div_up((uint64_t)adj_cols * div_up(S, PT_BLOCK), 256)), dim3(256), 0, stream,
+                           a.adj_rows, adj_cols * div_up(S, PT_BLOCK));
     if (nrc) hipLaunchKernelGGL(k_zero, dim3(div_up(3ull * items, 256)), dim3(256), 0, stream, a.partial, (uint32_t)(3 * items));
     if (adj) hipLaunchKernelGGL(k_pt_init<true>, dim3(div_up(S, PT_BLOCK)), dim3(PT_BLOCK), 0, stream, a);
     else hipLaunchKernelGGL(k_pt_init<false>, dim3(div_up(S, PT_BLOCK)), dim3(PT_BLOCK), 0, stream, a);
@@ -1081,6 +1124,12 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
     }
     if (!adj) {
         hipLaunchKernelGGL(k_pt_finish, dim3(div_up(P, PT_BLOCK)), dim3(PT_BLOCK), 0, stream, a);
+        RT_HIP(c, hipGetLastError());
+        ++launches;
+    }
+    if (adj_cols) {
+        hipLaunchKernelGGL(k_adj_reduce, dim3(adj_cols), dim3(256), 0, stream, (const float*)a.adj_rows,
+                           div_up(S, PT_BLOCK), d_dcolor);
         RT_HIP(c, hipGetLastError());
         ++launches;
     }
