@@ -251,6 +251,13 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* s) {
             }
     }
     DeviceGuard g(c->device);
+    // from here on a failure leaves the context without a scene: buffers of
+    // the previous one may already have been reallocated
+    c->V = c->T = c->M = 0;
+    c->built = false;
+    c->num_lights = 0;
+    c->num_light_objs = 0;
+    c->light_area = 0.0f;
     int rc;
     if (!lights.empty()) {
         if ((rc = ensure(c, c->lights, sizeof(LightTri) * lights.size()))) return rc;
@@ -268,7 +275,11 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* s) {
         (rc = ensure(c, c->mat, sizeof(rt_material) * M)) || (rc = ensure(c, c->shade_geom, sizeof(ShadeGeom) * T)) ||
         (rc = ensure(c, c->shade_nrm, sizeof(ShadeNrm) * T)))
         return rc;
-    DevBuf tmat;
+    struct TmpBuf {  // freed on every return
+        DevBuf b;
+        ~TmpBuf() { if (b.ptr) (void)hipFree(b.ptr); }
+    } tmp;
+    DevBuf& tmat = tmp.b;
     if ((rc = ensure(c, c->vnrm, 12 * V)) || (rc = ensure(c, tmat, 4 * T))) return rc;
     RT_HIP(c, hipMemcpyAsync(c->pos.ptr, s->pos, 12 * V, hipMemcpyHostToDevice, c->stream));
     RT_HIP(c, hipMemcpyAsync(c->idx.ptr, s->idx, 12 * T, hipMemcpyHostToDevice, c->stream));
@@ -280,7 +291,6 @@ int rt_scene_upload(rt_ctx* c, const rt_scene_desc* s) {
                        (const uint32_t*)tmat.ptr, (ShadeGeom*)c->shade_geom.ptr, (ShadeNrm*)c->shade_nrm.ptr);
     RT_HIP(c, hipGetLastError());
     RT_HIP(c, hipStreamSynchronize(c->stream));
-    release(c, tmat);
     c->V = (uint32_t)V;
     c->T = (uint32_t)T;
     c->M = (uint32_t)M;
