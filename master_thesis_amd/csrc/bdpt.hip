@@ -388,8 +388,10 @@ int launch_bdpt(rt_ctx* c, const rt_camera* cam, const rt_bdpt_settings* s, floa
         hipLaunchKernelGGL(k_bd_init, dim3(grid), dim3(BD_BLOCK), 0, stream, a);
         int cur = 0;
         for (uint32_t i = 0; i < VJ; ++i) {
-            if ((rc = launch_trace(c, a.q_ray[cur], a.B, a.hits, false, &a.ctr[cur ? BC_N1 : BC_N0]))) return rc;
-            RT_HIP(c, hipMemsetAsync(&a.ctr[cur ? BC_N0 : BC_N1], 0, 4, stream));
+            // the trace clears the queue counter the step appends to
+            if ((rc = launch_trace(c, a.q_ray[cur], a.B, a.hits, false, &a.ctr[cur ? BC_N1 : BC_N0],
+                                   &a.ctr[cur ? BC_N0 : BC_N1])))
+                return rc;
             hipLaunchKernelGGL(k_bd_step<false>, dim3(grid), dim3(BD_BLOCK), 0, stream, a, i, cur);
             cur ^= 1;
             launches += 2;
@@ -397,8 +399,9 @@ int launch_bdpt(rt_ctx* c, const rt_camera* cam, const rt_bdpt_settings* s, floa
         RT_HIP(c, hipMemsetAsync(&a.ctr[cur ? BC_N1 : BC_N0], 0, 4, stream));
         hipLaunchKernelGGL(k_bd_light_init, dim3(grid), dim3(BD_BLOCK), 0, stream, a, cur);
         for (uint32_t i = 1; i < LJ; ++i) {
-            if ((rc = launch_trace(c, a.q_ray[cur], a.B, a.hits, false, &a.ctr[cur ? BC_N1 : BC_N0]))) return rc;
-            RT_HIP(c, hipMemsetAsync(&a.ctr[cur ? BC_N0 : BC_N1], 0, 4, stream));
+            if ((rc = launch_trace(c, a.q_ray[cur], a.B, a.hits, false, &a.ctr[cur ? BC_N1 : BC_N0],
+                                   &a.ctr[cur ? BC_N0 : BC_N1])))
+                return rc;
             hipLaunchKernelGGL(k_bd_step<true>, dim3(grid), dim3(BD_BLOCK), 0, stream, a, i, cur);
             cur ^= 1;
             launches += 2;
@@ -412,10 +415,9 @@ int launch_bdpt(rt_ctx* c, const rt_camera* cam, const rt_bdpt_settings* s, floa
         launches += 6;
     }
     RT_HIP(c, hipEventRecord(c->ev[1], stream));
-    uint32_t flags[2] = {0, 0};
-    RT_HIP(c, hipMemcpyAsync(flags, c->work.ptr, 8, hipMemcpyDeviceToHost, stream));
-    RT_HIP(c, hipStreamSynchronize(stream));
-    if (flags[1]) return fail(c, RT_E_DEVICE, "rt_render_bdpt: traversal exceeded its stack or step bound");
+    uint32_t err = 0;
+    if ((rc = trace_error(c, &err))) return rc;
+    if (err) return fail(c, RT_E_DEVICE, "rt_render_bdpt: traversal exceeded its stack or step bound");
     float ms = 0.0f;
     RT_HIP(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
     c->timing.last_call_ms = ms;

@@ -360,10 +360,9 @@ int rt_ddgi_update(rt_ddgi* g, const rt_settings* s) {
     RT_HIP(c, hipGetLastError());
     if ((rc = launch_update(g))) return rc;
     RT_HIP(c, hipEventRecord(c->ev[1], c->stream));
-    uint32_t flags[2] = {0, 0};
-    RT_HIP(c, hipMemcpyAsync(flags, c->work.ptr, 8, hipMemcpyDeviceToHost, c->stream));
-    RT_HIP(c, hipStreamSynchronize(c->stream));
-    if (flags[1]) return dg_fail(g, RT_E_DEVICE, "rt_ddgi_update: traversal exceeded its stack or step bound");
+    uint32_t err = 0;
+    if ((rc = trace_error(c, &err))) return rc;
+    if (err) return dg_fail(g, RT_E_DEVICE, "rt_ddgi_update: traversal exceeded its stack or step bound");
     float ms = 0.0f;
     RT_HIP(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
     c->timing.last_call_ms = ms;

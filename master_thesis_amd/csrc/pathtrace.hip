@@ -1084,10 +1084,12 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
     uint32_t launches = 1;
     while (live > 0) {
         for (int rep = 0; rep < 8 && iter < bound; ++rep, ++iter) {
-            if ((rc = launch_trace(c, a.q_ray[cur], S, a.hits, false, &a.ctr[cur ? C_N1 : C_N0]))) return rc;
-            RT_HIP(c, hipMemsetAsync(&a.ctr[cur ? C_N0 : C_N1], 0, 4, stream));
+            // the trace clears the counters the shade step appends to (the
+            // next queue's; the shadow rays')
+            if ((rc = launch_trace(c, a.q_ray[cur], S, a.hits, false, &a.ctr[cur ? C_N1 : C_N0],
+                                   &a.ctr[cur ? C_N0 : C_N1], nrc && a.nee ? &a.ctr[C_SN] : nullptr)))
+                return rc;
             if (nrc) {
-                if (a.nee) RT_HIP(c, hipMemsetAsync(&a.ctr[C_SN], 0, 4, stream));
                 hipLaunchKernelGGL(k_pt_shade_nrc, dim3(div_up(S, PT_BLOCK)), dim3(PT_BLOCK), 0, stream, a, cur);
                 RT_HIP(c, hipGetLastError());
                 if (a.nee) {
@@ -1154,7 +1156,9 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
     RT_HIP(c, hipEventRecord(c->ev[1], stream));
     unsigned long long rays = 0;
     RT_HIP(c, hipMemcpyAsync(&rays, &a.ctr[C_RAYS], 8, hipMemcpyDeviceToHost, stream));
-    RT_HIP(c, hipStreamSynchronize(stream));
+    uint32_t err = 0;
+    if ((rc = trace_error(c, &err))) return rc;  // synchronises the stream
+    if (err) return fail(c, RT_E_DEVICE, "rt_render: traversal exceeded its stack or step bound");
     float ms = 0.0f;
     RT_HIP(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
     if (nrc) {

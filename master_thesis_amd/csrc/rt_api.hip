@@ -122,6 +122,10 @@ int rt_create(rt_ctx** out, int device) {
         const float v = strtof(r, nullptr);
         c->bvh_leaf_alpha = v >= 0.0f ? v : 0.0f;
     }
+    if (const char* r = getenv("MTRT_TRACE_MAX_STEPS")) {  // test hook: a smaller per-ray step bound
+        const unsigned long v = strtoul(r, nullptr, 10);
+        c->trace_max_steps = (uint32_t)(v > 0xFFFFFFFFul ? 0xFFFFFFFFul : v);
+    }
     if (const char* r = getenv("MTRT_TRACE_SLICE")) {  // tuning knob, rounded to 128..2^24
         long v = strtol(r, nullptr, 10);
         v = v < 128 ? 128 : (v > (1l << 24) ? (1l << 24) : v);
@@ -176,6 +180,10 @@ const char* rt_last_error(const rt_ctx* c) {
 
 int rt_set_stream(rt_ctx* c, void* s) {
     if (!c) return RT_E_INVALID;
+    // work on the old stream finishes first: consecutive launches share
+    // device-side counters that each launch prepares for the next
+    DeviceGuard g(c->device);
+    RT_HIP(c, hipStreamSynchronize(c->stream));
     c->stream = s ? (hipStream_t)s : c->own_stream;
     return RT_OK;
 }
@@ -369,14 +377,13 @@ int rt_trace_closest(rt_ctx* c, const rt_ray* rays, uint32_t n, rt_hit* hits, in
     if (ptr_kind == RT_PTR_HOST)
         RT_HIP(c, hipMemcpyAsync(hits, d_hits, sizeof(rt_hit) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
     if (ptr_kind == RT_PTR_HOST || counters) {
-        uint32_t flags[2];
         unsigned long long st[4];
-        RT_HIP(c, hipMemcpyAsync(flags, c->work.ptr, 8, hipMemcpyDeviceToHost, c->stream));
         if (counters) RT_HIP(c, hipMemcpyAsync(st, c->stats.ptr, 32, hipMemcpyDeviceToHost, c->stream));
-        RT_HIP(c, hipStreamSynchronize(c->stream));
-        if (flags[1])
-            return fail(c, RT_E_DEVICE, flags[1] == 1 ? "rt_trace_closest: traversal stack exceeded its bound"
-                                                      : "rt_trace_closest: traversal exceeded its step bound");
+        uint32_t err = 0;
+        if ((rc = trace_error(c, &err))) return rc;
+        if (err)
+            return fail(c, RT_E_DEVICE, err == 1 ? "rt_trace_closest: traversal stack exceeded its bound"
+                                                 : "rt_trace_closest: traversal exceeded its step bound");
         if (counters) {
             counters->rays = st[0];
             counters->node_visits = st[1];
@@ -410,11 +417,10 @@ static int occlusion_query(rt_ctx* c, const char* what, const void* in, size_t i
     if (rc) return rc;
     c->last_op = 1;
     if (ptr_kind == RT_PTR_HOST) {
-        uint32_t flags[2];
         RT_HIP(c, hipMemcpyAsync(out, d_out, n, hipMemcpyDeviceToHost, c->stream));
-        RT_HIP(c, hipMemcpyAsync(flags, c->work.ptr, 8, hipMemcpyDeviceToHost, c->stream));
-        RT_HIP(c, hipStreamSynchronize(c->stream));
-        if (flags[1]) return fail(c, RT_E_DEVICE, std::string(what) + ": traversal exceeded its stack or step bound");
+        uint32_t err = 0;
+        if ((rc = trace_error(c, &err))) return rc;
+        if (err) return fail(c, RT_E_DEVICE, std::string(what) + ": traversal exceeded its stack or step bound");
     }
     return RT_OK;
 }

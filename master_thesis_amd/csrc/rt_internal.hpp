@@ -70,6 +70,9 @@ struct rt_ctx {
     rt::DevBuf keys[2], vals[2], hist, inner, leafbox, innerbox, boxtab, queue[2], counters;
     // trace scratch
     rt::DevBuf rays, hits, spill, work, stats;
+    uint32_t work_half = 0;  // traversal: which half of `work` the next launch counts rays in
+    int trace_per_cu[3] = {0, 0, 0};  // traversal: resident workgroups per CU (closest, counting, any)
+    uint32_t trace_max_steps = 0;  // traversal: per-ray step bound (0: from the tree size; MTRT_TRACE_MAX_STEPS)
     uint32_t trace_refill_min = 8;  // traversal: idle lanes that trigger a ray refill (MTRT_TRACE_REFILL)
     uint32_t trace_slice = 16384;   // traversal: rays per XCD slice, a multiple of 128 (MTRT_TRACE_SLICE)
     float bvh_leaf_alpha = 1.0f;    // build: node-step / triangle-step cost of the leaf test (MTRT_BVH_ALPHA)
@@ -110,12 +113,19 @@ int build_lbvh(rt_ctx* c, uint32_t leaf_size);
 // trace.hip
 constexpr int TRACE_BLOCK = 256;
 constexpr int STACK_LDS = 32;
-// n: ray count, or an upper bound on it when n_dev (device-resident count) is set
+// n: ray count, or an upper bound on it when n_dev (device-resident count) is
+// set.  zero0 / zero1: device words the launch sets to 0 before any later
+// launch on the stream runs (queue counters of a wavefront loop, so that the
+// loop needs no separate memset).
 int launch_trace(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, bool stats,
-                 const uint32_t* n_dev = nullptr);
+                 const uint32_t* n_dev = nullptr, uint32_t* zero0 = nullptr, uint32_t* zero1 = nullptr);
 // occlusion: d_occluded[i] = 1 when ray i hits any triangle in (tmin, tmax)
 int launch_trace_any(rt_ctx* c, const rt_ray* d_rays, uint32_t n, uint8_t* d_occluded,
-                     const uint32_t* n_dev = nullptr);
+                     const uint32_t* n_dev = nullptr, uint32_t* zero0 = nullptr);
+// Traversal error of the launches since the last call (0 none, 1 stack
+// overflow, 2 step bound; both mean a corrupt tree): read, then cleared.
+// Synchronises the stream.
+int trace_error(rt_ctx* c, uint32_t* err);
 // Scene::isOccluded for n segments (start, end: 6 floats each)
 int launch_occluded(rt_ctx* c, const float* d_segs, uint32_t n, uint8_t* d_occluded);
 

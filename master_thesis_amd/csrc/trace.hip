@@ -49,7 +49,11 @@ struct TraceArgs {
     const uint32_t* n_dev;  // when set, the ray count is read from device memory
     const rt_bvh4_node* nodes;
     const rt_tri_rec* tris;
-    uint32_t* work;        // [1] = stack overflow flag; [32 (1 + x)] = next ray of XCD x's slices
+    uint32_t* work;        // [32 (1 + x)] = next ray of XCD x's slices (zero at launch)
+    uint32_t* work_next;   // the other half of the buffer: zeroed here for the next launch
+    uint32_t* err;         // sticky: 1 = stack overflow, 2 = step bound
+    uint32_t* zero0;       // optional words zeroed for the launches after this one
+    uint32_t* zero1;
     uint32_t* spill;       // [spill_cap][grid threads]
     uint32_t spill_cap;
     uint32_t spill_stride;
@@ -153,7 +157,7 @@ __device__ __forceinline__ bool lane_step(const TraceArgs& a, Lane& L, uint32_t*
                     if (L.sp >= (uint32_t)STACK_LDS) {
                         const uint32_t deep = L.sp - STACK_LDS;
                         if (deep < a.spill_cap) a.spill[(size_t)deep * a.spill_stride + gtid] = stk[slot];
-                        else a.work[1] = 1u;  // cannot happen for a stack sized from the depth
+                        else *a.err = 1u;  // cannot happen for a stack sized from the depth
                     }
                     stk[slot] = ref[c];
                     ++L.sp;
@@ -196,7 +200,7 @@ __device__ __forceinline__ bool lane_step(const TraceArgs& a, Lane& L, uint32_t*
     }
     if (L.sp == 0) return true;
     if (++L.steps > a.max_steps) {  // cannot happen for a tree; ends the ray instead of looping
-        a.work[1] = 2u;
+        *a.err = 2u;
         return true;
     }
     --L.sp;
@@ -217,6 +221,13 @@ template <bool STATS, bool ANY>
 __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceArgs a) {
     __shared__ uint32_t stk[STACK_LDS * TRACE_BLOCK];
     const uint32_t tid = threadIdx.x;
+    if (blockIdx.x == 0) {
+        // the counters of the next launch (they last served the launch before
+        // this one, which has finished), and the caller's words
+        for (uint32_t i = tid; i < XCD_COUNTER_STRIDE * (1u + XCDS); i += TRACE_BLOCK) a.work_next[i] = 0u;
+        if (tid == 0 && a.zero0) *a.zero0 = 0u;
+        if (tid == 0 && a.zero1) *a.zero1 = 0u;
+    }
     const uint32_t gtid = blockIdx.x * TRACE_BLOCK + tid;
     const uint32_t lane = tid & 63u;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -325,15 +336,26 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceArgs a) {
 }  // namespace
 
 namespace {
+constexpr uint32_t WORK_HALF = XCD_COUNTER_STRIDE * (1 + XCDS);  // words per half of `work`
+constexpr uint32_t WORK_ERR = 2 * WORK_HALF;                      // the sticky error word
+
 int launch(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, uint8_t* d_occl, bool stats,
-           const uint32_t* n_dev, const float* d_segs = nullptr) {
-    if (n == 0) return RT_OK;
+           const uint32_t* n_dev, const float* d_segs = nullptr, uint32_t* zero0 = nullptr,
+           uint32_t* zero1 = nullptr) {
+    if (n == 0) {
+        if (zero0) RT_HIP(c, hipMemsetAsync(zero0, 0, 4, c->stream));
+        if (zero1) RT_HIP(c, hipMemsetAsync(zero1, 0, 4, c->stream));
+        return RT_OK;
+    }
     const bool any = d_occl != nullptr;
-    int per_cu = 0;
-    if (any) RT_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false, true>, TRACE_BLOCK, 0));
-    else if (stats) RT_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<true, false>, TRACE_BLOCK, 0));
-    else RT_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false, false>, TRACE_BLOCK, 0));
-    if (per_cu < 1) per_cu = 1;
+    // resident workgroups per CU of each variant, asked once per context
+    int& per_cu = c->trace_per_cu[any ? 2 : (stats ? 1 : 0)];
+    if (per_cu == 0) {
+        if (any) RT_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false, true>, TRACE_BLOCK, 0));
+        else if (stats) RT_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<true, false>, TRACE_BLOCK, 0));
+        else RT_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false, false>, TRACE_BLOCK, 0));
+        if (per_cu < 1) per_cu = 1;
+    }
     uint32_t grid = (uint32_t)(c->num_cus * per_cu);
     uint32_t need_blocks = (n + TRACE_BLOCK - 1) / TRACE_BLOCK;
     if (grid > need_blocks) grid = need_blocks;
@@ -342,10 +364,15 @@ int launch(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, uint8_t*
     const uint32_t spill_cap = max_stack > (uint32_t)STACK_LDS ? max_stack - STACK_LDS : 1u;
     int rc;
     if ((rc = ensure(c, c->spill, 4ull * spill_cap * threads))) return rc;
-    const size_t work_bytes = 4ull * XCD_COUNTER_STRIDE * (1 + XCDS);
-    if ((rc = ensure(c, c->work, work_bytes))) return rc;
+    // two halves of ray counters, each zeroed by the launch before the one
+    // that uses it, and the error word: cleared once, at allocation
+    if (!c->work.ptr) {
+        const size_t work_bytes = 4ull * (WORK_ERR + 32);
+        if ((rc = ensure(c, c->work, work_bytes))) return rc;
+        RT_HIP(c, hipMemsetAsync(c->work.ptr, 0, work_bytes, c->stream));
+        c->work_half = 0;
+    }
     if ((rc = ensure(c, c->stats, 64))) return rc;
-    RT_HIP(c, hipMemsetAsync(c->work.ptr, 0, work_bytes, c->stream));
     if (stats) RT_HIP(c, hipMemsetAsync(c->stats.ptr, 0, 32, c->stream));
     TraceArgs a;
     a.rays = d_rays;
@@ -356,11 +383,15 @@ int launch(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, uint8_t*
     a.n_dev = n_dev;
     a.nodes = (const rt_bvh4_node*)c->nodes.ptr;
     a.tris = (const rt_tri_rec*)c->tris.ptr;
-    a.work = (uint32_t*)c->work.ptr;
+    a.work = (uint32_t*)c->work.ptr + WORK_HALF * c->work_half;
+    a.work_next = (uint32_t*)c->work.ptr + WORK_HALF * (c->work_half ^ 1u);
+    a.err = (uint32_t*)c->work.ptr + WORK_ERR;
+    a.zero0 = zero0;
+    a.zero1 = zero1;
     a.spill = (uint32_t*)c->spill.ptr;
     a.spill_cap = spill_cap;
     a.spill_stride = threads;
-    a.max_steps = 2u * c->num_nodes + 2u * c->T + 16u;
+    a.max_steps = c->trace_max_steps ? c->trace_max_steps : 2u * c->num_nodes + 2u * c->T + 16u;
     a.refill_min = c->trace_refill_min;
     a.slice = c->trace_slice;
     a.stats = (unsigned long long*)c->stats.ptr;
@@ -369,17 +400,29 @@ int launch(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, uint8_t*
     else if (stats) hipLaunchKernelGGL((k_trace<true, false>), dim3(grid), dim3(TRACE_BLOCK), 0, c->stream, a);
     else hipLaunchKernelGGL((k_trace<false, false>), dim3(grid), dim3(TRACE_BLOCK), 0, c->stream, a);
     RT_HIP(c, hipGetLastError());
+    c->work_half ^= 1u;
     RT_HIP(c, hipEventRecord(c->ev[3], c->stream));
     c->timing.launches = 1;
     return RT_OK;
 }
 }  // namespace
 
-int launch_trace(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, bool stats, const uint32_t* n_dev) {
-    return launch(c, d_rays, n, d_hits, nullptr, stats, n_dev);
+int launch_trace(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, bool stats, const uint32_t* n_dev,
+                 uint32_t* zero0, uint32_t* zero1) {
+    return launch(c, d_rays, n, d_hits, nullptr, stats, n_dev, nullptr, zero0, zero1);
 }
-int launch_trace_any(rt_ctx* c, const rt_ray* d_rays, uint32_t n, uint8_t* d_occluded, const uint32_t* n_dev) {
-    return launch(c, d_rays, n, nullptr, d_occluded, false, n_dev);
+int launch_trace_any(rt_ctx* c, const rt_ray* d_rays, uint32_t n, uint8_t* d_occluded, const uint32_t* n_dev,
+                     uint32_t* zero0) {
+    return launch(c, d_rays, n, nullptr, d_occluded, false, n_dev, nullptr, zero0);
+}
+int trace_error(rt_ctx* c, uint32_t* err) {
+    *err = 0;
+    if (!c->work.ptr) return RT_OK;
+    uint32_t* w = (uint32_t*)c->work.ptr + WORK_ERR;
+    RT_HIP(c, hipMemcpyAsync(err, w, 4, hipMemcpyDeviceToHost, c->stream));
+    RT_HIP(c, hipStreamSynchronize(c->stream));
+    if (*err) RT_HIP(c, hipMemsetAsync(w, 0, 4, c->stream));
+    return RT_OK;
 }
 int launch_occluded(rt_ctx* c, const float* d_segs, uint32_t n, uint8_t* d_occluded) {
     return launch(c, nullptr, n, nullptr, d_occluded, false, nullptr, d_segs);

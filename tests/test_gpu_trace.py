@@ -169,3 +169,38 @@ def test_occluded_segments_match_oracle(tracer):
     want = O.occluded_brute(sc, np.concatenate([a, b], axis=1))
     np.testing.assert_array_equal(got, want)
     assert not got[: n // 10].any()
+
+
+def test_step_bound_reports_an_error_then_clears(monkeypatch):
+    """The per-ray step bound (a corrupt tree would otherwise loop) ends the
+    ray and fails the call with an error instead of hanging; the error does
+    not outlive the call that saw it, and a context with the normal bound
+    traces the same rays correctly.  The bound is forced from the host
+    (MTRT_TRACE_MAX_STEPS, read when a context is created)."""
+    from master_thesis_amd.tracer import Tracer
+    sc = scenes.triangle_soup(4096, seed=12)
+    rng = np.random.default_rng(3)
+    n = 4096
+    r = np.zeros(n, RAY_DTYPE)
+    r["o"] = rng.uniform(0.0, 1.0, (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    r["d"] = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    r["tmax"] = np.inf
+    away = np.zeros(64, RAY_DTYPE)  # rays that miss the root box: one step each
+    away["o"] = (5.0, 5.0, 5.0)
+    away["d"] = (1.0, 0.0, 0.0)
+    away["tmax"] = np.inf
+    monkeypatch.setenv("MTRT_TRACE_MAX_STEPS", "1")
+    with Tracer(0) as t:
+        t.upload(sc)
+        t.build(4)
+        with pytest.raises(RTError, match="step bound"):
+            t.trace(r)
+        assert np.all(t.trace(away)["tri"] == RT_MISS)  # the next call starts clean
+        with pytest.raises(RTError, match="step bound"):
+            t.trace(r)
+    monkeypatch.delenv("MTRT_TRACE_MAX_STEPS")
+    with Tracer(0) as t:
+        t.upload(sc)
+        t.build(4)
+        _assert_same_hits(t.trace(r), O.trace_brute(sc, r), "normal bound")
