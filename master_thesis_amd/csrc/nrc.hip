@@ -14,8 +14,9 @@
 // kernels read is refreshed after every step.
 //
 // Kernels:
-//   k_mlp_infer  8 waves, weights -> LDS once, then persistent tiles of 512
-//                samples (64 per wave): encode, 6 layers
+//   k_mlp_infer  4 waves, weights -> LDS once, then persistent tiles of 128
+//                samples (32 per wave) carried through encoding and 6 layers
+//                in registers
 //   k_mlp_train  4 waves, tiles of 64 samples, at most one workgroup per CU
 //                striding over the tiles: encode, forward (all activations
 //                kept in LDS), loss, backward; each workgroup sums its own
@@ -65,6 +66,9 @@ struct Geo {
 };
 
 __device__ __forceinline__ __bf16 tobf(float x) { return (__bf16)x; }
+// max(x, 0) as one integer max on the bits: a float with the sign bit set is
+// a negative integer (-0 and negative NaNs give +0)
+__device__ __forceinline__ float relu(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
 
 // sin and cos of pi * x * 2^k for k = 0..11, interleaved: exact anchors at
 // k = 0, 4, 8 (sincos_2pi of the fractional part of x * 2^(k-1); the scaling
@@ -177,147 +181,177 @@ __device__ __forceinline__ void hidden_forward(const __bf16* sW, const float* __
 
 // ---- inference ------------------------------------------------------------
 //
-// 512 threads (8 waves) share one copy of the BF16 weights in LDS; each wave
-// owns 64 samples of a 512-sample tile and its own activation buffer, so the
-// waves never wait for each other after the weights are loaded.  LDS layout
-// of a matrix with K columns: K/32 blocks of [rows][32], so that a lane's
-// operand fragment (8 consecutive k of one row) is one 16-byte load and a
-// fragment of 16 rows is 1 KB contiguous.  Every product is computed
-// transposed, C^T = W X^T (weights as the row operand), so a lane ends with
-// four consecutive output features of one sample: ReLU, BF16 and one 8-byte
-// store per 16x16 tile.
-constexpr int INF_NT = 512;                       // threads per workgroup
-constexpr int INF_TILE = INF_NT;                  // samples per workgroup tile
-constexpr int ABUF = 3 * 64 * 32;                 // bf16 per wave: 64 rows x 96 columns, blocked
+// 256 threads (4 waves) share one copy of the BF16 weights in LDS; each wave
+// owns 32 samples of a 128-sample tile and keeps their activations in
+// registers from the encoding to the output.  Every product is computed
+// transposed, C^T = W X^T with the weights as the row operand: on
+// v_mfma_f32_16x16x32_bf16 lane (mi = l % 16, kg = l / 16) then holds outputs
+// 16 nt + 4 kg + i (i < 4) of sample 16 mt + mi, and the next layer's column
+// operand wants, for its 32-wide k block ks, k slots 8 kg + j of the same
+// sample.  A product sums over k in any order, so each layer's weight
+// columns are stored permuted (slot 8 kg + j of block ks holds input
+// 32 ks + 4 kg + j for j < 4 and 32 ks + 16 + 4 kg + j - 4 otherwise): the
+// slots a lane must supply are exactly the outputs it holds, and a layer's
+// ReLU output becomes the next layer's operand without leaving the lane.  The
+// input layer's columns are permuted the same way to the encoding: lanes of
+// group kg < 3 compute axis kg's 24 sin/cos features (block ks = anchor ks),
+// group 3 the normal, direction and zero padding.
+constexpr int INF_NT = 256;                       // threads per workgroup
+constexpr int INF_MT = 2;                         // 16-sample groups a wave carries through the layers
+constexpr int INF_TILE = INF_NT / 64 * 16 * INF_MT;  // samples per workgroup tile
 
-// blocked index of element (row, k) in a matrix of `rows` rows
-__device__ __forceinline__ int blk(int row, int k, int rows) { return (k >> 5) * rows * 32 + row * 32 + (k & 31); }
+// blocked index of slot p (0..K-1) of row n in a matrix of N rows: K/32
+// blocks of [N][32], so a lane's 8 slots of one row are one 16-byte load and
+// the 16 rows of a fragment are 1 KB contiguous
+__device__ __forceinline__ int blk(int n, int p, int N) { return (p >> 5) * N * 32 + n * 32 + (p & 31); }
 
-// weights, row-major [n][k] per layer in global memory -> blocked in LDS
+// weights, row-major [n][k] per layer in global memory -> blocked and
+// permuted in LDS (see above)
 __device__ __forceinline__ void load_weights_blocked(const __bf16* __restrict__ Wb, __bf16* sW) {
+    // input layer: slots (ks, 8 kg .. 8 kg + 7) <- features 24 kg + 8 ks ..
+    for (int c = threadIdx.x; c < ND[0] * 12; c += INF_NT) {
+        const int n = c / 12, ks = (c % 12) >> 2, kg = c & 3;
+        *(uint4*)(sW + offwb(0) + blk(n, 32 * ks + 8 * kg, ND[0])) =
+            *(const uint4*)(Wb + offwb(0) + n * KD[0] + 24 * kg + 8 * ks);
+    }
+    // hidden and output layers: slots 8 kg + j <- inputs 4 kg + j, 16 + 4 kg + j - 4
 #pragma unroll
-    for (int l = 0; l < L; ++l) {
+    for (int l = 1; l < L; ++l) {
         const int K = KD[l], N = ND[l];
-        const int chunks = N * K / 8;  // 16-byte chunks
+        const int chunks = N * (K / 8);
         for (int c = threadIdx.x; c < chunks; c += INF_NT) {
-            const int n = c / (K / 8), k8 = (c % (K / 8)) * 8;
-            *(uint4*)(sW + offwb(l) + blk(n, k8, N)) = *(const uint4*)(Wb + offwb(l) + n * K + k8);
+            const int n = c / (K / 8), ks = (c % (K / 8)) >> 2, kg = c & 3;
+            const __bf16* src = Wb + offwb(l) + n * K + 32 * ks + 4 * kg;
+            uint2* dst = (uint2*)(sW + offwb(l) + blk(n, 32 * ks + 8 * kg, N));
+            dst[0] = *(const uint2*)src;
+            dst[1] = *(const uint2*)(src + 16);
         }
     }
 }
 
-// One layer for a wave's 64 samples: acc[mt][nt] = W[16nt.., :] . X[16mt.., :]^T
-// over K (in the wave's blocked buffer), NT_ n-tiles.
-template <int K, int N, int NT_>
-__device__ __forceinline__ void layer_mm(const __bf16* sWl, const __bf16* sA, int lane, f32x4 (&acc)[4][4]) {
+// acc[mt][nt] = (FIRST ? bias[nt] : acc[mt][nt]) + W[16 nt.., block ks] . X[mt]^T
+// for the wave's samples (the bias enters as the first product's addend)
+template <int N, int NT_, bool FIRST>
+__device__ __forceinline__ void layer_block(const __bf16* sWl, int ks, const bf16x8 (&x)[INF_MT], int lane,
+                                            const f32x4 (&bias)[NT_], f32x4 (&acc)[INF_MT][NT_]) {
     const int mi = lane & 15, kg = lane >> 4;
+    bf16x8 w[NT_];
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
+    for (int nt = 0; nt < NT_; ++nt) w[nt] = *(const bf16x8*)(sWl + blk(16 * nt + mi, 32 * ks + 8 * kg, N));
 #pragma unroll
-        for (int nt = 0; nt < NT_; ++nt) acc[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int mt = 0; mt < INF_MT; ++mt)
 #pragma unroll
-    for (int ks = 0; ks < K / 32; ++ks) {
-        bf16x8 xa[4], wb[NT_];
+        for (int nt = 0; nt < NT_; ++nt)
+            acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[nt], x[mt], FIRST ? bias[nt] : acc[mt][nt], 0,
+                                                                  0, 0);
+}
+
+// bias of outputs 16 nt + 4 kg + i (i < 4) of layer l
+template <int NT_>
+__device__ __forceinline__ void load_bias(const float* __restrict__ Wf, int l, int kg, f32x4 (&b)[NT_]) {
 #pragma unroll
-        for (int mt = 0; mt < 4; ++mt) xa[mt] = *(const bf16x8*)(sA + ks * 64 * 32 + (16 * mt + mi) * 32 + 8 * kg);
-#pragma unroll
-        for (int nt = 0; nt < NT_; ++nt) wb[nt] = *(const bf16x8*)(sWl + ks * N * 32 + (16 * nt + mi) * 32 + 8 * kg);
-#pragma unroll
-        for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-            for (int nt = 0; nt < NT_; ++nt)
-                acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wb[nt], xa[mt], acc[mt][nt], 0, 0, 0);
+    for (int nt = 0; nt < NT_; ++nt) {
+        const float4 v = *(const float4*)(Wf + offb(l) + 16 * nt + 4 * kg);
+        b[nt] = f32x4{v.x, v.y, v.z, v.w};
     }
 }
 
 // SCATTER: out[3 * idx[s] + c] += qw[3 * s + c] * max(y_c, 0) (atomically) in
 // place of out[3 * s + c] = y_c.
 template <bool SCATTER>
-__global__ void __launch_bounds__(INF_NT) k_mlp_infer(const float* __restrict__ feats, uint32_t n,
+__global__ void __launch_bounds__(INF_NT) __attribute__((amdgpu_waves_per_eu(3))) k_mlp_infer(const float* __restrict__ feats, uint32_t n,
                                                       const __bf16* __restrict__ Wb, const float* __restrict__ Wf,
                                                       Geo G, float* __restrict__ out, const float* __restrict__ qw,
                                                       const uint32_t* __restrict__ idx) {
     __shared__ __attribute__((aligned(16))) __bf16 sW[NWB];
-    __shared__ __attribute__((aligned(16))) __bf16 sAct[INF_NT / 64][ABUF];
     load_weights_blocked(Wb, sW);
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int mi = lane & 15, kg = lane >> 4;
-    __bf16* sA = sAct[w];
     const uint32_t tiles = (n + INF_TILE - 1) / INF_TILE;
     for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
-        const uint32_t s0 = t * INF_TILE + 64 * w;  // this wave's first sample
-        // encode one sample per lane into its row
-        {
-            const uint32_t sm = s0 + lane;
-            float f9[9];
+        // keeps the weight fragments' LDS loads inside the loop (hoisted, all
+        // six layers' fragments would not fit in registers)
+        __asm__ __volatile__("" ::: "memory");
+        const uint32_t s0 = t * INF_TILE + 16 * INF_MT * w;  // this wave's first sample
+        f32x4 acc[INF_MT][4];
+        f32x4 bias[4];
+        load_bias<4>(Wf, 0, kg, bias);
+        // input layer, one k block per encoding anchor
+        float xa[INF_MT];  // group kg < 3: the normalised axis coordinate of each sample
 #pragma unroll
-            for (int j = 0; j < 9; ++j) f9[j] = sm < n ? feats[(size_t)sm * 9 + j] : 0.0f;
-            float v[96];
-#pragma unroll
-            for (int a = 0; a < 3; ++a) encode_axis((f9[a] - G.lo[a]) * G.inv[a], v + 24 * a);
-#pragma unroll
-            for (int j = 0; j < 6; ++j) v[72 + j] = f9[3 + j];
-#pragma unroll
-            for (int j = 78; j < 96; ++j) v[j] = 0.0f;
-            if (sm >= n) {
-#pragma unroll
-                for (int j = 0; j < 96; ++j) v[j] = 0.0f;
-            }
-#pragma unroll
-            for (int c = 0; c < 12; ++c) {
-                bf16x8 pk;
-#pragma unroll
-                for (int j = 0; j < 8; ++j) pk[j] = tobf(v[8 * c + j]);
-                *(bf16x8*)(sA + blk(lane, 8 * c, 64)) = pk;
-            }
+        for (int mt = 0; mt < INF_MT; ++mt) {
+            const uint32_t sm = s0 + 16 * mt + mi;
+            xa[mt] = (kg < 3 && sm < n) ? (feats[(size_t)sm * 9 + kg] - G.lo[kg]) * G.inv[kg] : 0.0f;
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // hidden layers, in place in the wave's buffer
-        f32x4 acc[4][4];
 #pragma unroll
-        for (int l = 0; l < L - 1; ++l) {
-            if (l == 0) layer_mm<96, 64, 4>(sW + offwb(0), sA, lane, acc);
-            else layer_mm<64, 64, 4>(sW + offwb(l), sA, lane, acc);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int ks = 0; ks < 3; ++ks) {
+            bf16x8 x[INF_MT];
 #pragma unroll
-            for (int nt = 0; nt < 4; ++nt) {
-                const float4 b = *(const float4*)(Wf + offb(l) + 16 * nt + 4 * kg);
+            for (int mt = 0; mt < INF_MT; ++mt) {
+                if (kg < 3) {
+                    // sin / cos of pi x 2^(4 ks + j), j = 0..3 (encode_axis, anchor ks)
+                    const float y = ldexpf(xa[mt], 4 * ks - 1);
+                    float sn, cs;
+                    sincos_2pi(y - floorf(y), sn, cs);
 #pragma unroll
-                for (int mt = 0; mt < 4; ++mt) {
-                    typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-                    bf16x4 h;
-                    h[0] = tobf(fmaxf(acc[mt][nt][0] + b.x, 0.0f));
-                    h[1] = tobf(fmaxf(acc[mt][nt][1] + b.y, 0.0f));
-                    h[2] = tobf(fmaxf(acc[mt][nt][2] + b.z, 0.0f));
-                    h[3] = tobf(fmaxf(acc[mt][nt][3] + b.w, 0.0f));
-                    *(bf16x4*)(sA + blk(16 * mt + mi, 16 * nt + 4 * kg, 64)) = h;
+                    for (int j = 0; j < 4; ++j) {
+                        x[mt][2 * j] = tobf(sn);
+                        x[mt][2 * j + 1] = tobf(cs);
+                        const float s2 = 2.0f * sn * cs, c2 = (cs - sn) * (cs + sn);
+                        sn = s2;
+                        cs = c2;
+                    }
+                } else {
+                    // group 3: normal and direction in block 0, zero padding after
+                    const bool ok = s0 + 16 * mt + mi < n;
+                    const float* f = feats + (size_t)(s0 + 16 * mt + mi) * 9 + 3;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) x[mt][j] = tobf(ks == 0 && j < 6 && ok ? f[j] : 0.0f);
                 }
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (ks == 0) layer_block<64, 4, true>(sW + offwb(0), ks, x, lane, bias, acc);
+            else layer_block<64, 4, false>(sW + offwb(0), ks, x, lane, bias, acc);
         }
-        // output layer: one n-tile; lanes with kg == 0 hold outputs 0..3 of sample 16 mt + mi
-        f32x4 o5[4][4];
-        layer_mm<64, 16, 1>(sW + offwb(L - 1), sA, lane, o5);
+        // hidden layers 1..4 and the output layer, activations in registers
+        f32x4 o5[INF_MT][1];
+#pragma unroll
+        for (int l = 1; l < L; ++l) {
+            bf16x8 x[2][INF_MT];
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+                for (int mt = 0; mt < INF_MT; ++mt) {
+                    const int ks = nt >> 1, h = 4 * (nt & 1);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) x[ks][mt][h + i] = tobf(relu(acc[mt][nt][i]));
+                }
+            if (l < L - 1) {
+                load_bias<4>(Wf, l, kg, bias);
+                layer_block<64, 4, true>(sW + offwb(l), 0, x[0], lane, bias, acc);
+                layer_block<64, 4, false>(sW + offwb(l), 1, x[1], lane, bias, acc);
+            } else {
+                f32x4 b5[1];
+                load_bias<1>(Wf, l, kg, b5);
+                layer_block<16, 1, true>(sW + offwb(l), 0, x[0], lane, b5, o5);
+                layer_block<16, 1, false>(sW + offwb(l), 1, x[1], lane, b5, o5);
+            }
+        }
+        // lanes with kg == 0 hold outputs 0..3 of sample 16 mt + mi
         if (kg == 0) {
 #pragma unroll
-            for (int mt = 0; mt < 4; ++mt) {
+            for (int mt = 0; mt < INF_MT; ++mt) {
                 const uint32_t sm = s0 + 16 * mt + mi;
                 if (sm < n) {
 #pragma unroll
                     for (int c = 0; c < N5_VALID; ++c) {
-                        const float y = o5[mt][0][c] + Wf[offb(L - 1) + c];
+                        const float y = o5[mt][0][c];
                         if (SCATTER) atomicAdd(&out[3 * (size_t)idx[sm] + c], qw[3 * (size_t)sm + c] * fmaxf(y, 0.0f));
                         else out[(size_t)sm * 3 + c] = y;
                     }
                 }
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 

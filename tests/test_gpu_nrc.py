@@ -236,8 +236,11 @@ def test_online_training_gain_and_bias(tracer):
     at 1 spp, 4-spp frames that stop at the cache after the first bounce reach
     the RMSE of light-sampled frames without the network while tracing less
     than half their rays (extension plus shadow rays), and the cache's bias
-    (the cached render at 2048 spp against the reference) stays within 3% of
-    the frame mean."""
+    (the cached render at 2048 spp against the reference) stays within 5% of
+    the frame mean.  Training records gather their light samples with float
+    atomics, so the network, and with it the bias, differs from run to run:
+    over repeated runs it spread about 1.5% of the frame mean around zero
+    (one run in eight near 3.4%)."""
     sc = scenes.cornell_closed()
     cache = _scene_cache(tracer, sc, seed=1)
     cam = H.closed_camera(256, 256)
@@ -267,5 +270,5 @@ def test_online_training_gain_and_bias(tracer):
     assert e["cached"] < 1.05 * e["light"]
     assert rays["cached"] < 0.5 * rays["light"]
     # the cache's bias: a few percent of the frame mean after 400 Adam steps
-    assert abs(bias) < 0.03 * ref.mean()
+    assert abs(bias) < 0.05 * ref.mean()
     cache.close()
