@@ -185,7 +185,8 @@ int  rt_create(rt_ctx** out, int device);
 /* Frees every device buffer the context owns (graphics_engine.cpp:12-14). */
 void rt_destroy(rt_ctx* ctx);
 const char* rt_last_error(const rt_ctx* ctx);
-/* Enqueue subsequent work on this hipStream_t (NULL = the context's own). */
+/* Enqueue subsequent work on this hipStream_t (NULL = the context's own).
+ * Waits for the work already enqueued on the previous stream first. */
 int  rt_set_stream(rt_ctx* ctx, void* hip_stream);
 int  rt_get_timing(const rt_ctx* ctx, rt_timing* out);
 
