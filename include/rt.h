@@ -175,7 +175,8 @@ typedef struct rt_timing {
     float last_call_ms;       /* device time of the whole last call (first to last op) */
     uint32_t launches;        /* kernel launches the last call made                    */
     uint32_t iterations;      /* render: extend/shade iterations of the last call      */
-    uint64_t rays_traced;     /* render: closest-hit queries of the last call          */
+    uint64_t rays_traced;     /* render: closest-hit queries the last call traced (a    *
+                               * pixel's fixed primary-ray prefix once per work item)  */
 } rt_timing;
 
 /* ---- context ------------------------------------------------------------- */
