@@ -219,6 +219,10 @@ def raycast(args, sc, rays, n_coh, workload, pmc_path):
             traffic = None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                # what HBM actually moved (memory-side counters, calibrated) against its peak: the
+                # algorithmic bytes above count node and triangle fetches the caches serve too
+                "traffic_gbps": traffic and round(traffic / (k_ms / 1e3) / 1e9, 1),
+                "traffic_frac": traffic and round(traffic / (k_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
                 "kernel": "k_trace", "kernel_ms": round(k_ms, 4),
                 "bytes_alg_per_launch": int(bytes_alg),
                 "node_visits_per_ray": round(cnt["node_visits"] / n, 3),
