@@ -50,7 +50,8 @@ def main() -> int:
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 512 MiB device copy (reference kernel for HBM counters)")
     ap.add_argument("--config", default="raycast",
-                    choices=["raycast", "terrain_raycast", "cornell", "terrain", "diff", "nrc", "bdpt", "denoise"],
+                    choices=["raycast", "terrain_raycast", "cornell", "terrain", "diff", "nrc", "bdpt", "denoise",
+                             "mlp"],
                     help="raycast: the headline microbench (default); cornell / terrain / diff / nrc: the other "
                          "workloads")
     ap.add_argument("--nrc-scene", default="terrain",
@@ -68,7 +69,7 @@ def main() -> int:
         return 2
     if args.config not in ("raycast", "terrain_raycast"):
         out = {"cornell": bench_cornell, "terrain": bench_terrain, "diff": bench_diff,
-               "nrc": bench_nrc, "bdpt": bench_bdpt, "denoise": bench_denoise}[args.config](args)
+               "nrc": bench_nrc, "bdpt": bench_bdpt, "denoise": bench_denoise, "mlp": bench_mlp}[args.config](args)
         print(json.dumps(out), flush=True)
         return 0
     if args.config == "terrain_raycast":
@@ -487,6 +488,72 @@ def bench_diff(args):
 
 
 
+def _time_mlp(t, cache, sc, P: int):
+    """Device time of one inference over P queries and of one 65,536-sample
+    training step (forward, backward, Adam), averaged over 20 calls each,
+    with events on the context's stream; features uniform in the scene box."""
+    import torch
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    t.set_stream(stream.cuda_stream)
+    lo, hi = np.asarray(sc.pos.min(axis=0)), np.asarray(sc.pos.max(axis=0))
+    g = torch.Generator(device="cuda").manual_seed(0)
+    feats = torch.rand((P, 9), device="cuda", generator=g)
+    feats[:, :3] = feats[:, :3] * torch.tensor(hi - lo, device="cuda") + torch.tensor(lo, device="cuda")
+    out = torch.empty((P, 3), device="cuda")
+    tgt = torch.rand((65536, 3), device="cuda", generator=g)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    for _ in range(3):
+        cache.infer(feats, out)
+    ev[0].record(stream)
+    for _ in range(20):
+        cache.infer(feats, out)
+    ev[1].record(stream)
+    tf = feats[:65536].contiguous()
+    for _ in range(3):
+        cache.train(tf, tgt, want_loss=False)
+    ev[2].record(stream)
+    for _ in range(20):
+        cache.train(tf, tgt, want_loss=False)
+    ev[3].record(stream)
+    torch.cuda.synchronize()
+    t.set_stream(None)
+    torch.cuda.set_stream(torch.cuda.default_stream())
+    return ev[0].elapsed_time(ev[1]) / 20, ev[2].elapsed_time(ev[3]) / 20
+
+
+def bench_mlp(args):
+    """The radiance cache's network alone: inference over one 1920x1080
+    frame of queries and a 65,536-sample training step, against the BF16
+    dense MFMA peak."""
+    from master_thesis_amd import scenes
+    from master_thesis_amd.cache import RadianceCache
+    from master_thesis_amd.tracer import Tracer
+    sc = scenes.cornell_closed()
+    P = 1920 * 1080
+    with Tracer(0) as t:
+        t.upload(sc)
+        t.build(args.leaf)
+        cache = RadianceCache(t, seed=1)
+        inf_ms, trn_ms = _time_mlp(t, cache, sc, P)
+        cache.close()
+    inf_tf = MLP_FWD_FLOP * P / (inf_ms * 1e-3) / 1e12
+    trn_tf = MLP_TRAIN_FLOP * 65536 / (trn_ms * 1e-3) / 1e12
+    return {"metric": "TFLOP/s radiance-cache network inference (1920x1080 queries, BF16 MFMA)",
+            "value": round(inf_tf, 1), "unit": "TFLOP/s", "higher_is_better": True, "n_gpus": 1,
+            "dtype": "bf16 MFMA with f32 accumulation", "data": "synthetic",
+            "mlp": {"infer_ms_per_frame": round(inf_ms, 4), "infer_tflops": round(inf_tf, 1),
+                    "train_ms_per_step_65536": round(trn_ms, 4), "train_tflops": round(trn_tf, 1)},
+            "roofline": {"bound": "mfma", "achieved": round(inf_tf, 1), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(inf_tf / BF16_PEAK_TFLOPS, 4), "traffic": None,
+                         "kernel": "k_mlp_infer", "flop_per_sample": MLP_FWD_FLOP},
+            "roofline_train": {"bound": "mfma", "achieved": round(trn_tf, 1), "peak": BF16_PEAK_TFLOPS,
+                               "unit": "TFLOP/s", "frac": round(trn_tf / BF16_PEAK_TFLOPS, 4),
+                               "kernel": "k_mlp_train + k_adam", "flop_per_sample": MLP_TRAIN_FLOP},
+            "config": {"workload": "78-input MLP 96->64x4->16 (3 outputs), 2,073,600 queries per inference, "
+                                   "65,536 samples per training step", "layers": 6}}
+
+
 # floating-point work of the cache network per sample, on its valid (unpadded)
 # inputs and outputs: 78*64 + 4*64*64 + 64*3 = 21,568 multiply-adds forward
 MLP_FWD_FLOP = 2 * (78 * 64 + 4 * 64 * 64 + 64 * 3)
@@ -696,33 +763,7 @@ def bench_nrc(args):
             ratio[name] = need_spp * res[name]["ms_per_spp"] / ms1
             res[name]["spp_for_equal_rmse"] = round(need_spp, 2)
         # MLP kernels on the device, timed with events on the context stream
-        stream = torch.cuda.Stream()
-        torch.cuda.set_stream(stream)
-        t.set_stream(stream.cuda_stream)
-        lo, hi = np.asarray(sc.pos.min(axis=0)), np.asarray(sc.pos.max(axis=0))
-        g = torch.Generator(device="cuda").manual_seed(0)
-        feats = torch.rand((P, 9), device="cuda", generator=g)
-        feats[:, :3] = feats[:, :3] * torch.tensor(hi - lo, device="cuda") + torch.tensor(lo, device="cuda")
-        out = torch.empty((P, 3), device="cuda")
-        tgt = torch.rand((65536, 3), device="cuda", generator=g)
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-        for _ in range(3):
-            cache.infer(feats, out)
-        ev[0].record(stream)
-        for _ in range(20):
-            cache.infer(feats, out)
-        ev[1].record(stream)
-        tf = feats[:65536].contiguous()
-        for _ in range(3):
-            cache.train(tf, tgt, want_loss=False)
-        ev[2].record(stream)
-        for _ in range(20):
-            cache.train(tf, tgt, want_loss=False)
-        ev[3].record(stream)
-        torch.cuda.synchronize()
-        t.set_stream(None)
-        inf_ms = ev[0].elapsed_time(ev[1]) / 20
-        trn_ms = ev[2].elapsed_time(ev[3]) / 20
+        inf_ms, trn_ms = _time_mlp(t, cache, sc, P)
         inf_tf = MLP_FWD_FLOP * P / (inf_ms * 1e-3) / 1e12
         trn_tf = MLP_TRAIN_FLOP * 65536 / (trn_ms * 1e-3) / 1e12
         cache.close()
