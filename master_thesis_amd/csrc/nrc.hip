@@ -19,7 +19,8 @@
 //                in registers
 //   k_mlp_train  4 waves, tiles of 64 samples, at most one workgroup per CU
 //                striding over the tiles: encode, forward (all activations
-//                kept in LDS), loss, backward; each workgroup sums its own
+//                kept in LDS, rows padded against bank conflicts; gradients
+//                also kept transposed), loss, backward; each workgroup sums its own
 //                partial gradient over its tiles in registers and writes it
 //                once (no atomics, no read-modify-write of global memory)
 //   k_adam       sums the partials in workgroup order (deterministic), Adam
@@ -35,6 +36,7 @@
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
 namespace rt {
 namespace nrc {
@@ -107,36 +109,59 @@ __device__ __forceinline__ void encode_group(const float* f9, int g, const Geo& 
     }
 }
 
-// 16x16 tile of C = A * B over K (multiple of 32; terms k >= kvalid are zero).
-// A(m, k) = A[m * sAm + k * sAk], B(k, n) = B[k * sBk + n * sBn].
-// Lane l supplies A row m0 + l%16 and B column n0 + l%16, k = 8*(l/16) + j;
-// on return acc[i] = C(m0 + 4*(l/16) + i, n0 + l%16).
+// Training buffers in LDS are row-major with rows padded by 8 BF16 (16 bytes),
+// so that the 16 rows of an operand fragment start in different LDS banks:
+// features [sample][XS], activations and their gradients [sample][HS], the
+// output-layer gradient [sample][D0S], weights [n][KD[l] + 8].  The gradients
+// are also kept transposed, [feature][TS], for the weight-gradient products,
+// which sum over samples.
+constexpr int XS = 96 + 8, HS = 64 + 8, D0S = 16 + 8, TS = 64 + 8;
+constexpr int offwp(int l) { int o = 0; for (int i = 0; i < l; ++i) o += ND[i] * (KD[i] + 8); return o; }
+constexpr int NWP = offwp(L);
+
+// 16x16 tile of C = A * B over K (multiple of 32; terms k >= kvalid are zero,
+// kvalid a multiple of 8).  A(m, k) = A[m * sAm + k * sAk], B(k, n) =
+// B[k * sBk + n * sBn].  Lane l supplies A row m0 + l%16 and B column
+// n0 + l%16, k = 8*(l/16) + j; on return acc[i] = C(m0 + 4*(l/16) + i, n0 +
+// l%16).  AV / BV: the operand's k is contiguous (sAk / sBk = 1), so a lane's
+// eight values are one 16-byte load; otherwise they are gathered one by one.
+template <bool AV, bool BV>
 __device__ __forceinline__ f32x4 tile_mm(const __bf16* A, int sAm, int sAk, const __bf16* B, int sBk, int sBn,
                                          int m0, int n0, int K, int kvalid, int lane) {
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
     const int mi = lane & 15, kg = lane >> 4;
     for (int k0 = 0; k0 < K; k0 += 32) {
         bf16x8 a, b;
+        const int kb = k0 + kg * 8;
+        if (AV) {
+            if (kb < kvalid) a = *(const bf16x8*)(A + (m0 + mi) * sAm + kb);
+            else a = bf16x8{};
+        }
+        if (BV) {
+            if (kb < kvalid) b = *(const bf16x8*)(B + (n0 + mi) * sBn + kb);
+            else b = bf16x8{};
+        }
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            const int k = k0 + kg * 8 + j;
-            if (k < kvalid) {
-                a[j] = A[(m0 + mi) * sAm + k * sAk];
-                b[j] = B[k * sBk + (n0 + mi) * sBn];
-            } else {
-                a[j] = tobf(0.0f);
-                b[j] = tobf(0.0f);
-            }
+            const int k = kb + j;
+            if (!AV) a[j] = k < kvalid ? A[(m0 + mi) * sAm + k * sAk] : tobf(0.0f);
+            if (!BV) b[j] = k < kvalid ? B[k * sBk + (n0 + mi) * sBn] : tobf(0.0f);
         }
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
     }
     return acc;
 }
 
-__device__ __forceinline__ void load_weights(const __bf16* __restrict__ Wb, __bf16* sW) {
-    const uint4* src = (const uint4*)Wb;
-    uint4* dst = (uint4*)sW;
-    for (int i = threadIdx.x; i < NWB / 8; i += NT) dst[i] = src[i];
+// weights, row-major [n][k] per layer -> rows padded to KD[l] + 8 in LDS
+__device__ __forceinline__ void load_weights_padded(const __bf16* __restrict__ Wb, __bf16* sW) {
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        const int K = KD[l], chunks = ND[l] * (K / 8);
+        for (int c = threadIdx.x; c < chunks; c += NT) {
+            const int n = c / (K / 8), k8 = (c % (K / 8)) * 8;
+            *(uint4*)(sW + offwp(l) + n * (K + 8) + k8) = *(const uint4*)(Wb + offwb(l) + n * K + k8);
+        }
+    }
 }
 
 __device__ __forceinline__ void encode_tile(const float* __restrict__ feats, uint32_t s0, uint32_t n, const Geo& G,
@@ -147,9 +172,9 @@ __device__ __forceinline__ void encode_tile(const float* __restrict__ feats, uin
 #pragma unroll
     for (int j = 0; j < 9; ++j) f9[j] = valid ? feats[(size_t)(s0 + s) * 9 + j] : 0.0f;
     if (valid) {
-        encode_group(f9, g, G, sX + s * 96);
+        encode_group(f9, g, G, sX + s * XS);
     } else {
-        for (int j = 0; j < 24; ++j) sX[s * 96 + g * 24 + j] = tobf(0.0f);
+        for (int j = 0; j < 24; ++j) sX[s * XS + g * 24 + j] = tobf(0.0f);
     }
 }
 
@@ -161,21 +186,23 @@ __device__ __forceinline__ void hidden_forward(const __bf16* sW, const float* __
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int mi = lane & 15, kg = lane >> 4;
     const __bf16* in = sX;
+    int in_stride = XS;
     for (int l = 0; l < L - 1; ++l) {
         __bf16* o = out(l);
         const int K = KD[l];
         for (int nt = 0; nt < ND[l] / 16; ++nt) {
-            f32x4 acc = tile_mm(in, K, 1, sW + offwb(l), 1, K, 16 * w, 16 * nt, K, K, lane);
+            f32x4 acc = tile_mm<true, true>(in, in_stride, 1, sW + offwp(l), 1, K + 8, 16 * w, 16 * nt, K, K, lane);
             const int col = 16 * nt + mi;
             const float b = Wf[offb(l) + col];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int row = 16 * w + 4 * kg + i;
-                o[row * 64 + col] = tobf(fmaxf(acc[i] + b, 0.0f));
+                o[row * HS + col] = tobf(fmaxf(acc[i] + b, 0.0f));
             }
         }
         __syncthreads();
         in = o;
+        in_stride = HS;
     }
 }
 
@@ -359,15 +386,16 @@ __global__ void __launch_bounds__(NT) k_mlp_train(const float* __restrict__ feat
                                                   uint32_t n, const __bf16* __restrict__ Wb,
                                                   const float* __restrict__ Wf, Geo G, float* __restrict__ partial,
                                                   float* __restrict__ loss_part) {
-    __shared__ __attribute__((aligned(16))) __bf16 sW[NWB];
-    __shared__ __attribute__((aligned(16))) __bf16 sX[TILE * 96];
-    __shared__ __attribute__((aligned(16))) __bf16 sH[L - 1][TILE * 64];
-    __shared__ __attribute__((aligned(16))) __bf16 sD[2][TILE * 64];
+    __shared__ __attribute__((aligned(16))) __bf16 sW[NWP];
+    __shared__ __attribute__((aligned(16))) __bf16 sX[TILE * XS];
+    __shared__ __attribute__((aligned(16))) __bf16 sH[L - 1][TILE * HS];
+    __shared__ __attribute__((aligned(16))) __bf16 sD[2][TILE * HS];
+    __shared__ __attribute__((aligned(16))) __bf16 sDT[2][64 * TS];  // sD transposed
     __shared__ float sLoss[NT / 64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int mi = lane & 15, kg = lane >> 4;
     float* part = partial + (size_t)blockIdx.x * NPARAM;
-    load_weights(Wb, sW);
+    load_weights_padded(Wb, sW);
     // tiles blockIdx.x, + gridDim.x, ...: the workgroup's partial gradient is
     // the sum over its tiles in that order (each element is always written by
     // the same thread, so the read-modify-write needs no synchronisation)
@@ -394,9 +422,10 @@ __global__ void __launch_bounds__(NT) k_mlp_train(const float* __restrict__ feat
     // output layer and loss: y = H4 W5^T + b5; dL/dy = 2 (y - t) / (y^2 + 0.01) / n
     float lsum = 0.0f;
     {
-        f32x4 acc = tile_mm(sH[L - 2], 64, 1, sW + offwb(L - 1), 1, 64, 16 * w, 0, 64, 64, lane);
+        f32x4 acc = tile_mm<true, true>(sH[L - 2], HS, 1, sW + offwp(L - 1), 1, 72, 16 * w, 0, 64, 64, lane);
         const int col = mi;
         const float b = Wf[offb(L - 1) + col];
+        bf16x4 dyt;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int row = 16 * w + 4 * kg + i;
@@ -410,8 +439,10 @@ __global__ void __launch_bounds__(NT) k_mlp_train(const float* __restrict__ feat
                 lsum += r * r / den;
                 dy = 2.0f * r / den / (float)n;
             }
-            sD[0][row * 16 + col] = tobf(dy);
+            sD[0][row * D0S + col] = tobf(dy);
+            dyt[i] = tobf(dy);
         }
+        *(bf16x4*)(sDT[0] + col * TS + 16 * w + 4 * kg) = dyt;
     }
     for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
     if (lane == 0) sLoss[w] = lsum;
@@ -420,25 +451,27 @@ __global__ void __launch_bounds__(NT) k_mlp_train(const float* __restrict__ feat
 
     // layer 5: dW5[n][k] = sum_s dY[s][n] H4[s][k] (16 x 64), one 16-column tile per wave
     {
-        const f32x4 acc = tile_mm(sD[0], 1, 16, sH[L - 2], 64, 1, 0, 16 * w, 64, 64, lane);
+        const f32x4 acc = tile_mm<true, false>(sDT[0], TS, 1, sH[L - 2], HS, 1, 0, 16 * w, 64, 64, lane);
 #pragma unroll
         for (int i = 0; i < 4; ++i) gw[GW_OFF[L - 1]][i] += acc[i];
     }
     if (threadIdx.x < 16) {
         float s = 0.0f;
-        for (int r = 0; r < TILE; ++r) s += (float)sD[0][r * 16 + threadIdx.x];
+        for (int r = 0; r < TILE; ++r) s += (float)sD[0][r * D0S + threadIdx.x];
         gb[L - 1] += s;
     }
     // dH4[s][k] = sum_n dY[s][n] W5[n][k] (K = 16 valid of 32), masked by H4 > 0 -> sD[1]
     for (int nt = 0; nt < 4; ++nt) {
-        f32x4 acc = tile_mm(sD[0], 16, 1, sW + offwb(L - 1), 64, 1, 16 * w, 16 * nt, 32, 16, lane);
+        f32x4 acc = tile_mm<true, false>(sD[0], D0S, 1, sW + offwp(L - 1), 72, 1, 16 * w, 16 * nt, 32, 16, lane);
         const int col = 16 * nt + mi;
+        bf16x4 gt;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int row = 16 * w + 4 * kg + i;
-            const float h = (float)sH[L - 2][row * 64 + col];
-            sD[1][row * 64 + col] = tobf(h > 0.0f ? acc[i] : 0.0f);
+            const float h = (float)sH[L - 2][row * HS + col];
+            sD[1][row * HS + col] = gt[i] = tobf(h > 0.0f ? acc[i] : 0.0f);
         }
+        *(bf16x4*)(sDT[1] + col * TS + 16 * w + 4 * kg) = gt;
     }
     __syncthreads();
 
@@ -446,32 +479,37 @@ __global__ void __launch_bounds__(NT) k_mlp_train(const float* __restrict__ feat
 #pragma unroll
     for (int l = L - 2; l >= 0; --l) {
         const __bf16* dZ = sD[cur];
+        const __bf16* dZT = sDT[cur];
         const __bf16* in = (l == 0) ? sX : sH[l - 1];
+        const int in_stride = (l == 0) ? XS : HS;
         const int K = KD[l];
         // dW_l[n][k] = sum_s dZ[s][n] in[s][k]: rows n in 16-row tiles (one per wave), K / 16 column tiles
 #pragma unroll
         for (int kt = 0; kt < K / 16; ++kt) {
-            const f32x4 acc = tile_mm(dZ, 1, 64, in, K, 1, 16 * w, 16 * kt, 64, 64, lane);
+            const f32x4 acc = tile_mm<true, false>(dZT, TS, 1, in, in_stride, 1, 16 * w, 16 * kt, 64, 64, lane);
 #pragma unroll
             for (int i = 0; i < 4; ++i) gw[GW_OFF[l] + kt][i] += acc[i];
         }
         if (threadIdx.x < 64) {
             float s = 0.0f;
-            for (int r = 0; r < TILE; ++r) s += (float)dZ[r * 64 + threadIdx.x];
+            for (int r = 0; r < TILE; ++r) s += (float)dZ[r * HS + threadIdx.x];
             gb[l] += s;
         }
         if (l > 0) {
             // dH_{l-1}[s][k] = sum_n dZ[s][n] W_l[n][k], masked by H_{l-1} > 0
             __bf16* dst = sD[cur ^ 1];
+            __bf16* dstT = sDT[cur ^ 1];
             for (int nt = 0; nt < 4; ++nt) {
-                f32x4 acc = tile_mm(dZ, 64, 1, sW + offwb(l), 64, 1, 16 * w, 16 * nt, 64, 64, lane);
+                f32x4 acc = tile_mm<true, false>(dZ, HS, 1, sW + offwp(l), K + 8, 1, 16 * w, 16 * nt, 64, 64, lane);
                 const int col = 16 * nt + mi;
+                bf16x4 gt;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const int row = 16 * w + 4 * kg + i;
-                    const float h = (float)sH[l - 1][row * 64 + col];
-                    dst[row * 64 + col] = tobf(h > 0.0f ? acc[i] : 0.0f);
+                    const float h = (float)sH[l - 1][row * HS + col];
+                    dst[row * HS + col] = gt[i] = tobf(h > 0.0f ? acc[i] : 0.0f);
                 }
+                *(bf16x4*)(dstT + col * TS + 16 * w + 4 * kg) = gt;
             }
             __syncthreads();
             cur ^= 1;
