@@ -18,9 +18,10 @@
 //                samples (32 per wave) carried through encoding and 6 layers
 //                in registers
 //   k_mlp_train  4 waves, tiles of 64 samples, at most one workgroup per CU
-//                striding over the tiles: encode, forward (all activations
-//                kept in LDS, rows padded against bank conflicts; gradients
-//                also kept transposed), loss, backward; each workgroup sums its own
+//                striding over the tiles: encode and forward in registers as
+//                in inference, activations also stored transposed in LDS;
+//                loss; backward with the gradients in both orientations;
+//                each workgroup sums its own
 //                partial gradient over its tiles in registers and writes it
 //                once (no atomics, no read-modify-write of global memory)
 //   k_adam       sums the partials in workgroup order (deterministic), Adam
@@ -72,138 +73,55 @@ __device__ __forceinline__ __bf16 tobf(float x) { return (__bf16)x; }
 // a negative integer (-0 and negative NaNs give +0)
 __device__ __forceinline__ float relu(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
 
-// sin and cos of pi * x * 2^k for k = 0..11, interleaved: exact anchors at
-// k = 0, 4, 8 (sincos_2pi of the fractional part of x * 2^(k-1); the scaling
-// and the subtraction of the floor are exact), then three angle doublings
-// from each anchor (error below 1e-6, far under the BF16 step).
-__device__ __forceinline__ void encode_axis(float x, float* sc) {
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const float y = ldexpf(x, 4 * a - 1);
+// Features of k block ks that lane group kg supplies for one sample (the
+// input layer's weight columns are stored in this order).  kg < 3: sin and
+// cos of pi x 2^(4 ks + j), j = 0..3, interleaved, for the sample's
+// coordinate x on axis kg normalised to the scene box: an exact anchor
+// (sincos_2pi of the fractional part of x 2^(4 ks - 1); the scaling and the
+// subtraction of the floor are exact), then three angle doublings (error
+// below 1e-6, far under the BF16 step).  kg = 3: the normal and the
+// direction (nd, read only when ok) in block 0, zero padding after.
+__device__ __forceinline__ bf16x8 feature_block(int kg, int ks, float x, const float* nd, bool ok) {
+    bf16x8 v;
+    if (kg < 3) {
+        const float y = ldexpf(x, 4 * ks - 1);
         float sn, cs;
         sincos_2pi(y - floorf(y), sn, cs);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            sc[8 * a + 2 * j] = sn;
-            sc[8 * a + 2 * j + 1] = cs;
+            v[2 * j] = tobf(sn);
+            v[2 * j + 1] = tobf(cs);
             const float s2 = 2.0f * sn * cs, c2 = (cs - sn) * (cs + sn);
             sn = s2;
             cs = c2;
         }
-    }
-}
-
-// Features of one sample, group g of four: g < 3 -> the 24 sin/cos features of
-// axis g; g = 3 -> normal, direction, zero padding (24 values).
-__device__ __forceinline__ void encode_group(const float* f9, int g, const Geo& G, __bf16* row) {
-    if (g < 3) {
-        float sc[24];
-        encode_axis((f9[g] - G.lo[g]) * G.inv[g], sc);
-#pragma unroll
-        for (int j = 0; j < 24; ++j) row[g * 24 + j] = tobf(sc[j]);
     } else {
 #pragma unroll
-        for (int j = 0; j < 6; ++j) row[72 + j] = tobf(f9[3 + j]);
-#pragma unroll
-        for (int j = 78; j < 96; ++j) row[j] = tobf(0.0f);
+        for (int j = 0; j < 8; ++j) v[j] = tobf(ks == 0 && j < 6 && ok ? nd[j] : 0.0f);
     }
+    return v;
 }
 
-// Training buffers in LDS are row-major with rows padded by 8 BF16 (16 bytes),
-// so that the 16 rows of an operand fragment start in different LDS banks:
-// features [sample][XS], activations and their gradients [sample][HS], the
-// output-layer gradient [sample][D0S], weights [n][KD[l] + 8].  The gradients
-// are also kept transposed, [feature][TS], for the weight-gradient products,
-// which sum over samples.
-constexpr int XS = 96 + 8, HS = 64 + 8, D0S = 16 + 8, TS = 64 + 8;
-constexpr int offwp(int l) { int o = 0; for (int i = 0; i < l; ++i) o += ND[i] * (KD[i] + 8); return o; }
-constexpr int NWP = offwp(L);
+// Training buffers in LDS: activations and gradients transposed,
+// [feature][sample] with rows of TS, and gradients also [sample][feature]
+// with rows of HS; rows padded by 8 BF16 (16 bytes) so that the 16 rows of an
+// operand fragment start in different LDS banks.
+constexpr int HS = 64 + 8, TS = 64 + 8;
 
-// 16x16 tile of C = A * B over K (multiple of 32; terms k >= kvalid are zero,
-// kvalid a multiple of 8).  A(m, k) = A[m * sAm + k * sAk], B(k, n) =
-// B[k * sBk + n * sBn].  Lane l supplies A row m0 + l%16 and B column
-// n0 + l%16, k = 8*(l/16) + j; on return acc[i] = C(m0 + 4*(l/16) + i, n0 +
-// l%16).  AV / BV: the operand's k is contiguous (sAk / sBk = 1), so a lane's
-// eight values are one 16-byte load; otherwise they are gathered one by one.
-template <bool AV, bool BV>
-__device__ __forceinline__ f32x4 tile_mm(const __bf16* A, int sAm, int sAk, const __bf16* B, int sBk, int sBn,
-                                         int m0, int n0, int K, int kvalid, int lane) {
+// 16x16 tile of C = A * B over K (multiple of 32), both operands with k
+// contiguous: A(m, k) = A[m * sAm + k], B(k, n) = B[n * sBn + k].  Lane l
+// supplies A row m0 + l%16 and B column n0 + l%16, k = 8*(l/16) + j (one
+// 16-byte load each); on return acc[i] = C(m0 + 4*(l/16) + i, n0 + l%16).
+__device__ __forceinline__ f32x4 tile_mm(const __bf16* A, int sAm, const __bf16* B, int sBn, int m0, int n0, int K,
+                                         int lane) {
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
     const int mi = lane & 15, kg = lane >> 4;
     for (int k0 = 0; k0 < K; k0 += 32) {
-        bf16x8 a, b;
-        const int kb = k0 + kg * 8;
-        if (AV) {
-            if (kb < kvalid) a = *(const bf16x8*)(A + (m0 + mi) * sAm + kb);
-            else a = bf16x8{};
-        }
-        if (BV) {
-            if (kb < kvalid) b = *(const bf16x8*)(B + (n0 + mi) * sBn + kb);
-            else b = bf16x8{};
-        }
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int k = kb + j;
-            if (!AV) a[j] = k < kvalid ? A[(m0 + mi) * sAm + k * sAk] : tobf(0.0f);
-            if (!BV) b[j] = k < kvalid ? B[k * sBk + (n0 + mi) * sBn] : tobf(0.0f);
-        }
+        const bf16x8 a = *(const bf16x8*)(A + (m0 + mi) * sAm + k0 + 8 * kg);
+        const bf16x8 b = *(const bf16x8*)(B + (n0 + mi) * sBn + k0 + 8 * kg);
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
     }
     return acc;
-}
-
-// weights, row-major [n][k] per layer -> rows padded to KD[l] + 8 in LDS
-__device__ __forceinline__ void load_weights_padded(const __bf16* __restrict__ Wb, __bf16* sW) {
-#pragma unroll
-    for (int l = 0; l < L; ++l) {
-        const int K = KD[l], chunks = ND[l] * (K / 8);
-        for (int c = threadIdx.x; c < chunks; c += NT) {
-            const int n = c / (K / 8), k8 = (c % (K / 8)) * 8;
-            *(uint4*)(sW + offwp(l) + n * (K + 8) + k8) = *(const uint4*)(Wb + offwb(l) + n * K + k8);
-        }
-    }
-}
-
-__device__ __forceinline__ void encode_tile(const float* __restrict__ feats, uint32_t s0, uint32_t n, const Geo& G,
-                                            __bf16* sX) {
-    const int s = threadIdx.x >> 2, g = threadIdx.x & 3;
-    float f9[9];
-    const bool valid = s0 + s < n;
-#pragma unroll
-    for (int j = 0; j < 9; ++j) f9[j] = valid ? feats[(size_t)(s0 + s) * 9 + j] : 0.0f;
-    if (valid) {
-        encode_group(f9, g, G, sX + s * XS);
-    } else {
-        for (int j = 0; j < 24; ++j) sX[s * XS + g * 24 + j] = tobf(0.0f);
-    }
-}
-
-// Forward through the hidden layers: layer l reads `in` (width KD[l]) and
-// writes bf16 ReLU activations to out(l).
-template <typename OutFn>
-__device__ __forceinline__ void hidden_forward(const __bf16* sW, const float* __restrict__ Wf, const __bf16* sX,
-                                               OutFn out) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int mi = lane & 15, kg = lane >> 4;
-    const __bf16* in = sX;
-    int in_stride = XS;
-    for (int l = 0; l < L - 1; ++l) {
-        __bf16* o = out(l);
-        const int K = KD[l];
-        for (int nt = 0; nt < ND[l] / 16; ++nt) {
-            f32x4 acc = tile_mm<true, true>(in, in_stride, 1, sW + offwp(l), 1, K + 8, 16 * w, 16 * nt, K, K, lane);
-            const int col = 16 * nt + mi;
-            const float b = Wf[offb(l) + col];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int row = 16 * w + 4 * kg + i;
-                o[row * HS + col] = tobf(fmaxf(acc[i] + b, 0.0f));
-            }
-        }
-        __syncthreads();
-        in = o;
-        in_stride = HS;
-    }
 }
 
 // ---- inference ------------------------------------------------------------
@@ -258,15 +176,15 @@ __device__ __forceinline__ void load_weights_blocked(const __bf16* __restrict__ 
 
 // acc[mt][nt] = (FIRST ? bias[nt] : acc[mt][nt]) + W[16 nt.., block ks] . X[mt]^T
 // for the wave's samples (the bias enters as the first product's addend)
-template <int N, int NT_, bool FIRST>
-__device__ __forceinline__ void layer_block(const __bf16* sWl, int ks, const bf16x8 (&x)[INF_MT], int lane,
-                                            const f32x4 (&bias)[NT_], f32x4 (&acc)[INF_MT][NT_]) {
+template <int N, int NT_, bool FIRST, int MT = INF_MT>
+__device__ __forceinline__ void layer_block(const __bf16* sWl, int ks, const bf16x8 (&x)[MT], int lane,
+                                            const f32x4 (&bias)[NT_], f32x4 (&acc)[MT][NT_]) {
     const int mi = lane & 15, kg = lane >> 4;
     bf16x8 w[NT_];
 #pragma unroll
     for (int nt = 0; nt < NT_; ++nt) w[nt] = *(const bf16x8*)(sWl + blk(16 * nt + mi, 32 * ks + 8 * kg, N));
 #pragma unroll
-    for (int mt = 0; mt < INF_MT; ++mt)
+    for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
         for (int nt = 0; nt < NT_; ++nt)
             acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[nt], x[mt], FIRST ? bias[nt] : acc[mt][nt], 0,
@@ -316,26 +234,8 @@ __global__ void __launch_bounds__(INF_NT) __attribute__((amdgpu_waves_per_eu(3))
             bf16x8 x[INF_MT];
 #pragma unroll
             for (int mt = 0; mt < INF_MT; ++mt) {
-                if (kg < 3) {
-                    // sin / cos of pi x 2^(4 ks + j), j = 0..3 (encode_axis, anchor ks)
-                    const float y = ldexpf(xa[mt], 4 * ks - 1);
-                    float sn, cs;
-                    sincos_2pi(y - floorf(y), sn, cs);
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        x[mt][2 * j] = tobf(sn);
-                        x[mt][2 * j + 1] = tobf(cs);
-                        const float s2 = 2.0f * sn * cs, c2 = (cs - sn) * (cs + sn);
-                        sn = s2;
-                        cs = c2;
-                    }
-                } else {
-                    // group 3: normal and direction in block 0, zero padding after
-                    const bool ok = s0 + 16 * mt + mi < n;
-                    const float* f = feats + (size_t)(s0 + 16 * mt + mi) * 9 + 3;
-#pragma unroll
-                    for (int j = 0; j < 8; ++j) x[mt][j] = tobf(ks == 0 && j < 6 && ok ? f[j] : 0.0f);
-                }
+                const uint32_t sm = s0 + 16 * mt + mi;
+                x[mt] = feature_block(kg, ks, xa[mt], feats + (size_t)sm * 9 + 3, sm < n);
             }
             if (ks == 0) layer_block<64, 4, true>(sW + offwb(0), ks, x, lane, bias, acc);
             else layer_block<64, 4, false>(sW + offwb(0), ks, x, lane, bias, acc);
@@ -382,23 +282,34 @@ __global__ void __launch_bounds__(INF_NT) __attribute__((amdgpu_waves_per_eu(3))
     }
 }
 
+// Slot of input k in layer l's blocked, permuted LDS rows (the inference
+// layout: see load_weights_blocked), for layers l >= 1.
+__device__ __forceinline__ int wslot(int k) {
+    const int q = k & 31;
+    return (k & ~31) + 8 * ((q & 15) >> 2) + 4 * (q >> 4) + (q & 3);
+}
+
+// One tile of 64 samples, 4 waves (wave w: samples 16 w .. 16 w + 15).
+// Forward as in inference, activations in registers, biases as the first
+// addend; every activation (and the encoding) is also written to LDS
+// transposed, [feature][sample], for the weight gradients, whose products
+// sum over samples.  The output-layer and hidden-layer gradients are kept in
+// both orientations: [sample][feature] for the activation gradients (their
+// row operand), [feature][sample] for the weight gradients.
 __global__ void __launch_bounds__(NT) k_mlp_train(const float* __restrict__ feats, const float* __restrict__ targets,
                                                   uint32_t n, const __bf16* __restrict__ Wb,
                                                   const float* __restrict__ Wf, Geo G, float* __restrict__ partial,
                                                   float* __restrict__ loss_part) {
-    __shared__ __attribute__((aligned(16))) __bf16 sW[NWP];
-    __shared__ __attribute__((aligned(16))) __bf16 sX[TILE * XS];
-    __shared__ __attribute__((aligned(16))) __bf16 sH[L - 1][TILE * HS];
-    __shared__ __attribute__((aligned(16))) __bf16 sD[2][TILE * HS];
-    __shared__ __attribute__((aligned(16))) __bf16 sDT[2][64 * TS];  // sD transposed
+    __shared__ __attribute__((aligned(16))) __bf16 sW[NWB];             // inference layout
+    __shared__ __attribute__((aligned(16))) __bf16 sXT[96 * TS];        // encoding, [feature][sample]
+    __shared__ __attribute__((aligned(16))) __bf16 sHT[L - 1][64 * TS];  // ReLU activations, [feature][sample]
+    __shared__ __attribute__((aligned(16))) __bf16 sD[2][TILE * HS];     // gradients, [sample][feature]
+    __shared__ __attribute__((aligned(16))) __bf16 sDT[2][64 * TS];      // gradients, [feature][sample]
     __shared__ float sLoss[NT / 64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int mi = lane & 15, kg = lane >> 4;
     float* part = partial + (size_t)blockIdx.x * NPARAM;
-    load_weights_padded(Wb, sW);
-    // tiles blockIdx.x, + gridDim.x, ...: the workgroup's partial gradient is
-    // the sum over its tiles in that order (each element is always written by
-    // the same thread, so the read-modify-write needs no synchronisation)
+    load_weights_blocked(Wb, sW);
     float lacc = 0.0f;
     // this thread's share of the workgroup's gradient, summed over its tiles
     // in tile order in registers and written once at the end: the weight
@@ -413,108 +324,146 @@ __global__ void __launch_bounds__(NT) k_mlp_train(const float* __restrict__ feat
     for (int l = 0; l < L; ++l) gb[l] = 0.0f;
     const uint32_t tiles = (n + TILE - 1) / TILE;
     for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-    const uint32_t s0 = tile * TILE;
-    __syncthreads();  // the previous tile's backward pass is done with sX / sH / sD
-    encode_tile(feats, s0, n, G, sX);
-    __syncthreads();
-    hidden_forward(sW, Wf, sX, [&](int l) { return sH[l]; });
-
-    // output layer and loss: y = H4 W5^T + b5; dL/dy = 2 (y - t) / (y^2 + 0.01) / n
-    float lsum = 0.0f;
-    {
-        f32x4 acc = tile_mm<true, true>(sH[L - 2], HS, 1, sW + offwp(L - 1), 1, 72, 16 * w, 0, 64, 64, lane);
-        const int col = mi;
-        const float b = Wf[offb(L - 1) + col];
-        bf16x4 dyt;
+        // keeps the weight fragments' LDS loads inside the loop
+        __asm__ __volatile__("" ::: "memory");
+        const uint32_t s0 = tile * TILE;
+        const int sw = 16 * w + mi;             // this lane's sample in the tile
+        const bool valid = s0 + sw < n;
+        __syncthreads();  // the previous tile's backward pass is done with the buffers
+        // ---- forward
+        f32x4 acc[1][4];
+        f32x4 bias[4];
+        load_bias<4>(Wf, 0, kg, bias);
+        const float xa = (kg < 3 && valid) ? (feats[(size_t)(s0 + sw) * 9 + kg] - G.lo[kg]) * G.inv[kg] : 0.0f;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int row = 16 * w + 4 * kg + i;
-            const uint32_t s = s0 + row;
-            float dy = 0.0f;
-            if (col < N5_VALID && s < n) {
-                const float y = acc[i] + b;
-                const float t = targets[(size_t)s * 3 + col];
-                const float den = y * y + 0.01f;
-                const float r = y - t;
-                lsum += r * r / den;
-                dy = 2.0f * r / den / (float)n;
-            }
-            sD[0][row * D0S + col] = tobf(dy);
-            dyt[i] = tobf(dy);
+        for (int ks = 0; ks < 3; ++ks) {
+            bf16x8 x[1];
+            x[0] = feature_block(kg, ks, xa, feats + (size_t)(s0 + sw) * 9 + 3, valid);
+            // slot (ks, 8 kg + j) holds feature 24 kg + 8 ks + j
+#pragma unroll
+            for (int j = 0; j < 8; ++j) sXT[(24 * kg + 8 * ks + j) * TS + sw] = x[0][j];
+            if (ks == 0) layer_block<64, 4, true, 1>(sW + offwb(0), ks, x, lane, bias, acc);
+            else layer_block<64, 4, false, 1>(sW + offwb(0), ks, x, lane, bias, acc);
         }
-        *(bf16x4*)(sDT[0] + col * TS + 16 * w + 4 * kg) = dyt;
-    }
-    for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
-    if (lane == 0) sLoss[w] = lsum;
-    __syncthreads();
-    if (threadIdx.x == 0) lacc += sLoss[0] + sLoss[1] + sLoss[2] + sLoss[3];
-
-    // layer 5: dW5[n][k] = sum_s dY[s][n] H4[s][k] (16 x 64), one 16-column tile per wave
-    {
-        const f32x4 acc = tile_mm<true, false>(sDT[0], TS, 1, sH[L - 2], HS, 1, 0, 16 * w, 64, 64, lane);
+        f32x4 o5[1][1];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) gw[GW_OFF[L - 1]][i] += acc[i];
-    }
-    if (threadIdx.x < 16) {
-        float s = 0.0f;
-        for (int r = 0; r < TILE; ++r) s += (float)sD[0][r * D0S + threadIdx.x];
-        gb[L - 1] += s;
-    }
-    // dH4[s][k] = sum_n dY[s][n] W5[n][k] (K = 16 valid of 32), masked by H4 > 0 -> sD[1]
-    for (int nt = 0; nt < 4; ++nt) {
-        f32x4 acc = tile_mm<true, false>(sD[0], D0S, 1, sW + offwp(L - 1), 72, 1, 16 * w, 16 * nt, 32, 16, lane);
-        const int col = 16 * nt + mi;
-        bf16x4 gt;
+        for (int l = 1; l < L; ++l) {
+            bf16x8 x[2][1];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int row = 16 * w + 4 * kg + i;
-            const float h = (float)sH[L - 2][row * HS + col];
-            sD[1][row * HS + col] = gt[i] = tobf(h > 0.0f ? acc[i] : 0.0f);
-        }
-        *(bf16x4*)(sDT[1] + col * TS + 16 * w + 4 * kg) = gt;
-    }
-    __syncthreads();
-
-    int cur = 1;
-#pragma unroll
-    for (int l = L - 2; l >= 0; --l) {
-        const __bf16* dZ = sD[cur];
-        const __bf16* dZT = sDT[cur];
-        const __bf16* in = (l == 0) ? sX : sH[l - 1];
-        const int in_stride = (l == 0) ? XS : HS;
-        const int K = KD[l];
-        // dW_l[n][k] = sum_s dZ[s][n] in[s][k]: rows n in 16-row tiles (one per wave), K / 16 column tiles
-#pragma unroll
-        for (int kt = 0; kt < K / 16; ++kt) {
-            const f32x4 acc = tile_mm<true, false>(dZT, TS, 1, in, in_stride, 1, 16 * w, 16 * kt, 64, 64, lane);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) gw[GW_OFF[l] + kt][i] += acc[i];
-        }
-        if (threadIdx.x < 64) {
-            float s = 0.0f;
-            for (int r = 0; r < TILE; ++r) s += (float)dZ[r * HS + threadIdx.x];
-            gb[l] += s;
-        }
-        if (l > 0) {
-            // dH_{l-1}[s][k] = sum_n dZ[s][n] W_l[n][k], masked by H_{l-1} > 0
-            __bf16* dst = sD[cur ^ 1];
-            __bf16* dstT = sDT[cur ^ 1];
             for (int nt = 0; nt < 4; ++nt) {
-                f32x4 acc = tile_mm<true, false>(dZ, HS, 1, sW + offwp(l), K + 8, 1, 16 * w, 16 * nt, 64, 64, lane);
+                const int ks = nt >> 1, h = 4 * (nt & 1);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const __bf16 v = tobf(relu(acc[0][nt][i]));
+                    x[ks][0][h + i] = v;
+                    sHT[l - 1][(16 * nt + 4 * kg + i) * TS + sw] = v;
+                }
+            }
+            if (l < L - 1) {
+                load_bias<4>(Wf, l, kg, bias);
+                layer_block<64, 4, true, 1>(sW + offwb(l), 0, x[0], lane, bias, acc);
+                layer_block<64, 4, false, 1>(sW + offwb(l), 1, x[1], lane, bias, acc);
+            } else {
+                f32x4 b5[1];
+                load_bias<1>(Wf, l, kg, b5);
+                layer_block<16, 1, true, 1>(sW + offwb(l), 0, x[0], lane, b5, o5);
+                layer_block<16, 1, false, 1>(sW + offwb(l), 1, x[1], lane, b5, o5);
+            }
+        }
+        // ---- loss: lane (mi, kg) holds outputs 4 kg + i of its sample;
+        // dL/dy = 2 (y - t) / (y^2 + 0.01) / n on the 3 valid outputs
+        float lsum = 0.0f;
+        {
+            bf16x4 dyt;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int c = 4 * kg + i;
+                float dy = 0.0f;
+                if (c < N5_VALID && valid) {
+                    const float y = o5[0][0][i];
+                    const float t = targets[(size_t)(s0 + sw) * 3 + c];
+                    const float den = y * y + 0.01f;
+                    const float r = y - t;
+                    lsum += r * r / den;
+                    dy = 2.0f * r / den / (float)n;
+                }
+                dyt[i] = tobf(dy);
+                sDT[0][c * TS + sw] = dyt[i];
+            }
+            *(bf16x4*)(sD[0] + sw * HS + 4 * kg) = dyt;
+        }
+        for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
+        if (lane == 0) sLoss[w] = lsum;
+        __syncthreads();
+        if (threadIdx.x == 0) lacc += sLoss[0] + sLoss[1] + sLoss[2] + sLoss[3];
+
+        // ---- backward
+        // layer 5: dW5[n][k] = sum_s dY[s][n] H4[s][k] (16 x 64), one 16-column tile per wave
+        {
+            const f32x4 g = tile_mm(sDT[0], TS, sHT[L - 2], TS, 0, 16 * w, 64, lane);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) gw[GW_OFF[L - 1]][i] += g[i];
+        }
+        if (threadIdx.x < 16) {
+            float sum = 0.0f;
+            for (int r = 0; r < TILE; ++r) sum += (float)sDT[0][threadIdx.x * TS + r];
+            gb[L - 1] += sum;
+        }
+        // dH_{l-1}[s][k] = sum_n dZ[s][n] W_l[n][k] (n < nvalid), masked by H_{l-1} > 0, into both layouts
+        auto act_grad = [&](int l, int nvalid, const __bf16* dZ, __bf16* dst, __bf16* dstT) {
+            const int N = ND[l];
+            const __bf16* Wl = sW + offwb(l);
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
                 const int col = 16 * nt + mi;
+                const int slot = wslot(col);
+                const __bf16* wcol = Wl + (slot >> 5) * N * 32 + (slot & 31);  // W_l[n][col] at wcol[n * 32]
+                f32x4 g = {0.0f, 0.0f, 0.0f, 0.0f};
+                for (int k0 = 0; k0 < ((nvalid + 31) & ~31); k0 += 32) {
+                    const int kb = k0 + 8 * kg;
+                    bf16x8 av, bv;
+                    if (kb < nvalid) av = *(const bf16x8*)(dZ + (16 * w + mi) * HS + kb);
+                    else av = bf16x8{};
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) bv[j] = kb + j < nvalid ? wcol[(kb + j) * 32] : tobf(0.0f);
+                    g = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, g, 0, 0, 0);
+                }
+                // g[i] = dH(sample 16 w + 4 kg + i, feature col)
+                const bf16x4 h = *(const bf16x4*)(sHT[l - 1] + col * TS + 16 * w + 4 * kg);
                 bf16x4 gt;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    const int row = 16 * w + 4 * kg + i;
-                    const float h = (float)sH[l - 1][row * HS + col];
-                    dst[row * HS + col] = gt[i] = tobf(h > 0.0f ? acc[i] : 0.0f);
+                    gt[i] = tobf((float)h[i] > 0.0f ? g[i] : 0.0f);
+                    dst[(16 * w + 4 * kg + i) * HS + col] = gt[i];
                 }
                 *(bf16x4*)(dstT + col * TS + 16 * w + 4 * kg) = gt;
             }
-            __syncthreads();
-            cur ^= 1;
+        };
+        act_grad(L - 1, N5_VALID, sD[0], sD[1], sDT[1]);
+        __syncthreads();
+        int cur = 1;
+#pragma unroll
+        for (int l = L - 2; l >= 0; --l) {
+            const __bf16* inT = (l == 0) ? sXT : sHT[l - 1];
+            const int K = KD[l];
+            // dW_l[n][k] = sum_s dZ[s][n] in[s][k]: rows n in 16-row tiles (one per wave), K / 16 column tiles
+#pragma unroll
+            for (int kt = 0; kt < K / 16; ++kt) {
+                const f32x4 g = tile_mm(sDT[cur], TS, inT, TS, 16 * w, 16 * kt, 64, lane);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) gw[GW_OFF[l] + kt][i] += g[i];
+            }
+            if (threadIdx.x < 64) {
+                float sum = 0.0f;
+                for (int r = 0; r < TILE; ++r) sum += (float)sDT[cur][threadIdx.x * TS + r];
+                gb[l] += sum;
+            }
+            if (l > 0) {
+                act_grad(l, 64, sD[cur], sD[cur ^ 1], sDT[cur ^ 1]);
+                __syncthreads();
+                cur ^= 1;
+            }
         }
-    }
     }  // tiles
     // the workgroup's partial gradient, written once
 #pragma unroll
