@@ -76,17 +76,20 @@ __device__ __forceinline__ float relu(float x) { return __int_as_float(max(__flo
 // Features of k block ks that lane group kg supplies for one sample (the
 // input layer's weight columns are stored in this order).  kg < 3: sin and
 // cos of pi x 2^(4 ks + j), j = 0..3, interleaved, for the sample's
-// coordinate x on axis kg normalised to the scene box: an exact anchor
-// (sincos_2pi of the fractional part of x 2^(4 ks - 1); the scaling and the
-// subtraction of the floor are exact), then three angle doublings (error
-// below 1e-6, far under the BF16 step).  kg = 3: the normal and the
-// direction (nd, read only when ok) in block 0, zero padding after.
+// coordinate x on axis kg normalised to the scene box: an anchor from the
+// hardware sine and cosine (v_sin_f32 / v_cos_f32, argument in revolutions)
+// of the fractional part of x 2^(4 ks - 1) (the scaling and the subtraction
+// of the floor are exact), then three angle doublings; the error stays far
+// under the BF16 step the features are rounded to (the inference and
+// gradient tests compare against the oracle's exact encoding).  kg = 3: the
+// normal and the direction (nd, read only when ok) in block 0, zero padding
+// after.
 __device__ __forceinline__ bf16x8 feature_block(int kg, int ks, float x, const float* nd, bool ok) {
     bf16x8 v;
     if (kg < 3) {
         const float y = ldexpf(x, 4 * ks - 1);
-        float sn, cs;
-        sincos_2pi(y - floorf(y), sn, cs);
+        const float f = y - floorf(y);  // revolutions
+        float sn = __builtin_amdgcn_sinf(f), cs = __builtin_amdgcn_cosf(f);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             v[2 * j] = tobf(sn);

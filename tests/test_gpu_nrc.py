@@ -66,7 +66,8 @@ def test_params_round_trip(cache):
 
 # Tolerances: the reference rounds inputs, weights, activations and
 # back-propagated gradients to BF16 exactly where the kernels do and sums in
-# float64; the kernels sum in FP32 MFMA order and encode with sinpif/cospif.
+# float64; the kernels sum in FP32 MFMA order and encode with the hardware
+# sine and cosine of three anchors and angle doubling.
 # Either can flip a BF16 rounding (2^-8 relative) now and then, so the bar is
 # well below the BF16 quantisation error itself (emulated vs exact network).
 @pytest.mark.parametrize("n", [1, 63, 64, 1000, 4097])
