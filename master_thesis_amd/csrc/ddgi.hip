@@ -79,26 +79,33 @@ __global__ void __launch_bounds__(DG_BLOCK) k_ddgi_resolve(uint32_t n, uint32_t 
 // One workgroup per probe: its surfels in LDS, one thread per tile texel.
 // Border texels compute the interior texel they mirror, so they hold the
 // same value.  first: no previous field to blend.
+// ppb probes per workgroup (their surfels side by side in LDS), one thread
+// per atlas texel; each texel sums its probe's rays in ray order.
 __global__ void __launch_bounds__(DG_BLOCK) k_ddgi_update(DdgiGrid g, const float* surf, float* irr, float* dep,
-                                                         uint32_t first) {
+                                                         uint32_t first, uint32_t ppb, uint32_t P) {
     __shared__ float ls[8 * MAX_RAYS];
-    const uint32_t p = blockIdx.x;
+    const uint32_t p0 = blockIdx.x * ppb;
     const uint32_t R = g.rays;
-    for (uint32_t k = threadIdx.x; k < 8 * R; k += blockDim.x) ls[k] = surf[(size_t)p * 8 * R + k];
+    const uint32_t np = min(ppb, P - p0);
+    for (uint32_t k = threadIdx.x; k < 8 * R * np; k += blockDim.x) ls[k] = surf[(size_t)p0 * 8 * R + k];
     __syncthreads();
     const uint32_t ts = g.side + 2;
-    for (uint32_t t = threadIdx.x; t < ts * ts; t += blockDim.x) {
-        const uint32_t tx = t % ts, ty = t / ts;
+    for (uint32_t t = threadIdx.x; t < ts * ts * np; t += blockDim.x) {
+        const uint32_t j = t / (ts * ts), tt = t % (ts * ts);
+        const uint32_t p = p0 + j;
+        const uint32_t tx = tt % ts, ty = tt / ts;
         uint32_t ix, iy;
         tile_source(g.side, tx, ty, ix, iy);
         const f3 nt = texel_dir(g.side, ix, iy);
+        const float* lp = ls + 8 * R * j;
         float sr = 0.0f, sgr = 0.0f, sb = 0.0f, tw = 0.0f, d0 = 0.0f, d1 = 0.0f, tdw = 0.0f;
         for (uint32_t i = 0; i < R; ++i) {
-            const float* s = ls + 8 * i;
+            const float* s = lp + 8 * i;
             float rd = g.max_dist;
             if (s[7] != 0.0f) rd = fminf(rd, s[6]);
             const float w = fmaxf(0.0f, dot(nt, mk(s[0], s[1], s[2])));
-            const float dw = powf(w, g.sharpness);
+            // rays behind the texel: w = 0, and 0^sharpness = 0 for a positive sharpness
+            const float dw = (w == 0.0f && g.sharpness > 0.0f) ? 0.0f : powf(w, g.sharpness);
             if (w > 1e-6f) {
                 sr = sr + w * s[3];
                 sgr = sgr + w * s[4];
@@ -170,8 +177,15 @@ void random_rotation(uint64_t seed, uint64_t update, float m[9]) {
 
 int launch_update(rt_ddgi* g) {
     rt_ctx* c = g->ctx;
-    hipLaunchKernelGGL(k_ddgi_update, dim3(g->P), dim3(DG_BLOCK), 0, c->stream, g->grid, (const float*)g->surf.ptr,
-                       (float*)g->irr.ptr, (float*)g->dep.ptr, g->updates == 0 ? 1u : 0u);
+    // probes per workgroup: as many as its threads have texels for and its
+    // LDS has surfels for
+    const uint32_t texels = (g->cfg.side + 2) * (g->cfg.side + 2);
+    uint32_t ppb = DG_BLOCK / texels;
+    if (ppb > MAX_RAYS / g->R) ppb = MAX_RAYS / g->R;
+    if (ppb < 1) ppb = 1;
+    hipLaunchKernelGGL(k_ddgi_update, dim3((g->P + ppb - 1) / ppb), dim3(DG_BLOCK), 0, c->stream, g->grid,
+                       (const float*)g->surf.ptr, (float*)g->irr.ptr, (float*)g->dep.ptr, g->updates == 0 ? 1u : 0u,
+                       ppb, g->P);
     RT_HIP(c, hipGetLastError());
     g->updates += 1;
     return RT_OK;
