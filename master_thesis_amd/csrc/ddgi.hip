@@ -76,18 +76,28 @@ __global__ void __launch_bounds__(DG_BLOCK) k_ddgi_resolve(uint32_t n, uint32_t 
     s[5] = s[5] + ab / fc;
 }
 
-// One workgroup per probe: its surfels in LDS, one thread per tile texel.
-// Border texels compute the interior texel they mirror, so they hold the
-// same value.  first: no previous field to blend.
+// w^s for a cosine w in [0, 1] and a sharpness s >= 0, as GLSL defines pow:
+// exp2(s * log2(w)), here on the hardware's base-2 logarithm and exponential
+// (v_log_f32, v_exp_f32).  s = 0 gives 1 and w = 0 gives 0, as powf does.
+__device__ inline float depth_weight(float w, float s) {
+    if (s == 0.0f) return 1.0f;
+    if (w == 0.0f) return 0.0f;
+    return __builtin_amdgcn_exp2f(s * __builtin_amdgcn_logf(w));
+}
+
 // ppb probes per workgroup (their surfels side by side in LDS), one thread
-// per atlas texel; each texel sums its probe's rays in ray order.
+// per atlas texel; each texel sums its probe's rays in ray order.  Border
+// texels compute the interior texel they mirror, so they hold the same
+// value.  first: no previous field to blend.
 __global__ void __launch_bounds__(DG_BLOCK) k_ddgi_update(DdgiGrid g, const float* surf, float* irr, float* dep,
                                                          uint32_t first, uint32_t ppb, uint32_t P) {
-    __shared__ float ls[8 * MAX_RAYS];
+    // a surfel is two float4: (dir.xyz, r), (g, b, distance, hit)
+    __shared__ float4 ls[2 * MAX_RAYS];
     const uint32_t p0 = blockIdx.x * ppb;
     const uint32_t R = g.rays;
     const uint32_t np = min(ppb, P - p0);
-    for (uint32_t k = threadIdx.x; k < 8 * R * np; k += blockDim.x) ls[k] = surf[(size_t)p0 * 8 * R + k];
+    const float4* src = reinterpret_cast<const float4*>(surf) + (size_t)p0 * 2 * R;
+    for (uint32_t k = threadIdx.x; k < 2 * R * np; k += blockDim.x) ls[k] = src[k];
     __syncthreads();
     const uint32_t ts = g.side + 2;
     for (uint32_t t = threadIdx.x; t < ts * ts * np; t += blockDim.x) {
@@ -97,19 +107,18 @@ __global__ void __launch_bounds__(DG_BLOCK) k_ddgi_update(DdgiGrid g, const floa
         uint32_t ix, iy;
         tile_source(g.side, tx, ty, ix, iy);
         const f3 nt = texel_dir(g.side, ix, iy);
-        const float* lp = ls + 8 * R * j;
+        const float4* lp = ls + 2 * R * j;
         float sr = 0.0f, sgr = 0.0f, sb = 0.0f, tw = 0.0f, d0 = 0.0f, d1 = 0.0f, tdw = 0.0f;
         for (uint32_t i = 0; i < R; ++i) {
-            const float* s = lp + 8 * i;
+            const float4 a = lp[2 * i], b = lp[2 * i + 1];
             float rd = g.max_dist;
-            if (s[7] != 0.0f) rd = fminf(rd, s[6]);
-            const float w = fmaxf(0.0f, dot(nt, mk(s[0], s[1], s[2])));
-            // rays behind the texel: w = 0, and 0^sharpness = 0 for a positive sharpness
-            const float dw = (w == 0.0f && g.sharpness > 0.0f) ? 0.0f : powf(w, g.sharpness);
+            if (b.w != 0.0f) rd = fminf(rd, b.z);
+            const float w = fmaxf(0.0f, dot(nt, mk(a.x, a.y, a.z)));
+            const float dw = depth_weight(w, g.sharpness);
             if (w > 1e-6f) {
-                sr = sr + w * s[3];
-                sgr = sgr + w * s[4];
-                sb = sb + w * s[5];
+                sr = sr + w * a.w;
+                sgr = sgr + w * b.x;
+                sb = sb + w * b.y;
                 tw = tw + w;
             }
             if (dw > 1e-6f) {

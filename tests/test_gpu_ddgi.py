@@ -37,9 +37,10 @@ def _random_surfels(n, seed):
     return s
 
 
-def test_update_from_surfels_matches_oracle(tracer):
+@pytest.mark.parametrize("sharpness", [0.0, 2.0, 50.0])
+def test_update_from_surfels_matches_oracle(tracer, sharpness):
     tracer.upload(scenes.cornell_box())
-    f = _field(tracer, hysteresis=0.6)
+    f = _field(tracer, hysteresis=0.6, sharpness=sharpness)
     n = f.num_probes * f.rays
     s1, s2 = _random_surfels(n, 1), _random_surfels(n, 2)
     f.update_surfels(s1)
@@ -49,8 +50,10 @@ def test_update_from_surfels_matches_oracle(tracer):
     o1 = O.ddgi_update(f.cfg, s1)
     o2 = O.ddgi_update(f.cfg, s2, *o1)
     for g, o in ((g1, o1), (g2, o2)):
-        # same sums in the same order: irradiance bit for bit; the depth
-        # weights go through powf (device and libm may differ by an ulp)
+        # same sums in the same order: irradiance bit for bit.  The depth
+        # weights are exp2(s log2 w) on the hardware's log and exp against
+        # the oracle's powf: the weighted moments agree within 5e-7 relative
+        # for sharpness 0.5 to 200 (exact at 0, where every weight is 1)
         np.testing.assert_array_equal(g[0], o[0])
         np.testing.assert_allclose(g[1], o[1], rtol=2e-6, atol=1e-6)
     f.close()
