@@ -615,7 +615,7 @@ def bench_nrc(args):
     """Learned radiance cache, equal-RMSE protocol.  Scene (default: the
     terrain, sun + sky, depth 8, 1920x1080), a light-sampled reference of
     --ref-spp samples (same expectation as the plain estimator; the plain
-    render's mean is reported beside it as the bias judge), then:
+    render's mean is reported beside it as the check on its bias), then:
       plain PT          the reference estimator, RMSE at 4 and 16 spp
       light sampling    the cache mode without the network, 1 and 4 spp
       cached            --frames frames of online training at 1 spp (about
@@ -721,8 +721,8 @@ def bench_nrc(args):
                                   "light_sampling_expected_rmse_64spp": round(
                                       float(np.hypot(res["light_sampling"]["c"] / 8.0,
                                                      res["light_sampling"]["c"] / np.sqrt(args.ref_spp))), 5)}}
-        # plain PT as the judge of the reference (and so of the cache's bias): frame means
-        res["judge"] = {"reference_mean": round(float(ref.mean()), 6), "plain_mean": round(float(plain_hi.mean()), 6),
+        # plain PT as the check on the reference (and so on the cache's bias): frame means
+        res["plain_check"] = {"reference_mean": round(float(ref.mean()), 6), "plain_mean": round(float(plain_hi.mean()), 6),
                         "plain_spp": max(16, args.ref_spp // 4)}
         # irradiance probes (the Vulkan renderer's DDGI term) in the cache's place: a grid over the scene's
         # bounds, one field update per frame (random rotation, multi-bounce, hysteresis), then the same

@@ -557,7 +557,7 @@ __global__ void __launch_bounds__(256) k_adj_reduce(const float* __restrict__ ro
 // bounce gets the sky without the sun.  A light hit that is not estimated
 // (seen directly, or after a mirror or refraction) contributes T * R_L, the
 // expectation of the light's own random factor.  Both forms have the same
-// expected pixel value, so the plain render stays the judge of the cache's
+// expected pixel value, so the plain render stays the check on the cache's
 // bias.
 __device__ __forceinline__ f3 light_factor(const rt_material& lm) {
     const float tdl = lm.diffuse_threshold;

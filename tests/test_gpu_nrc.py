@@ -233,7 +233,7 @@ def test_constant_cache_closed_room_is_pi_rho_c(tracer):
 def test_online_training_gain_and_bias(tracer):
     """Closed Cornell-style box (no path escapes), 256 x 256, depth 8.
     Reference: 8192 light-sampled spp, whose mean the plain render confirms
-    (the plain estimator is the judge).  After 200 frames of online training
+    (the plain estimator is the yardstick).  After 200 frames of online training
     at 1 spp, 4-spp frames that stop at the cache after the first bounce reach
     the RMSE of light-sampled frames without the network while tracing less
     than half their rays (extension plus shadow rays), and the cache's bias
