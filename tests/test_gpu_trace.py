@@ -86,6 +86,34 @@ def test_large_soup_vs_cpu_bvh(tracer):
     assert (got["tri"] != RT_MISS).sum() > 1000
 
 
+def test_terrain_full_size_vs_cpu_bvh(tracer):
+    """The 9.69M-triangle terrain at full size: primary rays of two views and
+    first diffuse bounces from their hits (the terrain ray-cast workload's
+    mix, at 320x180) agree bit for bit with the oracle's own BVH."""
+    sc = scenes.terrain_scene()
+    info = _upload(tracer, sc, 4)
+    assert info["num_triangles"] == sc.num_triangles > 9_000_000
+    prim = np.concatenate([scenes.pinhole_rays((42.0 * np.cos(a), 22.0, 42.0 * np.sin(a)), (0.0, 0.0, 0.0),
+                                               45.0, 320, 180) for a in (0.3, 2.4)])
+    h = tracer.trace(prim)
+    hit = np.nonzero(h["tri"] != RT_MISS)[0]
+    assert hit.size > 0.5 * prim.shape[0]
+    rng = np.random.default_rng(5)
+    p = prim["o"][hit].astype(np.float64) + h["t"][hit][:, None] * prim["d"][hit].astype(np.float64)
+    w = rng.normal(size=p.shape)
+    w /= np.linalg.norm(w, axis=1, keepdims=True)
+    w[:, 1] = np.abs(w[:, 1])  # upper hemisphere: bounces off the ground and the spheres' tops
+    sec = np.zeros(hit.size, RAY_DTYPE)
+    sec["o"] = (p + 0.01 * w).astype(np.float32)
+    sec["d"] = w.astype(np.float32)
+    sec["tmax"] = np.inf
+    rays = np.concatenate([prim, sec])
+    got = tracer.trace(rays)
+    want, _ = O.CpuBVH(sc).trace(rays)
+    _assert_same_hits(got, want, "terrain")
+    assert (got["tri"][prim.shape[0]:] != RT_MISS).sum() > 1000
+
+
 def test_edge_case_rays(tracer):
     sc = scenes.cornell_box()
     _upload(tracer, sc)
