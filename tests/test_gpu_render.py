@@ -197,6 +197,24 @@ def test_multi_view_with_sun_matches_oracle(tracer):
     assert not np.array_equal(rgb[0], rgb[1])
 
 
+def test_terrain_full_size_deterministic_bit_exact(tracer):
+    """The 9.69M-triangle terrain with the terrain workload's sun and sky, at
+    full size: two views of 64x36, 2 spp, depth 8, deterministic mode, every
+    pixel bit-identical to the oracle's render (which builds its own BVH)."""
+    from master_thesis_amd._lib import RT_RENDER_SKY, RT_RENDER_SUN
+    sc = scenes.terrain_scene()
+    _setup(tracer, sc)
+    cams = scenes.terrain_cameras(2, 64, 36)
+    s = make_settings(spp=2, max_depth=8, seed=5, flags=RT_RENDER_DETERMINISTIC | RT_RENDER_SKY | RT_RENDER_SUN,
+                      sky=(0.45, 0.55, 0.75), sun_dir=(0.4, 0.8, 0.3), sun_radius_deg=5.0,
+                      sun_radiance=(40.0, 36.0, 30.0))
+    rgb, rgb8 = tracer.render_views(cams, s, want_rgb8=True)
+    want, want8 = O.render_views(sc, cams, s)
+    assert np.array_equal(rgb.view(np.uint32), want.view(np.uint32))
+    assert np.array_equal(rgb8, want8)
+    assert rgb.mean() > 0.1
+
+
 # ---- the bidirectional tracer
 
 @pytest.mark.parametrize("vj,lj,md", [(5, 5, 5), (3, 4, 2), (1, 2, 1)])
