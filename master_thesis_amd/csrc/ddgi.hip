@@ -79,10 +79,14 @@ __global__ void __launch_bounds__(DG_BLOCK) k_ddgi_resolve(uint32_t n, uint32_t 
 // w^s for a cosine w in [0, 1] and a sharpness s >= 0, as GLSL defines pow:
 // exp2(s * log2(w)), here on the hardware's base-2 logarithm and exponential
 // (v_log_f32, v_exp_f32).  s = 0 gives 1 and w = 0 gives 0, as powf does.
+// A subnormal w is scaled by 2^64 before the logarithm (the instruction
+// takes normal inputs), so a small s still weights such a ray as powf would.
 __device__ inline float depth_weight(float w, float s) {
     if (s == 0.0f) return 1.0f;
     if (w == 0.0f) return 0.0f;
-    return __builtin_amdgcn_exp2f(s * __builtin_amdgcn_logf(w));
+    const bool sub = w < 1.17549435e-38f;
+    const float lw = __builtin_amdgcn_logf(sub ? w * 18446744073709551616.0f : w) - (sub ? 64.0f : 0.0f);
+    return __builtin_amdgcn_exp2f(s * lw);
 }
 
 // ppb probes per workgroup (their surfels side by side in LDS), one thread

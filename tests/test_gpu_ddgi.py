@@ -37,7 +37,7 @@ def _random_surfels(n, seed):
     return s
 
 
-@pytest.mark.parametrize("sharpness", [0.0, 2.0, 50.0])
+@pytest.mark.parametrize("sharpness", [0.0, 0.05, 2.0, 50.0])
 def test_update_from_surfels_matches_oracle(tracer, sharpness):
     tracer.upload(scenes.cornell_box())
     f = _field(tracer, hysteresis=0.6, sharpness=sharpness)
@@ -53,7 +53,7 @@ def test_update_from_surfels_matches_oracle(tracer, sharpness):
         # same sums in the same order: irradiance bit for bit.  The depth
         # weights are exp2(s log2 w) on the hardware's log and exp against
         # the oracle's powf: the weighted moments agree within 5e-7 relative
-        # for sharpness 0.5 to 200 (exact at 0, where every weight is 1)
+        # for sharpness 0.05 to 200 (exact at 0, where every weight is 1)
         np.testing.assert_array_equal(g[0], o[0])
         np.testing.assert_allclose(g[1], o[1], rtol=2e-6, atol=1e-6)
     f.close()
