@@ -78,3 +78,16 @@ def test_host_errors_are_reported(tmp_path):
         H.load_scene(str(tmp_path / "missing.scene"), str(tmp_path))
     with pytest.raises(L.RTError, match="fov"):
         H.camera((0, 0, 0), 1.0, 0.0, 0.0, 8, 8)
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    """Without the built HIP library the product path raises at once; it has
+    no CPU fallback to take instead."""
+    import os
+    import subprocess
+    import sys
+    env = dict(os.environ, MTRT_LIB=str(tmp_path / "absent" / "libmtrt.so"))
+    code = "from master_thesis_amd.tracer import Tracer\nTracer(0)\n"
+    p = subprocess.run([sys.executable, "-c", code], cwd=ROOT, env=env, capture_output=True, text=True, timeout=300)
+    assert p.returncode != 0
+    assert "is not built" in p.stderr
