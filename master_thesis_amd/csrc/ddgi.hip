@@ -27,7 +27,7 @@ struct rt_ddgi {
     uint32_t P = 0, R = 0, W = 0, H = 0;
     uint64_t updates = 0;
     float rot[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-    DevBuf irr, dep, surf, rays, hits, sh_ray, sh_w, sh_occ, io_in, io_out;
+    DevBuf irr, dep, atl, surf, rays, hits, sh_ray, sh_w, sh_occ, io_in, io_out;  // atl: texel records of irr + dep
 };
 
 namespace {
@@ -94,7 +94,7 @@ __device__ inline float depth_weight(float w, float s) {
 // texels compute the interior texel they mirror, so they hold the same
 // value.  first: no previous field to blend.
 __global__ void __launch_bounds__(DG_BLOCK) k_ddgi_update(DdgiGrid g, const float* surf, float* irr, float* dep,
-                                                         uint32_t first, uint32_t ppb, uint32_t P) {
+                                                         float4* atl, uint32_t first, uint32_t ppb, uint32_t P) {
     // a surfel is two float4: (dir.xyz, r), (g, b, distance, hit)
     __shared__ float4 ls[2 * MAX_RAYS];
     const uint32_t p0 = blockIdx.x * ppb;
@@ -145,26 +145,36 @@ __global__ void __launch_bounds__(DG_BLOCK) k_ddgi_update(DdgiGrid g, const floa
         }
         irr[4 * q] = nr; irr[4 * q + 1] = ng; irr[4 * q + 2] = nb; irr[4 * q + 3] = 0.0f;
         dep[2 * q] = nd0; dep[2 * q + 1] = nd1;
+        atl[2 * q] = make_float4(nr, ng, nb, 0.0f);
+        atl[2 * q + 1] = make_float4(nd0, nd1, 0.0f, 0.0f);
     }
 }
 
-__global__ void __launch_bounds__(DG_BLOCK) k_ddgi_eval(DdgiGrid g, const float* irr, const float* dep,
-                                                       const float* f, uint32_t n, float* out) {
+// texel records from the planar atlases (after rt_ddgi_set_atlas)
+__global__ void __launch_bounds__(DG_BLOCK) k_ddgi_pack(uint32_t n, const float* irr, const float* dep, float4* atl) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    atl[2 * (size_t)i] = make_float4(irr[4 * (size_t)i], irr[4 * (size_t)i + 1], irr[4 * (size_t)i + 2], 0.0f);
+    atl[2 * (size_t)i + 1] = make_float4(dep[2 * (size_t)i], dep[2 * (size_t)i + 1], 0.0f, 0.0f);
+}
+
+__global__ void __launch_bounds__(DG_BLOCK) k_ddgi_eval(DdgiGrid g, const float4* atl, const float* f, uint32_t n,
+                                                       float* out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float* q = f + 9 * (size_t)i;
-    const f3 e = ddgi_lookup(g, irr, dep, mk(q[0], q[1], q[2]), mk(q[3], q[4], q[5]), mk(q[6], q[7], q[8]));
+    const f3 e = ddgi_lookup(g, atl, mk(q[0], q[1], q[2]), mk(q[3], q[4], q[5]), mk(q[6], q[7], q[8]));
     out[3 * (size_t)i] = e.x; out[3 * (size_t)i + 1] = e.y; out[3 * (size_t)i + 2] = e.z;
 }
 
 // acc[3 idx[i] + ch] += w[3 i + ch] * lookup_i[ch]
-__global__ void __launch_bounds__(DG_BLOCK) k_ddgi_scatter(DdgiGrid g, const float* irr, const float* dep,
-                                                          const float* f, const float* w, const uint32_t* idx,
-                                                          uint32_t n, float* acc) {
+__global__ void __launch_bounds__(DG_BLOCK) k_ddgi_scatter(DdgiGrid g, const float4* atl, const float* f,
+                                                          const float* w, const uint32_t* idx, uint32_t n,
+                                                          float* acc) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float* q = f + 9 * (size_t)i;
-    const f3 e = ddgi_lookup(g, irr, dep, mk(q[0], q[1], q[2]), mk(q[3], q[4], q[5]), mk(q[6], q[7], q[8]));
+    const f3 e = ddgi_lookup(g, atl, mk(q[0], q[1], q[2]), mk(q[3], q[4], q[5]), mk(q[6], q[7], q[8]));
     float* a = acc + 3 * (size_t)idx[i];
     atomicAdd(a, w[3 * (size_t)i] * e.x);
     atomicAdd(a + 1, w[3 * (size_t)i + 1] * e.y);
@@ -197,8 +207,8 @@ int launch_update(rt_ddgi* g) {
     if (ppb > MAX_RAYS / g->R) ppb = MAX_RAYS / g->R;
     if (ppb < 1) ppb = 1;
     hipLaunchKernelGGL(k_ddgi_update, dim3((g->P + ppb - 1) / ppb), dim3(DG_BLOCK), 0, c->stream, g->grid,
-                       (const float*)g->surf.ptr, (float*)g->irr.ptr, (float*)g->dep.ptr, g->updates == 0 ? 1u : 0u,
-                       ppb, g->P);
+                       (const float*)g->surf.ptr, (float*)g->irr.ptr, (float*)g->dep.ptr, (float4*)g->atl.ptr,
+                       g->updates == 0 ? 1u : 0u, ppb, g->P);
     RT_HIP(c, hipGetLastError());
     g->updates += 1;
     return RT_OK;
@@ -230,7 +240,7 @@ int ddgi_query_scatter(rt_ddgi* g, const float* d_feats, const float* d_w, const
     if (n == 0) return RT_OK;
     rt_ctx* c = g->ctx;
     hipLaunchKernelGGL(k_ddgi_scatter, dim3(blocks(n)), dim3(DG_BLOCK), 0, c->stream, g->grid,
-                       (const float*)g->irr.ptr, (const float*)g->dep.ptr, d_feats, d_w, d_idx, n, d_acc);
+                       (const float4*)g->atl.ptr, d_feats, d_w, d_idx, n, d_acc);
     RT_HIP(c, hipGetLastError());
     return RT_OK;
 }
@@ -242,7 +252,7 @@ void ddgi_detach(rt_ddgi* g) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    DevBuf* all[] = {&g->irr, &g->dep, &g->surf, &g->rays, &g->hits, &g->sh_ray, &g->sh_w, &g->sh_occ, &g->io_in, &g->io_out};
+    DevBuf* all[] = {&g->irr, &g->dep, &g->atl, &g->surf, &g->rays, &g->hits, &g->sh_ray, &g->sh_w, &g->sh_occ, &g->io_in, &g->io_out};
     for (DevBuf* b : all) {
         if (b->ptr) (void)hipFree(b->ptr);
         b->ptr = nullptr;
@@ -316,12 +326,14 @@ int rt_ddgi_create(rt_ctx* c, const rt_ddgi_config* cfg, rt_ddgi** out) {
     const size_t texels = (size_t)W * H;
     int rc;
     if ((rc = ensure(c, g->irr, 16 * texels)) || (rc = ensure(c, g->dep, 8 * texels)) ||
+        (rc = ensure(c, g->atl, 32 * texels)) ||
         (rc = ensure(c, g->surf, 32 * (size_t)P * g->R))) {
         rt_ddgi_destroy(g);
         return rc;
     }
     hipError_t e = hipMemsetAsync(g->irr.ptr, 0, 16 * texels, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(g->dep.ptr, 0, 8 * texels, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(g->atl.ptr, 0, 32 * texels, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(g->surf.ptr, 0, 32 * (size_t)P * g->R, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
@@ -372,8 +384,7 @@ int rt_ddgi_update(rt_ddgi* g, const rt_settings* s) {
     sh.rays = (const rt_ray*)g->rays.ptr;
     sh.hits = (const rt_hit*)g->hits.ptr;
     sh.surf = (float*)g->surf.ptr;
-    sh.irr = (const float*)g->irr.ptr;
-    sh.dep = (const float*)g->dep.ptr;
+    sh.atl = (const float4*)g->atl.ptr;
     sh.multi = (g->cfg.flags & RT_DDGI_MULTI_BOUNCE) && g->updates > 0 ? 1u : 0u;
     sh.update = (uint32_t)g->updates;
     sh.count = g->cfg.shadow_count;
@@ -457,6 +468,9 @@ int rt_ddgi_set_atlas(rt_ddgi* g, const float* irradiance, const float* depth, i
     const hipMemcpyKind kind = ptr_kind == RT_PTR_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
     RT_HIP(c, hipMemcpyAsync(g->irr.ptr, irradiance, 16 * texels, kind, c->stream));
     RT_HIP(c, hipMemcpyAsync(g->dep.ptr, depth, 8 * texels, kind, c->stream));
+    hipLaunchKernelGGL(k_ddgi_pack, dim3(blocks(texels)), dim3(DG_BLOCK), 0, c->stream, (uint32_t)texels,
+                       (const float*)g->irr.ptr, (const float*)g->dep.ptr, (float4*)g->atl.ptr);
+    RT_HIP(c, hipGetLastError());
     RT_HIP(c, hipStreamSynchronize(c->stream));
     if (g->updates == 0) g->updates = 1;  // later updates blend into the set field
     return RT_OK;
@@ -478,8 +492,8 @@ int rt_ddgi_eval(rt_ddgi* g, const float* samples, uint32_t n, float* out, int p
         if ((rc = ensure(c, g->io_out, 12 * (size_t)n))) return rc;
         dout = (float*)g->io_out.ptr;
     }
-    hipLaunchKernelGGL(k_ddgi_eval, dim3(blocks(n)), dim3(DG_BLOCK), 0, c->stream, g->grid, (const float*)g->irr.ptr,
-                       (const float*)g->dep.ptr, (const float*)df, n, dout);
+    hipLaunchKernelGGL(k_ddgi_eval, dim3(blocks(n)), dim3(DG_BLOCK), 0, c->stream, g->grid, (const float4*)g->atl.ptr,
+                       (const float*)df, n, dout);
     RT_HIP(c, hipGetLastError());
     if (ptr_kind == RT_PTR_HOST) {
         RT_HIP(c, hipMemcpyAsync(out, dout, 12 * (size_t)n, hipMemcpyDeviceToHost, c->stream));

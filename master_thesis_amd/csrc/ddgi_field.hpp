@@ -113,13 +113,15 @@ RT_HD size_t tile_texel(const DdgiGrid& g, uint32_t p, uint32_t tx, uint32_t ty)
     return (size_t)((p / g.tiles_x) * ts + ty) * g.atlas_w + (p % g.tiles_x) * ts + tx;
 }
 
-// Bilinear footprint of octahedral coordinate (ox, oy) in probe p's tile:
+// Bilinear footprint of octahedral coordinate (ox, oy) in the tile at tile
+// row trow, tile column tcol (probe (x, y, z) of the grid: z, x + y count_x):
 // texel centres at k + 0.5, the map spanning tile coordinates [1, side + 1].
+// Texel indices fit 32 bits (an atlas holds at most 2^28 texels).
 struct Bilin {
-    size_t t00, t10, t01, t11;
+    uint32_t t00, t10, t01, t11;
     float ax, ay;
 };
-RT_HD Bilin bilinear(const DdgiGrid& g, uint32_t p, float ox, float oy) {
+RT_HD Bilin bilinear(const DdgiGrid& g, uint32_t trow, uint32_t tcol, float ox, float oy) {
     const float fs = (float)g.side;
     float fx = (ox + 1.0f) * 0.5f * fs + 0.5f;
     float fy = (oy + 1.0f) * 0.5f * fs + 0.5f;
@@ -132,10 +134,13 @@ RT_HD Bilin bilinear(const DdgiGrid& g, uint32_t p, float ox, float oy) {
     const uint32_t ix = (uint32_t)x0, iy = (uint32_t)y0;
     const uint32_t ix1 = ix + 1 > g.side + 1 ? g.side + 1 : ix + 1;
     const uint32_t iy1 = iy + 1 > g.side + 1 ? g.side + 1 : iy + 1;
-    b.t00 = tile_texel(g, p, ix, iy);
-    b.t10 = tile_texel(g, p, ix1, iy);
-    b.t01 = tile_texel(g, p, ix, iy1);
-    b.t11 = tile_texel(g, p, ix1, iy1);
+    const uint32_t ts = g.side + 2;
+    const uint32_t r0 = (trow * ts + iy) * g.atlas_w + tcol * ts;
+    const uint32_t r1 = (trow * ts + iy1) * g.atlas_w + tcol * ts;
+    b.t00 = r0 + ix;
+    b.t10 = r0 + ix1;
+    b.t01 = r1 + ix;
+    b.t11 = r1 + ix1;
     return b;
 }
 RT_HD float blend(const Bilin& b, float v00, float v10, float v01, float v11) {
@@ -146,8 +151,9 @@ RT_HD float blend(const Bilin& b, float v00, float v10, float v01, float v11) {
 
 // getIrradiance (majercik2019_final_raygen.rgen:43-91) at hit x with shading
 // normal n, reached by a ray of direction vdir; energy_preservation applied.
-// irr: 4 floats per texel (RGB, unused), dep: 2 (mean, mean of squares).
-RT_HD f3 ddgi_lookup(const DdgiGrid& g, const float* irr, const float* dep, f3 x, f3 n, f3 vdir) {
+// atl: two float4 per texel, (R, G, B, 0) then (mean, mean of squares, 0, 0),
+// so the four texels of a bilinear footprint sit in two cache lines.
+RT_HD f3 ddgi_lookup(const DdgiGrid& g, const float4* atl, f3 x, f3 n, f3 vdir) {
     const float xs[3] = {x.x, x.y, x.z};
     uint32_t bc[3];
     float alpha[3];
@@ -168,7 +174,6 @@ RT_HD f3 ddgi_lookup(const DdgiGrid& g, const float* irr, const float* dep, f3 x
         const uint32_t off[3] = {i & 1u, (i >> 1) & 1u, (i >> 2) & 1u};
         uint32_t pc[3];
         for (int k = 0; k < 3; ++k) pc[k] = bc[k] + off[k] > g.count[k] - 1 ? g.count[k] - 1 : bc[k] + off[k];
-        const uint32_t pidx = pc[0] + pc[1] * g.count[0] + pc[2] * g.count[0] * g.count[1];
         const f3 pp = mk(g.spacing[0] * (float)pc[0] + g.start[0], g.spacing[1] * (float)pc[1] + g.start[1],
                          g.spacing[2] * (float)pc[2] + g.start[2]);
         const f3 ptp = add(sub(x, pp), bias);  // probeToPoint
@@ -181,10 +186,13 @@ RT_HD f3 ddgi_lookup(const DdgiGrid& g, const float* irr, const float* dep, f3 x
         w = w * (tx * ty * tz);
         float ox, oy;
         oct_encode(normalize_ref(mk(-dtp.x, -dtp.y, -dtp.z)), ox, oy);
-        const Bilin b = bilinear(g, pidx, ox, oy);
+        const Bilin b = bilinear(g, pc[2], pc[0] + pc[1] * g.count[0], ox, oy);
         const float dist = sqrtf(dot(ptp, ptp));
-        const float mean = blend(b, dep[2 * b.t00], dep[2 * b.t10], dep[2 * b.t01], dep[2 * b.t11]);
-        const float m2 = blend(b, dep[2 * b.t00 + 1], dep[2 * b.t10 + 1], dep[2 * b.t01 + 1], dep[2 * b.t11 + 1]);
+        const float4 i00 = atl[2 * b.t00], i10 = atl[2 * b.t10], i01 = atl[2 * b.t01], i11 = atl[2 * b.t11];
+        const float4 d00 = atl[2 * b.t00 + 1], d10 = atl[2 * b.t10 + 1], d01 = atl[2 * b.t01 + 1],
+                     d11 = atl[2 * b.t11 + 1];
+        const float mean = blend(b, d00.x, d10.x, d01.x, d11.x);
+        const float m2 = blend(b, d00.y, d10.y, d01.y, d11.y);
         const float var = fabsf(mean * mean - m2);
         const float dd = fmaxf(dist - mean, 0.0f);
         const float den = var + dd * dd;
@@ -192,9 +200,9 @@ RT_HD f3 ddgi_lookup(const DdgiGrid& g, const float* irr, const float* dep, f3 x
         cheb = fmaxf(cheb * cheb * cheb, 0.0f);
         if (ref_vis ? dist <= mean : dist > mean) w = w * cheb;
         w = fmaxf(0.0001f, w);
-        float ir = blend(b, irr[4 * b.t00], irr[4 * b.t10], irr[4 * b.t01], irr[4 * b.t11]);
-        float ig = blend(b, irr[4 * b.t00 + 1], irr[4 * b.t10 + 1], irr[4 * b.t01 + 1], irr[4 * b.t11 + 1]);
-        float ib = blend(b, irr[4 * b.t00 + 2], irr[4 * b.t10 + 2], irr[4 * b.t01 + 2], irr[4 * b.t11 + 2]);
+        float ir = blend(b, i00.x, i10.x, i01.x, i11.x);
+        float ig = blend(b, i00.y, i10.y, i01.y, i11.y);
+        float ib = blend(b, i00.z, i10.z, i01.z, i11.z);
         if (!linear) {
             ir = sqrtf(fmaxf(ir, 0.0f));
             ig = sqrtf(fmaxf(ig, 0.0f));

@@ -924,7 +924,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_ddgi_shade(PTArgs a, DdgiShade b) 
                 b.sh_w[3 * q] = C.x; b.sh_w[3 * q + 1] = C.y; b.sh_w[3 * q + 2] = C.z;
             }
             if (b.multi) {
-                const f3 e = ddgi_lookup(b.grid, b.irr, b.dep, p, nrm, d);
+                const f3 e = ddgi_lookup(b.grid, b.atl, p, nrm, d);
                 rad = mulv(scale(0.5f * m.diffuse_threshold, mk(m.color[0], m.color[1], m.color[2])), e);
             }
         }

@@ -165,8 +165,7 @@ struct DdgiShade {
     const rt_ray* rays;
     const rt_hit* hits;
     float* surf;              // n * 8
-    const float* irr;         // the field before this update (multi-bounce term)
-    const float* dep;
+    const float4* atl;        // the field before this update (multi-bounce term), texel records
     uint32_t multi;           // add the field's diffuse term at surface hits
     uint32_t update;          // update index: Philox counter word
     uint32_t count;           // light samples per ray
