@@ -23,6 +23,17 @@ struct DdgiGrid {
     uint32_t flags;                   // RT_DDGI_*
 };
 
+// GLSL's normalize in single precision, as the probe lookup's shaders compute
+// it: a * (1 / sqrt(a . a)), IEEE square root and division
+RT_HD f3 normalize_f(f3 a) {
+    const float m = sqrtf(dot(a, a));
+    if (m != 0.0f) {
+        const float f = 1.0f / m;
+        a = mk(a.x * f, a.y * f, a.z * f);
+    }
+    return a;
+}
+
 // octEncode (res/shader/helper.glsl:79-86)
 RT_HD void oct_encode(f3 v, float& ox, float& oy) {
     const float l1 = fabsf(v.x) + fabsf(v.y) + fabsf(v.z);
@@ -177,7 +188,7 @@ RT_HD f3 ddgi_lookup(const DdgiGrid& g, const float4* atl, f3 x, f3 n, f3 vdir) 
         const f3 pp = mk(g.spacing[0] * (float)pc[0] + g.start[0], g.spacing[1] * (float)pc[1] + g.start[1],
                          g.spacing[2] * (float)pc[2] + g.start[2]);
         const f3 ptp = add(sub(x, pp), bias);  // probeToPoint
-        const f3 dtp = normalize_ref(mk(-ptp.x, -ptp.y, -ptp.z));
+        const f3 dtp = normalize_f(mk(-ptp.x, -ptp.y, -ptp.z));
         float w = (dot(dtp, n) + 1.0f) * 0.5f;
         // lerp(1 - alpha, alpha, offset) per axis
         const float tx = off[0] ? alpha[0] : 1.0f - alpha[0];
@@ -185,7 +196,7 @@ RT_HD f3 ddgi_lookup(const DdgiGrid& g, const float4* atl, f3 x, f3 n, f3 vdir) 
         const float tz = off[2] ? alpha[2] : 1.0f - alpha[2];
         w = w * (tx * ty * tz);
         float ox, oy;
-        oct_encode(normalize_ref(mk(-dtp.x, -dtp.y, -dtp.z)), ox, oy);
+        oct_encode(normalize_f(mk(-dtp.x, -dtp.y, -dtp.z)), ox, oy);
         const Bilin b = bilinear(g, pc[2], pc[0] + pc[1] * g.count[0], ox, oy);
         const float dist = sqrtf(dot(ptp, ptp));
         const float4 i00 = atl[2 * b.t00], i10 = atl[2 * b.t10], i01 = atl[2 * b.t01], i11 = atl[2 * b.t11];

@@ -68,6 +68,19 @@ static inline v3 vnormalize(v3 a) {
     return a;
 }
 
+/* GLSL's normalize in single precision, as the probe lookup's shaders
+ * compute it: a * (1 / sqrt(a . a)), IEEE square root and division */
+static inline v3 vnormalize_f(v3 a) {
+    const float m = sqrtf(vdot(a, a));
+    if (m != 0.0f) {
+        const float f = 1.0f / m;
+        a.x = a.x * f;
+        a.y = a.y * f;
+        a.z = a.z * f;
+    }
+    return a;
+}
+
 /* ------------------------------------------------------------------ */
 /* worker pool: threads pull chunks of [0, n) from an atomic counter   */
 typedef void (*range_fn)(void* arg, uint32_t b, uint32_t e, int tid);
@@ -1566,7 +1579,7 @@ static v3 ddgi_lookup(const rt_ddgi_config* g, const float* irr, const float* de
                          g->probe_spacing[1] * (float)pc[1] + g->probe_start[1],
                          g->probe_spacing[2] * (float)pc[2] + g->probe_start[2]);
         const v3 ptp = vadd(vsub(x, pp), bias);
-        const v3 dtp = vnormalize(V3(-ptp.x, -ptp.y, -ptp.z));
+        const v3 dtp = vnormalize_f(V3(-ptp.x, -ptp.y, -ptp.z));
         float w = (vdot(dtp, n) + 1.0f) * 0.5f;
         const float tx = off[0] ? al[0] : 1.0f - al[0];
         const float ty = off[1] ? al[1] : 1.0f - al[1];
@@ -1574,7 +1587,7 @@ static v3 ddgi_lookup(const rt_ddgi_config* g, const float* irr, const float* de
         w = w * (tx * ty * tz);
         float ox, oy, ax, ay;
         size_t t[4];
-        oct_enc(vnormalize(V3(-dtp.x, -dtp.y, -dtp.z)), &ox, &oy);
+        oct_enc(vnormalize_f(V3(-dtp.x, -dtp.y, -dtp.z)), &ox, &oy);
         dg_bilinear(g, pidx, ox, oy, t, &ax, &ay);
         const float dist = sqrtf(vdot(ptp, ptp));
         const float mean = dg_blend(dep, 2, 0, t, ax, ay);
