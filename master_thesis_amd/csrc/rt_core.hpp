@@ -215,13 +215,21 @@ RT_HD f3 refract_ref(f3 I, f3 N, float eta) {
     return sub(ei, mn);
 }
 
+// sin(acos(x)) for the branch test of customRefract, as sqrt((1 - x)(1 + x))
+// in double: for a float x both factors are exact, so the result carries two
+// roundings and is the same on host and device (libm's sin and acos differ
+// between the two).  NaN for |x| > 1, as acos gives.  It stays within 2^-50
+// of glibc's sin(acos(x)) over [-1, 1] (tested), so the float angle it yields
+// can differ from the reference's only by one rounding near a float boundary.
+RT_HD double sin_acos_ref(double x) { return sqrt((1.0 - x) * (1.0 + x)); }
+
 RT_HD f3 custom_refract_ref(f3 I, f3 N, float eta) {
     if ((double)dot(N, I) > 0.0) {
         N = mk(-N.x, -N.y, -N.z);
     } else {
         eta = (float)(1.0 / (double)eta);
     }
-    float angle = (float)(sin(acos((double)dot(N, I))) * (double)eta);
+    float angle = (float)(sin_acos_ref((double)dot(N, I)) * (double)eta);
     if (-1.0f < angle && angle < 1.0f) return refract_ref(I, N, eta);
     return reflect_ref(I, N);
 }

@@ -143,6 +143,53 @@ def cornell_closed(light_color=(15.0, 15.0, 15.0)) -> Scene:
                  materials=sc.materials)
 
 
+SPECULAR_WEIGHTS = np.array([
+    # diffuse, reflect, transparent weight, refraction index per material
+    (1.0, 0.0, 0.0, 1.0),   # 0 white walls
+    (1.0, 0.0, 0.0, 1.0),   # 1 red
+    (1.0, 0.0, 0.0, 1.0),   # 2 green
+    (1.0, 0.0, 0.0, 1.0),   # 3 light
+    (0.0, 1.0, 0.0, 1.0),   # 4 mirror box
+    (0.0, 0.0, 1.0, 1.5),   # 5 glass ball
+    (0.5, 0.3, 0.2, 1.33),  # 6 mixed ball: all three branches
+], np.float32)
+
+
+def cornell_specular(light_color=(15.0, 15.0, 15.0), ball_level: int = 3) -> Scene:
+    """The Cornell-style box with the tall box as a mirror (reflect weight 1),
+    a glass ball (transparent weight 1, refraction index 1.5, as the
+    reference's cornell_box_with_ball.scene sets its ball) and a ball mixing
+    all three branches (weights 0.5 / 0.3 / 0.2, index 1.33).  The balls are
+    smooth-shaded icospheres (vertex normals = unit positions).  Materials
+    and their weights: SPECULAR_WEIGHTS."""
+    h = CORNELL_HALF
+    b = _Builder()
+    W, R, G, L, MIRROR, GLASS, MIXED = range(7)
+    s = 2 * h
+    b.quad((-h, -h, -h), (0, 0, s), (0, s, 0), (1, 0, 0), W)      # back wall x = -h
+    b.quad((-h, -h, -h), (s, 0, 0), (0, 0, s), (0, 1, 0), W)      # floor y = -h
+    b.quad((-h, h, -h), (0, 0, s), (s, 0, 0), (0, -1, 0), W)      # ceiling y = +h
+    b.quad((-h, -h, -h), (0, s, 0), (s, 0, 0), (0, 0, 1), G)      # wall z = -h (green)
+    b.quad((-h, -h, h), (s, 0, 0), (0, s, 0), (0, 0, -1), R)      # wall z = +h (red)
+    b.box((-1.85, -h, -2.05), (-0.20, -h + 3.30, -0.40), MIRROR)  # tall box
+    lh = 0.70
+    b.quad((-lh, h - 0.01, -lh), (0, 0, 2 * lh), (2 * lh, 0, 0), (0, -1, 0), L)  # light
+    v, f = icosphere(ball_level)
+    for centre, radius, m in (((0.9, -h + 0.9, 1.2), 0.9, GLASS), ((0.6, -h + 0.7, -1.3), 0.7, MIXED)):
+        n0 = b.nv
+        for p in v:
+            b.pos.append((np.asarray(centre) + radius * p).astype(np.float32))
+            b.nrm.append(p.astype(np.float32))
+        b.idx += [tuple(int(x) + n0 for x in t) for t in f]
+        b.mat += [m] * len(f)
+        b.nv += len(v)
+    mats = [material(WHITE), material(RED), material(GREEN), material(light_color, light=True)]
+    mats += [material(WHITE, *SPECULAR_WEIGHTS[MIRROR][:3]),
+             material((1.0, 1.0, 1.0), *SPECULAR_WEIGHTS[GLASS][:3], refraction_index=SPECULAR_WEIGHTS[GLASS][3]),
+             material((0.8, 0.6, 0.3), *SPECULAR_WEIGHTS[MIXED][:3], refraction_index=SPECULAR_WEIGHTS[MIXED][3])]
+    return b.scene(mats)
+
+
 def cornell_indirect(light_color=(15.0, 15.0, 15.0)) -> Scene:
     """The Cornell-style box lit only indirectly: the light is a quad facing
     the ceiling on top of a white slab (the lamp) 0.8 below it, so the

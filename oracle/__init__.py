@@ -32,6 +32,7 @@ def load() -> C.CDLL:
         lib.or_intersect.argtypes = [F, F, F, F, F, F]
         lib.or_intersect.restype = C.c_int
         lib.or_barycentric.argtypes = [F, F, F, F, F]
+        lib.or_reflect.argtypes = [F, F, F]
         lib.or_trace_brute.argtypes = [P, P, C.c_uint32, P, C.c_uint32, P, C.c_int]
         lib.or_bvh_build.argtypes = [P, P, C.c_uint32, C.c_int]
         lib.or_bvh_build.restype = P
@@ -82,6 +83,29 @@ def intersect(o, d, v0, v1, v2):
     out = np.zeros(3, np.float32)
     hit = load().or_intersect(*[fptr(a) for a in arrs], fptr(out))
     return bool(hit), out
+
+
+def reflect(I, N) -> np.ndarray:
+    i, n = (np.asarray(a, np.float32) for a in (I, N))
+    out = np.zeros(3, np.float32)
+    load().or_reflect(fptr(i), fptr(n), fptr(out))
+    return out
+
+
+def custom_refract(I, N, eta: float) -> np.ndarray:
+    lib = load()
+    lib.or_custom_refract.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float, C.POINTER(C.c_float)]
+    i, n = (np.asarray(a, np.float32) for a in (I, N))
+    out = np.zeros(3, np.float32)
+    lib.or_custom_refract(fptr(i), fptr(n), eta, fptr(out))
+    return out
+
+
+def sin_acos(x: float) -> float:
+    lib = load()
+    lib.or_sin_acos.argtypes = [C.c_double]
+    lib.or_sin_acos.restype = C.c_double
+    return lib.or_sin_acos(x)
 
 
 def barycentric(v0, v1, v2, p):

@@ -844,12 +844,25 @@ static v3 refract_ref(v3 I, v3 N, float eta) {
     v3 mn = V3((float)(m * (double)N.x), (float)(m * (double)N.y), (float)(m * (double)N.z));
     return vsub(ei, mn);
 }
+/* sin(acos(x)) as sqrt((1 - x)(1 + x)) in double: both factors are exact for
+ * a float x, so host and device round identically; NaN for |x| > 1 as acos.
+ * Within 2^-50 of glibc's sin(acos(x)) on [-1, 1] (tests/test_oracle.py). */
+double or_sin_acos(double x) { return sqrt((1.0 - x) * (1.0 + x)); }
 static v3 custom_refract_ref(v3 I, v3 N, float eta) {
     if ((double)vdot(N, I) > 0.0) N = V3(-N.x, -N.y, -N.z);
     else eta = (float)(1.0 / (double)eta);
-    float angle = (float)(sin(acos((double)vdot(N, I))) * (double)eta);
+    float angle = (float)(or_sin_acos((double)vdot(N, I)) * (double)eta);
     if (-1.0f < angle && angle < 1.0f) return refract_ref(I, N, eta);
     return reflect_ref(I, N);
+}
+/* vector.h:319-345 entry points for the tests */
+void or_reflect(const float I[3], const float N[3], float out[3]) {
+    v3 r = reflect_ref(V3(I[0], I[1], I[2]), V3(N[0], N[1], N[2]));
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+void or_custom_refract(const float I[3], const float N[3], float eta, float out[3]) {
+    v3 r = custom_refract_ref(V3(I[0], I[1], I[2]), V3(N[0], N[1], N[2]), eta);
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
 }
 
 /* ------------------------------------------------------------------ */

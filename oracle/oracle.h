@@ -74,6 +74,11 @@ void or_vec4_mat(const float m[16], const float v[4], float out[4]);
 /* Primary ray direction, graphic/path_tracer.cpp:19-23. */
 void or_primary_dir(const float view_inv[16], const float proj_inv[16], uint32_t px, uint32_t py, uint32_t w,
                     uint32_t h, float out[3]);
+/* reflect and customRefract, math/vector.h:319-345, with the reference's
+ * float/double promotions; or_sin_acos is customRefract's sin(acos(x)). */
+void or_reflect(const float I[3], const float N[3], float out[3]);
+void or_custom_refract(const float I[3], const float N[3], float eta, float out[3]);
+double or_sin_acos(double x);
 
 /* The diffuse path tracer, graphic/path_tracer.cpp:18-97 driven per pixel as
  * graphic/graphics_engine.cpp:59-97 does (atomic pixel counter, `threads`

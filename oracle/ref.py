@@ -26,7 +26,11 @@ def load() -> C.CDLL:
         P = C.c_void_p
         lib.ref_intersect.argtypes = [P] * 7
         lib.ref_intersect.restype = C.c_int
-        lib.ref_scene_create.argtypes = [P, P, P, P, C.c_uint32, P, P, C.c_uint32]
+        lib.ref_scene_create.argtypes = [P, P, P, P, C.c_uint32, P, P, C.c_uint32, P]
+        lib.ref_reflect.argtypes = [P, P, P]
+        lib.ref_scene_load.argtypes = [C.c_char_p, C.c_char_p]
+        lib.ref_scene_load.restype = P
+        lib.ref_custom_refract.argtypes = [P, P, C.c_float, P]
         lib.ref_scene_create.restype = P
         lib.ref_scene_free.argtypes = [P]
         lib.ref_scene_trace.argtypes = [P, P, C.c_uint32, P, P, P, P]
@@ -55,14 +59,26 @@ class RefScene:
     """The reference's Scene/GraphicsObject/Mesh built from world-space arrays
     (one object per material, identity transform)."""
 
-    def __init__(self, scene):
+    def __init__(self, scene, weights=None):
+        """weights: optional (M, 4) diffuse, reflect, transparent weights and
+        refraction index per material (default diffuse only)."""
         self.keep = [np.ascontiguousarray(a) for a in (scene.pos, scene.nrm, scene.idx, scene.tri_mat)]
         self.colors = np.ascontiguousarray(scene.materials["color"].reshape(-1), np.float32)
         self.lights = np.ascontiguousarray(scene.materials["light_source"], np.uint32)
+        self.weights = None if weights is None else np.ascontiguousarray(weights, np.float32).reshape(-1, 4)
         pos, nrm, idx, tm = self.keep
         self.h = load().ref_scene_create(pos.ctypes.data, nrm.ctypes.data, idx.ctypes.data, tm.ctypes.data,
                                          idx.shape[0], self.colors.ctypes.data, self.lights.ctypes.data,
-                                         self.lights.shape[0])
+                                         self.lights.shape[0],
+                                         None if self.weights is None else self.weights.ctypes.data)
+
+    @classmethod
+    def from_file(cls, scene_path: str, basepath: str) -> "RefScene":
+        """The scene as MeshManager reads it (meshes from basepath + ../res/obj/)."""
+        self = cls.__new__(cls)
+        self.keep = []
+        self.h = load().ref_scene_load(str(scene_path).encode(), str(basepath).encode())
+        return self
 
     def trace(self, rays: np.ndarray):
         n = rays.shape[0]
@@ -105,6 +121,20 @@ class RefScene:
             load().ref_scene_free(self.h)
         except Exception:
             pass
+
+
+def reflect(I, N) -> np.ndarray:
+    i, n = (np.ascontiguousarray(x, np.float32) for x in (I, N))
+    out = np.zeros(3, np.float32)
+    load().ref_reflect(i.ctypes.data, n.ctypes.data, out.ctypes.data)
+    return out
+
+
+def custom_refract(I, N, eta: float) -> np.ndarray:
+    i, n = (np.ascontiguousarray(x, np.float32) for x in (I, N))
+    out = np.zeros(3, np.float32)
+    load().ref_custom_refract(i.ctypes.data, n.ctypes.data, eta, out.ctypes.data)
+    return out
 
 
 def camera(position, angle0: float, angle1: float, w: int, h: int):

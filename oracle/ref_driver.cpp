@@ -31,8 +31,13 @@ struct RefScene {
     std::vector<Mesh*> meshes;
     std::vector<GraphicsObject*> objects;
     std::vector<GraphicsObject*> lights;
+    MeshManager* manager = nullptr;  // owns meshes and objects of a scene read from a file
     Scene scene;
     ~RefScene() {
+        if (manager) {
+            delete manager;
+            return;
+        }
         for (auto* o : objects) delete o;
         for (auto* m : meshes) delete m;
     }
@@ -69,10 +74,24 @@ int ref_intersect(const float* v0, const float* v1, const float* v2, const float
     return 1;
 }
 
+// reflect and customRefract (math/vector.h:319-345), the reference's own
+// templates on Vector3f.
+void ref_reflect(const float* I, const float* N, float* out) {
+    Vector3f r = reflect(v3(I), v3(N));
+    for (int k = 0; k < 3; ++k) out[k] = r[k];
+}
+void ref_custom_refract(const float* I, const float* N, float eta, float* out) {
+    Vector3f r = customRefract(v3(I), v3(N), eta);
+    for (int k = 0; k < 3; ++k) out[k] = r[k];
+}
+
 // A reference Scene from world-space arrays: one Mesh + GraphicsObject per
 // material index (identity transform), material thresholds from weights.
+// weights (may be NULL: diffuse only): per material diffuse, reflect and
+// transparent weight and refraction index, 4 floats, given to the object
+// before GraphicsObject::init derives its thresholds (graphics_object.cpp:48-51).
 void* ref_scene_create(const float* pos, const float* nrm, const uint32_t* idx, const uint32_t* tri_mat, uint32_t T,
-                       const float* colors, const uint32_t* is_light, uint32_t M) {
+                       const float* colors, const uint32_t* is_light, uint32_t M, const float* weights) {
     auto* s = new RefScene();
     for (uint32_t m = 0; m < M; ++m) {
         Mesh* mesh = new Mesh();
@@ -95,12 +114,34 @@ void* ref_scene_create(const float* pos, const float* nrm, const uint32_t* idx, 
         s->meshes.push_back(mesh);
         auto* o = new GraphicsObject(mesh, Vector3f({0.0f, 0.0f, 0.0f}));
         o->color = v3(colors + 3 * m);
+        if (weights) {
+            o->diffuseWeight = weights[4 * m + 0];
+            o->reflectWeight = weights[4 * m + 1];
+            o->transparentWeight = weights[4 * m + 2];
+            o->refractionIndex = weights[4 * m + 3];
+        }
         if (is_light[m]) {
             o->lightSource = true;
             s->lights.push_back(o);
         }
         o->init();
         s->objects.push_back(o);
+        s->scene.addObject(o);
+    }
+    s->scene.init();
+    return s;
+}
+
+// A reference Scene read from a .scene file by MeshManager, each object
+// initialised and added as GraphicsEngine::init does (graphics_engine.cpp:20-30).
+void* ref_scene_load(const char* scene_path, const char* basepath) {
+    auto* s = new RefScene();
+    s->manager = new MeshManager(basepath);
+    s->manager->createObjectsFromFile(scene_path);
+    for (GraphicsObject* o : s->manager->getCreatedObjects()) {
+        o->init();
+        s->objects.push_back(o);
+        if (o->lightSource) s->lights.push_back(o);
         s->scene.addObject(o);
     }
     s->scene.init();

@@ -34,6 +34,26 @@ def test_cornell_deterministic_bit_exact(tracer, spp, depth, seed):
     assert rgb.mean() > 0.05
 
 
+@pytest.mark.parametrize("spp,depth,seed", [(8, 6, 1), (5, 8, 2)])
+def test_specular_transmissive_deterministic_bit_exact(tracer, spp, depth, seed):
+    """reflect and customRefract on the device (path_tracer.cpp:62-71): the
+    box with a mirror box, a glass ball (index 1.5) and a ball mixing all
+    three branches, deterministic mode, every pixel bit-identical to the
+    oracle (same promotions to double, same sin(acos) form on both sides)."""
+    sc = scenes.cornell_specular()
+    _setup(tracer, sc)
+    cam = H.cornell_camera(48, 40)
+    s = make_settings(spp=spp, max_depth=depth, seed=seed, flags=RT_RENDER_DETERMINISTIC)
+    rgb, rgb8 = tracer.render(cam, s, want_rgb8=True)
+    want, want8 = O.render(sc, cam, s)
+    diff = np.nonzero(rgb.view(np.uint32) != want.view(np.uint32))
+    assert diff[0].size == 0, f"{diff[0].size} channels differ, first at {diff[0][:3]},{diff[1][:3]}"
+    assert np.array_equal(rgb8, want8)
+    assert rgb.mean() > 0.05
+    fast, _ = tracer.render(cam, make_settings(spp=spp, max_depth=depth, seed=seed))
+    assert np.allclose(fast, want, rtol=2e-5, atol=1e-5)
+
+
 def test_throughput_mode_matches_within_summation_error(tracer):
     sc = scenes.cornell_box()
     _setup(tracer, sc)
@@ -230,6 +250,21 @@ def test_bdpt_matches_oracle(tracer, vj, lj, md):
     s = make_settings(spp=8, max_depth=5, seed=17, sample_offset=3)
     got, _ = tracer.render_bdpt(cam, 8, vj, lj, md, seed=17, sample_offset=3)
     want = O.render_bdpt(sc, cam, s, vj, lj, md)
+    np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-6)
+    assert (got != 0).mean() > 0.5
+
+
+def test_bdpt_specular_matches_oracle(tracer):
+    """The bidirectional tracer's vision and light paths through the mirror
+    and the glass balls (bidirectional_path_tracer.cpp:104-110 take the same
+    branches) against the oracle."""
+    sc = scenes.cornell_specular()
+    tracer.upload(sc)
+    tracer.build(2)
+    cam = H.cornell_camera(40, 32)
+    s = make_settings(spp=8, max_depth=6, seed=23)
+    got, _ = tracer.render_bdpt(cam, 8, 6, 5, 6, seed=23)
+    want = O.render_bdpt(sc, cam, s, 6, 5, 6)
     np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-6)
     assert (got != 0).mean() > 0.5
 

@@ -122,6 +122,78 @@ def test_barycentric_closed_form():
         assert np.allclose(b, w, atol=2e-3)
 
 
+# ---------------------------------------------------------------- reflect / refract
+def _unit(v):
+    v = np.asarray(v, np.float64)
+    return (v / np.linalg.norm(v)).astype(np.float32)
+
+
+def test_reflect_closed_form():
+    """Mirror about the normal: the normal component flips, the tangential
+    one stays; reflecting twice gives the input back (within rounding)."""
+    I = _unit((1, -1, 0))
+    r = O.reflect(I, (0, 1, 0))
+    assert np.allclose(r, _unit((1, 1, 0)), atol=1e-7)
+    rng = np.random.default_rng(5)
+    for _ in range(500):
+        I, N = _unit(rng.normal(size=3)), _unit(rng.normal(size=3))
+        r = O.reflect(I, N)
+        assert abs(float(np.dot(r, N)) + float(np.dot(I, N))) < 1e-6
+        assert np.allclose(r - np.dot(r, N) * N, I - np.dot(I, N) * N, atol=1e-6)
+        assert np.allclose(O.reflect(r, N), I, atol=1e-6)
+
+
+def test_custom_refract_snell_and_total_internal_reflection():
+    """customRefract (math/vector.h:332-345): entering (n.i < 0) bends with
+    sin t = sin i / eta; leaving (n.i > 0) the normal is flipped and
+    sin t = eta sin i, or, past the critical angle sin i >= 1 / eta, the ray
+    is reflected.  The refracted ray is unit length, stays in the plane of
+    incidence and crosses the surface; a swapped normal flip or a missing
+    1 / eta fails these."""
+    rng = np.random.default_rng(6)
+    n_refract = n_tir = 0
+    for _ in range(3000):
+        N = _unit(rng.normal(size=3))
+        I = _unit(rng.normal(size=3))
+        eta = np.float32(rng.uniform(1.05, 2.4))
+        c = float(np.dot(N, I))
+        sin_i = math.sqrt(max(0.0, 1 - c * c))
+        out = O.custom_refract(I, N, eta)
+        ratio = (1 / float(eta)) if c <= 0 else float(eta)
+        if sin_i * ratio < 1 - 1e-5:
+            n_refract += 1
+            assert abs(np.linalg.norm(out) - 1) < 2e-6
+            side = -1.0 if c <= 0 else 1.0          # the refracted ray keeps going through
+            assert np.sign(np.dot(out, N)) == side
+            sin_t = np.linalg.norm(np.cross(out, N))
+            assert abs(sin_t - sin_i * ratio) < 2e-5
+            assert abs(np.dot(np.cross(I, N), out)) < 2e-5  # plane of incidence
+        elif sin_i * ratio > 1 + 1e-5:
+            n_tir += 1
+            assert c > 0
+            assert np.array_equal(out, O.reflect(I, N))
+    assert n_refract > 1500 and n_tir > 200
+    # normal incidence passes straight through either way
+    assert np.allclose(O.custom_refract((0, -1, 0), (0, 1, 0), np.float32(1.5)), (0, -1, 0), atol=1e-7)
+    assert np.allclose(O.custom_refract((0, 1, 0), (0, 1, 0), np.float32(1.5)), (0, 1, 0), atol=1e-7)
+
+
+def test_sin_acos_against_libm():
+    """customRefract's sin(acos(x)) as sqrt((1 - x)(1 + x)): within 2^-50 of
+    glibc's sin(acos(x)) for float x over [-1, 1], dense near both ends where
+    acos is ill-conditioned; NaN outside like acos."""
+    rng = np.random.default_rng(7)
+    one = np.float32(1.0)
+    xs = np.concatenate([rng.uniform(-1, 1, 50000).astype(np.float32),
+                         one - np.arange(4000, dtype=np.float32) * np.float32(2 ** -24),
+                         -one + np.arange(4000, dtype=np.float32) * np.float32(2 ** -24),
+                         np.float32([0.0, -0.0, 0.5, -0.5, 2 ** -30])])
+    worst = max(abs(O.sin_acos(float(x)) - math.sin(math.acos(float(x)))) for x in xs)
+    assert worst <= 2 ** -50, worst
+    assert O.sin_acos(1.0) == 0.0
+    assert math.isnan(O.sin_acos(1.0000001)) and math.isnan(O.sin_acos(-1.0000001))
+
+
 # ---------------------------------------------------------------- closest hit
 def _rays(o, d):
     r = np.zeros(len(o), RAY_DTYPE)
