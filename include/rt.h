@@ -217,6 +217,21 @@ int  rt_bvh_export(rt_ctx* ctx, rt_bvh4_node* nodes, uint32_t* num_nodes,
 int  rt_trace_closest(rt_ctx* ctx, const rt_ray* rays, uint32_t n,
                       rt_hit* hits, int ptr_kind, rt_trace_counters* counters);
 
+/* Allocation-free closest hit.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same results as
+ * rt_trace_closest; device pointers only (rays, hits and status on the
+ * context's device), enqueued on the context stream, never waits.  It calls
+ * no hipMalloc / hipFree: its traversal scratch (ray counters and per-lane
+ * stack spill) is a fixed array of RT_NOALLOC_SCRATCH_BYTES in device memory,
+ * one per GPU, held until the process ends; a BVH too deep for it returns
+ * RT_E_UNSUPPORTED.  Calls that overlap on one GPU (from any context) share
+ * that array and race: order them on one stream.  status (device, may be
+ * NULL) receives 0, or 1 / 2 when the tree turned out corrupt (stack or step
+ * bound exceeded). */
+#define RT_NOALLOC_SCRATCH_BYTES (32u << 20)
+int  rt_trace_closest_noalloc(rt_ctx* ctx, const rt_ray* rays, uint32_t n, rt_hit* hits, uint32_t* status);
+
 /* Occlusion: occluded[i] = 1 when ray i hits any triangle with t in
  * (max(tmin, 1e-7), tmax), else 0 (the same acceptance as rt_trace_closest,
  * so occluded[i] == (closest hit != RT_MISS)).  The walk ends at the first

@@ -24,6 +24,7 @@ RT_E_STATE = -4
 RT_E_UNSUPPORTED = -5
 RT_PTR_HOST = 0
 RT_PTR_DEVICE = 1
+RT_NOALLOC_SCRATCH_BYTES = 32 << 20  # rt_trace_closest_noalloc's fixed scratch per GPU
 RT_MISS = 0xFFFFFFFF
 RT_CHILD_EMPTY = 0xFFFFFFFF
 RT_LEAF_BIT = 0x80000000
@@ -176,6 +177,7 @@ SIGNATURES = {
     "rt_build": (C.c_int, [_P, C.POINTER(rt_build_opts), C.POINTER(rt_bvh_info)]),
     "rt_bvh_export": (C.c_int, [_P, _P, C.POINTER(C.c_uint32), _P, C.POINTER(C.c_uint32)]),
     "rt_trace_closest": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_int, C.POINTER(rt_trace_counters)]),
+    "rt_trace_closest_noalloc": (C.c_int, [_P, _P, C.c_uint32, _P, _P]),
     "rt_trace_any": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_int]),
     "rt_occluded": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_int]),
     "rt_render": (C.c_int, [_P, C.POINTER(rt_camera), C.POINTER(rt_settings), _P, _P, C.c_int]),

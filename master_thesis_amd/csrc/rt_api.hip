@@ -139,6 +139,11 @@ int rt_create(rt_ctx** out, int device) {
         return fail(nullptr, RT_E_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     }
     c->stream = c->own_stream;
+    if (noalloc_scratch_init(c) != RT_OK) {
+        const std::string msg = c->err;
+        rt_destroy(c);
+        return fail(nullptr, RT_E_DEVICE, "rt_create: " + msg);
+    }
     for (int k = 0; k < rt_ctx::NUM_EV; ++k) {
         e = hipEventCreate(&c->ev[k]);
         if (e != hipSuccess) {
@@ -392,6 +397,18 @@ int rt_trace_closest(rt_ctx* c, const rt_ray* rays, uint32_t n, rt_hit* hits, in
         }
     }
     return RT_OK;
+}
+
+int rt_trace_closest_noalloc(rt_ctx* c, const rt_ray* rays, uint32_t n, rt_hit* hits, uint32_t* status) {
+    if (!c) return RT_E_INVALID;
+    if (n > 0 && (!rays || !hits)) return fail(c, RT_E_INVALID, "rt_trace_closest_noalloc: NULL rays or hits");
+    if (!c->built) return fail(c, RT_E_STATE, "rt_trace_closest_noalloc: call rt_build first");
+    if (n == 0) return RT_OK;
+    DeviceGuard g(c->device);
+    // no timing events are recorded around this call (rt_get_timing keeps
+    // the last timed call's figures)
+    c->last_op = 0;
+    return trace_noalloc(c, rays, n, hits, status);
 }
 
 // any-hit queries: rays (32 B each) or segments (24 B each) -> one byte per query

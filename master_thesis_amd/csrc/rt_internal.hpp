@@ -76,6 +76,7 @@ struct rt_ctx {
     uint32_t trace_refill_min = 8;  // traversal: idle lanes that trigger a ray refill (MTRT_TRACE_REFILL)
     uint32_t trace_slice = 16384;   // traversal: rays per XCD slice, a multiple of 128 (MTRT_TRACE_SLICE)
     float bvh_leaf_alpha = 1.0f;    // build: node-step / triangle-step cost of the leaf test (MTRT_BVH_ALPHA)
+    uint32_t* noalloc_scratch = nullptr;  // this device's copy of the allocation-free trace's scratch array
     // path tracer state
     rt::DevBuf pt_state, pt_queue[2], pt_image, pt_img8, pt_dimg, pt_dcolor, pt_ctr, pt_partial, pt_cams;
     rt::DevBuf pt_nrc_query, pt_nrc_train;  // cache queries (features, weights, pixel) and training records
@@ -128,6 +129,12 @@ int launch_trace_any(rt_ctx* c, const rt_ray* d_rays, uint32_t n, uint8_t* d_occ
 int trace_error(rt_ctx* c, uint32_t* err);
 // Scene::isOccluded for n segments (start, end: 6 floats each)
 int launch_occluded(rt_ctx* c, const float* d_segs, uint32_t n, uint8_t* d_occluded);
+// Looks up the device address of the allocation-free trace's scratch array on
+// the context's device (once per device; the device must be current).
+int noalloc_scratch_init(rt_ctx* c);
+// rt_trace_closest_noalloc's launch: zeroes the scratch's counters and
+// *d_status (may be NULL) with a fill kernel, then traces with the scratch
+int trace_noalloc(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, uint32_t* d_status);
 
 // nrc.hip: cache inference / training on device-resident samples
 int nrc_infer_device(rt_nrc* s, const float* d_feats, uint32_t n, float* d_out);

@@ -33,6 +33,8 @@
 // (Scene::isOccluded, graphic/scene.cpp:58-83) build the ray from (start,
 // end) as the reference does and accept a triangle when its hit point's
 // squared distance from start is <= |end - start|^2.
+#include <mutex>
+
 #include "rt_internal.hpp"
 #include "rt_core.hpp"
 
@@ -426,6 +428,95 @@ int trace_error(rt_ctx* c, uint32_t* err) {
 }
 int launch_occluded(rt_ctx* c, const float* d_segs, uint32_t n, uint8_t* d_occluded) {
     return launch(c, nullptr, n, nullptr, d_occluded, false, nullptr, d_segs);
+}
+
+// ---- allocation-free closest hit (rt_trace_closest_noalloc) ----------------
+// Its scratch lives in a file-scope device array, one copy per GPU, kept
+// until the process ends: NOALLOC_HEADER_WORDS of ray counters (two halves as
+// in `work`) and the error word, then the per-lane spill stacks.  Host code
+// never passes `scratch` itself (in host code the name is a stand-in, not a
+// device address) but the address hipGetSymbolAddress returns for the current
+// device, kept per device.
+static __device__ __align__(256) uint32_t scratch[RT_NOALLOC_SCRATCH_BYTES / 4];
+
+namespace {
+constexpr uint32_t NOALLOC_HEADER_WORDS = 1024;
+static_assert(2 * WORK_HALF + 1 <= NOALLOC_HEADER_WORDS, "counter header too small");
+
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_dev[256] = {};  // per device ordinal
+
+__global__ void k_zero_words(uint32_t* p, uint32_t n, uint32_t* status) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+    if (status && blockIdx.x == 0 && threadIdx.x == 0) *status = 0u;
+}
+}  // namespace
+
+int noalloc_scratch_init(rt_ctx* c) {
+    if (c->device < 0 || c->device >= 256) return fail(c, RT_E_UNSUPPORTED, "device ordinal beyond 255");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[c->device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(c, e, "hipGetSymbolAddress(scratch)");
+        if (!p) return fail(c, RT_E_DEVICE, "hipGetSymbolAddress(scratch) returned NULL");
+        g_scratch_dev[c->device] = (uint32_t*)p;
+    }
+    c->noalloc_scratch = g_scratch_dev[c->device];
+    return RT_OK;
+}
+
+int trace_noalloc(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, uint32_t* d_status) {
+    uint32_t* base = c->noalloc_scratch;
+    if (!base) return fail(c, RT_E_STATE, "rt_trace_closest_noalloc: no scratch address for this device");
+    int& per_cu = c->trace_per_cu[0];
+    if (per_cu == 0) {
+        RT_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false, false>, TRACE_BLOCK, 0));
+        if (per_cu < 1) per_cu = 1;
+    }
+    const uint32_t max_stack = 3u * c->bvh_depth + 1u;
+    const uint32_t spill_cap = max_stack > (uint32_t)STACK_LDS ? max_stack - STACK_LDS : 1u;
+    const size_t block_bytes = 4ull * spill_cap * TRACE_BLOCK;
+    const size_t spill_bytes = (size_t)RT_NOALLOC_SCRATCH_BYTES - 4ull * NOALLOC_HEADER_WORDS;
+    const size_t fit = spill_bytes / block_bytes;
+    if (fit == 0)
+        return fail(c, RT_E_UNSUPPORTED, "rt_trace_closest_noalloc: a BVH of depth " + std::to_string(c->bvh_depth) +
+                                             " needs " + std::to_string(block_bytes) +
+                                             " bytes of traversal stack per workgroup, beyond the " +
+                                             std::to_string(spill_bytes) + "-byte scratch");
+    uint32_t grid = (uint32_t)(c->num_cus * per_cu);
+    const uint32_t need_blocks = (n + TRACE_BLOCK - 1) / TRACE_BLOCK;
+    if (grid > need_blocks) grid = need_blocks;
+    if (grid > fit) grid = (uint32_t)fit;
+    // counters and the error word start at zero: a fill kernel, not a
+    // memset, so that the call can be captured into a graph
+    hipLaunchKernelGGL(k_zero_words, dim3(4), dim3(256), 0, c->stream, base, NOALLOC_HEADER_WORDS, d_status);
+    RT_HIP(c, hipGetLastError());
+    TraceArgs a;
+    a.rays = d_rays;
+    a.hits = d_hits;
+    a.occluded = nullptr;
+    a.segs = nullptr;
+    a.n = n;
+    a.n_dev = nullptr;
+    a.nodes = (const rt_bvh4_node*)c->nodes.ptr;
+    a.tris = (const rt_tri_rec*)c->tris.ptr;
+    a.work = base;
+    a.work_next = base + WORK_HALF;
+    a.err = d_status ? d_status : base + 2 * WORK_HALF;
+    a.zero0 = nullptr;
+    a.zero1 = nullptr;
+    a.spill = base + NOALLOC_HEADER_WORDS;
+    a.spill_cap = spill_cap;
+    a.spill_stride = grid * TRACE_BLOCK;
+    a.max_steps = c->trace_max_steps ? c->trace_max_steps : 2u * c->num_nodes + 2u * c->T + 16u;
+    a.refill_min = c->trace_refill_min;
+    a.slice = c->trace_slice;
+    a.stats = nullptr;
+    hipLaunchKernelGGL((k_trace<false, false>), dim3(grid), dim3(TRACE_BLOCK), 0, c->stream, a);
+    RT_HIP(c, hipGetLastError());
+    c->timing.launches = 2;
+    return RT_OK;
 }
 
 }  // namespace rt

@@ -108,6 +108,17 @@ class Tracer:
             return hits, {f: getattr(cnt, f) for f, _ in L.rt_trace_counters._fields_}
         return hits
 
+    def trace_noalloc(self, rays, hits, status=None) -> None:
+        """Closest hit through rt_trace_closest_noalloc: rays (n*8 float32)
+        and hits (n*4 int32) are CUDA tensors the caller owns, status an
+        optional one-word int32 CUDA tensor; enqueued on the context stream,
+        nothing is allocated."""
+        assert L._is_device(rays) and L._is_device(hits), "device tensors only"
+        n = rays.numel() // 8
+        assert hits.numel() >= 4 * n
+        self._check(self.lib.rt_trace_closest_noalloc(self.h, L._ptr(rays), n, L._ptr(hits),
+                                                      None if status is None else L._ptr(status)))
+
     def trace_any(self, rays, out=None):
         """Occlusion flags (uint8, 1 = some triangle within (tmin, tmax)).
         rays: numpy RAY_DTYPE array or a CUDA tensor of n*8 float32."""
