@@ -362,9 +362,10 @@ typedef struct rt_nrc_stats {
  * renderer (see the cache section above). */
 int  rt_render_nrc(rt_ctx* ctx, rt_nrc* nrc, const rt_camera* cams, uint32_t num_views, const rt_settings* s,
                    const rt_nrc_render* opts, float* rgb, uint8_t* rgb8, int ptr_kind, rt_nrc_stats* stats);
-/* Copies the training records of the last rt_render_nrc (in no particular
- * order) to host memory: feats 9 and targets 3 floats per record; *count is
- * the in-out capacity (both NULL: query the count). */
+/* Copies the training records of the last rt_render_nrc to host memory, in
+ * the order of their samples (view, pixel, sample: the order the cache
+ * trains on them): feats 9 and targets 3 floats per record; *count is the
+ * in-out capacity (both NULL: query the count). */
 int  rt_render_nrc_records(rt_ctx* ctx, float* feats, float* targets, uint32_t* count);
 
 /* ---- post-filters ----------------------------------------------------------

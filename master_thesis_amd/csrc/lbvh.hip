@@ -482,6 +482,32 @@ inline uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs 
 
 }  // namespace
 
+int radix_sort_pairs(rt_ctx* c, uint32_t* keys[2], uint32_t* vals[2], uint32_t n, uint32_t passes, int* cur_out) {
+    int cur = 0;
+    *cur_out = 0;
+    if (n == 0 || passes == 0) return RT_OK;
+    hipStream_t st = c->stream;
+    const uint32_t nblocks = blocks_for(n, RS_TILE);
+    const uint32_t scan_tiles = (256u * nblocks + SCAN_TILE - 1) / SCAN_TILE;
+    int rc;
+    if ((rc = ensure(c, c->hist, 4ull * 256 * nblocks + 4ull * scan_tiles))) return rc;
+    uint32_t* hist = (uint32_t*)c->hist.ptr;
+    uint32_t* tsum = hist + 256u * nblocks;
+    for (uint32_t pass = 0; pass < passes; ++pass) {
+        const int shift = (int)pass * 8;
+        hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(RS_BLOCK), 0, st, keys[cur], n, shift, nblocks, hist);
+        hipLaunchKernelGGL(k_scan_tiles, dim3(scan_tiles), dim3(256), 0, st, hist, 256u * nblocks, tsum);
+        hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(256), 0, st, tsum, scan_tiles);
+        hipLaunchKernelGGL(k_scan_add, dim3(scan_tiles), dim3(256), 0, st, hist, 256u * nblocks, (const uint32_t*)tsum);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nblocks), dim3(RS_BLOCK), 0, st, keys[cur], vals[cur], keys[cur ^ 1],
+                           vals[cur ^ 1], n, shift, nblocks, (const uint32_t*)hist);
+        cur ^= 1;
+    }
+    RT_HIP(c, hipGetLastError());
+    *cur_out = cur;
+    return RT_OK;
+}
+
 int build_lbvh(rt_ctx* c, uint32_t leaf_size) {
     const uint32_t n = c->T;
     const size_t N = n;
@@ -514,23 +540,8 @@ int build_lbvh(rt_ctx* c, uint32_t leaf_size) {
                        (const uint32_t*)c->idx.ptr, c->scene_lo[0], c->scene_lo[1], c->scene_lo[2], inv[0], inv[1],
                        inv[2], keys[0], vals[0], leafbox);
 
-    const uint32_t nblocks = blocks_for(n, RS_TILE);
-    const uint32_t scan_tiles = (256u * nblocks + SCAN_TILE - 1) / SCAN_TILE;
-    if ((rc = ensure(c, c->hist, 4ull * 256 * nblocks + 4ull * scan_tiles))) return rc;
-    uint32_t* hist = (uint32_t*)c->hist.ptr;
-    uint32_t* tsum = hist + 256u * nblocks;
     int cur = 0;
-    for (int pass = 0; pass < 4; ++pass) {
-        int shift = pass * 8;
-        hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(RS_BLOCK), 0, st, keys[cur], n, shift, nblocks, hist);
-        hipLaunchKernelGGL(k_scan_tiles, dim3(scan_tiles), dim3(256), 0, st, hist, 256u * nblocks, tsum);
-        hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(256), 0, st, tsum, scan_tiles);
-        hipLaunchKernelGGL(k_scan_add, dim3(scan_tiles), dim3(256), 0, st, hist, 256u * nblocks, (const uint32_t*)tsum);
-        hipLaunchKernelGGL(k_rs_scatter, dim3(nblocks), dim3(RS_BLOCK), 0, st, keys[cur], vals[cur], keys[cur ^ 1],
-                           vals[cur ^ 1], n, shift, nblocks, (const uint32_t*)hist);
-        cur ^= 1;
-    }
-    RT_HIP(c, hipGetLastError());
+    if ((rc = radix_sort_pairs(c, keys, vals, n, 4, &cur))) return rc;
 
     uint32_t* ib = (uint32_t*)c->inner.ptr;
     Inner in{ib, ib + N, ib + 2 * N, ib + 3 * N, ib + 4 * N, ib + 5 * N};

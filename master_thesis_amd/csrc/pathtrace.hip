@@ -129,7 +129,8 @@ struct PTArgs {
     uint32_t q_cap;
     float* t_feat;      // t_cap * 9
     float* t_tgt;       // t_cap * 3
-    uint32_t t_cap;
+    uint32_t* t_key;    // t_cap: the record's sample, (view * P + pixel) * key_n + sample - sample_offset
+    uint32_t t_cap, key_n;
     uint32_t* s_ridx;   // S: training record of the slot's sample (NO_REC: none)
     float* s_Tq;        // 3 * S: its throughput from the query vertex on
     // light sampling (cache mode): shadow rays to emissive triangles and the sun
@@ -632,6 +633,16 @@ __device__ __forceinline__ void add3(float* dst, f3 v) {
     atomicAdd(dst + 2, v.z);
 }
 
+// A training record's target has one writer per launch (the slot that owns
+// the record: its path end in the shade step, or its one shadow ray in the
+// resolve step), and the launches run in path order, so the target is a
+// plain sum in the order of the path's vertices.
+__device__ __forceinline__ void acc3(float* dst, f3 v) {
+    dst[0] = dst[0] + v.x;
+    dst[1] = dst[1] + v.y;
+    dst[2] = dst[2] + v.z;
+}
+
 // Shade step of rt_render_nrc: the plain state machine plus the query
 // vertex, training records and light sampling.
 __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade_nrc(PTArgs a, int cur) {
@@ -648,7 +659,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade_nrc(PTArgs a, int cur) {
     bool push_q = false, want_rec = false, push_sh = false;
     float f9[9];
     f3 qw = mk(0.0f, 0.0f, 0.0f), shw_pix = mk(0.0f, 0.0f, 0.0f), shw_rec = mk(0.0f, 0.0f, 0.0f);
-    uint32_t pidx = 0, ridx = NO_REC;
+    uint32_t pidx = 0, ridx = NO_REC, rkey = 0;
     bool sh_to_rec = false;
     rt_ray sh;
     if (k < n) {
@@ -716,6 +727,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade_nrc(PTArgs a, int cur) {
                         want_rec = true;
                         rec = true;
                         Tq = mk(1.0f, 1.0f, 1.0f);
+                        rkey = vp * a.key_n + (gsmp - a.sample_offset);
                     } else if (a.nrc_query) {
                         push_q = true;
                         // probe field: the diffuse bounce's expectation, albedo / 2 * lookup
@@ -768,7 +780,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade_nrc(PTArgs a, int cur) {
                 a.s_sum[3 * slot] = a.s_sum[3 * slot] + T.x;
                 a.s_sum[3 * slot + 1] = a.s_sum[3 * slot + 1] + T.y;
                 a.s_sum[3 * slot + 2] = a.s_sum[3 * slot + 2] + T.z;
-                if (rec && !want_rec && ridx != NO_REC) add3(&a.t_tgt[3 * (size_t)ridx], Tq);
+                if (rec && !want_rec && ridx != NO_REC) acc3(&a.t_tgt[3 * (size_t)ridx], Tq);
             }
             ++smp;
             if (smp < a.spc) {
@@ -817,6 +829,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade_nrc(PTArgs a, int cur) {
                 for (int j = 0; j < 9; ++j) a.t_feat[9 * (size_t)ridx + j] = f9[j];
                 a.t_tgt[3 * (size_t)ridx] = 0.0f; a.t_tgt[3 * (size_t)ridx + 1] = 0.0f;
                 a.t_tgt[3 * (size_t)ridx + 2] = 0.0f;
+                a.t_key[ridx] = rkey;
             }
             a.s_ridx[slot] = ridx;
         }
@@ -854,7 +867,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_shadow_resolve(PTArgs a) {
     const float* w = a.sh_w + 6 * (size_t)i;
     add3(&a.partial[3 * (size_t)a.sh_idx[2 * (size_t)i]], mk(w[0], w[1], w[2]));
     const uint32_t r = a.sh_idx[2 * (size_t)i + 1];
-    if (r != NO_REC) add3(&a.t_tgt[3 * (size_t)r], mk(w[3], w[4], w[5]));
+    if (r != NO_REC) acc3(&a.t_tgt[3 * (size_t)r], mk(w[3], w[4], w[5]));
 }
 
 __global__ void __launch_bounds__(PT_BLOCK) k_pt_finish(PTArgs a) {
@@ -878,6 +891,22 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_finish(PTArgs a) {
 __global__ void k_zero(float* p, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.0f;
+}
+
+__global__ void k_iota(uint32_t* p, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = i;
+}
+
+// training records in sample order: out[i] = record order[i]
+__global__ void k_gather_records(const float* __restrict__ feat, const float* __restrict__ tgt,
+                                 const uint32_t* __restrict__ order, uint32_t n, float* __restrict__ ofeat,
+                                 float* __restrict__ otgt) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t r = order[i];
+    for (int j = 0; j < 9; ++j) ofeat[9 * (size_t)i + j] = feat[9 * (size_t)r + j];
+    for (int j = 0; j < 3; ++j) otgt[3 * (size_t)i + j] = tgt[3 * (size_t)r + j];
 }
 
 // Radiance of the probe rays of a field update (rt_ddgi_update): what a path
@@ -979,13 +1008,22 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
     int rc;
     const size_t szS = S;
     // cache mode: queries flushed every 8 iterations (at most S per
-    // iteration), training records capped at 4 frames' worth of steps
+    // iteration); training records: 4 frames' worth of steps, and at least
+    // twice the expected number of training samples (a binomial count, so
+    // the capacity is practically never reached and no record is dropped)
     const size_t q_cap = nrc ? 8 * szS : 0;
-    const size_t t_cap = nrc ? (size_t)4 * nf->opt.batch * nf->opt.max_steps : 0;
+    size_t t_cap = nrc ? (size_t)4 * nf->opt.batch * nf->opt.max_steps : 0;
+    if (nrc && nf->opt.train_every && !(nf->opt.flags & RT_NRC_NO_RECORDS)) {
+        const size_t expect = (size_t)((P * N + nf->opt.train_every - 1) / nf->opt.train_every);
+        if (2 * expect + 4096 > t_cap) t_cap = 2 * expect + 4096;
+    }
+    if (nrc && P * N > 0xFFFFFFFFull) return fail(c, RT_E_UNSUPPORTED, "rt_render_nrc: pixels * samples above 2^32");
+    if (t_cap > 0x3FFFFFFFull) return fail(c, RT_E_UNSUPPORTED, "rt_render_nrc: training records above 2^30");
     // shadow rays (one per live slot and iteration): ray, 6 weights, 2 indices, flag
     const size_t sh_bytes = szS * (sizeof(rt_ray) + 24 + 8 + 1);
     if (nrc && ((rc = ensure(c, c->pt_nrc_query, q_cap * (36 + 12 + 4))) ||
-                (rc = ensure(c, c->pt_nrc_train, (t_cap ? t_cap : 1) * (36 + 12))) ||
+                (rc = ensure(c, c->pt_nrc_train, (t_cap ? t_cap : 1) * (36 + 12 + 4))) ||
+                (rc = ensure(c, c->pt_nrc_sorted, (t_cap ? t_cap : 1) * (36 + 12 + 12))) ||
                 (rc = ensure(c, c->pt_shadow, sh_bytes))))
         return rc;
     if ((rc = ensure(c, c->pt_state, szS * (4 * 3 + 12 * 2) + (adj ? szS * (8 * ADJ_MAX_DEPTH + 4) : 0) +
@@ -1048,7 +1086,9 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
     a.q_cap = (uint32_t)q_cap;
     a.t_feat = nrc ? (float*)c->pt_nrc_train.ptr : nullptr;
     a.t_tgt = nrc ? a.t_feat + 9 * t_cap : nullptr;
+    a.t_key = nrc ? (uint32_t*)(a.t_tgt + 3 * t_cap) : nullptr;
     a.t_cap = (uint32_t)t_cap;
+    a.key_n = N;
     // queue buffers hold S slot ids then S rays (16-byte aligned)
     for (int q = 0; q < 2; ++q) {
         a.q_slot[q] = (uint32_t*)c->pt_queue[q].ptr;
@@ -1143,11 +1183,31 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
         const uint32_t tn = hc[C_TN];
         nf->stats->records = tn;
         nf->stats->shadow_rays = (uint64_t)hc[C_SHRAYS] | ((uint64_t)hc[C_SHRAYS + 1] << 32);
-        c->nrc_records = tn < a.t_cap ? tn : a.t_cap;
+        const uint32_t avail = tn < a.t_cap ? tn : a.t_cap;
+        c->nrc_records = avail;
         c->nrc_tcap = a.t_cap;
+        // records in the order of their samples, (view, pixel, sample), so
+        // that the batches, and with them the trained network, do not
+        // depend on the order in which slots took record indices
+        float* sfeat = (float*)c->pt_nrc_sorted.ptr;
+        float* stgt = sfeat + 9 * (size_t)a.t_cap;
+        uint32_t* skey = (uint32_t*)(stgt + 3 * (size_t)a.t_cap);
+        if (avail > 0) {
+            uint32_t* keys[2] = {a.t_key, skey};
+            uint32_t* vals[2] = {skey + a.t_cap, skey + 2 * (size_t)a.t_cap};
+            hipLaunchKernelGGL(k_iota, dim3(div_up(avail, PT_BLOCK)), dim3(PT_BLOCK), 0, stream, vals[0], avail);
+            uint32_t bits = 1;
+            while (bits < 32 && (P * N - 1) >> bits) ++bits;
+            int sc = 0;
+            if ((rc = radix_sort_pairs(c, keys, vals, avail, (bits + 7) / 8, &sc))) return rc;
+            hipLaunchKernelGGL(k_gather_records, dim3(div_up(avail, PT_BLOCK)), dim3(PT_BLOCK), 0, stream,
+                               (const float*)a.t_feat, (const float*)a.t_tgt, (const uint32_t*)vals[sc], avail,
+                               sfeat, stgt);
+            RT_HIP(c, hipGetLastError());
+            launches += 2 + 5 * ((bits + 7) / 8);
+        }
         if (!(nf->opt.flags & RT_NRC_NO_TRAIN) && nf->cache) {
-            const uint32_t avail = tn < a.t_cap ? tn : a.t_cap;
-            if ((rc = nrc_train_records(nf->cache, a.t_feat, a.t_tgt, avail, nf->opt.batch, nf->opt.max_steps,
+            if ((rc = nrc_train_records(nf->cache, sfeat, stgt, avail, nf->opt.batch, nf->opt.max_steps,
                                         &nf->stats->steps, &nf->stats->loss)))
                 return rc;
             launches += 2 * nf->stats->steps;

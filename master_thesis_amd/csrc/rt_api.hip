@@ -169,7 +169,7 @@ void rt_destroy(rt_ctx* c) {
                      &c->innerbox, &c->boxtab, &c->queue[0], &c->queue[1], &c->counters, &c->rays, &c->hits,
                      &c->spill, &c->work, &c->stats, &c->pt_state, &c->pt_queue[0], &c->pt_queue[1],
                      &c->pt_image, &c->pt_img8, &c->pt_dimg, &c->pt_dcolor, &c->pt_ctr, &c->pt_partial,
-                     &c->pt_cams, &c->pt_nrc_query, &c->pt_nrc_train, &c->pt_shadow, &c->lights,
+                     &c->pt_cams, &c->pt_nrc_query, &c->pt_nrc_train, &c->pt_nrc_sorted, &c->pt_shadow, &c->lights,
                      &c->light_objs, &c->light_obj_tris, &c->bd_vert, &c->bd_state, &c->bd_queue[0],
                      &c->bd_queue[1], &c->bd_seg, &c->bd_occ, &c->bd_val};
     for (DevBuf* b : all) release(c, *b);
@@ -584,7 +584,7 @@ int rt_render_nrc_records(rt_ctx* c, float* feats, float* targets, uint32_t* cou
     if (cap < c->nrc_records) return fail(c, RT_E_INVALID, "rt_render_nrc_records: capacity below the record count");
     if (c->nrc_records == 0) return RT_OK;
     DeviceGuard g(c->device);
-    const float* base = (const float*)c->pt_nrc_train.ptr;
+    const float* base = (const float*)c->pt_nrc_sorted.ptr;
     if (feats)
         RT_HIP(c, hipMemcpyAsync(feats, base, 36 * (size_t)c->nrc_records, hipMemcpyDeviceToHost, c->stream));
     if (targets)

@@ -80,6 +80,7 @@ struct rt_ctx {
     // path tracer state
     rt::DevBuf pt_state, pt_queue[2], pt_image, pt_img8, pt_dimg, pt_dcolor, pt_ctr, pt_partial, pt_cams;
     rt::DevBuf pt_nrc_query, pt_nrc_train;  // cache queries (features, weights, pixel) and training records
+    rt::DevBuf pt_nrc_sorted;               // training records in sample order, and the sort's key / index buffers
     rt::DevBuf pt_shadow;                   // shadow rays of light sampling (rays, weights, indices, flags)
     rt::DevBuf bd_vert, bd_state, bd_queue[2], bd_seg, bd_occ, bd_val;  // bidirectional tracer
     uint32_t nrc_records = 0, nrc_tcap = 0; // training records of the last cached render, record capacity
@@ -110,6 +111,10 @@ int ensure(rt_ctx* c, DevBuf& b, size_t bytes);
 
 // lbvh.hip
 int build_lbvh(rt_ctx* c, uint32_t leaf_size);
+// Stable LSD radix sort of n (key, value) pairs by the low 8 * passes bits of
+// the keys, ping-ponging between keys[0]/vals[0] (input) and keys[1]/vals[1];
+// *cur is the index of the buffers holding the result.  Uses c->hist.
+int radix_sort_pairs(rt_ctx* c, uint32_t* keys[2], uint32_t* vals[2], uint32_t n, uint32_t passes, int* cur);
 
 // trace.hip
 constexpr int TRACE_BLOCK = 256;
