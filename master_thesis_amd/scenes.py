@@ -25,6 +25,18 @@ class Scene:
     idx: np.ndarray        # (T, 3) uint32
     tri_mat: np.ndarray    # (T,) uint32
     materials: np.ndarray  # (M,) MATERIAL_DTYPE
+    # (T,) uint32 object of each triangle as the reference's scene files would
+    # group them (one GraphicsObject per quad or box); None: one per material
+    tri_obj: np.ndarray | None = None
+
+    def objects(self) -> tuple[np.ndarray, np.ndarray]:
+        """(object per triangle, material per object)."""
+        if self.tri_obj is None:
+            return self.tri_mat.copy(), np.arange(len(self.materials), dtype=np.uint32)
+        O = int(self.tri_obj.max()) + 1
+        obj_mat = np.zeros(O, np.uint32)
+        obj_mat[self.tri_obj] = self.tri_mat
+        return self.tri_obj, obj_mat
 
     @property
     def num_triangles(self) -> int:
@@ -58,11 +70,17 @@ def material(color, diffuse=1.0, reflect=0.0, transparent=0.0, refraction_index=
 
 class _Builder:
     def __init__(self):
-        self.pos, self.nrm, self.idx, self.mat = [], [], [], []
+        self.pos, self.nrm, self.idx, self.mat, self.obj = [], [], [], [], []
         self.nv = 0
+        self.nobj = 0
+        self._in_box = False
 
     def quad(self, corner, ea, eb, normal, m):
-        """Two triangles (c, c+a, c+a+b), (c, c+a+b, c+b), flat normal."""
+        """Two triangles (c, c+a, c+a+b), (c, c+a+b, c+b), flat normal; its
+        own object unless part of a box."""
+        if not self._in_box:
+            self.nobj += 1
+        self.obj += [self.nobj - 1] * 2
         c = np.asarray(corner, np.float32)
         a = np.asarray(ea, np.float32)
         b = np.asarray(eb, np.float32)
@@ -80,12 +98,15 @@ class _Builder:
         x0, y0, z0 = lo
         x1, y1, z1 = hi
         dx, dy, dz = x1 - x0, y1 - y0, z1 - z0
+        self.nobj += 1
+        self._in_box = True
         self.quad((x0, y0, z0), (0, 0, dz), (0, dy, 0), (-1, 0, 0), m)
         self.quad((x1, y0, z0), (0, dy, 0), (0, 0, dz), (1, 0, 0), m)
         self.quad((x0, y0, z0), (dx, 0, 0), (0, 0, dz), (0, -1, 0), m)
         self.quad((x0, y1, z0), (0, 0, dz), (dx, 0, 0), (0, 1, 0), m)
         self.quad((x0, y0, z0), (0, dy, 0), (dx, 0, 0), (0, 0, -1), m)
         self.quad((x0, y0, z1), (dx, 0, 0), (0, dy, 0), (0, 0, 1), m)
+        self._in_box = False
 
     def scene(self, materials) -> Scene:
         s = Scene(
@@ -94,6 +115,7 @@ class _Builder:
             idx=np.asarray(self.idx, np.uint32).reshape(-1, 3),
             tri_mat=np.asarray(self.mat, np.uint32),
             materials=np.stack(materials),
+            tri_obj=np.asarray(self.obj, np.uint32) if len(self.obj) == len(self.mat) else None,
         )
         s.validate()
         return s
@@ -182,6 +204,8 @@ def cornell_specular(light_color=(15.0, 15.0, 15.0), ball_level: int = 3) -> Sce
             b.nrm.append(p.astype(np.float32))
         b.idx += [tuple(int(x) + n0 for x in t) for t in f]
         b.mat += [m] * len(f)
+        b.obj += [b.nobj] * len(f)
+        b.nobj += 1
         b.nv += len(v)
     mats = [material(WHITE), material(RED), material(GREEN), material(light_color, light=True)]
     mats += [material(WHITE, *SPECULAR_WEIGHTS[MIRROR][:3]),

@@ -287,6 +287,34 @@ def render_cached(scene, cams, settings, cache_rgb, q_vertex: int = 1, train_eve
     return rgb, int(counts[0]), int(counts[1])
 
 
+def render_faithful(scene, cam, spp: int, max_depth: int, threads: int = 8, seed: int = 1,
+                    pixel_limit: int | None = None):
+    """The reference renderer in faithful mode (oracle/faithful.cpp): its
+    two-level agglomerative BVHs, enumerate-all queries with a vector per
+    node, one mutex-guarded mt19937, one pixel per atomic fetch_add on
+    `threads` workers.  Objects as the scene groups them (Scene.objects()).
+    Renders the first pixel_limit pixels (default all).  Returns (rgb (H, W,
+    3) float32, rgb8 (H, W, 3) uint8, seconds without the BVH builds)."""
+    lib = load()
+    P = C.c_void_p
+    lib.or_render_faithful.argtypes = [P, P, P, P, C.c_uint32, P, C.c_uint32, P, P, C.c_uint32, C.c_uint32,
+                                       C.c_uint32, C.c_uint32, C.c_uint32, P, P]
+    lib.or_render_faithful.restype = C.c_int
+    lib.or_faithful_seconds.restype = C.c_double
+    tri_obj, obj_mat = scene.objects()
+    arrs = [np.ascontiguousarray(a) for a in (scene.pos, scene.nrm, scene.idx, tri_obj.astype(np.uint32),
+                                              obj_mat.astype(np.uint32), scene.materials)]
+    H, W = cam.height, cam.width
+    rgb = np.zeros((H, W, 3), np.float32)
+    rgb8 = np.zeros((H, W, 3), np.uint8)
+    lim = W * H if pixel_limit is None else int(pixel_limit)
+    rc = lib.or_render_faithful(_p(arrs[0]), _p(arrs[1]), _p(arrs[2]), _p(arrs[3]), arrs[2].shape[0], _p(arrs[4]),
+                                arrs[4].shape[0], _p(arrs[5]), C.addressof(cam), spp, max_depth, threads, seed, lim,
+                                _p(rgb), _p(rgb8))
+    assert rc == 0, rc
+    return rgb, rgb8, float(lib.or_faithful_seconds())
+
+
 def render_bdpt(scene, cam, settings, vision_jumps: int = 5, light_jumps: int = 5, max_depth: int = 5,
                 threads: int = 8) -> np.ndarray:
     """The bidirectional path tracer (rt_render_bdpt's checker); rgb (H, W, 3)."""

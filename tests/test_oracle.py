@@ -324,3 +324,28 @@ def test_bidirectional_equals_path_tracer_when_paths_end_on_lights():
     bd = O.render_bdpt(sc, cam, s, vision_jumps=5, light_jumps=5, max_depth=5)
     np.testing.assert_array_equal(bd, pt)
 
+
+
+def test_faithful_mode_matches_oracle_estimator():
+    """The faithful-mode CPU baseline (mt19937 stream, the reference's BVH
+    queries) estimates the oracle's image (Philox stream, binned-SAH BVH):
+    block means within the spread across oracle seeds; on an all-light room
+    it gives pi * E like the oracle; a first-pixels run touches only those
+    pixels."""
+    sc = scenes.cornell_box()
+    cam = H.camera(scenes.cornell_camera_position(), 1.0, 0.0, np.float32(math.pi / 4), 32, 32)
+    img = O.render_faithful(sc, cam, 256, 5, threads=4, seed=5)[0]
+    runs = np.stack([O.render(sc, cam, make_settings(spp=256, max_depth=5, seed=s))[0] for s in range(1, 13)])
+    b = img.reshape(4, 8, 4, 8, 3).mean(axis=(1, 3))
+    ob = runs.reshape(12, 4, 8, 4, 8, 3).mean(axis=(2, 4))
+    z = (b - ob.mean(axis=0)) / np.maximum(ob.std(axis=0, ddof=1) * math.sqrt(1 + 1 / 12), 1e-6)
+    assert np.all(np.abs(z) < 7), np.abs(z).max()
+    E = 1.5
+    light = scenes.material((E, E, E), light=True)
+    room = scenes.closed_room(light, light)
+    rc = H.camera((0.0, 0.0, 0.0), 0.0, -1.0, math.pi / 4, 32, 32)
+    im = O.render_faithful(room, rc, 64, 5, threads=4, seed=2)[0]
+    sd = 2 * math.pi * E / math.sqrt(12 * 64 * 32 * 32)
+    assert abs(im.mean() - math.pi * E) < 5 * sd
+    part = O.render_faithful(sc, cam, 4, 5, threads=2, seed=3, pixel_limit=600)[0].reshape(-1, 3)
+    assert np.all(part[600:] == 0) and part[:600].max() > 0

@@ -262,3 +262,18 @@ def test_bidirectional_tracer_specular_statistically_matches_reference():
     runs = np.stack([O.render_bdpt(sc, cam, make_settings(spp=spp, max_depth=6, seed=s), 6, 5, 6)
                      for s in range(1, 17)])
     _blocks_match(ref, runs)
+
+
+def test_faithful_mode_statistically_matches_reference():
+    """The CPU baseline's faithful mode (oracle/faithful.cpp: the reference's
+    BVHs, queries, shared mutex-guarded mt19937 and pixel counter restated)
+    renders what the compiled reference renders, objects grouped alike (one
+    per quad or box): block means within the spread across its own seeds."""
+    sc = scenes.cornell_box()
+    tri_obj, obj_mat = sc.objects()
+    cam = H.camera(scenes.cornell_camera_position(), 1.0, 0.0, np.float32(math.pi / 4), 32, 32)
+    spp, depth = 256, 5
+    obj_scene = scenes.Scene(sc.pos, sc.nrm, sc.idx, tri_obj, sc.materials[obj_mat])
+    ref = R.RefScene(obj_scene).render(cam, spp, depth, threads=8)
+    runs = np.stack([O.render_faithful(sc, cam, spp, depth, threads=1, seed=s)[0] for s in range(1, 17)])
+    _blocks_match(ref, runs)
