@@ -8,10 +8,12 @@ on the sphere).  A step is one closest-hit pass over all 16M rays with the
 rays already resident in HBM; the BVH is built once before timing.
 
 Prints one JSON line (metric Mrays/s, higher is better) with the dominant
-kernel's roofline (HBM-bound: algorithmic bytes = 32 B ray + 16 B hit + 128 B
-per BVH4 node fetched + 40 B per triangle tested, node/triangle counts from
-the kernel's own counters) and the CPU oracle's BVH traversal timed on the
-host cores on a bounded sample of the same rays.
+kernel's roofline (HBM: algorithmic bytes = 32 B ray + 16 B hit + 64 B per
+BVH4 traversal node fetched + 40 B per triangle tested, node/triangle counts
+from the kernel's own counters; beside them the memory-side traffic, L2 hit
+rate and VALU issue rate that rocprofv3 counters measured on the same
+kernel sources) and the CPU oracle's BVH traversal timed on the host cores on
+a bounded sample of the same rays.
 """
 from __future__ import annotations
 
@@ -28,7 +30,20 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+NODE_BYTES = 64  # rt_bvh4q_node, the traversal's node
 PMC_SUMMARY = ROOT / "profiles" / "trace_raycast_1M_16M_pmc.json"
+# the sources that define the traversal kernel and the tree it walks: counter
+# summaries record their hash, and figures from other sources are not used
+KERNEL_SOURCES = ["include/rt.h", "master_thesis_amd/csrc/rt_core.hpp", "master_thesis_amd/csrc/rt_internal.hpp",
+                  "master_thesis_amd/csrc/trace.hip", "master_thesis_amd/csrc/lbvh.hip"]
+
+
+def kernel_source_hash() -> str:
+    import hashlib
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        h.update((ROOT / f).read_bytes())
+    return h.hexdigest()[:16]
 
 
 def log(msg: str) -> None:
@@ -204,7 +219,7 @@ def raycast(args, sc, rays, n_coh, workload, pmc_path):
         # counters pass (same kernel with per-wave counters): node / triangle fetches for the roofline
         _, cnt = t.trace(d_rays, d_hits, counters=True)
         torch.cuda.synchronize()
-        bytes_alg = 32 * cnt["rays"] + 16 * cnt["rays"] + 128 * cnt["node_visits"] + 40 * cnt["tri_tests"]
+        bytes_alg = 32 * cnt["rays"] + 16 * cnt["rays"] + NODE_BYTES * cnt["node_visits"] + 40 * cnt["tri_tests"]
 
         # per-half rates (informational)
         halves = {"coherent": None, "incoherent": None}
@@ -244,20 +259,28 @@ def raycast(args, sc, rays, n_coh, workload, pmc_path):
     value = n / (ms_per_step / 1e3) / 1e6
     k_ms = float(np.mean(kernel_ms))
     achieved = bytes_alg / (k_ms / 1e3) / 1e9
-    traffic = None
+    # counter summary (profiles/): used only when it was measured on these kernel sources
+    pmc, src = None, kernel_source_hash()
     if pmc_path.exists():
         try:
-            traffic = json.loads(pmc_path.read_text()).get("hbm_bytes_per_launch")
+            pmc = json.loads(pmc_path.read_text())
         except (ValueError, OSError):
-            traffic = None
+            pmc = None
+    same = pmc is not None and pmc.get("source_hash") == src
+    traffic = pmc.get("hbm_bytes_per_launch") if same else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                # what HBM actually moved (memory-side counters, calibrated) against its peak: the
-                # algorithmic bytes above count node and triangle fetches the caches serve too
+                # memory-side bytes (L2 misses to the fabric, MALL and HBM; FETCH_SIZE + WRITE_SIZE,
+                # calibrated) over this run's kernel time, against HBM's peak
                 "traffic_gbps": traffic and round(traffic / (k_ms / 1e3) / 1e9, 1),
                 "traffic_frac": traffic and round(traffic / (k_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                "l2_hit_rate": pmc.get("l2_hit_rate") if same else None,
+                # VALU instructions issued per CU cycle (one wave64 instruction per CU per cycle is the peak)
+                "valu_issue_frac": pmc.get("valu_issue_frac") if same else None,
+                "counters": (pmc_path.relative_to(ROOT).as_posix() if same else
+                             f"none for these kernel sources ({src})"),
                 "kernel": "k_trace", "kernel_ms": round(k_ms, 4),
-                "bytes_alg_per_launch": int(bytes_alg),
+                "bytes_alg_per_launch": int(bytes_alg), "node_bytes": NODE_BYTES,
                 "node_visits_per_ray": round(cnt["node_visits"] / n, 3),
                 "tri_tests_per_ray": round(cnt["tri_tests"] / n, 3)}
 

@@ -95,10 +95,11 @@ typedef struct rt_scene_desc {
 } rt_scene_desc;
 
 /* ---- acceleration structure --------------------------------------------- */
-/* 4-wide BVH node, 128 bytes (one HBM/L2 line), structure-of-arrays over the
- * four children.  child[k]: RT_CHILD_EMPTY, an inner node index, or a leaf
- * RT_LEAF_BIT | (count-1) << RT_LEAF_COUNT_SHIFT | first (index into the
- * leaf-ordered triangle records).  Node 0 is the root. */
+/* 4-wide BVH node as the build produces it, 128 bytes, structure-of-arrays
+ * over the four children, float boxes.  child[k]: RT_CHILD_EMPTY, an inner
+ * node index, or a leaf RT_LEAF_BIT | (count-1) << RT_LEAF_COUNT_SHIFT |
+ * first (index into the leaf-ordered triangle records).  Node 0 is the root.
+ * The traversal reads the 64-byte quantised form below. */
 #define RT_CHILD_EMPTY      0xFFFFFFFFu
 #define RT_LEAF_BIT         0x80000000u
 #define RT_LEAF_COUNT_SHIFT 27
@@ -111,6 +112,24 @@ typedef struct rt_bvh4_node {
     uint32_t child[4];
     uint32_t meta[4];            /* [0] = parent node index, rest reserved */
 } rt_bvh4_node;
+
+/* Traversal node, 64 bytes (two per 128-byte cache line): the same node
+ * with its children's boxes quantised to 8 bits against the node's own box
+ * (Ylitie, Karras and Laine, "Efficient incoherent ray traversal on GPUs
+ * through compressed wide BVHs", HPG 2017).  On axis k the plane of byte q
+ * is origin[k] + q * 2^exp[k]; low bounds are rounded down and high bounds
+ * up, so every quantised box contains the float box it stands for.  An
+ * empty child has low 255 and high 0.  child[] as in rt_bvh4_node. */
+typedef struct rt_bvh4q_node {
+    float origin[3];             /* low corner of the union of the children's boxes */
+    int8_t exp[3];               /* per-axis scale 2^exp                              */
+    uint8_t pad;
+    uint8_t lo_x[4], hi_x[4];
+    uint8_t lo_y[4], hi_y[4];
+    uint8_t lo_z[4], hi_z[4];
+    uint32_t parent, reserved;
+    uint32_t child[4];
+} rt_bvh4q_node;
 
 /* Leaf-ordered triangle record, 48 bytes: v0, edge0 = v1 - v0,
  * edge1 = v2 - v0 (triangle.cpp:4-17) and the caller's triangle index. */
@@ -138,7 +157,7 @@ typedef struct rt_bvh_info {
  * atomic per wave). */
 typedef struct rt_trace_counters {
     uint64_t rays;
-    uint64_t node_visits;     /* BVH4 nodes fetched (128 B each)             */
+    uint64_t node_visits;     /* traversal nodes fetched (64 B each)         */
     uint64_t tri_tests;       /* triangle records fetched (48 B each)        */
     uint64_t hits;
 } rt_trace_counters;
@@ -210,6 +229,11 @@ int  rt_build(rt_ctx* ctx, const rt_build_opts* opts, rt_bvh_info* info);
  * query sizes; *num_nodes / *num_tris are in-out capacities. */
 int  rt_bvh_export(rt_ctx* ctx, rt_bvh4_node* nodes, uint32_t* num_nodes,
                    rt_tri_rec* tris, uint32_t* num_tris);
+
+/* Copies the traversal's quantised nodes out (for checking and for
+ * replaying the walk on the CPU): *num_nodes is the in-out capacity (nodes
+ * NULL: query the count). */
+int  rt_bvh_export_traversal(rt_ctx* ctx, rt_bvh4q_node* nodes, uint32_t* num_nodes);
 
 /* Closest hit for n rays.  Replaces Scene::traceRay's geometric part
  * (graphic/scene.cpp:22-41 -> graphics_object.cpp:64-87 ->

@@ -53,6 +53,11 @@ NODE_DTYPE = np.dtype([
     ("lo_x", "<f4", 4), ("hi_x", "<f4", 4), ("lo_y", "<f4", 4), ("hi_y", "<f4", 4),
     ("lo_z", "<f4", 4), ("hi_z", "<f4", 4), ("child", "<u4", 4), ("meta", "<u4", 4),
 ])
+QNODE_DTYPE = np.dtype([
+    ("origin", "<f4", 3), ("exp", "i1", 3), ("pad", "u1"), ("lo_x", "u1", 4), ("hi_x", "u1", 4),
+    ("lo_y", "u1", 4), ("hi_y", "u1", 4), ("lo_z", "u1", 4), ("hi_z", "u1", 4), ("parent", "<u4"),
+    ("reserved", "<u4"), ("child", "<u4", 4),
+])
 TRI_DTYPE = np.dtype([("v0", "<f4", 3), ("id", "<u4"), ("e0", "<f4", 3), ("pad0", "<u4"),
                       ("e1", "<f4", 3), ("pad1", "<u4")])
 
@@ -129,6 +134,7 @@ assert C.sizeof(rt_hit) == 16 and HIT_DTYPE.itemsize == 16
 assert C.sizeof(rt_material) == 32 and MATERIAL_DTYPE.itemsize == 32
 assert C.sizeof(rt_bvh4_node) == 128 and NODE_DTYPE.itemsize == 128
 assert C.sizeof(rt_tri_rec) == 48 and TRI_DTYPE.itemsize == 48
+assert QNODE_DTYPE.itemsize == 64
 
 class rt_nrc_config(C.Structure):
     _fields_ = [("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
@@ -176,6 +182,7 @@ SIGNATURES = {
     "rt_set_material_colors": (C.c_int, [_P, _P, C.c_int]),
     "rt_build": (C.c_int, [_P, C.POINTER(rt_build_opts), C.POINTER(rt_bvh_info)]),
     "rt_bvh_export": (C.c_int, [_P, _P, C.POINTER(C.c_uint32), _P, C.POINTER(C.c_uint32)]),
+    "rt_bvh_export_traversal": (C.c_int, [_P, _P, C.POINTER(C.c_uint32)]),
     "rt_trace_closest": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_int, C.POINTER(rt_trace_counters)]),
     "rt_trace_closest_noalloc": (C.c_int, [_P, _P, C.c_uint32, _P, _P]),
     "rt_trace_any": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_int]),

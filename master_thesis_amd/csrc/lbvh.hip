@@ -15,6 +15,8 @@
 //      becomes a leaf over a contiguous range of the sorted order when its
 //      surface-area cost is no more than splitting it once (k_collapse)
 //   6. leaf-ordered triangle records v0 / edge0 / edge1 / id (k_tri_records)
+//   7. the traversal's 64-byte nodes: each node's child boxes quantised to
+//      8 bits against the node's own box, rounded outward (k_quantize)
 // The topology does not affect results: the traversal's answer is defined by
 // the closest-hit rule alone (see trace.hip).  Boxes are padded outward so the
 // box test never rejects a triangle the intersection test would accept.
@@ -482,6 +484,55 @@ inline uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs 
 
 }  // namespace
 
+// The traversal's 64-byte node (rt_bvh4q_node) from a build node: per axis
+// the origin is the low corner of the union of the valid children's boxes
+// and the scale the least power of two with 255 steps covering its extent;
+// low bounds round down and high bounds up (in double, where the offsets
+// from the origin are exact), so the quantised boxes contain the float ones.
+__global__ void k_quantize(const rt_bvh4_node* __restrict__ nodes, uint32_t n, rt_bvh4q_node* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const rt_bvh4_node nd = nodes[i];
+    const float* lo3[3] = {nd.lo_x, nd.lo_y, nd.lo_z};
+    const float* hi3[3] = {nd.hi_x, nd.hi_y, nd.hi_z};
+    rt_bvh4q_node q;
+    uint8_t* qlo[3] = {q.lo_x, q.lo_y, q.lo_z};
+    uint8_t* qhi[3] = {q.hi_x, q.hi_y, q.hi_z};
+    for (int k = 0; k < 3; ++k) {
+        float lo = INFINITY, hi = -INFINITY;
+        for (int c = 0; c < 4; ++c)
+            if (nd.child[c] != RT_CHILD_EMPTY) {
+                lo = fminf(lo, lo3[k][c]);
+                hi = fmaxf(hi, hi3[k][c]);
+            }
+        if (!(lo <= hi)) lo = hi = 0.0f;
+        const double ext = (double)hi - (double)lo;
+        int e = -100;
+        if (ext > 0.0) {
+            const double step = ext / 255.0;
+            e = ilogb(step);
+            if (ldexp(1.0, e) < step) ++e;
+            e = e < -100 ? -100 : (e > 100 ? 100 : e);
+        }
+        q.origin[k] = lo;
+        q.exp[k] = (int8_t)e;
+        for (int c = 0; c < 4; ++c) {
+            double a = 255.0, b = 0.0;
+            if (nd.child[c] != RT_CHILD_EMPTY) {
+                a = fmin(fmax(floor(ldexp((double)lo3[k][c] - (double)lo, -e)), 0.0), 255.0);
+                b = fmin(fmax(ceil(ldexp((double)hi3[k][c] - (double)lo, -e)), 0.0), 255.0);
+            }
+            qlo[k][c] = (uint8_t)a;
+            qhi[k][c] = (uint8_t)b;
+        }
+    }
+    q.pad = 0;
+    q.parent = nd.meta[0];
+    q.reserved = 0;
+    for (int c = 0; c < 4; ++c) q.child[c] = nd.child[c];
+    out[i] = q;
+}
+
 int radix_sort_pairs(rt_ctx* c, uint32_t* keys[2], uint32_t* vals[2], uint32_t n, uint32_t passes, int* cur_out) {
     int cur = 0;
     *cur_out = 0;
@@ -600,6 +651,9 @@ int build_lbvh(rt_ctx* c, uint32_t leaf_size) {
             q ^= 1;
         }
     }
+    if ((rc = ensure(c, c->qnodes, sizeof(rt_bvh4q_node) * ((size_t)num_nodes + 1)))) return rc;
+    hipLaunchKernelGGL(k_quantize, dim3(blocks_for(num_nodes, 256)), dim3(256), 0, st,
+                       (const rt_bvh4_node*)c->nodes.ptr, num_nodes, (rt_bvh4q_node*)c->qnodes.ptr);
     RT_HIP(c, hipGetLastError());
     RT_HIP(c, hipEventRecord(c->ev[1], st));
     RT_HIP(c, hipEventSynchronize(c->ev[1]));

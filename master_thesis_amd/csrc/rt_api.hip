@@ -164,7 +164,7 @@ void rt_destroy(rt_ctx* c) {
     for (rt_nrc* n : caches) nrc_detach(n);
     std::vector<rt_ddgi*> fields = c->probe_fields;
     for (rt_ddgi* g : fields) ddgi_detach(g);
-    DevBuf* all[] = {&c->pos, &c->idx, &c->mat, &c->shade_geom, &c->shade_nrm, &c->vnrm, &c->nodes, &c->tris,
+    DevBuf* all[] = {&c->pos, &c->idx, &c->mat, &c->shade_geom, &c->shade_nrm, &c->vnrm, &c->nodes, &c->qnodes, &c->tris,
                      &c->keys[0], &c->keys[1], &c->vals[0], &c->vals[1], &c->hist, &c->inner, &c->leafbox,
                      &c->innerbox, &c->boxtab, &c->queue[0], &c->queue[1], &c->counters, &c->rays, &c->hits,
                      &c->spill, &c->work, &c->stats, &c->pt_state, &c->pt_queue[0], &c->pt_queue[1],
@@ -353,6 +353,20 @@ int rt_bvh_export(rt_ctx* c, rt_bvh4_node* nodes, uint32_t* num_nodes, rt_tri_re
         RT_HIP(c, hipMemcpyAsync(nodes, c->nodes.ptr, sizeof(rt_bvh4_node) * c->num_nodes, hipMemcpyDeviceToHost,
                                  c->stream));
     if (tris) RT_HIP(c, hipMemcpyAsync(tris, c->tris.ptr, sizeof(rt_tri_rec) * c->T, hipMemcpyDeviceToHost, c->stream));
+    RT_HIP(c, hipStreamSynchronize(c->stream));
+    return RT_OK;
+}
+
+int rt_bvh_export_traversal(rt_ctx* c, rt_bvh4q_node* nodes, uint32_t* num_nodes) {
+    if (!c || !num_nodes) return RT_E_INVALID;
+    if (!c->built) return fail(c, RT_E_STATE, "rt_bvh_export_traversal: no BVH built");
+    const uint32_t cap = *num_nodes;
+    *num_nodes = c->num_nodes;
+    if (!nodes) return RT_OK;
+    if (cap < c->num_nodes) return fail(c, RT_E_INVALID, "rt_bvh_export_traversal: output capacity too small");
+    DeviceGuard g(c->device);
+    RT_HIP(c, hipMemcpyAsync(nodes, c->qnodes.ptr, sizeof(rt_bvh4q_node) * c->num_nodes, hipMemcpyDeviceToHost,
+                             c->stream));
     RT_HIP(c, hipStreamSynchronize(c->stream));
     return RT_OK;
 }

@@ -63,7 +63,7 @@ struct rt_ctx {
     uint32_t num_light_objs = 0;
     // BVH
     bool built = false;
-    rt::DevBuf nodes, tris;
+    rt::DevBuf nodes, tris, qnodes;  // build nodes (float boxes), triangle records, traversal nodes (64 B)
     uint32_t num_nodes = 0, bvh_depth = 0;
     rt_bvh_info info{};
     // build scratch

@@ -22,9 +22,11 @@
 // traversal on GPUs", HPG 2009) from a wave-local pool of consecutive rays.
 // One loop iteration is one traversal step.  Each lane keeps a 32-entry ring
 // stack in LDS; deeper entries spill to a per-lane global area sized from the
-// BVH depth.  The slab test reads each axis's near and far planes directly
-// (chosen by the sign of the direction), and the triangle test rejects most
-// candidates before its IEEE division (intersect_mt_fast).
+// BVH depth.  Nodes are the 64-byte quantised form (rt_bvh4q_node, two per
+// cache line): the slab test decodes each child plane with one FMA from two
+// per-node, per-axis terms, taking the near and far bytes by the sign of the
+// direction; the triangle test rejects most candidates before its IEEE
+// division (intersect_mt_fast).
 //
 // Any-hit mode (occlusion, k_trace<.., true>): the same walk, ending a ray at
 // the first triangle the test accepts; the output is one flag per ray.  As
@@ -49,7 +51,7 @@ struct TraceArgs {
 
     uint32_t n;
     const uint32_t* n_dev;  // when set, the ray count is read from device memory
-    const rt_bvh4_node* nodes;
+    const rt_bvh4q_node* nodes;  // the 64-byte quantised traversal nodes
     const rt_tri_rec* tris;
     uint32_t* work;        // [32 (1 + x)] = next ray of XCD x's slices (zero at launch)
     uint32_t* work_next;   // the other half of the buffer: zeroed here for the next launch
@@ -84,7 +86,7 @@ struct Lane {
     float best_d2, best_t, best_u, best_v;
     float seg_d2;  // segment query: accept hits with squared distance <= seg_d2
     uint32_t best_id;
-    uint32_t offx, offy, offz;  // byte offset of the near plane (lo or hi) per axis
+    uint32_t negx, negy, negz;  // per axis: 1 when the direction is negative
     uint32_t cur, sp, steps;
 };
 
@@ -103,11 +105,10 @@ __device__ __forceinline__ bool lane_start(Lane& L, const rt_ray& r) {
     L.nox = -L.o.x * L.ix;
     L.noy = -L.o.y * L.iy;
     L.noz = -L.o.z * L.iz;
-    // lo_x at byte 0, hi_x at 16, lo_y at 32 ...: for a positive direction the
-    // near plane is the low bound
-    L.offx = (__float_as_uint(L.ix) >> 31) * 16u;
-    L.offy = 32u + (__float_as_uint(L.iy) >> 31) * 16u;
-    L.offz = 64u + (__float_as_uint(L.iz) >> 31) * 16u;
+    // for a negative direction the near plane is the high bound
+    L.negx = __float_as_uint(L.ix) >> 31;
+    L.negy = __float_as_uint(L.iy) >> 31;
+    L.negz = __float_as_uint(L.iz) >> 31;
     L.best_d2 = INFINITY;
     L.best_t = L.best_u = L.best_v = 0.0f;
     L.best_id = RT_MISS;
@@ -124,23 +125,33 @@ template <bool STATS, bool ANY>
 __device__ __forceinline__ bool lane_step(const TraceArgs& a, Lane& L, uint32_t* stk, uint32_t tid, uint32_t gtid,
                                           uint64_t& c_nodes, uint64_t& c_tris) {
     if (!(L.cur & RT_LEAF_BIT)) {
-        const char* nb = (const char*)(a.nodes + L.cur);
-        const float4 nx = *(const float4*)(nb + L.offx), fx = *(const float4*)(nb + (L.offx ^ 16u));
-        const float4 ny = *(const float4*)(nb + L.offy), fy = *(const float4*)(nb + (L.offy ^ 16u));
-        const float4 nz = *(const float4*)(nb + L.offz), fz = *(const float4*)(nb + (L.offz ^ 16u));
-        const uint4 ch = *(const uint4*)(nb + 96);
+        // the 64-byte node as four 16-byte loads: origin + exponents, the x
+        // and y bytes, the z bytes, the child references.  The plane of byte
+        // q is origin + q * 2^e, so its entry distance is
+        // (origin - o) / d + q * (2^e / d): one FMA per plane from two
+        // per-axis terms, and the near plane's bytes are the low ones when
+        // the direction is positive.
+        const uint4* qp = (const uint4*)(a.nodes + L.cur);
+        const uint4 w0 = qp[0], w1 = qp[1], w2 = qp[2], ch = qp[3];
         if (STATS) ++c_nodes;
-        const float nxa[4] = {nx.x, nx.y, nx.z, nx.w}, fxa[4] = {fx.x, fx.y, fx.z, fx.w};
-        const float nya[4] = {ny.x, ny.y, ny.z, ny.w}, fya[4] = {fy.x, fy.y, fy.z, fy.w};
-        const float nza[4] = {nz.x, nz.y, nz.z, nz.w}, fza[4] = {fz.x, fz.y, fz.z, fz.w};
+        const int ex = (int)(int8_t)(w0.w & 255u), ey = (int)(int8_t)((w0.w >> 8) & 255u),
+                  ez = (int)(int8_t)((w0.w >> 16) & 255u);
+        const float ax = fmaf(__uint_as_float(w0.x), L.ix, L.nox), bx = ldexpf(L.ix, ex);
+        const float ay = fmaf(__uint_as_float(w0.y), L.iy, L.noy), by = ldexpf(L.iy, ey);
+        const float az = fmaf(__uint_as_float(w0.z), L.iz, L.noz), bz = ldexpf(L.iz, ez);
+        const uint32_t nxw = L.negx ? w1.y : w1.x, fxw = L.negx ? w1.x : w1.y;
+        const uint32_t nyw = L.negy ? w1.w : w1.z, fyw = L.negy ? w1.z : w1.w;
+        const uint32_t nzw = L.negz ? w2.y : w2.x, fzw = L.negz ? w2.x : w2.y;
         uint32_t ref[4] = {ch.x, ch.y, ch.z, ch.w};
         float key[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const float tn = fmaxf(fmaxf(fmaf(nxa[c], L.ix, L.nox), fmaf(nya[c], L.iy, L.noy)),
-                                   fmaxf(fmaf(nza[c], L.iz, L.noz), L.tlo));
-            const float tf = fminf(fminf(fmaf(fxa[c], L.ix, L.nox), fmaf(fya[c], L.iy, L.noy)),
-                                   fminf(fmaf(fza[c], L.iz, L.noz), L.tcull));
+            const float tn = fmaxf(fmaxf(fmaf((float)((nxw >> (8 * c)) & 255u), bx, ax),
+                                         fmaf((float)((nyw >> (8 * c)) & 255u), by, ay)),
+                                   fmaxf(fmaf((float)((nzw >> (8 * c)) & 255u), bz, az), L.tlo));
+            const float tf = fminf(fminf(fmaf((float)((fxw >> (8 * c)) & 255u), bx, ax),
+                                         fmaf((float)((fyw >> (8 * c)) & 255u), by, ay)),
+                                   fminf(fmaf((float)((fzw >> (8 * c)) & 255u), bz, az), L.tcull));
             key[c] = (tn <= tf && ref[c] != RT_CHILD_EMPTY) ? tn : INFINITY;
         }
         // sort the four (key, ref) pairs ascending: 5-comparator network
@@ -383,7 +394,7 @@ int launch(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, uint8_t*
     a.segs = d_segs;
     a.n = n;
     a.n_dev = n_dev;
-    a.nodes = (const rt_bvh4_node*)c->nodes.ptr;
+    a.nodes = (const rt_bvh4q_node*)c->qnodes.ptr;
     a.tris = (const rt_tri_rec*)c->tris.ptr;
     a.work = (uint32_t*)c->work.ptr + WORK_HALF * c->work_half;
     a.work_next = (uint32_t*)c->work.ptr + WORK_HALF * (c->work_half ^ 1u);
@@ -499,7 +510,7 @@ int trace_noalloc(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, u
     a.segs = nullptr;
     a.n = n;
     a.n_dev = nullptr;
-    a.nodes = (const rt_bvh4_node*)c->nodes.ptr;
+    a.nodes = (const rt_bvh4q_node*)c->qnodes.ptr;
     a.tris = (const rt_tri_rec*)c->tris.ptr;
     a.work = base;
     a.work_next = base + WORK_HALF;

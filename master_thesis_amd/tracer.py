@@ -84,6 +84,15 @@ class Tracer:
         self._check(self.lib.rt_bvh_export(self.h, nodes.ctypes.data, C.byref(nn), tris.ctypes.data, C.byref(nt)))
         return nodes, tris
 
+    def export_traversal(self) -> np.ndarray:
+        """The traversal's 64-byte quantised nodes (QNODE_DTYPE), node order
+        as export_bvh."""
+        nn = C.c_uint32(0)
+        self._check(self.lib.rt_bvh_export_traversal(self.h, None, C.byref(nn)))
+        q = np.zeros(nn.value, L.QNODE_DTYPE)
+        self._check(self.lib.rt_bvh_export_traversal(self.h, q.ctypes.data, C.byref(nn)))
+        return q
+
     # -- queries ------------------------------------------------------------
     def trace(self, rays, hits=None, counters: bool = False):
         """Closest hit.  rays: numpy RAY_DTYPE array (host) or a contiguous

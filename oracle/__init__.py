@@ -174,12 +174,22 @@ class CpuBVH:
 
 
 def trace_bvh4(nodes, tris, depth: int, rays: np.ndarray, threads: int = 8):
+    """Replay of the GPU walk over exported traversal nodes (QNODE_DTYPE)."""
     from master_thesis_amd._lib import HIT_DTYPE
     hits = np.zeros(rays.shape[0], HIT_DTYPE)
     cnt = (C.c_uint64 * 4)()
     load().or_trace_bvh4(_p(nodes), nodes.shape[0], _p(tris), tris.shape[0], depth, _p(np.ascontiguousarray(rays)),
                          rays.shape[0], _p(hits), threads, cnt)
     return hits, dict(rays=cnt[0], node_visits=cnt[1], tri_tests=cnt[2], hits=cnt[3])
+
+
+def check_quantized(qnodes, nodes) -> int:
+    """0 when the traversal's quantised nodes stand soundly for the build nodes."""
+    lib = load()
+    lib.or_check_quantized.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32]
+    lib.or_check_quantized.restype = C.c_int
+    assert qnodes.shape[0] == nodes.shape[0]
+    return lib.or_check_quantized(_p(qnodes), _p(nodes), nodes.shape[0])
 
 
 def check_bvh4(nodes, tris, scene):

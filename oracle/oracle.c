@@ -636,7 +636,7 @@ void or_trace_bvh(const or_bvh* b, const void* rays, uint32_t n, void* hits, int
 /* ------------------------------------------------------------------ */
 /* BVH4 traversal in the GPU kernel's exact order (trace.hip) */
 typedef struct {
-    const rt_bvh4_node* nodes;
+    const rt_bvh4q_node* nodes;
     const rt_tri_rec* tris;
     const rt_ray* rays;
     rt_hit* hits;
@@ -654,16 +654,23 @@ static best_t trace_bvh4_one(const bvh4_job* j, const rt_ray* r, uint64_t* c_nod
     uint32_t sp = 0, cur = 0;
     for (;;) {
         if (!(cur & RT_LEAF_BIT)) {
-            const rt_bvh4_node* nd = &j->nodes[cur];
+            /* quantised node: plane of byte q on axis k is origin + q * 2^exp,
+             * its distance (origin - o) / d + q * (2^exp / d) by one fma */
+            const rt_bvh4q_node* nd = &j->nodes[cur];
             (*c_nodes)++;
+            const float ax = fmaf(nd->origin[0], p.ix, p.nox), bx = ldexpf(p.ix, nd->exp[0]);
+            const float ay = fmaf(nd->origin[1], p.iy, p.noy), by = ldexpf(p.iy, nd->exp[1]);
+            const float az = fmaf(nd->origin[2], p.iz, p.noz), bz = ldexpf(p.iz, nd->exp[2]);
             float key[4];
             uint32_t ref[4];
             for (int c = 0; c < 4; ++c) {
-                int h;
-                float tn = slab(&p, nd->lo_x[c], nd->lo_y[c], nd->lo_z[c], nd->hi_x[c], nd->hi_y[c], nd->hi_z[c],
-                                tcull, &h);
+                const float x0 = fmaf((float)nd->lo_x[c], bx, ax), x1 = fmaf((float)nd->hi_x[c], bx, ax);
+                const float y0 = fmaf((float)nd->lo_y[c], by, ay), y1 = fmaf((float)nd->hi_y[c], by, ay);
+                const float z0 = fmaf((float)nd->lo_z[c], bz, az), z1 = fmaf((float)nd->hi_z[c], bz, az);
+                const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), p.tlo));
+                const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tcull));
                 ref[c] = nd->child[c];
-                key[c] = (h && ref[c] != RT_CHILD_EMPTY) ? tn : INFINITY;
+                key[c] = (tn <= tf && ref[c] != RT_CHILD_EMPTY) ? tn : INFINITY;
             }
 #define CSWAP(i, k)                                                              \
     if (key[k] < key[i]) {                                                       \
@@ -713,7 +720,7 @@ void or_trace_bvh4(const void* nodes, uint32_t num_nodes, const void* tris, uint
     (void)num_nodes;
     (void)T;
     bvh4_job* j = (bvh4_job*)calloc(1, sizeof(bvh4_job));
-    j->nodes = (const rt_bvh4_node*)nodes;
+    j->nodes = (const rt_bvh4q_node*)nodes;
     j->tris = (const rt_tri_rec*)tris;
     j->rays = (const rt_ray*)rays;
     j->hits = (rt_hit*)hits;
@@ -725,6 +732,37 @@ void or_trace_bvh4(const void* nodes, uint32_t num_nodes, const void* tris, uint
             for (int k = 0; k < 4; ++k) counters[k] += j->cnt[t][k];
     }
     free(j);
+}
+
+/* the quantised traversal nodes against the build nodes: same children and
+ * parent, every valid child's decoded box (origin + q * 2^exp, evaluated
+ * exactly in double) contains its float box, empty children encode as
+ * low 255 / high 0.  Returns 0 when sound. */
+int or_check_quantized(const void* qnodes_v, const void* nodes_v, uint32_t num_nodes) {
+    const rt_bvh4q_node* q = (const rt_bvh4q_node*)qnodes_v;
+    const rt_bvh4_node* f = (const rt_bvh4_node*)nodes_v;
+    for (uint32_t i = 0; i < num_nodes; ++i) {
+        if (q[i].parent != f[i].meta[0]) return 1;
+        const uint8_t* qlo[3] = {q[i].lo_x, q[i].lo_y, q[i].lo_z};
+        const uint8_t* qhi[3] = {q[i].hi_x, q[i].hi_y, q[i].hi_z};
+        const float* flo[3] = {f[i].lo_x, f[i].lo_y, f[i].lo_z};
+        const float* fhi[3] = {f[i].hi_x, f[i].hi_y, f[i].hi_z};
+        for (int c = 0; c < 4; ++c) {
+            if (q[i].child[c] != f[i].child[c]) return 2;
+            for (int k = 0; k < 3; ++k) {
+                if (f[i].child[c] == RT_CHILD_EMPTY) {
+                    if (qlo[k][c] != 255 || qhi[k][c] != 0) return 3;
+                    continue;
+                }
+                const double s = ldexp(1.0, q[i].exp[k]);
+                const double lo = (double)q[i].origin[k] + qlo[k][c] * s, hi = (double)q[i].origin[k] + qhi[k][c] * s;
+                if (!(lo <= (double)flo[k][c]) || !(hi >= (double)fhi[k][c])) return 4;
+                /* tight: one step looser at most */
+                if ((double)flo[k][c] - lo >= s || hi - (double)fhi[k][c] >= s) return 5;
+            }
+        }
+    }
+    return 0;
 }
 
 /* structural check: returns 0 when sound */

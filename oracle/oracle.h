@@ -59,11 +59,15 @@ void or_bvh_free(or_bvh* b);
 uint32_t or_bvh_num_nodes(const or_bvh* b);
 void or_trace_bvh(const or_bvh* b, const void* rays, uint32_t n, void* hits, int threads, uint64_t counters[4]);
 
-/* Traversal of an exported GPU BVH4 (rt_bvh4_node[], rt_tri_rec[]) with the
- * GPU kernel's exact visiting order: same hits and same node / triangle
- * counts as the GPU.  counters: rays, nodes, tris, hits. */
+/* Traversal of an exported GPU BVH4 (its quantised traversal nodes,
+ * rt_bvh4q_node[], and rt_tri_rec[]) with the GPU kernel's exact visiting
+ * order and plane arithmetic: same hits and same node / triangle counts as
+ * the GPU.  counters: rays, nodes, tris, hits. */
 void or_trace_bvh4(const void* nodes, uint32_t num_nodes, const void* tris, uint32_t T, uint32_t depth,
                    const void* rays, uint32_t n, void* hits, int threads, uint64_t counters[4]);
+/* The quantised traversal nodes against the build nodes (same children,
+ * decoded boxes contain the float boxes, within one step).  0 if sound. */
+int or_check_quantized(const void* qnodes, const void* nodes, uint32_t num_nodes);
 /* Structural check of an exported BVH4: every triangle in exactly one leaf,
  * every child box contains its subtree.  Returns 0 if sound, else a code. */
 int or_check_bvh4(const void* nodes, uint32_t num_nodes, const void* tris, uint32_t T, const float* pos,

@@ -63,10 +63,13 @@ def test_bvh_structure_and_counters(tracer):
     assert rc == 0, f"BVH4 check failed with code {rc}"
     assert depth == info["depth"]
     assert nodes.shape[0] == info["num_nodes"]
+    # the traversal's quantised nodes contain the build's float boxes
+    qn = tracer.export_traversal()
+    assert O.check_quantized(qn, nodes) == 0
     coh, inc = scenes.microbench_rays(128, 64, 8192, seed=3)
     rays = np.concatenate([coh, inc])
     got, cnt = tracer.trace(rays, counters=True)
-    want, ocnt = O.trace_bvh4(nodes, tris, info["depth"], rays)
+    want, ocnt = O.trace_bvh4(qn, tris, info["depth"], rays)
     _assert_same_hits(got, want, "bvh4 replay")
     assert cnt == ocnt, (cnt, ocnt)
     brute = O.trace_brute(sc, rays[::7].copy())

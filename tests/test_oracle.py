@@ -349,3 +349,59 @@ def test_faithful_mode_matches_oracle_estimator():
     assert abs(im.mean() - math.pi * E) < 5 * sd
     part = O.render_faithful(sc, cam, 4, 5, threads=2, seed=3, pixel_limit=600)[0].reshape(-1, 3)
     assert np.all(part[600:] == 0) and part[:600].max() > 0
+
+
+def _quantize_ref(lo, hi, valid):
+    """Outward 8-bit quantisation of child boxes (lo, hi: (4, 3)) against
+    their union, as rt_bvh4q_node stores it (restated here in numpy)."""
+    from master_thesis_amd._lib import QNODE_DTYPE
+    q = np.zeros((), QNODE_DTYPE)
+    for k, ax in enumerate("xyz"):
+        o = np.float32(lo[valid, k].min())
+        ext = float(hi[valid, k].max()) - float(o)
+        e = math.frexp(ext / 255.0)[1] if ext > 0 else -100
+        if math.ldexp(1.0, e - 1) >= ext / 255.0:
+            e -= 1
+        q["origin"][k] = o
+        q["exp"][k] = e
+        for c in range(4):
+            a = math.floor(math.ldexp(float(lo[c, k]) - float(o), -e)) if valid[c] else 255
+            b = math.ceil(math.ldexp(float(hi[c, k]) - float(o), -e)) if valid[c] else 0
+            q["lo_" + ax][c] = min(max(a, 0), 255)
+            q["hi_" + ax][c] = min(max(b, 0), 255)
+    return q
+
+
+def test_quantized_node_check_catches_inward_rounding():
+    """or_check_quantized accepts outward-rounded 8-bit child boxes and
+    rejects a node whose low bound was rounded to nearest (it can cut a
+    sliver off the float box), a changed child, or an empty child's code."""
+    from master_thesis_amd._lib import NODE_DTYPE, QNODE_DTYPE
+    rng = np.random.default_rng(9)
+    nodes = np.zeros(200, NODE_DTYPE)
+    qn = np.zeros(200, QNODE_DTYPE)
+    for i in range(200):
+        valid = np.array([True, True, i % 3 != 0, i % 5 != 0])
+        c = rng.uniform(-10, 10, (4, 3)).astype(np.float32)
+        ext = rng.lognormal(-2, 2, (4, 3)).astype(np.float32)
+        lo, hi = (c - ext).astype(np.float32), (c + ext).astype(np.float32)
+        for k, ax in enumerate("xyz"):
+            nodes[i]["lo_" + ax] = lo[:, k]
+            nodes[i]["hi_" + ax] = hi[:, k]
+        nodes[i]["child"] = np.where(valid, np.arange(4) + 4 * i + 1, 0xFFFFFFFF)
+        nodes[i]["meta"][0] = i // 4
+        qn[i] = _quantize_ref(lo, hi, valid)
+        qn[i]["child"] = nodes[i]["child"]
+        qn[i]["parent"] = i // 4
+    assert O.check_quantized(qn, nodes) == 0
+    bad = qn.copy()
+    for i in range(200):  # round the first child's low x bound to nearest
+        o, e = float(bad[i]["origin"][0]), int(bad[i]["exp"][0])
+        bad[i]["lo_x"][0] = min(255, round(math.ldexp(float(nodes[i]["lo_x"][0]) - o, -e)))
+    assert O.check_quantized(bad, nodes) == 4
+    bad = qn.copy()
+    bad[1]["child"][0] ^= 1
+    assert O.check_quantized(bad, nodes) == 2
+    bad = qn.copy()
+    bad[0]["lo_y"][2] = 0
+    assert O.check_quantized(bad, nodes) == 3
