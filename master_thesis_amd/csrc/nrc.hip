@@ -21,11 +21,13 @@
 //                striding over the tiles: encode and forward in registers as
 //                in inference, activations also stored transposed in LDS;
 //                loss; backward with the gradients in both orientations;
-//                each workgroup sums its own
-//                partial gradient over its tiles in registers and writes it
-//                once (no atomics, no read-modify-write of global memory)
-//   k_adam       sums the partials in workgroup order (deterministic), Adam
-//                update, BF16 refresh
+//                bias gradients come out of the weight-gradient products
+//                through a row of ones under the activations; each
+//                workgroup sums its own partial gradient over its tiles in
+//                registers and writes it once (no atomics, no
+//                read-modify-write of global memory)
+//   k_adam       sums the partials in a fixed order (eight interleaved runs,
+//                then their sum: deterministic), Adam update, BF16 refresh
 // Every GEMM is a sum of 16x16 output tiles of v_mfma_f32_16x16x32_bf16.
 #include <math.h>
 #include <stdint.h>
@@ -57,12 +59,17 @@ constexpr int NPARAM = offw(L);
 // bf16 weight layout: [W0 W1 ... W5]
 constexpr int offwb(int l) { int o = 0; for (int i = 0; i < l; ++i) o += ND[i] * KD[i]; return o; }
 constexpr int NWB = offwb(L);
-// training: 16 x 16 weight-gradient tiles a wave accumulates per layer (K / 16
-// for the hidden layers, one for the 16-row output layer) and their offsets
-constexpr int GW_OFF[L] = {0, 6, 10, 14, 18, 22};
-constexpr int GW_TILES = 23;
+// training: 16 x 16 weight-gradient tiles a wave accumulates per layer and
+// their offsets: K / 16 + 1 for the hidden layers (the last tile's first
+// column is the bias gradient: the activations carry a row of ones there),
+// 6 for the input layer (the encoding's padding column 78 is the ones row),
+// and for the 16-row output layer one weight tile plus its bias tile
+constexpr int GW_OFF[L] = {0, 6, 11, 16, 21, 26};
+constexpr int GW_TILES = 28;
+constexpr int BIAS_COL0 = 78;  // input layer: padding column that carries the ones row
 static_assert(NPARAM == 23888, "parameter count");
 static_assert(NWB == 23552, "bf16 weight count");
+
 
 struct Geo {
     float lo[3], inv[3];
@@ -305,7 +312,7 @@ __global__ void __launch_bounds__(NT) k_mlp_train(const float* __restrict__ feat
                                                   float* __restrict__ loss_part) {
     __shared__ __attribute__((aligned(16))) __bf16 sW[NWB];             // inference layout
     __shared__ __attribute__((aligned(16))) __bf16 sXT[96 * TS];        // encoding, [feature][sample]
-    __shared__ __attribute__((aligned(16))) __bf16 sHT[L - 1][64 * TS];  // ReLU activations, [feature][sample]
+    __shared__ __attribute__((aligned(16))) __bf16 sHT[L - 1][80 * TS];  // ReLU activations, [feature][sample], then a row of ones
     __shared__ __attribute__((aligned(16))) __bf16 sD[2][TILE * HS];     // gradients, [sample][feature]
     __shared__ __attribute__((aligned(16))) __bf16 sDT[2][64 * TS];      // gradients, [feature][sample]
     __shared__ float sLoss[NT / 64];
@@ -322,9 +329,13 @@ __global__ void __launch_bounds__(NT) k_mlp_train(const float* __restrict__ feat
     f32x4 gw[GW_TILES];
 #pragma unroll
     for (int i = 0; i < GW_TILES; ++i) gw[i] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    float gb[L];
-#pragma unroll
-    for (int l = 0; l < L; ++l) gb[l] = 0.0f;
+    // activation rows 64..79: ones, then zeros, so that each layer's last
+    // weight-gradient tile sums the output gradients over samples (the bias
+    // gradient) in its first column
+    for (int e = threadIdx.x; e < (L - 1) * 16 * TS; e += NT) {
+        const int l = e / (16 * TS), r = (e / TS) % 16;
+        sHT[l][(64 + r) * TS + e % TS] = tobf(r == 0 ? 1.0f : 0.0f);
+    }
     const uint32_t tiles = (n + TILE - 1) / TILE;
     for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
         // keeps the weight fragments' LDS loads inside the loop
@@ -342,9 +353,12 @@ __global__ void __launch_bounds__(NT) k_mlp_train(const float* __restrict__ feat
         for (int ks = 0; ks < 3; ++ks) {
             bf16x8 x[1];
             x[0] = feature_block(kg, ks, xa, feats + (size_t)(s0 + sw) * 9 + 3, valid);
-            // slot (ks, 8 kg + j) holds feature 24 kg + 8 ks + j
+            // slot (ks, 8 kg + j) holds feature 24 kg + 8 ks + j; the padding
+            // feature BIAS_COL0 is stored as 1 for the weight gradients
+            // (the forward product sees it as 0)
 #pragma unroll
-            for (int j = 0; j < 8; ++j) sXT[(24 * kg + 8 * ks + j) * TS + sw] = x[0][j];
+            for (int j = 0; j < 8; ++j)
+                sXT[(24 * kg + 8 * ks + j) * TS + sw] = (24 * kg + 8 * ks + j == BIAS_COL0) ? tobf(1.0f) : x[0][j];
             if (ks == 0) layer_block<64, 4, true, 1>(sW + offwb(0), ks, x, lane, bias, acc);
             else layer_block<64, 4, false, 1>(sW + offwb(0), ks, x, lane, bias, acc);
         }
@@ -406,11 +420,11 @@ __global__ void __launch_bounds__(NT) k_mlp_train(const float* __restrict__ feat
             const f32x4 g = tile_mm(sDT[0], TS, sHT[L - 2], TS, 0, 16 * w, 64, lane);
 #pragma unroll
             for (int i = 0; i < 4; ++i) gw[GW_OFF[L - 1]][i] += g[i];
-        }
-        if (threadIdx.x < 16) {
-            float sum = 0.0f;
-            for (int r = 0; r < TILE; ++r) sum += (float)sDT[0][threadIdx.x * TS + r];
-            gb[L - 1] += sum;
+            if (w == 0) {  // columns 64..79: the ones row, so column 64 is the bias gradient
+                const f32x4 gbias = tile_mm(sDT[0], TS, sHT[L - 2], TS, 0, 64, 64, lane);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) gw[GW_OFF[L - 1] + 1][i] += gbias[i];
+            }
         }
         // dH_{l-1}[s][k] = sum_n dZ[s][n] W_l[n][k] (n < nvalid), masked by H_{l-1} > 0, into both layouts
         auto act_grad = [&](int l, int nvalid, const __bf16* dZ, __bf16* dst, __bf16* dstT) {
@@ -449,17 +463,14 @@ __global__ void __launch_bounds__(NT) k_mlp_train(const float* __restrict__ feat
         for (int l = L - 2; l >= 0; --l) {
             const __bf16* inT = (l == 0) ? sXT : sHT[l - 1];
             const int K = KD[l];
-            // dW_l[n][k] = sum_s dZ[s][n] in[s][k]: rows n in 16-row tiles (one per wave), K / 16 column tiles
+            // dW_l[n][k] = sum_s dZ[s][n] in[s][k]: rows n in 16-row tiles (one per wave), K / 16
+            // column tiles, plus the ones-row tile (bias) for the hidden layers
+            const int KT = (l == 0) ? K / 16 : K / 16 + 1;
 #pragma unroll
-            for (int kt = 0; kt < K / 16; ++kt) {
+            for (int kt = 0; kt < KT; ++kt) {
                 const f32x4 g = tile_mm(sDT[cur], TS, inT, TS, 16 * w, 16 * kt, 64, lane);
 #pragma unroll
                 for (int i = 0; i < 4; ++i) gw[GW_OFF[l] + kt][i] += g[i];
-            }
-            if (threadIdx.x < 64) {
-                float sum = 0.0f;
-                for (int r = 0; r < TILE; ++r) sum += (float)sDT[cur][threadIdx.x * TS + r];
-                gb[l] += sum;
             }
             if (l > 0) {
                 act_grad(l, 64, sD[cur], sD[cur ^ 1], sDT[cur ^ 1]);
@@ -468,7 +479,8 @@ __global__ void __launch_bounds__(NT) k_mlp_train(const float* __restrict__ feat
             }
         }
     }  // tiles
-    // the workgroup's partial gradient, written once
+    // the workgroup's partial gradient, written once; the ones-row columns
+    // go to the biases
 #pragma unroll
     for (int l = 0; l < L; ++l) {
         const int K = KD[l];
@@ -480,11 +492,20 @@ __global__ void __launch_bounds__(NT) k_mlp_train(const float* __restrict__ feat
                 // layer 5 (16 x 64): wave w holds columns 16 w .. 16 w + 15 of all 16 rows
                 const int row = (l == L - 1) ? 4 * kg + i : 16 * w + 4 * kg + i;
                 const int col = (l == L - 1) ? 16 * w + mi : 16 * kt + mi;
-                part[offw(l) + row * K + col] = gw[GW_OFF[l] + kt][i];
+                const float g = gw[GW_OFF[l] + kt][i];
+                if (l == 0 && col == BIAS_COL0) {
+                    part[offb(0) + row] = g;
+                    part[offw(0) + row * K + col] = 0.0f;
+                } else {
+                    part[offw(l) + row * K + col] = g;
+                }
             }
         }
-        const int nb = (l == L - 1) ? 16 : 64;
-        if ((int)threadIdx.x < nb) part[offb(l) + threadIdx.x] = gb[l];
+        if (l > 0 && mi == 0 && (l < L - 1 || w == 0)) {
+            const int bt = (l == L - 1) ? GW_OFF[l] + 1 : GW_OFF[l] + K / 16;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) part[offb(l) + ((l == L - 1) ? 4 * kg + i : 16 * w + 4 * kg + i)] = gw[bt][i];
+        }
     }
     if (threadIdx.x == 0) loss_part[blockIdx.x] = lacc;
 }
@@ -492,21 +513,45 @@ __global__ void __launch_bounds__(NT) k_mlp_train(const float* __restrict__ feat
 // Adam with bias correction on the summed partial gradients (workgroup order,
 // so the sum is deterministic); refreshes the bf16 weight copy.  With lr = 0
 // it only writes the summed gradient (gradient queries).
-__global__ void k_adam(const float* __restrict__ partial, uint32_t nparts, float* __restrict__ grad,
-                       float* __restrict__ Wf, float* __restrict__ m, float* __restrict__ v, __bf16* __restrict__ Wb,
-                       float lr, float b1, float b2, float eps, float bc1, float bc2, int update) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= NPARAM) return;
-    float g = 0.0f;
-    uint32_t w = 0;
-    for (; w + 8 <= nparts; w += 8) {  // eight loads in flight, summed in order
-        float v[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = partial[(size_t)(w + j) * NPARAM + p];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) g += v[j];
+// parameter p's BF16 copy (weights only; the biases stay FP32)
+__device__ __forceinline__ void store_bf16(__bf16* Wb, int p, float w) {
+    for (int l = 0; l < L; ++l) {
+        const int o = offw(l), nw = ND[l] * KD[l];
+        if (p >= o && p < o + nw) Wb[offwb(l) + (p - o)] = tobf(w);
     }
-    for (; w < nparts; ++w) g += partial[(size_t)w * NPARAM + p];
+}
+
+// A block of ADAM_NT threads serves 64 parameters: wave q sums partials
+// q, q + QN, q + 2 QN, ... in order (each load one coalesced 256-byte row,
+// eight in flight), then wave 0 adds the QN sums in wave order.
+constexpr int ADAM_NT = 512;
+constexpr int QN = ADAM_NT / 64;
+__global__ void __launch_bounds__(ADAM_NT) k_adam(const float* __restrict__ partial, uint32_t nparts,
+                                                  float* __restrict__ grad, float* __restrict__ Wf,
+                                                  float* __restrict__ m, float* __restrict__ v,
+                                                  __bf16* __restrict__ Wb, float lr, float b1, float b2, float eps,
+                                                  float bc1, float bc2, int update) {
+    __shared__ float s[QN][64];
+    const int q = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int p = blockIdx.x * 64 + lane;
+    float g = 0.0f;
+    if (p < NPARAM) {
+        uint32_t w = q;
+        for (; w + 7 * QN < nparts; w += 8 * QN) {
+            float x[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) x[j] = partial[(size_t)(w + QN * j) * NPARAM + p];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) g += x[j];
+        }
+        for (; w < nparts; w += QN) g += partial[(size_t)w * NPARAM + p];
+    }
+    s[q][lane] = g;
+    __syncthreads();
+    if (q != 0 || p >= NPARAM) return;
+    g = s[0][lane];
+#pragma unroll
+    for (int k = 1; k < QN; ++k) g += s[k][lane];
     grad[p] = g;
     if (!update) return;
     const float mm = b1 * m[p] + (1.0f - b1) * g;
@@ -515,19 +560,13 @@ __global__ void k_adam(const float* __restrict__ partial, uint32_t nparts, float
     v[p] = vv;
     const float wnew = Wf[p] - lr * (mm / bc1) / (sqrtf(vv / bc2) + eps);
     Wf[p] = wnew;
-    for (int l = 0; l < L; ++l) {
-        const int o = offw(l), nw = ND[l] * KD[l];
-        if (p >= o && p < o + nw) Wb[offwb(l) + (p - o)] = tobf(wnew);
-    }
+    store_bf16(Wb, p, wnew);
 }
 
 __global__ void k_refresh_bf16(const float* __restrict__ Wf, __bf16* __restrict__ Wb) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= NPARAM) return;
-    for (int l = 0; l < L; ++l) {
-        const int o = offw(l), nw = ND[l] * KD[l];
-        if (p >= o && p < o + nw) Wb[offwb(l) + (p - o)] = tobf(Wf[p]);
-    }
+    store_bf16(Wb, p, Wf[p]);
 }
 
 // batch sample j <- record b + stride * j
@@ -610,7 +649,7 @@ int run_train(rt_nrc* s, const float* feats, const float* targets, uint32_t n, i
         bc1 = 1.0f - powf(s->cfg.beta1, (float)s->steps);
         bc2 = 1.0f - powf(s->cfg.beta2, (float)s->steps);
     }
-    hipLaunchKernelGGL(k_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, c->stream, (const float*)s->partial.ptr, wgs,
+    hipLaunchKernelGGL(k_adam, dim3((NPARAM + 63) / 64), dim3(ADAM_NT), 0, c->stream, (const float*)s->partial.ptr, wgs,
                        (float*)s->grad.ptr, (float*)s->wf.ptr, (float*)s->m.ptr, (float*)s->v.ptr,
                        (__bf16*)s->wb.ptr, s->cfg.lr, s->cfg.beta1, s->cfg.beta2, s->cfg.eps, bc1, bc2,
                        update ? 1 : 0);

@@ -1013,11 +1013,14 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
     // the capacity is practically never reached and no record is dropped)
     const size_t q_cap = nrc ? 8 * szS : 0;
     size_t t_cap = nrc ? (size_t)4 * nf->opt.batch * nf->opt.max_steps : 0;
-    if (nrc && nf->opt.train_every && !(nf->opt.flags & RT_NRC_NO_RECORDS)) {
+    const bool records = nrc && nf->opt.train_every && !(nf->opt.flags & RT_NRC_NO_RECORDS);
+    if (records) {
         const size_t expect = (size_t)((P * N + nf->opt.train_every - 1) / nf->opt.train_every);
         if (2 * expect + 4096 > t_cap) t_cap = 2 * expect + 4096;
+        // a record's sort key is its sample's index in the frame
+        if (P * N > 0xFFFFFFFFull)
+            return fail(c, RT_E_UNSUPPORTED, "rt_render_nrc: training records with pixels * samples above 2^32");
     }
-    if (nrc && P * N > 0xFFFFFFFFull) return fail(c, RT_E_UNSUPPORTED, "rt_render_nrc: pixels * samples above 2^32");
     if (t_cap > 0x3FFFFFFFull) return fail(c, RT_E_UNSUPPORTED, "rt_render_nrc: training records above 2^30");
     // shadow rays (one per live slot and iteration): ray, 6 weights, 2 indices, flag
     const size_t sh_bytes = szS * (sizeof(rt_ray) + 24 + 8 + 1);
