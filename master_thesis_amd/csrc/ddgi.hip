@@ -168,17 +168,19 @@ __global__ void __launch_bounds__(DG_BLOCK) k_ddgi_eval(DdgiGrid g, const float4
 }
 
 // acc[3 idx[i] + ch] += w[3 i + ch] * lookup_i[ch]
+// n_dev (may be NULL): the query count is read from device memory (at most n_max)
 __global__ void __launch_bounds__(DG_BLOCK) k_ddgi_scatter(DdgiGrid g, const float4* atl, const float* f,
-                                                          const float* w, const uint32_t* idx, uint32_t n,
-                                                          float* acc) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float* q = f + 9 * (size_t)i;
-    const f3 e = ddgi_lookup(g, atl, mk(q[0], q[1], q[2]), mk(q[3], q[4], q[5]), mk(q[6], q[7], q[8]));
-    float* a = acc + 3 * (size_t)idx[i];
-    atomicAdd(a, w[3 * (size_t)i] * e.x);
-    atomicAdd(a + 1, w[3 * (size_t)i + 1] * e.y);
-    atomicAdd(a + 2, w[3 * (size_t)i + 2] * e.z);
+                                                          const float* w, const uint32_t* idx, uint32_t n_max,
+                                                          const uint32_t* n_dev, float* acc) {
+    const uint32_t n = n_dev ? min(*n_dev, n_max) : n_max;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float* q = f + 9 * (size_t)i;
+        const f3 e = ddgi_lookup(g, atl, mk(q[0], q[1], q[2]), mk(q[3], q[4], q[5]), mk(q[6], q[7], q[8]));
+        float* a = acc + 3 * (size_t)idx[i];
+        atomicAdd(a, w[3 * (size_t)i] * e.x);
+        atomicAdd(a + 1, w[3 * (size_t)i + 1] * e.y);
+        atomicAdd(a + 2, w[3 * (size_t)i + 2] * e.z);
+    }
 }
 
 inline uint32_t blocks(uint64_t n) { return (uint32_t)((n + DG_BLOCK - 1) / DG_BLOCK); }
@@ -236,11 +238,14 @@ int check_kind(rt_ddgi* g, int ptr_kind, const char* fn) {
 namespace rt {
 
 int ddgi_query_scatter(rt_ddgi* g, const float* d_feats, const float* d_w, const uint32_t* d_idx, uint32_t n,
-                       float* d_acc) {
+                       float* d_acc, const uint32_t* d_n) {
     if (n == 0) return RT_OK;
     rt_ctx* c = g->ctx;
-    hipLaunchKernelGGL(k_ddgi_scatter, dim3(blocks(n)), dim3(DG_BLOCK), 0, c->stream, g->grid,
-                       (const float4*)g->atl.ptr, d_feats, d_w, d_idx, n, d_acc);
+    // with a device count, a grid of a few waves per CU strides over the queries
+    uint32_t grid = blocks(n);
+    if (d_n && grid > 8u * (uint32_t)c->num_cus) grid = 8u * (uint32_t)c->num_cus;
+    hipLaunchKernelGGL(k_ddgi_scatter, dim3(grid), dim3(DG_BLOCK), 0, c->stream, g->grid,
+                       (const float4*)g->atl.ptr, d_feats, d_w, d_idx, n, d_n, d_acc);
     RT_HIP(c, hipGetLastError());
     return RT_OK;
 }

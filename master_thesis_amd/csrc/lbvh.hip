@@ -32,9 +32,11 @@ constexpr int RS_TILE = RS_BLOCK * RS_ITEMS;
 
 // counters[] slots
 constexpr int CTR_NODES = 0;
-constexpr int CTR_QOUT = 1;
+constexpr int CTR_Q0 = 1;      // entries of the collapse queues 0 and 1 (one level reads one, appends to the other)
 constexpr int CTR_LEAVES = 2;
 constexpr int CTR_ERROR = 3;   // set when a structural bound is exceeded (never for a valid tree)
+constexpr int CTR_Q1 = 4;
+constexpr int CTR_DEPTH = 5;   // levels with a non-empty input queue
 
 __device__ __forceinline__ uint32_t expand10(uint32_t v) {
     v = (v * 0x00010001u) & 0xFF0000FFu;
@@ -363,79 +365,93 @@ __device__ __forceinline__ void make_entry(uint32_t ref, const Inner& in, const 
 // atomic per counter and workgroup.
 constexpr uint32_t COLLAPSE_BLOCK = 128;
 
-__global__ void __launch_bounds__(COLLAPSE_BLOCK) k_collapse(const uint4* __restrict__ qin, uint32_t nq,
+__global__ void __launch_bounds__(COLLAPSE_BLOCK) k_collapse(const uint4* __restrict__ qin,
+                                                             const uint32_t* __restrict__ nq_dev,
                                                              uint4* __restrict__ qout, Inner in,
                                                              const uint32_t* __restrict__ vals, const float* leafbox,
                                                              const float* innerbox, uint32_t leaf_size, float alpha,
                                                              float pad, rt_bvh4_node* __restrict__ nodes,
                                                              uint32_t max_nodes, uint32_t max_queue,
-                                                             uint32_t* counters) {
+                                                             uint32_t* counters, int out_slot) {
     __shared__ uint32_t lds_alloc[3 * (COLLAPSE_BLOCK / 64 + 1)];
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = k < nq;
-    uint4 it = make_uint4(0u, 0u, 0u, 0u);
-    Entry e[4];
-    int cnt = 0;
-    if (active) {
-        it = qin[k];
-        cnt = 2;
-        make_entry(in.left[it.x], in, vals, leafbox, innerbox, leaf_size, alpha, e[0]);
-        make_entry(in.right[it.x], in, vals, leafbox, innerbox, leaf_size, alpha, e[1]);
-        while (cnt < 4) {
-            int best = -1;
-            float best_a = -1.0f;
-            for (int c = 0; c < cnt; ++c) {
-                if (!e[c].leaf) {
-                    float a = half_area(e[c].b);
-                    if (a > best_a) { best_a = a; best = c; }
+    // the level's queue length, read on the device; the grid strides over it
+    const uint32_t nq = *nq_dev;
+    for (uint32_t k0 = blockIdx.x * COLLAPSE_BLOCK; k0 < nq; k0 += gridDim.x * COLLAPSE_BLOCK) {
+        const uint32_t k = k0 + threadIdx.x;
+        const bool active = k < nq;
+        uint4 it = make_uint4(0u, 0u, 0u, 0u);
+        Entry e[4];
+        int cnt = 0;
+        if (active) {
+            it = qin[k];
+            cnt = 2;
+            make_entry(in.left[it.x], in, vals, leafbox, innerbox, leaf_size, alpha, e[0]);
+            make_entry(in.right[it.x], in, vals, leafbox, innerbox, leaf_size, alpha, e[1]);
+            while (cnt < 4) {
+                int best = -1;
+                float best_a = -1.0f;
+                for (int c = 0; c < cnt; ++c) {
+                    if (!e[c].leaf) {
+                        float a = half_area(e[c].b);
+                        if (a > best_a) { best_a = a; best = c; }
+                    }
                 }
+                if (best < 0) break;
+                uint32_t r = e[best].ref;
+                make_entry(in.left[r], in, vals, leafbox, innerbox, leaf_size, alpha, e[best]);
+                make_entry(in.right[r], in, vals, leafbox, innerbox, leaf_size, alpha, e[cnt]);
+                ++cnt;
             }
-            if (best < 0) break;
-            uint32_t r = e[best].ref;
-            make_entry(in.left[r], in, vals, leafbox, innerbox, leaf_size, alpha, e[best]);
-            make_entry(in.right[r], in, vals, leafbox, innerbox, leaf_size, alpha, e[cnt]);
-            ++cnt;
         }
-    }
-    uint32_t inner = 0, leaves = 0;
-    for (int c = 0; c < cnt; ++c) {
-        if (e[c].leaf) ++leaves;
-        else ++inner;
-    }
-    const uint32_t counts[3] = {inner, inner, leaves};
-    uint32_t* const ctr[3] = {&counters[CTR_NODES], &counters[CTR_QOUT], &counters[CTR_LEAVES]};
-    uint32_t base[3];
-    block_alloc_counts<COLLAPSE_BLOCK, 3>(counts, ctr, lds_alloc, base);
-    if (!active) return;
-    rt_bvh4_node nd;
-    uint32_t j = 0;
-    for (int c = 0; c < 4; ++c) {
-        if (c < cnt) {
-            nd.lo_x[c] = e[c].b[0] - pad; nd.hi_x[c] = e[c].b[3] + pad;
-            nd.lo_y[c] = e[c].b[1] - pad; nd.hi_y[c] = e[c].b[4] + pad;
-            nd.lo_z[c] = e[c].b[2] - pad; nd.hi_z[c] = e[c].b[5] + pad;
-            if (e[c].leaf) {
-                nd.child[c] = RT_LEAF_BIT | ((e[c].count - 1u) << RT_LEAF_COUNT_SHIFT) | e[c].first;
-            } else {
-                const uint32_t slot = base[0] + j, q = base[1] + j;
-                ++j;
-                if (slot < max_nodes && q < max_queue) {
-                    qout[q] = make_uint4(e[c].ref, slot, it.y, 0u);
-                    nd.child[c] = slot;
+        uint32_t inner = 0, leaves = 0;
+        for (int c = 0; c < cnt; ++c) {
+            if (e[c].leaf) ++leaves;
+            else ++inner;
+        }
+        const uint32_t counts[3] = {inner, inner, leaves};
+        uint32_t* const ctr[3] = {&counters[CTR_NODES], &counters[out_slot], &counters[CTR_LEAVES]};
+        uint32_t base[3];
+        block_alloc_counts<COLLAPSE_BLOCK, 3>(counts, ctr, lds_alloc, base);
+        if (!active) continue;
+        rt_bvh4_node nd;
+        uint32_t j = 0;
+        for (int c = 0; c < 4; ++c) {
+            if (c < cnt) {
+                nd.lo_x[c] = e[c].b[0] - pad; nd.hi_x[c] = e[c].b[3] + pad;
+                nd.lo_y[c] = e[c].b[1] - pad; nd.hi_y[c] = e[c].b[4] + pad;
+                nd.lo_z[c] = e[c].b[2] - pad; nd.hi_z[c] = e[c].b[5] + pad;
+                if (e[c].leaf) {
+                    nd.child[c] = RT_LEAF_BIT | ((e[c].count - 1u) << RT_LEAF_COUNT_SHIFT) | e[c].first;
                 } else {
-                    counters[CTR_ERROR] = 1u;
-                    nd.child[c] = RT_CHILD_EMPTY;
+                    const uint32_t slot = base[0] + j, q = base[1] + j;
+                    ++j;
+                    if (slot < max_nodes && q < max_queue) {
+                        qout[q] = make_uint4(e[c].ref, slot, it.y, 0u);
+                        nd.child[c] = slot;
+                    } else {
+                        counters[CTR_ERROR] = 1u;
+                        nd.child[c] = RT_CHILD_EMPTY;
+                    }
                 }
+            } else {
+                nd.lo_x[c] = nd.lo_y[c] = nd.lo_z[c] = INFINITY;
+                nd.hi_x[c] = nd.hi_y[c] = nd.hi_z[c] = -INFINITY;
+                nd.child[c] = RT_CHILD_EMPTY;
             }
-        } else {
-            nd.lo_x[c] = nd.lo_y[c] = nd.lo_z[c] = INFINITY;
-            nd.hi_x[c] = nd.hi_y[c] = nd.hi_z[c] = -INFINITY;
-            nd.child[c] = RT_CHILD_EMPTY;
         }
+        nd.meta[0] = it.z;
+        nd.meta[1] = nd.meta[2] = nd.meta[3] = 0u;
+        nodes[it.y] = nd;
     }
-    nd.meta[0] = it.z;
-    nd.meta[1] = nd.meta[2] = nd.meta[3] = 0u;
-    nodes[it.y] = nd;
+}
+
+// Before a collapse level: the queue it appends to starts empty, and a
+// non-empty input queue counts one level of depth.
+__global__ void k_level_prep(uint32_t* counters, int in_slot, int out_slot) {
+    if (threadIdx.x == 0) {
+        if (counters[in_slot] > 0) counters[CTR_DEPTH] += 1u;
+        counters[out_slot] = 0u;
+    }
 }
 
 // Root of a scene with <= leaf_size triangles: one leaf child.
@@ -624,33 +640,49 @@ int build_lbvh(rt_ctx* c, uint32_t leaf_size) {
                                (const float*)(table + 6ull * nb * (k - 1)), table + 6ull * nb * k);
         hipLaunchKernelGGL(k_node_boxes, dim3(blocks_for(n - 1, 256)), dim3(256), 0, st, n - 1, in, (const float*)sbox,
                            (const float*)table, nb, innerbox);
-        uint32_t init[4] = {1u, 0u, 0u, 0u};
+        uint32_t init[6] = {1u, 1u, 0u, 0u, 0u, 0u};  // root node; queue 0 holds the root's entry
         RT_HIP(c, hipMemcpyAsync(counters, init, sizeof(init), hipMemcpyHostToDevice, st));
         uint4 root = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
         RT_HIP(c, hipMemcpyAsync(c->queue[0].ptr, &root, sizeof(root), hipMemcpyHostToDevice, st));
-        uint32_t nq = 1;
-        int q = 0;
-        depth = 0;
+        // One collapse level per launch, each reading its queue's length on
+        // the device (a persistent grid strides over it).  The host copies the
+        // counters to pinned memory after each level and reads the previous
+        // level's copy, so it never drains the stream; the level launched
+        // after the last non-empty one finds an empty queue and does nothing.
+        if ((rc = ensure_snap(c))) return rc;
         const uint32_t max_nodes = (uint32_t)(N + 1), max_queue = (uint32_t)(N / 2 + 2);
-        while (nq > 0) {
-            if (depth >= 128) return fail(c, RT_E_DEVICE, "rt_build: BVH deeper than 128 levels");
-            ++depth;
-            RT_HIP(c, hipMemsetAsync(counters + CTR_QOUT, 0, 4, st));
-            hipLaunchKernelGGL(k_collapse, dim3(blocks_for(nq, COLLAPSE_BLOCK)), dim3(COLLAPSE_BLOCK), 0, st, (const uint4*)c->queue[q].ptr, nq,
-                               (uint4*)c->queue[q ^ 1].ptr, in, (const uint32_t*)vals[cur], (const float*)leafbox,
-                               (const float*)innerbox, leaf_size, c->bvh_leaf_alpha, pad, nodes, max_nodes, max_queue,
-                               counters);
-            uint32_t h[4];
-            RT_HIP(c, hipMemcpyAsync(h, counters, sizeof(h), hipMemcpyDeviceToHost, st));
-            RT_HIP(c, hipStreamSynchronize(st));
-            nq = h[CTR_QOUT];
-            num_nodes = h[CTR_NODES];
-            num_leaves = h[CTR_LEAVES];
-            if (h[CTR_ERROR] || nq > max_queue || num_nodes > max_nodes)
-                return fail(c, RT_E_DEVICE, "rt_build: BVH collapse exceeded its structural bounds");
+        const uint32_t cgrid = 8u * (uint32_t)c->num_cus;
+        int q = 0;
+        for (uint32_t level = 0;; ++level) {
+            if (level >= 130) return fail(c, RT_E_DEVICE, "rt_build: BVH deeper than 128 levels");
+            const int in_slot = q ? CTR_Q1 : CTR_Q0, out_slot = q ? CTR_Q0 : CTR_Q1;
+            hipLaunchKernelGGL(k_level_prep, dim3(1), dim3(64), 0, st, counters, in_slot, out_slot);
+            hipLaunchKernelGGL(k_collapse, dim3(cgrid), dim3(COLLAPSE_BLOCK), 0, st, (const uint4*)c->queue[q].ptr,
+                               (const uint32_t*)&counters[in_slot], (uint4*)c->queue[q ^ 1].ptr, in,
+                               (const uint32_t*)vals[cur], (const float*)leafbox, (const float*)innerbox, leaf_size,
+                               c->bvh_leaf_alpha, pad, nodes, max_nodes, max_queue, counters, out_slot);
+            RT_HIP(c, hipGetLastError());
+            RT_HIP(c, hipMemcpyAsync(c->snap + 16 * (level & 1), counters, 8 * sizeof(uint32_t),
+                                     hipMemcpyDeviceToHost, st));
+            RT_HIP(c, hipEventRecord(c->snap_ev[level & 1], st));
             q ^= 1;
+            if (level == 0) continue;
+            RT_HIP(c, hipEventSynchronize(c->snap_ev[(level - 1) & 1]));
+            const uint32_t* h = c->snap + 16 * ((level - 1) & 1);
+            if (h[CTR_ERROR]) return fail(c, RT_E_DEVICE, "rt_build: BVH collapse exceeded its structural bounds");
+            // the queue level - 1 appended to is the one this level read
+            if (h[(level - 1) & 1 ? CTR_Q0 : CTR_Q1] == 0) break;
         }
+        uint32_t h[8];
+        RT_HIP(c, hipMemcpyAsync(h, counters, sizeof(h), hipMemcpyDeviceToHost, st));
+        RT_HIP(c, hipStreamSynchronize(st));
+        num_nodes = h[CTR_NODES];
+        num_leaves = h[CTR_LEAVES];
+        depth = h[CTR_DEPTH];
+        if (h[CTR_ERROR] || num_nodes > max_nodes)
+            return fail(c, RT_E_DEVICE, "rt_build: BVH collapse exceeded its structural bounds");
     }
+    RT_HIP(c, hipGetLastError());
     if ((rc = ensure(c, c->qnodes, sizeof(rt_bvh4q_node) * ((size_t)num_nodes + 1)))) return rc;
     hipLaunchKernelGGL(k_quantize, dim3(blocks_for(num_nodes, 256)), dim3(256), 0, st,
                        (const rt_bvh4_node*)c->nodes.ptr, num_nodes, (rt_bvh4q_node*)c->qnodes.ptr);

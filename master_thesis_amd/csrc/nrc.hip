@@ -212,12 +212,16 @@ __device__ __forceinline__ void load_bias(const float* __restrict__ Wf, int l, i
 }
 
 // SCATTER: out[3 * idx[s] + c] += qw[3 * s + c] * max(y_c, 0) (atomically) in
-// place of out[3 * s + c] = y_c.
+// place of out[3 * s + c] = y_c.  n_dev (may be NULL): the sample count is
+// read from device memory (at most n_max).
 template <bool SCATTER>
-__global__ void __launch_bounds__(INF_NT) __attribute__((amdgpu_waves_per_eu(3))) k_mlp_infer(const float* __restrict__ feats, uint32_t n,
+__global__ void __launch_bounds__(INF_NT) __attribute__((amdgpu_waves_per_eu(3))) k_mlp_infer(const float* __restrict__ feats, uint32_t n_max,
+                                                      const uint32_t* __restrict__ n_dev,
                                                       const __bf16* __restrict__ Wb, const float* __restrict__ Wf,
                                                       Geo G, float* __restrict__ out, const float* __restrict__ qw,
                                                       const uint32_t* __restrict__ idx) {
+    const uint32_t n = n_dev ? min(*n_dev, n_max) : n_max;
+    if (n == 0) return;
     __shared__ __attribute__((aligned(16))) __bf16 sW[NWB];
     load_weights_blocked(Wb, sW);
     __syncthreads();
@@ -678,19 +682,19 @@ int nrc_infer_device(rt_nrc* s, const float* d_feats, uint32_t n, float* d_out) 
     if (n == 0) return RT_OK;
     const uint32_t tiles = (n + INF_TILE - 1) / INF_TILE;
     const uint32_t grid = tiles < (uint32_t)s->infer_grid ? tiles : (uint32_t)s->infer_grid;
-    hipLaunchKernelGGL(k_mlp_infer<false>, dim3(grid), dim3(INF_NT), 0, c->stream, d_feats, n, (const __bf16*)s->wb.ptr,
+    hipLaunchKernelGGL(k_mlp_infer<false>, dim3(grid), dim3(INF_NT), 0, c->stream, d_feats, n, nullptr, (const __bf16*)s->wb.ptr,
                        (const float*)s->wf.ptr, s->geo, d_out, nullptr, nullptr);
     RT_HIP(c, hipGetLastError());
     return RT_OK;
 }
 int nrc_infer_scatter(rt_nrc* s, const float* d_feats, const float* d_w, const uint32_t* d_idx, uint32_t n,
-                      float* d_acc) {
+                      float* d_acc, const uint32_t* d_n) {
     rt_ctx* c = s->ctx;
     if (n == 0) return RT_OK;
     const uint32_t tiles = (n + INF_TILE - 1) / INF_TILE;
     const uint32_t grid = tiles < (uint32_t)s->infer_grid ? tiles : (uint32_t)s->infer_grid;
-    hipLaunchKernelGGL(k_mlp_infer<true>, dim3(grid), dim3(INF_NT), 0, c->stream, d_feats, n, (const __bf16*)s->wb.ptr,
-                       (const float*)s->wf.ptr, s->geo, d_acc, d_w, d_idx);
+    hipLaunchKernelGGL(k_mlp_infer<true>, dim3(grid), dim3(INF_NT), 0, c->stream, d_feats, n, d_n,
+                       (const __bf16*)s->wb.ptr, (const float*)s->wf.ptr, s->geo, d_acc, d_w, d_idx);
     RT_HIP(c, hipGetLastError());
     return RT_OK;
 }

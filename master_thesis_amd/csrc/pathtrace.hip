@@ -80,7 +80,8 @@ constexpr int C_TN = 7;     // training records made
 constexpr int C_QDROP = 8;  // queries dropped for want of room (stays 0: the host flushes in time)
 constexpr int C_SN = 9;     // shadow rays queued this iteration
 constexpr int C_SHRAYS = 10;  // 64-bit count of shadow rays (slots 10-11)
-constexpr int C_WORDS = 12;
+constexpr int C_QTOT = 12;    // 64-bit count of cache queries answered (slots 12-13)
+constexpr int C_WORDS = 14;
 
 struct PTArgs {
     const ShadeGeom* geom;
@@ -224,6 +225,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_init(PTArgs a) {
         a.ctr[C_N1] = 0;
         *(unsigned long long*)&a.ctr[C_RAYS] = 0ull;
         *(unsigned long long*)&a.ctr[C_SHRAYS] = 0ull;
+        *(unsigned long long*)&a.ctr[C_QTOT] = 0ull;
         a.ctr[C_SN] = 0;
         a.ctr[C_QN] = 0;
         a.ctr[C_TN] = 0;
@@ -888,6 +890,14 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_finish(PTArgs a) {
     }
 }
 
+// after a cache flush: the queries answered join the total, the queue restarts
+__global__ void k_flush_queries(uint32_t* ctr) {
+    if (threadIdx.x == 0) {
+        *(unsigned long long*)&ctr[C_QTOT] += ctr[C_QN];
+        ctr[C_QN] = 0;
+    }
+}
+
 __global__ void k_zero(float* p, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.0f;
@@ -1123,9 +1133,20 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
     const uint64_t bound = (uint64_t)div_up(items, S) * a.spc * s->max_depth + 2;
     int cur = 0;
     uint64_t iter = 0;
-    uint32_t live = S;
     uint32_t launches = 1;
-    while (live > 0) {
+    // The host enqueues chunks of 8 extend/shade iterations and never drains
+    // the stream inside the frame: after each chunk it copies the counters to
+    // pinned memory behind an event, and before deciding on the next chunk it
+    // waits only for the previous chunk's copy, so the device always has the
+    // current chunk queued.  Iterations run after the last path ended are
+    // no-ops (every kernel reads its count on the device).  Cache queries are
+    // answered once per chunk, the inference reading the queue's count on the
+    // device.
+    if ((rc = ensure_snap(c))) return rc;
+    int snap_cur[2] = {0, 0};
+    uint64_t chunk = 0;
+    bool done = false;
+    while (!done) {
         for (int rep = 0; rep < 8 && iter < bound; ++rep, ++iter) {
             // the trace clears the counters the shade step appends to (the
             // next queue's; the shadow rays')
@@ -1151,21 +1172,33 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
             launches += 2;
             cur ^= 1;
         }
-        uint32_t hc[C_WORDS];
-        RT_HIP(c, hipMemcpyAsync(hc, a.ctr, sizeof(hc), hipMemcpyDeviceToHost, stream));
-        RT_HIP(c, hipStreamSynchronize(stream));
-        live = hc[cur ? C_N1 : C_N0];
-        if (nrc && hc[C_QDROP] != 0) return fail(c, RT_E_DEVICE, "rt_render_nrc: cache query buffer overflow");
-        if (nrc && hc[C_QN] > 0) {
+        if (nrc && a.nrc_query) {
             // the queued samples' cache predictions, added to their pixels
-            if (nf->ddgi) rc = ddgi_query_scatter(nf->ddgi, a.q_feat, a.q_w, a.q_idx, hc[C_QN], a.partial);
-            else rc = nrc_infer_scatter(nf->cache, a.q_feat, a.q_w, a.q_idx, hc[C_QN], a.partial);
+            // (at most q_cap: 8 iterations of at most S queries each)
+            if (nf->ddgi) rc = ddgi_query_scatter(nf->ddgi, a.q_feat, a.q_w, a.q_idx, a.q_cap, a.partial, &a.ctr[C_QN]);
+            else rc = nrc_infer_scatter(nf->cache, a.q_feat, a.q_w, a.q_idx, a.q_cap, a.partial, &a.ctr[C_QN]);
             if (rc) return rc;
-            RT_HIP(c, hipMemsetAsync(&a.ctr[C_QN], 0, 4, stream));
-            nf->stats->queries += hc[C_QN];
-            ++launches;
+            hipLaunchKernelGGL(k_flush_queries, dim3(1), dim3(64), 0, stream, a.ctr);
+            RT_HIP(c, hipGetLastError());
+            launches += 2;
         }
-        if (live > 0 && iter >= bound) return fail(c, RT_E_DEVICE, "rt_render: paths still live past the iteration bound");
+        const int slot = (int)(chunk & 1);
+        RT_HIP(c, hipMemcpyAsync(c->snap + 16 * slot, a.ctr, C_WORDS * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                 stream));
+        RT_HIP(c, hipEventRecord(c->snap_ev[slot], stream));
+        snap_cur[slot] = cur;
+        ++chunk;
+        // the previous chunk's counters (or, at the iteration bound, this one's)
+        const bool at_bound = iter >= bound;
+        if (chunk >= 2 || at_bound) {
+            const int ps = at_bound ? slot : (int)((chunk - 2) & 1);
+            RT_HIP(c, hipEventSynchronize(c->snap_ev[ps]));
+            const uint32_t* hc = c->snap + 16 * ps;
+            if (nrc && hc[C_QDROP] != 0) return fail(c, RT_E_DEVICE, "rt_render_nrc: cache query buffer overflow");
+            const uint32_t live = hc[snap_cur[ps] ? C_N1 : C_N0];
+            if (live == 0) done = true;
+            else if (at_bound) return fail(c, RT_E_DEVICE, "rt_render: paths still live past the iteration bound");
+        }
     }
     if (!adj) {
         hipLaunchKernelGGL(k_pt_finish, dim3(div_up(P, PT_BLOCK)), dim3(PT_BLOCK), 0, stream, a);
@@ -1185,6 +1218,7 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
         RT_HIP(c, hipStreamSynchronize(stream));
         const uint32_t tn = hc[C_TN];
         nf->stats->records = tn;
+        nf->stats->queries = (uint64_t)hc[C_QTOT] | ((uint64_t)hc[C_QTOT + 1] << 32);
         nf->stats->shadow_rays = (uint64_t)hc[C_SHRAYS] | ((uint64_t)hc[C_SHRAYS + 1] << 32);
         const uint32_t avail = tn < a.t_cap ? tn : a.t_cap;
         c->nrc_records = avail;

@@ -45,6 +45,13 @@ int ensure(rt_ctx* c, DevBuf& b, size_t bytes) {
     return RT_OK;
 }
 
+int ensure_snap(rt_ctx* c) {
+    if (c->snap) return RT_OK;
+    RT_HIP(c, hipHostMalloc((void**)&c->snap, 2 * 16 * sizeof(uint32_t), hipHostMallocDefault));
+    for (int k = 0; k < 2; ++k) RT_HIP(c, hipEventCreateWithFlags(&c->snap_ev[k], hipEventDisableTiming));
+    return RT_OK;
+}
+
 static void release(rt_ctx* c, DevBuf& b) {
     if (b.ptr) (void)hipFree(b.ptr);
     b.ptr = nullptr;
@@ -175,6 +182,9 @@ void rt_destroy(rt_ctx* c) {
     for (DevBuf* b : all) release(c, *b);
     for (int k = 0; k < rt_ctx::NUM_EV; ++k)
         if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
+    for (int k = 0; k < 2; ++k)
+        if (c->snap_ev[k]) (void)hipEventDestroy(c->snap_ev[k]);
+    if (c->snap) (void)hipHostFree(c->snap);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
 }

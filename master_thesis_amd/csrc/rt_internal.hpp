@@ -87,6 +87,11 @@ struct rt_ctx {
     std::vector<rt_nrc*> caches;            // live radiance caches; rt_destroy detaches them
     std::vector<rt_ddgi*> probe_fields;     // live irradiance-probe fields; rt_destroy detaches them
     uint32_t render_iters = 0;
+    // pinned host copies of device counters (2 x 16 words) and their copies'
+    // completion events: the path tracer's and the BVH build's loops read
+    // them one step behind instead of draining the stream
+    uint32_t* snap = nullptr;
+    hipEvent_t snap_ev[2] = {nullptr, nullptr};
 
     // ev[0..1] bracket a whole call, ev[2..3] the last traversal kernel,
     // ev[4] the end of a cached render's path phase
@@ -102,6 +107,8 @@ namespace rt {
 int fail(rt_ctx* c, int code, const std::string& msg);
 int hip_fail(rt_ctx* c, hipError_t e, const char* what);
 int ensure(rt_ctx* c, DevBuf& b, size_t bytes);
+// allocates the pinned counter copies and their events once per context
+int ensure_snap(rt_ctx* c);
 
 #define RT_HIP(c, expr)                                      \
     do {                                                     \
@@ -145,7 +152,7 @@ int trace_noalloc(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, u
 int nrc_infer_device(rt_nrc* s, const float* d_feats, uint32_t n, float* d_out);
 // acc[3 * idx[i] + ch] += w[3 * i + ch] * max(prediction_i[ch], 0) (atomically)
 int nrc_infer_scatter(rt_nrc* s, const float* d_feats, const float* d_w, const uint32_t* d_idx, uint32_t n,
-                      float* d_acc);
+                      float* d_acc, const uint32_t* d_n = nullptr);
 // up to max_steps Adam steps over `count` records (feats 9 + targets 3
 // floats each, in two arrays); batch b of `batch` records takes records
 // b + stride * j with stride = count / batch.  Returns the steps taken.
@@ -166,7 +173,7 @@ struct NrcFrame {
 // ddgi.hip: probe-field lookups of queued cache queries, added to their
 // pixels: acc[3 * idx[i] + ch] += w[3 * i + ch] * lookup_i[ch]
 int ddgi_query_scatter(rt_ddgi* g, const float* d_feats, const float* d_w, const uint32_t* d_idx, uint32_t n,
-                       float* d_acc);
+                       float* d_acc, const uint32_t* d_n = nullptr);
 rt_ctx* ddgi_ctx(const rt_ddgi* g);
 // frees a field's device memory and cuts it from its context (rt_destroy)
 void ddgi_detach(rt_ddgi* g);
