@@ -27,7 +27,9 @@ struct rt_ddgi {
     uint32_t P = 0, R = 0, W = 0, H = 0;
     uint64_t updates = 0;
     float rot[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-    DevBuf irr, dep, atl, surf, rays, hits, sh_ray, sh_w, sh_occ, io_in, io_out;  // atl: texel records of irr + dep
+    // atl: the field, one 32-byte record per texel (irradiance RGB + 0, depth moments + 0, 0); the API's
+    // planar atlases are unpacked from it and packed into it
+    DevBuf atl, surf, rays, hits, sh_ray, sh_w, sh_occ, io_in, io_out;
 };
 
 namespace {
@@ -93,8 +95,8 @@ __device__ inline float depth_weight(float w, float s) {
 // per atlas texel; each texel sums its probe's rays in ray order.  Border
 // texels compute the interior texel they mirror, so they hold the same
 // value.  first: no previous field to blend.
-__global__ void __launch_bounds__(DG_BLOCK) k_ddgi_update(DdgiGrid g, const float* surf, float* irr, float* dep,
-                                                         float4* atl, uint32_t first, uint32_t ppb, uint32_t P) {
+__global__ void __launch_bounds__(DG_BLOCK) k_ddgi_update(DdgiGrid g, const float* surf, float4* atl, uint32_t first,
+                                                         uint32_t ppb, uint32_t P) {
     // a surfel is two float4: (dir.xyz, r), (g, b, distance, hit)
     __shared__ float4 ls[2 * MAX_RAYS];
     const uint32_t p0 = blockIdx.x * ppb;
@@ -137,25 +139,38 @@ __global__ void __launch_bounds__(DG_BLOCK) k_ddgi_update(DdgiGrid g, const floa
         const size_t q = tile_texel(g, p, tx, ty);
         if (!first) {
             const float h = g.hysteresis, h1 = 1.0f - g.hysteresis;
-            nr = h * irr[4 * q] + h1 * nr;
-            ng = h * irr[4 * q + 1] + h1 * ng;
-            nb = h * irr[4 * q + 2] + h1 * nb;
-            nd0 = h * dep[2 * q] + h1 * nd0;
-            nd1 = h * dep[2 * q + 1] + h1 * nd1;
+            const float4 pi = atl[2 * q], pd = atl[2 * q + 1];
+            nr = h * pi.x + h1 * nr;
+            ng = h * pi.y + h1 * ng;
+            nb = h * pi.z + h1 * nb;
+            nd0 = h * pd.x + h1 * nd0;
+            nd1 = h * pd.y + h1 * nd1;
         }
-        irr[4 * q] = nr; irr[4 * q + 1] = ng; irr[4 * q + 2] = nb; irr[4 * q + 3] = 0.0f;
-        dep[2 * q] = nd0; dep[2 * q + 1] = nd1;
         atl[2 * q] = make_float4(nr, ng, nb, 0.0f);
         atl[2 * q + 1] = make_float4(nd0, nd1, 0.0f, 0.0f);
     }
 }
 
-// texel records from the planar atlases (after rt_ddgi_set_atlas)
+// texel records from the planar atlases (rt_ddgi_set_atlas)
 __global__ void __launch_bounds__(DG_BLOCK) k_ddgi_pack(uint32_t n, const float* irr, const float* dep, float4* atl) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     atl[2 * (size_t)i] = make_float4(irr[4 * (size_t)i], irr[4 * (size_t)i + 1], irr[4 * (size_t)i + 2], 0.0f);
     atl[2 * (size_t)i + 1] = make_float4(dep[2 * (size_t)i], dep[2 * (size_t)i + 1], 0.0f, 0.0f);
+}
+
+// planar atlases from the texel records (rt_ddgi_get_atlas); either output may be NULL
+__global__ void __launch_bounds__(DG_BLOCK) k_ddgi_unpack(uint32_t n, const float4* atl, float* irr, float* dep) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = atl[2 * (size_t)i], b = atl[2 * (size_t)i + 1];
+    if (irr) {
+        irr[4 * (size_t)i] = a.x; irr[4 * (size_t)i + 1] = a.y; irr[4 * (size_t)i + 2] = a.z;
+        irr[4 * (size_t)i + 3] = 0.0f;
+    }
+    if (dep) {
+        dep[2 * (size_t)i] = b.x; dep[2 * (size_t)i + 1] = b.y;
+    }
 }
 
 __global__ void __launch_bounds__(DG_BLOCK) k_ddgi_eval(DdgiGrid g, const float4* atl, const float* f, uint32_t n,
@@ -209,8 +224,7 @@ int launch_update(rt_ddgi* g) {
     if (ppb > MAX_RAYS / g->R) ppb = MAX_RAYS / g->R;
     if (ppb < 1) ppb = 1;
     hipLaunchKernelGGL(k_ddgi_update, dim3((g->P + ppb - 1) / ppb), dim3(DG_BLOCK), 0, c->stream, g->grid,
-                       (const float*)g->surf.ptr, (float*)g->irr.ptr, (float*)g->dep.ptr, (float4*)g->atl.ptr,
-                       g->updates == 0 ? 1u : 0u, ppb, g->P);
+                       (const float*)g->surf.ptr, (float4*)g->atl.ptr, g->updates == 0 ? 1u : 0u, ppb, g->P);
     RT_HIP(c, hipGetLastError());
     g->updates += 1;
     return RT_OK;
@@ -257,7 +271,7 @@ void ddgi_detach(rt_ddgi* g) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    DevBuf* all[] = {&g->irr, &g->dep, &g->atl, &g->surf, &g->rays, &g->hits, &g->sh_ray, &g->sh_w, &g->sh_occ, &g->io_in, &g->io_out};
+    DevBuf* all[] = {&g->atl, &g->surf, &g->rays, &g->hits, &g->sh_ray, &g->sh_w, &g->sh_occ, &g->io_in, &g->io_out};
     for (DevBuf* b : all) {
         if (b->ptr) (void)hipFree(b->ptr);
         b->ptr = nullptr;
@@ -330,15 +344,12 @@ int rt_ddgi_create(rt_ctx* c, const rt_ddgi_config* cfg, rt_ddgi** out) {
     c->probe_fields.push_back(g);  // from here on rt_ddgi_destroy releases whatever was allocated
     const size_t texels = (size_t)W * H;
     int rc;
-    if ((rc = ensure(c, g->irr, 16 * texels)) || (rc = ensure(c, g->dep, 8 * texels)) ||
-        (rc = ensure(c, g->atl, 32 * texels)) ||
+    if ((rc = ensure(c, g->atl, 32 * texels)) ||
         (rc = ensure(c, g->surf, 32 * (size_t)P * g->R))) {
         rt_ddgi_destroy(g);
         return rc;
     }
-    hipError_t e = hipMemsetAsync(g->irr.ptr, 0, 16 * texels, c->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(g->dep.ptr, 0, 8 * texels, c->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(g->atl.ptr, 0, 32 * texels, c->stream);
+    hipError_t e = hipMemsetAsync(g->atl.ptr, 0, 32 * texels, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(g->surf.ptr, 0, 32 * (size_t)P * g->R, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
@@ -454,9 +465,20 @@ int rt_ddgi_get_atlas(rt_ddgi* g, float* irradiance, float* depth, int ptr_kind)
     rt_ctx* c = g->ctx;
     (void)hipSetDevice(c->device);
     const size_t texels = (size_t)g->W * g->H;
-    const hipMemcpyKind kind = ptr_kind == RT_PTR_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
-    if (irradiance) RT_HIP(c, hipMemcpyAsync(irradiance, g->irr.ptr, 16 * texels, kind, c->stream));
-    if (depth) RT_HIP(c, hipMemcpyAsync(depth, g->dep.ptr, 8 * texels, kind, c->stream));
+    float* di = irradiance;
+    float* dd = depth;
+    if (ptr_kind == RT_PTR_HOST) {  // unpack into device staging, then copy out
+        if ((rc = ensure(c, g->io_out, 24 * texels))) return rc;
+        di = irradiance ? (float*)g->io_out.ptr : nullptr;
+        dd = depth ? (float*)g->io_out.ptr + 4 * texels : nullptr;
+    }
+    hipLaunchKernelGGL(k_ddgi_unpack, dim3(blocks(texels)), dim3(DG_BLOCK), 0, c->stream, (uint32_t)texels,
+                       (const float4*)g->atl.ptr, di, dd);
+    RT_HIP(c, hipGetLastError());
+    if (ptr_kind == RT_PTR_HOST) {
+        if (irradiance) RT_HIP(c, hipMemcpyAsync(irradiance, di, 16 * texels, hipMemcpyDeviceToHost, c->stream));
+        if (depth) RT_HIP(c, hipMemcpyAsync(depth, dd, 8 * texels, hipMemcpyDeviceToHost, c->stream));
+    }
     RT_HIP(c, hipStreamSynchronize(c->stream));
     return RT_OK;
 }
@@ -470,11 +492,18 @@ int rt_ddgi_set_atlas(rt_ddgi* g, const float* irradiance, const float* depth, i
     rt_ctx* c = g->ctx;
     (void)hipSetDevice(c->device);
     const size_t texels = (size_t)g->W * g->H;
-    const hipMemcpyKind kind = ptr_kind == RT_PTR_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
-    RT_HIP(c, hipMemcpyAsync(g->irr.ptr, irradiance, 16 * texels, kind, c->stream));
-    RT_HIP(c, hipMemcpyAsync(g->dep.ptr, depth, 8 * texels, kind, c->stream));
-    hipLaunchKernelGGL(k_ddgi_pack, dim3(blocks(texels)), dim3(DG_BLOCK), 0, c->stream, (uint32_t)texels,
-                       (const float*)g->irr.ptr, (const float*)g->dep.ptr, (float4*)g->atl.ptr);
+    const float* si = irradiance;
+    const float* sd = depth;
+    if (ptr_kind == RT_PTR_HOST) {  // stage on the device, then pack
+        if ((rc = ensure(c, g->io_in, 24 * texels))) return rc;
+        RT_HIP(c, hipMemcpyAsync(g->io_in.ptr, irradiance, 16 * texels, hipMemcpyHostToDevice, c->stream));
+        RT_HIP(c, hipMemcpyAsync((float*)g->io_in.ptr + 4 * texels, depth, 8 * texels, hipMemcpyHostToDevice,
+                                 c->stream));
+        si = (const float*)g->io_in.ptr;
+        sd = (const float*)g->io_in.ptr + 4 * texels;
+    }
+    hipLaunchKernelGGL(k_ddgi_pack, dim3(blocks(texels)), dim3(DG_BLOCK), 0, c->stream, (uint32_t)texels, si, sd,
+                       (float4*)g->atl.ptr);
     RT_HIP(c, hipGetLastError());
     RT_HIP(c, hipStreamSynchronize(c->stream));
     if (g->updates == 0) g->updates = 1;  // later updates blend into the set field

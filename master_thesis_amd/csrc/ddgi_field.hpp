@@ -23,8 +23,13 @@ struct DdgiGrid {
     uint32_t flags;                   // RT_DDGI_*
 };
 
-// GLSL's normalize in single precision, as the probe lookup's shaders compute
-// it: a * (1 / sqrt(a . a)), IEEE square root and division
+// Single-precision normalisation for the probe lookup, chosen for speed: a *
+// (1 / sqrt(a . a)) with IEEE square root and division, the same on host and
+// device.  GLSL defines normalize only as x / length(x), and drivers may use
+// an approximate inverse square root, so whether this rounds as the
+// reference's shaders do is not pinned.  The update's texel directions
+// (oct_decode) keep the double normalize_ref: they are computed once per
+// texel and update, off the lookup's hot path.
 RT_HD f3 normalize_f(f3 a) {
     const float m = sqrtf(dot(a, a));
     if (m != 0.0f) {

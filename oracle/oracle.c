@@ -68,8 +68,9 @@ static inline v3 vnormalize(v3 a) {
     return a;
 }
 
-/* GLSL's normalize in single precision, as the probe lookup's shaders
- * compute it: a * (1 / sqrt(a . a)), IEEE square root and division */
+/* The probe lookup's single-precision normalisation (as the device's
+ * normalize_f): a * (1 / sqrt(a . a)), IEEE square root and division.  How
+ * the reference's GLSL normalize rounds is not pinned. */
 static inline v3 vnormalize_f(v3 a) {
     const float m = sqrtf(vdot(a, a));
     if (m != 0.0f) {
