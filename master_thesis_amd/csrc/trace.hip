@@ -121,6 +121,12 @@ __device__ __forceinline__ bool lane_start(Lane& L, const rt_ray& r) {
 
 // One traversal step (one inner node, or one leaf's triangles, then a pop).
 // Returns true when the ray is finished.
+// the child reference a sort key names (its two low bits)
+__device__ __forceinline__ uint32_t child_of(const uint32_t (&ref)[4], uint32_t key) {
+    const uint32_t lo = (key & 1u) ? ref[1] : ref[0], hi = (key & 1u) ? ref[3] : ref[2];
+    return (key & 2u) ? hi : lo;
+}
+
 template <bool STATS, bool ANY>
 __device__ __forceinline__ bool lane_step(const TraceArgs& a, Lane& L, uint32_t* stk, uint32_t tid, uint32_t gtid,
                                           uint64_t& c_nodes, uint64_t& c_tris) {
@@ -142,8 +148,13 @@ __device__ __forceinline__ bool lane_step(const TraceArgs& a, Lane& L, uint32_t*
         const uint32_t nxw = L.negx ? w1.y : w1.x, fxw = L.negx ? w1.x : w1.y;
         const uint32_t nyw = L.negy ? w1.w : w1.z, fyw = L.negy ? w1.z : w1.w;
         const uint32_t nzw = L.negz ? w2.y : w2.x, fzw = L.negz ? w2.x : w2.y;
-        uint32_t ref[4] = {ch.x, ch.y, ch.z, ch.w};
-        float key[4];
+        const uint32_t ref[4] = {ch.x, ch.y, ch.z, ch.w};
+        // sort key of child c: its entry distance's bits (non-negative, sign
+        // cleared so that -0 sorts as 0) with the two low mantissa bits
+        // replaced by c, or 0x7F800000 | c (above every finite distance) when
+        // the child is missed or empty; the keys then sort as integers and
+        // name their children
+        uint32_t pk[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             const float tn = fmaxf(fmaxf(fmaf((float)((nxw >> (8 * c)) & 255u), bx, ax),
@@ -152,31 +163,33 @@ __device__ __forceinline__ bool lane_step(const TraceArgs& a, Lane& L, uint32_t*
             const float tf = fminf(fminf(fmaf((float)((fxw >> (8 * c)) & 255u), bx, ax),
                                          fmaf((float)((fyw >> (8 * c)) & 255u), by, ay)),
                                    fminf(fmaf((float)((fzw >> (8 * c)) & 255u), bz, az), L.tcull));
-            key[c] = (tn <= tf && ref[c] != RT_CHILD_EMPTY) ? tn : INFINITY;
+            pk[c] = ((tn <= tf && ref[c] != RT_CHILD_EMPTY) ? (__float_as_uint(tn) & 0x7FFFFFFCu) : 0x7F800000u) |
+                    (uint32_t)c;
         }
-        // sort the four (key, ref) pairs ascending: 5-comparator network
-#define RT_CSWAP(i, j)                                                        \
-    if (key[j] < key[i]) {                                                    \
-        float tk = key[i]; key[i] = key[j]; key[j] = tk;                      \
-        uint32_t tr = ref[i]; ref[i] = ref[j]; ref[j] = tr;                   \
+        // ascending: 5-comparator network on integer min / max
+#define RT_CSWAP(i, j)                                  \
+    {                                                   \
+        const uint32_t lo = min(pk[i], pk[j]), hi = max(pk[i], pk[j]); \
+        pk[i] = lo;                                     \
+        pk[j] = hi;                                     \
     }
         RT_CSWAP(0, 1) RT_CSWAP(2, 3) RT_CSWAP(0, 2) RT_CSWAP(1, 3) RT_CSWAP(1, 2)
 #undef RT_CSWAP
-        if (key[0] < INFINITY) {
+        if (pk[0] < 0x7F800000u) {
 #pragma unroll
             for (int c = 3; c >= 1; --c) {
-                if (key[c] < INFINITY) {
+                if (pk[c] < 0x7F800000u) {
                     const uint32_t slot = (L.sp & (STACK_LDS - 1)) * TRACE_BLOCK + tid;
                     if (L.sp >= (uint32_t)STACK_LDS) {
                         const uint32_t deep = L.sp - STACK_LDS;
                         if (deep < a.spill_cap) a.spill[(size_t)deep * a.spill_stride + gtid] = stk[slot];
                         else *a.err = 1u;  // cannot happen for a stack sized from the depth
                     }
-                    stk[slot] = ref[c];
+                    stk[slot] = child_of(ref, pk[c]);
                     ++L.sp;
                 }
             }
-            L.cur = ref[0];
+            L.cur = child_of(ref, pk[0]);
             return false;
         }
     } else {

@@ -662,8 +662,10 @@ static best_t trace_bvh4_one(const bvh4_job* j, const rt_ray* r, uint64_t* c_nod
             const float ax = fmaf(nd->origin[0], p.ix, p.nox), bx = ldexpf(p.ix, nd->exp[0]);
             const float ay = fmaf(nd->origin[1], p.iy, p.noy), by = ldexpf(p.iy, nd->exp[1]);
             const float az = fmaf(nd->origin[2], p.iz, p.noz), bz = ldexpf(p.iz, nd->exp[2]);
-            float key[4];
-            uint32_t ref[4];
+            /* sort keys as the kernel forms them: the entry distance's bits
+             * (sign cleared) with the two low bits replaced by the child's
+             * slot, or 0x7F800000 | slot for a missed or empty child */
+            uint32_t pk[4], ref[4];
             for (int c = 0; c < 4; ++c) {
                 const float x0 = fmaf((float)nd->lo_x[c], bx, ax), x1 = fmaf((float)nd->hi_x[c], bx, ax);
                 const float y0 = fmaf((float)nd->lo_y[c], by, ay), y1 = fmaf((float)nd->hi_y[c], by, ay);
@@ -671,19 +673,21 @@ static best_t trace_bvh4_one(const bvh4_job* j, const rt_ray* r, uint64_t* c_nod
                 const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), p.tlo));
                 const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tcull));
                 ref[c] = nd->child[c];
-                key[c] = (tn <= tf && ref[c] != RT_CHILD_EMPTY) ? tn : INFINITY;
+                uint32_t bits;
+                memcpy(&bits, &tn, 4);
+                pk[c] = ((tn <= tf && ref[c] != RT_CHILD_EMPTY) ? (bits & 0x7FFFFFFCu) : 0x7F800000u) | (uint32_t)c;
             }
 #define CSWAP(i, k)                                                              \
-    if (key[k] < key[i]) {                                                       \
-        float tk = key[i]; key[i] = key[k]; key[k] = tk;                         \
-        uint32_t tr = ref[i]; ref[i] = ref[k]; ref[k] = tr;                      \
+    {                                                                            \
+        const uint32_t lo = pk[i] < pk[k] ? pk[i] : pk[k], hi = pk[i] < pk[k] ? pk[k] : pk[i]; \
+        pk[i] = lo; pk[k] = hi;                                                  \
     }
             CSWAP(0, 1) CSWAP(2, 3) CSWAP(0, 2) CSWAP(1, 3) CSWAP(1, 2)
 #undef CSWAP
-            if (key[0] < INFINITY) {
+            if (pk[0] < 0x7F800000u) {
                 for (int c = 3; c >= 1; --c)
-                    if (key[c] < INFINITY && sp < j->stack_cap) stack[sp++] = ref[c];
-                cur = ref[0];
+                    if (pk[c] < 0x7F800000u && sp < j->stack_cap) stack[sp++] = ref[pk[c] & 3u];
+                cur = ref[pk[0] & 3u];
                 continue;
             }
         } else {
