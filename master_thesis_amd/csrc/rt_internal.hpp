@@ -125,7 +125,9 @@ int radix_sort_pairs(rt_ctx* c, uint32_t* keys[2], uint32_t* vals[2], uint32_t n
 
 // trace.hip
 constexpr int TRACE_BLOCK = 256;
-constexpr int STACK_LDS = 32;
+// 16 LDS entries per lane (16 KiB per 256-lane block) leave room for six
+// waves per SIMD; deeper entries go to the global spill area.
+constexpr int STACK_LDS = 16;
 // n: ray count, or an upper bound on it when n_dev (device-resident count) is
 // set.  zero0 / zero1: device words the launch sets to 0 before any later
 // launch on the stream runs (queue counters of a wavefront loop, so that the

@@ -20,7 +20,7 @@
 // Execution: persistent waves; each lane fetches a new ray as soon as its ray
 // finishes (dynamic fetch, Aila & Laine, "Understanding the efficiency of ray
 // traversal on GPUs", HPG 2009) from a wave-local pool of consecutive rays.
-// One loop iteration is one traversal step.  Each lane keeps a 32-entry ring
+// One loop iteration is one traversal step.  Each lane keeps a 16-entry ring
 // stack in LDS; deeper entries spill to a per-lane global area sized from the
 // BVH depth.  Nodes are the 64-byte quantised form (rt_bvh4q_node, two per
 // cache line): the slab test decodes each child plane with one FMA from two
@@ -243,8 +243,12 @@ __device__ __forceinline__ bool lane_step(const TraceArgs& a, Lane& L, uint32_t*
 // (dynamic fetch, Aila & Laine 2009), from a wave-local pool of POOL_CHUNK
 // consecutive rays refilled with one global atomic, so that a wave is not
 // held up by its slowest ray.
+// Waves per SIMD: six for the closest-hit and any-hit walks (the walk is
+// latency bound, so the extra waves hide the node and triangle loads; seven
+// is slower), five for the counting variant, whose counters would spill at six.
 template <bool STATS, bool ANY>
-__global__ void __launch_bounds__(TRACE_BLOCK) k_trace(TraceArgs a) {
+__global__ void __launch_bounds__(TRACE_BLOCK)
+__attribute__((amdgpu_waves_per_eu(STATS ? 5 : 6, STATS ? 5 : 6))) k_trace(TraceArgs a) {
     __shared__ uint32_t stk[STACK_LDS * TRACE_BLOCK];
     const uint32_t tid = threadIdx.x;
     if (blockIdx.x == 0) {
