@@ -190,7 +190,8 @@ typedef struct rt_settings {
 } rt_settings;
 
 typedef struct rt_timing {
-    float last_kernel_ms;     /* device time of the dominant kernel of the last call   */
+    float last_kernel_ms;     /* device time of the dominant kernel of the last call   *
+                               * (traversal, post-filter; 0 after a render)            */
     float last_call_ms;       /* device time of the whole last call (first to last op) */
     uint32_t launches;        /* kernel launches the last call made                    */
     uint32_t iterations;      /* render: extend/shade iterations of the last call      */
@@ -349,7 +350,9 @@ int  rt_nrc_infer(rt_nrc* nrc, const float* feats, uint32_t n, float* out, int p
  * host memory of the parameter count; loss (may be NULL) the mean loss. */
 int  rt_nrc_gradient(rt_nrc* nrc, const float* feats, const float* targets, uint32_t n, float* grad,
                      float* loss, int ptr_kind);
-/* One Adam step on n samples; loss (may be NULL) receives the mean loss. */
+/* One Adam step on n samples; loss (may be NULL) receives the mean loss.
+ * rt_nrc_infer, rt_nrc_gradient and rt_nrc_train are not timed (no event
+ * markers around a short step): rt_get_timing after them reports zeros. */
 int  rt_nrc_train(rt_nrc* nrc, const float* feats, const float* targets, uint32_t n, float* loss, int ptr_kind);
 
 /* Path tracing with the cache (rt_render_nrc).  A sample's path runs as in

@@ -17,16 +17,14 @@
 //   k_mlp_infer  4 waves, weights -> LDS once, then persistent tiles of 128
 //                samples (32 per wave) carried through encoding and 6 layers
 //                in registers
-//   k_mlp_train  4 waves, tiles of 64 samples, at most one workgroup per CU
-//                striding over the tiles: encode and forward in registers as
-//                in inference, activations also stored transposed in LDS;
-//                loss; backward with the gradients in both orientations;
-//                bias gradients come out of the weight-gradient products
-//                through a row of ones under the activations; each
-//                workgroup sums its own partial gradient over its tiles in
-//                registers and writes it once (no atomics, no
-//                read-modify-write of global memory)
-//   k_adam       sums the partials in a fixed order (eight interleaved runs,
+//   k_mlp_train  8 waves (4 for small batches), tiles of 16 samples per
+//                wave, at most one workgroup per CU striding over the tiles:
+//                forward and the activation gradients in registers, the
+//                weight gradients of each layer from a [feature][sample]
+//                staging buffer in LDS; each workgroup sums its own partial
+//                gradient over its tiles in registers and writes it once
+//                (no atomics, no read-modify-write of global memory)
+//   k_adam       sums the partials in a fixed order (one run per wave,
 //                then their sum: deterministic), Adam update, BF16 refresh
 // Every GEMM is a sum of 16x16 output tiles of v_mfma_f32_16x16x32_bf16.
 #include <math.h>
@@ -49,8 +47,6 @@ constexpr int KD[L] = {96, 64, 64, 64, 64, 64};        // input width (padded)
 constexpr int ND[L] = {64, 64, 64, 64, 64, 16};        // output width (padded)
 constexpr int K0_VALID = 78, N5_VALID = 3;
 constexpr int FREQS = 12;
-constexpr int TILE = 64;                               // samples per workgroup tile
-constexpr int NT = 256;                                // threads per workgroup
 
 // fp32 parameter layout: [W0 b0 W1 b1 ... W5 b5], W_l row-major [n][k]
 constexpr int offw(int l) { int o = 0; for (int i = 0; i < l; ++i) o += ND[i] * KD[i] + ND[i]; return o; }
@@ -59,14 +55,6 @@ constexpr int NPARAM = offw(L);
 // bf16 weight layout: [W0 W1 ... W5]
 constexpr int offwb(int l) { int o = 0; for (int i = 0; i < l; ++i) o += ND[i] * KD[i]; return o; }
 constexpr int NWB = offwb(L);
-// training: 16 x 16 weight-gradient tiles a wave accumulates per layer and
-// their offsets: K / 16 + 1 for the hidden layers (the last tile's first
-// column is the bias gradient: the activations carry a row of ones there),
-// 6 for the input layer (the encoding's padding column 78 is the ones row),
-// and for the 16-row output layer one weight tile plus its bias tile
-constexpr int GW_OFF[L] = {0, 6, 11, 16, 21, 26};
-constexpr int GW_TILES = 28;
-constexpr int BIAS_COL0 = 78;  // input layer: padding column that carries the ones row
 static_assert(NPARAM == 23888, "parameter count");
 static_assert(NWB == 23552, "bf16 weight count");
 
@@ -75,10 +63,36 @@ struct Geo {
     float lo[3], inv[3];
 };
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
 __device__ __forceinline__ __bf16 tobf(float x) { return (__bf16)x; }
-// max(x, 0) as one integer max on the bits: a float with the sign bit set is
-// a negative integer (-0 and negative NaNs give +0)
-__device__ __forceinline__ float relu(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
+// a and b rounded to BF16 (to nearest even) in one dword, a in the low half
+// (one v_cvt_pk_bf16_f32)
+__device__ __forceinline__ uint32_t pack2(float a, float b) {
+    const f32x2 v = {a, b};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
+}
+// max(x, 0) of both BF16 halves as one packed 16-bit integer max: a BF16
+// with the sign bit set is a negative integer (-0 gives +0)
+__device__ __forceinline__ uint32_t relu2(uint32_t x) {
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s16x2, x), s16x2{0, 0}));
+}
+// per BF16 half: g where the ReLU output h (never negative) is non-zero,
+// else +0; g times min(h, 1) as packed 16-bit integers
+__device__ __forceinline__ uint32_t mask2(uint32_t g, uint32_t h) {
+    const u16x2 m = __builtin_elementwise_min(__builtin_bit_cast(u16x2, h), u16x2{1, 1});
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, g) * m);
+}
+// ReLU of the 4 consecutive outputs an accumulator holds, as BF16 pairs:
+// slots d, d + 1 of a column-operand fragment
+__device__ __forceinline__ void relu_tile(const f32x4& a, u32x4& x, int d) {
+    x[d] = relu2(pack2(a[0], a[1]));
+    x[d + 1] = relu2(pack2(a[2], a[3]));
+}
 
 // Features of k block ks that lane group kg supplies for one sample (the
 // input layer's weight columns are stored in this order).  kg < 3: sin and
@@ -89,49 +103,45 @@ __device__ __forceinline__ float relu(float x) { return __int_as_float(max(__flo
 // of the floor are exact), then three angle doublings; the error stays far
 // under the BF16 step the features are rounded to (the inference and
 // gradient tests compare against the oracle's exact encoding).  kg = 3: the
-// normal and the direction (nd, read only when ok) in block 0, zero padding
+// normal and the direction (nd) in block 0, zero padding
 // after.
-__device__ __forceinline__ bf16x8 feature_block(int kg, int ks, float x, const float* nd, bool ok) {
-    bf16x8 v;
+__device__ __forceinline__ bf16x8 feature_block(int kg, int ks, float x, const float (&nd)[6]) {
+    u32x4 v;
     if (kg < 3) {
         const float y = ldexpf(x, 4 * ks - 1);
         const float f = y - floorf(y);  // revolutions
         float sn = __builtin_amdgcn_sinf(f), cs = __builtin_amdgcn_cosf(f);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            v[2 * j] = tobf(sn);
-            v[2 * j + 1] = tobf(cs);
+            v[j] = pack2(sn, cs);
             const float s2 = 2.0f * sn * cs, c2 = (cs - sn) * (cs + sn);
             sn = s2;
             cs = c2;
         }
     } else {
 #pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = tobf(ks == 0 && j < 6 && ok ? nd[j] : 0.0f);
+        for (int j = 0; j < 4; ++j) v[j] = ks == 0 && j < 3 ? pack2(nd[2 * j], nd[2 * j + 1]) : 0u;
     }
-    return v;
+    return __builtin_bit_cast(bf16x8, v);
 }
 
-// Training buffers in LDS: activations and gradients transposed,
-// [feature][sample] with rows of TS, and gradients also [sample][feature]
-// with rows of HS; rows padded by 8 BF16 (16 bytes) so that the 16 rows of an
-// operand fragment start in different LDS banks.
-constexpr int HS = 64 + 8, TS = 64 + 8;
-
-// 16x16 tile of C = A * B over K (multiple of 32), both operands with k
-// contiguous: A(m, k) = A[m * sAm + k], B(k, n) = B[n * sBn + k].  Lane l
-// supplies A row m0 + l%16 and B column n0 + l%16, k = 8*(l/16) + j (one
-// 16-byte load each); on return acc[i] = C(m0 + 4*(l/16) + i, n0 + l%16).
-__device__ __forceinline__ f32x4 tile_mm(const __bf16* A, int sAm, const __bf16* B, int sBn, int m0, int n0, int K,
-                                         int lane) {
-    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    const int mi = lane & 15, kg = lane >> 4;
-    for (int k0 = 0; k0 < K; k0 += 32) {
-        const bf16x8 a = *(const bf16x8*)(A + (m0 + mi) * sAm + k0 + 8 * kg);
-        const bf16x8 b = *(const bf16x8*)(B + (n0 + mi) * sBn + k0 + 8 * kg);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+// The inputs lane group kg needs of sample sm (zeros past the end; n >= 1):
+// the coordinate on axis kg normalised to the scene box (kg < 3), the normal
+// and the direction (used by kg = 3).  Every lane issues the same loads.
+__device__ __forceinline__ void load_features(const float* __restrict__ feats, uint32_t sm, uint32_t n, int kg,
+                                              const Geo& G, float& xa, float (&nd)[6]) {
+    const bool v = sm < n;
+    const float* f = feats + (size_t)(v ? sm : 0u) * 9;
+    const int a = kg < 3 ? kg : 0;
+    const float lo = a == 0 ? G.lo[0] : (a == 1 ? G.lo[1] : G.lo[2]);
+    const float inv = a == 0 ? G.inv[0] : (a == 1 ? G.inv[1] : G.inv[2]);
+    const float x = f[a];
+    xa = (v && kg < 3) ? (x - lo) * inv : 0.0f;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        const float e = f[3 + j];
+        nd[j] = v ? e : 0.0f;
     }
-    return acc;
 }
 
 // ---- inference ------------------------------------------------------------
@@ -155,32 +165,56 @@ constexpr int INF_NT = 256;                       // threads per workgroup
 constexpr int INF_MT = 2;                         // 16-sample groups a wave carries through the layers
 constexpr int INF_TILE = INF_NT / 64 * 16 * INF_MT;  // samples per workgroup tile
 
-// blocked index of slot p (0..K-1) of row n in a matrix of N rows: K/32
-// blocks of [N][32], so a lane's 8 slots of one row are one 16-byte load and
-// the 16 rows of a fragment are 1 KB contiguous
-__device__ __forceinline__ int blk(int n, int p, int N) { return (p >> 5) * N * 32 + n * 32 + (p & 31); }
+// Every layer's BF16 weights are kept in the order the kernels read them, so
+// that a workgroup fills LDS with one straight copy:
+//  - forward, [W0 .. W5]: per layer, operand fragments of 16 rows x 32 slots
+//    (blk), slots permuted so that a lane's 8 slots of one row are the inputs
+//    it holds (enc_slot for the input layer, wslot for the others);
+//  - transposed, [W1^T .. W5^T] (training only): per layer, fragments of 16
+//    rows (inputs) x 32 slots (outputs), slot wslot(o) of row k holding W[o][k],
+//    the row operand of the activation gradients W^T dZ; the output layer's
+//    16 outputs fill half of one block, the other slots are zero.
+constexpr int offwt(int l) { return NWB + (l - 1) * 64 * 64; }  // l = 1..5
+constexpr int NWALL = NWB + 4 * 64 * 64 + 64 * 32;
 
-// weights, row-major [n][k] per layer in global memory -> blocked and
-// permuted in LDS (see above)
-__device__ __forceinline__ void load_weights_blocked(const __bf16* __restrict__ Wb, __bf16* sW) {
-    // input layer: slots (ks, 8 kg .. 8 kg + 7) <- features 24 kg + 8 ks ..
-    for (int c = threadIdx.x; c < ND[0] * 12; c += INF_NT) {
-        const int n = c / 12, ks = (c % 12) >> 2, kg = c & 3;
-        *(uint4*)(sW + offwb(0) + blk(n, 32 * ks + 8 * kg, ND[0])) =
-            *(const uint4*)(Wb + offwb(0) + n * KD[0] + 24 * kg + 8 * ks);
-    }
-    // hidden and output layers: slots 8 kg + j <- inputs 4 kg + j, 16 + 4 kg + j - 4
+// index of slot p (0..K-1) of row n in a matrix of N rows stored as 16 x 32
+// operand fragments (k block major, then row tile), each fragment in lane
+// order: lane (mi, kg) of a v_mfma_f32_16x16x32_bf16 operand, which supplies
+// row mi and slots 8 kg .. 8 kg + 7, finds them as 16 bytes at lane index
+// 16 kg + mi, so a wave reads a fragment as one contiguous kilobyte (no LDS
+// bank conflicts)
+__device__ __forceinline__ int blk(int n, int p, int N) {
+    return (((p >> 5) * (N >> 4) + (n >> 4)) * 64 + ((p >> 3) & 3) * 16 + (n & 15)) * 8 + (p & 7);
+}
+// slot of encoding feature k (0..95) in the input layer's rows: slot
+// (ks, 8 kg + j) holds feature 24 kg + 8 ks + j
+__device__ __forceinline__ int enc_slot(int k) {
+    const int kg = k / 24, r = k % 24;
+    return 32 * (r >> 3) + 8 * kg + (r & 7);
+}
+// slot of input k in the rows of a layer l >= 1: slot 8 kg + j of block ks
+// holds input 32 ks + 4 kg + j for j < 4 and 32 ks + 16 + 4 kg + j - 4
+// otherwise
+__device__ __forceinline__ int wslot(int k) {
+    const int q = k & 31;
+    return (k & ~31) + 8 * ((q & 15) >> 2) + 4 * (q >> 4) + (q & 3);
+}
+
+// global -> LDS copy of the first COUNT (multiple of 8) BF16 weights: every
+// thread's loads are issued before its first LDS write
+template <int NTH, int COUNT>
+__device__ __forceinline__ void copy_weights(const __bf16* __restrict__ Wb, __bf16* sW) {
+    constexpr int V = COUNT / 8, IT = (V + NTH - 1) / NTH;
+    u32x4 r[IT];
 #pragma unroll
-    for (int l = 1; l < L; ++l) {
-        const int K = KD[l], N = ND[l];
-        const int chunks = N * (K / 8);
-        for (int c = threadIdx.x; c < chunks; c += INF_NT) {
-            const int n = c / (K / 8), ks = (c % (K / 8)) >> 2, kg = c & 3;
-            const __bf16* src = Wb + offwb(l) + n * K + 32 * ks + 4 * kg;
-            uint2* dst = (uint2*)(sW + offwb(l) + blk(n, 32 * ks + 8 * kg, N));
-            dst[0] = *(const uint2*)src;
-            dst[1] = *(const uint2*)(src + 16);
-        }
+    for (int i = 0; i < IT; ++i) {
+        const int c = min((int)threadIdx.x + i * NTH, V - 1);
+        r[i] = ((const u32x4*)Wb)[c];
+    }
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int c = threadIdx.x + i * NTH;
+        if (c < V) ((u32x4*)sW)[c] = r[i];
     }
 }
 
@@ -201,14 +235,26 @@ __device__ __forceinline__ void layer_block(const __bf16* sWl, int ks, const bf1
                                                                   0, 0);
 }
 
+// The biases in LDS, layer l's at sB[64 l ..] (the output layer's 16 last):
+// a lane reads them with one base address and constant offsets.
+constexpr int NBIAS = (L - 1) * 64 + 16;
+template <int NTH>
+__device__ __forceinline__ void copy_bias(const float* __restrict__ Wf, float* sB) {
+#pragma unroll
+    for (int e = threadIdx.x; e < NBIAS; e += NTH) {
+        int src = 0;
+#pragma unroll
+        for (int l = 0; l < L; ++l)
+            if ((e >> 6) == l) src = offb(l) + (e & 63);
+        sB[e] = Wf[src];
+    }
+}
+
 // bias of outputs 16 nt + 4 kg + i (i < 4) of layer l
 template <int NT_>
-__device__ __forceinline__ void load_bias(const float* __restrict__ Wf, int l, int kg, f32x4 (&b)[NT_]) {
+__device__ __forceinline__ void load_bias(const float* sB, int l, int kg, f32x4 (&b)[NT_]) {
 #pragma unroll
-    for (int nt = 0; nt < NT_; ++nt) {
-        const float4 v = *(const float4*)(Wf + offb(l) + 16 * nt + 4 * kg);
-        b[nt] = f32x4{v.x, v.y, v.z, v.w};
-    }
+    for (int nt = 0; nt < NT_; ++nt) b[nt] = *(const f32x4*)(sB + 64 * l + 16 * nt + 4 * kg);
 }
 
 // SCATTER: out[3 * idx[s] + c] += qw[3 * s + c] * max(y_c, 0) (atomically) in
@@ -223,7 +269,9 @@ __global__ void __launch_bounds__(INF_NT) __attribute__((amdgpu_waves_per_eu(3))
     const uint32_t n = n_dev ? min(*n_dev, n_max) : n_max;
     if (n == 0) return;
     __shared__ __attribute__((aligned(16))) __bf16 sW[NWB];
-    load_weights_blocked(Wb, sW);
+    __shared__ __attribute__((aligned(16))) float sB[NBIAS];
+    copy_weights<INF_NT, NWB>(Wb, sW);
+    copy_bias<INF_NT>(Wf, sB);
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int mi = lane & 15, kg = lane >> 4;
@@ -235,22 +283,16 @@ __global__ void __launch_bounds__(INF_NT) __attribute__((amdgpu_waves_per_eu(3))
         const uint32_t s0 = t * INF_TILE + 16 * INF_MT * w;  // this wave's first sample
         f32x4 acc[INF_MT][4];
         f32x4 bias[4];
-        load_bias<4>(Wf, 0, kg, bias);
+        load_bias<4>(sB, 0, kg, bias);
         // input layer, one k block per encoding anchor
-        float xa[INF_MT];  // group kg < 3: the normalised axis coordinate of each sample
+        float xa[INF_MT], nd[INF_MT][6];
 #pragma unroll
-        for (int mt = 0; mt < INF_MT; ++mt) {
-            const uint32_t sm = s0 + 16 * mt + mi;
-            xa[mt] = (kg < 3 && sm < n) ? (feats[(size_t)sm * 9 + kg] - G.lo[kg]) * G.inv[kg] : 0.0f;
-        }
+        for (int mt = 0; mt < INF_MT; ++mt) load_features(feats, s0 + 16 * mt + mi, n, kg, G, xa[mt], nd[mt]);
 #pragma unroll
         for (int ks = 0; ks < 3; ++ks) {
             bf16x8 x[INF_MT];
 #pragma unroll
-            for (int mt = 0; mt < INF_MT; ++mt) {
-                const uint32_t sm = s0 + 16 * mt + mi;
-                x[mt] = feature_block(kg, ks, xa[mt], feats + (size_t)sm * 9 + 3, sm < n);
-            }
+            for (int mt = 0; mt < INF_MT; ++mt) x[mt] = feature_block(kg, ks, xa[mt], nd[mt]);
             if (ks == 0) layer_block<64, 4, true>(sW + offwb(0), ks, x, lane, bias, acc);
             else layer_block<64, 4, false>(sW + offwb(0), ks, x, lane, bias, acc);
         }
@@ -258,22 +300,23 @@ __global__ void __launch_bounds__(INF_NT) __attribute__((amdgpu_waves_per_eu(3))
         f32x4 o5[INF_MT][1];
 #pragma unroll
         for (int l = 1; l < L; ++l) {
-            bf16x8 x[2][INF_MT];
+            u32x4 xd[2][INF_MT];
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt)
 #pragma unroll
-                for (int mt = 0; mt < INF_MT; ++mt) {
-                    const int ks = nt >> 1, h = 4 * (nt & 1);
+                for (int mt = 0; mt < INF_MT; ++mt) relu_tile(acc[mt][nt], xd[nt >> 1][mt], 2 * (nt & 1));
+            bf16x8 x[2][INF_MT];
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) x[ks][mt][h + i] = tobf(relu(acc[mt][nt][i]));
-                }
+            for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+                for (int mt = 0; mt < INF_MT; ++mt) x[ks][mt] = __builtin_bit_cast(bf16x8, xd[ks][mt]);
             if (l < L - 1) {
-                load_bias<4>(Wf, l, kg, bias);
+                load_bias<4>(sB, l, kg, bias);
                 layer_block<64, 4, true>(sW + offwb(l), 0, x[0], lane, bias, acc);
                 layer_block<64, 4, false>(sW + offwb(l), 1, x[1], lane, bias, acc);
             } else {
                 f32x4 b5[1];
-                load_bias<1>(Wf, l, kg, b5);
+                load_bias<1>(sB, l, kg, b5);
                 layer_block<16, 1, true>(sW + offwb(l), 0, x[0], lane, b5, o5);
                 layer_block<16, 1, false>(sW + offwb(l), 1, x[1], lane, b5, o5);
             }
@@ -296,219 +339,307 @@ __global__ void __launch_bounds__(INF_NT) __attribute__((amdgpu_waves_per_eu(3))
     }
 }
 
-// Slot of input k in layer l's blocked, permuted LDS rows (the inference
-// layout: see load_weights_blocked), for layers l >= 1.
-__device__ __forceinline__ int wslot(int k) {
-    const int q = k & 31;
-    return (k & ~31) + 8 * ((q & 15) >> 2) + 4 * (q >> 4) + (q & 3);
+// ---- training ---------------------------------------------------------------
+//
+// A workgroup of NW waves (8, or 4 for small batches) takes tiles of 16 NW
+// samples, 16 per wave, at most one workgroup per CU striding over the tiles.
+// Per tile:
+//  - forward as in inference (one 16-sample group per wave); the five ReLU
+//    outputs stay in registers as column operands, the encoding is also
+//    written to its own staging buffer, [feature][sample];
+//  - loss, and the output gradient dZ5 in the same column-operand layout;
+//  - backward, layer l = 5 .. 0: the wave writes dZ_l and the layer's input
+//    H_{l-1} (l > 0) into the staging buffers transposed, [feature][sample]; computes dZ_{l-1} = (W_l^T dZ_l) masked by
+//    H_{l-1} > 0 in registers (the transposed weights are the row operand and
+//    dZ_l the column operand, so the product lands in the column-operand
+//    layout again, slots permuted as in the forward); then, after a barrier,
+//    its share of dW_l = sum over the tile's samples of dZ_l H_{l-1}^T from
+//    the staging buffers (one 16-row block of outputs, a group of 16-column
+//    input tiles).  A bias gradient is the same product against a column
+//    operand that is one in column 0 and zero elsewhere; for the input layer
+//    it is the encoding's padding feature ENC_ONE, staged as 1 (the forward
+//    sees 0 there and its weights stay 0).
+// Each thread sums its share of the gradient over the workgroup's tiles, in
+// tile order, in registers and writes it once (no atomics, no
+// read-modify-write of global memory): the result does not depend on
+// scheduling.
+constexpr int ENC_ONE = 78;
+
+template <int NW>
+struct TrainShape {
+    static constexpr int NTH = 64 * NW;           // threads per workgroup
+    static constexpr int TILE = 16 * NW;          // samples per tile
+    static constexpr int SR = TILE + 8;           // staging row pitch (BF16): consecutive rows 16 bytes apart in bank order
+    static constexpr int G = NW / 4;              // waves sharing a 16-row block of outputs
+    static constexpr int BH = 4 / G;              // hidden layers: input tiles per wave
+    static constexpr int B0 = 6 / G;              // input layer: input tiles per wave
+    static constexpr int B5 = (5 + NW - 1) / NW;  // output layer: tiles per wave (4 weight tiles, 1 bias tile)
+};
+
+// writes the values of column-operand fragment v (k block ks) of the lane's
+// sample column into S[feature][col], for features below nrows
+__device__ __forceinline__ void stage_act(__bf16* S, int sr, const bf16x8& v, int ks, int kg, int col, int nrows) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int f = 32 * ks + (j < 4 ? 4 * kg + j : 16 + 4 * kg + j - 4);
+        if (f < nrows) S[f * sr + col] = v[j];
+    }
 }
 
-// One tile of 64 samples, 4 waves (wave w: samples 16 w .. 16 w + 15).
-// Forward as in inference, activations in registers, biases as the first
-// addend; every activation (and the encoding) is also written to LDS
-// transposed, [feature][sample], for the weight gradients, whose products
-// sum over samples.  The output-layer and hidden-layer gradients are kept in
-// both orientations: [sample][feature] for the activation gradients (their
-// row operand), [feature][sample] for the weight gradients.
-__global__ void __launch_bounds__(NT) k_mlp_train(const float* __restrict__ feats, const float* __restrict__ targets,
-                                                  uint32_t n, const __bf16* __restrict__ Wb,
-                                                  const float* __restrict__ Wf, Geo G, float* __restrict__ partial,
-                                                  float* __restrict__ loss_part) {
-    __shared__ __attribute__((aligned(16))) __bf16 sW[NWB];             // inference layout
-    __shared__ __attribute__((aligned(16))) __bf16 sXT[96 * TS];        // encoding, [feature][sample]
-    __shared__ __attribute__((aligned(16))) __bf16 sHT[L - 1][80 * TS];  // ReLU activations, [feature][sample], then a row of ones
-    __shared__ __attribute__((aligned(16))) __bf16 sD[2][TILE * HS];     // gradients, [sample][feature]
-    __shared__ __attribute__((aligned(16))) __bf16 sDT[2][64 * TS];      // gradients, [feature][sample]
-    __shared__ float sLoss[NT / 64];
+// 16 x 32 operand fragment of rows r0 .. r0 + 15 and samples 32 kk .. 32 kk + 31
+// of a staging buffer
+__device__ __forceinline__ bf16x8 stage_frag(const __bf16* S, int sr, int r0, int kk, int mi, int kg) {
+    return *(const bf16x8*)(S + (r0 + mi) * sr + 32 * kk + 8 * kg);
+}
+
+template <int NW>
+__global__ void __launch_bounds__(64 * NW) k_mlp_train(const float* __restrict__ feats, const float* __restrict__ targets,
+                                                       uint32_t n, const __bf16* __restrict__ Wb,
+                                                       const float* __restrict__ Wf, Geo G, float* __restrict__ partial,
+                                                       float* __restrict__ loss_part) {
+    using S = TrainShape<NW>;
+    constexpr int SR = S::SR;
+    constexpr int KS = S::TILE / 32;  // k steps of a weight-gradient product
+    __shared__ __attribute__((aligned(16))) __bf16 sW[NWALL];    // forward and transposed weights
+    __shared__ __attribute__((aligned(16))) __bf16 sZ[64 * SR];  // dZ_l, [output][sample]
+    __shared__ __attribute__((aligned(16))) __bf16 sH[64 * SR];  // H_{l-1}, [input][sample]
+    __shared__ __attribute__((aligned(16))) __bf16 sX[96 * SR];  // the encoding, [feature][sample]
+    __shared__ __attribute__((aligned(16))) float sB[NBIAS];
+    __shared__ float sLoss[NW];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int mi = lane & 15, kg = lane >> 4;
-    float* part = partial + (size_t)blockIdx.x * NPARAM;
-    load_weights_blocked(Wb, sW);
-    float lacc = 0.0f;
-    // this thread's share of the workgroup's gradient, summed over its tiles
-    // in tile order in registers and written once at the end: the weight
-    // tiles of every layer (wave w owns rows 16 w .. 16 w + 15 of each dW,
-    // GW_OFF[l] + kt indexes layer l's column tile kt) and, for threads < 64,
-    // one bias element per layer
-    f32x4 gw[GW_TILES];
+    const int rb = w % 4, cg = w / 4;  // this wave's 16-row block of outputs and its group of input tiles
+    const int col = 16 * w + mi;       // this lane's sample column in a tile
+    const uint32_t tiles = (n + S::TILE - 1) / S::TILE;
+    // a tile's inputs (coordinate, normal and direction, targets), loaded one
+    // tile ahead: the first tile's before the weight copy, each later one's
+    // at the start of the tile before it, so that the loads overlap work
+    float nx_xa, nx_nd[6], nx_tg[N5_VALID];
+    auto load_in = [&](uint32_t t) {
+        const uint32_t sm = t * S::TILE + col;
+        load_features(feats, sm, n, kg, G, nx_xa, nx_nd);
+        const size_t r = sm < n ? sm : 0u;
 #pragma unroll
-    for (int i = 0; i < GW_TILES; ++i) gw[i] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    // activation rows 64..79: ones, then zeros, so that each layer's last
-    // weight-gradient tile sums the output gradients over samples (the bias
-    // gradient) in its first column
-    for (int e = threadIdx.x; e < (L - 1) * 16 * TS; e += NT) {
-        const int l = e / (16 * TS), r = (e / TS) % 16;
-        sHT[l][(64 + r) * TS + e % TS] = tobf(r == 0 ? 1.0f : 0.0f);
-    }
-    const uint32_t tiles = (n + TILE - 1) / TILE;
+        for (int c = 0; c < N5_VALID; ++c) {
+            const float e = targets[r * 3 + c];
+            nx_tg[c] = sm < n ? e : 0.0f;
+        }
+    };
+    load_in(blockIdx.x);
+    copy_weights<S::NTH, NWALL>(Wb, sW);
+    copy_bias<S::NTH>(Wf, sB);
+    bf16x8 ones;  // column operand of the bias products
+#pragma unroll
+    for (int j = 0; j < 8; ++j) ones[j] = tobf(mi == 0 ? 1.0f : 0.0f);
+    // this thread's share of the workgroup's gradient: input layer tiles,
+    // hidden layer tiles (the last one the bias tile, used when cg == 0),
+    // output layer tiles
+    f32x4 g0[S::B0], gh[L - 2][S::BH + 1], g5[S::B5];
+#pragma unroll
+    for (int j = 0; j < S::B0; ++j) g0[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int l = 0; l < L - 2; ++l)
+#pragma unroll
+        for (int j = 0; j <= S::BH; ++j) gh[l][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int j = 0; j < S::B5; ++j) g5[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    float lacc = 0.0f;
+    const float two_over_n = 2.0f / (float)n;
+    __syncthreads();  // weights in LDS
     for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
         // keeps the weight fragments' LDS loads inside the loop
         __asm__ __volatile__("" ::: "memory");
-        const uint32_t s0 = tile * TILE;
-        const int sw = 16 * w + mi;             // this lane's sample in the tile
-        const bool valid = s0 + sw < n;
-        __syncthreads();  // the previous tile's backward pass is done with the buffers
+        const bool valid = tile * S::TILE + col < n;
+        const float xa = nx_xa;
+        float nd[6], tg[N5_VALID];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) nd[j] = nx_nd[j];
+#pragma unroll
+        for (int c = 0; c < N5_VALID; ++c) tg[c] = nx_tg[c];
+        if (tile + gridDim.x < tiles) load_in(tile + gridDim.x);
         // ---- forward
         f32x4 acc[1][4];
         f32x4 bias[4];
-        load_bias<4>(Wf, 0, kg, bias);
-        const float xa = (kg < 3 && valid) ? (feats[(size_t)(s0 + sw) * 9 + kg] - G.lo[kg]) * G.inv[kg] : 0.0f;
+        load_bias<4>(sB, 0, kg, bias);
 #pragma unroll
         for (int ks = 0; ks < 3; ++ks) {
-            bf16x8 x[1];
-            x[0] = feature_block(kg, ks, xa, feats + (size_t)(s0 + sw) * 9 + 3, valid);
-            // slot (ks, 8 kg + j) holds feature 24 kg + 8 ks + j; the padding
-            // feature BIAS_COL0 is stored as 1 for the weight gradients
-            // (the forward product sees it as 0)
+            const bf16x8 xk[1] = {feature_block(kg, ks, xa, nd)};
 #pragma unroll
-            for (int j = 0; j < 8; ++j)
-                sXT[(24 * kg + 8 * ks + j) * TS + sw] = (24 * kg + 8 * ks + j == BIAS_COL0) ? tobf(1.0f) : x[0][j];
-            if (ks == 0) layer_block<64, 4, true, 1>(sW + offwb(0), ks, x, lane, bias, acc);
-            else layer_block<64, 4, false, 1>(sW + offwb(0), ks, x, lane, bias, acc);
+            for (int j = 0; j < 8; ++j) {
+                const int f = 24 * kg + 8 * ks + j;
+                sX[f * SR + col] = f == ENC_ONE ? tobf(1.0f) : xk[0][j];
+            }
+            if (ks == 0) layer_block<64, 4, true, 1>(sW + offwb(0), ks, xk, lane, bias, acc);
+            else layer_block<64, 4, false, 1>(sW + offwb(0), ks, xk, lane, bias, acc);
         }
+        u32x4 h[L - 1][2];  // ReLU outputs of layers 0 .. 4, k blocks 0 and 1 (BF16 pairs)
         f32x4 o5[1][1];
 #pragma unroll
         for (int l = 1; l < L; ++l) {
-            bf16x8 x[2][1];
 #pragma unroll
-            for (int nt = 0; nt < 4; ++nt) {
-                const int ks = nt >> 1, h = 4 * (nt & 1);
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const __bf16 v = tobf(relu(acc[0][nt][i]));
-                    x[ks][0][h + i] = v;
-                    sHT[l - 1][(16 * nt + 4 * kg + i) * TS + sw] = v;
-                }
-            }
+            for (int nt = 0; nt < 4; ++nt) relu_tile(acc[0][nt], h[l - 1][nt >> 1], 2 * (nt & 1));
+            const bf16x8 xk0[1] = {__builtin_bit_cast(bf16x8, h[l - 1][0])};
+            const bf16x8 xk1[1] = {__builtin_bit_cast(bf16x8, h[l - 1][1])};
             if (l < L - 1) {
-                load_bias<4>(Wf, l, kg, bias);
-                layer_block<64, 4, true, 1>(sW + offwb(l), 0, x[0], lane, bias, acc);
-                layer_block<64, 4, false, 1>(sW + offwb(l), 1, x[1], lane, bias, acc);
+                load_bias<4>(sB, l, kg, bias);
+                layer_block<64, 4, true, 1>(sW + offwb(l), 0, xk0, lane, bias, acc);
+                layer_block<64, 4, false, 1>(sW + offwb(l), 1, xk1, lane, bias, acc);
             } else {
                 f32x4 b5[1];
-                load_bias<1>(Wf, l, kg, b5);
-                layer_block<16, 1, true, 1>(sW + offwb(l), 0, x[0], lane, b5, o5);
-                layer_block<16, 1, false, 1>(sW + offwb(l), 1, x[1], lane, b5, o5);
+                load_bias<1>(sB, l, kg, b5);
+                layer_block<16, 1, true, 1>(sW + offwb(l), 0, xk0, lane, b5, o5);
+                layer_block<16, 1, false, 1>(sW + offwb(l), 1, xk1, lane, b5, o5);
             }
         }
         // ---- loss: lane (mi, kg) holds outputs 4 kg + i of its sample;
-        // dL/dy = 2 (y - t) / (y^2 + 0.01) / n on the 3 valid outputs
-        float lsum = 0.0f;
-        {
-            bf16x4 dyt;
+        // with q = (y - t) / (y^2 + 0.01), the loss term is (y - t) q and
+        // dL/dy = q 2 / n on the 3 valid outputs, which are slots 0 .. 2 of
+        // the output layer's gradient fragment
+        float lsum = 0.0f, dy[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int c = 4 * kg + i;
-                float dy = 0.0f;
-                if (c < N5_VALID && valid) {
-                    const float y = o5[0][0][i];
-                    const float t = targets[(size_t)(s0 + sw) * 3 + c];
-                    const float den = y * y + 0.01f;
-                    const float r = y - t;
-                    lsum += r * r / den;
-                    dy = 2.0f * r / den / (float)n;
-                }
-                dyt[i] = tobf(dy);
-                sDT[0][c * TS + sw] = dyt[i];
+        for (int i = 0; i < 4; ++i) {
+            const int c = 4 * kg + i;
+            if (c < N5_VALID && valid) {
+                const float y = o5[0][0][i];
+                const float t = tg[i];
+                const float r = y - t;
+                const float q = r / (y * y + 0.01f);
+                lsum += r * q;
+                dy[i] = q * two_over_n;
             }
-            *(bf16x4*)(sD[0] + sw * HS + 4 * kg) = dyt;
         }
+        u32x4 dz[2] = {u32x4{pack2(dy[0], dy[1]), pack2(dy[2], dy[3]), 0u, 0u}, u32x4{0u, 0u, 0u, 0u}};
         for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
         if (lane == 0) sLoss[w] = lsum;
-        __syncthreads();
-        if (threadIdx.x == 0) lacc += sLoss[0] + sLoss[1] + sLoss[2] + sLoss[3];
 
         // ---- backward
-        // layer 5: dW5[n][k] = sum_s dY[s][n] H4[s][k] (16 x 64), one 16-column tile per wave
-        {
-            const f32x4 g = tile_mm(sDT[0], TS, sHT[L - 2], TS, 0, 16 * w, 64, lane);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) gw[GW_OFF[L - 1]][i] += g[i];
-            if (w == 0) {  // columns 64..79: the ones row, so column 64 is the bias gradient
-                const f32x4 gbias = tile_mm(sDT[0], TS, sHT[L - 2], TS, 0, 64, 64, lane);
-#pragma unroll
-                for (int i = 0; i < 4; ++i) gw[GW_OFF[L - 1] + 1][i] += gbias[i];
-            }
-        }
-        // dH_{l-1}[s][k] = sum_n dZ[s][n] W_l[n][k] (n < nvalid), masked by H_{l-1} > 0, into both layouts
-        auto act_grad = [&](int l, int nvalid, const __bf16* dZ, __bf16* dst, __bf16* dstT) {
-            const int N = ND[l];
-            const __bf16* Wl = sW + offwb(l);
-#pragma unroll
-            for (int nt = 0; nt < 4; ++nt) {
-                const int col = 16 * nt + mi;
-                const int slot = wslot(col);
-                const __bf16* wcol = Wl + (slot >> 5) * N * 32 + (slot & 31);  // W_l[n][col] at wcol[n * 32]
-                f32x4 g = {0.0f, 0.0f, 0.0f, 0.0f};
-                for (int k0 = 0; k0 < ((nvalid + 31) & ~31); k0 += 32) {
-                    const int kb = k0 + 8 * kg;
-                    bf16x8 av, bv;
-                    if (kb < nvalid) av = *(const bf16x8*)(dZ + (16 * w + mi) * HS + kb);
-                    else av = bf16x8{};
-#pragma unroll
-                    for (int j = 0; j < 8; ++j) bv[j] = kb + j < nvalid ? wcol[(kb + j) * 32] : tobf(0.0f);
-                    g = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, g, 0, 0, 0);
-                }
-                // g[i] = dH(sample 16 w + 4 kg + i, feature col)
-                const bf16x4 h = *(const bf16x4*)(sHT[l - 1] + col * TS + 16 * w + 4 * kg);
-                bf16x4 gt;
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    gt[i] = tobf((float)h[i] > 0.0f ? g[i] : 0.0f);
-                    dst[(16 * w + 4 * kg + i) * HS + col] = gt[i];
-                }
-                *(bf16x4*)(dstT + col * TS + 16 * w + 4 * kg) = gt;
-            }
-        };
-        act_grad(L - 1, N5_VALID, sD[0], sD[1], sDT[1]);
-        __syncthreads();
-        int cur = 1;
-#pragma unroll
-        for (int l = L - 2; l >= 0; --l) {
-            const __bf16* inT = (l == 0) ? sXT : sHT[l - 1];
-            const int K = KD[l];
-            // dW_l[n][k] = sum_s dZ[s][n] in[s][k]: rows n in 16-row tiles (one per wave), K / 16
-            // column tiles, plus the ones-row tile (bias) for the hidden layers
-            const int KT = (l == 0) ? K / 16 : K / 16 + 1;
-#pragma unroll
-            for (int kt = 0; kt < KT; ++kt) {
-                const f32x4 g = tile_mm(sDT[cur], TS, inT, TS, 16 * w, 16 * kt, 64, lane);
-#pragma unroll
-                for (int i = 0; i < 4; ++i) gw[GW_OFF[l] + kt][i] += g[i];
-            }
+        for (int l = L - 1; l >= 0; --l) {
+            stage_act(sZ, SR, __builtin_bit_cast(bf16x8, dz[0]), 0, kg, col, ND[l]);
+            if (l < L - 1) stage_act(sZ, SR, __builtin_bit_cast(bf16x8, dz[1]), 1, kg, col, 64);
             if (l > 0) {
-                act_grad(l, 64, sD[cur], sD[cur ^ 1], sDT[cur ^ 1]);
-                __syncthreads();
-                cur ^= 1;
+                stage_act(sH, SR, __builtin_bit_cast(bf16x8, h[l - 1][0]), 0, kg, col, 64);
+                stage_act(sH, SR, __builtin_bit_cast(bf16x8, h[l - 1][1]), 1, kg, col, 64);
+            }
+            // dZ_{l-1} = (W_l^T dZ_l) masked by H_{l-1} > 0, in registers
+            u32x4 dzn[2];
+            if (l > 0) {
+                f32x4 g[4];
+#pragma unroll
+                for (int mt = 0; mt < 4; ++mt) g[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int ks = 0; ks < (l == L - 1 ? 1 : 2); ++ks)
+#pragma unroll
+                    for (int mt = 0; mt < 4; ++mt) {
+                        const bf16x8 a = *(const bf16x8*)(sW + offwt(l) + blk(16 * mt + mi, 32 * ks + 8 * kg, 64));
+                        g[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf16x8, dz[ks]), g[mt],
+                                                                        0, 0, 0);
+                    }
+#pragma unroll
+                for (int mt = 0; mt < 4; ++mt) {
+                    const int ks = mt >> 1, d = 2 * (mt & 1);
+                    dzn[ks][d] = mask2(pack2(g[mt][0], g[mt][1]), h[l - 1][ks][d]);
+                    dzn[ks][d + 1] = mask2(pack2(g[mt][2], g[mt][3]), h[l - 1][ks][d + 1]);
+                }
+            }
+            __syncthreads();  // staging written
+            if (l == L - 1 && threadIdx.x == 0)
+                for (int q = 0; q < NW; ++q) lacc += sLoss[q];
+            // this wave's share of dW_l (and the bias) over the tile's samples:
+            // every operand fragment first, then the products
+            if (l == L - 1) {
+                bf16x8 af[KS], bfr[KS][S::B5];
+#pragma unroll
+                for (int kk = 0; kk < KS; ++kk) {
+                    af[kk] = stage_frag(sZ, SR, 0, kk, mi, kg);
+#pragma unroll
+                    for (int j = 0; j < S::B5; ++j) {
+                        const int b = w + NW * j;
+                        bfr[kk][j] = b < 4 ? stage_frag(sH, SR, 16 * b, kk, mi, kg) : ones;
+                    }
+                }
+#pragma unroll
+                for (int kk = 0; kk < KS; ++kk)
+#pragma unroll
+                    for (int j = 0; j < S::B5; ++j)
+                        if (w + NW * j <= 4) g5[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[kk], bfr[kk][j], g5[j], 0, 0, 0);
+            } else if (l == 0) {
+                bf16x8 af[KS], bfr[KS][S::B0];
+#pragma unroll
+                for (int kk = 0; kk < KS; ++kk) {
+                    af[kk] = stage_frag(sZ, SR, 16 * rb, kk, mi, kg);
+#pragma unroll
+                    for (int j = 0; j < S::B0; ++j) bfr[kk][j] = stage_frag(sX, SR, 16 * (cg * S::B0 + j), kk, mi, kg);
+                }
+#pragma unroll
+                for (int kk = 0; kk < KS; ++kk)
+#pragma unroll
+                    for (int j = 0; j < S::B0; ++j) g0[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[kk], bfr[kk][j], g0[j], 0, 0, 0);
+            } else {
+                bf16x8 af[KS], bfr[KS][S::BH];
+#pragma unroll
+                for (int kk = 0; kk < KS; ++kk) {
+                    af[kk] = stage_frag(sZ, SR, 16 * rb, kk, mi, kg);
+#pragma unroll
+                    for (int j = 0; j < S::BH; ++j) bfr[kk][j] = stage_frag(sH, SR, 16 * (cg * S::BH + j), kk, mi, kg);
+                }
+#pragma unroll
+                for (int kk = 0; kk < KS; ++kk) {
+#pragma unroll
+                    for (int j = 0; j < S::BH; ++j)
+                        gh[l - 1][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[kk], bfr[kk][j], gh[l - 1][j], 0, 0, 0);
+                    if (cg == 0)
+                        gh[l - 1][S::BH] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[kk], ones, gh[l - 1][S::BH], 0, 0, 0);
+                }
+            }
+            __syncthreads();  // staging read
+            if (l > 0) {
+                dz[0] = dzn[0];
+                dz[1] = dzn[1];
             }
         }
     }  // tiles
-    // the workgroup's partial gradient, written once; the ones-row columns
-    // go to the biases
+    // the workgroup's partial gradient, written once; lane (mi, kg) of tile
+    // (row block r, column tile b) holds rows 16 r + 4 kg + i, column 16 b + mi.
+    // (The lane index passes through an empty asm so that the compiler does
+    // not compute these addresses before the loop and hold them across it.)
+    float* part = partial + (size_t)blockIdx.x * NPARAM;
+    auto put = [&](int q, float v) { part[q] = v; };
+    int lane_o = lane;
+    __asm__ __volatile__("" : "+v"(lane_o));
+    const int mo = lane_o & 15, ko = lane_o >> 4;
 #pragma unroll
-    for (int l = 0; l < L; ++l) {
-        const int K = KD[l];
-        const int nkt = (l == L - 1) ? 1 : K / 16;
+    for (int j = 0; j < S::B0; ++j) {
+        const int c = 16 * (cg * S::B0 + j) + mo;
 #pragma unroll
-        for (int kt = 0; kt < nkt; ++kt) {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                // layer 5 (16 x 64): wave w holds columns 16 w .. 16 w + 15 of all 16 rows
-                const int row = (l == L - 1) ? 4 * kg + i : 16 * w + 4 * kg + i;
-                const int col = (l == L - 1) ? 16 * w + mi : 16 * kt + mi;
-                const float g = gw[GW_OFF[l] + kt][i];
-                if (l == 0 && col == BIAS_COL0) {
-                    part[offb(0) + row] = g;
-                    part[offw(0) + row * K + col] = 0.0f;
-                } else {
-                    part[offw(l) + row * K + col] = g;
-                }
+        for (int i = 0; i < 4; ++i) {
+            const int r = 16 * rb + 4 * ko + i;
+            if (c == ENC_ONE) {
+                put(offb(0) + r, g0[j][i]);
+                put(offw(0) + r * KD[0] + c, 0.0f);
+            } else {
+                put(offw(0) + r * KD[0] + c, g0[j][i]);
             }
         }
-        if (l > 0 && mi == 0 && (l < L - 1 || w == 0)) {
-            const int bt = (l == L - 1) ? GW_OFF[l] + 1 : GW_OFF[l] + K / 16;
+    }
 #pragma unroll
-            for (int i = 0; i < 4; ++i) part[offb(l) + ((l == L - 1) ? 4 * kg + i : 16 * w + 4 * kg + i)] = gw[bt][i];
+    for (int l = 1; l < L - 1; ++l) {
+#pragma unroll
+        for (int j = 0; j < S::BH; ++j)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                put(offw(l) + (16 * rb + 4 * ko + i) * 64 + 16 * (cg * S::BH + j) + mo, gh[l - 1][j][i]);
+        if (cg == 0 && mo == 0)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) put(offb(l) + 16 * rb + 4 * ko + i, gh[l - 1][S::BH][i]);
+    }
+#pragma unroll
+    for (int j = 0; j < S::B5; ++j) {
+        const int b = w + NW * j;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (b < 4) put(offw(L - 1) + (4 * ko + i) * 64 + 16 * b + mo, g5[j][i]);
+            else if (b == 4 && mo == 0) put(offb(L - 1) + 4 * ko + i, g5[j][i]);
         }
     }
     if (threadIdx.x == 0) loss_part[blockIdx.x] = lacc;
@@ -517,17 +648,25 @@ __global__ void __launch_bounds__(NT) k_mlp_train(const float* __restrict__ feat
 // Adam with bias correction on the summed partial gradients (workgroup order,
 // so the sum is deterministic); refreshes the bf16 weight copy.  With lr = 0
 // it only writes the summed gradient (gradient queries).
-// parameter p's BF16 copy (weights only; the biases stay FP32)
+// parameter p's BF16 copies (weights only; the biases stay FP32): the
+// forward layout and, for layers 1 .. 5, the transposed one
 __device__ __forceinline__ void store_bf16(__bf16* Wb, int p, float w) {
+#pragma unroll
     for (int l = 0; l < L; ++l) {
         const int o = offw(l), nw = ND[l] * KD[l];
-        if (p >= o && p < o + nw) Wb[offwb(l) + (p - o)] = tobf(w);
+        if (p >= o && p < o + nw) {
+            const int r = (p - o) / KD[l], k = (p - o) % KD[l];
+            const __bf16 b = tobf(w);
+            Wb[offwb(l) + blk(r, l == 0 ? enc_slot(k) : wslot(k), ND[l])] = b;
+            if (l > 0) Wb[offwt(l) + blk(k, wslot(r), 64)] = b;
+        }
     }
 }
 
 // A block of ADAM_NT threads serves 64 parameters: wave q sums partials
 // q, q + QN, q + 2 QN, ... in order (each load one coalesced 256-byte row,
-// eight in flight), then wave 0 adds the QN sums in wave order.
+// all of them in flight together), then wave 0 adds the QN sums in wave
+// order.
 constexpr int ADAM_NT = 512;
 constexpr int QN = ADAM_NT / 64;
 __global__ void __launch_bounds__(ADAM_NT) k_adam(const float* __restrict__ partial, uint32_t nparts,
@@ -540,15 +679,17 @@ __global__ void __launch_bounds__(ADAM_NT) k_adam(const float* __restrict__ part
     const int p = blockIdx.x * 64 + lane;
     float g = 0.0f;
     if (p < NPARAM) {
-        uint32_t w = q;
-        for (; w + 7 * QN < nparts; w += 8 * QN) {
-            float x[8];
+        // up to 256 partials: every load of the wave in flight at once
+        constexpr int PW = 256 / QN;
+        float x[PW];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) x[j] = partial[(size_t)(w + QN * j) * NPARAM + p];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) g += x[j];
+        for (int j = 0; j < PW; ++j) {
+            const uint32_t w = q + QN * j;
+            x[j] = w < nparts ? partial[(size_t)w * NPARAM + p] : 0.0f;
         }
-        for (; w < nparts; w += QN) g += partial[(size_t)w * NPARAM + p];
+#pragma unroll
+        for (int j = 0; j < PW; ++j) g += x[j];
+        for (uint32_t w = q + QN * PW; w < nparts; w += QN) g += partial[(size_t)w * NPARAM + p];
     }
     s[q][lane] = g;
     __syncthreads();
@@ -636,17 +777,23 @@ int run_train(rt_nrc* s, const float* feats, const float* targets, uint32_t n, i
     if ((rc = stage(s, s->feats, feats, (size_t)n * 36, ptr_kind, &df)) ||
         (rc = stage(s, s->targets, targets, (size_t)n * 12, ptr_kind, &dt)))
         return rc;
-    const uint32_t tiles = (n + TILE - 1) / TILE;
-    // one workgroup per CU at most (the kernel holds 116 KB of LDS); each
-    // strides over tiles, so k_adam sums at most num_cus partial gradients
+    // one workgroup per CU at most (142 KB of LDS with 8 waves, 111 KB with
+    // 4); each strides over tiles, so k_adam sums at most num_cus partial
+    // gradients.  8-wave tiles when they fill every CU, else 4-wave ones.
+    const bool wide = (n + TrainShape<8>::TILE - 1) / TrainShape<8>::TILE >= (uint32_t)c->num_cus;
+    const uint32_t tile = wide ? TrainShape<8>::TILE : TrainShape<4>::TILE;
+    const uint32_t tiles = (n + tile - 1) / tile;
     const uint32_t wgs = tiles < (uint32_t)c->num_cus ? tiles : (uint32_t)c->num_cus;
     if ((rc = ensure(c, s->partial, (size_t)wgs * NPARAM * 4)) || (rc = ensure(c, s->loss, (size_t)wgs * 4))) return rc;
-    RT_HIP(c, hipEventRecord(c->ev[2], c->stream));
-    hipLaunchKernelGGL(k_mlp_train, dim3(wgs), dim3(NT), 0, c->stream, (const float*)df, (const float*)dt, n,
-                       (const __bf16*)s->wb.ptr, (const float*)s->wf.ptr, s->geo, (float*)s->partial.ptr,
-                       (float*)s->loss.ptr);
+    if (wide)
+        hipLaunchKernelGGL(k_mlp_train<8>, dim3(wgs), dim3(TrainShape<8>::NTH), 0, c->stream, (const float*)df,
+                           (const float*)dt, n, (const __bf16*)s->wb.ptr, (const float*)s->wf.ptr, s->geo,
+                           (float*)s->partial.ptr, (float*)s->loss.ptr);
+    else
+        hipLaunchKernelGGL(k_mlp_train<4>, dim3(wgs), dim3(TrainShape<4>::NTH), 0, c->stream, (const float*)df,
+                           (const float*)dt, n, (const __bf16*)s->wb.ptr, (const float*)s->wf.ptr, s->geo,
+                           (float*)s->partial.ptr, (float*)s->loss.ptr);
     RT_HIP(c, hipGetLastError());
-    RT_HIP(c, hipEventRecord(c->ev[3], c->stream));
     float bc1 = 1.0f, bc2 = 1.0f;
     if (update) {
         s->steps += 1;
@@ -658,7 +805,9 @@ int run_train(rt_nrc* s, const float* feats, const float* targets, uint32_t n, i
                        (__bf16*)s->wb.ptr, s->cfg.lr, s->cfg.beta1, s->cfg.beta2, s->cfg.eps, bc1, bc2,
                        update ? 1 : 0);
     RT_HIP(c, hipGetLastError());
-    c->last_op = 3;
+    // untimed: two event markers would cost a tenth of a short step
+    c->last_op = 0;
+    c->timing = rt_timing{};
     if (grad_out) RT_HIP(c, hipMemcpyAsync(grad_out, s->grad.ptr, NPARAM * 4, hipMemcpyDeviceToHost, c->stream));
     if (loss) {
         std::vector<float> lp(wgs);
@@ -769,7 +918,7 @@ int rt_nrc_create(rt_ctx* c, const rt_nrc_config* cfg, rt_nrc** out) {
     }
     (void)hipSetDevice(c->device);
     int rc;
-    if ((rc = ensure(c, s->wf, NPARAM * 4)) || (rc = ensure(c, s->wb, NWB * 2)) || (rc = ensure(c, s->m, NPARAM * 4)) ||
+    if ((rc = ensure(c, s->wf, NPARAM * 4)) || (rc = ensure(c, s->wb, NWALL * 2)) || (rc = ensure(c, s->m, NPARAM * 4)) ||
         (rc = ensure(c, s->v, NPARAM * 4)) || (rc = ensure(c, s->grad, NPARAM * 4))) {
         rt_nrc_destroy(s);
         return rc;
@@ -790,6 +939,8 @@ int rt_nrc_create(rt_ctx* c, const rt_nrc_config* cfg, rt_nrc** out) {
     hipError_t e = hipMemcpy(s->wf.ptr, w.data(), NPARAM * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(s->m.ptr, 0, NPARAM * 4);
     if (e == hipSuccess) e = hipMemset(s->v.ptr, 0, NPARAM * 4);
+    // the transposed output layer's unused slots stay zero
+    if (e == hipSuccess) e = hipMemset(s->wb.ptr, 0, NWALL * 2);
     if (e != hipSuccess) {
         rt_nrc_destroy(s);
         return hip_fail(c, e, "rt_nrc_create");
@@ -855,10 +1006,9 @@ int rt_nrc_infer(rt_nrc* s, const float* feats, uint32_t n, float* out, int ptr_
         if ((rc = ensure(c, s->out, (size_t)n * 12))) return rc;
         dout = (float*)s->out.ptr;
     }
-    RT_HIP(c, hipEventRecord(c->ev[2], c->stream));
     if ((rc = nrc_infer_device(s, (const float*)df, n, dout))) return rc;
-    RT_HIP(c, hipEventRecord(c->ev[3], c->stream));
-    c->last_op = 3;
+    c->last_op = 0;  // untimed, as training
+    c->timing = rt_timing{};
     if (ptr_kind == RT_PTR_HOST) {
         RT_HIP(c, hipMemcpyAsync(out, dout, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream));
         RT_HIP(c, hipStreamSynchronize(c->stream));

@@ -151,8 +151,11 @@ int rt_create(rt_ctx** out, int device) {
         rt_destroy(c);
         return fail(nullptr, RT_E_DEVICE, "rt_create: " + msg);
     }
+    // the ev[] events only time work (host reads of device data wait on
+    // snap_ev or a stream synchronisation), so they skip the system-scope
+    // cache fence that an event record carries by default
     for (int k = 0; k < rt_ctx::NUM_EV; ++k) {
-        e = hipEventCreate(&c->ev[k]);
+        e = hipEventCreateWithFlags(&c->ev[k], hipEventDisableSystemFence);
         if (e != hipSuccess) {
             rt_destroy(c);
             return fail(nullptr, RT_E_DEVICE, std::string("hipEventCreate: ") + hipGetErrorString(e));
@@ -207,12 +210,14 @@ int rt_get_timing(const rt_ctx* cc, rt_timing* out) {
     rt_ctx* c = const_cast<rt_ctx*>(cc);
     if (!c || !out) return RT_E_INVALID;
     DeviceGuard g(c->device);
-    if (c->last_op != 0) {
+    if (c->last_op == 1 || c->last_op == 5) {  // calls that bracket their kernel with ev[2] / ev[3]
         RT_HIP(c, hipEventSynchronize(c->ev[3]));
         float ms = 0.0f;
         RT_HIP(c, hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
         c->timing.last_kernel_ms = ms;
         if (c->last_op == 1) c->timing.last_call_ms = ms;
+    } else if (c->last_op != 0) {
+        c->timing.last_kernel_ms = 0.0f;  // renders and probe updates time the whole call only
     }
     *out = c->timing;
     return RT_OK;

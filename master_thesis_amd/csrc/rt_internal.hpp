@@ -98,7 +98,7 @@ struct rt_ctx {
     static constexpr int NUM_EV = 5;
     hipEvent_t ev[NUM_EV] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     rt_timing timing{};
-    int last_op = 0;  // 1 = trace, 2 = render
+    int last_op = 0;  // 0 = untimed, 1 = trace, 2 = render, 4 = probe update, 5 = post-filter
 };
 
 namespace rt {
