@@ -124,7 +124,7 @@ def main() -> int:
         workload = (f"ray-cast on the terrain: {sc.num_triangles} triangles (fBm heightfield 2048x2048 + 64 "
                     f"icospheres), {rays.shape[0]} rays ({n_coh} primary rays of 4 views at 1920x1080 + "
                     f"{rays.shape[0] - n_coh} first diffuse bounces from their hits), closest hit")
-        out = raycast(args, sc, rays, n_coh, workload, ROOT / "profiles" / "trace_terrain_16M_pmc.json")
+        out = raycast(args, sc, rays, n_coh, workload, ROOT / "profiles" / "trace_terrain_16M_pmc.json", cluster=True)
         out["metric"] = "Grays/s closest-hit (9.69M-triangle terrain)"
         out["value"] = round(out["value"] / 1e3, 4)
         out["unit"] = "Grays/s"
@@ -189,9 +189,10 @@ def terrain_rays(views: int = 4, width: int = 1920, height: int = 1080, seed: in
     return sc, np.concatenate([prim, sec]), prim.shape[0]
 
 
-def raycast(args, sc, rays, n_coh, workload, pmc_path):
+def raycast(args, sc, rays, n_coh, workload, pmc_path, cluster=False):
     """Closest hit of all rays, resident in HBM, one step = one pass; the
-    JSON record of the measurement."""
+    JSON record of the measurement.  cluster: the clustered hierarchy
+    (RT_BUILD_CLUSTER) in place of the Morton one."""
     import torch
 
     from master_thesis_amd.tracer import Tracer
@@ -212,8 +213,8 @@ def raycast(args, sc, rays, n_coh, workload, pmc_path):
     with Tracer(0) as t:
         t.set_stream(stream.cuda_stream)
         t.upload(sc)
-        info = t.build(args.leaf)
-        info = t.build(args.leaf)  # second build: steady-state build time
+        info = t.build(args.leaf, cluster=cluster)
+        info = t.build(args.leaf, cluster=cluster)  # second build: steady-state build time
         log(f"[bench] BVH4: {info['num_nodes']} nodes, depth {info['depth']}, build {info['build_ms']:.2f} ms")
 
         # counters pass (same kernel with per-wave counters): node / triangle fetches for the roofline
@@ -302,7 +303,9 @@ def raycast(args, sc, rays, n_coh, workload, pmc_path):
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": workload,
-                   "triangles": sc.num_triangles, "rays": n, "bvh": f"BVH4 LBVH, SAH leaves of <= {args.leaf}",
+                   "triangles": sc.num_triangles, "rays": n,
+                   "bvh": f"BVH4 from the {'clustered (PLOC)' if cluster else 'Morton-order (LBVH)'} binary tree, "
+                          f"SAH leaves of <= {args.leaf}",
                    "bvh_build_ms": round(info["build_ms"], 3),
                    "mrays_per_s_coherent": halves["coherent"] and round(halves["coherent"], 1),
                    "mrays_per_s_incoherent": halves["incoherent"] and round(halves["incoherent"], 1),
@@ -460,8 +463,8 @@ def bench_terrain(args):
                       sun_dir=(0.4, 0.8, 0.3), sun_radius_deg=5.0, sun_radiance=(40.0, 36.0, 30.0))
     with Tracer(0) as t:
         t.upload(sc)
-        info = t.build(args.leaf)
-        info = t.build(args.leaf)
+        info = t.build(args.leaf, cluster=True)
+        info = t.build(args.leaf, cluster=True)
         log(f"[bench] BVH4 {info['num_nodes']} nodes depth {info['depth']} build {info['build_ms']:.1f} ms")
         rgb, ms, rays = _render_timed(t, cams, s, 1)
     P = 1920 * 1080 * len(cams)
@@ -733,7 +736,7 @@ def bench_nrc(args):
     res = {}
     with Tracer(0) as t:
         t.upload(sc)
-        t.build(args.leaf)
+        t.build(args.leaf, cluster=args.nrc_scene == "terrain")
         cache = RadianceCache(t, lr=1e-2, seed=1)
         t0 = time.time()
         ref, _ = cache.render(cam, S(args.ref_spp, 0, 99), train=False, query=False, train_every=0)

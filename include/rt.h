@@ -141,8 +141,14 @@ typedef struct rt_tri_rec {
 
 typedef struct rt_build_opts {
     uint32_t leaf_size;   /* max triangles per leaf, 1..16 (0 = default 4); a subtree this small becomes a leaf when its surface-area cost says so */
-    uint32_t flags;       /* reserved, 0 */
+    uint32_t flags;       /* 0 or RT_BUILD_CLUSTER */
 } rt_build_opts;
+/* The default hierarchy is that of the Morton order of the triangles.  This
+ * flag clusters them instead (agglomerative, as the reference's builder,
+ * in parallel: PLOC): a lower surface-area cost, slower to build; fewer
+ * nodes per ray on scenes like the 9.69M-triangle terrain, more on a dense
+ * soup of mixed sizes.  Results do not depend on the flag. */
+#define RT_BUILD_CLUSTER 1u
 
 typedef struct rt_bvh_info {
     uint32_t num_nodes;       /* BVH4 nodes                                  */

@@ -94,6 +94,9 @@ class rt_tri_rec(C.Structure):
                 ("e1", C.c_float * 3), ("pad1", C.c_uint32)]
 
 
+RT_BUILD_CLUSTER = 1
+
+
 class rt_build_opts(C.Structure):
     _fields_ = [("leaf_size", C.c_uint32), ("flags", C.c_uint32)]
 

@@ -6,10 +6,15 @@
 //   1. 30-bit Morton code of each triangle's box centre (k_morton)
 //   2. LSD radix sort, 4 passes of 8 bits, stable (k_rs_hist, k_scan_*,
 //      k_rs_scatter)
-//   3. binary hierarchy from the sorted codes (Karras, "Maximizing parallelism
-//      in the construction of BVHs, octrees, and k-d trees", HPG 2012) (k_karras)
-//   4. node boxes as unions of their sorted leaf ranges: block unions, a
-//      sparse table over blocks, one range query per node (k_node_boxes)
+//   3. binary hierarchy from the sorted codes (Karras, "Maximizing
+//      parallelism in the construction of BVHs, octrees, and k-d trees",
+//      HPG 2012) (k_karras); with RT_BUILD_CLUSTER, parallel locally-ordered
+//      clustering of the sorted triangles instead (PLOC, k_ploc_*), its
+//      leaves then renumbered in depth-first order so that every subtree
+//      covers a contiguous run
+//   4. node boxes: unions of sorted leaf ranges (block unions, a sparse table
+//      over blocks, one range query per node) (k_node_boxes); the clustering
+//      makes its unions as it merges
 //   5. top-down collapse to 4-wide nodes: a node opens its largest-area inner
 //      child until it has four children; a subtree of <= leaf_size triangles
 //      becomes a leaf over a contiguous range of the sorted order when its
@@ -20,6 +25,8 @@
 // The topology does not affect results: the traversal's answer is defined by
 // the closest-hit rule alone (see trace.hip).  Boxes are padded outward so the
 // box test never rejects a triangle the intersection test would accept.
+#include <algorithm>
+
 #include "rt_internal.hpp"
 #include "rt_core.hpp"
 
@@ -312,6 +319,165 @@ __global__ void k_node_boxes(uint32_t ninner, Inner in, const float* __restrict_
     for (int k = 0; k < 6; ++k) innerbox[6u * p + k] = b[k];
 }
 
+// ---- PLOC: binary hierarchy by parallel locally-ordered clustering --------
+//
+// Meister and Bittner, "Parallel locally-ordered clustering for bounding
+// volume hierarchy construction", IEEE TVCG 24(12), 2018.  The clusters start
+// as the triangles in Morton order.  Every iteration each cluster finds, among
+// the PLOC_R clusters on either side in that order, the one whose union box
+// has the smallest surface area (ties: the lower index); mutual nearest
+// neighbours merge into a new inner node, which takes the lower one's place,
+// and the list is compacted in order (two exclusive scans give the new nodes'
+// numbers and the survivors' positions).  The best pair of the whole list
+// under (area, lower index, higher index) is always mutual, so every
+// iteration merges at least once and n - 1 merges end at one cluster.
+// Inner nodes are numbered from n - 2 down in the order they are made, so
+// the root is node 0 and every node is made in a later iteration than its
+// children.  During the build in.last[x] holds x's triangle count.  Then a
+// top-down pass, one launch per iteration from the last, gives every node
+// the range [first, last] of positions it covers in depth-first leaf order
+// and writes the triangle ids in that order: every subtree is a contiguous
+// run again, as the collapse and the triangle records expect.  Nothing here
+// depends on scheduling: the tree is a function of the triangles.
+constexpr int PLOC_R = 16;
+constexpr int PLOC_BLOCK = 256;
+constexpr int CTR_PM = 8;    // live clusters at the start of an iteration: slot 8 + parity
+constexpr int CTR_PC = 10;   // inner nodes made before an iteration: slot 10 + parity
+constexpr int CTR_CUM = 16;  // counters[16 + t]: inner nodes made before iteration t
+constexpr uint32_t PLOC_MAX_ITERS = 240;
+
+__device__ __forceinline__ float union_half_area(const float* a, const float* b) {
+    const float x = fmaxf(a[3], b[3]) - fminf(a[0], b[0]);
+    const float y = fmaxf(a[4], b[4]) - fminf(a[1], b[1]);
+    const float z = fmaxf(a[5], b[5]) - fminf(a[2], b[2]);
+    return x * y + y * z + z * x;
+}
+
+__device__ __forceinline__ uint32_t ref_size(uint32_t ref, const uint32_t* last) {
+    return (ref & RT_LEAF_BIT) ? 1u : last[ref];
+}
+
+// initial clusters: leaf i of the Morton order
+__global__ void k_ploc_init(uint32_t n, uint32_t* __restrict__ cr) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) cr[i] = RT_LEAF_BIT | i;
+}
+
+// nn[i]: the neighbour within PLOC_R of cluster i with the smallest union
+// area (the first of equal ones); a lone cluster is its own neighbour
+__global__ void __launch_bounds__(PLOC_BLOCK) k_ploc_nn(const float* __restrict__ cb, const uint32_t* __restrict__ ctr,
+                                                        int par, uint32_t* __restrict__ nn) {
+    __shared__ float sb[(PLOC_BLOCK + 2 * PLOC_R) * 6];
+    const uint32_t m = ctr[CTR_PM + par];
+    for (uint32_t b0 = blockIdx.x * PLOC_BLOCK; b0 < m; b0 += gridDim.x * PLOC_BLOCK) {
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < (PLOC_BLOCK + 2 * PLOC_R) * 6; k += PLOC_BLOCK) {
+            const int64_t g = (int64_t)b0 - PLOC_R + (int64_t)(k / 6);
+            sb[k] = (g >= 0 && g < (int64_t)m) ? cb[(size_t)g * 6 + k % 6] : 0.0f;
+        }
+        __syncthreads();
+        const uint32_t i = b0 + threadIdx.x;
+        if (i < m) {
+            const float* bi = sb + (threadIdx.x + PLOC_R) * 6;
+            float best = INFINITY;
+            uint32_t bj = i;
+            for (int d = -PLOC_R; d <= PLOC_R; ++d) {
+                const int64_t j = (int64_t)i + d;
+                if (d == 0 || j < 0 || j >= (int64_t)m) continue;
+                const float a = union_half_area(bi, sb + (threadIdx.x + PLOC_R + d) * 6);
+                if (a < best) {
+                    best = a;
+                    bj = (uint32_t)j;
+                }
+            }
+            nn[i] = bj;
+        }
+    }
+}
+
+// lead[i] = 1 where cluster i merges with a higher mutual neighbour, keep[i]
+// = 0 where it is absorbed by a lower one; entries m .. m_ub - 1 are zero
+__global__ void k_ploc_flags(const uint32_t* __restrict__ nn, const uint32_t* __restrict__ ctr, int par, uint32_t m_ub,
+                             uint32_t* __restrict__ lead, uint32_t* __restrict__ keep) {
+    const uint32_t m = ctr[CTR_PM + par];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m_ub; i += gridDim.x * blockDim.x) {
+        uint32_t l = 0u, k = 0u;
+        if (i < m) {
+            const uint32_t j = nn[i];
+            const bool mutual = j != i && nn[j] == i;
+            l = (mutual && i < j) ? 1u : 0u;
+            k = (mutual && i > j) ? 0u : 1u;
+        }
+        lead[i] = l;
+        keep[i] = k;
+    }
+}
+
+// merges and compaction; lead_x / keep_x are the exclusive scans of the flags
+__global__ void k_ploc_merge(const uint32_t* __restrict__ cr, const float* __restrict__ cb,
+                             const uint32_t* __restrict__ nn, const uint32_t* __restrict__ lead_x,
+                             const uint32_t* __restrict__ keep_x, uint32_t* ctr, int par, uint32_t n, uint32_t t,
+                             Inner in, float* __restrict__ innerbox, uint32_t* __restrict__ cr_out,
+                             float* __restrict__ cb_out) {
+    const uint32_t m = ctr[CTR_PM + par], made = ctr[CTR_PC + par];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+        const uint32_t j = nn[i];
+        const bool mutual = j != i && nn[j] == i;
+        const uint32_t p = keep_x[i];
+        if (mutual && i < j) {
+            const uint32_t slot = (n - 2u) - (made + lead_x[i]);
+            const uint32_t a = cr[i], b = cr[j];
+            in.left[slot] = a;
+            in.right[slot] = b;
+            in.last[slot] = ref_size(a, in.last) + ref_size(b, in.last);
+            cr_out[p] = slot;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const float lo = fminf(cb[6u * i + k], cb[6u * j + k]), hi = fmaxf(cb[6u * i + 3 + k], cb[6u * j + 3 + k]);
+                innerbox[6u * slot + k] = lo;
+                innerbox[6u * slot + 3 + k] = hi;
+                cb_out[6u * p + k] = lo;
+                cb_out[6u * p + 3 + k] = hi;
+            }
+        } else if (!mutual) {
+            cr_out[p] = cr[i];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) cb_out[6u * p + k] = cb[6u * i + k];
+        }
+        if (i == m - 1u) {
+            const uint32_t made_next = made + lead_x[i] + ((mutual && i < j) ? 1u : 0u);
+            ctr[CTR_PM + (par ^ 1)] = p + ((mutual && i > j) ? 0u : 1u);
+            ctr[CTR_PC + (par ^ 1)] = made_next;
+            ctr[CTR_CUM + t + 1u] = made_next;
+        }
+    }
+}
+
+// the nodes made in iteration t (their parents are done): children's first
+// positions, the triangle ids of leaf children in depth-first order, and the
+// node's last position
+__global__ void k_ploc_topdown(const uint32_t* __restrict__ ctr, uint32_t t, uint32_t n, Inner in,
+                               const uint32_t* __restrict__ vals_in, uint32_t* __restrict__ vals_out) {
+    const uint32_t lo = (n - 1u) - ctr[CTR_CUM + t + 1u], hi = (n - 1u) - ctr[CTR_CUM + t];
+    for (uint32_t s = lo + blockIdx.x * blockDim.x + threadIdx.x; s < hi; s += gridDim.x * blockDim.x) {
+        const uint32_t f = in.first[s], a = in.left[s], b = in.right[s], cnt = in.last[s];
+        const uint32_t ca = ref_size(a, in.last);
+        if (a & RT_LEAF_BIT) {
+            vals_out[f] = vals_in[a & ~RT_LEAF_BIT];
+            in.left[s] = RT_LEAF_BIT | f;
+        } else {
+            in.first[a] = f;
+        }
+        if (b & RT_LEAF_BIT) {
+            vals_out[f + ca] = vals_in[b & ~RT_LEAF_BIT];
+            in.right[s] = RT_LEAF_BIT | (f + ca);
+        } else {
+            in.first[b] = f + ca;
+        }
+        in.last[s] = f + cnt - 1u;
+    }
+}
+
 struct Entry {
     uint32_t ref, first, count;
     bool leaf;
@@ -575,7 +741,76 @@ int radix_sort_pairs(rt_ctx* c, uint32_t* keys[2], uint32_t* vals[2], uint32_t n
     return RT_OK;
 }
 
-int build_lbvh(rt_ctx* c, uint32_t leaf_size) {
+// PLOC hierarchy over the n triangles in Morton order (vals_morton); writes
+// in (left, right, first, last), the inner boxes and vals_tree, the triangle
+// ids in depth-first leaf order.  cr: two n-entry cluster buffers.  The host
+// reads each iteration's cluster count from a pinned copy one iteration
+// behind (it never drains the stream); the iteration launched after the last
+// merge finds one cluster and changes nothing.
+int build_ploc(rt_ctx* c, uint32_t n, const uint32_t* vals_morton, uint32_t* vals_tree, uint32_t* cr[2],
+               const float* leafbox, const Inner& in, float* innerbox) {
+    const size_t N = n;
+    hipStream_t st = c->stream;
+    const uint32_t stiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    int rc;
+    if ((rc = ensure(c, c->boxtab, 48ull * N)) || (rc = ensure(c, c->hist, 8ull * stiles + 64)) ||
+        (rc = ensure_snap(c)))
+        return rc;
+    float* cb[2] = {(float*)c->boxtab.ptr, (float*)c->boxtab.ptr + 6ull * N};
+    uint32_t* nn = (uint32_t*)c->queue[0].ptr;
+    uint32_t* lead = (uint32_t*)c->queue[1].ptr;
+    uint32_t* keep = lead + N;
+    uint32_t* tsum0 = (uint32_t*)c->hist.ptr;
+    uint32_t* tsum1 = tsum0 + stiles;
+    uint32_t* ctr = (uint32_t*)c->counters.ptr;
+    hipLaunchKernelGGL(k_sorted_boxes, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, vals_morton, leafbox, cb[0]);
+    hipLaunchKernelGGL(k_ploc_init, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, cr[0]);
+    uint32_t init[CTR_CUM + 1] = {};
+    init[CTR_PM] = n;
+    RT_HIP(c, hipMemcpyAsync(ctr, init, sizeof(init), hipMemcpyHostToDevice, st));
+    const uint32_t gcap = 16u * (uint32_t)c->num_cus;
+    uint32_t m_ub = n, iters = 0;
+    for (uint32_t t = 0;; ++t) {
+        if (t >= PLOC_MAX_ITERS) return fail(c, RT_E_DEVICE, "rt_build: clustering did not converge");
+        const int par = (int)(t & 1u);
+        const uint32_t g = std::min(blocks_for(m_ub, PLOC_BLOCK), gcap);
+        hipLaunchKernelGGL(k_ploc_nn, dim3(g), dim3(PLOC_BLOCK), 0, st, (const float*)cb[par], (const uint32_t*)ctr, par,
+                           nn);
+        hipLaunchKernelGGL(k_ploc_flags, dim3(g), dim3(256), 0, st, (const uint32_t*)nn, (const uint32_t*)ctr, par, m_ub,
+                           lead, keep);
+        const uint32_t tiles = (m_ub + SCAN_TILE - 1) / SCAN_TILE;
+        uint32_t* arr[2] = {lead, keep};
+        uint32_t* ts[2] = {tsum0, tsum1};
+        for (int k = 0; k < 2; ++k) {
+            hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(256), 0, st, arr[k], m_ub, ts[k]);
+            hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(256), 0, st, ts[k], tiles);
+            hipLaunchKernelGGL(k_scan_add, dim3(tiles), dim3(256), 0, st, arr[k], m_ub, (const uint32_t*)ts[k]);
+        }
+        hipLaunchKernelGGL(k_ploc_merge, dim3(g), dim3(256), 0, st, (const uint32_t*)cr[par], (const float*)cb[par],
+                           (const uint32_t*)nn, (const uint32_t*)lead, (const uint32_t*)keep, ctr, par, n, t, in,
+                           innerbox, cr[par ^ 1], cb[par ^ 1]);
+        RT_HIP(c, hipGetLastError());
+        RT_HIP(c, hipMemcpyAsync(c->snap + 16 * (t & 1u), ctr + CTR_PM, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                 st));
+        RT_HIP(c, hipEventRecord(c->snap_ev[t & 1u], st));
+        iters = t + 1;
+        if (t == 0) continue;
+        RT_HIP(c, hipEventSynchronize(c->snap_ev[(t - 1) & 1u]));
+        const uint32_t* h = c->snap + 16 * ((t - 1) & 1u);
+        const uint32_t m_after = h[((t - 1) & 1u) ^ 1u];  // clusters after iteration t - 1
+        if (m_after <= 1u) break;
+        m_ub = m_after;  // bounds the clusters of iteration t + 1
+    }
+    RT_HIP(c, hipMemsetAsync(in.first, 0, sizeof(uint32_t), st));  // the root covers positions from 0
+    const uint32_t gt = std::min(blocks_for(n / 2 + 1, 256), gcap);
+    for (uint32_t t = iters; t-- > 0;)
+        hipLaunchKernelGGL(k_ploc_topdown, dim3(gt), dim3(256), 0, st, (const uint32_t*)ctr, t, n, in, vals_morton,
+                           vals_tree);
+    RT_HIP(c, hipGetLastError());
+    return RT_OK;
+}
+
+int build_lbvh(rt_ctx* c, uint32_t leaf_size, uint32_t flags) {
     const uint32_t n = c->T;
     const size_t N = n;
     int rc;
@@ -584,7 +819,7 @@ int build_lbvh(rt_ctx* c, uint32_t leaf_size) {
         (rc = ensure(c, c->leafbox, 24 * N)) || (rc = ensure(c, c->inner, 6 * 4 * N)) ||
         (rc = ensure(c, c->innerbox, 24 * N)) ||
         (rc = ensure(c, c->queue[0], 16 * (N / 2 + 2))) || (rc = ensure(c, c->queue[1], 16 * (N / 2 + 2))) ||
-        (rc = ensure(c, c->counters, 64)) || (rc = ensure(c, c->nodes, sizeof(rt_bvh4_node) * (N + 1))) ||
+        (rc = ensure(c, c->counters, 4 * (CTR_CUM + PLOC_MAX_ITERS + 2))) || (rc = ensure(c, c->nodes, sizeof(rt_bvh4_node) * (N + 1))) ||
         (rc = ensure(c, c->tris, sizeof(rt_tri_rec) * N)))
         return rc;
     hipStream_t st = c->stream;
@@ -616,13 +851,14 @@ int build_lbvh(rt_ctx* c, uint32_t leaf_size) {
     rt_bvh4_node* nodes = (rt_bvh4_node*)c->nodes.ptr;
     uint32_t* counters = (uint32_t*)c->counters.ptr;
 
-    hipLaunchKernelGGL(k_tri_records, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, (const uint32_t*)vals[cur],
-                       (const float*)c->pos.ptr, (const uint32_t*)c->idx.ptr, (rt_tri_rec*)c->tris.ptr);
-
+    const uint32_t* tv = vals[cur];  // triangle ids in leaf order
     uint32_t depth = 1, num_nodes = 1, num_leaves = 1;
     if (n <= leaf_size) {
         hipLaunchKernelGGL(k_root_leaf, dim3(1), dim3(1), 0, st, n, (const float*)leafbox, (const uint32_t*)vals[cur],
                            pad, nodes);
+    } else if (flags & RT_BUILD_CLUSTER) {
+        if ((rc = build_ploc(c, n, vals[cur], vals[cur ^ 1], keys, leafbox, in, innerbox))) return rc;
+        tv = vals[cur ^ 1];
     } else {
         hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1, 256)), dim3(256), 0, st, (const uint32_t*)keys[cur], (int)n,
                            in);
@@ -640,6 +876,10 @@ int build_lbvh(rt_ctx* c, uint32_t leaf_size) {
                                (const float*)(table + 6ull * nb * (k - 1)), table + 6ull * nb * k);
         hipLaunchKernelGGL(k_node_boxes, dim3(blocks_for(n - 1, 256)), dim3(256), 0, st, n - 1, in, (const float*)sbox,
                            (const float*)table, nb, innerbox);
+    }
+    hipLaunchKernelGGL(k_tri_records, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, tv, (const float*)c->pos.ptr,
+                       (const uint32_t*)c->idx.ptr, (rt_tri_rec*)c->tris.ptr);
+    if (n > leaf_size) {
         uint32_t init[6] = {1u, 1u, 0u, 0u, 0u, 0u};  // root node; queue 0 holds the root's entry
         RT_HIP(c, hipMemcpyAsync(counters, init, sizeof(init), hipMemcpyHostToDevice, st));
         uint4 root = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
@@ -659,7 +899,7 @@ int build_lbvh(rt_ctx* c, uint32_t leaf_size) {
             hipLaunchKernelGGL(k_level_prep, dim3(1), dim3(64), 0, st, counters, in_slot, out_slot);
             hipLaunchKernelGGL(k_collapse, dim3(cgrid), dim3(COLLAPSE_BLOCK), 0, st, (const uint4*)c->queue[q].ptr,
                                (const uint32_t*)&counters[in_slot], (uint4*)c->queue[q ^ 1].ptr, in,
-                               (const uint32_t*)vals[cur], (const float*)leafbox, (const float*)innerbox, leaf_size,
+                               tv, (const float*)leafbox, (const float*)innerbox, leaf_size,
                                c->bvh_leaf_alpha, pad, nodes, max_nodes, max_queue, counters, out_slot);
             RT_HIP(c, hipGetLastError());
             RT_HIP(c, hipMemcpyAsync(c->snap + 16 * (level & 1), counters, 8 * sizeof(uint32_t),

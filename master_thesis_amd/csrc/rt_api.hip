@@ -347,8 +347,10 @@ int rt_build(rt_ctx* c, const rt_build_opts* o, rt_bvh_info* info) {
     if (c->T == 0) return fail(c, RT_E_STATE, "rt_build: no scene uploaded");
     uint32_t leaf = (o && o->leaf_size) ? o->leaf_size : 4u;
     if (leaf > RT_MAX_LEAF_SIZE) return fail(c, RT_E_INVALID, "rt_build: leaf_size above 16");
+    const uint32_t flags = o ? o->flags : 0u;
+    if (flags & ~(uint32_t)RT_BUILD_CLUSTER) return fail(c, RT_E_INVALID, "rt_build: unknown flags");
     DeviceGuard g(c->device);
-    int rc = build_lbvh(c, leaf);
+    int rc = build_lbvh(c, leaf, flags);
     if (rc) return rc;
     if (info) *info = c->info;
     return RT_OK;

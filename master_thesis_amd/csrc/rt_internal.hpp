@@ -117,7 +117,7 @@ int ensure_snap(rt_ctx* c);
     } while (0)
 
 // lbvh.hip
-int build_lbvh(rt_ctx* c, uint32_t leaf_size);
+int build_lbvh(rt_ctx* c, uint32_t leaf_size, uint32_t flags);
 // Stable LSD radix sort of n (key, value) pairs by the low 8 * passes bits of
 // the keys, ping-ponging between keys[0]/vals[0] (input) and keys[1]/vals[1];
 // *cur is the index of the buffers holding the result.  Uses c->hist.

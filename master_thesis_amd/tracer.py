@@ -69,8 +69,10 @@ class Tracer:
             rgb = np.ascontiguousarray(rgb, np.float32)
         self._check(self.lib.rt_set_material_colors(self.h, L._ptr(rgb), kind))
 
-    def build(self, leaf_size: int = 4) -> dict:
-        opts = L.rt_build_opts(leaf_size, 0)
+    def build(self, leaf_size: int = 4, cluster: bool = False) -> dict:
+        """Builds the BVH4: the hierarchy of the Morton order, or with
+        cluster=True the clustered one (RT_BUILD_CLUSTER)."""
+        opts = L.rt_build_opts(leaf_size, L.RT_BUILD_CLUSTER if cluster else 0)
         info = L.rt_bvh_info()
         self._check(self.lib.rt_build(self.h, C.byref(opts), C.byref(info)))
         self.info = {f: getattr(info, f) for f, _ in L.rt_bvh_info._fields_}

@@ -154,9 +154,9 @@ def _constant_params(c):
     return p
 
 
-def _scene_cache(tracer, sc, seed=0):
+def _scene_cache(tracer, sc, seed=0, cluster=False):
     tracer.upload(sc)
-    tracer.build(2)
+    tracer.build(2, cluster=cluster)
     return RadianceCache(tracer, lr=1e-2, seed=seed)
 
 
@@ -323,7 +323,7 @@ def test_terrain_constant_cache_matches_oracle(tracer, terrain):
     sampling, a constant cache: queries and records as the oracle's
     constant-cache render, which builds its own BVH; records in the same
     order with the same features and targets; images to summation order."""
-    cache = _scene_cache(tracer, terrain)
+    cache = _scene_cache(tracer, terrain, cluster=True)  # the terrain workloads' hierarchy
     c = np.array([0.5, 0.4, 0.3], np.float32)
     cache.set_params(_constant_params(c))
     cams = scenes.terrain_cameras(2, 48, 27)
@@ -347,7 +347,7 @@ def test_terrain_online_training_gain(tracer, terrain):
     the cache's bias (its 256-spp image against the reference) stays within
     a few percent of the frame mean."""
     cam = scenes.terrain_cameras(1, 128, 72)[0]
-    cache = _scene_cache(tracer, terrain, seed=1)
+    cache = _scene_cache(tracer, terrain, seed=1, cluster=True)
     ref, _ = cache.render(cam, _terrain_settings(1024, 0, 99), train=False, query=False, train_every=0)
 
     def frames(mode):

@@ -22,9 +22,9 @@ def _assert_same_hits(got, want, label=""):
     assert bad.size == 0, f"{label}: {bad.size} of {len(g)} rays differ; first {bad[:5]}: got {got[bad[:3]]} want {want[bad[:3]]}"
 
 
-def _upload(tracer, sc, leaf=4):
+def _upload(tracer, sc, leaf=4, cluster=False):
     tracer.upload(sc)
-    return tracer.build(leaf)
+    return tracer.build(leaf, cluster=cluster)
 
 
 def test_cornell_rays_bit_exact(tracer):
@@ -43,21 +43,29 @@ def test_cornell_rays_bit_exact(tracer):
     assert (got["tri"] != RT_MISS).mean() > 0.8  # open on one side: most rays still hit
 
 
-@pytest.mark.parametrize("T,leaf", [(1, 4), (3, 1), (5, 4), (64, 2), (1000, 4), (1000, 16), (30000, 4)])
-def test_soup_bit_exact_vs_brute(tracer, T, leaf):
+@pytest.mark.parametrize("T,leaf,cluster", [(1, 4, False), (3, 1, False), (5, 4, False), (64, 2, False),
+                                            (1000, 4, False), (1000, 16, False), (30000, 4, False), (2, 1, True),
+                                            (3, 1, True), (5, 4, True), (64, 2, True), (1000, 4, True),
+                                            (1000, 16, True), (30000, 4, True)])
+def test_soup_bit_exact_vs_brute(tracer, T, leaf, cluster):
     sc = scenes.triangle_soup(T, seed=T, mu=-2.0 if T < 100 else -4.5)
-    info = _upload(tracer, sc, leaf)
+    info = _upload(tracer, sc, leaf, cluster)
     assert info["num_triangles"] == T
     coh, inc = scenes.microbench_rays(64, 32, 3000, seed=T + 1)
     rays = np.concatenate([coh, inc])
     got = tracer.trace(rays)
     want = O.trace_brute(sc, rays)
-    _assert_same_hits(got, want, f"soup T={T} leaf={leaf}")
+    _assert_same_hits(got, want, f"soup T={T} leaf={leaf} cluster={cluster}")
 
 
-def test_bvh_structure_and_counters(tracer):
+@pytest.mark.parametrize("cluster", [False, True])
+def test_bvh_structure_and_counters(tracer, cluster):
+    """Both hierarchies (Morton order, clustered): a sound BVH4, quantised
+    boxes that contain the float ones, and the oracle's replay of the
+    exported nodes finds the same hits with the same node and triangle
+    counts."""
     sc = scenes.triangle_soup(50000, seed=77)
-    info = _upload(tracer, sc, 4)
+    info = _upload(tracer, sc, 4, cluster)
     nodes, tris = tracer.export_bvh()
     rc, depth = O.check_bvh4(nodes, tris, sc)
     assert rc == 0, f"BVH4 check failed with code {rc}"
@@ -94,7 +102,7 @@ def test_terrain_full_size_vs_cpu_bvh(tracer):
     first diffuse bounces from their hits (the terrain ray-cast workload's
     mix, at 320x180) agree bit for bit with the oracle's own BVH."""
     sc = scenes.terrain_scene()
-    info = _upload(tracer, sc, 4)
+    info = _upload(tracer, sc, 4, cluster=True)  # the hierarchy the terrain workloads use
     assert info["num_triangles"] == sc.num_triangles > 9_000_000
     prim = np.concatenate([scenes.pinhole_rays((42.0 * np.cos(a), 22.0, 42.0 * np.sin(a)), (0.0, 0.0, 0.0),
                                                45.0, 320, 180) for a in (0.3, 2.4)])
