@@ -22,7 +22,8 @@
 //                forward and the activation gradients in registers, the
 //                weight gradients of each layer from a [feature][sample]
 //                staging buffer in LDS; each workgroup sums its own partial
-//                gradient over its tiles in registers and writes it once
+//                gradient over its tiles in registers and writes it once,
+//                through LDS as whole 256-byte rows of 64-parameter blocks
 //                (no atomics, no read-modify-write of global memory)
 //   k_adam       sums the partials in a fixed order (one run per wave,
 //                then their sum: deterministic), Adam update, BF16 refresh
@@ -56,6 +57,14 @@ constexpr int NPARAM = offw(L);
 constexpr int offwb(int l) { int o = 0; for (int i = 0; i < l; ++i) o += ND[i] * KD[i]; return o; }
 constexpr int NWB = offwb(L);
 static_assert(NPARAM == 23888, "parameter count");
+// Per-workgroup partial gradients are stored by blocks of 64 parameters:
+// block b of every workgroup's partial, then block b + 1, so that the sum
+// over workgroups of one block reads one contiguous run.
+constexpr int PART_BLOCKS = (NPARAM + 63) / 64;
+constexpr int PART_STRIDE = 64 * PART_BLOCKS;  // floats per workgroup, the last block padded
+__device__ __forceinline__ size_t part_index(int block, uint32_t wg, uint32_t nparts) {
+    return ((size_t)block * nparts + wg) * 64;
+}
 static_assert(NWB == 23552, "bf16 weight count");
 
 
@@ -400,10 +409,15 @@ __global__ void __launch_bounds__(64 * NW) k_mlp_train(const float* __restrict__
     using S = TrainShape<NW>;
     constexpr int SR = S::SR;
     constexpr int KS = S::TILE / 32;  // k steps of a weight-gradient product
-    __shared__ __attribute__((aligned(16))) __bf16 sW[NWALL];    // forward and transposed weights
-    __shared__ __attribute__((aligned(16))) __bf16 sZ[64 * SR];  // dZ_l, [output][sample]
-    __shared__ __attribute__((aligned(16))) __bf16 sH[64 * SR];  // H_{l-1}, [input][sample]
-    __shared__ __attribute__((aligned(16))) __bf16 sX[96 * SR];  // the encoding, [feature][sample]
+    // one LDS block: the weights and the three staging buffers while the
+    // tiles run, the workgroup's whole partial gradient (fp32) at the end
+    constexpr int LDS_MAIN = 2 * (NWALL + 64 * SR + 64 * SR + 96 * SR);
+    static_assert(LDS_MAIN >= 4 * PART_STRIDE, "the partial gradient must fit the LDS block");
+    __shared__ __attribute__((aligned(16))) unsigned char smem[LDS_MAIN];
+    __bf16* const sW = (__bf16*)smem;                   // forward and transposed weights
+    __bf16* const sZ = sW + NWALL;                      // dZ_l, [output][sample]
+    __bf16* const sH = sZ + 64 * SR;                    // H_{l-1}, [input][sample]
+    __bf16* const sX = sH + 64 * SR;                    // the encoding, [feature][sample]
     __shared__ __attribute__((aligned(16))) float sB[NBIAS];
     __shared__ float sLoss[NW];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -599,12 +613,15 @@ __global__ void __launch_bounds__(64 * NW) k_mlp_train(const float* __restrict__
             }
         }
     }  // tiles
-    // the workgroup's partial gradient, written once; lane (mi, kg) of tile
-    // (row block r, column tile b) holds rows 16 r + 4 kg + i, column 16 b + mi.
-    // (The lane index passes through an empty asm so that the compiler does
-    // not compute these addresses before the loop and hold them across it.)
-    float* part = partial + (size_t)blockIdx.x * NPARAM;
-    auto put = [&](int q, float v) { part[q] = v; };
+    // the workgroup's partial gradient: first into LDS in parameter order
+    // (lane (mi, kg) of tile (row block r, column tile b) holds rows
+    // 16 r + 4 kg + i, column 16 b + mi), then out as whole 256-byte rows of
+    // the blocked layout.  (The lane index passes through an empty asm so
+    // that the compiler does not compute these addresses before the loop
+    // and hold them across it.)
+    __syncthreads();  // every wave is done with the weights and the staging buffers
+    float* const sP = (float*)smem;
+    auto put = [&](int q, float v) { sP[q] = v; };
     int lane_o = lane;
     __asm__ __volatile__("" : "+v"(lane_o));
     const int mo = lane_o & 15, ko = lane_o >> 4;
@@ -643,6 +660,14 @@ __global__ void __launch_bounds__(64 * NW) k_mlp_train(const float* __restrict__
         }
     }
     if (threadIdx.x == 0) loss_part[blockIdx.x] = lacc;
+    if (threadIdx.x < PART_STRIDE - NPARAM) sP[NPARAM + threadIdx.x] = 0.0f;  // the last block's padding
+    __syncthreads();
+    for (int f = threadIdx.x; f < PART_STRIDE / 4; f += S::NTH) {
+        const int p = 4 * f;
+        // streaming stores: k_adam reads each row once, right after
+        __builtin_nontemporal_store(*(const f32x4*)(sP + p),
+                                    (f32x4*)(partial + part_index(p / 64, blockIdx.x, gridDim.x) + p % 64));
+    }
 }
 
 // Adam with bias correction on the summed partial gradients (workgroup order,
@@ -685,11 +710,11 @@ __global__ void __launch_bounds__(ADAM_NT) k_adam(const float* __restrict__ part
 #pragma unroll
         for (int j = 0; j < PW; ++j) {
             const uint32_t w = q + QN * j;
-            x[j] = w < nparts ? partial[(size_t)w * NPARAM + p] : 0.0f;
+            x[j] = w < nparts ? __builtin_nontemporal_load(partial + part_index(blockIdx.x, w, nparts) + lane) : 0.0f;
         }
 #pragma unroll
         for (int j = 0; j < PW; ++j) g += x[j];
-        for (uint32_t w = q + QN * PW; w < nparts; w += QN) g += partial[(size_t)w * NPARAM + p];
+        for (uint32_t w = q + QN * PW; w < nparts; w += QN) g += partial[part_index(blockIdx.x, w, nparts) + lane];
     }
     s[q][lane] = g;
     __syncthreads();
@@ -784,7 +809,7 @@ int run_train(rt_nrc* s, const float* feats, const float* targets, uint32_t n, i
     const uint32_t tile = wide ? TrainShape<8>::TILE : TrainShape<4>::TILE;
     const uint32_t tiles = (n + tile - 1) / tile;
     const uint32_t wgs = tiles < (uint32_t)c->num_cus ? tiles : (uint32_t)c->num_cus;
-    if ((rc = ensure(c, s->partial, (size_t)wgs * NPARAM * 4)) || (rc = ensure(c, s->loss, (size_t)wgs * 4))) return rc;
+    if ((rc = ensure(c, s->partial, (size_t)wgs * PART_STRIDE * 4)) || (rc = ensure(c, s->loss, (size_t)wgs * 4))) return rc;
     if (wide)
         hipLaunchKernelGGL(k_mlp_train<8>, dim3(wgs), dim3(TrainShape<8>::NTH), 0, c->stream, (const float*)df,
                            (const float*)dt, n, (const __bf16*)s->wb.ptr, (const float*)s->wf.ptr, s->geo,
