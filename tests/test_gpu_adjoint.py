@@ -95,6 +95,48 @@ def test_gpu_adjoint_matches_central_differences(tracer, m, ch):
     assert abs(g[m, ch] - fd) <= 2e-2 * abs(fd) + 1e-3, (g[m, ch], fd)
 
 
+def test_full_configuration_l2_gradient(tracer):
+    """The differentiable workload at its full size: 64 albedo groups,
+    256 x 256, 32 spp, depth 4, L2 loss against a render with seed-drawn
+    true albedos.  The GPU's gradient (forward render, then the adjoint of
+    w = 2 (image - target)) equals the oracle's adjoint of the same w, and
+    central differences of the GPU renderer's loss (h = 1e-3, common random
+    numbers) for the group with the largest gradient and one near the
+    median."""
+    cam = H.cornell_camera(256, 256)
+    s = make_settings(spp=32, max_depth=4, seed=13, flags=RT_RENDER_DETERMINISTIC)
+    _setup(tracer, scenes.cornell_groups(seed=99))
+    target, _ = tracer.render(cam, s)
+    target = target.astype(np.float64)
+    sc = scenes.cornell_groups(seed=11)
+    _setup(tracer, sc)
+    img, _ = tracer.render(cam, s)
+    w = (2.0 * (img.astype(np.float64) - target)).astype(np.float32)
+    g = tracer.render_adjoint(cam, s, w)
+    want = O.render_adjoint(sc, cam, s, w)
+    scale = np.abs(want).max()
+    assert scale > 0
+    assert np.allclose(g, want, rtol=1e-4, atol=1e-5 * scale), np.abs(g - want).max()
+    base = sc.materials["color"].copy()
+    h = np.float32(1e-3)
+
+    def loss(c):
+        tracer.set_material_colors(c)
+        im, _ = tracer.render(cam, s)
+        return float(np.sum((im.astype(np.float64) - target) ** 2))
+
+    flat = np.abs(want[:64]).ravel()
+    picks = [int(np.argmax(flat)), int(np.argsort(flat)[flat.size // 2])]
+    for k in picks:
+        m, ch = divmod(k, 3)
+        cp, cm = base.copy(), base.copy()
+        cp[m, ch] += h
+        cm[m, ch] -= h
+        fd = (loss(cp) - loss(cm)) / (float(cp[m, ch]) - float(cm[m, ch]))
+        assert abs(g[m, ch] - fd) <= 2e-2 * abs(fd) + 1e-4 * scale, (m, ch, g[m, ch], fd)
+    tracer.set_material_colors(base)
+
+
 def test_autograd_function_gradient_and_descent(tracer):
     import torch
     from master_thesis_amd import diff

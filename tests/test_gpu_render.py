@@ -34,6 +34,22 @@ def test_cornell_deterministic_bit_exact(tracer, spp, depth, seed):
     assert rgb.mean() > 0.05
 
 
+def test_cornell_full_configuration_bit_exact(tracer):
+    """The Cornell-style box at its full benchmark configuration (256 x 256,
+    16 spp, depth 5, FOV 40): the deterministic render equals the oracle's
+    bit for bit, float image and 8-bit frame."""
+    sc = scenes.cornell_box()
+    _setup(tracer, sc)
+    cam = H.cornell_camera(256, 256)
+    s = make_settings(spp=16, max_depth=5, seed=5, flags=RT_RENDER_DETERMINISTIC)
+    rgb, rgb8 = tracer.render(cam, s, want_rgb8=True)
+    want, want8 = O.render(sc, cam, s)
+    diff = np.nonzero(rgb.view(np.uint32) != want.view(np.uint32))
+    assert diff[0].size == 0, f"{diff[0].size} channels differ, first at {diff[0][:3]},{diff[1][:3]}"
+    assert np.array_equal(rgb8, want8)
+    assert 0.3 < rgb.mean() < 1.0
+
+
 @pytest.mark.parametrize("spp,depth,seed", [(8, 6, 1), (5, 8, 2)])
 def test_specular_transmissive_deterministic_bit_exact(tracer, spp, depth, seed):
     """reflect and customRefract on the device (path_tracer.cpp:62-71): the
