@@ -3,12 +3,15 @@
 // res/shader/denoise_gauss_comp.comp:17-33 filter) and MedianDenoiser
 // (res/shader/denoise_median_comp.comp:22-46, a 3x3 median per channel).
 //
-// Both are HBM-bound stencils: a 16x16 tile of output pixels reads its
-// (16 + 2 off)^2 input footprint into LDS once, then every output pixel sums
-// (Gauss) or sorts (median) from LDS.  The arithmetic is the shaders':
-// out = sum over x (outer) and y (inner) of in(p + (x, y)) * g[x + off] *
-// g[y + off], taps outside the image skipped; the median network of the
-// shader on the 9 taps, taps outside the image 0.
+// Each workgroup computes a 64-pixel-wide tile of output pixels from its
+// input footprint staged in LDS once; each thread computes R vertically
+// adjacent pixels of one column, so a column of taps is read from LDS once
+// for all R.  The arithmetic is the shaders': out = sum over x (outer) and y
+// (inner) of (in(p + (x, y)) * g[x + off]) * g[y + off], taps outside the
+// image skipped (the product in(p + (x, y)) * g[x + off] depends on the tap
+// and x only, so it is formed once per tap and x and shared by the pixels
+// that use it: the same roundings); the median network of the shader on the
+// 9 taps, taps outside the image 0.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -19,25 +22,33 @@
 namespace rt {
 namespace {
 
-constexpr int DN_TILE = 16;
+constexpr int DN_W = 64;               // tile width: one column per thread
+constexpr int DN_G = 4;                // row groups (threads per column)
+constexpr int DN_NT = DN_W * DN_G;     // threads per workgroup
+// Rows per thread, R (the tile is DN_W x DN_G R): sharing a column of taps
+// over more rows saves LDS reads and products but costs registers, so it
+// pays for wide kernels only (measured at 3840 x 2160: Gauss 5 x 5 best at
+// R = 2, 19 x 19 at R = 8; the median at R = 4).
+constexpr int gauss_rows(int off) { return off >= 4 ? 8 : 2; }
+constexpr int MEDIAN_ROWS = 4;
 constexpr int MAX_KERNEL = 19;  // odd sizes up to the shader's 20-entry weight array
 
 struct Weights {
     float g[MAX_KERNEL + 1];
 };
 
-// A SPAN x SPAN pixel footprint whose top-left pixel is (x0, y0) into LDS
-// (3 floats per pixel, row-major), one pixel per lane and step; pixels
-// outside the image read 0.  (Flat float-row loads, and staging the output
-// tile through LDS for row stores, both measured slower.)
-template <int SPAN>
+// The SW x SH pixel footprint whose top-left pixel is (x0, y0) into LDS, 3
+// floats per pixel, row-major: each footprint row is 3 SW consecutive floats
+// of the image row, loaded as floats by consecutive lanes; pixels outside the
+// image read 0.
+template <int SW, int SH>
 __device__ __forceinline__ void load_tile(const float* in, float* tile, int x0, int y0, uint32_t W, uint32_t H) {
-    for (int i = threadIdx.x; i < SPAN * SPAN; i += blockDim.x) {
-        const int tx = i % SPAN, ty = i / SPAN;
-        const int gx = x0 + tx, gy = y0 + ty;
+    constexpr int RF = 3 * SW;  // floats per footprint row
+    for (int i = threadIdx.x; i < RF * SH; i += DN_NT) {
+        const int ty = i / RF, f = i - ty * RF;
+        const int gx = x0 + f / 3, gy = y0 + ty;
         const bool in_img = gx >= 0 && gx < (int)W && gy >= 0 && gy < (int)H;
-        const size_t src = in_img ? 3 * ((size_t)gy * W + gx) : 0;
-        for (int c = 0; c < 3; ++c) tile[3 * i + c] = in_img ? in[src + c] : 0.0f;
+        tile[i] = in_img ? in[3 * ((size_t)gy * W + x0) + f] : 0.0f;
     }
 }
 
@@ -45,49 +56,53 @@ __device__ __forceinline__ void load_tile(const float* in, float* tile, int x0, 
 // tiles away from the image border skip the per-tap bounds test (every tap
 // is inside, the same terms in the same order).
 template <int OFF>
-__global__ void __launch_bounds__(DN_TILE* DN_TILE) k_gauss(const float* in, float* out, uint32_t W, uint32_t H,
-                                                             Weights w) {
-    constexpr int SPAN = DN_TILE + 2 * OFF;
-    __shared__ float tile[SPAN * SPAN * 3];
-    const int x0 = (int)blockIdx.x * DN_TILE - OFF, y0 = (int)blockIdx.y * DN_TILE - OFF;
-    load_tile<SPAN>(in, tile, x0, y0, W, H);
+__global__ void __launch_bounds__(DN_NT) k_gauss(const float* in, float* out, uint32_t W, uint32_t H, Weights w) {
+    constexpr int DN_R = gauss_rows(OFF), DN_H = DN_G * DN_R;
+    constexpr int SW = DN_W + 2 * OFF, SH = DN_H + 2 * OFF, NR = DN_R + 2 * OFF;
+    __shared__ float tile[SW * SH * 3];
+    const int x0 = (int)blockIdx.x * DN_W - OFF, y0 = (int)blockIdx.y * DN_H - OFF;
+    load_tile<SW, SH>(in, tile, x0, y0, W, H);
     __syncthreads();
-    const int lx = threadIdx.x % DN_TILE, ly = threadIdx.x / DN_TILE;
-    const int px = (int)blockIdx.x * DN_TILE + lx, py = (int)blockIdx.y * DN_TILE + ly;
-    const bool interior = x0 >= 0 && y0 >= 0 && x0 + SPAN <= (int)W && y0 + SPAN <= (int)H;  // block-uniform
-    float r = 0.0f, g = 0.0f, b = 0.0f;
-    if (px >= (int)W || py >= (int)H) {
-        // outside the image: nothing to compute, the store skips it
-    } else if (interior) {
+    const int lx = threadIdx.x % DN_W, lr = (threadIdx.x / DN_W) * DN_R;  // column, first row in the tile
+    const int px = (int)blockIdx.x * DN_W + lx, py = (int)blockIdx.y * DN_H + lr;
+    if (px >= (int)W || py >= (int)H) return;
+    const bool interior = x0 >= 0 && y0 >= 0 && x0 + SW <= (int)W && y0 + SH <= (int)H;  // block-uniform
+    float acc[DN_R][3];
 #pragma unroll
-        for (int x = -OFF; x <= OFF; ++x) {
+    for (int p = 0; p < DN_R; ++p) acc[p][0] = acc[p][1] = acc[p][2] = 0.0f;
 #pragma unroll
-            for (int y = -OFF; y <= OFF; ++y) {
-                const float* t = tile + 3 * ((ly + OFF + y) * SPAN + (lx + OFF + x));
-                const float wx = w.g[x + OFF], wy = w.g[y + OFF];
-                r = r + t[0] * wx * wy;
-                g = g + t[1] * wx * wy;
-                b = b + t[2] * wx * wy;
-            }
+    for (int x = -OFF; x <= OFF; ++x) {
+        const float wx = w.g[x + OFF];
+        const bool col_in = interior || (px + x >= 0 && px + x < (int)W);
+        // tap rows lr .. lr + NR - 1 of column lx + OFF + x, times wx
+        float tw[NR][3];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const float* t = tile + 3 * ((lr + r) * SW + (lx + OFF + x));
+            tw[r][0] = t[0] * wx;
+            tw[r][1] = t[1] * wx;
+            tw[r][2] = t[2] * wx;
         }
-    } else {
 #pragma unroll
-        for (int x = -OFF; x <= OFF; ++x) {
+        for (int y = -OFF; y <= OFF; ++y) {
+            const float wy = w.g[y + OFF];
 #pragma unroll
-            for (int y = -OFF; y <= OFF; ++y) {
-                const int qx = px + x, qy = py + y;
-                if (qx < 0 || qx >= (int)W || qy < 0 || qy >= (int)H) continue;
-                const float* t = tile + 3 * ((ly + OFF + y) * SPAN + (lx + OFF + x));
-                const float wx = w.g[x + OFF], wy = w.g[y + OFF];
-                r = r + t[0] * wx * wy;
-                g = g + t[1] * wx * wy;
-                b = b + t[2] * wx * wy;
+            for (int p = 0; p < DN_R; ++p) {
+                const int r = p + y + OFF;
+                if (interior || (col_in && py + p + y >= 0 && py + p + y < (int)H)) {
+                    acc[p][0] = acc[p][0] + tw[r][0] * wy;
+                    acc[p][1] = acc[p][1] + tw[r][1] * wy;
+                    acc[p][2] = acc[p][2] + tw[r][2] * wy;
+                }
             }
         }
     }
-    if (px < (int)W && py < (int)H) {
-        float* o = out + 3 * ((size_t)py * W + px);
-        o[0] = r; o[1] = g; o[2] = b;
+#pragma unroll
+    for (int p = 0; p < DN_R; ++p) {
+        if (py + p < (int)H) {
+            float* o = out + 3 * ((size_t)(py + p) * W + px);
+            o[0] = acc[p][0]; o[1] = acc[p][1]; o[2] = acc[p][2];
+        }
     }
 }
 
@@ -98,33 +113,46 @@ __device__ __forceinline__ void s2(float& a, float& b) {
     b = fmaxf(t, b);
 }
 
-__global__ void __launch_bounds__(DN_TILE* DN_TILE) k_median(const float* in, float* out, uint32_t W, uint32_t H) {
-    constexpr int span = DN_TILE + 2;
-    __shared__ float tile[span * span * 3];
-    load_tile<span>(in, tile, (int)blockIdx.x * DN_TILE - 1, (int)blockIdx.y * DN_TILE - 1, W, H);
+__global__ void __launch_bounds__(DN_NT) k_median(const float* in, float* out, uint32_t W, uint32_t H) {
+    constexpr int DN_R = MEDIAN_ROWS, DN_H = DN_G * DN_R;
+    constexpr int SW = DN_W + 2, SH = DN_H + 2, NR = DN_R + 2;
+    __shared__ float tile[SW * SH * 3];
+    load_tile<SW, SH>(in, tile, (int)blockIdx.x * DN_W - 1, (int)blockIdx.y * DN_H - 1, W, H);
     __syncthreads();
-    const int lx = threadIdx.x % DN_TILE, ly = threadIdx.x / DN_TILE;
-    const int px = (int)blockIdx.x * DN_TILE + lx, py = (int)blockIdx.y * DN_TILE + ly;
+    const int lx = threadIdx.x % DN_W, lr = (threadIdx.x / DN_W) * DN_R;
+    const int px = (int)blockIdx.x * DN_W + lx, py = (int)blockIdx.y * DN_H + lr;
     if (px >= (int)W || py >= (int)H) return;
-    float* o = out + 3 * ((size_t)py * W + px);
+#pragma unroll
     for (int c = 0; c < 3; ++c) {
-        // v[(dx + 1) * 3 + (dy + 1)]
-        float v[9];
-        for (int dx = -1; dx <= 1; ++dx)
-            for (int dy = -1; dy <= 1; ++dy) v[(dx + 1) * 3 + (dy + 1)] = tile[3 * ((ly + 1 + dy) * span + (lx + 1 + dx)) + c];
-        // mnmx6(0..5)
-        s2(v[0], v[3]); s2(v[1], v[4]); s2(v[2], v[5]);
-        s2(v[0], v[1]); s2(v[0], v[2]);
-        s2(v[4], v[5]); s2(v[3], v[5]);
-        // mnmx5(1, 2, 3, 4, 6)
-        s2(v[1], v[2]); s2(v[3], v[4]);
-        s2(v[1], v[3]); s2(v[1], v[6]);
-        s2(v[4], v[6]); s2(v[2], v[6]);
-        // mnmx4(2, 3, 4, 7)
-        s2(v[2], v[3]); s2(v[4], v[7]); s2(v[2], v[4]); s2(v[3], v[7]);
-        // mnmx3(3, 4, 8)
-        s2(v[4], v[8]); s2(v[3], v[8]); s2(v[3], v[4]);
-        o[c] = v[4];
+        // the three tap columns of this thread's pixels, rows lr .. lr + NR - 1
+        float col[3][NR];
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+            for (int r = 0; r < NR; ++r) col[dx][r] = tile[3 * ((lr + r) * SW + (lx + dx)) + c];
+#pragma unroll
+        for (int p = 0; p < DN_R; ++p) {
+            if (py + p >= (int)H) break;
+            // v[(dx + 1) * 3 + (dy + 1)]
+            float v[9];
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+                for (int dy = 0; dy < 3; ++dy) v[dx * 3 + dy] = col[dx][p + dy];
+            // mnmx6(0..5)
+            s2(v[0], v[3]); s2(v[1], v[4]); s2(v[2], v[5]);
+            s2(v[0], v[1]); s2(v[0], v[2]);
+            s2(v[4], v[5]); s2(v[3], v[5]);
+            // mnmx5(1, 2, 3, 4, 6)
+            s2(v[1], v[2]); s2(v[3], v[4]);
+            s2(v[1], v[3]); s2(v[1], v[6]);
+            s2(v[4], v[6]); s2(v[2], v[6]);
+            // mnmx4(2, 3, 4, 7)
+            s2(v[2], v[3]); s2(v[4], v[7]); s2(v[2], v[4]); s2(v[3], v[7]);
+            // mnmx3(3, 4, 8)
+            s2(v[4], v[8]); s2(v[3], v[8]); s2(v[3], v[4]);
+            out[3 * ((size_t)(py + p) * W + px) + c] = v[4];
+        }
     }
 }
 
@@ -188,12 +216,13 @@ int rt_denoise_gauss(rt_ctx* c, const float* in, float* out, uint32_t width, uin
     const float* din;
     float* dout;
     if ((rc = stage_image(c, in, out, bytes, ptr_kind, &din, &dout))) return rc;
-    const dim3 grid((width + DN_TILE - 1) / DN_TILE, (height + DN_TILE - 1) / DN_TILE);
     RT_HIP(c, hipEventRecord(c->ev[2], c->stream));
     switch (kernel_size / 2) {
-#define RT_GAUSS_CASE(o)                                                                                  \
-    case o:                                                                                               \
-        hipLaunchKernelGGL(k_gauss<o>, grid, dim3(DN_TILE * DN_TILE), 0, c->stream, din, dout, width, height, w); \
+#define RT_GAUSS_CASE(o)                                                                                       \
+    case o:                                                                                                    \
+        hipLaunchKernelGGL(k_gauss<o>,                                                                         \
+                           dim3((width + DN_W - 1) / DN_W, (height + DN_G * gauss_rows(o) - 1) / (DN_G * gauss_rows(o))), \
+                           dim3(DN_NT), 0, c->stream, din, dout, width, height, w);                            \
         break;
         RT_GAUSS_CASE(0) RT_GAUSS_CASE(1) RT_GAUSS_CASE(2) RT_GAUSS_CASE(3) RT_GAUSS_CASE(4)
         RT_GAUSS_CASE(5) RT_GAUSS_CASE(6) RT_GAUSS_CASE(7) RT_GAUSS_CASE(8) RT_GAUSS_CASE(9)
@@ -220,9 +249,9 @@ int rt_denoise_median(rt_ctx* c, const float* in, float* out, uint32_t width, ui
     const float* din;
     float* dout;
     if ((rc = stage_image(c, in, out, bytes, ptr_kind, &din, &dout))) return rc;
-    const dim3 grid((width + DN_TILE - 1) / DN_TILE, (height + DN_TILE - 1) / DN_TILE);
+    const dim3 grid((width + DN_W - 1) / DN_W, (height + DN_G * MEDIAN_ROWS - 1) / (DN_G * MEDIAN_ROWS));
     RT_HIP(c, hipEventRecord(c->ev[2], c->stream));
-    hipLaunchKernelGGL(k_median, grid, dim3(DN_TILE * DN_TILE), 0, c->stream, din, dout, width, height);
+    hipLaunchKernelGGL(k_median, grid, dim3(DN_NT), 0, c->stream, din, dout, width, height);
     RT_HIP(c, hipGetLastError());
     RT_HIP(c, hipEventRecord(c->ev[3], c->stream));
     c->last_op = 5;
