@@ -50,6 +50,29 @@ def test_cornell_full_configuration_bit_exact(tracer):
     assert 0.3 < rgb.mean() < 1.0
 
 
+def test_converged_radiance_against_cpu_4096spp(tracer):
+    """Converged radiance: the GPU's Cornell render (throughput mode, depth
+    5) against the oracle CPU renderer's own 4096-spp image of the view
+    (independent random numbers).  At 1024 spp the relative RMSE is below
+    0.03, what the two renders' noise allows (the 16-spp frame's 0.175
+    scaled by sqrt(16/1024), with the reference's 4096-spp noise beside
+    it), and the 256-spp render is 1.6-2.8 times as far: the difference is
+    noise that falls as 1/sqrt(spp), not bias."""
+    sc = scenes.cornell_box()
+    _setup(tracer, sc)
+    cam = H.cornell_camera(48, 48)
+    ref, _ = O.render(sc, cam, make_settings(spp=4096, max_depth=5, seed=2))
+    ref = ref.astype(np.float64)
+
+    def rel_rmse(spp):
+        img, _ = tracer.render(cam, make_settings(spp=spp, max_depth=5, seed=1))
+        return float(np.sqrt(np.mean((img - ref) ** 2)) / np.sqrt(np.mean(ref ** 2)))
+
+    e1024, e256 = rel_rmse(1024), rel_rmse(256)
+    assert e1024 < 0.03, e1024
+    assert 1.6 < e256 / e1024 < 2.8, (e256, e1024)
+
+
 @pytest.mark.parametrize("spp,depth,seed", [(8, 6, 1), (5, 8, 2)])
 def test_specular_transmissive_deterministic_bit_exact(tracer, spp, depth, seed):
     """reflect and customRefract on the device (path_tracer.cpp:62-71): the
