@@ -145,6 +145,60 @@ def test_edge_case_rays(tracer):
     assert got["tri"][1] != RT_MISS and got["tri"][2] != RT_MISS and got["tri"][5] != RT_MISS
 
 
+def _degenerate_scene(seed: int = 8):
+    """A scene flat along z (every triangle in the plane z = 0: zero extent
+    for the Morton codes and every box on that axis), with zero-area
+    triangles (three collinear corners, a corner repeated, all three equal)
+    and exact duplicates of earlier triangles under higher indices."""
+    rng = np.random.default_rng(seed)
+    T = 3000
+    c = rng.uniform(0.0, 1.0, (T, 1, 2))
+    v = c + rng.uniform(-0.05, 0.05, (T, 3, 2))
+    verts = np.concatenate([v, np.zeros((T, 3, 1))], axis=2).astype(np.float32)
+    k = np.arange(0, 300)
+    verts[k, 2] = verts[k, 0] + 2.0 * (verts[k, 1] - verts[k, 0])  # collinear
+    verts[300:400, 1] = verts[300:400, 0]                          # a corner repeated
+    verts[400:450, 1:] = verts[400:450, :1]                        # a point
+    dup = rng.choice(np.arange(450, T), 500, replace=False)
+    verts = np.concatenate([verts, verts[dup]])
+    n = verts.shape[0]
+    pos = verts.reshape(-1, 3)
+    nrm = np.tile(np.float32([0, 0, 1]), (3 * n, 1))
+    sc = scenes.Scene(pos=np.ascontiguousarray(pos), nrm=nrm, idx=np.arange(3 * n, dtype=np.uint32).reshape(n, 3),
+                      tri_mat=np.zeros(n, np.uint32), materials=np.stack([scenes.material(scenes.WHITE)]))
+    sc.validate()
+    return sc, dup
+
+
+@pytest.mark.parametrize("cluster", [False, True])
+def test_flat_degenerate_and_duplicate_triangles(tracer, cluster):
+    """The flat scene with zero-area triangles and duplicates: hits equal
+    brute force for both hierarchies; zero-area triangles are never hit; a
+    duplicate is never reported (its original, at the same distance, has
+    the lower index); rays in the scene's plane miss."""
+    sc, dup = _degenerate_scene()
+    _upload(tracer, sc, 4, cluster)
+    rng = np.random.default_rng(9)
+    r = np.zeros(20000, RAY_DTYPE)
+    r["o"][:, :2] = rng.uniform(-0.1, 1.1, (20000, 2))
+    r["o"][:, 2] = rng.uniform(0.1, 2.0, 20000)
+    d = rng.normal(size=(20000, 3))
+    d[:, :2] *= 0.2
+    d[:, 2] = -np.abs(d[:, 2]) - 0.05
+    d[-500:, 2] = 0.0  # in the plane
+    r["o"][-500:, 2] = 0.0
+    r["d"] = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    r["tmax"] = np.inf
+    got = tracer.trace(r)
+    want = O.trace_brute(sc, r)
+    _assert_same_hits(got, want, f"flat cluster={cluster}")
+    hit = got["tri"][got["tri"] != RT_MISS]
+    assert hit.size > 5000
+    assert not np.any(hit < 450), "a zero-area triangle was hit"
+    assert not np.any(hit >= 3000), "a duplicate won over its lower-index original"
+    assert np.all(got["tri"][-500:] == RT_MISS)
+
+
 def test_empty_ray_batch(tracer):
     sc = scenes.cornell_box()
     _upload(tracer, sc)
