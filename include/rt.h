@@ -242,8 +242,13 @@ int  rt_bvh_export(rt_ctx* ctx, rt_bvh4_node* nodes, uint32_t* num_nodes,
  * NULL: query the count). */
 int  rt_bvh_export_traversal(rt_ctx* ctx, rt_bvh4q_node* nodes, uint32_t* num_nodes);
 
-/* Closest hit for n rays.  Replaces Scene::traceRay's geometric part
- * (graphic/scene.cpp:22-41 -> graphics_object.cpp:64-87 ->
+/* Most rays (or segments) one trace call takes: each call hands its rays
+ * to the waves in chunks of 128 counted in 32 bits; more returns
+ * RT_E_UNSUPPORTED (split the batch). */
+#define RT_MAX_RAYS 0xFFFFFF00u
+
+/* Closest hit for n rays (n <= RT_MAX_RAYS).  Replaces Scene::traceRay's
+ * geometric part (graphic/scene.cpp:22-41 -> graphics_object.cpp:64-87 ->
  * triangle.cpp:21-45).  counters may be NULL. */
 int  rt_trace_closest(rt_ctx* ctx, const rt_ray* rays, uint32_t n,
                       rt_hit* hits, int ptr_kind, rt_trace_counters* counters);

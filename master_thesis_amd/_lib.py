@@ -25,6 +25,7 @@ RT_E_UNSUPPORTED = -5
 RT_PTR_HOST = 0
 RT_PTR_DEVICE = 1
 RT_NOALLOC_SCRATCH_BYTES = 32 << 20  # rt_trace_closest_noalloc's fixed scratch per GPU
+RT_MAX_RAYS = 0xFFFFFF00  # rays (or segments) per trace call
 RT_MISS = 0xFFFFFFFF
 RT_CHILD_EMPTY = 0xFFFFFFFF
 RT_LEAF_BIT = 0x80000000

@@ -392,6 +392,7 @@ int rt_trace_closest(rt_ctx* c, const rt_ray* rays, uint32_t n, rt_hit* hits, in
                      rt_trace_counters* counters) {
     if (!c) return RT_E_INVALID;
     if (n > 0 && (!rays || !hits)) return fail(c, RT_E_INVALID, "rt_trace_closest: NULL rays or hits");
+    if (n > RT_MAX_RAYS) return fail(c, RT_E_UNSUPPORTED, "rt_trace_closest: more than RT_MAX_RAYS rays in one call");
     if (ptr_kind != RT_PTR_HOST && ptr_kind != RT_PTR_DEVICE)
         return fail(c, RT_E_INVALID, "rt_trace_closest: bad ptr_kind");
     if (!c->built) return fail(c, RT_E_STATE, "rt_trace_closest: call rt_build first");
@@ -433,6 +434,8 @@ int rt_trace_closest(rt_ctx* c, const rt_ray* rays, uint32_t n, rt_hit* hits, in
 int rt_trace_closest_noalloc(rt_ctx* c, const rt_ray* rays, uint32_t n, rt_hit* hits, uint32_t* status) {
     if (!c) return RT_E_INVALID;
     if (n > 0 && (!rays || !hits)) return fail(c, RT_E_INVALID, "rt_trace_closest_noalloc: NULL rays or hits");
+    if (n > RT_MAX_RAYS)
+        return fail(c, RT_E_UNSUPPORTED, "rt_trace_closest_noalloc: more than RT_MAX_RAYS rays in one call");
     if (!c->built) return fail(c, RT_E_STATE, "rt_trace_closest_noalloc: call rt_build first");
     if (n == 0) return RT_OK;
     DeviceGuard g(c->device);
@@ -447,6 +450,7 @@ static int occlusion_query(rt_ctx* c, const char* what, const void* in, size_t i
                            int ptr_kind, bool segments) {
     if (!c) return RT_E_INVALID;
     if (n > 0 && (!in || !out)) return fail(c, RT_E_INVALID, std::string(what) + ": NULL argument");
+    if (n > RT_MAX_RAYS) return fail(c, RT_E_UNSUPPORTED, std::string(what) + ": more than RT_MAX_RAYS queries in one call");
     if (ptr_kind != RT_PTR_HOST && ptr_kind != RT_PTR_DEVICE) return fail(c, RT_E_INVALID, std::string(what) + ": bad ptr_kind");
     if (!c->built) return fail(c, RT_E_STATE, std::string(what) + ": call rt_build first");
     if (n == 0) return RT_OK;

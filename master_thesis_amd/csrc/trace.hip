@@ -372,6 +372,8 @@ constexpr uint32_t WORK_ERR = 2 * WORK_HALF;                      // the sticky 
 int launch(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, uint8_t* d_occl, bool stats,
            const uint32_t* n_dev, const float* d_segs = nullptr, uint32_t* zero0 = nullptr,
            uint32_t* zero1 = nullptr) {
+    // the kernel's 32-bit ray chunks (pool_next + POOL_CHUNK) stay below 2^32
+    if (n > RT_MAX_RAYS) return fail(c, RT_E_UNSUPPORTED, "trace: more than RT_MAX_RAYS rays in one launch");
     if (n == 0) {
         if (zero0) RT_HIP(c, hipMemsetAsync(zero0, 0, 4, c->stream));
         if (zero1) RT_HIP(c, hipMemsetAsync(zero1, 0, 4, c->stream));
@@ -497,6 +499,7 @@ int noalloc_scratch_init(rt_ctx* c) {
 int trace_noalloc(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, uint32_t* d_status) {
     uint32_t* base = c->noalloc_scratch;
     if (!base) return fail(c, RT_E_STATE, "rt_trace_closest_noalloc: no scratch address for this device");
+    if (n > RT_MAX_RAYS) return fail(c, RT_E_UNSUPPORTED, "rt_trace_closest_noalloc: more than RT_MAX_RAYS rays");
     int& per_cu = c->trace_per_cu[0];
     if (per_cu == 0) {
         RT_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false, false>, TRACE_BLOCK, 0));

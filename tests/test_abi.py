@@ -62,6 +62,21 @@ def test_create_without_out_pointer():
     assert b"NULL" in lib.rt_last_error(None)
 
 
+def test_header_constants_match_binding():
+    """Every #define of include/rt.h that the Python binding mirrors has the
+    same value there."""
+    import re
+    text = (ROOT / "include" / "rt.h").read_text()
+    found = 0
+    for name, expr in re.findall(r"^#define\s+(RT_\w+)\s+(.+?)\s*(?:/\*.*)?$", text, re.M):
+        if not hasattr(L, name):
+            continue
+        value = eval(re.sub(r"\b(0x[0-9A-Fa-f]+|\d+)u\b", r"\1", expr), {})
+        assert getattr(L, name) == value, (name, expr, getattr(L, name))
+        found += 1
+    assert found >= 10
+
+
 def test_struct_layouts_match_header():
     # sizes are asserted at import; offsets of the fields the kernels read
     assert L.rt_ray.dx.offset == 16 and L.rt_ray.tmax.offset == 28

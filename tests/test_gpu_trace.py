@@ -199,6 +199,28 @@ def test_flat_degenerate_and_duplicate_triangles(tracer, cluster):
     assert np.all(got["tri"][-500:] == RT_MISS)
 
 
+def test_ray_count_above_the_limit_is_refused(tracer):
+    """More than RT_MAX_RAYS rays (or segments) in one call returns
+    RT_E_UNSUPPORTED before any memory is touched (the traversal counts its
+    ray chunks in 32 bits); the context stays usable."""
+    import ctypes as C
+    import torch
+    from master_thesis_amd import _lib as L
+    sc = scenes.cornell_box()
+    _upload(tracer, sc)
+    buf = torch.zeros(64, dtype=torch.float32, device="cuda")
+    p = C.c_void_p(buf.data_ptr())
+    lib = tracer.lib
+    n = L.RT_MAX_RAYS + 1
+    assert lib.rt_trace_closest(tracer.h, p, n, p, L.RT_PTR_DEVICE, None) == L.RT_E_UNSUPPORTED
+    assert b"RT_MAX_RAYS" in lib.rt_last_error(tracer.h)
+    assert lib.rt_trace_any(tracer.h, p, n, p, L.RT_PTR_DEVICE) == L.RT_E_UNSUPPORTED
+    assert lib.rt_occluded(tracer.h, p, n, p, L.RT_PTR_DEVICE) == L.RT_E_UNSUPPORTED
+    assert lib.rt_trace_closest_noalloc(tracer.h, p, n, p, None) == L.RT_E_UNSUPPORTED
+    rays = scenes.incoherent_rays(1000, seed=4)
+    _assert_same_hits(tracer.trace(rays), O.trace_brute(sc, rays), "after refusal")
+
+
 def test_empty_ray_batch(tracer):
     sc = scenes.cornell_box()
     _upload(tracer, sc)
