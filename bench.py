@@ -597,10 +597,37 @@ def _time_mlp(t, cache, sc, P: int):
     return ev[0].elapsed_time(ev[1]) / 20, ev[2].elapsed_time(ev[3]) / 20
 
 
+def _time_train_step(t, cache, sc, n: int, reps: int = 10) -> float:
+    """Device time of one n-sample training step (forward, backward, Adam),
+    averaged over `reps` steps with events on the context's stream."""
+    import torch
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    t.set_stream(stream.cuda_stream)
+    lo, hi = np.asarray(sc.pos.min(axis=0)), np.asarray(sc.pos.max(axis=0))
+    g = torch.Generator(device="cuda").manual_seed(1)
+    f = torch.rand((n, 9), device="cuda", generator=g)
+    f[:, :3] = f[:, :3] * torch.tensor(hi - lo, device="cuda") + torch.tensor(lo, device="cuda")
+    tg = torch.rand((n, 3), device="cuda", generator=g)
+    for _ in range(2):
+        cache.train(f, tg, want_loss=False)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        cache.train(f, tg, want_loss=False)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t.set_stream(None)
+    torch.cuda.set_stream(torch.cuda.default_stream())
+    return e0.elapsed_time(e1) / reps
+
+
 def bench_mlp(args):
     """The radiance cache's network alone: inference over one 1920x1080
     frame of queries and a 65,536-sample training step, against the BF16
-    dense MFMA peak."""
+    dense MFMA peak.  Beside the step, a 2^20-sample step: a step's cost is
+    a fixed part (the exchange of the workgroups' partial gradients and
+    Adam) plus a part per sample, and the large step shows the latter."""
     from master_thesis_amd import scenes
     from master_thesis_amd.cache import RadianceCache
     from master_thesis_amd.tracer import Tracer
@@ -611,7 +638,9 @@ def bench_mlp(args):
         t.build(args.leaf)
         cache = RadianceCache(t, seed=1)
         inf_ms, trn_ms = _time_mlp(t, cache, sc, P)
+        big_ms = _time_train_step(t, cache, sc, 1 << 20)
         cache.close()
+    big_tf = MLP_TRAIN_FLOP * (1 << 20) / (big_ms * 1e-3) / 1e12
     inf_tf = MLP_FWD_FLOP * P / (inf_ms * 1e-3) / 1e12
     trn_tf = MLP_TRAIN_FLOP * 65536 / (trn_ms * 1e-3) / 1e12
     return {"metric": "TFLOP/s radiance-cache network inference (1920x1080 queries, BF16 MFMA)",
@@ -624,7 +653,9 @@ def bench_mlp(args):
                          "kernel": "k_mlp_infer", "flop_per_sample": MLP_FWD_FLOP},
             "roofline_train": {"bound": "mfma", "achieved": round(trn_tf, 1), "peak": BF16_PEAK_TFLOPS,
                                "unit": "TFLOP/s", "frac": round(trn_tf / BF16_PEAK_TFLOPS, 4),
-                               "kernel": "k_mlp_train + k_adam", "flop_per_sample": MLP_TRAIN_FLOP},
+                               "kernel": "k_mlp_train + k_adam", "flop_per_sample": MLP_TRAIN_FLOP,
+                               "step_2p20_samples": {"ms": round(big_ms, 4), "tflops": round(big_tf, 1),
+                                                     "frac": round(big_tf / BF16_PEAK_TFLOPS, 4)}},
             "config": {"workload": "78-input MLP 96->64x4->16 (3 outputs), 2,073,600 queries per inference, "
                                    "65,536 samples per training step", "layers": 6}}
 
