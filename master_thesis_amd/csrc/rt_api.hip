@@ -133,6 +133,10 @@ int rt_create(rt_ctx** out, int device) {
         const unsigned long v = strtoul(r, nullptr, 10);
         c->trace_max_steps = (uint32_t)(v > 0xFFFFFFFFul ? 0xFFFFFFFFul : v);
     }
+    if (const char* r = getenv("MTRT_TRACE_UNIFIED_MAX")) {  // tuning knob: rays (0: the default rule)
+        const unsigned long v = strtoul(r, nullptr, 10);
+        c->trace_unified_max = (uint32_t)(v > 0xFFFFFFFFul ? 0xFFFFFFFFul : v);
+    }
     if (const char* r = getenv("MTRT_TRACE_SLICE")) {  // tuning knob, rounded to 128..2^24
         long v = strtol(r, nullptr, 10);
         v = v < 128 ? 128 : (v > (1l << 24) ? (1l << 24) : v);

@@ -63,6 +63,7 @@ struct TraceArgs {
     uint32_t spill_stride;
     uint32_t max_steps;    // bound on steps per ray (each node and triangle is visited at most once)
     uint32_t refill_min;   // idle lanes that trigger a refill (1..64)
+    uint32_t unified_max;  // launches of at most this many rays load each step's record in one round
     uint32_t slice;        // rays per XCD slice (a multiple of POOL_CHUNK)
     unsigned long long* stats;  // rays, nodes, tris, hits
 };
@@ -127,18 +128,41 @@ __device__ __forceinline__ uint32_t child_of(const uint32_t (&ref)[4], uint32_t 
     return (key & 2u) ? hi : lo;
 }
 
+// Where a step loads its record.  Split (launches with many rays): lanes at
+// nodes load the node inside the node block and lanes at triangles the
+// triangle inside the triangle block, so a wave's loads go out in two
+// rounds; with the GPU full of rays this spreads the requests of the
+// incoherent, cache-missing walks best.  Unified (launches with few rays,
+// which are bound by how long their slowest walks take): every lane loads
+// its record, a node's 64 bytes or a triangle's 48, before either block, so
+// a step waits for one round of loads.  The choice is per launch, from the
+// ray count (TraceArgs::unified_max).
 template <bool STATS, bool ANY>
 __device__ __forceinline__ bool lane_step(const TraceArgs& a, Lane& L, uint32_t* stk, uint32_t tid, uint32_t gtid,
-                                          uint64_t& c_nodes, uint64_t& c_tris) {
-    if (!(L.cur & RT_LEAF_BIT)) {
+                                          uint64_t& c_nodes, uint64_t& c_tris, bool unified) {
+    const bool at_leaf = (L.cur & RT_LEAF_BIT) != 0u;
+    uint4 w0, w1, w2, ch;
+    if (unified) {
+        const uint4* rec =
+            at_leaf ? (const uint4*)(a.tris + (L.cur & RT_LEAF_FIRST_MASK)) : (const uint4*)(a.nodes + L.cur);
+        w0 = rec[0]; w1 = rec[1]; w2 = rec[2]; ch = rec[at_leaf ? 0 : 3];
+        // (an empty asm that reads all four keeps the compiler from moving
+        // parts of these loads into the blocks below)
+        typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+        __asm__ __volatile__("" ::"v"(__builtin_bit_cast(v4u, w0)), "v"(__builtin_bit_cast(v4u, w1)),
+                             "v"(__builtin_bit_cast(v4u, w2)), "v"(__builtin_bit_cast(v4u, ch)));
+    }
+    if (!at_leaf) {
         // the 64-byte node as four 16-byte loads: origin + exponents, the x
         // and y bytes, the z bytes, the child references.  The plane of byte
         // q is origin + q * 2^e, so its entry distance is
         // (origin - o) / d + q * (2^e / d): one FMA per plane from two
         // per-axis terms, and the near plane's bytes are the low ones when
         // the direction is positive.
-        const uint4* qp = (const uint4*)(a.nodes + L.cur);
-        const uint4 w0 = qp[0], w1 = qp[1], w2 = qp[2], ch = qp[3];
+        if (!unified) {
+            const uint4* qp = (const uint4*)(a.nodes + L.cur);
+            w0 = qp[0]; w1 = qp[1]; w2 = qp[2]; ch = qp[3];
+        }
         if (STATS) ++c_nodes;
         const int ex = (int)(int8_t)(w0.w & 255u), ey = (int)(int8_t)((w0.w >> 8) & 255u),
                   ez = (int)(int8_t)((w0.w >> 16) & 255u);
@@ -198,8 +222,12 @@ __device__ __forceinline__ bool lane_step(const TraceArgs& a, Lane& L, uint32_t*
         // the leaf reference shrinks in place to the remaining triangles
         const uint32_t first = L.cur & RT_LEAF_FIRST_MASK;
         const uint32_t rest = (L.cur >> RT_LEAF_COUNT_SHIFT) & 15u;  // triangles after this one
-        const float4* rp = (const float4*)(a.tris + first);
-        const float4 p0 = rp[0], p1 = rp[1], p2 = rp[2];
+        if (!unified) {
+            const uint4* rp = (const uint4*)(a.tris + first);
+            w0 = rp[0]; w1 = rp[1]; w2 = rp[2];
+        }
+        const float4 p0 = __builtin_bit_cast(float4, w0), p1 = __builtin_bit_cast(float4, w1),
+                     p2 = __builtin_bit_cast(float4, w2);
         if (STATS) ++c_tris;
         float t, u, v;
         if (intersect_mt_fast(L.o, L.d, mk(p0.x, p0.y, p0.z), mk(p1.x, p1.y, p1.z), mk(p2.x, p2.y, p2.z), L.tmin,
@@ -263,6 +291,7 @@ __attribute__((amdgpu_waves_per_eu(STATS ? 5 : 6, STATS ? 5 : 6))) k_trace(Trace
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     uint64_t c_nodes = 0, c_tris = 0, c_hits = 0, c_rays = 0;
     const uint32_t n = a.n_dev ? *a.n_dev : a.n;
+    const bool unified = n <= a.unified_max;  // launch-uniform
     uint32_t pool_next = 0, pool_end = 0;  // wave-uniform
     bool exhausted = false;                // wave-uniform
     uint32_t part = blockIdx.x % XCDS;     // wave-uniform: the XCD whose slices this wave takes
@@ -339,7 +368,7 @@ __attribute__((amdgpu_waves_per_eu(STATS ? 5 : 6, STATS ? 5 : 6))) k_trace(Trace
             }
         }
         if (__ballot(!retired) == 0ull) break;
-        if (have && lane_step<STATS, ANY>(a, L, stk, tid, gtid, c_nodes, c_tris)) {
+        if (have && lane_step<STATS, ANY>(a, L, stk, tid, gtid, c_nodes, c_tris, unified)) {
             const bool hit = L.best_id != RT_MISS;
             if (ANY) {
                 a.occluded[ri] = hit ? 1 : 0;
@@ -367,6 +396,14 @@ __attribute__((amdgpu_waves_per_eu(STATS ? 5 : 6, STATS ? 5 : 6))) k_trace(Trace
 
 namespace {
 constexpr uint32_t WORK_HALF = XCD_COUNTER_STRIDE * (1 + XCDS);  // words per half of `work`
+
+// Launches up to this many rays take the unified loads: four rays per
+// resident lane (measured on the 1M-triangle soup: unified loads are
+// 2-13 % faster up to 2^20 incoherent rays, 7-9 % slower at 2^22-2^23, and
+// never slower for coherent rays); MTRT_TRACE_UNIFIED_MAX overrides it.
+uint32_t unified_max(const rt_ctx* c, uint32_t threads) {
+    return c->trace_unified_max ? c->trace_unified_max : 4u * threads;
+}
 constexpr uint32_t WORK_ERR = 2 * WORK_HALF;                      // the sticky error word
 
 int launch(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, uint8_t* d_occl, bool stats,
@@ -425,6 +462,7 @@ int launch(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, uint8_t*
     a.spill_stride = threads;
     a.max_steps = c->trace_max_steps ? c->trace_max_steps : 2u * c->num_nodes + 2u * c->T + 16u;
     a.refill_min = c->trace_refill_min;
+    a.unified_max = unified_max(c, threads);
     a.slice = c->trace_slice;
     a.stats = (unsigned long long*)c->stats.ptr;
     RT_HIP(c, hipEventRecord(c->ev[2], c->stream));
@@ -542,6 +580,7 @@ int trace_noalloc(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, u
     a.spill_stride = grid * TRACE_BLOCK;
     a.max_steps = c->trace_max_steps ? c->trace_max_steps : 2u * c->num_nodes + 2u * c->T + 16u;
     a.refill_min = c->trace_refill_min;
+    a.unified_max = unified_max(c, grid * TRACE_BLOCK);
     a.slice = c->trace_slice;
     a.stats = nullptr;
     hipLaunchKernelGGL((k_trace<false, false>), dim3(grid), dim3(TRACE_BLOCK), 0, c->stream, a);
