@@ -78,7 +78,9 @@ constexpr int C_RAYS = 4;   // 64-bit count of traced rays (slots 4-5)
 constexpr int C_QN = 6;     // cache queries queued since the last flush
 constexpr int C_TN = 7;     // training records made
 constexpr int C_QDROP = 8;  // queries dropped for want of room (stays 0: the host flushes in time)
-constexpr int C_SN = 9;     // shadow rays queued this iteration
+constexpr int C_SN0 = 9;    // shadow rays queued by an iteration that read queue 0
+constexpr int C_SN1 = 3;    // ... queue 1 (their trace runs with the next iteration's rays)
+__host__ __device__ constexpr int sn_slot(int par) { return par ? C_SN1 : C_SN0; }
 constexpr int C_SHRAYS = 10;  // 64-bit count of shadow rays (slots 10-11)
 constexpr int C_QTOT = 12;    // 64-bit count of cache queries answered (slots 12-13)
 constexpr int C_WORDS = 14;
@@ -226,7 +228,8 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_init(PTArgs a) {
         *(unsigned long long*)&a.ctr[C_RAYS] = 0ull;
         *(unsigned long long*)&a.ctr[C_SHRAYS] = 0ull;
         *(unsigned long long*)&a.ctr[C_QTOT] = 0ull;
-        a.ctr[C_SN] = 0;
+        a.ctr[C_SN0] = 0;
+        a.ctr[C_SN1] = 0;
         a.ctr[C_QN] = 0;
         a.ctr[C_TN] = 0;
         a.ctr[C_QDROP] = 0;
@@ -810,7 +813,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade_nrc(PTArgs a, int cur) {
     __shared__ uint32_t lds_alloc[4 * (PT_BLOCK / 64 + 1)];
     {
         const bool f[4] = {push_q, want_rec, push_sh, need_item};
-        uint32_t* const c[4] = {&a.ctr[C_QN], &a.ctr[C_TN], &a.ctr[C_SN], &a.ctr[C_ITEM]};
+        uint32_t* const c[4] = {&a.ctr[C_QN], &a.ctr[C_TN], &a.ctr[sn_slot(cur)], &a.ctr[C_ITEM]};
         uint32_t pos[4];
         block_alloc<PT_BLOCK, 4>(f, c, lds_alloc, lane, lt, pos);
         if (push_q) {
@@ -860,9 +863,10 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade_nrc(PTArgs a, int cur) {
     }
 }
 
-// Unoccluded light samples add their weights to the pixel sums and records.
-__global__ void __launch_bounds__(PT_BLOCK) k_shadow_resolve(PTArgs a) {
-    const uint32_t n = a.ctr[C_SN];
+// Unoccluded light samples add their weights to the pixel sums and records
+// (the shadow rays queued by the iteration of parity par).
+__global__ void __launch_bounds__(PT_BLOCK) k_shadow_resolve(PTArgs a, int par) {
+    const uint32_t n = a.ctr[sn_slot(par)];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) *(unsigned long long*)&a.ctr[C_SHRAYS] += n;
     if (i >= n || a.sh_occ[i]) return;
@@ -1147,21 +1151,44 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
     uint64_t chunk = 0;
     bool done = false;
     while (!done) {
+        // Shadow rays of a step are traced with the next step's rays, in one
+        // mixed launch, and their weights added right after it, before that
+        // step's shading; the last step of a chunk traces and resolves its own
+        // at once, so every cache flush still follows all light samples of
+        // the steps before it (the pixel sums take their terms in the same
+        // order either way).
+        bool pending = false;  // the previous step's shadow rays await their trace
         for (int rep = 0; rep < 8 && iter < bound; ++rep, ++iter) {
             // the trace clears the counters the shade step appends to (the
-            // next queue's; the shadow rays')
-            if ((rc = launch_trace(c, a.q_ray[cur], S, a.hits, false, &a.ctr[cur ? C_N1 : C_N0],
-                                   &a.ctr[cur ? C_N0 : C_N1], nrc && a.nee ? &a.ctr[C_SN] : nullptr)))
+            // next queue's; this step's shadow rays')
+            uint32_t* const z_sh = nrc && a.nee ? &a.ctr[sn_slot(cur)] : nullptr;
+            if (pending) {
+                if ((rc = launch_trace_mixed(c, a.q_ray[cur], &a.ctr[cur ? C_N1 : C_N0], a.hits, a.sh_ray,
+                                             &a.ctr[sn_slot(cur ^ 1)], a.sh_occ, 2 * S, &a.ctr[cur ? C_N0 : C_N1],
+                                             z_sh)))
+                    return rc;
+                hipLaunchKernelGGL(k_shadow_resolve, dim3(div_up(S, PT_BLOCK)), dim3(PT_BLOCK), 0, stream, a, cur ^ 1);
+                RT_HIP(c, hipGetLastError());
+                ++launches;
+                pending = false;
+            } else if ((rc = launch_trace(c, a.q_ray[cur], S, a.hits, false, &a.ctr[cur ? C_N1 : C_N0],
+                                          &a.ctr[cur ? C_N0 : C_N1], z_sh))) {
                 return rc;
+            }
             if (nrc) {
                 hipLaunchKernelGGL(k_pt_shade_nrc, dim3(div_up(S, PT_BLOCK)), dim3(PT_BLOCK), 0, stream, a, cur);
                 RT_HIP(c, hipGetLastError());
                 if (a.nee) {
-                    // light samples: occlusion of the queued shadow rays, then their weights
-                    if ((rc = launch_trace_any(c, a.sh_ray, S, a.sh_occ, &a.ctr[C_SN]))) return rc;
-                    hipLaunchKernelGGL(k_shadow_resolve, dim3(div_up(S, PT_BLOCK)), dim3(PT_BLOCK), 0, stream, a);
-                    RT_HIP(c, hipGetLastError());
-                    launches += 2;
+                    if (rep < 7 && iter + 1 < bound) {
+                        pending = true;
+                    } else {
+                        // light samples: occlusion of the queued shadow rays, then their weights
+                        if ((rc = launch_trace_any(c, a.sh_ray, S, a.sh_occ, &a.ctr[sn_slot(cur)]))) return rc;
+                        hipLaunchKernelGGL(k_shadow_resolve, dim3(div_up(S, PT_BLOCK)), dim3(PT_BLOCK), 0, stream,
+                                           a, cur);
+                        RT_HIP(c, hipGetLastError());
+                        launches += 2;
+                    }
                 }
             } else if (adj) {
                 hipLaunchKernelGGL(k_pt_shade<true>, dim3(div_up(S, PT_BLOCK)), dim3(PT_BLOCK), 0, stream, a, cur);

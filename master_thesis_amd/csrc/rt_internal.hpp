@@ -71,7 +71,7 @@ struct rt_ctx {
     // trace scratch
     rt::DevBuf rays, hits, spill, work, stats;
     uint32_t work_half = 0;  // traversal: which half of `work` the next launch counts rays in
-    int trace_per_cu[3] = {0, 0, 0};  // traversal: resident workgroups per CU (closest, counting, any)
+    int trace_per_cu[4] = {0, 0, 0, 0};  // traversal: resident workgroups per CU (closest, counting, any, mixed)
     uint32_t trace_max_steps = 0;  // traversal: per-ray step bound (0: from the tree size; MTRT_TRACE_MAX_STEPS)
     uint32_t trace_refill_min = 8;  // traversal: idle lanes that trigger a ray refill (MTRT_TRACE_REFILL)
     uint32_t trace_slice = 16384;   // traversal: rays per XCD slice, a multiple of 128 (MTRT_TRACE_SLICE)
@@ -141,6 +141,12 @@ int launch_trace(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, bo
 // occlusion: d_occluded[i] = 1 when ray i hits any triangle in (tmin, tmax)
 int launch_trace_any(rt_ctx* c, const rt_ray* d_rays, uint32_t n, uint8_t* d_occluded,
                      const uint32_t* n_dev = nullptr, uint32_t* zero0 = nullptr);
+// Both in one launch: closest hit for the *n_dev rays of d_rays into d_hits
+// and any hit for the *n2_dev rays of d_rays2 into d_occluded; capacity
+// bounds the two counts together.
+int launch_trace_mixed(rt_ctx* c, const rt_ray* d_rays, const uint32_t* n_dev, rt_hit* d_hits, const rt_ray* d_rays2,
+                       const uint32_t* n2_dev, uint8_t* d_occluded, uint32_t capacity, uint32_t* zero0 = nullptr,
+                       uint32_t* zero1 = nullptr);
 // Traversal error of the launches since the last call (0 none, 1 stack
 // overflow, 2 step bound; both mean a corrupt tree): read, then cleared.
 // Synchronises the stream.

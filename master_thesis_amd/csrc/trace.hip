@@ -51,6 +51,8 @@ struct TraceArgs {
 
     uint32_t n;
     const uint32_t* n_dev;  // when set, the ray count is read from device memory
+    const rt_ray* rays2;    // mixed launches: any-hit rays after the n closest-hit ones,
+    const uint32_t* n2_dev; // their count (device), their flags into `occluded`
     const rt_bvh4q_node* nodes;  // the 64-byte quantised traversal nodes
     const rt_tri_rec* tris;
     uint32_t* work;        // [32 (1 + x)] = next ray of XCD x's slices (zero at launch)
@@ -89,6 +91,7 @@ struct Lane {
     uint32_t best_id;
     uint32_t negx, negy, negz;  // per axis: 1 when the direction is negative
     uint32_t cur, sp, steps;
+    bool any;  // mixed launches: the ray is an any-hit one
 };
 
 __device__ __forceinline__ bool lane_start(Lane& L, const rt_ray& r) {
@@ -137,7 +140,7 @@ __device__ __forceinline__ uint32_t child_of(const uint32_t (&ref)[4], uint32_t 
 // its record, a node's 64 bytes or a triangle's 48, before either block, so
 // a step waits for one round of loads.  The choice is per launch, from the
 // ray count (TraceArgs::unified_max).
-template <bool STATS, bool ANY>
+template <bool STATS, bool ANY, bool MIXED>
 __device__ __forceinline__ bool lane_step(const TraceArgs& a, Lane& L, uint32_t* stk, uint32_t tid, uint32_t gtid,
                                           uint64_t& c_nodes, uint64_t& c_tris, bool unified) {
     const bool at_leaf = (L.cur & RT_LEAF_BIT) != 0u;
@@ -234,7 +237,7 @@ __device__ __forceinline__ bool lane_step(const TraceArgs& a, Lane& L, uint32_t*
                               L.tmax, L.tcull, t, u, v) &&
             t > L.tmin && t < L.tmax) {
             const uint32_t id = __float_as_uint(p0.w);
-            if (ANY) {
+            if (ANY || (MIXED && L.any)) {
                 if (!a.segs || dist2(L.o, hit_point(L.o, L.d, t)) <= L.seg_d2) {
                     L.best_id = id;
                     return true;
@@ -274,7 +277,11 @@ __device__ __forceinline__ bool lane_step(const TraceArgs& a, Lane& L, uint32_t*
 // Waves per SIMD: six for the closest-hit and any-hit walks (the walk is
 // latency bound, so the extra waves hide the node and triangle loads; seven
 // is slower), five for the counting variant, whose counters would spill at six.
-template <bool STATS, bool ANY>
+// MIXED: one launch for two batches, closest hit for the n rays of `rays`
+// (hits into `hits`) and any hit for the n2 rays of `rays2` (flags into
+// `occluded`), so that a path tracer's shadow rays of one step travel with
+// the next step's rays instead of in a launch of their own.
+template <bool STATS, bool ANY, bool MIXED = false>
 __global__ void __launch_bounds__(TRACE_BLOCK)
 __attribute__((amdgpu_waves_per_eu(STATS ? 5 : 6, STATS ? 5 : 6))) k_trace(TraceArgs a) {
     __shared__ uint32_t stk[STACK_LDS * TRACE_BLOCK];
@@ -290,8 +297,9 @@ __attribute__((amdgpu_waves_per_eu(STATS ? 5 : 6, STATS ? 5 : 6))) k_trace(Trace
     const uint32_t lane = tid & 63u;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     uint64_t c_nodes = 0, c_tris = 0, c_hits = 0, c_rays = 0;
-    const uint32_t n = a.n_dev ? *a.n_dev : a.n;
-    const bool unified = n <= a.unified_max;  // launch-uniform
+    const uint32_t n1 = a.n_dev ? *a.n_dev : a.n;  // closest-hit rays (any-hit ones in an any-hit launch)
+    const uint32_t n = MIXED ? n1 + *a.n2_dev : n1;  // rays in all
+    const bool unified = n <= a.unified_max;          // launch-uniform
     uint32_t pool_next = 0, pool_end = 0;  // wave-uniform
     bool exhausted = false;                // wave-uniform
     uint32_t part = blockIdx.x % XCDS;     // wave-uniform: the XCD whose slices this wave takes
@@ -346,12 +354,15 @@ __attribute__((amdgpu_waves_per_eu(STATS ? 5 : 6, STATS ? 5 : 6))) k_trace(Trace
                             L.seg_d2 = d2;
                             // conservative: a hit within the segment has t <= |end - start| (1 + 2^-10) + sigma
                             L.tcull = fmaf(sqrtf(d2), 1.0009765625f, L.sigma);
+                        } else if (MIXED) {
+                            L.any = ri >= n1;
+                            ok = lane_start(L, L.any ? a.rays2[ri - n1] : a.rays[ri]);
                         } else {
                             ok = lane_start(L, a.rays[ri]);
                         }
                         if (!ok) {
-                            if (ANY) {
-                                a.occluded[ri] = 0;
+                            if (ANY || (MIXED && L.any)) {
+                                a.occluded[MIXED ? ri - n1 : ri] = 0;
                             } else {
                                 rt_hit h;
                                 h.tri = RT_MISS; h.t = 0.0f; h.u = 0.0f; h.v = 0.0f;
@@ -368,10 +379,10 @@ __attribute__((amdgpu_waves_per_eu(STATS ? 5 : 6, STATS ? 5 : 6))) k_trace(Trace
             }
         }
         if (__ballot(!retired) == 0ull) break;
-        if (have && lane_step<STATS, ANY>(a, L, stk, tid, gtid, c_nodes, c_tris, unified)) {
+        if (have && lane_step<STATS, ANY, MIXED>(a, L, stk, tid, gtid, c_nodes, c_tris, unified)) {
             const bool hit = L.best_id != RT_MISS;
-            if (ANY) {
-                a.occluded[ri] = hit ? 1 : 0;
+            if (ANY || (MIXED && L.any)) {
+                a.occluded[MIXED ? ri - n1 : ri] = hit ? 1 : 0;
             } else {
                 rt_hit h;
                 h.tri = L.best_id;
@@ -406,9 +417,12 @@ uint32_t unified_max(const rt_ctx* c, uint32_t threads) {
 }
 constexpr uint32_t WORK_ERR = 2 * WORK_HALF;                      // the sticky error word
 
+// d_rays2 set: a mixed launch, closest hit for d_rays (count *n_dev) and any
+// hit for d_rays2 (count *n2_dev, flags into d_occl); n is then the two
+// batches' capacity together.
 int launch(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, uint8_t* d_occl, bool stats,
            const uint32_t* n_dev, const float* d_segs = nullptr, uint32_t* zero0 = nullptr,
-           uint32_t* zero1 = nullptr) {
+           uint32_t* zero1 = nullptr, const rt_ray* d_rays2 = nullptr, const uint32_t* n2_dev = nullptr) {
     // the kernel's 32-bit ray chunks (pool_next + POOL_CHUNK) stay below 2^32
     if (n > RT_MAX_RAYS) return fail(c, RT_E_UNSUPPORTED, "trace: more than RT_MAX_RAYS rays in one launch");
     if (n == 0) {
@@ -416,11 +430,14 @@ int launch(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, uint8_t*
         if (zero1) RT_HIP(c, hipMemsetAsync(zero1, 0, 4, c->stream));
         return RT_OK;
     }
-    const bool any = d_occl != nullptr;
+    const bool mixed = d_rays2 != nullptr;
+    const bool any = d_occl != nullptr && !mixed;
     // resident workgroups per CU of each variant, asked once per context
-    int& per_cu = c->trace_per_cu[any ? 2 : (stats ? 1 : 0)];
+    int& per_cu = c->trace_per_cu[mixed ? 3 : (any ? 2 : (stats ? 1 : 0))];
     if (per_cu == 0) {
-        if (any) RT_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false, true>, TRACE_BLOCK, 0));
+        if (mixed)
+            RT_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false, false, true>, TRACE_BLOCK, 0));
+        else if (any) RT_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false, true>, TRACE_BLOCK, 0));
         else if (stats) RT_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<true, false>, TRACE_BLOCK, 0));
         else RT_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false, false>, TRACE_BLOCK, 0));
         if (per_cu < 1) per_cu = 1;
@@ -450,6 +467,8 @@ int launch(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, uint8_t*
     a.segs = d_segs;
     a.n = n;
     a.n_dev = n_dev;
+    a.rays2 = d_rays2;
+    a.n2_dev = n2_dev;
     a.nodes = (const rt_bvh4q_node*)c->qnodes.ptr;
     a.tris = (const rt_tri_rec*)c->tris.ptr;
     a.work = (uint32_t*)c->work.ptr + WORK_HALF * c->work_half;
@@ -466,7 +485,8 @@ int launch(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, uint8_t*
     a.slice = c->trace_slice;
     a.stats = (unsigned long long*)c->stats.ptr;
     RT_HIP(c, hipEventRecord(c->ev[2], c->stream));
-    if (any) hipLaunchKernelGGL((k_trace<false, true>), dim3(grid), dim3(TRACE_BLOCK), 0, c->stream, a);
+    if (mixed) hipLaunchKernelGGL((k_trace<false, false, true>), dim3(grid), dim3(TRACE_BLOCK), 0, c->stream, a);
+    else if (any) hipLaunchKernelGGL((k_trace<false, true>), dim3(grid), dim3(TRACE_BLOCK), 0, c->stream, a);
     else if (stats) hipLaunchKernelGGL((k_trace<true, false>), dim3(grid), dim3(TRACE_BLOCK), 0, c->stream, a);
     else hipLaunchKernelGGL((k_trace<false, false>), dim3(grid), dim3(TRACE_BLOCK), 0, c->stream, a);
     RT_HIP(c, hipGetLastError());
@@ -484,6 +504,11 @@ int launch_trace(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, bo
 int launch_trace_any(rt_ctx* c, const rt_ray* d_rays, uint32_t n, uint8_t* d_occluded, const uint32_t* n_dev,
                      uint32_t* zero0) {
     return launch(c, d_rays, n, nullptr, d_occluded, false, n_dev, nullptr, zero0);
+}
+int launch_trace_mixed(rt_ctx* c, const rt_ray* d_rays, const uint32_t* n_dev, rt_hit* d_hits, const rt_ray* d_rays2,
+                       const uint32_t* n2_dev, uint8_t* d_occluded, uint32_t capacity, uint32_t* zero0,
+                       uint32_t* zero1) {
+    return launch(c, d_rays, capacity, d_hits, d_occluded, false, n_dev, nullptr, zero0, zero1, d_rays2, n2_dev);
 }
 int trace_error(rt_ctx* c, uint32_t* err) {
     *err = 0;
@@ -568,6 +593,8 @@ int trace_noalloc(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, u
     a.segs = nullptr;
     a.n = n;
     a.n_dev = nullptr;
+    a.rays2 = nullptr;
+    a.n2_dev = nullptr;
     a.nodes = (const rt_bvh4q_node*)c->qnodes.ptr;
     a.tris = (const rt_tri_rec*)c->tris.ptr;
     a.work = base;
