@@ -28,8 +28,8 @@ constexpr int DN_NT = DN_W * DN_G;     // threads per workgroup
 // Rows per thread, R (the tile is DN_W x DN_G R): sharing a column of taps
 // over more rows saves LDS reads and products but costs registers, so it
 // pays for wide kernels only (measured at 3840 x 2160: Gauss 5 x 5 best at
-// R = 2, 19 x 19 at R = 8; the median at R = 4).
-constexpr int gauss_rows(int off) { return off >= 4 ? 8 : 2; }
+// R = 2, 19 x 19 at R = 4; the median at R = 4).
+constexpr int gauss_rows(int off) { return off >= 4 ? 4 : 2; }
 constexpr int MEDIAN_ROWS = 4;
 constexpr int MAX_KERNEL = 19;  // odd sizes up to the shader's 20-entry weight array
 
@@ -38,42 +38,44 @@ struct Weights {
 };
 
 // The SW x SH pixel footprint whose top-left pixel is (x0, y0) into LDS, 3
-// floats per pixel, row-major: each footprint row is 3 SW consecutive floats
-// of the image row, loaded as floats by consecutive lanes; pixels outside the
-// image read 0.
+// floats per pixel, row-major.  Thread f < 3 SW owns float f of every
+// footprint row (one image column and channel), so its column test is made
+// once and each row costs one load at a running offset, all of them in
+// flight together; pixels outside the image read 0.  Image offsets fit in
+// 32 bits (at most 3 * 2^28 floats).
 template <int SW, int SH>
 __device__ __forceinline__ void load_tile(const float* in, float* tile, int x0, int y0, uint32_t W, uint32_t H) {
     constexpr int RF = 3 * SW;  // floats per footprint row
-    for (int i = threadIdx.x; i < RF * SH; i += DN_NT) {
-        const int ty = i / RF, f = i - ty * RF;
-        const int gx = x0 + f / 3, gy = y0 + ty;
-        const bool in_img = gx >= 0 && gx < (int)W && gy >= 0 && gy < (int)H;
-        tile[i] = in_img ? in[3 * ((size_t)gy * W + x0) + f] : 0.0f;
+    static_assert(RF <= DN_NT, "one pass of the workgroup loads a footprint row");
+    const int f = (int)threadIdx.x;
+    if (f >= RF) return;
+    const int gx = x0 + f / 3;
+    const bool col_in = gx >= 0 && gx < (int)W;
+    const int row = 3 * (int)W;
+    int off = y0 * row + 3 * x0 + f;
+#pragma unroll
+    for (int ty = 0; ty < SH; ++ty, off += row) {
+        const int gy = y0 + ty;
+        tile[ty * RF + f] = (col_in && gy >= 0 && gy < (int)H) ? in[off] : 0.0f;
     }
 }
 
-// OFF = kernel_size / 2, a compile-time constant so the tap loops unroll;
-// tiles away from the image border skip the per-tap bounds test (every tap
-// is inside, the same terms in the same order).
-template <int OFF>
-__global__ void __launch_bounds__(DN_NT) k_gauss(const float* in, float* out, uint32_t W, uint32_t H, Weights w) {
-    constexpr int DN_R = gauss_rows(OFF), DN_H = DN_G * DN_R;
-    constexpr int SW = DN_W + 2 * OFF, SH = DN_H + 2 * OFF, NR = DN_R + 2 * OFF;
-    __shared__ float tile[SW * SH * 3];
-    const int x0 = (int)blockIdx.x * DN_W - OFF, y0 = (int)blockIdx.y * DN_H - OFF;
-    load_tile<SW, SH>(in, tile, x0, y0, W, H);
-    __syncthreads();
-    const int lx = threadIdx.x % DN_W, lr = (threadIdx.x / DN_W) * DN_R;  // column, first row in the tile
-    const int px = (int)blockIdx.x * DN_W + lx, py = (int)blockIdx.y * DN_H + lr;
-    if (px >= (int)W || py >= (int)H) return;
-    const bool interior = x0 >= 0 && y0 >= 0 && x0 + SW <= (int)W && y0 + SH <= (int)H;  // block-uniform
+// A thread's DN_R pixels of one tile: OFF = kernel_size / 2, a compile-time
+// constant so the tap loops unroll; EDGE: the footprint reaches past the
+// image, so taps outside it are skipped (interior tiles test nothing: every
+// tap is inside, the same terms in the same order).
+template <int OFF, bool EDGE>
+__device__ __forceinline__ void gauss_pixels(const float* tile, float* out, const Weights& w, int lx, int lr, int px,
+                                             int py, uint32_t W, uint32_t H) {
+    constexpr int DN_R = gauss_rows(OFF);
+    constexpr int SW = DN_W + 2 * OFF, NR = DN_R + 2 * OFF;
     float acc[DN_R][3];
 #pragma unroll
     for (int p = 0; p < DN_R; ++p) acc[p][0] = acc[p][1] = acc[p][2] = 0.0f;
 #pragma unroll
     for (int x = -OFF; x <= OFF; ++x) {
         const float wx = w.g[x + OFF];
-        const bool col_in = interior || (px + x >= 0 && px + x < (int)W);
+        const bool col_in = !EDGE || (px + x >= 0 && px + x < (int)W);
         // tap rows lr .. lr + NR - 1 of column lx + OFF + x, times wx
         float tw[NR][3];
 #pragma unroll
@@ -89,7 +91,7 @@ __global__ void __launch_bounds__(DN_NT) k_gauss(const float* in, float* out, ui
 #pragma unroll
             for (int p = 0; p < DN_R; ++p) {
                 const int r = p + y + OFF;
-                if (interior || (col_in && py + p + y >= 0 && py + p + y < (int)H)) {
+                if (!EDGE || (col_in && py + p + y >= 0 && py + p + y < (int)H)) {
                     acc[p][0] = acc[p][0] + tw[r][0] * wy;
                     acc[p][1] = acc[p][1] + tw[r][1] * wy;
                     acc[p][2] = acc[p][2] + tw[r][2] * wy;
@@ -99,11 +101,28 @@ __global__ void __launch_bounds__(DN_NT) k_gauss(const float* in, float* out, ui
     }
 #pragma unroll
     for (int p = 0; p < DN_R; ++p) {
-        if (py + p < (int)H) {
+        if (!EDGE || py + p < (int)H) {
             float* o = out + 3 * ((size_t)(py + p) * W + px);
             o[0] = acc[p][0]; o[1] = acc[p][1]; o[2] = acc[p][2];
         }
     }
+}
+
+template <int OFF>
+__global__ void __launch_bounds__(DN_NT) k_gauss(const float* in, float* out, uint32_t W, uint32_t H, Weights w) {
+    constexpr int DN_R = gauss_rows(OFF), DN_H = DN_G * DN_R;
+    constexpr int SW = DN_W + 2 * OFF, SH = DN_H + 2 * OFF;
+    __shared__ float tile[SW * SH * 3];
+    const int x0 = (int)blockIdx.x * DN_W - OFF, y0 = (int)blockIdx.y * DN_H - OFF;
+    load_tile<SW, SH>(in, tile, x0, y0, W, H);
+    __syncthreads();
+    const int lx = threadIdx.x % DN_W, lr = (threadIdx.x / DN_W) * DN_R;  // column, first row in the tile
+    const int px = (int)blockIdx.x * DN_W + lx, py = (int)blockIdx.y * DN_H + lr;
+    if (px >= (int)W || py >= (int)H) return;
+    if (x0 >= 0 && y0 >= 0 && x0 + SW <= (int)W && y0 + SH <= (int)H)  // block-uniform
+        gauss_pixels<OFF, false>(tile, out, w, lx, lr, px, py, W, H);
+    else
+        gauss_pixels<OFF, true>(tile, out, w, lx, lr, px, py, W, H);
 }
 
 // the shader's compare-exchange: a = min(a, b), b = max(old a, b)

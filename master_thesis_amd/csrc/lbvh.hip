@@ -853,6 +853,7 @@ int build_lbvh(rt_ctx* c, uint32_t leaf_size, uint32_t flags) {
 
     const uint32_t* tv = vals[cur];  // triangle ids in leaf order
     uint32_t depth = 1, num_nodes = 1, num_leaves = 1;
+    std::vector<uint32_t> level_start = {0u, 1u};  // first node of each level (the collapse allocates level by level)
     if (n <= leaf_size) {
         hipLaunchKernelGGL(k_root_leaf, dim3(1), dim3(1), 0, st, n, (const float*)leafbox, (const uint32_t*)vals[cur],
                            pad, nodes);
@@ -910,6 +911,8 @@ int build_lbvh(rt_ctx* c, uint32_t leaf_size, uint32_t flags) {
             RT_HIP(c, hipEventSynchronize(c->snap_ev[(level - 1) & 1]));
             const uint32_t* h = c->snap + 16 * ((level - 1) & 1);
             if (h[CTR_ERROR]) return fail(c, RT_E_DEVICE, "rt_build: BVH collapse exceeded its structural bounds");
+            // after level l - 1, the nodes allocated so far are levels 0 .. l
+            level_start.push_back(h[CTR_NODES]);
             // the queue level - 1 appended to is the one this level read
             if (h[(level - 1) & 1 ? CTR_Q0 : CTR_Q1] == 0) break;
         }
@@ -934,6 +937,8 @@ int build_lbvh(rt_ctx* c, uint32_t leaf_size, uint32_t flags) {
 
     c->num_nodes = num_nodes;
     c->bvh_depth = depth;
+    if (level_start.back() != num_nodes) level_start.push_back(num_nodes);
+    set_coop_splits(c, level_start);
     c->built = true;
     c->info.num_nodes = num_nodes;
     c->info.depth = depth;

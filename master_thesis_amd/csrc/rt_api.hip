@@ -137,6 +137,14 @@ int rt_create(rt_ctx** out, int device) {
         const unsigned long v = strtoul(r, nullptr, 10);
         c->trace_unified_max = (uint32_t)(v > 0xFFFFFFFFul ? 0xFFFFFFFFul : v);
     }
+    if (const char* r = getenv("MTRT_TRACE_COOP")) {  // tuning knob, 0..6: smallest group width exponent (0: off)
+        long v = strtol(r, nullptr, 10);
+        c->trace_coop = (uint32_t)(v < 0 ? 0 : (v > 6 ? 6 : v));
+    }
+    if (const char* r = getenv("MTRT_TRACE_COOP_NODES")) {  // tuning knob, 1..4096
+        long v = strtol(r, nullptr, 10);
+        c->trace_coop_nodes = (uint32_t)(v < 1 ? 1 : (v > 4096 ? 4096 : v));
+    }
     if (const char* r = getenv("MTRT_TRACE_SLICE")) {  // tuning knob, rounded to 128..2^24
         long v = strtol(r, nullptr, 10);
         v = v < 128 ? 128 : (v > (1l << 24) ? (1l << 24) : v);

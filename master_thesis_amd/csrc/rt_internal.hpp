@@ -66,6 +66,15 @@ struct rt_ctx {
     rt::DevBuf nodes, tris, qnodes;  // build nodes (float boxes), triangle records, traversal nodes (64 B)
     uint32_t num_nodes = 0, bvh_depth = 0;
     rt_bvh_info info{};
+    // cooperative traversal of small launches (groups of 2^g lanes per ray):
+    // nodes with index < coop_split[g] are walked by every lane of a group,
+    // the subtrees below them are dealt out; coop_gmax = the largest g the
+    // tree has a split for (0: not used).  The collapse numbers nodes level by
+    // level, so a split is the first index of a level.
+    uint32_t coop_split[7] = {0, 0, 0, 0, 0, 0, 0};
+    uint32_t coop_gmax = 0;
+    uint32_t trace_coop = 1;         // launches with >= 2^trace_coop lanes per ray walk cooperatively (0: never; MTRT_TRACE_COOP)
+    uint32_t trace_coop_nodes = 16;  // nodes per lane of a group at its split level (MTRT_TRACE_COOP_NODES)
     // build scratch
     rt::DevBuf keys[2], vals[2], hist, inner, leafbox, innerbox, boxtab, queue[2], counters;
     // trace scratch
@@ -147,6 +156,10 @@ int launch_trace_any(rt_ctx* c, const rt_ray* d_rays, uint32_t n, uint8_t* d_occ
 int launch_trace_mixed(rt_ctx* c, const rt_ray* d_rays, const uint32_t* n_dev, rt_hit* d_hits, const rt_ray* d_rays2,
                        const uint32_t* n2_dev, uint8_t* d_occluded, uint32_t capacity, uint32_t* zero0 = nullptr,
                        uint32_t* zero1 = nullptr);
+// Sets the context's cooperative-traversal splits from the node count of each
+// BVH level (level_start[l] = index of level l's first node, the last entry
+// the node count).
+void set_coop_splits(rt_ctx* c, const std::vector<uint32_t>& level_start);
 // Traversal error of the launches since the last call (0 none, 1 stack
 // overflow, 2 step bound; both mean a corrupt tree): read, then cleared.
 // Synchronises the stream.

@@ -68,6 +68,13 @@ struct TraceArgs {
     uint32_t unified_max;  // launches of at most this many rays load each step's record in one round
     uint32_t slice;        // rays per XCD slice (a multiple of POOL_CHUNK)
     unsigned long long* stats;  // rays, nodes, tris, hits
+    // cooperative walk (k_trace_coop): launches of at most coop_max rays go to
+    // it (k_trace returns at once), with 2^g lanes per ray, g <= coop_gmax,
+    // and the nodes below index coop_split[g] walked by every lane of a group
+    uint32_t coop_max;
+    uint32_t coop_gmax;
+    uint32_t coop_split[7];
+    uint32_t coop_zero;    // the cooperative launch is the only one: it zeroes zero0 / zero1
 };
 
 constexpr uint32_t POOL_CHUNK = 128;  // rays a wave takes from a counter at a time
@@ -140,9 +147,18 @@ __device__ __forceinline__ uint32_t child_of(const uint32_t (&ref)[4], uint32_t 
 // its record, a node's 64 bytes or a triangle's 48, before either block, so
 // a step waits for one round of loads.  The choice is per launch, from the
 // ray count (TraceArgs::unified_max).
-template <bool STATS, bool ANY, bool MIXED>
+// Cooperative walks (COOP): the group's lane `sub` of 2^g pushes a child of
+// a node above the split (index < split) only when the child is above the
+// split too or is dealt to it (coop_owner); below the split it pushes every
+// child as usual.
+__device__ __forceinline__ uint32_t coop_owner(uint32_t ref, uint32_t g) {
+    return g ? (ref * 0x9E3779B1u) >> (32u - g) : 0u;
+}
+
+template <bool STATS, bool ANY, bool MIXED, bool COOP = false>
 __device__ __forceinline__ bool lane_step(const TraceArgs& a, Lane& L, uint32_t* stk, uint32_t tid, uint32_t gtid,
-                                          uint64_t& c_nodes, uint64_t& c_tris, bool unified) {
+                                          uint64_t& c_nodes, uint64_t& c_tris, bool unified, uint32_t split = 0,
+                                          uint32_t g = 0, uint32_t sub = 0) {
     const bool at_leaf = (L.cur & RT_LEAF_BIT) != 0u;
     uint4 w0, w1, w2, ch;
     if (unified) {
@@ -192,6 +208,13 @@ __device__ __forceinline__ bool lane_step(const TraceArgs& a, Lane& L, uint32_t*
                                    fminf(fmaf((float)((fzw >> (8 * c)) & 255u), bz, az), L.tcull));
             pk[c] = ((tn <= tf && ref[c] != RT_CHILD_EMPTY) ? (__float_as_uint(tn) & 0x7FFFFFFCu) : 0x7F800000u) |
                     (uint32_t)c;
+        }
+        if (COOP && L.cur < split) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const bool above = (ref[c] & RT_LEAF_BIT) == 0u && ref[c] < split;
+                if (!above && coop_owner(ref[c], g) != sub) pk[c] = 0x7F800000u | (uint32_t)c;
+            }
         }
         // ascending: 5-comparator network on integer min / max
 #define RT_CSWAP(i, j)                                  \
@@ -299,6 +322,7 @@ __attribute__((amdgpu_waves_per_eu(STATS ? 5 : 6, STATS ? 5 : 6))) k_trace(Trace
     uint64_t c_nodes = 0, c_tris = 0, c_hits = 0, c_rays = 0;
     const uint32_t n1 = a.n_dev ? *a.n_dev : a.n;  // closest-hit rays (any-hit ones in an any-hit launch)
     const uint32_t n = MIXED ? n1 + *a.n2_dev : n1;  // rays in all
+    if (n <= a.coop_max) return;                      // the cooperative walk takes this launch
     const bool unified = n <= a.unified_max;          // launch-uniform
     uint32_t pool_next = 0, pool_end = 0;  // wave-uniform
     bool exhausted = false;                // wave-uniform
@@ -403,6 +427,90 @@ __attribute__((amdgpu_waves_per_eu(STATS ? 5 : 6, STATS ? 5 : 6))) k_trace(Trace
     }
 }
 
+// Few rays: a group of 2^g lanes (g <= coop_gmax, as large as the grid
+// allows for n rays) walks each ray.  Every lane of a group walks the nodes
+// above the split (index < coop_split[g]: the top levels); each child of
+// such a node that is not itself above the split is dealt to one lane by a
+// hash of its reference, and that lane walks its whole subtree.  So every
+// subtree below the split is walked by exactly one lane, and the union of
+// the group's walks covers the tree as one walk does.  After every step the
+// lanes of a group take the smallest culling bound among them (a box is
+// still skipped only when it cannot hold a hit closer than one already
+// found), and at the end the group's hit is the closer() minimum of its
+// lanes' hits: the same answer as one lane walking alone.  A small launch
+// lasts as long as its longest walk (a few hundred dependent steps for rays
+// skimming a terrain), which a group cuts by up to its width.  Any hit: a hit
+// found by one lane ends the group's walk.
+template <bool ANY, bool MIXED>
+__global__ void __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6, 6))) k_trace_coop(TraceArgs a) {
+    __shared__ uint32_t stk[STACK_LDS * TRACE_BLOCK];
+    const uint32_t tid = threadIdx.x;
+    if (a.coop_zero && blockIdx.x == 0 && tid == 0) {
+        if (a.zero0) *a.zero0 = 0u;
+        if (a.zero1) *a.zero1 = 0u;
+    }
+    const uint32_t n1 = a.n_dev ? *a.n_dev : a.n;
+    const uint32_t n = MIXED ? n1 + *a.n2_dev : n1;
+    if (n == 0 || n > a.coop_max) return;  // empty, or the plain walk takes this launch
+    const uint32_t lanes = gridDim.x * TRACE_BLOCK;
+    uint32_t g = 0;  // launch-uniform: the widest group with n groups in the grid
+    while (g < a.coop_gmax && (lanes >> (g + 1u)) >= n) ++g;
+    const uint32_t gtid = blockIdx.x * TRACE_BLOCK + tid;
+    const uint32_t ri = gtid >> g, sub = gtid & ((1u << g) - 1u);
+    if (ri >= n) return;  // whole groups: a group's lanes are consecutive in one wave
+    const uint32_t split = a.coop_split[g];
+    const uint32_t lane = tid & 63u;
+    // this lane's group in the wave's ballot masks
+    const uint64_t gmask = (g == 6u ? ~0ull : ((1ull << (1u << g)) - 1ull)) << (lane & ~((1u << g) - 1u));
+    Lane L;
+    bool ok;
+    if (MIXED) {
+        L.any = ri >= n1;
+        ok = lane_start(L, L.any ? a.rays2[ri - n1] : a.rays[ri]);
+    } else {
+        ok = lane_start(L, a.rays[ri]);
+    }
+    const bool any_ray = ANY || (MIXED && L.any);  // group-uniform
+    bool done = !ok;
+    uint64_t c_nodes = 0, c_tris = 0;
+    while (true) {
+        if (!done) done = lane_step<false, ANY, MIXED, true>(a, L, stk, tid, gtid, c_nodes, c_tris, true, split, g, sub);
+        if (ANY || MIXED) {
+            if (__ballot(any_ray && L.best_id != RT_MISS) & gmask) done = true;
+        }
+        if (__ballot(!done) == 0ull) break;
+        // the group's culling bound: the smallest of its lanes'
+        float tc = L.tcull;
+        for (uint32_t s = 1; s < (1u << g); s <<= 1) tc = fminf(tc, __shfl_xor(tc, (int)s));
+        L.tcull = tc;
+    }
+    if (any_ray) {
+        // (the ballot outside the sub == 0 test: it sees only the lanes that run it)
+        const bool hit = (__ballot(L.best_id != RT_MISS) & gmask) != 0ull;
+        if (sub == 0) a.occluded[MIXED ? ri - n1 : ri] = hit ? 1 : 0;
+        return;
+    }
+    float d2 = L.best_d2, t = L.best_t, u = L.best_u, v = L.best_v;
+    uint32_t id = L.best_id;
+    for (uint32_t s = 1; s < (1u << g); s <<= 1) {
+        const float d2o = __shfl_xor(d2, (int)s), to = __shfl_xor(t, (int)s), uo = __shfl_xor(u, (int)s),
+                    vo = __shfl_xor(v, (int)s);
+        const uint32_t ido = __shfl_xor(id, (int)s);
+        if (closer(d2o, ido, d2, id)) {
+            d2 = d2o; id = ido; t = to; u = uo; v = vo;
+        }
+    }
+    if (sub == 0) {
+        const bool hit = id != RT_MISS;
+        rt_hit h;
+        h.tri = id;
+        h.t = hit ? t : 0.0f;
+        h.u = hit ? u : 0.0f;
+        h.v = hit ? v : 0.0f;
+        a.hits[ri] = h;
+    }
+}
+
 }  // namespace
 
 namespace {
@@ -442,10 +550,16 @@ int launch(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, uint8_t*
         else RT_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false, false>, TRACE_BLOCK, 0));
         if (per_cu < 1) per_cu = 1;
     }
+    // cooperative walks for small launches (not for the counting variant or
+    // segment queries): the grid then leaves room for 64 lanes per ray
+    const bool coop = c->trace_coop > 0 && c->coop_gmax > 0 && !stats && !d_segs;
+    const uint64_t want = coop ? 64ull * n : (uint64_t)n;
     uint32_t grid = (uint32_t)(c->num_cus * per_cu);
-    uint32_t need_blocks = (n + TRACE_BLOCK - 1) / TRACE_BLOCK;
-    if (grid > need_blocks) grid = need_blocks;
+    const uint64_t need_blocks = (want + TRACE_BLOCK - 1) / TRACE_BLOCK;
+    if (grid > need_blocks) grid = (uint32_t)need_blocks;
     const uint32_t threads = grid * TRACE_BLOCK;
+    // launches of at most coop_max rays have at least 2^trace_coop lanes per ray
+    const uint32_t coop_max = coop ? threads >> c->trace_coop : 0u;
     const uint32_t max_stack = 3u * c->bvh_depth + 1u;
     const uint32_t spill_cap = max_stack > (uint32_t)STACK_LDS ? max_stack - STACK_LDS : 1u;
     int rc;
@@ -484,13 +598,31 @@ int launch(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, uint8_t*
     a.unified_max = unified_max(c, threads);
     a.slice = c->trace_slice;
     a.stats = (unsigned long long*)c->stats.ptr;
+    a.coop_max = coop_max;
+    a.coop_gmax = c->coop_gmax;
+    for (int g = 0; g < 7; ++g) a.coop_split[g] = c->coop_split[g];
+    a.coop_zero = 0;
+    // A count on the device: both kernels, each returning at once when the
+    // launch is the other's (the plain one first: it also zeroes the next
+    // launch's counters).  A host count: the one that takes it.
+    const bool run_plain = !coop || n_dev || n > coop_max;
+    const bool run_coop = coop && (n_dev || n <= coop_max);
     RT_HIP(c, hipEventRecord(c->ev[2], c->stream));
-    if (mixed) hipLaunchKernelGGL((k_trace<false, false, true>), dim3(grid), dim3(TRACE_BLOCK), 0, c->stream, a);
-    else if (any) hipLaunchKernelGGL((k_trace<false, true>), dim3(grid), dim3(TRACE_BLOCK), 0, c->stream, a);
-    else if (stats) hipLaunchKernelGGL((k_trace<true, false>), dim3(grid), dim3(TRACE_BLOCK), 0, c->stream, a);
-    else hipLaunchKernelGGL((k_trace<false, false>), dim3(grid), dim3(TRACE_BLOCK), 0, c->stream, a);
-    RT_HIP(c, hipGetLastError());
-    c->work_half ^= 1u;
+    if (run_plain) {
+        if (mixed) hipLaunchKernelGGL((k_trace<false, false, true>), dim3(grid), dim3(TRACE_BLOCK), 0, c->stream, a);
+        else if (any) hipLaunchKernelGGL((k_trace<false, true>), dim3(grid), dim3(TRACE_BLOCK), 0, c->stream, a);
+        else if (stats) hipLaunchKernelGGL((k_trace<true, false>), dim3(grid), dim3(TRACE_BLOCK), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_trace<false, false>), dim3(grid), dim3(TRACE_BLOCK), 0, c->stream, a);
+        RT_HIP(c, hipGetLastError());
+        c->work_half ^= 1u;  // (a launch of the cooperative walk alone leaves the counters untouched)
+    }
+    if (run_coop) {
+        a.coop_zero = run_plain ? 0u : 1u;
+        if (mixed) hipLaunchKernelGGL((k_trace_coop<false, true>), dim3(grid), dim3(TRACE_BLOCK), 0, c->stream, a);
+        else if (any) hipLaunchKernelGGL((k_trace_coop<true, false>), dim3(grid), dim3(TRACE_BLOCK), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_trace_coop<false, false>), dim3(grid), dim3(TRACE_BLOCK), 0, c->stream, a);
+        RT_HIP(c, hipGetLastError());
+    }
     RT_HIP(c, hipEventRecord(c->ev[3], c->stream));
     c->timing.launches = 1;
     return RT_OK;
@@ -509,6 +641,21 @@ int launch_trace_mixed(rt_ctx* c, const rt_ray* d_rays, const uint32_t* n_dev, r
                        const uint32_t* n2_dev, uint8_t* d_occluded, uint32_t capacity, uint32_t* zero0,
                        uint32_t* zero1) {
     return launch(c, d_rays, capacity, d_hits, d_occluded, false, n_dev, nullptr, zero0, zero1, d_rays2, n2_dev);
+}
+void set_coop_splits(rt_ctx* c, const std::vector<uint32_t>& level_start) {
+    // group width 2^g: split at the first level (below the root) holding at
+    // least trace_coop_nodes nodes per lane of the group
+    c->coop_gmax = 0;
+    for (int g = 0; g < 7; ++g) c->coop_split[g] = 0;
+    const size_t levels = level_start.size() - 1;  // the last entry is the node count
+    for (uint32_t g = 1; g <= 6; ++g) {
+        const uint64_t need = (uint64_t)c->trace_coop_nodes << g;
+        size_t l = 1;
+        while (l < levels && level_start[l + 1] - level_start[l] < need) ++l;
+        if (l >= levels) break;
+        c->coop_split[g] = level_start[l];
+        c->coop_gmax = g;
+    }
 }
 int trace_error(rt_ctx* c, uint32_t* err) {
     *err = 0;
@@ -595,6 +742,10 @@ int trace_noalloc(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, u
     a.n_dev = nullptr;
     a.rays2 = nullptr;
     a.n2_dev = nullptr;
+    a.coop_max = 0;  // the plain walk only
+    a.coop_gmax = 0;
+    for (int g = 0; g < 7; ++g) a.coop_split[g] = 0;
+    a.coop_zero = 0;
     a.nodes = (const rt_bvh4q_node*)c->qnodes.ptr;
     a.tris = (const rt_tri_rec*)c->tris.ptr;
     a.work = base;
