@@ -145,6 +145,10 @@ int rt_create(rt_ctx** out, int device) {
         long v = strtol(r, nullptr, 10);
         c->trace_coop_nodes = (uint32_t)(v < 1 ? 1 : (v > 4096 ? 4096 : v));
     }
+    if (const char* r = getenv("MTRT_TRACE_COOP_GMAX")) {  // tuning knob, 1..6
+        long v = strtol(r, nullptr, 10);
+        c->trace_coop_gmax = (uint32_t)(v < 1 ? 1 : (v > 6 ? 6 : v));
+    }
     if (const char* r = getenv("MTRT_TRACE_SLICE")) {  // tuning knob, rounded to 128..2^24
         long v = strtol(r, nullptr, 10);
         v = v < 128 ? 128 : (v > (1l << 24) ? (1l << 24) : v);

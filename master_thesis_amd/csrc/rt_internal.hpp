@@ -74,7 +74,8 @@ struct rt_ctx {
     uint32_t coop_split[7] = {0, 0, 0, 0, 0, 0, 0};
     uint32_t coop_gmax = 0;
     uint32_t trace_coop = 1;         // launches with >= 2^trace_coop lanes per ray walk cooperatively (0: never; MTRT_TRACE_COOP)
-    uint32_t trace_coop_nodes = 16;  // nodes per lane of a group at its split level (MTRT_TRACE_COOP_NODES)
+    uint32_t trace_coop_nodes = 256;  // nodes per lane of a group at its split level (MTRT_TRACE_COOP_NODES)
+    uint32_t trace_coop_gmax = 6;    // widest group: 2^trace_coop_gmax lanes (MTRT_TRACE_COOP_GMAX, 1..6)
     // build scratch
     rt::DevBuf keys[2], vals[2], hist, inner, leafbox, innerbox, boxtab, queue[2], counters;
     // trace scratch

@@ -648,7 +648,7 @@ void set_coop_splits(rt_ctx* c, const std::vector<uint32_t>& level_start) {
     c->coop_gmax = 0;
     for (int g = 0; g < 7; ++g) c->coop_split[g] = 0;
     const size_t levels = level_start.size() - 1;  // the last entry is the node count
-    for (uint32_t g = 1; g <= 6; ++g) {
+    for (uint32_t g = 1; g <= c->trace_coop_gmax; ++g) {
         const uint64_t need = (uint64_t)c->trace_coop_nodes << g;
         size_t l = 1;
         while (l < levels && level_start[l + 1] - level_start[l] < need) ++l;

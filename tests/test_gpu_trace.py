@@ -88,26 +88,29 @@ def test_bvh_structure_and_counters(tracer, cluster):
 def test_cooperative_walk_equals_plain_walk(monkeypatch, cluster):
     """Small launches walk each ray with a group of lanes (up to 64; the
     subtrees below a split level dealt out among them), larger ones one ray
-    per lane.  At every batch size, with the cooperative walk on and off
-    (MTRT_TRACE_COOP, read when a context is created), closest hits are
-    bit-identical to the oracle's brute force and any-hit flags equal hit
-    presence."""
+    per lane.  At every batch size, with the cooperative walk off, on with
+    the default split and on with the shallowest splits (so the widest
+    groups: MTRT_TRACE_COOP and MTRT_TRACE_COOP_NODES, read when a context is
+    created), closest hits are bit-identical to the oracle's brute force and
+    any-hit flags equal hit presence."""
     from master_thesis_amd.tracer import Tracer
     sc = scenes.triangle_soup(30000, seed=21, mu=-4.5)
     coh, inc = scenes.microbench_rays(128, 64, 24000, seed=22)
     rays = np.concatenate([inc[:12000], coh, inc[12000:]])
     want = O.trace_brute(sc, rays)
-    for coop in ("1", "0"):
+    for coop, nodes in (("1", "256"), ("1", "1"), ("0", "256")):
         monkeypatch.setenv("MTRT_TRACE_COOP", coop)
+        monkeypatch.setenv("MTRT_TRACE_COOP_NODES", nodes)
         with Tracer(0) as t:
             t.upload(sc)
             t.build(4, cluster=cluster)
             for n in (1, 7, 64, 1000, 5000, rays.shape[0]):
-                label = f"coop={coop} cluster={cluster} n={n}"
+                label = f"coop={coop} nodes={nodes} cluster={cluster} n={n}"
                 _assert_same_hits(t.trace(rays[:n].copy()), want[:n], label)
                 np.testing.assert_array_equal(t.trace_any(rays[:n].copy()),
                                               (want["tri"][:n] != RT_MISS).astype(np.uint8), label)
     monkeypatch.delenv("MTRT_TRACE_COOP")
+    monkeypatch.delenv("MTRT_TRACE_COOP_NODES")
 
 
 def test_large_soup_vs_cpu_bvh(tracer):
