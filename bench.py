@@ -109,7 +109,13 @@ def main() -> int:
     ap.add_argument("--res", default="1920x1080", help="nrc: image size")
     ap.add_argument("--views", type=int, default=16, help="terrain: camera views batched in one launch")
     ap.add_argument("--spp", type=int, default=0, help="cornell / terrain / diff: override samples per pixel")
+    ap.add_argument("--summarize-pmc", metavar="DIR", default="",
+                    help="raycast / terrain_raycast: write the counter summaries under profiles/ from the "
+                         "rocprofv3 CSV files of the passes in PMC_PASSES under DIR (no GPU work)")
     args = ap.parse_args()
+    if args.summarize_pmc:
+        summarize_pmc(args.config, Path(args.summarize_pmc))
+        return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (rays and pixels would be sharded "
               "by range with the BVH replicated on every GPU). Run with --gpus 1.", file=sys.stderr)
@@ -918,5 +924,143 @@ def bench_nrc(args):
             "config": {"workload": f"{args.nrc_scene}, {W}x{Hh}, depth {depth}, {args.frames} frames of online "
                                    f"training at 1 spp, reference {args.ref_spp} spp light-sampled",
                        "triangles": int(sc.num_triangles)}}
+PMC_PASSES = {  # rocprofv3 passes behind a traversal counter summary, one run each
+    "fetch": "--pmc FETCH_SIZE",
+    "write": "--pmc WRITE_SIZE TCC_HIT_sum TCC_MISS_sum",
+    "sq": "--pmc SQ_INSTS_VALU SQ_ACTIVE_INST_VALU SQ_BUSY_CYCLES SQ_WAVES SQ_WAVE_CYCLES SQ_INSTS_VMEM_RD "
+          "SQ_INSTS_SALU SQ_INSTS_LDS TCC_EA0_RDREQ_sum GRBM_GUI_ACTIVE",
+    "stats": "--kernel-trace --stats",
+}
+PMC_WORKLOADS = {"raycast": ("trace_raycast_1M_16M", 16777216), "terrain_raycast": ("trace_terrain_16M", 16588800)}
+TRACE_KERNEL = "k_trace<false, false, false>"  # the timed closest-hit walk
+STATS_KERNEL = "k_trace<true, false, false>"   # its counting variant, launched once before the timed walks
+COPY_KERNEL = "copyBuffer"                      # the 512 MiB calibration copy
+CALIB_BYTES = 512 * 1024 * 1024
+
+
+def pmc_commands(config: str) -> list[str]:
+    """The rocprofv3 runs (from /tmp, R = repository root) behind the summary."""
+    out = []
+    for name, opt in PMC_PASSES.items():
+        steps = "--steps 5 --warmup 2" if name == "stats" else "--steps 3 --warmup 1 --calibrate-copy"
+        out.append(f"rocprofv3 {opt} --output-format csv -d $R/gpurun_out/pmc -o {config}_{name} -- "
+                   f"python3 $R/bench.py --config {config} {steps} --no-cpu --no-halves")
+    return out
+
+
+def summarize_pmc(config: str, pmc_dir: Path) -> None:
+    """Turns the rocprofv3 CSV files of pmc_commands(config) under pmc_dir into
+    profiles/<name>_pmc.json, _pmc.md and _kernel_stats.md: per launch of the
+    timed closest-hit walk, the memory-side bytes (FETCH_SIZE and WRITE_SIZE
+    scaled by what they read for a 512 MiB device copy in the same run), the
+    L2 hit rate, instruction counts and the VALU issue rate."""
+    import csv
+    name, rays = PMC_WORKLOADS[config]
+
+    def rows(kind: str, suffix: str):
+        files = sorted(pmc_dir.rglob(f"{config}_{kind}*{suffix}"))
+        if not files:
+            raise SystemExit(f"no {config}_{kind}*{suffix} under {pmc_dir}")
+        for f in files:
+            with open(f, newline="") as fh:
+                yield from csv.DictReader(fh)
+
+    def after_counting_pass(rs) -> int:
+        # the timed launches follow the counting walk's one launch; walks before
+        # it (the terrain's own ray generation) trace other rays on another tree
+        ids = [int(r["Dispatch_Id"]) for r in rs if STATS_KERNEL in r["Kernel_Name"]]
+        if not ids:
+            raise SystemExit("no counting launch in the profile")
+        return max(ids)
+
+    def counters(kind: str) -> dict:
+        # per dispatch: {counter: value} for the walk's and the copy's launches
+        per, rs = {}, list(rows(kind, "counter_collection.csv"))
+        first = after_counting_pass(rs)
+        for r in rs:
+            k = r["Kernel_Name"]
+            tag = "trace" if TRACE_KERNEL in k else ("copy" if COPY_KERNEL in k else None)
+            if tag is None or (tag == "trace" and int(r["Dispatch_Id"]) <= first):
+                continue
+            d = per.setdefault((tag, r["Dispatch_Id"]), {})
+            d[r["Counter_Name"]] = d.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+        trace = [v for (tag, _), v in per.items() if tag == "trace"]
+        copy = [v for (tag, _), v in per.items() if tag == "copy" and v.get("FETCH_SIZE", v.get("WRITE_SIZE", 0)) > 1e5]
+        if not trace:
+            raise SystemExit(f"no {TRACE_KERNEL} launches in the {kind} pass")
+        mean = lambda vs, c: float(np.mean([v[c] for v in vs]))  # noqa: E731
+        return {"trace": trace, "copy": copy, "mean": mean}
+
+    fetch, write, sq = counters("fetch"), counters("write"), counters("sq")
+    m = fetch["mean"]
+    fetch_kib = m(fetch["trace"], "FETCH_SIZE")
+    fetch_factor = CALIB_BYTES / 1024 / m(fetch["copy"], "FETCH_SIZE")
+    write_kib = m(write["trace"], "WRITE_SIZE")
+    write_factor = CALIB_BYTES / 1024 / m(write["copy"], "WRITE_SIZE")
+    hbm = (fetch_kib * fetch_factor + write_kib * write_factor) * 1024
+    hit, miss = m(write["trace"], "TCC_HIT_sum"), m(write["trace"], "TCC_MISS_sum")
+    s = {c: m(sq["trace"], c) for c in ("SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_VMEM_RD", "SQ_INSTS_LDS",
+                                          "TCC_EA0_RDREQ_sum", "GRBM_GUI_ACTIVE", "SQ_BUSY_CYCLES",
+                                          "SQ_ACTIVE_INST_VALU", "SQ_WAVES", "SQ_WAVE_CYCLES")}
+    cycles = s["GRBM_GUI_ACTIVE"] / 8  # GRBM_GUI_ACTIVE sums the 8 XCDs
+    valu_issue = s["SQ_INSTS_VALU"] / (256 * cycles)
+
+    stats = list(rows("stats", "kernel_stats.csv"))
+    kt = list(rows("stats", "kernel_trace.csv"))
+    first = after_counting_pass(kt)
+    launches = [r for r in kt if TRACE_KERNEL in r["Kernel_Name"] and int(r["Dispatch_Id"]) > first]
+    durs = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6 for r in launches]
+    k_ms = float(np.mean(durs))
+    short = lambda k: k.split("(")[0].replace("void ", "").replace("rt::", "").replace("(anonymous namespace)::", "")  # noqa: E731
+
+    out = {"kernel": TRACE_KERNEL, "workload": config, "rays_per_launch": rays, "source_hash": kernel_source_hash(),
+           "launches_counted": len(fetch["trace"]), "fetch_kib_raw": round(fetch_kib, 1),
+           "write_kib_raw": round(write_kib, 1), "fetch_factor": round(fetch_factor, 4),
+           "write_factor": round(write_factor, 4), "hbm_bytes_per_launch": int(hbm),
+           "bytes_per_ray": round(hbm / rays, 1), "tcc_hit": int(hit), "tcc_miss": int(miss),
+           "l2_hit_rate": round(hit / (hit + miss), 4), "tcc_ea0_rdreq": int(s["TCC_EA0_RDREQ_sum"]),
+           "sq_insts_valu": int(s["SQ_INSTS_VALU"]), "sq_insts_salu": int(s["SQ_INSTS_SALU"]),
+           "sq_insts_vmem_rd": int(s["SQ_INSTS_VMEM_RD"]), "sq_insts_lds": int(s["SQ_INSTS_LDS"]),
+           "grbm_gui_active_per_xcd": int(cycles), "cus": 256, "xcds": 8,
+           "valu_issue_frac": round(valu_issue, 4), "kernel_ms_trace": round(k_ms, 4),
+           "kernel_launches_traced": len(durs)}
+    prof = ROOT / "profiles"
+    (prof / f"{name}_pmc.json").write_text(json.dumps(out, indent=1) + "\n")
+    cmds = "\n".join(pmc_commands(config))
+    (prof / f"{name}_pmc.md").write_text(
+        f"# Counters: {TRACE_KERNEL} on {config} ({rays} rays per launch)\n\n"
+        f"Kernel sources hashed {out['source_hash']} (`bench.py` `KERNEL_SOURCES`); the bench uses these figures "
+        f"only for the same hash.\n\nCommands (one pass each, `R` = repository root, run from `/tmp`), then "
+        f"`python3 bench.py --config {config} --summarize-pmc $R/gpurun_out/pmc`:\n\n```\n{cmds}\n```\n\n"
+        f"Timed launches of the walk (the counting pass's launch is another kernel), mean per launch:\n\n"
+        f"| figure | value |\n|---|---|\n"
+        f"| kernel duration (kernel trace, {len(durs)} launches) | {k_ms:.4f} ms |\n"
+        f"| FETCH_SIZE raw / calibrated | {fetch_kib:.0f} KiB × {fetch_factor:.3f} |\n"
+        f"| WRITE_SIZE raw / calibrated | {write_kib:.0f} KiB × {write_factor:.3f} |\n"
+        f"| memory-side bytes per launch | {hbm / 1e9:.2f} GB ({hbm / rays:.0f} B per ray) |\n"
+        f"| memory-side rate at the traced duration | {hbm / k_ms / 1e9:.2f} TB/s ({hbm / k_ms / 1e9 / 8:.2f} of 8 TB/s) |\n"
+        f"| L2 hits / misses (TCC_HIT_sum / TCC_MISS_sum) | {hit:.0f} / {miss:.0f}: hit rate {hit / (hit + miss):.3f} |\n"
+        f"| VALU instructions (SQ_INSTS_VALU, wave64) | {s['SQ_INSTS_VALU']:.0f} ({s['SQ_INSTS_VALU'] * 64 / rays:.0f} lane-slots per ray) |\n"
+        f"| SALU / VMEM read / LDS instructions | {s['SQ_INSTS_SALU']:.0f} / {s['SQ_INSTS_VMEM_RD']:.0f} / {s['SQ_INSTS_LDS']:.0f} |\n"
+        f"| cycles per XCD (GRBM_GUI_ACTIVE / 8) | {cycles:.0f} |\n"
+        f"| VALU issue rate: instructions / (256 CUs × cycles) | {valu_issue:.3f} of one wave64 instruction per CU per cycle |\n\n"
+        f"Calibration: a 512 MiB torch device copy (`__amd_rocclr_copyBuffer`) in the same runs reads FETCH_SIZE "
+        f"{m(fetch['copy'], 'FETCH_SIZE'):.0f} KiB (factor {fetch_factor:.3f}) and WRITE_SIZE "
+        f"{m(write['copy'], 'WRITE_SIZE'):.0f} KiB (factor {write_factor:.3f}).\n"
+        f"Memory-side bytes are L2 misses to the fabric: the 256 MB MALL serves part of them, so HBM moved at most "
+        f"this much.\n")
+    tot = sum(float(r["TotalDurationNs"]) for r in stats)
+    table = "\n".join(f"| `{short(r['Name'])}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
+                      f"{float(r['AverageNs']) / 1e6:.4f} | {100 * float(r['TotalDurationNs']) / tot:.2f} |"
+                      for r in sorted(stats, key=lambda r: -float(r["TotalDurationNs"]))[:16])
+    (prof / f"{name}_kernel_stats.md").write_text(
+        f"# Kernel stats: bench.py --config {config} (rocprofv3 --kernel-trace --stats)\n\n"
+        f"```\ncd /tmp && {pmc_commands(config)[-1]}\n```\n\n"
+        f"Launches of {TRACE_KERNEL} over all {rays} rays (after the counting pass): {len(durs)}, mean "
+        f"{k_ms:.4f} ms = {rays / k_ms / 1e3:.1f} Mrays/s.\n\n"
+        f"| kernel | calls | total ms | mean ms | % |\n|---|---|---|---|---|\n{table}\n")
+    log(f"[pmc] {name}: {json.dumps(out)}")
+
+
 if __name__ == "__main__":
     sys.exit(main())
