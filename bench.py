@@ -112,7 +112,13 @@ def main() -> int:
     ap.add_argument("--summarize-pmc", metavar="DIR", default="",
                     help="raycast / terrain_raycast: write the counter summaries under profiles/ from the "
                          "rocprofv3 CSV files of the passes in PMC_PASSES under DIR (no GPU work)")
+    ap.add_argument("--summarize-stats", nargs=3, metavar=("DIR", "NAME", "CMD"), default=None,
+                    help="write profiles/NAME_kernel_stats.md from the rocprofv3 --kernel-trace --stats CSV files "
+                         "under DIR that the command CMD wrote (no GPU work)")
     args = ap.parse_args()
+    if args.summarize_stats:
+        summarize_stats(Path(args.summarize_stats[0]), *args.summarize_stats[1:])
+        return 0
     if args.summarize_pmc:
         summarize_pmc(args.config, Path(args.summarize_pmc))
         return 0
@@ -1011,7 +1017,6 @@ def summarize_pmc(config: str, pmc_dir: Path) -> None:
     launches = [r for r in kt if TRACE_KERNEL in r["Kernel_Name"] and int(r["Dispatch_Id"]) > first]
     durs = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6 for r in launches]
     k_ms = float(np.mean(durs))
-    short = lambda k: k.split("(")[0].replace("void ", "").replace("rt::", "").replace("(anonymous namespace)::", "")  # noqa: E731
 
     out = {"kernel": TRACE_KERNEL, "workload": config, "rays_per_launch": rays, "source_hash": kernel_source_hash(),
            "launches_counted": len(fetch["trace"]), "fetch_kib_raw": round(fetch_kib, 1),
@@ -1050,7 +1055,7 @@ def summarize_pmc(config: str, pmc_dir: Path) -> None:
         f"Memory-side bytes are L2 misses to the fabric: the 256 MB MALL serves part of them, so HBM moved at most "
         f"this much.\n")
     tot = sum(float(r["TotalDurationNs"]) for r in stats)
-    table = "\n".join(f"| `{short(r['Name'])}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
+    table = "\n".join(f"| `{_short_kernel(r['Name'])}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
                       f"{float(r['AverageNs']) / 1e6:.4f} | {100 * float(r['TotalDurationNs']) / tot:.2f} |"
                       for r in sorted(stats, key=lambda r: -float(r["TotalDurationNs"]))[:16])
     (prof / f"{name}_kernel_stats.md").write_text(
@@ -1060,6 +1065,30 @@ def summarize_pmc(config: str, pmc_dir: Path) -> None:
         f"{k_ms:.4f} ms = {rays / k_ms / 1e3:.1f} Mrays/s.\n\n"
         f"| kernel | calls | total ms | mean ms | % |\n|---|---|---|---|---|\n{table}\n")
     log(f"[pmc] {name}: {json.dumps(out)}")
+
+
+def _short_kernel(k: str) -> str:
+    return k.replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "").replace("rt::", "")
+
+
+def summarize_stats(d: Path, name: str, cmd: str) -> None:
+    """profiles/<name>_kernel_stats.md: every kernel's calls and time from the
+    *_kernel_stats.csv files rocprofv3 wrote under d."""
+    import csv
+    rows = []
+    for f in sorted(d.rglob("*kernel_stats.csv")):
+        with open(f, newline="") as fh:
+            rows += list(csv.DictReader(fh))
+    if not rows:
+        raise SystemExit(f"no kernel_stats.csv under {d}")
+    tot = sum(float(r["TotalDurationNs"]) for r in rows)
+    table = "\n".join(f"| `{_short_kernel(r['Name'])}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
+                      f"{float(r['AverageNs']) / 1e3:.1f} | {100 * float(r['TotalDurationNs']) / tot:.1f} |"
+                      for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])))
+    (ROOT / "profiles" / f"{name}_kernel_stats.md").write_text(
+        f"# Kernel trace: {name}\n\nCommand (run from /tmp on the MI355X box, repo at $R), then "
+        f"`python3 bench.py --summarize-stats DIR {name} CMD`:\n\n```\ncd /tmp && {cmd}\n```\n\n"
+        f"| kernel | calls | total ms | avg us | % |\n|---|---|---|---|---|\n{table}\n")
 
 
 if __name__ == "__main__":

@@ -17,7 +17,8 @@
 //
 // Scheduling: a slot owns one work item (pixel, chunk of samples) and runs
 // its samples one after another, so each pixel's sum has the reference's
-// order.  Every iteration is two kernels over the live slots:
+// order.  A pixel's N samples are split into K chunks (K <= N, chunk c holds
+// samples [c N / K, (c + 1) N / K)) when there are fewer pixels than slots.  Every iteration is two kernels over the live slots:
 //   extend  closest hit of every queued ray (trace.hip, device-resident count)
 //   shade   the state machine above; a finished sample restarts the slot on
 //           the next sample; a finished item takes the next item from a
@@ -92,7 +93,7 @@ struct PTArgs {
     const rt_camera* cams;  // V cameras in device memory
     uint32_t W, H, P;       // image size, pixels per view
     uint32_t V, VP;         // views, V * P
-    uint32_t K, spc, items, S;
+    uint32_t K, spp, items, S;  // chunks per pixel, samples per pixel, work items, slots
     uint32_t max_depth, sample_offset, flags;
     uint32_t seed_lo, seed_hi;
     float sky[3];
@@ -169,6 +170,11 @@ __device__ __forceinline__ rt_ray make_ray(f3 o, f3 d) {
 }
 
 // vp = view * P + pixel
+// first sample of chunk c of a pixel (chunk c holds the samples up to the next chunk's first)
+__device__ __forceinline__ uint32_t chunk_first(const PTArgs& a, uint32_t c) {
+    return (uint32_t)(((uint64_t)c * a.spp) / a.K);
+}
+
 __device__ __forceinline__ void start_sample(const PTArgs& a, uint32_t slot, uint32_t vp, rt_ray& r, bool adj) {
     const rt_camera& c = a.cams[vp / a.P];
     const uint32_t pix = vp % a.P;
@@ -361,6 +367,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
         const uint32_t vp = item % a.VP;   // view * P + pixel: index into images and partial sums
         const uint32_t chunk = item / a.VP;
         const uint32_t view = vp / a.P, pix = vp % a.P;
+        const uint32_t c0 = chunk_first(a, chunk), cn = chunk_first(a, chunk + 1) - c0;
         uint32_t smp = a.s_smp[slot];
         uint32_t st = a.s_state[slot];
         uint32_t depth = st & 0xFFFFu;
@@ -395,7 +402,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
             }
         }
         if (at_vertex)
-            vertex_step<ADJ>(a, slot, pix, view, a.sample_offset + chunk * a.spc + smp, depth, p, nrm, mid, o, d, T,
+            vertex_step<ADJ>(a, slot, pix, view, a.sample_offset + c0 + smp, depth, p, nrm, mid, o, d, T,
                              end, contrib);
         if (end) {
             // the sample ended: add it, then start the next one at the item's
@@ -417,7 +424,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
                 }
             }
             ++smp;
-            bool item_done = smp >= a.spc;
+            bool item_done = smp >= cn;
             if (!item_done) {
                 const size_t S = a.S;
                 const uint32_t w10 = ((const uint32_t*)a.s_fv)[10 * S + slot];
@@ -430,7 +437,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
                     // every remaining sample escapes along the primary ray: 1 times the miss radiance
                     const f3 t0 = miss_radiance(a, d, true);
                     if (!ADJ)
-                        for (; smp < a.spc; ++smp) {
+                        for (; smp < cn; ++smp) {
                             sum[0] = sum[0] + t0.x;
                             sum[1] = sum[1] + t0.y;
                             sum[2] = sum[2] + t0.z;
@@ -445,7 +452,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
                     end = false;
                     contrib = false;
                     if (ADJ) a.s_nv[slot] = 0;
-                    vertex_step<ADJ>(a, slot, pix, view, a.sample_offset + chunk * a.spc + smp, depth, p, nrm, mid,
+                    vertex_step<ADJ>(a, slot, pix, view, a.sample_offset + c0 + smp, depth, p, nrm, mid,
                                      o, d, T, end, contrib);
                     if (end) {
                         if (contrib) {
@@ -458,7 +465,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(PTArgs a, int cur) {
                             }
                         }
                         ++smp;
-                        if (smp >= a.spc) {
+                        if (smp >= cn) {
                             item_done = true;
                         } else {
                             start_sample(a, slot, vp, nr, ADJ);
@@ -676,6 +683,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade_nrc(PTArgs a, int cur) {
         const uint32_t chunk = item / a.VP;
         const uint32_t view = vp / a.P, pix = vp % a.P;
         pidx = chunk * a.VP + vp;
+        const uint32_t c0 = chunk_first(a, chunk), cn = chunk_first(a, chunk + 1) - c0;
         uint32_t smp = a.s_smp[slot];
         const uint32_t st = a.s_state[slot];
         uint32_t depth = st & 0xFFFFu;
@@ -710,7 +718,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade_nrc(PTArgs a, int cur) {
             } else {
                 cull = false;
                 const rt_material m = a.mat[mid];
-                const uint32_t gsmp = a.sample_offset + chunk * a.spc + smp;
+                const uint32_t gsmp = a.sample_offset + c0 + smp;
                 const bool light_end = m.light_source && ndotd <= 0.0f;
                 bool stop = false;
                 if (light_end && a.nee) {
@@ -788,7 +796,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade_nrc(PTArgs a, int cur) {
                 if (rec && !want_rec && ridx != NO_REC) acc3(&a.t_tgt[3 * (size_t)ridx], Tq);
             }
             ++smp;
-            if (smp < a.spc) {
+            if (smp < cn) {
                 a.s_smp[slot] = smp;
                 start_sample(a, slot, vp, nr, false);
                 alive = true;
@@ -1001,17 +1009,15 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
     const uint64_t P = P1 * V;  // view-major (view, pixel) pairs
     if (P > (1ull << 31)) return fail(c, RT_E_UNSUPPORTED, "rt_render: views * pixels above 2^31");
     const uint32_t N = s->spp;
-    // samples of a pixel are split into K chunks (K | N) when there are too
+    // samples of a pixel are split into K <= N chunks when there are too
     // few pixels to fill the device; deterministic mode keeps one chunk so
     // every pixel's sum has the reference's sample order.  (The adjoint
     // scatters per path, so chunks change only its summation order.)
-    const uint64_t target_slots = 1ull << 20;
+    const uint64_t target_slots = 1ull << c->pt_slots_log2;
     uint32_t K = 1;
     if (!(s->flags & RT_RENDER_DETERMINISTIC)) {
-        uint64_t want = (target_slots + P - 1) / P;
-        if (want > N) want = N;
-        for (uint32_t k = (uint32_t)want; k >= 1; --k)
-            if (N % k == 0) { K = k; break; }
+        const uint64_t want = (target_slots + P - 1) / P;
+        K = (uint32_t)(want < N ? want : N);
     }
     const uint64_t items = P * K;
     if (items > 0xFFFFFFFFull) return fail(c, RT_E_UNSUPPORTED, "rt_render: pixels * chunks above 2^32");
@@ -1059,7 +1065,7 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
     a.cams = (const rt_camera*)c->pt_cams.ptr;
     a.W = cam->width; a.H = cam->height; a.P = (uint32_t)P1;
     a.V = V; a.VP = (uint32_t)P;
-    a.K = K; a.spc = N / K; a.items = (uint32_t)items; a.S = S;
+    a.K = K; a.spp = N; a.items = (uint32_t)items; a.S = S;
     a.max_depth = s->max_depth; a.sample_offset = s->sample_offset; a.flags = s->flags;
     a.seed_lo = (uint32_t)s->seed; a.seed_hi = (uint32_t)(s->seed >> 32);
     for (int k = 0; k < 3; ++k) {
@@ -1132,9 +1138,13 @@ int launch_render(rt_ctx* c, const rt_camera* cams, uint32_t V, const rt_setting
     if (adj) hipLaunchKernelGGL(k_pt_init<true>, dim3(div_up(S, PT_BLOCK)), dim3(PT_BLOCK), 0, stream, a);
     else hipLaunchKernelGGL(k_pt_init<false>, dim3(div_up(S, PT_BLOCK)), dim3(PT_BLOCK), 0, stream, a);
     RT_HIP(c, hipGetLastError());
-    // upper bound on iterations: every slot runs ceil(items / S) items of
-    // spc samples of at most max_depth extends
-    const uint64_t bound = (uint64_t)div_up(items, S) * a.spc * s->max_depth + 2;
+    // upper bound on iterations: an item is at most ceil(N / K) samples of at
+    // most max_depth extends (L iterations), and a slot takes the next item as
+    // soon as its own ends, so by iteration (ceil(items / S) - 1) L every slot
+    // has finished that many items and every item has started: the last one
+    // ends by ceil(items / S) L.  (Reaching the bound ends the frame: the
+    // host then waits for the chunk it just queued instead of queueing more.)
+    const uint64_t bound = (uint64_t)div_up(items, S) * div_up(N, K) * s->max_depth + 2;
     int cur = 0;
     uint64_t iter = 0;
     uint32_t launches = 1;

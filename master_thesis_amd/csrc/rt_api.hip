@@ -149,6 +149,10 @@ int rt_create(rt_ctx** out, int device) {
         long v = strtol(r, nullptr, 10);
         c->trace_coop_gmax = (uint32_t)(v < 1 ? 1 : (v > 6 ? 6 : v));
     }
+    if (const char* r = getenv("MTRT_PT_SLOTS_LOG2")) {  // tuning knob, 16..24: path slots in flight
+        long v = strtol(r, nullptr, 10);
+        c->pt_slots_log2 = (uint32_t)(v < 16 ? 16 : (v > 24 ? 24 : v));
+    }
     if (const char* r = getenv("MTRT_TRACE_SLICE")) {  // tuning knob, rounded to 128..2^24
         long v = strtol(r, nullptr, 10);
         v = v < 128 ? 128 : (v > (1l << 24) ? (1l << 24) : v);

@@ -105,6 +105,29 @@ def test_throughput_mode_matches_within_summation_error(tracer):
     assert np.allclose(a, b, rtol=2e-5, atol=1e-5)
 
 
+@pytest.mark.parametrize("slots_log2,spp", [(16, 100), (16, 37), (17, 7), (18, 256)])
+def test_uneven_sample_chunks_match_within_summation_error(monkeypatch, slots_log2, spp):
+    """With fewer pixels than path slots the throughput mode splits each
+    pixel's samples into K = min(slots / pixels, spp) chunks, K need not
+    divide spp (32 x 32 pixels: K = 64 chunks of 1 or 2 of 100 samples, 37
+    chunks of one, 7 of one, 256 of one): every sample is still taken once,
+    so the image equals the deterministic one up to summation order, which
+    equals the oracle bit for bit."""
+    from master_thesis_amd.tracer import Tracer
+    monkeypatch.setenv("MTRT_PT_SLOTS_LOG2", str(slots_log2))  # read when a context is created
+    sc = scenes.cornell_box()
+    cam = H.cornell_camera(32, 32)
+    det = make_settings(spp=spp, max_depth=5, seed=6, flags=RT_RENDER_DETERMINISTIC)
+    fast = make_settings(spp=spp, max_depth=5, seed=6, flags=0)
+    with Tracer(0) as t:
+        _setup(t, sc)
+        a, _ = t.render(cam, det)
+        b, _ = t.render(cam, fast)
+    want, _ = O.render(sc, cam, det)
+    assert np.array_equal(a.view(np.uint32), want.view(np.uint32))
+    assert np.allclose(a, b, rtol=2e-5, atol=1e-5)
+
+
 def test_sample_offset_selects_disjoint_samples(tracer):
     sc = scenes.cornell_box()
     _setup(tracer, sc)
