@@ -146,7 +146,9 @@ __device__ __forceinline__ uint32_t child_of(const uint32_t (&ref)[4], uint32_t 
 // which are bound by how long their slowest walks take): every lane loads
 // its record, a node's 64 bytes or a triangle's 48, before either block, so
 // a step waits for one round of loads.  The choice is per launch, from the
-// ray count (TraceArgs::unified_max).
+// ray count (TraceArgs::unified_max), and in larger launches per wave: a
+// wave whose rays all point into one octant (coherent rays, whose records
+// the caches mostly hold) takes the unified form too.
 // Cooperative walks (COOP): the group's lane `sub` of 2^g pushes a child of
 // a node above the split (index < split) only when the child is above the
 // split too or is dealt to it (coop_owner); below the split it pushes every
@@ -324,6 +326,7 @@ __attribute__((amdgpu_waves_per_eu(STATS ? 5 : 6, STATS ? 5 : 6))) k_trace(Trace
     const uint32_t n = MIXED ? n1 + *a.n2_dev : n1;  // rays in all
     if (n <= a.coop_max) return;                      // the cooperative walk takes this launch
     const bool unified = n <= a.unified_max;          // launch-uniform
+    bool coherent = false;                            // wave-uniform: every ray of the wave in one octant
     uint32_t pool_next = 0, pool_end = 0;  // wave-uniform
     bool exhausted = false;                // wave-uniform
     uint32_t part = blockIdx.x % XCDS;     // wave-uniform: the XCD whose slices this wave takes
@@ -398,12 +401,18 @@ __attribute__((amdgpu_waves_per_eu(STATS ? 5 : 6, STATS ? 5 : 6))) k_trace(Trace
                     }
                 }
                 pool_next = min(pool_end, pool_next + (uint32_t)__popcll(need));
+                if (!unified) {
+                    const uint64_t act = __ballot(have), bx = __ballot(have && L.negx),
+                                   by = __ballot(have && L.negy), bz = __ballot(have && L.negz);
+                    coherent = act != 0ull && (bx == 0ull || bx == act) && (by == 0ull || by == act) &&
+                               (bz == 0ull || bz == act);
+                }
             } else if (exhausted && !have) {
                 retired = true;
             }
         }
         if (__ballot(!retired) == 0ull) break;
-        if (have && lane_step<STATS, ANY, MIXED>(a, L, stk, tid, gtid, c_nodes, c_tris, unified)) {
+        if (have && lane_step<STATS, ANY, MIXED>(a, L, stk, tid, gtid, c_nodes, c_tris, unified || coherent)) {
             const bool hit = L.best_id != RT_MISS;
             if (ANY || (MIXED && L.any)) {
                 a.occluded[MIXED ? ri - n1 : ri] = hit ? 1 : 0;
