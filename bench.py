@@ -288,8 +288,11 @@ def raycast(args, sc, rays, n_coh, workload, pmc_path, cluster=False):
                 "traffic_gbps": traffic and round(traffic / (k_ms / 1e3) / 1e9, 1),
                 "traffic_frac": traffic and round(traffic / (k_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
                 "l2_hit_rate": pmc.get("l2_hit_rate") if same else None,
-                # VALU instructions issued per CU cycle (one wave64 instruction per CU per cycle is the peak)
-                "valu_issue_frac": pmc.get("valu_issue_frac") if same else None,
+                # wave64 VALU instructions per CU per cycle: a wave64 op that occupies its SIMD for 4
+                # cycles allows 1.0; measured values at or above 1 mean the VALU pipes are saturated
+                "valu_insts_per_cu_cycle": pmc.get("valu_insts_per_cu_cycle") if same else None,
+                # rocprof's VALUBusy for gfx950: SQ_ACTIVE_INST_VALU / CUs / GRBM_GUI_ACTIVE cycles
+                "valu_busy": pmc.get("valu_busy") if same else None,
                 "counters": (pmc_path.relative_to(ROOT).as_posix() if same else
                              f"none for these kernel sources ({src})"),
                 "kernel": "k_trace", "kernel_ms": round(k_ms, 4),
@@ -1010,6 +1013,7 @@ def summarize_pmc(config: str, pmc_dir: Path) -> None:
                                           "SQ_ACTIVE_INST_VALU", "SQ_WAVES", "SQ_WAVE_CYCLES")}
     cycles = s["GRBM_GUI_ACTIVE"] / 8  # GRBM_GUI_ACTIVE sums the 8 XCDs
     valu_issue = s["SQ_INSTS_VALU"] / (256 * cycles)
+    valu_busy = s["SQ_ACTIVE_INST_VALU"] / (256 * cycles)
 
     stats = list(rows("stats", "kernel_stats.csv"))
     kt = list(rows("stats", "kernel_trace.csv"))
@@ -1027,7 +1031,8 @@ def summarize_pmc(config: str, pmc_dir: Path) -> None:
            "sq_insts_valu": int(s["SQ_INSTS_VALU"]), "sq_insts_salu": int(s["SQ_INSTS_SALU"]),
            "sq_insts_vmem_rd": int(s["SQ_INSTS_VMEM_RD"]), "sq_insts_lds": int(s["SQ_INSTS_LDS"]),
            "grbm_gui_active_per_xcd": int(cycles), "cus": 256, "xcds": 8,
-           "valu_issue_frac": round(valu_issue, 4), "kernel_ms_trace": round(k_ms, 4),
+           "valu_insts_per_cu_cycle": round(valu_issue, 4), "valu_busy": round(valu_busy, 4),
+           "kernel_ms_trace": round(k_ms, 4),
            "kernel_launches_traced": len(durs)}
     prof = ROOT / "profiles"
     (prof / f"{name}_pmc.json").write_text(json.dumps(out, indent=1) + "\n")
@@ -1048,7 +1053,11 @@ def summarize_pmc(config: str, pmc_dir: Path) -> None:
         f"| VALU instructions (SQ_INSTS_VALU, wave64) | {s['SQ_INSTS_VALU']:.0f} ({s['SQ_INSTS_VALU'] * 64 / rays:.0f} lane-slots per ray) |\n"
         f"| SALU / VMEM read / LDS instructions | {s['SQ_INSTS_SALU']:.0f} / {s['SQ_INSTS_VMEM_RD']:.0f} / {s['SQ_INSTS_LDS']:.0f} |\n"
         f"| cycles per XCD (GRBM_GUI_ACTIVE / 8) | {cycles:.0f} |\n"
-        f"| VALU issue rate: instructions / (256 CUs × cycles) | {valu_issue:.3f} of one wave64 instruction per CU per cycle |\n\n"
+        f"| VALU issue rate: SQ_INSTS_VALU / (256 CUs × cycles) | {valu_issue:.3f} wave64 instructions per CU per cycle |\n"
+        f"| VALUBusy (rocprof's gfx950 form: SQ_ACTIVE_INST_VALU / 256 CUs / cycles) | {valu_busy:.3f} |\n\n"
+        f"A wave64 VALU instruction that holds its SIMD for 4 cycles allows one instruction per CU per cycle; the "
+        f"measured rate reaching or passing 1 (some of gfx950's VALU operations complete in fewer cycles) says the "
+        f"VALU pipes are saturated.\n"
         f"Calibration: a 512 MiB torch device copy (`__amd_rocclr_copyBuffer`) in the same runs reads FETCH_SIZE "
         f"{m(fetch['copy'], 'FETCH_SIZE'):.0f} KiB (factor {fetch_factor:.3f}) and WRITE_SIZE "
         f"{m(write['copy'], 'WRITE_SIZE'):.0f} KiB (factor {write_factor:.3f}).\n"

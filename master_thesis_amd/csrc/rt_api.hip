@@ -125,6 +125,10 @@ int rt_create(rt_ctx** out, int device) {
         long v = strtol(r, nullptr, 10);
         c->trace_refill_min = (uint32_t)(v < 1 ? 1 : (v > 64 ? 64 : v));
     }
+    if (const char* r = getenv("MTRT_TRACE_REFILL_LARGE")) {  // tuning knob, 1..64
+        long v = strtol(r, nullptr, 10);
+        c->trace_refill_large = (uint32_t)(v < 1 ? 1 : (v > 64 ? 64 : v));
+    }
     if (const char* r = getenv("MTRT_BVH_ALPHA")) {  // tuning knob, >= 0
         const float v = strtof(r, nullptr);
         c->bvh_leaf_alpha = v >= 0.0f ? v : 0.0f;

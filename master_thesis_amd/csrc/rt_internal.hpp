@@ -84,6 +84,7 @@ struct rt_ctx {
     int trace_per_cu[4] = {0, 0, 0, 0};  // traversal: resident workgroups per CU (closest, counting, any, mixed)
     uint32_t trace_max_steps = 0;  // traversal: per-ray step bound (0: from the tree size; MTRT_TRACE_MAX_STEPS)
     uint32_t trace_refill_min = 8;  // traversal: idle lanes that trigger a ray refill (MTRT_TRACE_REFILL)
+    uint32_t trace_refill_large = 24;  // the same in launches too large for the unified loads (MTRT_TRACE_REFILL_LARGE)
     uint32_t pt_slots_log2 = 24;     // path tracers: at most 2^this paths in flight (MTRT_PT_SLOTS_LOG2)
     uint32_t trace_slice = 16384;   // traversal: rays per XCD slice, a multiple of 128 (MTRT_TRACE_SLICE)
     uint32_t trace_unified_max = 0;  // traversal: launches of at most this many rays load each step in one round (0: 4 per resident lane; MTRT_TRACE_UNIFIED_MAX)

@@ -64,7 +64,8 @@ struct TraceArgs {
     uint32_t spill_cap;
     uint32_t spill_stride;
     uint32_t max_steps;    // bound on steps per ray (each node and triangle is visited at most once)
-    uint32_t refill_min;   // idle lanes that trigger a refill (1..64)
+    uint32_t refill_min;   // idle lanes that trigger a refill (1..64) in launches of the unified form
+    uint32_t refill_min_large;  // the same in larger launches
     uint32_t unified_max;  // launches of at most this many rays load each step's record in one round
     uint32_t slice;        // rays per XCD slice (a multiple of POOL_CHUNK)
     unsigned long long* stats;  // rays, nodes, tris, hits
@@ -326,6 +327,9 @@ __attribute__((amdgpu_waves_per_eu(STATS ? 5 : 6, STATS ? 5 : 6))) k_trace(Trace
     const uint32_t n = MIXED ? n1 + *a.n2_dev : n1;  // rays in all
     if (n <= a.coop_max) return;                      // the cooperative walk takes this launch
     const bool unified = n <= a.unified_max;          // launch-uniform
+    // large launches refill later: rays that start together stay in step
+    // longer, and their waves more often take the unified loads
+    const uint32_t refill_min = unified ? a.refill_min : a.refill_min_large;
     bool coherent = false;                            // wave-uniform: every ray of the wave in one octant
     uint32_t pool_next = 0, pool_end = 0;  // wave-uniform
     bool exhausted = false;                // wave-uniform
@@ -338,7 +342,7 @@ __attribute__((amdgpu_waves_per_eu(STATS ? 5 : 6, STATS ? 5 : 6))) k_trace(Trace
         const uint64_t need = __ballot(!have && !retired);
         // refill once enough lanes are idle (or none is busy), so lanes that
         // start together stay in step
-        if (need && ((uint32_t)__popcll(need) >= a.refill_min || __ballot(have) == 0ull)) {
+        if (need && ((uint32_t)__popcll(need) >= refill_min || __ballot(have) == 0ull)) {
             if (pool_next >= pool_end && !exhausted) {
                 const int leader = __ffsll((unsigned long long)need) - 1;
                 while (true) {
@@ -604,6 +608,7 @@ int launch(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, uint8_t*
     a.spill_stride = threads;
     a.max_steps = c->trace_max_steps ? c->trace_max_steps : 2u * c->num_nodes + 2u * c->T + 16u;
     a.refill_min = c->trace_refill_min;
+    a.refill_min_large = c->trace_refill_large;
     a.unified_max = unified_max(c, threads);
     a.slice = c->trace_slice;
     a.stats = (unsigned long long*)c->stats.ptr;
@@ -767,6 +772,7 @@ int trace_noalloc(rt_ctx* c, const rt_ray* d_rays, uint32_t n, rt_hit* d_hits, u
     a.spill_stride = grid * TRACE_BLOCK;
     a.max_steps = c->trace_max_steps ? c->trace_max_steps : 2u * c->num_nodes + 2u * c->T + 16u;
     a.refill_min = c->trace_refill_min;
+    a.refill_min_large = c->trace_refill_large;
     a.unified_max = unified_max(c, grid * TRACE_BLOCK);
     a.slice = c->trace_slice;
     a.stats = nullptr;

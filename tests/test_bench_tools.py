@@ -80,7 +80,8 @@ def test_pmc_summary_uses_the_timed_walks_and_the_copy_calibration(profile_dir):
     assert out["l2_hit_rate"] == 0.75
     assert out["kernel_ms_trace"] == 5.0
     # 64000 wave instructions over 256 CUs x 500 cycles per XCD
-    assert out["valu_issue_frac"] == round(64000 / (256 * 500), 4)
+    assert out["valu_insts_per_cu_cycle"] == round(64000 / (256 * 500), 4)
+    assert out["valu_busy"] == round(1000 / (256 * 500), 4)
     assert out["source_hash"] == bench.kernel_source_hash()
     md = (root / "profiles" / "trace_raycast_1M_16M_kernel_stats.md").read_text()
     assert "`k_trace<false, false, false>` | 3 |" in md and "mean 5.0000 ms" in md
